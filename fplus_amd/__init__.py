@@ -1,0 +1,288 @@
+"""fplus_amd -- the FPLUS sparse LU on MI355X.
+
+The numeric factorization (Ruiz equilibration, the supernodal left-looking LU with partial pivoting) and the triangular
+solves run as HIP kernels on one GPU behind the FPLUS C interface (include/fplus.h, libfplus_amd.so).  This module
+binds that interface with ctypes and provides ``SparseLU``, the drop-in for ``scipy.sparse.linalg.splu`` of the
+reference (reference sparse_lu_full_pivot.py:630-836): ``solve(b, trans)``, ``L``, ``U``, ``perm_r``, ``perm_c``,
+``shape``, ``nnz``.
+"""
+import ctypes as C
+
+import numpy as np
+import scipy.sparse as sp
+
+from ._lib import (FPLUS_ORDER_AMD_ATA, FPLUS_ORDER_AUTO, FPLUS_ORDER_COLAMD, FPLUS_PIVOT_PARTIAL,  # noqa: F401
+                   FPLUS_PIVOT_REL_DEFAULT, Factors, FplusError, Options, Stats, check, lib)
+
+__all__ = ["SparseLU", "Symbolic", "Numeric", "default_options", "FplusError", "ruiz", "factor_sn_perm"]
+
+_I64P = C.POINTER(C.c_int64)
+_DP = C.POINTER(C.c_double)
+
+
+def _p64(a):
+    return a.ctypes.data_as(_I64P)
+
+
+def _vp(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+def _canonical_csc(A):
+    A = sp.csc_matrix(A)
+    if not A.has_canonical_format:
+        A = A.copy()
+        A.sum_duplicates()
+    A.sort_indices()
+    n = A.shape[0]
+    return n, np.ascontiguousarray(A.indptr, np.int64), np.ascontiguousarray(A.indices, np.int64), A.data
+
+
+def default_options(**kw):
+    o = Options()
+    lib().fplus_default_options(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise TypeError(f"unknown option {k}")
+        setattr(o, k, v)
+    return o
+
+
+class Symbolic:
+    """fplus_analyze: dense-line pinning, the fill-reducing order, the column-etree postorder"""
+
+    def __init__(self, n, Ap, Ai, pin_dense=True, tau_density=0.5, tau_min_nnz=10, tau_row_mask=None, tau_col_mask=None,
+                 postorder=True, ordering=FPLUS_ORDER_COLAMD):
+        self.n = n
+        self._Ap = np.ascontiguousarray(Ap, np.int64)
+        self._Ai = np.ascontiguousarray(Ai, np.int64)
+        self.ptr = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rm = None if tau_row_mask is None else np.ascontiguousarray(tau_row_mask, np.uint8)
+        cm = None if tau_col_mask is None else np.ascontiguousarray(tau_col_mask, np.uint8)
+        self._masks = (rm, cm)
+        rc = lib().fplus_analyze_ord(n, _p64(self._Ap), _p64(self._Ai), int(pin_dense), tau_density, tau_min_nnz,
+                                     None if rm is None else _vp(rm), None if cm is None else _vp(cm), int(postorder),
+                                     ordering, C.byref(self.ptr), err, 512)
+        check(rc, err)
+        self.row_perm = np.ctypeslib.as_array(lib().fplus_symbolic_row_perm(self.ptr), (n,)).copy() if n else np.zeros(0, np.int64)
+        self.col_perm = np.ctypeslib.as_array(lib().fplus_symbolic_col_perm(self.ptr), (n,)).copy() if n else np.zeros(0, np.int64)
+        self.n_dense_rows = lib().fplus_symbolic_n_dense_rows(self.ptr)
+        self.n_dense_cols = lib().fplus_symbolic_n_dense_cols(self.ptr)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().fplus_symbolic_free(self.ptr)
+            self.ptr = None
+
+
+class Numeric:
+    """fplus_factor on a symbolic analysis: the device factorization, its composed orders and scalings"""
+
+    def __init__(self, sym, Ap, Ai, Ax, options=None):
+        self.sym = sym
+        self.n = sym.n
+        self.is_complex = bool(np.iscomplexobj(Ax))
+        self._Ax = np.ascontiguousarray(Ax, dtype=np.complex128 if self.is_complex else np.float64)
+        Ap = np.ascontiguousarray(Ap, np.int64)
+        Ai = np.ascontiguousarray(Ai, np.int64)
+        self.ptr = C.c_void_p()
+        err = C.create_string_buffer(512)
+        o = options if options is not None else default_options()
+        rc = lib().fplus_factor(sym.ptr, _p64(Ap), _p64(Ai), _vp(self._Ax), int(self.is_complex), C.byref(o), C.byref(self.ptr),
+                                err, 512)
+        check(rc, err)
+        n = self.n
+        self.p = np.ctypeslib.as_array(lib().fplus_numeric_p(self.ptr), (n,)).copy()
+        self.q = np.ctypeslib.as_array(lib().fplus_numeric_q(self.ptr), (n,)).copy()
+        self.Dr = np.ctypeslib.as_array(lib().fplus_numeric_Dr(self.ptr), (n,)).copy()
+        self.Dc = np.ctypeslib.as_array(lib().fplus_numeric_Dc(self.ptr), (n,)).copy()
+
+    def solve(self, b, trans=0):
+        """solve in place on a copy: trans 0 A x = b, 1 A^T x = b, 2 A^H x = b; b of shape (n,) or (n, k)"""
+        b = np.asarray(b)
+        one = b.ndim == 1
+        dt = np.complex128 if self.is_complex else (np.complex128 if np.iscomplexobj(b) else np.float64)
+        if np.iscomplexobj(b) and not self.is_complex:   # a real factorization: real and imaginary parts apart
+            xr = self.solve(b.real, trans)
+            xi = self.solve(b.imag, trans)
+            return xr + 1j * xi
+        x = np.array(b.reshape(self.n, -1), dtype=dt, order="F", copy=True)
+        err = C.create_string_buffer(512)
+        rc = lib().fplus_solve(self.ptr, C.c_void_p(x.ctypes.data), x.shape[1], self.n, trans, err, 512)
+        check(rc, err)
+        return x[:, 0] if one else x
+
+    def factors(self):
+        """the CSC factors of diag(Dr) A[p, :][:, q] diag(Dc) = L U as scipy matrices, and the pivot order"""
+        f = lib().fplus_numeric_factors(self.ptr)
+        if not f:
+            raise FplusError(-5, "factor export failed")
+        f = f.contents
+        n = f.n
+        nL, nU = f.Lp[n], f.Up[n]
+        dt = np.complex128 if f.is_complex else np.float64
+        vs = 2 if f.is_complex else 1
+        Lp = np.ctypeslib.as_array(f.Lp, (n + 1,)).copy()
+        Li = np.ctypeslib.as_array(f.Li, (max(nL, 1),))[:nL].copy()
+        Lx = np.ctypeslib.as_array(C.cast(f.Lx, _DP), (max(nL * vs, 1),))[:nL * vs].copy().view(dt)
+        Up = np.ctypeslib.as_array(f.Up, (n + 1,)).copy()
+        Ui = np.ctypeslib.as_array(f.Ui, (max(nU, 1),))[:nU].copy()
+        Ux = np.ctypeslib.as_array(C.cast(f.Ux, _DP), (max(nU * vs, 1),))[:nU * vs].copy().view(dt)
+        p = np.ctypeslib.as_array(f.p, (n,)).copy()
+        L = sp.csc_matrix((Lx, Li, Lp), shape=(n, n))
+        U = sp.csc_matrix((Ux, Ui, Up), shape=(n, n))
+        return dict(L=L, U=U, p=p, nsupernodes=f.nsupernodes, Lp=Lp, Li=Li, Lx=Lx, Up=Up, Ui=Ui, Ux=Ux)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().fplus_numeric_free(self.ptr)
+            self.ptr = None
+
+
+class SparseLU:
+    """Drop-in for the object ``scipy.sparse.linalg.splu`` returns (reference sparse_lu_full_pivot.py:630-836).
+
+    ``SparseLU(A, **kw)`` analyses and factors A (canonical CSC int64 inside) on the GPU; keyword arguments are the
+    fplus_options fields (``relax_rows``, ``equilibrate``, ``pivot_rel``, ...) plus the analysis settings
+    ``pin_dense_rows``, ``tau_row_mask``, ``tau_col_mask``, ``postorder`` and ``colamd`` (False: the identity orders).
+    Attributes: ``L``, ``U`` (CSC, of the scaled permuted matrix), ``perm_r``, ``perm_c``, ``shape``, ``nnz``.
+    ``solve(b, trans='N'|'T'|'H')`` for b of shape (n,) or (n, k)."""
+
+    def __init__(self, A, colamd=True, pin_dense_rows=True, tau_density=0.5, tau_min_nnz=10, tau_row_mask=None,
+                 tau_col_mask=None, postorder=True, ordering="colamd", **opts):
+        n, Ap, Ai, Ax = _canonical_csc(A)
+        self._n = n
+        ords = {"colamd": FPLUS_ORDER_COLAMD, "amd_ata": FPLUS_ORDER_AMD_ATA, "auto": FPLUS_ORDER_AUTO}
+        if colamd:
+            self._sym = Symbolic(n, Ap, Ai, pin_dense=pin_dense_rows, tau_density=tau_density, tau_min_nnz=tau_min_nnz,
+                                 tau_row_mask=tau_row_mask, tau_col_mask=tau_col_mask, postorder=postorder,
+                                 ordering=ords[ordering])
+        else:
+            self._sym = _IdentitySymbolic(n, Ap, Ai)
+        o = default_options(**{k: v for k, v in opts.items() if k in dict(Options._fields_)})
+        self._num = Numeric(self._sym, Ap, Ai, Ax, o)
+        self._f = None
+        self._is_complex = self._num.is_complex
+
+    def _factors(self):
+        if self._f is None:
+            self._f = self._num.factors()
+        return self._f
+
+    @property
+    def L(self):
+        return self._factors()["L"]
+
+    @property
+    def U(self):
+        return self._factors()["U"]
+
+    @property
+    def perm_r(self):
+        return self._num.p
+
+    @property
+    def perm_c(self):
+        return self._num.q
+
+    @property
+    def shape(self):
+        return (self._n, self._n)
+
+    @property
+    def nnz(self):
+        f = self._factors()
+        return f["L"].nnz + f["U"].nnz
+
+    def solve(self, b, trans="N"):
+        if trans not in ("N", "T", "H"):
+            raise ValueError(f"trans must be 'N', 'T', or 'H'; got {trans!r}")
+        tr = 0 if trans == "N" else (1 if trans == "T" or not self._is_complex else 2)
+        return self._num.solve(b, tr)
+
+
+class _IdentitySymbolic(Symbolic):
+    """the identity orders (colamd=False), through fplus_amd_symbolic_from_orders"""
+
+    def __init__(self, n, Ap, Ai):
+        self.n = n
+        self._Ap = np.ascontiguousarray(Ap, np.int64)
+        self._Ai = np.ascontiguousarray(Ai, np.int64)
+        self.row_perm = np.arange(n, dtype=np.int64)
+        self.col_perm = np.arange(n, dtype=np.int64)
+        self.n_dense_rows = self.n_dense_cols = 0
+        self.ptr = C.c_void_p()
+        err = C.create_string_buffer(512)
+        rc = lib().fplus_amd_symbolic_from_orders(n, _p64(self._Ap), _p64(self._Ai), _p64(self.row_perm), _p64(self.col_perm),
+                                                  C.byref(self.ptr), err, 512)
+        check(rc, err)
+
+
+def symbolic_from_orders(n, Ap, Ai, row_perm, col_perm):
+    """a Symbolic with the given orders (fplus_amd_symbolic_from_orders)"""
+    s = _IdentitySymbolic.__new__(_IdentitySymbolic)
+    s.n = n
+    s._Ap = np.ascontiguousarray(Ap, np.int64)
+    s._Ai = np.ascontiguousarray(Ai, np.int64)
+    s.row_perm = np.ascontiguousarray(row_perm, np.int64)
+    s.col_perm = np.ascontiguousarray(col_perm, np.int64)
+    s.n_dense_rows = s.n_dense_cols = 0
+    s.ptr = C.c_void_p()
+    err = C.create_string_buffer(512)
+    rc = lib().fplus_amd_symbolic_from_orders(n, _p64(s._Ap), _p64(s._Ai), _p64(s.row_perm), _p64(s.col_perm), C.byref(s.ptr),
+                                              err, 512)
+    check(rc, err)
+    return s
+
+
+def stats(num):
+    """fplus_amd_numeric_stats of a Numeric as a dict"""
+    st = Stats()
+    check(lib().fplus_amd_numeric_stats(num.ptr, C.byref(st)), None)
+    return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+
+def ruiz(n, indptr, indices, data, is_csc=True, max_iter=20, tol=1e-2):
+    """fplus_ruiz on the device: (scaled data, Dr, Dc)"""
+    x = np.array(data, copy=True)
+    Dr = np.empty(n)
+    Dc = np.empty(n)
+    err = C.create_string_buffer(512)
+    ip = np.ascontiguousarray(indptr, np.int64)
+    ix = np.ascontiguousarray(indices, np.int64)
+    rc = lib().fplus_ruiz(n, _p64(ip), _p64(ix), _vp(x), int(np.iscomplexobj(x)), int(is_csc), max_iter, tol, 0,
+                          Dr.ctypes.data_as(_DP), Dc.ctypes.data_as(_DP), err, 512)
+    check(rc, err)
+    return x, Dr, Dc
+
+
+def factor_sn_perm(n, Ap, Ai, Ax, row_perm, col_perm, equilibrate=True, relax_rows=32, pivot_rel=FPLUS_PIVOT_REL_DEFAULT,
+                   ruiz_max_iter=20, ruiz_tol=1e-2):
+    """fplus_factor_sn_perm: the factors of the permuted, equilibrated matrix as numpy arrays, and Dr, Dc"""
+    Ax = np.ascontiguousarray(Ax)
+    cplx = int(np.iscomplexobj(Ax))
+    f = Factors()
+    Dr = np.empty(n)
+    Dc = np.empty(n)
+    err = C.create_string_buffer(512)
+    rc = lib().fplus_factor_sn_perm(n, _p64(np.ascontiguousarray(Ap, np.int64)), _p64(np.ascontiguousarray(Ai, np.int64)),
+                                    _vp(Ax), cplx, _p64(np.ascontiguousarray(row_perm, np.int64)),
+                                    _p64(np.ascontiguousarray(col_perm, np.int64)), int(equilibrate), ruiz_max_iter, ruiz_tol, 0,
+                                    relax_rows, 512, 1, 5e7, pivot_rel, C.byref(f), Dr.ctypes.data_as(_DP),
+                                    Dc.ctypes.data_as(_DP), err, 512)
+    check(rc, err)
+    try:
+        nL, nU = f.Lp[n], f.Up[n]
+        dt = np.complex128 if cplx else np.float64
+        vs = 2 if cplx else 1
+        out = dict(n=n, nsn=f.nsupernodes, Lp=np.ctypeslib.as_array(f.Lp, (n + 1,)).copy(),
+                   Li=np.ctypeslib.as_array(f.Li, (max(nL, 1),))[:nL].copy(),
+                   Lx=np.ctypeslib.as_array(C.cast(f.Lx, _DP), (max(nL * vs, 1),))[:nL * vs].copy().view(dt),
+                   Up=np.ctypeslib.as_array(f.Up, (n + 1,)).copy(),
+                   Ui=np.ctypeslib.as_array(f.Ui, (max(nU, 1),))[:nU].copy(),
+                   Ux=np.ctypeslib.as_array(C.cast(f.Ux, _DP), (max(nU * vs, 1),))[:nU * vs].copy().view(dt),
+                   p=np.ctypeslib.as_array(f.p, (n,)).copy(), Dr=Dr, Dc=Dc)
+        return out
+    finally:
+        lib().fplus_factors_free(C.byref(f))

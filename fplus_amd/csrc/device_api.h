@@ -1,0 +1,91 @@
+// device_api.h -- the host-side interface of the HIP kernels (launchers) and device-memory helpers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace fpa {
+
+struct GpuError : std::runtime_error {
+    explicit GpuError(const std::string& m) : std::runtime_error(m) {}
+};
+
+inline void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw GpuError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// the library's device context: device 0 of the process, one stream at default priority
+struct DevCtx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int ncu = 0;
+};
+DevCtx& devctx();   // initialises on first use; throws GpuError when no device is usable
+
+template <class T> struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t cnt) { alloc(cnt); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; return *this; }
+    ~DevBuf() { release(); }
+    void alloc(size_t cnt)
+    {
+        release();
+        n = cnt;
+        hip_check(hipMalloc((void**)&p, (cnt > 0 ? cnt : 1) * sizeof(T)), "hipMalloc");
+    }
+    // grow to at least cnt (contents not kept)
+    void reserve(size_t cnt) { if (cnt > n || !p) alloc(cnt + cnt / 4 + 16); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr; n = 0;
+    }
+    void upload(const T* h, size_t cnt, hipStream_t s, size_t off = 0)
+    {
+        if (cnt) hip_check(hipMemcpyAsync(p + off, h, cnt * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+    }
+    void download(T* h, size_t cnt, hipStream_t s, size_t off = 0) const
+    {
+        if (cnt) hip_check(hipMemcpyAsync(h, p + off, cnt * sizeof(T), hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
+    }
+};
+
+inline void dev_sync(hipStream_t s) { hip_check(hipStreamSynchronize(s), "hipStreamSynchronize"); }
+
+// ---------------------------------------------------------------- kernels: fills, gathers, Ruiz (ruiz.hip)
+namespace k {
+void fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
+void fill_f64(double* p, int64_t n, double v, hipStream_t s);
+void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+// dst[o] = src[map[o]] for entries of vs doubles
+void gather_vals(int vs, int64_t cnt, const int64_t* map, const double* src, double* dst, hipStream_t s);
+// Ruiz, one iteration in pieces (erow / ecol: row and column of every entry, int32)
+struct RuizStatus {
+    unsigned long long res_bits;   // max 0.5 |log m| over the lines seen (non-negative double)
+    int64_t bad_row, bad_col;      // first all-zero row / column, INT64_MAX when none
+};
+void ruiz_rowmax(int vs, int64_t nnz, const int32_t* erow, const double* x, unsigned long long* rmax, hipStream_t s);
+void ruiz_line_finish(int64_t n, const unsigned long long* m, double* D, double* scale, RuizStatus* st, int is_col,
+                      hipStream_t s);
+void ruiz_scale_colmax(int vs, int64_t nnz, const int32_t* erow, const int32_t* ecol, double* x, const double* rscale,
+                       unsigned long long* cmax, hipStream_t s);
+void ruiz_scale_cols(int vs, int64_t nnz, const int32_t* ecol, double* x, const double* cscale, hipStream_t s);
+void ruiz_scale_all(int64_t m, double* x, double sc, hipStream_t s);
+void ruiz_scale_vec2(int64_t n, double* a, double* b, double h, hipStream_t s);
+// |x| extremes over all doubles (bits), line maxima by max(|re|, |im|)
+void absminmax(int64_t m, const double* x, unsigned long long* amax, unsigned long long* amin, hipStream_t s);
+void line_absmax(int vs, int64_t nnz, const int32_t* erow, const int32_t* ecol, const double* x,
+                 unsigned long long* lmax /* 2n: rows then columns */, int64_t n, hipStream_t s);
+void min_positive(int64_t m, const unsigned long long* v, unsigned long long* out, hipStream_t s);
+}  // namespace k
+
+}  // namespace fpa

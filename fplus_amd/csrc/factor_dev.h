@@ -1,0 +1,69 @@
+// factor_dev.h -- device side of the supernodal factorization: block / task descriptors and kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#define FPA_SLAB 32        // columns per update slab
+#define FPA_ROW_TILE 64    // rows per update GEMM tile
+#define FPA_PANEL_NB 32    // sub-panel width of the dense panel LU
+
+namespace fpa {
+
+inline int col_tile(int vs) { return vs == 1 ? 64 : 32; }   // panel columns per update tile
+
+// A block: the columns [c0, c0 + w) of one supernode computed in one panel, stored column-major over its rows (ld =
+// nrows): the pivot rows of its columns first, in column order, then the supernode's other rows as they stood at the
+// end of that panel.  Values are the L factor below the unit diagonal (the entries above it are U values, unused by
+// the updates).  rows hold row indices of the permuted matrix.
+struct BlockDesc {
+    double* vals;      // nrows * w scalars (a scalar is 1 or 2 doubles)
+    int32_t* rows;     // nrows
+    int32_t nrows, w;
+    int32_t c0, sn;
+};
+
+// one slab of a block applied to some panel columns: the pivot rows [k0, k1) of the block are solved with the unit
+// lower triangle of the block, the rows [k1, nrows) updated by the rectangle below
+struct SlabTask {
+    int32_t block, k0, k1;
+    int32_t col_off, ncol;   // panel-local column indices at cols[col_off .. col_off + ncol)
+    int32_t pad;
+};
+
+struct GemmTile {
+    int32_t task, row0, col0, pad;   // rows [row0, row0 + 64) of the block, columns [col0, col0 + 64) of the task's list
+};
+
+struct PanelStatus {
+    int32_t err;         // 0, or 1 + the panel-local column of a singular pivot
+    int32_t pad;
+    double best, thr;    // the failing column's best modulus and threshold
+};
+
+// one block column copied out of the panel workspace: dst[r] = Wl[pos[pos_off + r] + ci * ldw], r < nrows
+struct ExtractCol {
+    double* dst;
+    int64_t pos_off;
+    int32_t nrows, ci;
+};
+
+namespace k {
+void extract_cols(int vs, int32_t ncols, const ExtractCol* d, const int32_t* pos, const double* Wl, int64_t ldw,
+                  hipStream_t s);
+void rowmap_set(int64_t cnt, const int32_t* rows, int32_t* rowmap, int reset, hipStream_t s);
+void scatter_cols(int vs, int32_t ncol, const int64_t* Bp, const int32_t* Bi, const double* Bx, int64_t j0,
+                  const int32_t* rowmap, double* W, int64_t ldw, double* colmax, hipStream_t s);
+void slab_solve(int vs, int32_t ntask, const SlabTask* tasks, const int32_t* cols, const BlockDesc* blocks,
+                const int32_t* rowmap, double* W, int64_t ldw, int32_t max_ctiles, hipStream_t s);
+void slab_gemm(int vs, int32_t ntile, const GemmTile* tiles, const SlabTask* tasks, const int32_t* cols,
+               const BlockDesc* blocks, const int32_t* rowmap, double* W, int64_t ldw, hipStream_t s);
+// dense partial-pivot LU of the panel's lower section (nr x P, column-major, ld), sub-panel [a, a + nb)
+void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
+           double pivot_rel, int32_t* piv, PanelStatus* st, hipStream_t s);
+void trsm_sub(int vs, double* Wl, int64_t ld, int32_t P, int32_t a, int32_t nb, const PanelStatus* st, hipStream_t s);
+void gemm_sub(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const PanelStatus* st,
+              hipStream_t s);
+}  // namespace k
+
+}  // namespace fpa

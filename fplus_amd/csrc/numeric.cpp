@@ -1,0 +1,643 @@
+// numeric.cpp -- the FPLUS C interface of the MI355X build: options, the pipelines (permute -> Ruiz -> supernodal LU),
+// the numeric object with its device factors, the solves, the CSC export, and the low-level entry points.
+//
+// Follows the reference's high-level layer (src/numeric.c:15-131, src/pipeline.c:84-127): the same composition of the
+// orders and scalings, the same error convention.  Every arithmetic step runs on the device; a missing device is
+// FPLUS_ERR_GPU.
+#include "host_common.h"
+#include "snfactor.h"
+#include "solve_dev.h"
+#include "symbolic.h"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cstdlib>
+#include <memory>
+#include <new>
+
+using namespace fpa;
+
+namespace {
+
+inline double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// the host side of a permuted pattern: B = A[row_perm, :][:, col_perm], canonical, and src[o] = the entry of A that
+// lands at o (src/pipeline.c:14-82 computes the same arrays)
+void permuted_pattern(int64_t n, const int64_t* Ap, const int64_t* Ai, const int64_t* rp, const int64_t* cp,
+                      std::vector<int64_t>& Bp, std::vector<int64_t>& Bi, std::vector<int64_t>& src)
+{
+    const int64_t nnz = Ap[n];
+    std::vector<int64_t> inv(n), rowp(n + 1, 0), tcol(nnz), tsrc(nnz), next(n + 1);
+    for (int64_t k = 0; k < n; k++) inv[rp[k]] = k;
+    Bp.assign(n + 1, 0);
+    for (int64_t k = 0; k < n; k++) Bp[k + 1] = Bp[k] + Ap[cp[k] + 1] - Ap[cp[k]];
+    for (int64_t q = 0; q < nnz; q++) rowp[inv[Ai[q]] + 1]++;
+    for (int64_t r = 0; r < n; r++) rowp[r + 1] += rowp[r];
+    std::copy(rowp.begin(), rowp.end() - 1, next.begin());
+    for (int64_t k = 0; k < n; k++)
+        for (int64_t q = Ap[cp[k]]; q < Ap[cp[k] + 1]; q++) {
+            const int64_t t = next[inv[Ai[q]]]++;
+            tcol[t] = k;
+            tsrc[t] = q;
+        }
+    Bi.resize(nnz);
+    src.resize(nnz);
+    std::copy(Bp.begin(), Bp.end() - 1, next.begin());
+    for (int64_t r = 0; r < n; r++)
+        for (int64_t t = rowp[r]; t < rowp[r + 1]; t++) {
+            const int64_t o = next[tcol[t]]++;
+            Bi[o] = r;
+            src[o] = tsrc[t];
+        }
+}
+
+std::shared_ptr<PatternCache> make_cache(int64_t n, const int64_t* Ap, const int64_t* Ai, const int64_t* rp, const int64_t* cp,
+                                         hipStream_t s)
+{
+    auto c = std::make_shared<PatternCache>();
+    c->n = n;
+    c->nnz = Ap[n];
+    std::vector<int64_t> src;
+    permuted_pattern(n, Ap, Ai, rp, cp, c->Bp, c->Bi, src);
+    std::vector<int32_t> bi32(c->nnz), ecol(c->nnz);
+    for (int64_t q = 0; q < c->nnz; q++) bi32[q] = (int32_t)c->Bi[q];
+    for (int64_t j = 0; j < n; j++)
+        for (int64_t q = c->Bp[j]; q < c->Bp[j + 1]; q++) ecol[q] = (int32_t)j;
+    c->d_src.alloc(c->nnz);
+    c->d_src.upload(src.data(), c->nnz, s);
+    c->d_Bp.alloc(n + 1);
+    c->d_Bp.upload(c->Bp.data(), n + 1, s);
+    c->d_Bi.alloc(c->nnz);
+    c->d_Bi.upload(bi32.data(), c->nnz, s);
+    c->d_ecol.alloc(c->nnz);
+    c->d_ecol.upload(ecol.data(), c->nnz, s);
+    dev_sync(s);
+    return c;
+}
+
+// Ruiz on the device (src/equil.c:186-441): the range guard, up to max_iter iterations, the fold of the prescale.
+int ruiz_device(int64_t n, int64_t nnz, const int32_t* d_erow, const int32_t* d_ecol, double* d_x, int vs, int max_iter,
+                double tol, double* d_Dr, double* d_Dc, int verbose, int* iters, char* err, size_t errlen)
+{
+    hipStream_t s = devctx().stream;
+    const double t0 = now_s();
+    DevBuf<unsigned long long> d_m(2 * (size_t)(n > 0 ? n : 1) + 2);
+    DevBuf<double> d_sc(2 * (size_t)(n > 0 ? n : 1));
+    DevBuf<k::RuizStatus> d_st(1);
+    unsigned long long* amax = d_m.p + 2 * n;
+    unsigned long long* amin = amax + 1;
+    // range guard
+    int k4 = 0;
+    {
+        const unsigned long long init[2] = {0ULL, (unsigned long long)0x7FF0000000000000ULL};   // 0, +inf
+        hip_check(hipMemcpyAsync(amax, init, sizeof(init), hipMemcpyHostToDevice, s), "upload");
+        k::absminmax(nnz * vs, d_x, amax, amin, s);
+        unsigned long long hv[2];
+        hip_check(hipMemcpyAsync(hv, amax, sizeof(hv), hipMemcpyDeviceToHost, s), "download");
+        dev_sync(s);
+        double vmax, vmin;
+        std::memcpy(&vmax, &hv[0], 8);
+        std::memcpy(&vmin, &hv[1], 8);
+        if (vmax > 0.0 && (vmax > 0x1p500 || vmin < 0x1p-500)) {
+            int emax = ilogb(vmax), emin = ilogb(vmin);
+            if (emax - emin > 990) {
+                k::fill_u64(d_m.p, 2 * n, 0ULL, s);
+                k::line_absmax(vs, nnz, d_erow, d_ecol, d_x, d_m.p, n, s);
+                unsigned long long lo = ULLONG_MAX;
+                hip_check(hipMemcpyAsync(amin, &lo, 8, hipMemcpyHostToDevice, s), "upload");
+                k::min_positive(2 * n, d_m.p, amin, s);
+                hip_check(hipMemcpyAsync(&lo, amin, 8, hipMemcpyDeviceToHost, s), "download");
+                dev_sync(s);
+                double lmin;
+                std::memcpy(&lmin, &lo, 8);
+                const int elmin = ilogb(lmin);
+                if (emax - elmin > 990) {
+                    errf(err, errlen, "line maxima span 2^%d .. 2^%d (entries 2^%d ..): wider than Ruiz equilibration can handle "
+                         "in double precision; factor without equilibration (equilibrate=False)", elmin, emax, emin);
+                    return FPLUS_ERR_BADARG;
+                }
+                emin = elmin;
+            }
+            k4 = -((emax + emin) / 2 / 4) * 4;
+            k::ruiz_scale_all(nnz * vs, d_x, ldexp(1.0, k4), s);
+        }
+    }
+    k::fill_f64(d_Dr, n, 1.0, s);
+    k::fill_f64(d_Dc, n, 1.0, s);
+    int used = 0;
+    double final_res = 0;
+    for (int it = 0; it < max_iter; it++) {
+        k::RuizStatus st0;
+        st0.res_bits = 0;
+        st0.bad_row = st0.bad_col = INT64_MAX;
+        d_st.upload(&st0, 1, s);
+        k::fill_u64(d_m.p, 2 * n, 0ULL, s);
+        k::ruiz_rowmax(vs, nnz, d_erow, d_x, d_m.p, s);
+        k::ruiz_line_finish(n, d_m.p, d_Dr, d_sc.p, d_st.p, 0, s);
+        k::ruiz_scale_colmax(vs, nnz, d_erow, d_ecol, d_x, d_sc.p, d_m.p + n, s);
+        k::ruiz_line_finish(n, d_m.p + n, d_Dc, d_sc.p + n, d_st.p, 1, s);
+        k::ruiz_scale_cols(vs, nnz, d_ecol, d_x, d_sc.p + n, s);
+        k::RuizStatus st;
+        d_st.download(&st, 1, s);
+        dev_sync(s);
+        if (st.bad_row != INT64_MAX) {
+            errf(err, errlen, "Row %ld of the matrix is entirely zero (structural singularity detected during equilibration).", (long)st.bad_row);
+            return FPLUS_ERR_SINGULAR;
+        }
+        if (st.bad_col != INT64_MAX) {
+            errf(err, errlen, "Column %ld of the matrix is entirely zero (structural singularity detected during equilibration).", (long)st.bad_col);
+            return FPLUS_ERR_SINGULAR;
+        }
+        double res;
+        std::memcpy(&res, &st.res_bits, 8);
+        used = it + 1;
+        final_res = res;
+        if (verbose >= 2) printf("[FPLUS]   iter %3d  residual=%.3e\n", it + 1, res);
+        if (res < tol) break;
+    }
+    if (k4 != 0) k::ruiz_scale_vec2(n, d_Dr, d_Dc, ldexp(1.0, k4 / 2), s);
+    dev_sync(s);
+    if (verbose >= 1)
+        printf("[FPLUS] Ruiz %s: %d iters, final res=%.3e, time=%.3fs (device)\n", final_res < tol ? "converged" : "max_iter reached",
+               used, final_res, now_s() - t0);
+    if (iters) *iters = used;
+    return FPLUS_OK;
+}
+
+int gpu_error(const std::exception& e, char* err, size_t errlen)
+{
+    errf(err, errlen, "GPU: %s", e.what());
+    return FPLUS_ERR_GPU;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- the numeric object
+struct fplus_numeric {
+    int64_t n = 0;
+    int is_complex = 0;
+    std::vector<int64_t> p, q;
+    std::vector<double> Dr, Dc;
+    DevFactor F;
+    int ruiz_iters = 0;
+    // solve data on the device
+    DevBuf<int64_t> d_p, d_q, d_Up;
+    DevBuf<int32_t> d_Ui, d_pinv;
+    DevBuf<double> d_Dr, d_Dc;
+    std::vector<int32_t> lev_lf, lev_ub, lev_utf, lev_ltb;   // level offsets into the block lists
+    DevBuf<int32_t> d_blk_lf, d_blk_ub, d_blk_utf, d_blk_ltb;
+    std::vector<int32_t> blk_lf, blk_ub, blk_utf, blk_ltb;
+    // the CSC export, built on first access
+    mutable std::mutex mu;
+    mutable bool exported = false;
+    mutable fplus_factors f{};
+    // timings of the last factorization
+    double t_ruiz = 0, t_factor = 0, t_total = 0;
+};
+
+namespace {
+
+// dependency levels of the blocks for the four sweeps, and the device copies the solves read
+void prepare_solves(fplus_numeric* num, hipStream_t s)
+{
+    DevFactor& F = num->F;
+    const int64_t n = F.n;
+    const int32_t nb = (int32_t)F.blocks.size();
+    std::vector<int32_t> colblk(n > 0 ? n : 1);
+    for (int32_t b = 0; b < nb; b++)
+        for (int32_t c = 0; c < F.blocks[b].w; c++) colblk[F.blocks[b].c0 + c] = b;
+    auto order_levels = [nb](const std::vector<int32_t>& lvl, std::vector<int32_t>& blk, std::vector<int32_t>& off) {
+        int32_t mx = 0;
+        for (int32_t b = 0; b < nb; b++) mx = std::max(mx, lvl[b]);
+        off.assign(mx + 2, 0);
+        for (int32_t b = 0; b < nb; b++) off[lvl[b] + 1]++;
+        for (int32_t L = 0; L <= mx; L++) off[L + 1] += off[L];
+        blk.resize(nb);
+        std::vector<int32_t> fill(off.begin(), off.end() - 1);
+        for (int32_t b = 0; b < nb; b++) blk[fill[lvl[b]]++] = b;
+    };
+    std::vector<int32_t> lvl(nb > 0 ? nb : 1, 0), rowlvl(n > 0 ? n : 1, 0);
+    // L y = b: a block waits for every block that writes its pivot rows
+    for (int32_t b = 0; b < nb; b++) {
+        const BlockDesc& B = F.blocks[b];
+        int32_t L = 0;
+        for (int32_t c = 0; c < B.w; c++) L = std::max(L, rowlvl[B.c0 + c]);
+        lvl[b] = L;
+        for (int32_t r = B.w; r < B.nrows; r++) {
+            int32_t& rl = rowlvl[F.pinv[F.block_rows[b][r]]];
+            rl = std::max(rl, L + 1);
+        }
+    }
+    order_levels(lvl, num->blk_lf, num->lev_lf);
+    // L^T x = y: a block waits for the blocks holding its rows below
+    for (int32_t b = nb - 1; b >= 0; b--) {
+        const BlockDesc& B = F.blocks[b];
+        int32_t L = 0;
+        for (int32_t r = B.w; r < B.nrows; r++) L = std::max(L, lvl[colblk[F.pinv[F.block_rows[b][r]]]] + 1);
+        lvl[b] = L;
+    }
+    // lvl is rewritten in place: blocks after b were assigned before b reads them
+    order_levels(lvl, num->blk_ltb, num->lev_ltb);
+    // U x = y: a block waits for every block whose columns have U entries in its rows
+    std::fill(rowlvl.begin(), rowlvl.end(), 0);
+    for (int32_t b = nb - 1; b >= 0; b--) {
+        const BlockDesc& B = F.blocks[b];
+        int32_t L = 0;
+        for (int32_t c = 0; c < B.w; c++) L = std::max(L, rowlvl[B.c0 + c]);
+        lvl[b] = L;
+        for (int32_t c = 0; c < B.w; c++)
+            for (int64_t qq = F.Up[B.c0 + c]; qq < F.Up[B.c0 + c + 1]; qq++) {
+                const int64_t i = F.Ui[qq];
+                if (i < B.c0) rowlvl[i] = std::max(rowlvl[i], L + 1);
+            }
+    }
+    order_levels(lvl, num->blk_ub, num->lev_ub);
+    // U^T y = b: a block waits for the blocks holding the rows of its U columns
+    for (int32_t b = 0; b < nb; b++) {
+        const BlockDesc& B = F.blocks[b];
+        int32_t L = 0;
+        for (int32_t c = 0; c < B.w; c++)
+            for (int64_t qq = F.Up[B.c0 + c]; qq < F.Up[B.c0 + c + 1]; qq++) {
+                const int64_t i = F.Ui[qq];
+                if (i < B.c0) L = std::max(L, lvl[colblk[i]] + 1);
+            }
+        lvl[b] = L;
+    }
+    order_levels(lvl, num->blk_utf, num->lev_utf);
+    auto up32 = [s](DevBuf<int32_t>& d, const std::vector<int32_t>& h) { d.alloc(h.size()); d.upload(h.data(), h.size(), s); };
+    up32(num->d_blk_lf, num->blk_lf);
+    up32(num->d_blk_ub, num->blk_ub);
+    up32(num->d_blk_utf, num->blk_utf);
+    up32(num->d_blk_ltb, num->blk_ltb);
+    std::vector<int32_t> ui32(F.Ui.size()), pinv32(n > 0 ? n : 1);
+    for (size_t q = 0; q < F.Ui.size(); q++) ui32[q] = (int32_t)F.Ui[q];
+    for (int64_t i = 0; i < n; i++) pinv32[i] = (int32_t)F.pinv[i];
+    up32(num->d_Ui, ui32);
+    up32(num->d_pinv, pinv32);
+    num->d_Up.alloc(n + 1);
+    num->d_Up.upload(F.Up.data(), n + 1, s);
+    num->d_p.alloc(n > 0 ? n : 1);
+    num->d_p.upload(num->p.data(), n, s);
+    num->d_q.alloc(n > 0 ? n : 1);
+    num->d_q.upload(num->q.data(), n, s);
+    num->d_Dr.alloc(n > 0 ? n : 1);
+    num->d_Dr.upload(num->Dr.data(), n, s);
+    num->d_Dc.alloc(n > 0 ? n : 1);
+    num->d_Dc.upload(num->Dc.data(), n, s);
+    dev_sync(s);
+}
+
+// the sweeps on w (n x nrhs, pivot order) in place
+void device_sweeps(const fplus_numeric* num, double* w, int32_t nrhs, int trans, hipStream_t s)
+{
+    const DevFactor& F = num->F;
+    const int vs = F.vs;
+    const BlockDesc* blocks = F.d_blocks.p;
+    const int64_t n = F.n;
+    if (trans == 0) {
+        for (size_t L = 0; L + 1 < num->lev_lf.size(); L++)
+            k::lfwd(vs, num->lev_lf[L + 1] - num->lev_lf[L], num->d_blk_lf.p + num->lev_lf[L], blocks, num->d_pinv.p, w, n, nrhs, s);
+        for (size_t L = 0; L + 1 < num->lev_ub.size(); L++)
+            k::ubwd(vs, num->lev_ub[L + 1] - num->lev_ub[L], num->d_blk_ub.p + num->lev_ub[L], blocks, num->d_Up.p, num->d_Ui.p,
+                    F.d_Ux.p, w, n, nrhs, s);
+    } else {
+        const int cj = trans == 2 && vs == 2;
+        for (size_t L = 0; L + 1 < num->lev_utf.size(); L++)
+            k::utfwd(vs, num->lev_utf[L + 1] - num->lev_utf[L], num->d_blk_utf.p + num->lev_utf[L], blocks, num->d_Up.p,
+                     num->d_Ui.p, F.d_Ux.p, w, n, nrhs, cj, s);
+        for (size_t L = 0; L + 1 < num->lev_ltb.size(); L++)
+            k::ltbwd(vs, num->lev_ltb[L + 1] - num->lev_ltb[L], num->d_blk_ltb.p + num->lev_ltb[L], blocks, num->d_pinv.p, w, n,
+                     nrhs, cj, s);
+    }
+}
+
+// the factors as CSC arrays on the host (src/factor_sn.c:741-808 layout): L with the unit diagonal first, then the
+// supernode's later pivot rows, then its other rows ascending by pivot index; U as stored (segments, diagonal last)
+void export_factors(const DevFactor& F, int is_complex, fplus_factors* f)
+{
+    const int64_t n = F.n;
+    const int vs = F.vs;
+    const size_t sz = vs * sizeof(double);
+    hipStream_t s = devctx().stream;
+    std::memset(f, 0, sizeof(*f));
+    f->n = n; f->is_complex = is_complex; f->csc = 1; f->nsupernodes = F.nsn;
+    int64_t nnzL = 0;
+    for (int64_t sidx = 0; sidx < F.nsn; sidx++) {
+        const int64_t w = F.sn_first[sidx + 1] - F.sn_first[sidx], nr = (int64_t)F.sn_rows[sidx].size();
+        nnzL += w * nr - w * (w - 1) / 2;
+    }
+    f->Lp = (int64_t*)malloc((size_t)(n + 1) * sizeof(int64_t));
+    f->Li = (int64_t*)malloc((size_t)(nnzL > 0 ? nnzL : 1) * sizeof(int64_t));
+    f->Lx = malloc((size_t)(nnzL > 0 ? nnzL : 1) * sz);
+    f->Up = (int64_t*)malloc((size_t)(n + 1) * sizeof(int64_t));
+    f->Ui = (int64_t*)malloc((size_t)(F.Ui.size() > 0 ? F.Ui.size() : 1) * sizeof(int64_t));
+    f->Ux = malloc((size_t)(F.nnzU > 0 ? F.nnzU : 1) * sz);
+    f->p = (int64_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    f->q = (int64_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    if (!f->Lp || !f->Li || !f->Lx || !f->Up || !f->Ui || !f->Ux || !f->p || !f->q) { fplus_factors_free(f); throw std::bad_alloc(); }
+    std::memcpy(f->Up, F.Up.data(), (n + 1) * sizeof(int64_t));
+    std::memcpy(f->Ui, F.Ui.data(), F.Ui.size() * sizeof(int64_t));
+    F.d_Ux.download((double*)f->Ux, (size_t)F.nnzU * vs, s);
+    for (int64_t i = 0; i < n; i++) { f->p[i] = F.p[i]; f->q[i] = i; }
+    // block values to the host, block by block
+    std::vector<std::vector<double>> bv(F.blocks.size());
+    for (size_t b = 0; b < F.blocks.size(); b++) {
+        const BlockDesc& B = F.blocks[b];
+        bv[b].resize((size_t)B.nrows * B.w * vs);
+        hip_check(hipMemcpyAsync(bv[b].data(), B.vals, bv[b].size() * sizeof(double), hipMemcpyDeviceToHost, s), "download L");
+    }
+    dev_sync(s);
+    std::vector<int32_t> colblk(n > 0 ? n : 1);
+    for (size_t b = 0; b < F.blocks.size(); b++)
+        for (int32_t c = 0; c < F.blocks[b].w; c++) colblk[F.blocks[b].c0 + c] = (int32_t)b;
+    std::vector<int64_t> posin(n > 0 ? n : 1, -1);   // block row of a row in the current column's block
+    double* Lx = (double*)f->Lx;
+    int64_t o = 0;
+    std::vector<std::pair<int64_t, int64_t>> ord;
+    for (int64_t sidx = 0; sidx < F.nsn; sidx++) {
+        const std::vector<int64_t>& rows = F.sn_rows[sidx];
+        const int64_t c0 = F.sn_first[sidx], w = F.sn_first[sidx + 1] - c0, nr = (int64_t)rows.size();
+        ord.clear();
+        for (int64_t t = w; t < nr; t++) ord.push_back({F.pinv[rows[t]], rows[t]});
+        std::sort(ord.begin(), ord.end());
+        int32_t curb = -1;
+        for (int64_t cc = 0; cc < w; cc++) {
+            const int64_t j = c0 + cc;
+            const int32_t b = colblk[j];
+            const BlockDesc& B = F.blocks[b];
+            if (b != curb) {
+                if (curb >= 0) for (int32_t r : F.block_rows[curb]) posin[r] = -1;
+                for (int32_t r = 0; r < B.nrows; r++) posin[F.block_rows[b][r]] = r;
+                curb = b;
+            }
+            const int64_t bc = j - B.c0;
+            const double* col = bv[b].data() + (size_t)bc * B.nrows * vs;
+            f->Lp[j] = o;
+            f->Li[o] = j;
+            Lx[o * vs] = 1.0;
+            if (vs == 2) Lx[o * vs + 1] = 0.0;
+            o++;
+            auto put = [&](int64_t row, int64_t pidx) {
+                f->Li[o] = pidx;
+                const int64_t r = posin[row];
+                for (int v = 0; v < vs; v++) Lx[o * vs + v] = r >= 0 ? col[r * vs + v] : 0.0;
+                o++;
+            };
+            for (int64_t rr = cc + 1; rr < w; rr++) put(rows[rr], c0 + rr);
+            for (const auto& pr : ord) put(pr.second, pr.first);
+        }
+        if (curb >= 0) for (int32_t r : F.block_rows[curb]) posin[r] = -1;
+    }
+    f->Lp[n] = o;
+}
+
+int check_options(const fplus_options* o, char* err, size_t errlen)
+{
+    if (o->pivot_strategy != FPLUS_PIVOT_PARTIAL && o->pivot_strategy != FPLUS_PIVOT_ROOK && o->pivot_strategy != FPLUS_PIVOT_FULL) {
+        errf(err, errlen, "pivot_strategy must be FPLUS_PIVOT_PARTIAL, _ROOK or _FULL (got %d)", o->pivot_strategy);
+        return FPLUS_ERR_BADARG;
+    }
+    if (!(o->drop_tol >= 0.0)) { errf(err, errlen, "drop_tol must be >= 0 (got %g)", o->drop_tol); return FPLUS_ERR_BADARG; }
+    if (o->pivot_strategy != FPLUS_PIVOT_PARTIAL || o->drop_tol != 0.0) {
+        errf(err, errlen, "rook / full pivoting and incomplete LU (drop_tol > 0) use the row kernel, which this MI355X build "
+             "does not provide: use pivot_strategy FPLUS_PIVOT_PARTIAL with drop_tol 0");
+        return FPLUS_ERR_BADARG;
+    }
+    if (!(o->pivot_rel >= 0.0)) { errf(err, errlen, "pivot_rel must be >= 0"); return FPLUS_ERR_BADARG; }
+    return FPLUS_OK;
+}
+
+int64_t panel_width(int64_t panel)
+{
+    // the structure does not depend on the width: 256 columns, the widest block the solve kernels take
+    (void)panel;
+    return 256;
+}
+
+// the common core: values on the device (d_Bx, permuted, canonical pattern cache c), Ruiz, the factorization
+int factor_core(fplus_numeric* num, const PatternCache& c, DevBuf<double>& d_Bx, int vs, int equilibrate, int ruiz_max_iter,
+                double ruiz_tol, int64_t relax_rows, int64_t panel, double pivot_rel, int verbose, DevBuf<double>& d_Drp,
+                DevBuf<double>& d_Dcp, char* err, size_t errlen)
+{
+    hipStream_t s = devctx().stream;
+    const int64_t n = c.n;
+    const double t0 = now_s();
+    d_Drp.alloc(n > 0 ? n : 1);
+    d_Dcp.alloc(n > 0 ? n : 1);
+    int rc = FPLUS_OK;
+    if (equilibrate) rc = ruiz_device(n, c.nnz, c.d_Bi.p, c.d_ecol.p, d_Bx.p, vs, ruiz_max_iter, ruiz_tol, d_Drp.p, d_Dcp.p, verbose,
+                                      &num->ruiz_iters, err, errlen);
+    else { k::fill_f64(d_Drp.p, n, 1.0, s); k::fill_f64(d_Dcp.p, n, 1.0, s); }
+    if (rc != FPLUS_OK) return rc;
+    const double t1 = now_s();
+    SnOptions so;
+    so.relax_rows = relax_rows;
+    so.pivot_rel = pivot_rel;
+    so.panel = panel_width(panel);
+    so.verbose = verbose;
+    if (verbose >= 1)
+        printf("[FPLUS] LU factorisation start: n=%ld, nnz(A)=%ld, %s, supernodal left-looking (device), pivot=partial, relax_rows=%ld, panel=%ld\n",
+               (long)n, (long)c.nnz, vs == 2 ? "complex128" : "float64", (long)relax_rows, (long)so.panel);
+    rc = sn_factor_device(n, c.Bp.data(), c.Bi.data(), c.d_Bp.p, c.d_Bi.p, d_Bx.p, vs, so, num->F, err, errlen);
+    num->t_ruiz = t1 - t0;
+    num->t_factor = now_s() - t1;
+    if (rc == FPLUS_OK && verbose >= 1)
+        printf("[FPLUS] LU factorisation complete: nnz(U)=%ld, supernodes=%ld, panels=%ld, levels=%ld, time=%.3fs (host %.3fs, wait %.3fs)\n",
+               (long)num->F.nnzU, (long)num->F.nsn, (long)num->F.panels, (long)num->F.levels, num->t_factor, num->F.t_host, num->F.t_wait);
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+void fplus_factors_free(fplus_factors* f)
+{
+    if (!f) return;
+    free(f->Lp); free(f->Li); free(f->Lx); free(f->Up); free(f->Ui); free(f->Ux); free(f->p); free(f->q);
+    f->Lp = f->Li = f->Up = f->Ui = f->p = f->q = nullptr;
+    f->Lx = f->Ux = nullptr;
+}
+
+void fplus_default_options(fplus_options* o)
+{
+    o->pin_dense = 1; o->tau_density = 0.5; o->tau_min_nnz = 10; o->postorder = 1;
+    o->equilibrate = 1; o->ruiz_max_iter = 20; o->ruiz_tol = 1e-2;
+    o->pivot_rel = FPLUS_PIVOT_REL_DEFAULT; o->pivot_strategy = FPLUS_PIVOT_PARTIAL; o->drop_tol = 0.0;
+    o->relax_rows = 32; o->panel = 512;
+    o->use_gpu = 1; o->gpu_min_flops = 5e7; o->verbose = 0;
+}
+
+int fplus_analyze_opts(int64_t n, const int64_t* Ap, const int64_t* Ai, const fplus_options* opt, fplus_symbolic** sym,
+                       char* err, size_t errlen)
+{
+    fplus_options d;
+    if (!opt) { fplus_default_options(&d); opt = &d; }
+    return fplus_analyze(n, Ap, Ai, opt->pin_dense, opt->tau_density, opt->tau_min_nnz, nullptr, nullptr, opt->postorder, sym, err, errlen);
+}
+
+void fplus_numeric_free(fplus_numeric* num)
+{
+    if (!num) return;
+    fplus_factors_free(&num->f);
+    delete num;
+}
+
+int fplus_factor(const fplus_symbolic* sym, const int64_t* Ap, const int64_t* Ai, const void* Ax, int is_complex,
+                 const fplus_options* opt, fplus_numeric** out, char* err, size_t errlen)
+{
+    *out = nullptr;
+    fplus_options d;
+    if (!opt) { fplus_default_options(&d); opt = &d; }
+    int rc = check_options(opt, err, errlen);
+    if (rc != FPLUS_OK) return rc;
+    const int64_t n = sym->n;
+    if (Ap[n] != sym->nnz) {
+        errf(err, errlen, "the matrix has %ld nonzeros, the symbolic analysis %ld", (long)Ap[n], (long)sym->nnz);
+        return FPLUS_ERR_BADARG;
+    }
+    rc = check_compressed(n, Ap, Ai, true, err, errlen);
+    if (rc == FPLUS_OK) rc = check_finite(Ap[n], Ax, is_complex, err, errlen);
+    if (rc != FPLUS_OK) return rc;
+    const int vs = is_complex ? 2 : 1;
+    std::unique_ptr<fplus_numeric> num;
+    try {
+        const double t0 = now_s();
+        hipStream_t s = devctx().stream;
+        std::shared_ptr<PatternCache> c;
+        {
+            std::lock_guard<std::mutex> g(sym->mu);
+            if (!sym->cache) sym->cache = make_cache(n, Ap, Ai, sym->row_perm.data(), sym->col_perm.data(), s);
+            c = sym->cache;
+        }
+        num.reset(new fplus_numeric());
+        num->n = n;
+        num->is_complex = is_complex;
+        const int64_t nnz = c->nnz;
+        DevBuf<double> d_Ax((size_t)(nnz > 0 ? nnz : 1) * vs), d_Bx((size_t)(nnz > 0 ? nnz : 1) * vs);
+        d_Ax.upload((const double*)Ax, (size_t)nnz * vs, s);
+        k::gather_vals(vs, nnz, c->d_src.p, d_Ax.p, d_Bx.p, s);
+        DevBuf<double> d_Drp, d_Dcp;
+        rc = factor_core(num.get(), *c, d_Bx, vs, opt->equilibrate, opt->ruiz_max_iter, opt->ruiz_tol, opt->relax_rows, opt->panel,
+                         opt->pivot_rel, opt->verbose, d_Drp, d_Dcp, err, errlen);
+        if (rc != FPLUS_OK) return rc;
+        std::vector<double> Drp(n), Dcp(n);
+        d_Drp.download(Drp.data(), n, s);
+        d_Dcp.download(Dcp.data(), n, s);
+        dev_sync(s);
+        const int64_t* rp = sym->row_perm.data();
+        const int64_t* cp = sym->col_perm.data();
+        num->p.resize(n); num->q.resize(n); num->Dr.resize(n); num->Dc.resize(n);
+        for (int64_t k2 = 0; k2 < n; k2++) {
+            num->p[k2] = rp[num->F.p[k2]];    // p = row_perm[p_lu]
+            num->q[k2] = cp[k2];              // q = col_perm[q_lu], q_lu the identity
+            num->Dr[rp[k2]] = Drp[k2];
+            num->Dc[cp[k2]] = Dcp[k2];
+        }
+        prepare_solves(num.get(), s);
+        num->t_total = now_s() - t0;
+    } catch (const GpuError& e) {
+        return gpu_error(e, err, errlen);
+    } catch (const std::bad_alloc&) {
+        errf(err, errlen, "out of memory");
+        return FPLUS_ERR_NOMEM;
+    } catch (const std::logic_error& e) {
+        errf(err, errlen, "%s", e.what());
+        return FPLUS_ERR_INTERNAL;
+    }
+    *out = num.release();
+    return FPLUS_OK;
+}
+
+int fplus_solve(const fplus_numeric* num, void* b, int64_t nrhs, int64_t ldb, int trans, char* err, size_t errlen)
+{
+    const int64_t n = num->n;
+    if (trans < 0 || trans > 2 || nrhs < 0 || ldb < n) { errf(err, errlen, "fplus_solve: bad trans, nrhs or ldb"); return FPLUS_ERR_BADARG; }
+    if (nrhs == 0 || n == 0) return FPLUS_OK;
+    const int vs = num->F.vs;
+    const int tr = vs == 1 ? (trans ? 1 : 0) : trans;
+    try {
+        hipStream_t s = devctx().stream;
+        const int64_t G = 64;   // right-hand sides per pass
+        const int64_t g = std::min(nrhs, G);
+        DevBuf<double> d_b((size_t)(n * g * vs)), d_w((size_t)(n * g * vs));
+        for (int64_t r0 = 0; r0 < nrhs; r0 += g) {
+            const int32_t m = (int32_t)std::min(g, nrhs - r0);
+            double* hb = (double*)b + (size_t)(r0 * ldb) * vs;
+            for (int32_t r = 0; r < m; r++) d_b.upload(hb + (size_t)r * ldb * vs, (size_t)n * vs, s, (size_t)r * n * vs);
+            if (tr == 0) k::gather_scale(vs, n, m, num->d_p.p, num->d_Dr.p, d_b.p, n, d_w.p, s);
+            else k::gather_scale(vs, n, m, num->d_q.p, num->d_Dc.p, d_b.p, n, d_w.p, s);
+            device_sweeps(num, d_w.p, m, tr, s);
+            if (tr == 0) k::scatter_scale(vs, n, m, num->d_q.p, num->d_Dc.p, d_w.p, d_b.p, n, s);
+            else k::scatter_scale(vs, n, m, num->d_p.p, num->d_Dr.p, d_w.p, d_b.p, n, s);
+            for (int32_t r = 0; r < m; r++) d_b.download(hb + (size_t)r * ldb * vs, (size_t)n * vs, s, (size_t)r * n * vs);
+            dev_sync(s);
+        }
+    } catch (const GpuError& e) {
+        return gpu_error(e, err, errlen);
+    } catch (const std::bad_alloc&) {
+        errf(err, errlen, "out of memory");
+        return FPLUS_ERR_NOMEM;
+    }
+    return FPLUS_OK;
+}
+
+const fplus_factors* fplus_numeric_factors(const fplus_numeric* num)
+{
+    std::lock_guard<std::mutex> g(num->mu);
+    if (!num->exported) {
+        try {
+            export_factors(num->F, num->is_complex, &num->f);
+            num->exported = true;
+        } catch (...) {
+            return nullptr;
+        }
+    }
+    return &num->f;
+}
+const int64_t* fplus_numeric_p(const fplus_numeric* num) { return num->p.data(); }
+const int64_t* fplus_numeric_q(const fplus_numeric* num) { return num->q.data(); }
+const double* fplus_numeric_Dr(const fplus_numeric* num) { return num->Dr.data(); }
+const double* fplus_numeric_Dc(const fplus_numeric* num) { return num->Dc.data(); }
+
+}  // extern "C"
+
+namespace fpa {
+int ruiz_device_entry(int64_t n, int64_t nnz, const int32_t* d_erow, const int32_t* d_ecol, double* d_x, int vs, int max_iter,
+                      double tol, double* d_Dr, double* d_Dc, int verbose, int* iters, char* err, size_t errlen)
+{
+    return ruiz_device(n, nnz, d_erow, d_ecol, d_x, vs, max_iter, tol, d_Dr, d_Dc, verbose, iters, err, errlen);
+}
+void export_factors_entry(const DevFactor& F, int is_complex, fplus_factors* f) { export_factors(F, is_complex, f); }
+}  // namespace fpa
+
+#include "fplus_amd.h"
+
+extern "C" int fplus_amd_numeric_stats(const fplus_numeric* num, fplus_amd_stats* st)
+{
+    if (!num || !st) return FPLUS_ERR_BADARG;
+    const DevFactor& F = num->F;
+    std::memset(st, 0, sizeof(*st));
+    for (int64_t s = 0; s < F.nsn; s++) {
+        const int64_t w = F.sn_first[s + 1] - F.sn_first[s], nr = (int64_t)F.sn_rows[s].size();
+        st->nnzL += w * nr - w * (w - 1) / 2;
+    }
+    st->nnzU = F.nnzU;
+    st->nsupernodes = F.nsn;
+    st->nblocks = (int64_t)F.blocks.size();
+    st->panels = F.panels;
+    st->levels = F.levels;
+    st->flops = F.flops;
+    st->gemm_flops = F.gemm_flops;
+    st->t_ruiz = num->t_ruiz;
+    st->t_factor = num->t_factor;
+    st->t_host = F.t_host;
+    st->t_wait = F.t_wait;
+    st->t_total = num->t_total;
+    st->ruiz_iters = num->ruiz_iters;
+    return FPLUS_OK;
+}

@@ -1,0 +1,626 @@
+// snfactor.cpp -- the supernodal left-looking LU with partial pivoting: the host runs the symbolic half of the
+// reference's panel kernel (reference src/factor_sn.c:97-210 search / relaxed joins, src/factor_sn_panel.inc:30-390
+// the three phases, src/factor_sn.c:609-808 the driver), the device every arithmetic operation.
+//
+// Per panel of P columns:
+//   1. host: each column's depth-first search over the supernodes complete before the panel (its reached segments and
+//      non-pivotal rows), the union search giving a topological order, the update tasks and their dependency levels,
+//      the rows the panel touches (the workspace rows);
+//   2. device: the panel's columns of A scattered into the workspace, every earlier supernode applied slab by slab
+//      (TRSM, then GEMM + relative-index scatter), one launch pair per level;
+//   3. device: dense partial-pivot LU of the workspace rows not yet pivotal; the P pivot positions come back;
+//   4. host: per column, the within-panel search, the U pattern, the relaxed-join decision and the new supernodes,
+//      pruning -- the reference's logic, fed with the device's pivots;
+//   5. device: the new supernode blocks and the U values copied out of the workspace.
+// The partition, the patterns and the pivot order are those of the reference (up to pivot candidates within
+// round-off of each other); the values agree to round-off (another summation order).
+#include "snfactor.h"
+
+#include "host_common.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <numeric>
+#include <stdexcept>
+
+namespace fpa {
+namespace {
+
+inline double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct HSn {
+    int64_t c0 = 0, w = 0;
+    std::vector<int64_t> ri;      // rows: the pivot rows of its columns first (column order), then the others
+    std::vector<int64_t> srch;    // closed: its rows past the pivot rows, the search list
+    int64_t srch_end = -1;        // end of the pivotal part of srch once pruned
+    bool closed = false;
+    double nzero = 0;             // explicit zeros from relaxed joins
+    std::vector<int32_t> blocks;  // device blocks in column order
+    int64_t nr() const { return (int64_t)ri.size(); }
+};
+
+// device arena of fixed-address chunks
+template <class T> struct Arena {
+    std::vector<DevBuf<T>>* chunks;
+    size_t used = 0, cap = 0;
+    size_t min_chunk;
+    T* take(size_t cnt)
+    {
+        if (chunks->empty() || used + cnt > cap) {
+            const size_t sz = std::max(min_chunk, cnt);
+            chunks->emplace_back(sz);
+            used = 0;
+            cap = sz;
+        }
+        T* p = chunks->back().p + used;
+        used += (cnt + 7) & ~(size_t)7;
+        return p;
+    }
+};
+
+class Driver {
+public:
+    Driver(int64_t n, const int64_t* Bp, const int64_t* Bi, const int64_t* d_Bp, const int32_t* d_Bi, const double* d_Bx,
+           int vs, const SnOptions& opt, DevFactor& F, char* err, size_t errlen)
+        : n_(n), Bp_(Bp), Bi_(Bi), d_Bp_(d_Bp), d_Bi_(d_Bi), d_Bx_(d_Bx), vs_(vs), opt_(opt), F_(F), err_(err), errlen_(errlen)
+    {
+    }
+    int run();
+
+private:
+    // the search (reference src/factor_sn.c:97-154)
+    const int64_t* srch_rows(int64_t s, int64_t open, int64_t* end) const
+    {
+        const HSn& S = sn_[s];
+        if (s == open) { *end = S.nr() - S.w; return S.ri.data() + S.w; }
+        *end = S.srch_end >= 0 ? S.srch_end : S.nr() - S.w;
+        return S.srch.data();
+    }
+    int64_t dfs(int64_t root, int64_t col, int64_t top, int64_t stamp, int64_t open, int64_t* ncand)
+    {
+        int64_t sp = 0;
+        stack_[0] = root; smark_[root] = stamp; seg0_[root] = col; pos_[root] = 0;
+        while (sp >= 0) {
+            const int64_t s = stack_[sp];
+            int64_t end;
+            const int64_t* rows = srch_rows(s, open, &end);
+            int64_t q = pos_[s], t = -1;
+            for (; q < end; q++) {
+                const int64_t i = rows[q], k = pinv_[i];
+                if (k < 0) {
+                    if (rmark_[i] != stamp) { rmark_[i] = stamp; cand_[(*ncand)++] = i; }
+                    continue;
+                }
+                const int64_t u = col_sn_[k];
+                if (smark_[u] == stamp) { if (k < seg0_[u]) seg0_[u] = k; continue; }
+                t = u; smark_[u] = stamp; seg0_[u] = k;
+                break;
+            }
+            if (t >= 0) { pos_[s] = q + 1; pos_[t] = 0; stack_[++sp] = t; }
+            else { sp--; topo_[--top] = s; }
+        }
+        return top;
+    }
+    bool relax_ok(const HSn& S, int64_t n_miss, int64_t extra) const
+    {
+        const int64_t mm = n_miss + extra;
+        if (!relax_auto_) return mm <= relax_;
+        if (mm > rz_cap_) return false;
+        const int64_t w = S.w + 1, r = S.nr() + n_miss;
+        if (w <= rz_w_[0]) return true;
+        const double z = S.nzero + (double)n_miss * (double)S.w + (double)extra, tot = (double)w * (double)r;
+        const double zmax = w <= rz_w_[1] ? rz_z_[0] : (w <= rz_w_[2] ? rz_z_[1] : rz_z_[2]);
+        return z <= zmax * tot;
+    }
+    void prune(int64_t s, int64_t pivrow)
+    {
+        HSn& S = sn_[s];
+        if (!S.closed || S.srch_end >= 0) return;
+        const int64_t len = (int64_t)S.srch.size();
+        int64_t* rows = S.srch.data();
+        bool hit = false;
+        for (int64_t r = 0; r < len; r++) if (rows[r] == pivrow) { hit = true; break; }
+        if (!hit) return;
+        int64_t a = 0, b = len;
+        while (a < b) {
+            if (pinv_[rows[a]] >= 0) { a++; continue; }
+            b--;
+            std::swap(rows[a], rows[b]);
+        }
+        S.srch_end = b;
+    }
+    int panel(int64_t j0, int64_t j1);
+    const BlockDesc* d_blocks_p() const { return F_.d_blocks.p; }
+    void d_blocks_upload()
+    {
+        const size_t nb = F_.blocks.size();
+        if (nb > blocks_up_) F_.d_blocks.upload(F_.blocks.data() + blocks_up_, nb - blocks_up_, s_, blocks_up_);
+        blocks_up_ = nb;
+    }
+    size_t blocks_up_ = 0;
+    void add_block(int64_t s, int64_t b0, int64_t b1, int64_t j0, const std::vector<int32_t>& lpos,
+                   std::vector<ExtractCol>& ex, std::vector<int32_t>& expos);
+
+    int64_t n_;
+    const int64_t *Bp_, *Bi_, *d_Bp_;
+    const int32_t* d_Bi_;
+    const double* d_Bx_;
+    int vs_;
+    SnOptions opt_;
+    DevFactor& F_;
+    char* err_;
+    size_t errlen_;
+
+    std::vector<HSn> sn_;
+    std::vector<int64_t> col_sn_, pinv_, p_, bpos_, rmark_, smark_, seg0_, topo_, cand_, stack_, pos_, miss_;
+    int64_t relax_ = 0, rz_w_[3] = {4, 16, 48}, rz_cap_ = 32;
+    double rz_z_[3] = {0.8, 0.1, 0.05};
+    bool relax_auto_ = false;
+    int64_t open_ = -1;
+    std::vector<int32_t> hrowpos_;    // workspace position of a row, -1 outside the panel
+    std::vector<int32_t> lposmap_;    // final lower-section position of a row after the panel LU
+    std::vector<int64_t> ustamp_, uidx_;
+    std::vector<int64_t> Ui_, Up_;
+    std::vector<int64_t> umap_;       // workspace offsets of the panel's U values
+    // device workspace
+    DevBuf<double> d_W_;
+    DevBuf<int32_t> d_rowmap_, d_univ_, d_cols_, d_piv_, d_pos_;
+    DevBuf<double> d_colmax_;
+    DevBuf<SlabTask> d_tasks_;
+    DevBuf<GemmTile> d_tiles_;
+    DevBuf<PanelStatus> d_status_;
+    DevBuf<int64_t> d_umap_;
+    DevBuf<ExtractCol> d_ex_;
+    Arena<double> vals_{nullptr, 0, 0, 0};
+    Arena<int32_t> rows_{nullptr, 0, 0, 0};
+    hipStream_t s_ = nullptr;
+};
+
+void Driver::add_block(int64_t s, int64_t b0, int64_t b1, int64_t j0, const std::vector<int32_t>& lpos,
+                       std::vector<ExtractCol>& ex, std::vector<int32_t>& expos)
+{
+    HSn& S = sn_[s];
+    const int64_t r0 = b0 - S.c0, nrows = S.nr() - r0, w = b1 - b0;
+    BlockDesc B;
+    B.vals = vals_.take((size_t)(nrows * w * vs_));
+    B.rows = rows_.take((size_t)nrows);
+    B.nrows = (int32_t)nrows;
+    B.w = (int32_t)w;
+    B.c0 = (int32_t)b0;
+    B.sn = (int32_t)s;
+    std::vector<int32_t> hr(nrows);
+    const int64_t pos_off = (int64_t)expos.size();
+    for (int64_t r = 0; r < nrows; r++) {
+        const int64_t row = S.ri[r0 + r];
+        hr[r] = (int32_t)row;
+        if (lpos[row] < 0) throw std::logic_error("internal: a block row outside the panel's lower rows");
+        expos.push_back(lpos[row]);
+    }
+    hip_check(hipMemcpyAsync(B.rows, hr.data(), nrows * sizeof(int32_t), hipMemcpyHostToDevice, s_), "upload block rows");
+    for (int64_t c = 0; c < w; c++) {
+        ExtractCol e;
+        e.dst = B.vals + (size_t)c * nrows * vs_;
+        e.pos_off = pos_off;
+        e.nrows = (int32_t)nrows;
+        e.ci = (int32_t)(b0 - j0 + c);
+        ex.push_back(e);
+    }
+    const int32_t id = (int32_t)F_.blocks.size();
+    F_.blocks.push_back(B);
+    F_.block_rows.push_back(std::move(hr));
+    S.blocks.push_back(id);
+}
+
+int Driver::panel(int64_t j0, int64_t j1)
+{
+    const int64_t np = j1 - j0, n = n_;
+    const int64_t open0 = open_, we_open = open0 >= 0 ? sn_[open0].w : 0;
+    const double th0 = now_s();
+    // ---- phase 1: per-column search over the supernodes complete before j0 (src/factor_sn_panel.inc:34-58)
+    std::vector<int64_t> ext_s, ext_kf, ext_off(np + 1), npl, np_off(np + 1);
+    for (int64_t ci = 0; ci < np; ci++) {
+        const int64_t c = j0 + ci, stamp = c + 1;
+        int64_t top = n, ncand = 0;
+        for (int64_t q = Bp_[c]; q < Bp_[c + 1]; q++) {
+            const int64_t i = Bi_[q], k = pinv_[i];
+            if (k < 0) {
+                if (rmark_[i] != stamp) { rmark_[i] = stamp; cand_[ncand++] = i; }
+                continue;
+            }
+            const int64_t s = col_sn_[k];
+            if (smark_[s] == stamp) { if (k < seg0_[s]) seg0_[s] = k; continue; }
+            top = dfs(s, k, top, stamp, open_, &ncand);
+        }
+        ext_off[ci] = (int64_t)ext_s.size();
+        np_off[ci] = (int64_t)npl.size();
+        for (int64_t t = top; t < n; t++) { ext_s.push_back(topo_[t]); ext_kf.push_back(seg0_[topo_[t]]); }
+        npl.insert(npl.end(), cand_.begin(), cand_.begin() + ncand);
+    }
+    ext_off[np] = (int64_t)ext_s.size();
+    np_off[np] = (int64_t)npl.size();
+    // ---- the union order (src/factor_sn_panel.inc:61-70)
+    std::vector<int64_t> un;
+    {
+        const int64_t us = 2 * n + 1 + j0;
+        int64_t top = n, dummy = 0;
+        for (size_t q = 0; q < ext_s.size(); q++) {
+            if (smark_[ext_s[q]] == us) continue;
+            top = dfs(ext_s[q], ext_kf[q], top, us, open_, &dummy);
+            dummy = 0;
+        }
+        for (int64_t t = top; t < n; t++) { uidx_[topo_[t]] = (int64_t)un.size(); ustamp_[topo_[t]] = us; un.push_back(topo_[t]); }
+    }
+    // ---- the workspace rows: U rows (pivotal before j0) first, then the rows not yet pivotal
+    std::vector<int32_t> univ;
+    for (int64_t ci = 0; ci < np; ci++)
+        for (int64_t q = ext_off[ci]; q < ext_off[ci + 1]; q++) {
+            const HSn& S = sn_[ext_s[q]];
+            const int64_t we = ext_s[q] == open0 ? we_open : S.w;
+            for (int64_t cc = ext_kf[q] - S.c0; cc < we; cc++) {
+                const int64_t row = S.ri[cc];
+                if (hrowpos_[row] < 0) { hrowpos_[row] = (int32_t)univ.size(); univ.push_back((int32_t)row); }
+            }
+        }
+    const int64_t Ru = (int64_t)univ.size();
+    for (int64_t row : npl)
+        if (hrowpos_[row] < 0) { hrowpos_[row] = (int32_t)univ.size(); univ.push_back((int32_t)row); }
+    if (open0 >= 0) {
+        const HSn& S = sn_[open0];
+        for (int64_t r = S.w; r < S.nr(); r++) {
+            const int64_t row = S.ri[r];
+            if (hrowpos_[row] < 0) { hrowpos_[row] = (int32_t)univ.size(); univ.push_back((int32_t)row); }
+        }
+    }
+    const int64_t Rl = (int64_t)univ.size() - Ru;
+    for (int64_t q = Bp_[j0]; q < Bp_[j1]; q++)   // every entry of the panel's columns lands in the workspace
+        if (hrowpos_[Bi_[q]] < 0) { errf(err_, errlen_, "internal: a column entry outside the panel rows"); return FPLUS_ERR_INTERNAL; }
+    const int64_t ldw = std::max<int64_t>((Ru + Rl + 1) & ~(int64_t)1, 2);
+    // ---- update tasks per union supernode and block, with dependency levels
+    const int SL = FPA_SLAB, CT = col_tile(vs_), RT = FPA_ROW_TILE;
+    std::vector<int64_t> ucnt(un.size() + 1, 0);
+    for (size_t q = 0; q < ext_s.size(); q++) ucnt[uidx_[ext_s[q]] + 1]++;
+    for (size_t u = 0; u < un.size(); u++) ucnt[u + 1] += ucnt[u];
+    std::vector<int64_t> ucol(ext_s.size()), ukf(ext_s.size());
+    {
+        std::vector<int64_t> fill(ucnt.begin(), ucnt.end() - 1);
+        for (int64_t ci = 0; ci < np; ci++)
+            for (int64_t q = ext_off[ci]; q < ext_off[ci + 1]; q++) {
+                const int64_t pos = fill[uidx_[ext_s[q]]]++;
+                ucol[pos] = ci;
+                ukf[pos] = ext_kf[q];
+            }
+    }
+    struct TaskL { SlabTask t; int32_t level; };
+    std::vector<TaskL> tl;
+    std::vector<int32_t> cols;
+    std::vector<int32_t> wl(Ru + Rl, 0);
+    int32_t maxlevel = 0;
+    for (size_t u = 0; u < un.size(); u++) {
+        const int64_t s = un[u], c0 = ucnt[u], c1 = ucnt[u + 1];
+        if (c1 == c0) continue;
+        const HSn& S = sn_[s];
+        for (int32_t b : S.blocks) {
+            const BlockDesc& B = F_.blocks[b];
+            const int64_t bb0 = B.c0, bb1 = B.c0 + B.w;
+            int64_t kmin = bb1;
+            const int64_t col_off = (int64_t)cols.size();
+            for (int64_t q = c0; q < c1; q++)
+                if (ukf[q] < bb1) { cols.push_back((int32_t)ucol[q]); kmin = std::min(kmin, ukf[q]); }
+            const int64_t ncol = (int64_t)cols.size() - col_off;
+            if (ncol == 0) continue;
+            const std::vector<int32_t>& br = F_.block_rows[b];
+            for (int64_t k0 = std::max<int64_t>(kmin, bb0) - bb0; k0 < B.w; k0 += SL) {
+                const int64_t k1 = std::min<int64_t>(k0 + SL, B.w);
+                int32_t lev = 0;
+                for (int64_t r = k0; r < k1; r++) {
+                    const int32_t hp = hrowpos_[br[r]];
+                    if (hp < 0) { errf(err_, errlen_, "internal: update row outside the panel rows"); return FPLUS_ERR_INTERNAL; }
+                    lev = std::max(lev, wl[hp]);
+                }
+                lev += 1;
+                for (int64_t r = k1; r < B.nrows; r++) {
+                    const int32_t hp = hrowpos_[br[r]];
+                    if (hp < 0) { errf(err_, errlen_, "internal: update row outside the panel rows"); return FPLUS_ERR_INTERNAL; }
+                    wl[hp] = std::max(wl[hp], lev);
+                }
+                TaskL t;
+                t.t.block = b; t.t.k0 = (int32_t)k0; t.t.k1 = (int32_t)k1;
+                t.t.col_off = (int32_t)col_off; t.t.ncol = (int32_t)ncol; t.t.pad = 0;
+                t.level = lev;
+                tl.push_back(t);
+                maxlevel = std::max(maxlevel, lev);
+                F_.gemm_flops += (vs_ == 1 ? 2.0 : 8.0) * (double)ncol * (double)(k1 - k0) * (double)(B.nrows - k1);
+            }
+        }
+    }
+    std::stable_sort(tl.begin(), tl.end(), [](const TaskL& a, const TaskL& b) { return a.level < b.level; });
+    std::vector<SlabTask> tasks(tl.size());
+    std::vector<int32_t> lev_task(maxlevel + 2, 0), lev_tile(maxlevel + 2, 0), lev_ct(maxlevel + 2, 0);
+    std::vector<GemmTile> tiles;
+    for (size_t i = 0; i < tl.size(); i++) {
+        tasks[i] = tl[i].t;
+        const int32_t L = tl[i].level;
+        lev_task[L + 1] = (int32_t)i + 1;
+        const BlockDesc& B = F_.blocks[tasks[i].block];
+        lev_ct[L] = std::max<int32_t>(lev_ct[L], (tasks[i].ncol + CT - 1) / CT);
+        for (int32_t r0 = tasks[i].k1; r0 < B.nrows; r0 += RT)
+            for (int32_t cc = 0; cc < tasks[i].ncol; cc += CT) {
+                GemmTile g; g.task = (int32_t)i; g.row0 = r0; g.col0 = cc; g.pad = 0;
+                tiles.push_back(g);
+            }
+        lev_tile[L + 1] = (int32_t)tiles.size();
+    }
+    for (int32_t L = 1; L <= maxlevel + 1; L++) {
+        lev_task[L] = std::max(lev_task[L], lev_task[L - 1]);
+        lev_tile[L] = std::max(lev_tile[L], lev_tile[L - 1]);
+    }
+    F_.levels += maxlevel;
+    F_.tasks += (int64_t)tasks.size();
+    F_.t_host += now_s() - th0;
+    // ---- device: workspace, scatter, updates, panel LU
+    d_W_.reserve((size_t)(ldw * np * vs_));
+    d_univ_.reserve(univ.size());
+    d_univ_.upload(univ.data(), univ.size(), s_);
+    k::rowmap_set((int64_t)univ.size(), d_univ_.p, d_rowmap_.p, 0, s_);
+    k::fill_f64(d_W_.p, ldw * np * vs_, 0.0, s_);
+    d_colmax_.reserve(np);
+    k::scatter_cols(vs_, (int32_t)np, d_Bp_, d_Bi_, d_Bx_, j0, d_rowmap_.p, d_W_.p, ldw, d_colmax_.p, s_);
+    if (!tasks.empty()) {
+        d_tasks_.reserve(tasks.size());
+        d_tasks_.upload(tasks.data(), tasks.size(), s_);
+        d_cols_.reserve(cols.size());
+        d_cols_.upload(cols.data(), cols.size(), s_);
+        d_tiles_.reserve(tiles.size());
+        d_tiles_.upload(tiles.data(), tiles.size(), s_);
+        for (int32_t L = 1; L <= maxlevel; L++) {
+            const int32_t t0 = lev_task[L], t1 = lev_task[L + 1];
+            if (t1 <= t0) continue;
+            k::slab_solve(vs_, t1 - t0, d_tasks_.p + t0, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, lev_ct[L], s_);
+            const int32_t g0 = lev_tile[L], g1 = lev_tile[L + 1];
+            k::slab_gemm(vs_, g1 - g0, d_tiles_.p + g0, d_tasks_.p, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, s_);
+        }
+    }
+    double* Wl = d_W_.p + Ru * vs_;
+    PanelStatus zero_st;
+    std::memset(&zero_st, 0, sizeof(zero_st));
+    d_status_.upload(&zero_st, 1, s_);
+    d_piv_.reserve(np);
+    for (int64_t a = 0; a < np; a += FPA_PANEL_NB) {
+        const int32_t nb = (int32_t)std::min<int64_t>(FPA_PANEL_NB, np - a);
+        k::getf2(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_colmax_.p, opt_.pivot_rel, d_piv_.p, d_status_.p, s_);
+        k::trsm_sub(vs_, Wl, ldw, (int32_t)np, (int32_t)a, nb, d_status_.p, s_);
+        k::gemm_sub(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_status_.p, s_);
+    }
+    std::vector<int32_t> piv(np);
+    PanelStatus st;
+    d_piv_.download(piv.data(), np, s_);
+    d_status_.download(&st, 1, s_);
+    const double tw0 = now_s();
+    dev_sync(s_);
+    F_.t_wait += now_s() - tw0;
+    const double th1 = now_s();
+    if (st.err) {
+        const int64_t j = j0 + st.err - 1;
+        errf(err_, errlen_, "Near-zero or zero pivot %.6e at step %ld (threshold %.6e). Matrix is singular.", st.best, (long)j, st.thr);
+        return FPLUS_ERR_SINGULAR;
+    }
+    // ---- replay the row swaps: the pivot row of every panel column, the final positions of the lower rows
+    std::vector<int32_t> lrows(univ.begin() + Ru, univ.end());
+    for (int64_t ci = 0; ci < np; ci++) std::swap(lrows[ci], lrows[piv[ci]]);
+    for (int64_t i = 0; i < Rl; i++) lposmap_[lrows[i]] = (int32_t)i;
+    // ---- phase 3 on the host, with the device's pivots (src/factor_sn_panel.inc:162-390)
+    const size_t u_begin = Ui_.size();
+    umap_.clear();
+    std::vector<std::pair<int64_t, int64_t>> touched;   // (supernode, first column in this panel)
+    for (int64_t ci = 0; ci < np; ci++) {
+        const int64_t j = j0 + ci, stamp2 = n + 1 + j;
+        int64_t ncand = 0, top = n;
+        for (int64_t q = np_off[ci]; q < np_off[ci + 1]; q++) { const int64_t i = npl[q]; cand_[ncand++] = i; rmark_[i] = stamp2; }
+        const int64_t n0 = ncand;
+        if (open0 >= 0 && sn_[open0].w > we_open && smark_[open0] != stamp2) {
+            for (int64_t q = ext_off[ci]; q < ext_off[ci + 1]; q++)
+                if (ext_s[q] == open0) { top = dfs(open0, sn_[open0].c0 + we_open, top, stamp2, open_, &ncand); break; }
+        }
+        for (int64_t q = 0; q < n0; q++) {
+            const int64_t i = cand_[q], k = pinv_[i];
+            if (k < 0) continue;
+            const int64_t s = col_sn_[k];
+            if (smark_[s] == stamp2) { if (k < seg0_[s]) seg0_[s] = k; continue; }
+            top = dfs(s, k, top, stamp2, open_, &ncand);
+        }
+        const int64_t pivrow = lrows[ci];
+        // U(:, j): pre-panel segments, within-panel segments, the diagonal; the value of each entry from the workspace
+        for (int64_t q = ext_off[ci]; q < ext_off[ci + 1]; q++) {
+            const HSn& S = sn_[ext_s[q]];
+            const int64_t we = ext_s[q] == open0 ? we_open : S.w;
+            for (int64_t cc = ext_kf[q] - S.c0; cc < we; cc++) {
+                Ui_.push_back(S.c0 + cc);
+                umap_.push_back((int64_t)hrowpos_[S.ri[cc]] + ci * ldw);
+            }
+        }
+        for (int64_t t = top; t < n; t++) {
+            const HSn& S = sn_[topo_[t]];
+            for (int64_t cc = seg0_[topo_[t]] - S.c0; cc < S.w; cc++) {
+                const int64_t col = S.c0 + cc;
+                Ui_.push_back(col);
+                umap_.push_back(Ru + (col - j0) + ci * ldw);
+            }
+        }
+        Ui_.push_back(j);
+        umap_.push_back(Ru + ci + ci * ldw);
+        Up_[j + 1] = (int64_t)Ui_.size();
+        // L(:, j): join the open supernode (exactly or relaxed), else open a new one
+        std::vector<int64_t>& npv = miss_;   // reused: the candidates still non-pivotal, then the missing rows
+        std::vector<int64_t> npl_l;
+        npl_l.reserve(ncand);
+        for (int64_t q = 0; q < ncand; q++) if (pinv_[cand_[q]] < 0) npl_l.push_back(cand_[q]);
+        int join = 0;
+        int64_t n_miss = 0;
+        if (open_ >= 0) {
+            HSn& S = sn_[open_];
+            int64_t in_S = 0;
+            for (int64_t r : npl_l) {
+                if (bpos_[r] >= S.w) in_S++;
+                else npv[n_miss++] = r;
+            }
+            const int64_t extra = (S.nr() - S.w) - in_S;
+            if (n_miss + extra == 0) join = 1;
+            else if (relax_ok(S, n_miss, extra)) join = 2;
+            if (join == 2) {
+                S.nzero += (double)n_miss * (double)S.w + (double)extra;
+                for (int64_t t = 0; t < n_miss; t++) { bpos_[npv[t]] = S.nr(); S.ri.push_back(npv[t]); }
+            }
+        }
+        if (join) {
+            HSn& S = sn_[open_];
+            const int64_t w = S.w, pp = bpos_[pivrow];
+            if (pp != w) {
+                const int64_t other = S.ri[w];
+                S.ri[w] = pivrow; S.ri[pp] = other;
+                bpos_[pivrow] = w; bpos_[other] = pp;
+            }
+            S.w = w + 1;
+            col_sn_[j] = open_;
+            if (touched.empty() || touched.back().first != open_) touched.push_back({open_, j});
+        } else {
+            if (open_ >= 0) {
+                HSn& S = sn_[open_];
+                S.srch.assign(S.ri.begin() + S.w, S.ri.end());
+                S.closed = true;
+                for (int64_t r : S.ri) bpos_[r] = -1;
+            }
+            const int64_t s = (int64_t)sn_.size();
+            sn_.emplace_back();
+            HSn& S = sn_.back();
+            S.c0 = j; S.w = 1;
+            S.ri.reserve(npl_l.size());
+            S.ri.push_back(pivrow); bpos_[pivrow] = 0;
+            for (int64_t r : npl_l) {
+                if (r == pivrow) continue;
+                bpos_[r] = S.nr();
+                S.ri.push_back(r);
+            }
+            col_sn_[j] = s;
+            open_ = s;
+            touched.push_back({s, j});
+        }
+        pinv_[pivrow] = j; p_[j] = pivrow;
+        // prune the closed supernodes this column reached (pre-panel, then within-panel)
+        for (int64_t q = ext_off[ci]; q < ext_off[ci + 1]; q++) if (ext_s[q] != open_) prune(ext_s[q], pivrow);
+        for (int64_t t = top; t < n; t++) if (topo_[t] != open_) prune(topo_[t], pivrow);
+    }
+    // ---- device: the new blocks and the U values out of the workspace
+    std::vector<ExtractCol> ex;
+    std::vector<int32_t> expos;
+    for (size_t t = 0; t < touched.size(); t++) {
+        const int64_t s = touched[t].first, b0 = touched[t].second;
+        const int64_t b1 = std::min<int64_t>(sn_[s].c0 + sn_[s].w, j1);
+        add_block(s, b0, b1, j0, lposmap_, ex, expos);
+    }
+    d_blocks_upload();
+    if (!ex.empty()) {
+        d_ex_.reserve(ex.size());
+        d_ex_.upload(ex.data(), ex.size(), s_);
+        d_pos_.reserve(expos.size());
+        d_pos_.upload(expos.data(), expos.size(), s_);
+        k::extract_cols(vs_, (int32_t)ex.size(), d_ex_.p, d_pos_.p, Wl, ldw, s_);
+    }
+    {
+        const int64_t cnt = (int64_t)umap_.size();
+        if (F_.nnzU + cnt > (int64_t)(F_.d_Ux.n / vs_)) {
+            DevBuf<double> nb((size_t)std::max<int64_t>(2 * (F_.nnzU + cnt), 1024) * vs_);
+            if (F_.nnzU > 0)
+                hip_check(hipMemcpyAsync(nb.p, F_.d_Ux.p, F_.nnzU * vs_ * sizeof(double), hipMemcpyDeviceToDevice, s_), "grow U");
+            dev_sync(s_);
+            F_.d_Ux = std::move(nb);
+        }
+        d_umap_.reserve(cnt);
+        d_umap_.upload(umap_.data(), cnt, s_);
+        k::gather_vals(vs_, cnt, d_umap_.p, d_W_.p, F_.d_Ux.p + F_.nnzU * vs_, s_);
+        F_.nnzU += cnt;
+        (void)u_begin;
+    }
+    k::rowmap_set((int64_t)univ.size(), d_univ_.p, d_rowmap_.p, 1, s_);
+    // the uploads above read host buffers that go out of scope: finish them before returning
+    dev_sync(s_);
+    for (int32_t row : univ) hrowpos_[row] = -1;
+    for (int32_t row : lrows) lposmap_[row] = -1;
+    F_.t_host += now_s() - th1;
+    F_.panels++;
+    return FPLUS_OK;
+}
+
+int Driver::run()
+{
+    const int64_t n = n_;
+    const size_t nn = (size_t)(n > 0 ? n : 1);
+    s_ = devctx().stream;
+    F_.n = n;
+    F_.vs = vs_;
+    relax_ = opt_.relax_rows > 0 ? opt_.relax_rows : 0;
+    relax_auto_ = opt_.relax_rows < 0;
+    const int64_t nnzA = Bp_[n];
+    if (n <= 4096 && nnzA <= 24 * n) {   // the automatic rule's bounds for small sparse matrices (src/factor_sn.c:644-645)
+        rz_w_[0] = 2; rz_w_[1] = 8; rz_w_[2] = 32; rz_z_[0] = 0.3; rz_z_[1] = 0.03; rz_z_[2] = 0.01; rz_cap_ = 8;
+    }
+    col_sn_.assign(nn, -1); pinv_.assign(nn, -1); p_.assign(nn, -1); bpos_.assign(nn, -1);
+    rmark_.assign(nn, 0); smark_.assign(nn, 0); seg0_.assign(nn, 0); topo_.assign(nn, 0); cand_.assign(nn, 0);
+    stack_.assign(nn, 0); pos_.assign(nn, 0); miss_.assign(nn, 0); ustamp_.assign(nn, 0); uidx_.assign(nn, 0);
+    hrowpos_.assign(nn, -1); lposmap_.assign(nn, -1);
+    sn_.reserve(nn);
+    Up_.assign(n + 1, 0);
+    Ui_.reserve((size_t)(4 * nnzA + n + 16));
+    d_rowmap_.alloc(nn);
+    k::fill_i32(d_rowmap_.p, n, -1, s_);
+    d_status_.alloc(1);
+    F_.d_blocks.alloc(nn);
+    F_.val_chunks.clear();
+    F_.row_chunks.clear();
+    vals_.chunks = &F_.val_chunks;
+    vals_.min_chunk = (size_t)1 << 25;
+    rows_.chunks = &F_.row_chunks;
+    rows_.min_chunk = (size_t)1 << 24;
+    const int64_t P = std::max<int64_t>(1, opt_.panel);
+    for (int64_t j0 = 0; j0 < n; j0 += P) {
+        const int rc = panel(j0, std::min(j0 + P, n));
+        if (rc != FPLUS_OK) return rc;
+        if (opt_.verbose >= 2 && ((j0 / P) % (n / (20 * P) + 1) == 0))
+            printf("[FPLUS]   [%5.1f%%] column %ld/%ld  supernodes=%ld (panel %ld)\n", 100.0 * (double)std::min(j0 + P, n) / (double)n,
+                   (long)std::min(j0 + P, n), (long)n, (long)sn_.size(), (long)P);
+    }
+    // the final structure
+    F_.nsn = (int64_t)sn_.size();
+    F_.sn_first.resize(F_.nsn + 1);
+    F_.sn_rows.resize(F_.nsn);
+    for (int64_t s = 0; s < F_.nsn; s++) { F_.sn_first[s] = sn_[s].c0; F_.sn_rows[s] = std::move(sn_[s].ri); }
+    F_.sn_first[F_.nsn] = n;
+    F_.p = p_;
+    F_.pinv = pinv_;
+    F_.Up = std::move(Up_);
+    F_.Ui = std::move(Ui_);
+    // factor flops c * sum_k l_k u_k (l_k below the diagonal of L(:, k), u_k right of the diagonal in U(k, :))
+    std::vector<int64_t> urow(nn, 0);
+    for (int64_t q = 0; q < (int64_t)F_.Ui.size(); q++) urow[F_.Ui[q]]++;
+    double fl = 0;
+    for (int64_t s = 0; s < F_.nsn; s++) {
+        const int64_t c0 = F_.sn_first[s], c1 = F_.sn_first[s + 1], nr = (int64_t)F_.sn_rows[s].size();
+        for (int64_t k = c0; k < c1; k++) fl += (double)(nr - (k - c0) - 1) * (double)(urow[k] - 1);
+    }
+    F_.flops = (vs_ == 1 ? 2.0 : 8.0) * fl;
+    return FPLUS_OK;
+}
+
+}  // namespace
+
+int sn_factor_device(int64_t n, const int64_t* Bp, const int64_t* Bi, const int64_t* d_Bp, const int32_t* d_Bi,
+                     const double* d_Bx, int vs, const SnOptions& opt, DevFactor& F, char* err, size_t errlen)
+{
+    Driver d(n, Bp, Bi, d_Bp, d_Bi, d_Bx, vs, opt, F, err, errlen);
+    return d.run();
+}
+
+}  // namespace fpa

@@ -1,0 +1,50 @@
+// snfactor.h -- the supernodal partial-pivoting factorization: host symbolic driver + device numerics.
+#pragma once
+#include "device_api.h"
+#include "factor_dev.h"
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+namespace fpa {
+
+// the factorization as the device holds it, plus the host structure the solves and the CSC export need
+struct DevFactor {
+    int64_t n = 0;
+    int vs = 1;                          // doubles per scalar: 1 real, 2 complex
+    int64_t nsn = 0;
+    // host structure
+    std::vector<int64_t> p;              // p[j] = pivot row of column j (rows of the matrix factored)
+    std::vector<int64_t> pinv;
+    std::vector<int64_t> sn_first;       // supernode s: columns sn_first[s] .. sn_first[s+1]-1
+    std::vector<std::vector<int64_t>> sn_rows;   // final rows of each supernode (pivot rows first)
+    std::vector<BlockDesc> blocks;       // host mirror of the block table (device pointers)
+    std::vector<std::vector<int32_t>> block_rows;   // host copy of every block's rows
+    std::vector<int64_t> Up, Ui;         // U in CSC, column j: earlier supernodes' segments, the diagonal last
+    // device
+    DevBuf<BlockDesc> d_blocks;
+    std::vector<DevBuf<double>> val_chunks;
+    std::vector<DevBuf<int32_t>> row_chunks;
+    DevBuf<double> d_Ux;                 // U values, in Ui order
+    int64_t nnzU = 0;
+    // statistics
+    double flops = 0, gemm_flops = 0;
+    int64_t panels = 0, levels = 0, tasks = 0;
+    double t_host = 0, t_wait = 0;
+};
+
+struct SnOptions {
+    int64_t relax_rows = 32;
+    double pivot_rel = 1.4901161193847656e-08;
+    int64_t panel = 256;
+    int verbose = 0;
+};
+
+// factor the n x n CSC matrix whose values are on the device (d_Bx, vs doubles per entry); the host holds its
+// pattern (Bp, Bi); d_Bp / d_Bi are the same pattern on the device.  Throws GpuError / std::bad_alloc; returns
+// FPLUS_OK or FPLUS_ERR_SINGULAR (message in err).
+int sn_factor_device(int64_t n, const int64_t* Bp, const int64_t* Bi, const int64_t* d_Bp, const int32_t* d_Bi,
+                     const double* d_Bx, int vs, const SnOptions& opt, DevFactor& F, char* err, size_t errlen);
+
+}  // namespace fpa
