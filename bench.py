@@ -1,0 +1,203 @@
+"""Benchmark of the FP64 GPU factor + solve path: seconds per system.
+
+A step is one factorization of the workload's matrix from values already resident in HBM (the permutation into the
+analysed order, Ruiz, the supernodal LU) plus one solve of 16 right-hand sides, also resident in HBM, through the C
+interface (fplus_amd_factor_device, fplus_amd_solve_device).  The symbolic analysis is done once, before the timed
+region (it is reused across refactorizations of one pattern).  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+
+The line carries the backward error of every right-hand side of the last step, the roofline of the dominant kernel
+(timed with HIP events on the library's stream over the timed region), and a CPU baseline: the reference's own column
+kernel (oracle/_ref, compiled from the reference's sources without a CBLAS) or the oracle, on one host thread, timed
+on a smaller member of the same matrix family and scaled to the workload by the ratio of the factor flops.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import fplus_amd  # noqa: E402
+from fplus_amd import _lib, gen  # noqa: E402
+
+# name: (generator, keyword arguments, description); "sample" is the CPU baseline's smaller member
+WORKLOADS = {
+    "shiftinvert-16x4x256": (gen.shift_invert, dict(modes=16, nfields=4, ncoef=256), "complex shift-invert pencil"),
+    "shiftinvert-32x4x512": (gen.shift_invert, dict(modes=32, nfields=4, ncoef=512), "complex shift-invert pencil"),
+    "shiftinvert-64x4x1024": (gen.shift_invert, dict(modes=64, nfields=4, ncoef=1024), "complex shift-invert pencil"),
+    "convdiff2d-96": (gen.convdiff2d, dict(g=96), "2D convection-diffusion"),
+    "convdiff3d-32": (gen.convdiff3d, dict(g=32), "3D convection-diffusion"),
+}
+CPU_SAMPLE = {
+    "shiftinvert": (gen.shift_invert, dict(modes=8, nfields=4, ncoef=128)),
+    "convdiff2d": (gen.convdiff2d, dict(g=96)),
+    "convdiff3d": (gen.convdiff3d, dict(g=16)),
+}
+DEFAULT_WORKLOAD = "shiftinvert-16x4x256"
+PEAK_FP64_MFMA = 78.6e12      # MI355X dense FP64 matrix, FLOP/s
+PEAK_HBM = 8.0e12             # MI355X HBM3E, B/s
+NRHS = 16
+
+
+def dev_upload(a):
+    p = C.c_void_p()
+    L = _lib.lib()
+    if L.fplus_amd_dev_alloc(a.nbytes, C.byref(p)) != 0:
+        raise RuntimeError("device allocation failed")
+    if L.fplus_amd_dev_copy(p, C.c_void_p(a.ctypes.data), a.nbytes, 0) != 0:
+        raise RuntimeError("upload failed")
+    return p
+
+
+def dev_download(p, a):
+    if _lib.lib().fplus_amd_dev_copy(C.c_void_p(a.ctypes.data), p, a.nbytes, 1) != 0:
+        raise RuntimeError("download failed")
+
+
+def factor_flops(n, Ap, Ai, Ax, sym):
+    """c * sum_k l_k u_k of the structure (the same count for any device or host factorization of these orders)"""
+    num = fplus_amd.Numeric(sym, Ap, Ai, Ax)
+    return fplus_amd.stats(num)["flops"], num
+
+
+def cpu_baseline(workload_flops, family):
+    """the reference column kernel (or the oracle) on one thread, on the family's CPU sample, scaled by flops"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    fn, kw = CPU_SAMPLE[family]
+    n, Ap, Ai, Ax = fn(**kw)
+    sym = fplus_amd.Symbolic(n, Ap, Ai)
+    fl, _ = factor_flops(n, Ap, Ai, Ax, sym)
+    Bp, Bi, Bx = O.permute(n, Ap, Ai, Ax, sym.row_perm, sym.col_perm)
+    kind = "reference" if O.ref_available() else "port"
+    b = gen.rhs(n, NRHS, np.iscomplexobj(Ax))
+    t0 = time.perf_counter()
+    if kind == "reference":
+        x, Dr, Dc = O.ref_ruiz(n, Bp, Bi, Bx)
+        f = O.ref_factor_sn(n, Bp, Bi, x, relax_rows=32)
+        for r in range(NRHS):
+            xs = np.ascontiguousarray(b[:, r])
+            O.ref().fplus_tri_solve(n, O._p64(f["Lp"]), O._p64(f["Li"]), f["Lx"].ctypes.data, O._p64(f["Up"]), O._p64(f["Ui"]),
+                                    f["Ux"].ctypes.data, xs.ctypes.data, 0, int(np.iscomplexobj(Ax)))
+    else:
+        num = O.Numeric(n, Ap, Ai, Ax, sym.row_perm, sym.col_perm)
+        num.solve(b)
+    t = time.perf_counter() - t0
+    scaled = t * workload_flops / fl
+    desc = ", ".join(f"{k}={v}" for k, v in kw.items())
+    return {"value": scaled, "unit": "s/system", "cores": 1, "kind": kind,
+            "sample": f"{family}({desc}), n={n}: factor (Ruiz + column kernel, relax 32) + {NRHS} solves in {t:.2f} s on one "
+                      f"thread, scaled by the factor flops ratio {workload_flops:.3e}/{fl:.3e}; CPU {platform.processor() or 'x86_64'}, "
+                      f"{os.cpu_count()} logical CPUs on the host"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the path would shard by "
+              "independent systems or elimination subtrees: see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    fn, kw, desc = WORKLOADS[args.workload]
+    n, Ap, Ai, Ax = fn(**kw)
+    cplx = np.iscomplexobj(Ax)
+    vs = 2 if cplx else 1
+    sym = fplus_amd.Symbolic(n, Ap, Ai)
+    L = _lib.lib()
+    opts = fplus_amd.default_options()
+    d_Ax = dev_upload(np.ascontiguousarray(Ax))
+    b = gen.rhs(n, NRHS, cplx)
+    bf = np.asfortranarray(b)
+    d_b = dev_upload(bf)
+    err = C.create_string_buffer(512)
+
+    def step():
+        num = C.c_void_p()
+        rc = L.fplus_amd_factor_device(sym.ptr, Ap.ctypes.data_as(_lib.I64P), Ai.ctypes.data_as(_lib.I64P), d_Ax, int(cplx),
+                                       C.byref(opts), C.byref(num), err, 512)
+        _lib.check(rc, err)
+        rc = L.fplus_amd_solve_device(num, d_b, NRHS, n, 0, err, 512)
+        _lib.check(rc, err)
+        return num
+
+    for _ in range(args.warmup):
+        num = step()
+        L.fplus_numeric_free(num)
+        L.fplus_amd_dev_copy(d_b, C.c_void_p(bf.ctypes.data), bf.nbytes, 0)
+    L.fplus_amd_profile(1)
+    L.fplus_amd_dev_sync()
+    t0 = time.perf_counter()
+    nums = []
+    for _ in range(args.steps):
+        nums.append(step())
+    L.fplus_amd_dev_sync()
+    t1 = time.perf_counter()
+    L.fplus_amd_profile(0)
+    sec = (t1 - t0) / args.steps
+    # correctness of the last step: solve the 16 right-hand sides again from b and check the backward errors
+    L.fplus_amd_dev_copy(d_b, C.c_void_p(bf.ctypes.data), bf.nbytes, 0)
+    _lib.check(L.fplus_amd_solve_device(nums[-1], d_b, NRHS, n, 0, err, 512), err)
+    x = np.empty_like(bf)
+    dev_download(d_b, x)
+    import scipy.sparse as sp
+    A = sp.csc_matrix((Ax, Ai, Ap), shape=(n, n))
+    normA = abs(A).sum(axis=0).max()
+    bes = [float(np.linalg.norm(A @ x[:, r] - b[:, r]) / (normA * np.linalg.norm(x[:, r]) + np.linalg.norm(b[:, r])))
+           for r in range(NRHS)]
+    st = _lib.Stats()
+    L.fplus_amd_numeric_stats(nums[-1], C.byref(st))
+    for nm in nums:
+        L.fplus_numeric_free(nm)
+    # the kernels' sums over the timed region
+    kern = {}
+    for name in ("slab_gemm", "slab_solve", "gemm_sub", "getf2", "trsm_sub", "lfwd_pull", "lfwd_push", "ub_dots", "ub_tri", "utfwd",
+                 "ltbwd", "ruiz_rowmax", "ruiz_scale_colmax", "ruiz_scale_cols"):
+        ms, cnt, work = C.c_double(), C.c_int64(), C.c_double()
+        L.fplus_amd_profile_get(name.encode(), C.byref(ms), C.byref(cnt), C.byref(work))
+        if cnt.value:
+            kern[name] = (ms.value / 1e3, cnt.value, work.value)
+    dom = max(kern, key=lambda k: kern[k][0])
+    t_dom, cnt_dom, work_dom = kern[dom]
+    mfma_kernels = {"slab_gemm", "gemm_sub", "getf2"}
+    bound = "mfma" if dom in mfma_kernels else "hbm"
+    achieved = work_dom / t_dom
+    peak = PEAK_FP64_MFMA if bound == "mfma" else PEAK_HBM
+    roof = {"kernel": dom, "bound": bound, "achieved": achieved / (1e12 if bound == "mfma" else 1e9),
+            "peak": peak / (1e12 if bound == "mfma" else 1e9), "unit": "TFLOP/s" if bound == "mfma" else "GB/s",
+            "frac": achieved / peak, "traffic": None, "launches": cnt_dom,
+            "avg_launch_ms": 1e3 * t_dom / cnt_dom, "work_per_launch": work_dom / cnt_dom}
+    family = args.workload.split("-")[0]
+    cpu = None if args.no_cpu else cpu_baseline(st.flops, family)
+    out = {
+        "metric": "factor+solve s/system at bwd err <=1e-13: mixed/learned vs FP64 GPU vs CPU",
+        "value": sec, "unit": "s/system", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128" if cplx else "f64", "data": "synthetic",
+        "config": {"workload": args.workload, "matrix": desc, **kw, "n": n, "nnz": int(Ap[-1]), "nrhs": NRHS,
+                   "precision": "fp64 (plain GPU drop-in)", "ordering": "fplus_analyze defaults", "relax_rows": 32},
+        "backward_error_max": max(bes),
+        "factor": {"flops": st.flops, "gemm_flops": st.gemm_flops, "nnzL": st.nnzL, "nnzU": st.nnzU,
+                   "supernodes": st.nsupernodes, "panels": st.panels, "levels": st.levels,
+                   "t_ruiz": st.t_ruiz, "t_factor": st.t_factor, "t_host": st.t_host, "gflops": st.flops / sec / 1e9},
+        "kernels": {k: {"s": v[0], "launches": v[1]} for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

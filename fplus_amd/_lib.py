@@ -97,6 +97,15 @@ SIGNATURES = {
     "fplus_numeric_free": (None, [VP]),
     "fplus_amd_symbolic_from_orders": (C.c_int, [C.c_int64, I64P, I64P, I64P, I64P, C.POINTER(VP), C.c_char_p, C.c_size_t]),
     "fplus_amd_numeric_stats": (C.c_int, [VP, VP]),
+    "fplus_amd_factor_device": (C.c_int, [VP, I64P, I64P, VP, C.c_int, C.POINTER(Options), C.POINTER(VP), C.c_char_p,
+                                          C.c_size_t]),
+    "fplus_amd_solve_device": (C.c_int, [VP, VP, C.c_int64, C.c_int64, C.c_int, C.c_char_p, C.c_size_t]),
+    "fplus_amd_dev_alloc": (C.c_int, [C.c_size_t, C.POINTER(VP)]),
+    "fplus_amd_dev_free": (C.c_int, [VP]),
+    "fplus_amd_dev_copy": (C.c_int, [VP, VP, C.c_size_t, C.c_int]),
+    "fplus_amd_dev_sync": (C.c_int, []),
+    "fplus_amd_profile": (C.c_int, [C.c_int]),
+    "fplus_amd_profile_get": (C.c_int, [C.c_char_p, DP, I64P, DP]),
 }
 
 _lib = None

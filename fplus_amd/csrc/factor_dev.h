@@ -6,7 +6,7 @@
 
 #define FPA_SLAB 32        // columns per update slab
 #define FPA_ROW_TILE 64    // rows per update GEMM tile
-#define FPA_PANEL_NB 32    // sub-panel width of the dense panel LU
+#define FPA_PANEL_NB 16    // sub-panel width of the dense panel LU
 
 namespace fpa {
 
@@ -57,7 +57,7 @@ void scatter_cols(int vs, int32_t ncol, const int64_t* Bp, const int32_t* Bi, co
 void slab_solve(int vs, int32_t ntask, const SlabTask* tasks, const int32_t* cols, const BlockDesc* blocks,
                 const int32_t* rowmap, double* W, int64_t ldw, int32_t max_ctiles, hipStream_t s);
 void slab_gemm(int vs, int32_t ntile, const GemmTile* tiles, const SlabTask* tasks, const int32_t* cols,
-               const BlockDesc* blocks, const int32_t* rowmap, double* W, int64_t ldw, hipStream_t s);
+               const BlockDesc* blocks, const int32_t* rowmap, double* W, int64_t ldw, double flops, hipStream_t s);
 // dense partial-pivot LU of the panel's lower section (nr x P, column-major, ld), sub-panel [a, a + nb)
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
            double pivot_rel, int32_t* piv, PanelStatus* st, hipStream_t s);

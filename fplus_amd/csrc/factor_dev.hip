@@ -14,6 +14,7 @@
 #include "dev_common.h"
 #include "device_api.h"
 #include "factor_dev.h"
+#include "prof.h"
 
 namespace fpa {
 namespace {
@@ -345,20 +346,23 @@ void slab_solve(int vs, int32_t ntask, const SlabTask* tasks, const int32_t* col
                 const int32_t* rowmap, double* W, int64_t ldw, int32_t max_ctiles, hipStream_t s)
 {
     if (ntask <= 0) return;
+    ProfScope ps("slab_solve", 0.0, s);
     const unsigned g = (unsigned)(ntask * max_ctiles);
     if (vs == 1) hipLaunchKernelGGL(k_slab_solve<double>, dim3(g), dim3(TB), 0, s, tasks, cols, blocks, rowmap, W, ldw, max_ctiles);
     else hipLaunchKernelGGL(k_slab_solve<Zd>, dim3(g), dim3(TB), 0, s, tasks, cols, blocks, rowmap, W, ldw, max_ctiles);
 }
 void slab_gemm(int vs, int32_t ntile, const GemmTile* tiles, const SlabTask* tasks, const int32_t* cols,
-               const BlockDesc* blocks, const int32_t* rowmap, double* W, int64_t ldw, hipStream_t s)
+               const BlockDesc* blocks, const int32_t* rowmap, double* W, int64_t ldw, double flops, hipStream_t s)
 {
     if (ntile <= 0) return;
+    ProfScope ps("slab_gemm", flops, s);
     if (vs == 1) hipLaunchKernelGGL(k_slab_gemm<double>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw);
     else hipLaunchKernelGGL(k_slab_gemm<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw);
 }
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
            double pivot_rel, int32_t* piv, PanelStatus* st, hipStream_t s)
 {
+    ProfScope ps("getf2", (vs == 1 ? 2.0 : 8.0) * (double)nr * nb * nb / 2, s);
     if (vs == 1) hipLaunchKernelGGL(k_getf2<double>, dim3(1), dim3(LU_TB), 0, s, Wl, ld, nr, P, a, nb, colmax, pivot_rel, piv, st);
     else hipLaunchKernelGGL(k_getf2<Zd>, dim3(1), dim3(LU_TB), 0, s, Wl, ld, nr, P, a, nb, colmax, pivot_rel, piv, st);
 }
@@ -366,6 +370,7 @@ void trsm_sub(int vs, double* Wl, int64_t ld, int32_t P, int32_t a, int32_t nb, 
 {
     const int32_t nc = P - a - nb;
     if (nc <= 0) return;
+    ProfScope ps("trsm_sub", 0.0, s);
     const unsigned g = (unsigned)((nc + TB - 1) / TB);
     if (vs == 1) hipLaunchKernelGGL(k_trsm_sub<double>, dim3(g), dim3(TB), 0, s, Wl, ld, P, a, nb, st);
     else hipLaunchKernelGGL(k_trsm_sub<Zd>, dim3(g), dim3(TB), 0, s, Wl, ld, P, a, nb, st);
@@ -375,6 +380,7 @@ void gemm_sub(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, 
 {
     const int32_t mr = nr - a - nb, mc = P - a - nb;
     if (mr <= 0 || mc <= 0) return;
+    ProfScope ps("gemm_sub", (vs == 1 ? 2.0 : 8.0) * mr * (double)mc * nb, s);
     const int ct = vs == 1 ? CTile<double>::v : CTile<Zd>::v;
     const dim3 g((unsigned)((mr + RT - 1) / RT), (unsigned)((mc + ct - 1) / ct));
     if (vs == 1) hipLaunchKernelGGL(k_gemm_sub<double>, g, dim3(TB), 0, s, Wl, ld, nr, P, a, nb, st);

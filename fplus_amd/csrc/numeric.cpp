@@ -8,6 +8,7 @@
 #include "snfactor.h"
 #include "solve_dev.h"
 #include "symbolic.h"
+#include "prof.h"
 
 #include <algorithm>
 #include <chrono>
@@ -191,6 +192,20 @@ struct fplus_numeric {
     std::vector<int32_t> lev_lf, lev_ub, lev_utf, lev_ltb;   // level offsets into the block lists
     DevBuf<int32_t> d_blk_lf, d_blk_ub, d_blk_utf, d_blk_ltb;
     std::vector<int32_t> blk_lf, blk_ub, blk_utf, blk_ltb;
+    // L y = b: every block's rows below have one contribution slot each; a row gathers the slots that hold its updates
+    DevBuf<int64_t> d_slot_off, d_incp, d_incs;
+    int64_t nslots = 0;
+    std::vector<int32_t> lev_lft, lev_ubt;                   // level offsets into the push / dot-product tiles
+    DevBuf<int2> d_lft, d_ubt;
+    std::vector<double> bytes_ub_tri;
+    std::vector<double> bytes_lf_pull, bytes_lf_push, bytes_ub, bytes_utf, bytes_ltb;
+    // U row-compressed (for U x = y): row pointers, column indices, values
+    DevBuf<int64_t> d_URp;
+    DevBuf<int32_t> d_URj;
+    DevBuf<double> d_URx;
+    // solve workspace, reused across calls (solves on one numeric object are serialized)
+    mutable std::mutex solve_mu;
+    mutable DevBuf<double> d_w, d_contrib, d_b;
     // the CSC export, built on first access
     mutable std::mutex mu;
     mutable bool exported = false;
@@ -268,6 +283,106 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         lvl[b] = L;
     }
     order_levels(lvl, num->blk_utf, num->lev_utf);
+    const double vb = 8.0 * F.vs;   // bytes per scalar
+    const int R0 = k::solve_rhs_per_pass(F.vs);
+    // contribution slots and the rows' lists of them (blocks in column order: a deterministic summation order)
+    {
+        std::vector<int64_t> slot_off(nb > 0 ? nb : 1), incp(n + 1, 0), incs;
+        int64_t ns = 0;
+        for (int32_t b = 0; b < nb; b++) {
+            slot_off[b] = ns;
+            const BlockDesc& B = F.blocks[b];
+            ns += B.nrows - B.w;
+            for (int32_t r = B.w; r < B.nrows; r++) incp[F.pinv[F.block_rows[b][r]] + 1]++;
+        }
+        for (int64_t i = 0; i < n; i++) incp[i + 1] += incp[i];
+        incs.resize(ns > 0 ? ns : 1);
+        std::vector<int64_t> fill(incp.begin(), incp.end() - 1);
+        for (int32_t b = 0; b < nb; b++) {
+            const BlockDesc& B = F.blocks[b];
+            for (int32_t r = B.w; r < B.nrows; r++) incs[fill[F.pinv[F.block_rows[b][r]]]++] = slot_off[b] + r - B.w;
+        }
+        num->nslots = ns;
+        auto up64 = [s](DevBuf<int64_t>& d, const std::vector<int64_t>& h) { d.alloc(h.size()); d.upload(h.data(), h.size(), s); };
+        up64(num->d_slot_off, slot_off);
+        up64(num->d_incp, incp);
+        up64(num->d_incs, incs);
+        // push tiles per level (256 rows each), and the bytes of every level of the four sweeps
+        std::vector<int2> tiles;
+        const size_t nlev = num->lev_lf.size() - 1;
+        num->lev_lft.assign(nlev + 1, 0);
+        num->bytes_lf_pull.assign(nlev, 0.0);
+        num->bytes_lf_push.assign(nlev, 0.0);
+        for (size_t L = 0; L < nlev; L++) {
+            for (int32_t k2 = num->lev_lf[L]; k2 < num->lev_lf[L + 1]; k2++) {
+                const int32_t b = num->blk_lf[k2];
+                const BlockDesc& B = F.blocks[b];
+                for (int32_t r0 = 0; r0 < B.nrows - B.w; r0 += 256) tiles.push_back(make_int2(b, r0));
+                double inc = 0;
+                for (int32_t c = 0; c < B.w; c++) inc += (double)(incp[B.c0 + c + 1] - incp[B.c0 + c]);
+                num->bytes_lf_pull[L] += (double)B.w * (B.w + 1) / 2 * vb + inc * (8.0 + R0 * vb) + 2.0 * B.w * R0 * vb;
+                num->bytes_lf_push[L] += (double)(B.nrows - B.w) * B.w * vb + (double)(B.nrows - B.w) * R0 * vb;
+            }
+            num->lev_lft[L + 1] = (int32_t)tiles.size();
+        }
+        num->d_lft.alloc(tiles.size() > 0 ? tiles.size() : 1);
+        num->d_lft.upload(tiles.data(), tiles.size(), s);
+    }
+    // U row-compressed: a counting sort of U's entries by row, columns ascending within a row
+    {
+        const int64_t nnzU = (int64_t)F.Ui.size();
+        std::vector<int64_t> URp(n + 1, 0), map(nnzU > 0 ? nnzU : 1);
+        std::vector<int32_t> URj(nnzU > 0 ? nnzU : 1);
+        for (int64_t q = 0; q < nnzU; q++) URp[F.Ui[q] + 1]++;
+        for (int64_t i = 0; i < n; i++) URp[i + 1] += URp[i];
+        std::vector<int64_t> fill(URp.begin(), URp.end() - 1);
+        for (int64_t j = 0; j < n; j++)
+            for (int64_t q = F.Up[j]; q < F.Up[j + 1]; q++) {
+                const int64_t o = fill[F.Ui[q]]++;
+                URj[o] = (int32_t)j;
+                map[o] = q;
+            }
+        num->d_URp.alloc(n + 1);
+        num->d_URp.upload(URp.data(), n + 1, s);
+        num->d_URj.alloc(URj.size());
+        num->d_URj.upload(URj.data(), URj.size(), s);
+        DevBuf<int64_t> d_map(map.size());
+        d_map.upload(map.data(), map.size(), s);
+        num->d_URx.alloc((size_t)(nnzU > 0 ? nnzU : 1) * F.vs);
+        k::gather_vals(F.vs, nnzU, d_map.p, F.d_Ux.p, num->d_URx.p, s);
+        dev_sync(s);
+        const size_t nub = num->lev_ub.size() - 1;
+        num->bytes_ub.assign(nub, 0.0);
+        num->bytes_ub_tri.assign(nub, 0.0);
+        num->lev_ubt.assign(nub + 1, 0);
+        std::vector<int2> ubt;
+        for (size_t L = 0; L < nub; L++) {
+            for (int32_t k2 = num->lev_ub[L]; k2 < num->lev_ub[L + 1]; k2++) {
+                const int32_t b = num->blk_ub[k2];
+                const BlockDesc& B = F.blocks[b];
+                const double ent = (double)(URp[B.c0 + B.w] - URp[B.c0]);
+                num->bytes_ub[L] += ent * (vb + 4.0) + 2.0 * B.w * R0 * vb;
+                num->bytes_ub_tri[L] += (double)B.w * (B.w + 1) / 2 * vb + 2.0 * B.w * R0 * vb;
+                for (int32_t t0 = 0; t0 < B.w; t0 += 4) ubt.push_back(make_int2(b, t0));
+            }
+            num->lev_ubt[L + 1] = (int32_t)ubt.size();
+        }
+        num->d_ubt.alloc(ubt.size() > 0 ? ubt.size() : 1);
+        num->d_ubt.upload(ubt.data(), ubt.size(), s);
+        const size_t nut = num->lev_utf.size() - 1, nlt = num->lev_ltb.size() - 1;
+        num->bytes_utf.assign(nut, 0.0);
+        num->bytes_ltb.assign(nlt, 0.0);
+        for (size_t L = 0; L < nut; L++)
+            for (int32_t k2 = num->lev_utf[L]; k2 < num->lev_utf[L + 1]; k2++) {
+                const BlockDesc& B = F.blocks[num->blk_utf[k2]];
+                num->bytes_utf[L] += (double)(F.Up[B.c0 + B.w] - F.Up[B.c0]) * (vb + 4.0) + 2.0 * B.w * R0 * vb;
+            }
+        for (size_t L = 0; L < nlt; L++)
+            for (int32_t k2 = num->lev_ltb[L]; k2 < num->lev_ltb[L + 1]; k2++) {
+                const BlockDesc& B = F.blocks[num->blk_ltb[k2]];
+                num->bytes_ltb[L] += (double)B.nrows * B.w * vb + (double)B.nrows * (4.0 + R0 * vb);
+            }
+    }
     auto up32 = [s](DevBuf<int32_t>& d, const std::vector<int32_t>& h) { d.alloc(h.size()); d.upload(h.data(), h.size(), s); };
     up32(num->d_blk_lf, num->blk_lf);
     up32(num->d_blk_ub, num->blk_ub);
@@ -291,27 +406,51 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
     dev_sync(s);
 }
 
-// the sweeps on w (n x nrhs, pivot order) in place
-void device_sweeps(const fplus_numeric* num, double* w, int32_t nrhs, int trans, hipStream_t s)
+// the sweeps on w (n x R row-major, pivot order, R <= solve_rhs_per_pass) in place
+void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, hipStream_t s)
 {
     const DevFactor& F = num->F;
     const int vs = F.vs;
     const BlockDesc* blocks = F.d_blocks.p;
-    const int64_t n = F.n;
     if (trans == 0) {
-        for (size_t L = 0; L + 1 < num->lev_lf.size(); L++)
-            k::lfwd(vs, num->lev_lf[L + 1] - num->lev_lf[L], num->d_blk_lf.p + num->lev_lf[L], blocks, num->d_pinv.p, w, n, nrhs, s);
-        for (size_t L = 0; L + 1 < num->lev_ub.size(); L++)
-            k::ubwd(vs, num->lev_ub[L + 1] - num->lev_ub[L], num->d_blk_ub.p + num->lev_ub[L], blocks, num->d_Up.p, num->d_Ui.p,
-                    F.d_Ux.p, w, n, nrhs, s);
+        num->d_contrib.reserve((size_t)(num->nslots > 0 ? num->nslots : 1) * R * vs);
+        for (size_t L = 0; L + 1 < num->lev_lf.size(); L++) {
+            k::lfwd_pull(vs, num->lev_lf[L + 1] - num->lev_lf[L], num->d_blk_lf.p + num->lev_lf[L], blocks, num->d_incp.p,
+                         num->d_incs.p, num->d_contrib.p, w, R, num->bytes_lf_pull[L], s);
+            k::lfwd_push(vs, num->lev_lft[L + 1] - num->lev_lft[L], num->d_lft.p + num->lev_lft[L], blocks, num->d_slot_off.p, w,
+                         num->d_contrib.p, R, num->bytes_lf_push[L], s);
+        }
+        for (size_t L = 0; L + 1 < num->lev_ub.size(); L++) {
+            k::ub_dots(vs, num->lev_ubt[L + 1] - num->lev_ubt[L], num->d_ubt.p + num->lev_ubt[L], blocks, num->d_URp.p, num->d_URj.p,
+                       num->d_URx.p, w, R, num->bytes_ub[L], s);
+            k::ub_tri(vs, num->lev_ub[L + 1] - num->lev_ub[L], num->d_blk_ub.p + num->lev_ub[L], blocks, w, R, num->bytes_ub_tri[L], s);
+        }
     } else {
         const int cj = trans == 2 && vs == 2;
         for (size_t L = 0; L + 1 < num->lev_utf.size(); L++)
             k::utfwd(vs, num->lev_utf[L + 1] - num->lev_utf[L], num->d_blk_utf.p + num->lev_utf[L], blocks, num->d_Up.p,
-                     num->d_Ui.p, F.d_Ux.p, w, n, nrhs, cj, s);
+                     num->d_Ui.p, F.d_Ux.p, w, R, cj, num->bytes_utf[L], s);
         for (size_t L = 0; L + 1 < num->lev_ltb.size(); L++)
-            k::ltbwd(vs, num->lev_ltb[L + 1] - num->lev_ltb[L], num->d_blk_ltb.p + num->lev_ltb[L], blocks, num->d_pinv.p, w, n,
-                     nrhs, cj, s);
+            k::ltbwd(vs, num->lev_ltb[L + 1] - num->lev_ltb[L], num->d_blk_ltb.p + num->lev_ltb[L], blocks, num->d_pinv.p, w, R,
+                     cj, num->bytes_ltb[L], s);
+    }
+}
+
+// solve for nrhs right-hand sides in device memory b (column-major, ld), in passes of solve_rhs_per_pass columns
+void solve_on_device(const fplus_numeric* num, double* b, int64_t nrhs, int64_t ldb, int tr, hipStream_t s)
+{
+    const int vs = num->F.vs;
+    const int64_t n = num->n;
+    const int32_t RP = k::solve_rhs_per_pass(vs);
+    num->d_w.reserve((size_t)(n * RP * vs));
+    for (int64_t r0 = 0; r0 < nrhs; r0 += RP) {
+        const int32_t R = (int32_t)std::min<int64_t>(RP, nrhs - r0);
+        double* br = b + (size_t)(r0 * ldb) * vs;
+        if (tr == 0) k::gather_scale(vs, n, R, num->d_p.p, num->d_Dr.p, br, ldb, num->d_w.p, s);
+        else k::gather_scale(vs, n, R, num->d_q.p, num->d_Dc.p, br, ldb, num->d_w.p, s);
+        device_sweeps(num, num->d_w.p, R, tr, s);
+        if (tr == 0) k::scatter_scale(vs, n, R, num->d_q.p, num->d_Dc.p, num->d_w.p, br, ldb, s);
+        else k::scatter_scale(vs, n, R, num->d_p.p, num->d_Dr.p, num->d_w.p, br, ldb, s);
     }
 }
 
@@ -446,8 +585,11 @@ int factor_core(fplus_numeric* num, const PatternCache& c, DevBuf<double>& d_Bx,
     num->t_ruiz = t1 - t0;
     num->t_factor = now_s() - t1;
     if (rc == FPLUS_OK && verbose >= 1)
-        printf("[FPLUS] LU factorisation complete: nnz(U)=%ld, supernodes=%ld, panels=%ld, levels=%ld, time=%.3fs (host %.3fs, wait %.3fs)\n",
-               (long)num->F.nnzU, (long)num->F.nsn, (long)num->F.panels, (long)num->F.levels, num->t_factor, num->F.t_host, num->F.t_wait);
+        printf("[FPLUS] LU factorisation complete: nnz(U)=%ld, supernodes=%ld, panels=%ld, levels=%ld, time=%.3fs (host %.3fs, wait %.3fs); "
+               "workspace rows per panel: %.0f pivotal, %.0f not (max %ld)\n",
+               (long)num->F.nnzU, (long)num->F.nsn, (long)num->F.panels, (long)num->F.levels, num->t_factor, num->F.t_host, num->F.t_wait,
+               (double)num->F.rows_upper / std::max<int64_t>(1, num->F.panels), (double)num->F.rows_lower / std::max<int64_t>(1, num->F.panels),
+               (long)num->F.rows_lower_max);
     return rc;
 }
 
@@ -487,8 +629,8 @@ void fplus_numeric_free(fplus_numeric* num)
     delete num;
 }
 
-int fplus_factor(const fplus_symbolic* sym, const int64_t* Ap, const int64_t* Ai, const void* Ax, int is_complex,
-                 const fplus_options* opt, fplus_numeric** out, char* err, size_t errlen)
+static int factor_entry(const fplus_symbolic* sym, const int64_t* Ap, const int64_t* Ai, const void* Ax, bool device_values,
+                        int is_complex, const fplus_options* opt, fplus_numeric** out, char* err, size_t errlen)
 {
     *out = nullptr;
     fplus_options d;
@@ -501,7 +643,7 @@ int fplus_factor(const fplus_symbolic* sym, const int64_t* Ap, const int64_t* Ai
         return FPLUS_ERR_BADARG;
     }
     rc = check_compressed(n, Ap, Ai, true, err, errlen);
-    if (rc == FPLUS_OK) rc = check_finite(Ap[n], Ax, is_complex, err, errlen);
+    if (rc == FPLUS_OK && !device_values) rc = check_finite(Ap[n], Ax, is_complex, err, errlen);
     if (rc != FPLUS_OK) return rc;
     const int vs = is_complex ? 2 : 1;
     std::unique_ptr<fplus_numeric> num;
@@ -518,9 +660,14 @@ int fplus_factor(const fplus_symbolic* sym, const int64_t* Ap, const int64_t* Ai
         num->n = n;
         num->is_complex = is_complex;
         const int64_t nnz = c->nnz;
-        DevBuf<double> d_Ax((size_t)(nnz > 0 ? nnz : 1) * vs), d_Bx((size_t)(nnz > 0 ? nnz : 1) * vs);
-        d_Ax.upload((const double*)Ax, (size_t)nnz * vs, s);
-        k::gather_vals(vs, nnz, c->d_src.p, d_Ax.p, d_Bx.p, s);
+        DevBuf<double> d_Ax, d_Bx((size_t)(nnz > 0 ? nnz : 1) * vs);
+        const double* src = static_cast<const double*>(Ax);
+        if (!device_values) {
+            d_Ax.alloc((size_t)(nnz > 0 ? nnz : 1) * vs);
+            d_Ax.upload((const double*)Ax, (size_t)nnz * vs, s);
+            src = d_Ax.p;
+        }
+        k::gather_vals(vs, nnz, c->d_src.p, src, d_Bx.p, s);
         DevBuf<double> d_Drp, d_Dcp;
         rc = factor_core(num.get(), *c, d_Bx, vs, opt->equilibrate, opt->ruiz_max_iter, opt->ruiz_tol, opt->relax_rows, opt->panel,
                          opt->pivot_rel, opt->verbose, d_Drp, d_Dcp, err, errlen);
@@ -553,6 +700,12 @@ int fplus_factor(const fplus_symbolic* sym, const int64_t* Ap, const int64_t* Ai
     return FPLUS_OK;
 }
 
+int fplus_factor(const fplus_symbolic* sym, const int64_t* Ap, const int64_t* Ai, const void* Ax, int is_complex,
+                 const fplus_options* opt, fplus_numeric** out, char* err, size_t errlen)
+{
+    return factor_entry(sym, Ap, Ai, Ax, false, is_complex, opt, out, err, errlen);
+}
+
 int fplus_solve(const fplus_numeric* num, void* b, int64_t nrhs, int64_t ldb, int trans, char* err, size_t errlen)
 {
     const int64_t n = num->n;
@@ -562,21 +715,14 @@ int fplus_solve(const fplus_numeric* num, void* b, int64_t nrhs, int64_t ldb, in
     const int tr = vs == 1 ? (trans ? 1 : 0) : trans;
     try {
         hipStream_t s = devctx().stream;
-        const int64_t G = 64;   // right-hand sides per pass
-        const int64_t g = std::min(nrhs, G);
-        DevBuf<double> d_b((size_t)(n * g * vs)), d_w((size_t)(n * g * vs));
-        for (int64_t r0 = 0; r0 < nrhs; r0 += g) {
-            const int32_t m = (int32_t)std::min(g, nrhs - r0);
-            double* hb = (double*)b + (size_t)(r0 * ldb) * vs;
-            for (int32_t r = 0; r < m; r++) d_b.upload(hb + (size_t)r * ldb * vs, (size_t)n * vs, s, (size_t)r * n * vs);
-            if (tr == 0) k::gather_scale(vs, n, m, num->d_p.p, num->d_Dr.p, d_b.p, n, d_w.p, s);
-            else k::gather_scale(vs, n, m, num->d_q.p, num->d_Dc.p, d_b.p, n, d_w.p, s);
-            device_sweeps(num, d_w.p, m, tr, s);
-            if (tr == 0) k::scatter_scale(vs, n, m, num->d_q.p, num->d_Dc.p, d_w.p, d_b.p, n, s);
-            else k::scatter_scale(vs, n, m, num->d_p.p, num->d_Dr.p, d_w.p, d_b.p, n, s);
-            for (int32_t r = 0; r < m; r++) d_b.download(hb + (size_t)r * ldb * vs, (size_t)n * vs, s, (size_t)r * n * vs);
-            dev_sync(s);
-        }
+        std::lock_guard<std::mutex> g(num->solve_mu);
+        num->d_b.reserve((size_t)(n * nrhs * vs));
+        for (int64_t r = 0; r < nrhs; r++)
+            num->d_b.upload((const double*)b + (size_t)(r * ldb) * vs, (size_t)n * vs, s, (size_t)(r * n) * vs);
+        solve_on_device(num, num->d_b.p, nrhs, n, tr, s);
+        for (int64_t r = 0; r < nrhs; r++)
+            num->d_b.download((double*)b + (size_t)(r * ldb) * vs, (size_t)n * vs, s, (size_t)(r * n) * vs);
+        dev_sync(s);
     } catch (const GpuError& e) {
         return gpu_error(e, err, errlen);
     } catch (const std::bad_alloc&) {
@@ -641,3 +787,78 @@ extern "C" int fplus_amd_numeric_stats(const fplus_numeric* num, fplus_amd_stats
     st->ruiz_iters = num->ruiz_iters;
     return FPLUS_OK;
 }
+
+extern "C" {
+int fplus_amd_factor_device(const fplus_symbolic* sym, const int64_t* Ap, const int64_t* Ai, const void* d_Ax, int is_complex,
+                            const fplus_options* opt, fplus_numeric** num, char* err, size_t errlen)
+{
+    return factor_entry(sym, Ap, Ai, d_Ax, true, is_complex, opt, num, err, errlen);
+}
+
+int fplus_amd_solve_device(const fplus_numeric* num, void* d_b, int64_t nrhs, int64_t ldb, int trans, char* err, size_t errlen)
+{
+    const int64_t n = num->n;
+    if (trans < 0 || trans > 2 || nrhs < 0 || ldb < n) { errf(err, errlen, "fplus_amd_solve_device: bad trans, nrhs or ldb"); return FPLUS_ERR_BADARG; }
+    if (nrhs == 0 || n == 0) return FPLUS_OK;
+    const int vs = num->F.vs;
+    const int tr = vs == 1 ? (trans ? 1 : 0) : trans;
+    try {
+        hipStream_t s = devctx().stream;
+        std::lock_guard<std::mutex> g(num->solve_mu);
+        solve_on_device(num, static_cast<double*>(d_b), nrhs, ldb, tr, s);
+        dev_sync(s);
+    } catch (const GpuError& e) {
+        return gpu_error(e, err, errlen);
+    }
+    return FPLUS_OK;
+}
+
+int fplus_amd_dev_alloc(size_t bytes, void** ptr)
+{
+    try {
+        devctx();
+        return hipMalloc(ptr, bytes > 0 ? bytes : 1) == hipSuccess ? FPLUS_OK : FPLUS_ERR_NOMEM;
+    } catch (const GpuError&) {
+        return FPLUS_ERR_GPU;
+    }
+}
+int fplus_amd_dev_free(void* ptr) { return hipFree(ptr) == hipSuccess ? FPLUS_OK : FPLUS_ERR_GPU; }
+int fplus_amd_dev_copy(void* dst, const void* src, size_t bytes, int kind)
+{
+    try {
+        hipStream_t s = devctx().stream;
+        const hipMemcpyKind k2 = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
+        if (hipMemcpyAsync(dst, src, bytes, k2, s) != hipSuccess) return FPLUS_ERR_GPU;
+        return hipStreamSynchronize(s) == hipSuccess ? FPLUS_OK : FPLUS_ERR_GPU;
+    } catch (const GpuError&) {
+        return FPLUS_ERR_GPU;
+    }
+}
+int fplus_amd_dev_sync(void)
+{
+    try {
+        return hipStreamSynchronize(devctx().stream) == hipSuccess ? FPLUS_OK : FPLUS_ERR_GPU;
+    } catch (const GpuError&) {
+        return FPLUS_ERR_GPU;
+    }
+}
+int fplus_amd_profile(int on)
+{
+    KernelProf& p = kprof();
+    std::lock_guard<std::mutex> g(p.mu);
+    p.collect();
+    if (on) p.sums.clear();
+    p.on = on != 0;
+    return FPLUS_OK;
+}
+int fplus_amd_profile_get(const char* kernel, double* ms, int64_t* launches, double* work)
+{
+    KernelProf& p = kprof();
+    std::lock_guard<std::mutex> g(p.mu);
+    p.collect();
+    auto it = p.sums.find(kernel);
+    if (it == p.sums.end()) { *ms = 0; *launches = 0; *work = 0; return FPLUS_OK; }
+    *ms = it->second.ms; *launches = it->second.launches; *work = it->second.work;
+    return FPLUS_OK;
+}
+}  // extern "C"

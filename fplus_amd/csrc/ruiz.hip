@@ -8,6 +8,7 @@
 // below need no ordering.  The passes are HBM-bound streams over the nnz entries.
 #include "dev_common.h"
 #include "device_api.h"
+#include "prof.h"
 
 #include <climits>
 
@@ -190,6 +191,7 @@ void gather_vals(int vs, int64_t cnt, const int64_t* map, const double* src, dou
 void ruiz_rowmax(int vs, int64_t nnz, const int32_t* erow, const double* x, unsigned long long* rmax, hipStream_t s)
 {
     if (nnz <= 0) return;
+    ProfScope ps("ruiz_rowmax", (double)nnz * (8.0 * vs + 4.0), s);
     if (vs == 1) hipLaunchKernelGGL(k_ruiz_rowmax<1>, dim3(grid_for(nnz)), dim3(TB), 0, s, nnz, erow, x, rmax);
     else hipLaunchKernelGGL(k_ruiz_rowmax<2>, dim3(grid_for(nnz)), dim3(TB), 0, s, nnz, erow, x, rmax);
 }
@@ -201,12 +203,14 @@ void ruiz_scale_colmax(int vs, int64_t nnz, const int32_t* erow, const int32_t* 
                        unsigned long long* cmax, hipStream_t s)
 {
     if (nnz <= 0) return;
+    ProfScope ps("ruiz_scale_colmax", (double)nnz * (16.0 * vs + 8.0), s);
     if (vs == 1) hipLaunchKernelGGL(k_ruiz_scale_colmax<1>, dim3(grid_for(nnz)), dim3(TB), 0, s, nnz, erow, ecol, x, rscale, cmax);
     else hipLaunchKernelGGL(k_ruiz_scale_colmax<2>, dim3(grid_for(nnz)), dim3(TB), 0, s, nnz, erow, ecol, x, rscale, cmax);
 }
 void ruiz_scale_cols(int vs, int64_t nnz, const int32_t* ecol, double* x, const double* cscale, hipStream_t s)
 {
     if (nnz <= 0) return;
+    ProfScope ps("ruiz_scale_cols", (double)nnz * (16.0 * vs + 4.0), s);
     if (vs == 1) hipLaunchKernelGGL(k_ruiz_scale_cols<1>, dim3(grid_for(nnz)), dim3(TB), 0, s, nnz, ecol, x, cscale);
     else hipLaunchKernelGGL(k_ruiz_scale_cols<2>, dim3(grid_for(nnz)), dim3(TB), 0, s, nnz, ecol, x, cscale);
 }
@@ -234,6 +238,12 @@ void min_positive(int64_t m, const unsigned long long* v, unsigned long long* ou
     if (m > 0) hipLaunchKernelGGL(k_min_positive, dim3(grid_for(m)), dim3(TB), 0, s, m, v, out);
 }
 }  // namespace k
+
+KernelProf& kprof()
+{
+    static KernelProf p;
+    return p;
+}
 
 DevCtx& devctx()
 {

@@ -279,6 +279,9 @@ int Driver::panel(int64_t j0, int64_t j1)
     for (int64_t q = Bp_[j0]; q < Bp_[j1]; q++)   // every entry of the panel's columns lands in the workspace
         if (hrowpos_[Bi_[q]] < 0) { errf(err_, errlen_, "internal: a column entry outside the panel rows"); return FPLUS_ERR_INTERNAL; }
     const int64_t ldw = std::max<int64_t>((Ru + Rl + 1) & ~(int64_t)1, 2);
+    F_.rows_upper += Ru;
+    F_.rows_lower += Rl;
+    F_.rows_lower_max = std::max(F_.rows_lower_max, Rl);
     // ---- update tasks per union supernode and block, with dependency levels
     const int SL = FPA_SLAB, CT = col_tile(vs_), RT = FPA_ROW_TILE;
     std::vector<int64_t> ucnt(un.size() + 1, 0);
@@ -340,6 +343,7 @@ int Driver::panel(int64_t j0, int64_t j1)
     std::stable_sort(tl.begin(), tl.end(), [](const TaskL& a, const TaskL& b) { return a.level < b.level; });
     std::vector<SlabTask> tasks(tl.size());
     std::vector<int32_t> lev_task(maxlevel + 2, 0), lev_tile(maxlevel + 2, 0), lev_ct(maxlevel + 2, 0);
+    std::vector<double> lev_flops(maxlevel + 2, 0.0);
     std::vector<GemmTile> tiles;
     for (size_t i = 0; i < tl.size(); i++) {
         tasks[i] = tl[i].t;
@@ -347,6 +351,7 @@ int Driver::panel(int64_t j0, int64_t j1)
         lev_task[L + 1] = (int32_t)i + 1;
         const BlockDesc& B = F_.blocks[tasks[i].block];
         lev_ct[L] = std::max<int32_t>(lev_ct[L], (tasks[i].ncol + CT - 1) / CT);
+        lev_flops[L] += (vs_ == 1 ? 2.0 : 8.0) * (double)tasks[i].ncol * (double)(tasks[i].k1 - tasks[i].k0) * (double)(B.nrows - tasks[i].k1);
         for (int32_t r0 = tasks[i].k1; r0 < B.nrows; r0 += RT)
             for (int32_t cc = 0; cc < tasks[i].ncol; cc += CT) {
                 GemmTile g; g.task = (int32_t)i; g.row0 = r0; g.col0 = cc; g.pad = 0;
@@ -381,7 +386,7 @@ int Driver::panel(int64_t j0, int64_t j1)
             if (t1 <= t0) continue;
             k::slab_solve(vs_, t1 - t0, d_tasks_.p + t0, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, lev_ct[L], s_);
             const int32_t g0 = lev_tile[L], g1 = lev_tile[L + 1];
-            k::slab_gemm(vs_, g1 - g0, d_tiles_.p + g0, d_tasks_.p, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, s_);
+            k::slab_gemm(vs_, g1 - g0, d_tiles_.p + g0, d_tasks_.p, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, lev_flops[L], s_);
         }
     }
     double* Wl = d_W_.p + Ru * vs_;

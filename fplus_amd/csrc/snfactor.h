@@ -32,6 +32,7 @@ struct DevFactor {
     double flops = 0, gemm_flops = 0;
     int64_t panels = 0, levels = 0, tasks = 0;
     double t_host = 0, t_wait = 0;
+    int64_t rows_upper = 0, rows_lower = 0, rows_lower_max = 0;   // workspace rows summed over the panels
 };
 
 struct SnOptions {

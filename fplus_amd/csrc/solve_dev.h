@@ -8,20 +8,24 @@
 
 namespace fpa {
 namespace k {
-// x: n x nrhs column-major (ld = n), indexed by pivot order.  blk: block ids of one level.  pinv_rows maps a
-// row of the factored matrix to its pivot position.
-void lfwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int32_t* pinv_rows, double* x,
-          int64_t ldx, int32_t nrhs, hipStream_t s);
-void ubwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
-          const double* Ux, double* x, int64_t ldx, int32_t nrhs, hipStream_t s);
+// x: n x R row-major (x[i * R + r]), indexed by pivot order; R <= solve_rhs_per_pass(vs).  blk: the blocks of one
+// level.  bytes: the algorithmic bytes of the launch (for the optional per-kernel timing).
+int solve_rhs_per_pass(int vs);
+void lfwd_pull(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int64_t* incp, const int64_t* incs,
+               const double* contrib, double* x, int32_t R, double bytes, hipStream_t s);
+void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* slot_off, const double* x,
+               double* contrib, int32_t R, double bytes, hipStream_t s);
+void ub_dots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* URp, const int32_t* URj,
+             const double* URx, double* x, int32_t R, double bytes, hipStream_t s);
+void ub_tri(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, double* x, int32_t R, double bytes, hipStream_t s);
 void utfwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
-           const double* Ux, double* x, int64_t ldx, int32_t nrhs, int conj, hipStream_t s);
-void ltbwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int32_t* pinv_rows, double* x,
-           int64_t ldx, int32_t nrhs, int conj, hipStream_t s);
-// w[k, r] = D[perm[k]] * b[perm[k], r]  /  x[perm[k], r] = D[perm[k]] * w[k, r]
-void gather_scale(int vs, int64_t n, int32_t nrhs, const int64_t* perm, const double* D, const double* b, int64_t ldb,
-                  double* w, hipStream_t s);
-void scatter_scale(int vs, int64_t n, int32_t nrhs, const int64_t* perm, const double* D, const double* w, double* b,
-                   int64_t ldb, hipStream_t s);
+           const double* Ux, double* x, int32_t R, int conj, double bytes, hipStream_t s);
+void ltbwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int32_t* pinv_rows, double* x, int32_t R,
+           int conj, double bytes, hipStream_t s);
+// w[k * R + r] = D[perm[k]] * b[perm[k] + r * ldb]  /  b[perm[k] + r * ldb] = D[perm[k]] * w[k * R + r]
+void gather_scale(int vs, int64_t n, int32_t R, const int64_t* perm, const double* D, const double* b, int64_t ldb, double* w,
+                  hipStream_t s);
+void scatter_scale(int vs, int64_t n, int32_t R, const int64_t* perm, const double* D, const double* w, double* b, int64_t ldb,
+                   hipStream_t s);
 }  // namespace k
 }  // namespace fpa

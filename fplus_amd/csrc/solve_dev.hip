@@ -1,83 +1,31 @@
 // solve_dev.hip -- the triangular solves on the device factors (gfx950).
 //
-// L is held as supernode blocks (factor_dev.h), U in CSC with its rows in pivot order and the diagonal last in every
-// column.  A solve walks the blocks level by level (a level's blocks are independent); one workgroup per block:
-//   lfwd   L y = b   the block's unit-lower triangle, then the rows below subtracted (push, atomic);
-//   ubwd   U x = y   columns of the block last to first, x_j = y_j / U_jj, U(:, j) subtracted (push);
-//   utfwd  U^T y = b (U^H with conj) dot products down U's columns (pull), the block's own rows in order;
-//   ltbwd  L^T x = y (L^H) dot products down the block's columns (pull), the block's triangle last to first.
-// Blocks are at most 256 columns wide (the panel width), so the triangular part keeps one row per thread.
+// L is held as supernode blocks (factor_dev.h), U in CSC (rows in pivot order, the diagonal last in every column) and,
+// for the solves with A, also row-compressed.  The right-hand sides live row-major during the sweeps, x[i * R + r] for
+// R right-hand sides, so one entry of a factor meets all R values of a row in one contiguous read.  A sweep walks the
+// blocks level by level (a level's blocks are independent), one launch per level, and every sweep pulls: each value is
+// written by one thread, no atomics, so the results are the same on every run.
+//   lfwd_pull   L y = b   a block's pivot rows gather the contributions the blocks before them left in their slots,
+//                         then the block's unit-lower triangle;
+//   lfwd_push   the block's rows below: contribution[slot + r] = L(r, block) y(block) (the slot is the row's own);
+//   ubwd        U x = y   a block's pivot rows: dot products along their U rows beyond the block (row-compressed U),
+//                         then the block's upper triangle from its columns, last to first;
+//   utfwd       U^T y = b (U^H): dot products down U's columns, the block's own rows in order;
+//   ltbwd       L^T x = y (L^H): dot products down the block's columns, the triangle last to first.
+// Blocks are at most 256 columns wide (the panel width): the triangles keep one row per thread.
 #include "dev_common.h"
 #include "device_api.h"
+#include "prof.h"
 #include "solve_dev.h"
 
 namespace fpa {
 namespace {
 
 constexpr int TB = 256;
+// right-hand sides per pass (a block's w x R values in at most 64 KB of LDS)
+template <class T> struct RM { static constexpr int v = 16; };
 
 template <class T> __device__ inline T cj(T v, int c) { return c ? Sc<T>::conj(v) : v; }
-
-template <class T>
-__global__ __launch_bounds__(TB) void k_lfwd(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
-                                              const int32_t* __restrict__ pinv_rows, double* __restrict__ xd, int64_t ldx,
-                                              int32_t nrhs)
-{
-    __shared__ T ys[TB];
-    const BlockDesc B = blocks[blk[blockIdx.x]];
-    const T* V = reinterpret_cast<const T*>(B.vals);
-    T* x = reinterpret_cast<T*>(xd);
-    const int t = threadIdx.x, w = B.w;
-    for (int32_t r = 0; r < nrhs; r++) {
-        T* xr = x + (int64_t)r * ldx;
-        T my = t < w ? xr[B.c0 + t] : Sc<T>::zero();
-        for (int c = 0; c < w; c++) {
-            if (t == c) ys[c] = my;
-            __syncthreads();
-            if (t > c && t < w) my = Sc<T>::fms(my, V[(int64_t)c * B.nrows + t], ys[c]);
-        }
-        if (t < w) xr[B.c0 + t] = my;
-        __syncthreads();
-        for (int32_t i = w + t; i < B.nrows; i += TB) {
-            T acc = Sc<T>::zero();
-            for (int c = 0; c < w; c++) acc = Sc<T>::fma_(acc, V[(int64_t)c * B.nrows + i], ys[c]);
-            atomic_sub(xr + pinv_rows[B.rows[i]], acc);
-        }
-        __syncthreads();
-    }
-}
-
-template <class T>
-__global__ __launch_bounds__(TB) void k_ubwd(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
-                                              const int64_t* __restrict__ Up, const int32_t* __restrict__ Ui,
-                                              const double* __restrict__ Uxd, double* __restrict__ xd, int64_t ldx, int32_t nrhs)
-{
-    __shared__ T ys[TB];
-    const BlockDesc B = blocks[blk[blockIdx.x]];
-    const T* Ux = reinterpret_cast<const T*>(Uxd);
-    T* x = reinterpret_cast<T*>(xd);
-    const int t = threadIdx.x, w = B.w;
-    const int64_t c0 = B.c0;
-    for (int32_t r = 0; r < nrhs; r++) {
-        T* xr = x + (int64_t)r * ldx;
-        if (t < w) ys[t] = xr[c0 + t];
-        __syncthreads();
-        for (int c = w - 1; c >= 0; c--) {
-            const int64_t q0 = Up[c0 + c], qd = Up[c0 + c + 1] - 1;
-            if (t == 0) ys[c] = Sc<T>::div(ys[c], Ux[qd]);
-            __syncthreads();
-            const T xc = ys[c];
-            for (int64_t q = q0 + t; q < qd; q += TB) {
-                const int64_t i = Ui[q];
-                if (i >= c0) ys[i - c0] = Sc<T>::fms(ys[i - c0], Ux[q], xc);
-                else atomic_sub(xr + i, Sc<T>::mul(Ux[q], xc));
-            }
-            __syncthreads();
-        }
-        if (t < w) xr[c0 + t] = ys[t];
-        __syncthreads();
-    }
-}
 
 template <class T>
 __device__ inline T wave_sum(T v)
@@ -89,119 +37,248 @@ __device__ inline T wave_sum(T v)
     return v;
 }
 
+// L y = b, part 1 (one workgroup per block of the level): pull the contributions into the pivot rows, solve the triangle
 template <class T>
-__global__ __launch_bounds__(TB) void k_utfwd(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
-                                               const int64_t* __restrict__ Up, const int32_t* __restrict__ Ui,
-                                               const double* __restrict__ Uxd, double* __restrict__ xd, int64_t ldx,
-                                               int32_t nrhs, int conj)
+__global__ __launch_bounds__(TB) void k_lfwd_pull(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
+                                                   const int64_t* __restrict__ incp, const int64_t* __restrict__ incs,
+                                                   const double* __restrict__ contribd, double* __restrict__ xd, int32_t R)
 {
-    __shared__ T ys[TB];
+    __shared__ T ys[TB * RM<T>::v];
     const BlockDesc B = blocks[blk[blockIdx.x]];
-    const T* Ux = reinterpret_cast<const T*>(Uxd);
+    const T* V = reinterpret_cast<const T*>(B.vals);
+    const T* contrib = reinterpret_cast<const T*>(contribd);
     T* x = reinterpret_cast<T*>(xd);
-    const int t = threadIdx.x, lane = t & 63, wid = t >> 6, w = B.w;
+    const int w = B.w;
     const int64_t c0 = B.c0;
-    for (int32_t r = 0; r < nrhs; r++) {
-        T* xr = x + (int64_t)r * ldx;
-        // the parts of the dot products over rows before the block: one wavefront per column
-        for (int c = wid; c < w; c += TB / 64) {
-            const int64_t q0 = Up[c0 + c], qd = Up[c0 + c + 1] - 1;
-            T acc = Sc<T>::zero();
-            for (int64_t q = q0 + lane; q < qd; q += 64) {
-                const int64_t i = Ui[q];
-                if (i < c0) acc = Sc<T>::fma_(acc, cj(Ux[q], conj), xr[i]);
-            }
-            acc = wave_sum(acc);
-            if (lane == 0) ys[c] = Sc<T>::sub(xr[c0 + c], acc);
+    for (int idx = threadIdx.x; idx < w * R; idx += TB) {
+        const int t = idx / R, r = idx % R;
+        const int64_t i = c0 + t;
+        T acc = x[i * R + r];
+        for (int64_t q = incp[i]; q < incp[i + 1]; q++) acc = Sc<T>::sub(acc, contrib[incs[q] * R + r]);
+        ys[t * R + r] = acc;
+    }
+    __syncthreads();
+    for (int c = 0; c < w - 1; c++) {
+        for (int idx = threadIdx.x; idx < (w - c - 1) * R; idx += TB) {
+            const int t = c + 1 + idx / R, r = idx % R;
+            ys[t * R + r] = Sc<T>::fms(ys[t * R + r], V[(int64_t)c * B.nrows + t], ys[c * R + r]);
         }
         __syncthreads();
-        // the block's own rows, in column order (wavefront 0)
-        if (wid == 0) {
-            for (int c = 0; c < w; c++) {
-                const int64_t q0 = Up[c0 + c], qd = Up[c0 + c + 1] - 1;
-                T acc = Sc<T>::zero();
-                for (int64_t q = q0 + lane; q < qd; q += 64) {
-                    const int64_t i = Ui[q];
-                    if (i >= c0) acc = Sc<T>::fma_(acc, cj(Ux[q], conj), ys[i - c0]);
-                }
-                acc = wave_sum(acc);
-                if (lane == 0) ys[c] = Sc<T>::div(Sc<T>::sub(ys[c], acc), cj(Ux[qd], conj));
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-            }
-        }
-        __syncthreads();
-        if (t < w) xr[c0 + t] = ys[t];
-        __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < w * R; idx += TB) x[(c0 + idx / R) * R + idx % R] = ys[idx];
+}
+
+// L y = b, part 2: the rows below a block, 256 rows per workgroup; slot_off[b] is the block's first slot
+template <class T>
+__global__ __launch_bounds__(TB) void k_lfwd_push(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
+                                                   const int64_t* __restrict__ slot_off, const double* __restrict__ xd,
+                                                   double* __restrict__ contribd, int32_t R)
+{
+    __shared__ T ys[TB * RM<T>::v];
+    const int2 tl = tiles[blockIdx.x];
+    const BlockDesc B = blocks[tl.x];
+    const T* V = reinterpret_cast<const T*>(B.vals);
+    const T* x = reinterpret_cast<const T*>(xd);
+    T* contrib = reinterpret_cast<T*>(contribd);
+    const int w = B.w;
+    for (int idx = threadIdx.x; idx < w * R; idx += TB) ys[idx] = x[(B.c0 + idx / R) * R + idx % R];
+    __syncthreads();
+    const int32_t row = w + tl.y + threadIdx.x;
+    if (row >= B.nrows) return;
+    T acc[RM<T>::v];
+#pragma unroll
+    for (int r = 0; r < RM<T>::v; r++) acc[r] = Sc<T>::zero();
+    for (int c = 0; c < w; c++) {
+        const T v = V[(int64_t)c * B.nrows + row];
+#pragma unroll
+        for (int r = 0; r < RM<T>::v; r++)
+            if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, ys[c * R + r]);
+    }
+    const int64_t slot = slot_off[tl.x] + row - w;
+#pragma unroll
+    for (int r = 0; r < RM<T>::v; r++)
+        if (r < R) contrib[slot * R + r] = acc[r];
+}
+
+// U x = y, part 1: the dot products of a block's pivot rows along their U rows beyond the block (row-compressed U),
+// one wavefront per row and every right-hand side at once, four rows per workgroup; x_i -= sum in place
+template <class T>
+__global__ __launch_bounds__(TB) void k_ub_dots(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
+                                                 const int64_t* __restrict__ URp, const int32_t* __restrict__ URj,
+                                                 const double* __restrict__ URxd, double* __restrict__ xd, int32_t R)
+{
+    const int2 tl = tiles[blockIdx.x];
+    const BlockDesc B = blocks[tl.x];
+    const T* URx = reinterpret_cast<const T*>(URxd);
+    T* x = reinterpret_cast<T*>(xd);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int t = tl.y + wid;
+    if (t >= B.w) return;
+    const int64_t i = (int64_t)B.c0 + t, c1 = (int64_t)B.c0 + B.w;
+    T acc[RM<T>::v];
+#pragma unroll
+    for (int r = 0; r < RM<T>::v; r++) acc[r] = Sc<T>::zero();
+    for (int64_t q = URp[i] + lane; q < URp[i + 1]; q += 64) {
+        const int64_t j = URj[q];
+        if (j < c1) continue;
+        const T u = URx[q];
+#pragma unroll
+        for (int r = 0; r < RM<T>::v; r++)
+            if (r < R) acc[r] = Sc<T>::fma_(acc[r], u, x[j * R + r]);
+    }
+#pragma unroll
+    for (int r = 0; r < RM<T>::v; r++) {
+        if (r >= R) break;
+        const T s = wave_sum(acc[r]);
+        if (lane == 0) x[i * R + r] = Sc<T>::sub(x[i * R + r], s);
     }
 }
 
+// U x = y, part 2 (one workgroup per block): the block's upper triangle, dense in the block's pivot rows (the panel
+// LU left U(block, block) above the block's diagonal), thread t owning row t, last column first
 template <class T>
-__global__ __launch_bounds__(TB) void k_ltbwd(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
-                                               const int32_t* __restrict__ pinv_rows, double* __restrict__ xd, int64_t ldx,
-                                               int32_t nrhs, int conj)
+__global__ __launch_bounds__(TB) void k_ub_tri(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
+                                                double* __restrict__ xd, int32_t R)
 {
-    __shared__ T ys[TB];
+    __shared__ T ys[TB * RM<T>::v];
     const BlockDesc B = blocks[blk[blockIdx.x]];
     const T* V = reinterpret_cast<const T*>(B.vals);
     T* x = reinterpret_cast<T*>(xd);
-    const int t = threadIdx.x, lane = t & 63, wid = t >> 6, w = B.w;
-    for (int32_t r = 0; r < nrhs; r++) {
-        T* xr = x + (int64_t)r * ldx;
-        for (int c = wid; c < w; c += TB / 64) {
-            T acc = Sc<T>::zero();
-            for (int32_t i = w + lane; i < B.nrows; i += 64)
-                acc = Sc<T>::fma_(acc, cj(V[(int64_t)c * B.nrows + i], conj), xr[pinv_rows[B.rows[i]]]);
-            acc = wave_sum(acc);
-            if (lane == 0) ys[c] = Sc<T>::sub(xr[B.c0 + c], acc);
-        }
+    const int w = B.w;
+    const int64_t c0 = B.c0;
+    for (int idx = threadIdx.x; idx < w * R; idx += TB) ys[idx] = x[(c0 + idx / R) * R + idx % R];
+    __syncthreads();
+    for (int c = w - 1; c >= 0; c--) {
+        const T d = V[(int64_t)c * B.nrows + c];
+        if ((int)threadIdx.x < R) ys[c * R + threadIdx.x] = Sc<T>::div(ys[c * R + threadIdx.x], d);
         __syncthreads();
-        // the triangle, last column first: x_c = ys_c - sum_{rr > c} L(rr, c) x_rr (thread rr holds x_rr)
-        T my = t < w ? ys[t] : Sc<T>::zero();
-        for (int c = w - 1; c >= 0; c--) {
-            // every thread rr > c contributes L(rr, c) x_rr; a block reduction into ys[c]
-            T part = (t > c && t < w) ? Sc<T>::mul(cj(V[(int64_t)c * B.nrows + t], conj), my) : Sc<T>::zero();
-            part = wave_sum(part);
-            __shared__ T red[TB / 64];
-            if (lane == 0) red[wid] = part;
-            __syncthreads();
-            if (t == c) {
-                T s = Sc<T>::zero();
-                for (int k2 = 0; k2 < TB / 64; k2++) s = Sc<T>::add(s, red[k2]);
-                my = Sc<T>::sub(my, s);
-            }
-            __syncthreads();
+        for (int idx = threadIdx.x; idx < c * R; idx += TB) {
+            const int t = idx / R, r = idx % R;
+            ys[t * R + r] = Sc<T>::fms(ys[t * R + r], V[(int64_t)c * B.nrows + t], ys[c * R + r]);
         }
-        if (t < w) xr[B.c0 + t] = my;
         __syncthreads();
     }
+    for (int idx = threadIdx.x; idx < w * R; idx += TB) x[(c0 + idx / R) * R + idx % R] = ys[idx];
 }
 
+// U^T y = b (U^H with conj), one workgroup per block
 template <class T>
-__global__ void k_gather_scale(int64_t n, int32_t nrhs, const int64_t* __restrict__ perm, const double* __restrict__ D,
+__global__ __launch_bounds__(TB) void k_utfwd(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
+                                               const int64_t* __restrict__ Up, const int32_t* __restrict__ Ui,
+                                               const double* __restrict__ Uxd, double* __restrict__ xd, int32_t R, int conj)
+{
+    __shared__ T ys[TB * RM<T>::v];
+    const BlockDesc B = blocks[blk[blockIdx.x]];
+    const T* Ux = reinterpret_cast<const T*>(Uxd);
+    T* x = reinterpret_cast<T*>(xd);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, w = B.w;
+    const int64_t c0 = B.c0;
+    // the dot products over rows before the block: one wavefront per column
+    for (int c = wid; c < w; c += TB / 64) {
+        const int64_t q0 = Up[c0 + c], qd = Up[c0 + c + 1] - 1;
+        T acc[RM<T>::v];
+#pragma unroll
+        for (int r = 0; r < RM<T>::v; r++) acc[r] = Sc<T>::zero();
+        for (int64_t q = q0 + lane; q < qd; q += 64) {
+            const int64_t i = Ui[q];
+            if (i >= c0) continue;
+            const T u = cj(Ux[q], conj);
+#pragma unroll
+            for (int r = 0; r < RM<T>::v; r++)
+                if (r < R) acc[r] = Sc<T>::fma_(acc[r], u, x[i * R + r]);
+        }
+#pragma unroll
+        for (int r = 0; r < RM<T>::v; r++) {
+            if (r >= R) break;
+            const T s = wave_sum(acc[r]);
+            if (lane == 0) ys[c * R + r] = Sc<T>::sub(x[(c0 + c) * R + r], s);
+        }
+    }
+    __syncthreads();
+    // the block's own rows, column by column; thread r handles right-hand side r
+    for (int c = 0; c < w; c++) {
+        const int64_t q0 = Up[c0 + c], qd = Up[c0 + c + 1] - 1;
+        if ((int)threadIdx.x < R) {
+            const int r = threadIdx.x;
+            T acc = ys[c * R + r];
+            for (int64_t q = q0; q < qd; q++) {
+                const int64_t i = Ui[q];
+                if (i >= c0) acc = Sc<T>::fms(acc, cj(Ux[q], conj), ys[(i - c0) * R + r]);
+            }
+            ys[c * R + r] = Sc<T>::div(acc, cj(Ux[qd], conj));
+        }
+        __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < w * R; idx += TB) x[(c0 + idx / R) * R + idx % R] = ys[idx];
+}
+
+// L^T x = y (L^H with conj), one workgroup per block
+template <class T>
+__global__ __launch_bounds__(TB) void k_ltbwd(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
+                                               const int32_t* __restrict__ pinv_rows, double* __restrict__ xd, int32_t R, int conj)
+{
+    __shared__ T ys[TB * RM<T>::v];
+    const BlockDesc B = blocks[blk[blockIdx.x]];
+    const T* V = reinterpret_cast<const T*>(B.vals);
+    T* x = reinterpret_cast<T*>(xd);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, w = B.w;
+    for (int c = wid; c < w; c += TB / 64) {
+        T acc[RM<T>::v];
+#pragma unroll
+        for (int r = 0; r < RM<T>::v; r++) acc[r] = Sc<T>::zero();
+        for (int32_t i = w + lane; i < B.nrows; i += 64) {
+            const T v = cj(V[(int64_t)c * B.nrows + i], conj);
+            const int64_t pi = pinv_rows[B.rows[i]];
+#pragma unroll
+            for (int r = 0; r < RM<T>::v; r++)
+                if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, x[pi * R + r]);
+        }
+#pragma unroll
+        for (int r = 0; r < RM<T>::v; r++) {
+            if (r >= R) break;
+            const T s = wave_sum(acc[r]);
+            if (lane == 0) ys[c * R + r] = Sc<T>::sub(x[(B.c0 + c) * R + r], s);
+        }
+    }
+    __syncthreads();
+    // the triangle, last column first: x_c = ys_c - sum_{rr > c} L(rr, c) x_rr
+    for (int c = w - 1; c >= 0; c--) {
+        if ((int)threadIdx.x < R) {
+            const int r = threadIdx.x;
+            T acc = ys[c * R + r];
+            for (int rr = c + 1; rr < w; rr++) acc = Sc<T>::fms(acc, cj(V[(int64_t)c * B.nrows + rr], conj), ys[rr * R + r]);
+            ys[c * R + r] = acc;
+        }
+        __syncthreads();
+    }
+    for (int idx = threadIdx.x; idx < w * R; idx += TB) x[(B.c0 + idx / R) * R + idx % R] = ys[idx];
+}
+
+// w[k * R + r] = D[perm[k]] * b[perm[k] + r * ldb]
+template <class T>
+__global__ void k_gather_scale(int64_t n, int32_t R, const int64_t* __restrict__ perm, const double* __restrict__ D,
                                const double* __restrict__ bd, int64_t ldb, double* __restrict__ wd)
 {
 #pragma clang fp contract(off)
     const T* b = reinterpret_cast<const T*>(bd);
     T* w = reinterpret_cast<T*>(wd);
-    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t src = perm[k];
-        const double d = D[src];
-        for (int32_t r = 0; r < nrhs; r++) w[k + (int64_t)r * n] = Sc<T>::scale(b[src + (int64_t)r * ldb], d);
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n * R; idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = idx / R, r = idx % R, src = perm[k];
+        w[idx] = Sc<T>::scale(b[src + r * ldb], D[src]);
     }
 }
 
+// b[perm[k] + r * ldb] = D[perm[k]] * w[k * R + r]
 template <class T>
-__global__ void k_scatter_scale(int64_t n, int32_t nrhs, const int64_t* __restrict__ perm, const double* __restrict__ D,
+__global__ void k_scatter_scale(int64_t n, int32_t R, const int64_t* __restrict__ perm, const double* __restrict__ D,
                                 const double* __restrict__ wd, double* __restrict__ bd, int64_t ldb)
 {
 #pragma clang fp contract(off)
     const T* w = reinterpret_cast<const T*>(wd);
     T* b = reinterpret_cast<T*>(bd);
-    for (int64_t k = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t dst = perm[k];
-        const double d = D[dst];
-        for (int32_t r = 0; r < nrhs; r++) b[dst + (int64_t)r * ldb] = Sc<T>::scale(w[k + (int64_t)r * n], d);
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < n * R; idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = idx / R, r = idx % R, dst = perm[k];
+        b[dst + r * ldb] = Sc<T>::scale(w[idx], D[dst]);
     }
 }
 
@@ -216,47 +293,68 @@ inline unsigned gridn(int64_t n)
 }  // namespace
 
 namespace k {
-void lfwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int32_t* pinv_rows, double* x,
-          int64_t ldx, int32_t nrhs, hipStream_t s)
+int solve_rhs_per_pass(int vs) { return vs == 1 ? RM<double>::v : RM<Zd>::v; }
+
+void lfwd_pull(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int64_t* incp, const int64_t* incs,
+               const double* contrib, double* x, int32_t R, double bytes, hipStream_t s)
 {
     if (nblk <= 0) return;
-    if (vs == 1) hipLaunchKernelGGL(k_lfwd<double>, dim3(nblk), dim3(TB), 0, s, blk, blocks, pinv_rows, x, ldx, nrhs);
-    else hipLaunchKernelGGL(k_lfwd<Zd>, dim3(nblk), dim3(TB), 0, s, blk, blocks, pinv_rows, x, ldx, nrhs);
+    ProfScope ps("lfwd_pull", bytes, s);
+    if (vs == 1) hipLaunchKernelGGL(k_lfwd_pull<double>, dim3(nblk), dim3(TB), 0, s, blk, blocks, incp, incs, contrib, x, R);
+    else hipLaunchKernelGGL(k_lfwd_pull<Zd>, dim3(nblk), dim3(TB), 0, s, blk, blocks, incp, incs, contrib, x, R);
 }
-void ubwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
-          const double* Ux, double* x, int64_t ldx, int32_t nrhs, hipStream_t s)
+void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* slot_off, const double* x,
+               double* contrib, int32_t R, double bytes, hipStream_t s)
+{
+    if (ntile <= 0) return;
+    ProfScope ps("lfwd_push", bytes, s);
+    if (vs == 1) hipLaunchKernelGGL(k_lfwd_push<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, slot_off, x, contrib, R);
+    else hipLaunchKernelGGL(k_lfwd_push<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, slot_off, x, contrib, R);
+}
+void ub_dots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* URp, const int32_t* URj,
+             const double* URx, double* x, int32_t R, double bytes, hipStream_t s)
+{
+    if (ntile <= 0) return;
+    ProfScope ps("ub_dots", bytes, s);
+    if (vs == 1) hipLaunchKernelGGL(k_ub_dots<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, URp, URj, URx, x, R);
+    else hipLaunchKernelGGL(k_ub_dots<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, URp, URj, URx, x, R);
+}
+void ub_tri(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, double* x, int32_t R, double bytes, hipStream_t s)
 {
     if (nblk <= 0) return;
-    if (vs == 1) hipLaunchKernelGGL(k_ubwd<double>, dim3(nblk), dim3(TB), 0, s, blk, blocks, Up, Ui, Ux, x, ldx, nrhs);
-    else hipLaunchKernelGGL(k_ubwd<Zd>, dim3(nblk), dim3(TB), 0, s, blk, blocks, Up, Ui, Ux, x, ldx, nrhs);
+    ProfScope ps("ub_tri", bytes, s);
+    if (vs == 1) hipLaunchKernelGGL(k_ub_tri<double>, dim3(nblk), dim3(TB), 0, s, blk, blocks, x, R);
+    else hipLaunchKernelGGL(k_ub_tri<Zd>, dim3(nblk), dim3(TB), 0, s, blk, blocks, x, R);
 }
 void utfwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
-           const double* Ux, double* x, int64_t ldx, int32_t nrhs, int conj, hipStream_t s)
+           const double* Ux, double* x, int32_t R, int conj, double bytes, hipStream_t s)
 {
     if (nblk <= 0) return;
-    if (vs == 1) hipLaunchKernelGGL(k_utfwd<double>, dim3(nblk), dim3(TB), 0, s, blk, blocks, Up, Ui, Ux, x, ldx, nrhs, conj);
-    else hipLaunchKernelGGL(k_utfwd<Zd>, dim3(nblk), dim3(TB), 0, s, blk, blocks, Up, Ui, Ux, x, ldx, nrhs, conj);
+    ProfScope ps("utfwd", bytes, s);
+    if (vs == 1) hipLaunchKernelGGL(k_utfwd<double>, dim3(nblk), dim3(TB), 0, s, blk, blocks, Up, Ui, Ux, x, R, conj);
+    else hipLaunchKernelGGL(k_utfwd<Zd>, dim3(nblk), dim3(TB), 0, s, blk, blocks, Up, Ui, Ux, x, R, conj);
 }
-void ltbwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int32_t* pinv_rows, double* x,
-           int64_t ldx, int32_t nrhs, int conj, hipStream_t s)
+void ltbwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int32_t* pinv_rows, double* x, int32_t R,
+           int conj, double bytes, hipStream_t s)
 {
     if (nblk <= 0) return;
-    if (vs == 1) hipLaunchKernelGGL(k_ltbwd<double>, dim3(nblk), dim3(TB), 0, s, blk, blocks, pinv_rows, x, ldx, nrhs, conj);
-    else hipLaunchKernelGGL(k_ltbwd<Zd>, dim3(nblk), dim3(TB), 0, s, blk, blocks, pinv_rows, x, ldx, nrhs, conj);
+    ProfScope ps("ltbwd", bytes, s);
+    if (vs == 1) hipLaunchKernelGGL(k_ltbwd<double>, dim3(nblk), dim3(TB), 0, s, blk, blocks, pinv_rows, x, R, conj);
+    else hipLaunchKernelGGL(k_ltbwd<Zd>, dim3(nblk), dim3(TB), 0, s, blk, blocks, pinv_rows, x, R, conj);
 }
-void gather_scale(int vs, int64_t n, int32_t nrhs, const int64_t* perm, const double* D, const double* b, int64_t ldb,
-                  double* w, hipStream_t s)
+void gather_scale(int vs, int64_t n, int32_t R, const int64_t* perm, const double* D, const double* b, int64_t ldb, double* w,
+                  hipStream_t s)
 {
-    if (n <= 0) return;
-    if (vs == 1) hipLaunchKernelGGL(k_gather_scale<double>, dim3(gridn(n)), dim3(TB), 0, s, n, nrhs, perm, D, b, ldb, w);
-    else hipLaunchKernelGGL(k_gather_scale<Zd>, dim3(gridn(n)), dim3(TB), 0, s, n, nrhs, perm, D, b, ldb, w);
+    if (n <= 0 || R <= 0) return;
+    if (vs == 1) hipLaunchKernelGGL(k_gather_scale<double>, dim3(gridn(n * R)), dim3(TB), 0, s, n, R, perm, D, b, ldb, w);
+    else hipLaunchKernelGGL(k_gather_scale<Zd>, dim3(gridn(n * R)), dim3(TB), 0, s, n, R, perm, D, b, ldb, w);
 }
-void scatter_scale(int vs, int64_t n, int32_t nrhs, const int64_t* perm, const double* D, const double* w, double* b,
-                   int64_t ldb, hipStream_t s)
+void scatter_scale(int vs, int64_t n, int32_t R, const int64_t* perm, const double* D, const double* w, double* b, int64_t ldb,
+                   hipStream_t s)
 {
-    if (n <= 0) return;
-    if (vs == 1) hipLaunchKernelGGL(k_scatter_scale<double>, dim3(gridn(n)), dim3(TB), 0, s, n, nrhs, perm, D, w, b, ldb);
-    else hipLaunchKernelGGL(k_scatter_scale<Zd>, dim3(gridn(n)), dim3(TB), 0, s, n, nrhs, perm, D, w, b, ldb);
+    if (n <= 0 || R <= 0) return;
+    if (vs == 1) hipLaunchKernelGGL(k_scatter_scale<double>, dim3(gridn(n * R)), dim3(TB), 0, s, n, R, perm, D, w, b, ldb);
+    else hipLaunchKernelGGL(k_scatter_scale<Zd>, dim3(gridn(n * R)), dim3(TB), 0, s, n, R, perm, D, w, b, ldb);
 }
 }  // namespace k
 }  // namespace fpa

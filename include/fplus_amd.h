@@ -27,6 +27,26 @@ typedef struct {
 } fplus_amd_stats;
 int fplus_amd_numeric_stats(const fplus_numeric *num, fplus_amd_stats *st);
 
+/* Device-resident inputs, for callers that keep the values and right-hand sides in HBM (refactorization loops,
+ * benchmarks): fplus_factor with Ax a device pointer (the analysed pattern Ap, Ai on the host), and fplus_solve on a
+ * device array b (column-major, leading dimension ldb), in place. */
+int fplus_amd_factor_device(const fplus_symbolic *sym, const int64_t *Ap, const int64_t *Ai, const void *d_Ax, int is_complex,
+                            const fplus_options *opt, fplus_numeric **num, char *err, size_t errlen);
+int fplus_amd_solve_device(const fplus_numeric *num, void *d_b, int64_t nrhs, int64_t ldb, int trans, char *err, size_t errlen);
+
+/* Device memory and the library's stream for those callers: allocate, free, copy (kind 0 host to device, 1 device to
+ * host, 2 device to device), wait for the library's stream. */
+int fplus_amd_dev_alloc(size_t bytes, void **ptr);
+int fplus_amd_dev_free(void *ptr);
+int fplus_amd_dev_copy(void *dst, const void *src, size_t bytes, int kind);
+int fplus_amd_dev_sync(void);
+
+/* Per-kernel timing with HIP events on the library's stream (benchmarks): on = 1 starts recording and clears the sums,
+ * on = 0 stops.  fplus_amd_profile_get returns, for a kernel name, the summed milliseconds, the launches and the summed
+ * algorithmic work (flops, or bytes for the streaming kernels) of the launches recorded so far. */
+int fplus_amd_profile(int on);
+int fplus_amd_profile_get(const char *kernel, double *ms, int64_t *launches, double *work);
+
 #ifdef __cplusplus
 }
 #endif
