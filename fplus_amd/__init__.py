@@ -14,7 +14,7 @@ import scipy.sparse as sp
 from ._lib import (FPLUS_ORDER_AMD_ATA, FPLUS_ORDER_AUTO, FPLUS_ORDER_COLAMD, FPLUS_PIVOT_PARTIAL,  # noqa: F401
                    FPLUS_PIVOT_REL_DEFAULT, Factors, FplusError, Options, Stats, check, lib)
 
-__all__ = ["SparseLU", "Symbolic", "Numeric", "default_options", "FplusError", "ruiz", "factor_sn_perm"]
+__all__ = ["SparseLU", "Symbolic", "Numeric", "SolvePlan", "default_options", "FplusError", "ruiz", "factor_sn_perm", "tri_solve"]
 
 _I64P = C.POINTER(C.c_int64)
 _DP = C.POINTER(C.c_double)
@@ -286,3 +286,56 @@ def factor_sn_perm(n, Ap, Ai, Ax, row_perm, col_perm, equilibrate=True, relax_ro
         return out
     finally:
         lib().fplus_factors_free(C.byref(f))
+
+
+class SolvePlan:
+    """A solve plan for caller-held CSC factors (reference _sparse_lu_c.c ``solve_plan``, src/solve_plan.c:81-207): the
+    patterns of L (unit lower, rows in pivot order) and U (upper, diagonal stored) go to the device once; ``solve``
+    uploads the values and the right-hand sides for each call."""
+
+    def __init__(self, Lp, Li, Up, Ui):
+        self.n = len(Lp) - 1
+        self._keep = [np.ascontiguousarray(a, np.int64) for a in (Lp, Li, Up, Ui)]
+        self.ptr = C.c_void_p()
+        err = C.create_string_buffer(256)
+        check(lib().fplus_solve_plan_create(self.n, *[_p64(a) for a in self._keep], C.byref(self.ptr), err, 256), err)
+
+    @property
+    def nsupernodes(self):
+        """the supernodes of L the reference's plan finds (-1 for a factor without the diagonal first in every column)"""
+        return lib().fplus_solve_plan_nsupernodes(self.ptr)
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            lib().fplus_solve_plan_free(self.ptr)
+            self.ptr = None
+
+
+def tri_solve(L, U, b, trans=0, plan=None):
+    """x = U^-1 L^-1 b (trans 0), L^-T U^-T b (1) or L^-H U^-H b (2) for CSC factors L = (Lp, Li, Lx), U = (Up, Ui, Ux);
+    b of shape (n,) or (n, k).  With a plan: fplus_tri_solve_plan (the sizes must match the plan's, else ValueError),
+    without: fplus_tri_solve per column (reference sparse_lu_full_pivot.py _tri_solve_c)."""
+    Lp, Li, Lx = (np.ascontiguousarray(a) for a in L)
+    Up, Ui, Ux = (np.ascontiguousarray(a) for a in U)
+    Lp, Li, Up, Ui = (np.ascontiguousarray(a, np.int64) for a in (Lp, Li, Up, Ui))
+    cplx = np.iscomplexobj(Lx) or np.iscomplexobj(Ux) or np.iscomplexobj(b)
+    dt = np.complex128 if cplx else np.float64
+    Lx, Ux = np.ascontiguousarray(Lx, dt), np.ascontiguousarray(Ux, dt)
+    b = np.asarray(b)
+    one = b.ndim == 1
+    n = len(Lp) - 1
+    X = np.array(b.reshape(n, -1), dtype=dt, order="F", copy=True)
+    k = X.shape[1]
+    if plan is not None:
+        if plan.n != n or plan._keep[0][-1] != Lp[-1] or plan._keep[2][-1] != Up[-1]:
+            raise ValueError("plan does not match the factors")
+        rc = lib().fplus_tri_solve_plan(plan.ptr, n, _p64(Lp), _p64(Li), _vp(Lx), _p64(Up), _p64(Ui), _vp(Ux), _vp(X), k, n,
+                                        int(trans), int(cplx))
+        check(rc, None)
+    else:
+        for r in range(k):
+            col = np.ascontiguousarray(X[:, r])
+            check(lib().fplus_tri_solve(n, _p64(Lp), _p64(Li), _vp(Lx), _p64(Up), _p64(Ui), _vp(Ux), _vp(col), int(trans),
+                                        int(cplx)), None)
+            X[:, r] = col
+    return X[:, 0] if one else X

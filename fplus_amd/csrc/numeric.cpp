@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <memory>
 #include <new>
+#include <thread>
 
 using namespace fpa;
 
@@ -187,7 +188,7 @@ struct fplus_numeric {
     int ruiz_iters = 0;
     // solve data on the device
     DevBuf<int64_t> d_p, d_q, d_Up;
-    DevBuf<int32_t> d_Ui, d_pinv;
+    DevBuf<int32_t> d_pinv;
     DevBuf<double> d_Dr, d_Dc;
     std::vector<int32_t> lev_lf, lev_ub, lev_utf, lev_ltb;   // level offsets into the block lists
     DevBuf<int32_t> d_blk_lf, d_blk_ub, d_blk_utf, d_blk_ltb;
@@ -211,7 +212,7 @@ struct fplus_numeric {
     mutable bool exported = false;
     mutable fplus_factors f{};
     // timings of the last factorization
-    double t_ruiz = 0, t_factor = 0, t_total = 0;
+    double t_ruiz = 0, t_factor = 0, t_total = 0, t_prepare = 0;
 };
 
 namespace {
@@ -328,20 +329,45 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         num->d_lft.alloc(tiles.size() > 0 ? tiles.size() : 1);
         num->d_lft.upload(tiles.data(), tiles.size(), s);
     }
-    // U row-compressed: a counting sort of U's entries by row, columns ascending within a row
+    // U row-compressed: a counting sort of U's entries by row, columns ascending within a row; threads take column
+    // ranges and place their entries after the earlier ranges' (the same arrays at any thread count)
     {
         const int64_t nnzU = (int64_t)F.Ui.size();
         std::vector<int64_t> URp(n + 1, 0), map(nnzU > 0 ? nnzU : 1);
         std::vector<int32_t> URj(nnzU > 0 ? nnzU : 1);
-        for (int64_t q = 0; q < nnzU; q++) URp[F.Ui[q] + 1]++;
-        for (int64_t i = 0; i < n; i++) URp[i + 1] += URp[i];
-        std::vector<int64_t> fill(URp.begin(), URp.end() - 1);
-        for (int64_t j = 0; j < n; j++)
-            for (int64_t q = F.Up[j]; q < F.Up[j + 1]; q++) {
-                const int64_t o = fill[F.Ui[q]]++;
-                URj[o] = (int32_t)j;
-                map[o] = q;
-            }
+        const int T = (int)std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(), nnzU / 1000000 + 1}));
+        std::vector<int64_t> jb(T + 1);
+        for (int t = 0; t <= T; t++) jb[t] = n * t / T;
+        std::vector<std::vector<int64_t>> cnt(T, std::vector<int64_t>(n > 0 ? n : 1, 0));
+        {
+            std::vector<std::thread> th;
+            for (int t = 0; t < T; t++)
+                th.emplace_back([&, t]() {
+                    std::vector<int64_t>& c = cnt[t];
+                    for (int64_t q = F.Up[jb[t]]; q < F.Up[jb[t + 1]]; q++) c[F.Ui[q]]++;
+                });
+            for (auto& x : th) x.join();
+        }
+        for (int64_t i = 0; i < n; i++) {   // URp, and every thread's first position in every row
+            int64_t o = URp[i];
+            for (int t = 0; t < T; t++) { const int64_t c = cnt[t][i]; cnt[t][i] = o; o += c; }
+            URp[i + 1] = o;
+        }
+        {
+            std::vector<std::thread> th;
+            for (int t = 0; t < T; t++)
+                th.emplace_back([&, t]() {
+                    std::vector<int64_t>& pos = cnt[t];
+                    for (int64_t j = jb[t]; j < jb[t + 1]; j++)
+                        for (int64_t q = F.Up[j]; q < F.Up[j + 1]; q++) {
+                            const int64_t o = pos[F.Ui[q]]++;
+                            URj[o] = (int32_t)j;
+                            map[o] = q;
+                        }
+                });
+            for (auto& x : th) x.join();
+        }
+        cnt.clear();
         num->d_URp.alloc(n + 1);
         num->d_URp.upload(URp.data(), n + 1, s);
         num->d_URj.alloc(URj.size());
@@ -388,10 +414,8 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
     up32(num->d_blk_ub, num->blk_ub);
     up32(num->d_blk_utf, num->blk_utf);
     up32(num->d_blk_ltb, num->blk_ltb);
-    std::vector<int32_t> ui32(F.Ui.size()), pinv32(n > 0 ? n : 1);
-    for (size_t q = 0; q < F.Ui.size(); q++) ui32[q] = (int32_t)F.Ui[q];
+    std::vector<int32_t> pinv32(n > 0 ? n : 1);
     for (int64_t i = 0; i < n; i++) pinv32[i] = (int32_t)F.pinv[i];
-    up32(num->d_Ui, ui32);
     up32(num->d_pinv, pinv32);
     num->d_Up.alloc(n + 1);
     num->d_Up.upload(F.Up.data(), n + 1, s);
@@ -429,7 +453,7 @@ void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, hi
         const int cj = trans == 2 && vs == 2;
         for (size_t L = 0; L + 1 < num->lev_utf.size(); L++)
             k::utfwd(vs, num->lev_utf[L + 1] - num->lev_utf[L], num->d_blk_utf.p + num->lev_utf[L], blocks, num->d_Up.p,
-                     num->d_Ui.p, F.d_Ux.p, w, R, cj, num->bytes_utf[L], s);
+                     num->F.d_Ui.p, F.d_Ux.p, w, R, cj, num->bytes_utf[L], s);
         for (size_t L = 0; L + 1 < num->lev_ltb.size(); L++)
             k::ltbwd(vs, num->lev_ltb[L + 1] - num->lev_ltb[L], num->d_blk_ltb.p + num->lev_ltb[L], blocks, num->d_pinv.p, w, R,
                      cj, num->bytes_ltb[L], s);
@@ -479,7 +503,7 @@ void export_factors(const DevFactor& F, int is_complex, fplus_factors* f)
     f->q = (int64_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
     if (!f->Lp || !f->Li || !f->Lx || !f->Up || !f->Ui || !f->Ux || !f->p || !f->q) { fplus_factors_free(f); throw std::bad_alloc(); }
     std::memcpy(f->Up, F.Up.data(), (n + 1) * sizeof(int64_t));
-    std::memcpy(f->Ui, F.Ui.data(), F.Ui.size() * sizeof(int64_t));
+    for (size_t q = 0; q < F.Ui.size(); q++) f->Ui[q] = F.Ui[q];
     F.d_Ux.download((double*)f->Ux, (size_t)F.nnzU * vs, s);
     for (int64_t i = 0; i < n; i++) { f->p[i] = F.p[i]; f->q[i] = i; }
     // block values to the host, block by block
@@ -590,6 +614,9 @@ int factor_core(fplus_numeric* num, const PatternCache& c, DevBuf<double>& d_Bx,
                (long)num->F.nnzU, (long)num->F.nsn, (long)num->F.panels, (long)num->F.levels, num->t_factor, num->F.t_host, num->F.t_wait,
                (double)num->F.rows_upper / std::max<int64_t>(1, num->F.panels), (double)num->F.rows_lower / std::max<int64_t>(1, num->F.panels),
                (long)num->F.rows_lower_max);
+    if (rc == FPLUS_OK && verbose >= 1)
+        printf("[FPLUS]   host phases: search %.3fs, tasks/levels %.3fs, symbolic with pivots %.3fs, out %.3fs\n", num->F.t_p1,
+               num->F.t_tasks, num->F.t_p3, num->F.t_out);
     return rc;
 }
 
@@ -685,7 +712,10 @@ static int factor_entry(const fplus_symbolic* sym, const int64_t* Ap, const int6
             num->Dr[rp[k2]] = Drp[k2];
             num->Dc[cp[k2]] = Dcp[k2];
         }
+        const double tps = now_s();
         prepare_solves(num.get(), s);
+        num->t_prepare = now_s() - tps;
+        if (opt->verbose >= 1) printf("[FPLUS] solve preparation %.3fs\n", num->t_prepare);
         num->t_total = now_s() - t0;
     } catch (const GpuError& e) {
         return gpu_error(e, err, errlen);

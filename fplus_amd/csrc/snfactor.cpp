@@ -164,7 +164,8 @@ private:
     std::vector<int32_t> hrowpos_;    // workspace position of a row, -1 outside the panel
     std::vector<int32_t> lposmap_;    // final lower-section position of a row after the panel LU
     std::vector<int64_t> ustamp_, uidx_;
-    std::vector<int64_t> Ui_, Up_;
+    std::vector<int32_t> Ui_;
+    std::vector<int64_t> Up_;
     std::vector<int64_t> umap_;       // workspace offsets of the panel's U values
     // device workspace
     DevBuf<double> d_W_;
@@ -242,6 +243,8 @@ int Driver::panel(int64_t j0, int64_t j1)
     }
     ext_off[np] = (int64_t)ext_s.size();
     np_off[np] = (int64_t)npl.size();
+    const double th_p1 = now_s();
+    F_.t_p1 += th_p1 - th0;
     // ---- the union order (src/factor_sn_panel.inc:61-70)
     std::vector<int64_t> un;
     {
@@ -365,6 +368,7 @@ int Driver::panel(int64_t j0, int64_t j1)
     }
     F_.levels += maxlevel;
     F_.tasks += (int64_t)tasks.size();
+    F_.t_tasks += now_s() - th_p1;
     F_.t_host += now_s() - th0;
     // ---- device: workspace, scatter, updates, panel LU
     d_W_.reserve((size_t)(ldw * np * vs_));
@@ -443,7 +447,7 @@ int Driver::panel(int64_t j0, int64_t j1)
             const HSn& S = sn_[ext_s[q]];
             const int64_t we = ext_s[q] == open0 ? we_open : S.w;
             for (int64_t cc = ext_kf[q] - S.c0; cc < we; cc++) {
-                Ui_.push_back(S.c0 + cc);
+                Ui_.push_back((int32_t)(S.c0 + cc));
                 umap_.push_back((int64_t)hrowpos_[S.ri[cc]] + ci * ldw);
             }
         }
@@ -451,11 +455,11 @@ int Driver::panel(int64_t j0, int64_t j1)
             const HSn& S = sn_[topo_[t]];
             for (int64_t cc = seg0_[topo_[t]] - S.c0; cc < S.w; cc++) {
                 const int64_t col = S.c0 + cc;
-                Ui_.push_back(col);
+                Ui_.push_back((int32_t)col);
                 umap_.push_back(Ru + (col - j0) + ci * ldw);
             }
         }
-        Ui_.push_back(j);
+        Ui_.push_back((int32_t)j);
         umap_.push_back(Ru + ci + ci * ldw);
         Up_[j + 1] = (int64_t)Ui_.size();
         // L(:, j): join the open supernode (exactly or relaxed), else open a new one
@@ -518,6 +522,8 @@ int Driver::panel(int64_t j0, int64_t j1)
         for (int64_t q = ext_off[ci]; q < ext_off[ci + 1]; q++) if (ext_s[q] != open_) prune(ext_s[q], pivrow);
         for (int64_t t = top; t < n; t++) if (topo_[t] != open_) prune(topo_[t], pivrow);
     }
+    const double th_out = now_s();
+    F_.t_p3 += th_out - th1;
     // ---- device: the new blocks and the U values out of the workspace
     std::vector<ExtractCol> ex;
     std::vector<int32_t> expos;
@@ -543,17 +549,25 @@ int Driver::panel(int64_t j0, int64_t j1)
             dev_sync(s_);
             F_.d_Ux = std::move(nb);
         }
+        if (F_.nnzU + cnt > (int64_t)F_.d_Ui.n) {
+            DevBuf<int32_t> nb((size_t)std::max<int64_t>(2 * (F_.nnzU + cnt), 1024));
+            if (F_.nnzU > 0)
+                hip_check(hipMemcpyAsync(nb.p, F_.d_Ui.p, F_.nnzU * sizeof(int32_t), hipMemcpyDeviceToDevice, s_), "grow U rows");
+            dev_sync(s_);
+            F_.d_Ui = std::move(nb);
+        }
+        F_.d_Ui.upload(Ui_.data() + u_begin, cnt, s_, F_.nnzU);
         d_umap_.reserve(cnt);
         d_umap_.upload(umap_.data(), cnt, s_);
         k::gather_vals(vs_, cnt, d_umap_.p, d_W_.p, F_.d_Ux.p + F_.nnzU * vs_, s_);
         F_.nnzU += cnt;
-        (void)u_begin;
     }
     k::rowmap_set((int64_t)univ.size(), d_univ_.p, d_rowmap_.p, 1, s_);
     // the uploads above read host buffers that go out of scope: finish them before returning
     dev_sync(s_);
     for (int32_t row : univ) hrowpos_[row] = -1;
     for (int32_t row : lrows) lposmap_[row] = -1;
+    F_.t_out += now_s() - th_out;
     F_.t_host += now_s() - th1;
     F_.panels++;
     return FPLUS_OK;

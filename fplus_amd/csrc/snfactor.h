@@ -21,17 +21,20 @@ struct DevFactor {
     std::vector<std::vector<int64_t>> sn_rows;   // final rows of each supernode (pivot rows first)
     std::vector<BlockDesc> blocks;       // host mirror of the block table (device pointers)
     std::vector<std::vector<int32_t>> block_rows;   // host copy of every block's rows
-    std::vector<int64_t> Up, Ui;         // U in CSC, column j: earlier supernodes' segments, the diagonal last
+    std::vector<int64_t> Up;             // U in CSC, column j: earlier supernodes' segments, the diagonal last
+    std::vector<int32_t> Ui;             // (row indices in pivot order)
     // device
     DevBuf<BlockDesc> d_blocks;
     std::vector<DevBuf<double>> val_chunks;
     std::vector<DevBuf<int32_t>> row_chunks;
     DevBuf<double> d_Ux;                 // U values, in Ui order
+    DevBuf<int32_t> d_Ui;                // U row indices on the device, filled panel by panel
     int64_t nnzU = 0;
     // statistics
     double flops = 0, gemm_flops = 0;
     int64_t panels = 0, levels = 0, tasks = 0;
     double t_host = 0, t_wait = 0;
+    double t_p1 = 0, t_tasks = 0, t_p3 = 0, t_out = 0;   // host phases: search, tasks and levels, symbolic with the pivots, blocks and U out
     int64_t rows_upper = 0, rows_lower = 0, rows_lower_max = 0;   // workspace rows summed over the panels
 };
 

@@ -657,6 +657,52 @@ int orc_tri_solve(int64_t n, const int64_t *Lp, const int64_t *Li, const double 
     return rc == 0 ? ORC_OK : ORC_SINGULAR;
 }
 
+/* ---------------------------------------------------------------- solve-plan supernodes (src/solve_plan.c:97-199) */
+int64_t orc_plan_nsupernodes(int64_t n, const int64_t *Lp, const int64_t *Li, const int64_t *Up, const int64_t *Ui)
+{
+    int64_t j, q, ns = 0, nseg = 0, odd = 0;
+    for (j = 0; j < n; j++)                       /* the layout: L(j, j) first in column j (:101-102) */
+        if (Lp[j] >= Lp[j + 1] || Li[Lp[j]] != j) return -1;
+    for (j = 0; j < n; j++) {                     /* column j+1 continues j: (j, j+1, R) then (j+1, R) (:108-117) */
+        int join = 0;
+        if (j + 1 < n) {
+            const int64_t a = Lp[j], la = Lp[j + 1] - a, b = Lp[j + 1], lb = Lp[j + 2] - b;
+            join = la >= 2 && lb == la - 1 && Li[a + 1] == j + 1;
+            for (q = 2; join && q < la; q++) join = Li[a + q] == Li[b + q - 1];
+        }
+        if (!join) ns++;
+    }
+    for (j = 0; j < n; j++) {                     /* U segments and odd entries (:137-150) */
+        int64_t prev = -2; int diag = 0;
+        for (q = Up[j]; q < Up[j + 1]; q++) {
+            const int64_t i = Ui[q];
+            if (i > j || (i == j && diag)) odd++;
+            if (i == j) diag = 1;
+            if (i >= j) { prev = -2; continue; }
+            if (i != prev + 1) nseg++;
+            prev = i;
+        }
+    }
+    for (j = 0; j < n && !odd; j++)               /* L entries on or above the diagonal after the first (:154-155) */
+        for (q = Lp[j] + 1; q < Lp[j + 1]; q++) if (Li[q] <= j) { odd++; break; }
+    /* plain sweeps (:156, :162): no segment lists, no overlap test */
+    if (!odd && Lp[n] + Up[n] < 4000000 && (nseg == 0 || Up[n] - n < 35 * nseg) && n <= INT32_MAX) return ns;
+    {   /* blocked: a row repeated above the diagonal of a U column makes segments overlap (:189-199) */
+        int64_t *last = malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+        if (!last) return -2;
+        for (j = 0; j < n; j++) last[j] = -1;
+        for (j = 0; j < n; j++)
+            for (q = Up[j]; q < Up[j + 1]; q++) {
+                const int64_t i = Ui[q];
+                if (i >= j) continue;
+                if (last[i] == j) { free(last); return -1; }
+                last[i] = j;
+            }
+        free(last);
+    }
+    return ns;
+}
+
 /* ---------------------------------------------------------------- composition and solve (src/numeric.c:39-131) */
 void orc_num_free(orc_num *num)
 {

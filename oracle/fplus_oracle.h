@@ -43,6 +43,10 @@ int orc_factor_sn(int64_t n, const int64_t *Ap, const int64_t *Ai, const double 
 void orc_lu_free(orc_lu *f);
 int orc_tri_solve(int64_t n, const int64_t *Lp, const int64_t *Li, const double *Lx, const int64_t *Up,
                   const int64_t *Ui, const double *Ux, double *x, int trans, int cplx);
+/* the supernode count of a solve plan (src/solve_plan.c:97-118 with the layout tests of :101-102, :133-160, :189-199):
+   -1 when L lacks the diagonal first in some column, or when the blocked layout meets a row repeated above the diagonal
+   of a U column */
+int64_t orc_plan_nsupernodes(int64_t n, const int64_t *Lp, const int64_t *Li, const int64_t *Up, const int64_t *Ui);
 /* the composed numeric object: permute, Ruiz, factor, composition of p, q, Dr, Dc into the original coordinates */
 typedef struct {
     orc_lu *lu;

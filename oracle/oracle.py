@@ -61,6 +61,8 @@ def lib():
         L.orc_tri_solve.argtypes = [C.c_int64, _I64P, _I64P, _DP, _I64P, _I64P, _DP, _DP, C.c_int, C.c_int]
         L.orc_backward_error.argtypes = [C.c_int64, _I64P, _I64P, _DP, C.c_int, _DP, _DP, C.c_int]
         L.orc_backward_error.restype = C.c_double
+        L.orc_plan_nsupernodes.argtypes = [C.c_int64, _I64P, _I64P, _I64P, _I64P]
+        L.orc_plan_nsupernodes.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -180,6 +182,32 @@ class Numeric:
             self.ptr = None
 
 
+def tri_solve(L, U, b, trans=0):
+    """the plain sweeps on CSC factors (reference src/solve.c:11-60), one right-hand side at a time"""
+    Lp, Li, Lx = L
+    Up, Ui, Ux = U
+    cplx = np.iscomplexobj(Lx) or np.iscomplexobj(Ux) or np.iscomplexobj(b)
+    dt = np.complex128 if cplx else np.float64
+    Lx, Ux = np.ascontiguousarray(Lx, dt), np.ascontiguousarray(Ux, dt)
+    Lp, Li, Up, Ui = (np.ascontiguousarray(a, np.int64) for a in (Lp, Li, Up, Ui))
+    b = np.asarray(b)
+    n = len(Lp) - 1
+    X = np.array(b.reshape(n, -1), dtype=dt, copy=True)
+    for r in range(X.shape[1]):
+        col = np.ascontiguousarray(X[:, r])
+        rc = lib().orc_tri_solve(n, _p64(Lp), _p64(Li), _pd(_f64view(Lx)), _p64(Up), _p64(Ui), _pd(_f64view(Ux)),
+                                 _pd(_f64view(col)), trans, int(cplx))
+        if rc != 0:
+            raise OracleError(rc, "tri_solve")
+        X[:, r] = col
+    return X[:, 0] if b.ndim == 1 else X
+
+
+def plan_nsupernodes(Lp, Li, Up, Ui):
+    Lp, Li, Up, Ui = (np.ascontiguousarray(a, np.int64) for a in (Lp, Li, Up, Ui))
+    return lib().orc_plan_nsupernodes(len(Lp) - 1, _p64(Lp), _p64(Li), _p64(Up), _p64(Ui))
+
+
 def backward_error(n, Ap, Ai, Ax, x, b, trans=0):
     cplx = int(np.iscomplexobj(Ax))
     x = np.ascontiguousarray(x, dtype=np.complex128 if cplx else np.float64)
@@ -213,8 +241,26 @@ def ref():
                                       C.c_double, C.c_double, C.POINTER(RefFactors), C.c_char_p, C.c_size_t]
         L.fplus_factors_free.argtypes = [C.POINTER(RefFactors)]
         L.fplus_tri_solve.argtypes = [C.c_int64, _I64P, _I64P, C.c_void_p, _I64P, _I64P, C.c_void_p, C.c_void_p, C.c_int, C.c_int]
+        L.fplus_solve_plan_create.argtypes = [C.c_int64, _I64P, _I64P, _I64P, _I64P, C.POINTER(C.c_void_p), C.c_char_p, C.c_size_t]
+        L.fplus_solve_plan_nsupernodes.argtypes = [C.c_void_p]
+        L.fplus_solve_plan_nsupernodes.restype = C.c_int64
+        L.fplus_solve_plan_free.argtypes = [C.c_void_p]
         _ref = L
     return _ref
+
+
+def ref_plan_nsupernodes(Lp, Li, Up, Ui):
+    """the reference's own solve plan (oracle/_ref) on these patterns: its supernode count"""
+    Lp, Li, Up, Ui = (np.ascontiguousarray(a, np.int64) for a in (Lp, Li, Up, Ui))
+    pl = C.c_void_p()
+    err = C.create_string_buffer(256)
+    rc = ref().fplus_solve_plan_create(len(Lp) - 1, _p64(Lp), _p64(Li), _p64(Up), _p64(Ui), C.byref(pl), err, 256)
+    if rc != 0:
+        raise OracleError(rc, err.value.decode())
+    try:
+        return ref().fplus_solve_plan_nsupernodes(pl)
+    finally:
+        ref().fplus_solve_plan_free(pl)
 
 
 def ref_ruiz(n, ptr, idx, x, is_csc=True, max_iter=20, tol=1e-2):
