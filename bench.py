@@ -163,7 +163,7 @@ def main():
         L.fplus_numeric_free(nm)
     # the kernels' sums over the timed region
     kern = {}
-    for name in ("slab_gemm", "slab_solve", "gemm_sub", "getf2", "trsm_sub", "lfwd_pull", "lfwd_push", "ub_dots", "ub_tri", "utfwd",
+    for name in ("blk_gemm", "blk_trsm", "gemm_sub", "getf2", "trsm_sub", "lfwd_pull", "lfwd_push", "ub_dots", "ub_tri", "utfwd",
                  "ltbwd", "ruiz_rowmax", "ruiz_scale_colmax", "ruiz_scale_cols"):
         ms, cnt, work = C.c_double(), C.c_int64(), C.c_double()
         L.fplus_amd_profile_get(name.encode(), C.byref(ms), C.byref(cnt), C.byref(work))
@@ -171,7 +171,7 @@ def main():
             kern[name] = (ms.value / 1e3, cnt.value, work.value)
     dom = max(kern, key=lambda k: kern[k][0])
     t_dom, cnt_dom, work_dom = kern[dom]
-    mfma_kernels = {"slab_gemm", "gemm_sub", "getf2"}
+    mfma_kernels = {"blk_gemm", "blk_trsm", "gemm_sub", "getf2"}
     bound = "mfma" if dom in mfma_kernels else "hbm"
     achieved = work_dom / t_dom
     peak = PEAK_FP64_MFMA if bound == "mfma" else PEAK_HBM

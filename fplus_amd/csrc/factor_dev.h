@@ -4,13 +4,14 @@
 
 #include <cstdint>
 
-#define FPA_SLAB 32        // columns per update slab
 #define FPA_ROW_TILE 64    // rows per update GEMM tile
+#define FPA_COL_TILE 64    // panel columns per update GEMM tile
 #define FPA_PANEL_NB 16    // sub-panel width of the dense panel LU
 
 namespace fpa {
 
-inline int col_tile(int vs) { return vs == 1 ? 64 : 32; }   // panel columns per update tile
+inline int col_tile(int vs) { return vs == 1 ? 64 : 32; }   // panel columns per tile of the panel-LU GEMM
+inline int trsm_cols(int vs) { return vs == 1 ? 8 : 4; }    // panel columns per workgroup of blk_trsm
 
 // A block: the columns [c0, c0 + w) of one supernode computed in one panel, stored column-major over its rows (ld =
 // nrows): the pivot rows of its columns first, in column order, then the supernode's other rows as they stood at the
@@ -23,12 +24,16 @@ struct BlockDesc {
     int32_t c0, sn;
 };
 
-// one slab of a block applied to some panel columns: the pivot rows [k0, k1) of the block are solved with the unit
-// lower triangle of the block, the rows [k1, nrows) updated by the rectangle below
-struct SlabTask {
-    int32_t block, k0, k1;
+// one block applied to some panel columns: the block's pivot rows [k0, w) (the segment the columns reach) are solved
+// with the block's unit lower triangle (blk_trsm), then the rows [w, nrows) updated by the rectangle below (blk_gemm)
+struct UpdTask {
+    int32_t block, k0;
     int32_t col_off, ncol;   // panel-local column indices at cols[col_off .. col_off + ncol)
-    int32_t pad;
+    int64_t xoff;            // the solved segment, compact: X[(k - k0) + c * (w - k0)] at xbuf + xoff scalars
+};
+
+struct TrsmTile {
+    int32_t task, col0;      // columns [col0, col0 + trsm_cols(vs)) of the task's list
 };
 
 struct GemmTile {
@@ -54,10 +59,12 @@ void extract_cols(int vs, int32_t ncols, const ExtractCol* d, const int32_t* pos
 void rowmap_set(int64_t cnt, const int32_t* rows, int32_t* rowmap, int reset, hipStream_t s);
 void scatter_cols(int vs, int32_t ncol, const int64_t* Bp, const int32_t* Bi, const double* Bx, int64_t j0,
                   const int32_t* rowmap, double* W, int64_t ldw, double* colmax, hipStream_t s);
-void slab_solve(int vs, int32_t ntask, const SlabTask* tasks, const int32_t* cols, const BlockDesc* blocks,
-                const int32_t* rowmap, double* W, int64_t ldw, int32_t max_ctiles, hipStream_t s);
-void slab_gemm(int vs, int32_t ntile, const GemmTile* tiles, const SlabTask* tasks, const int32_t* cols,
-               const BlockDesc* blocks, const int32_t* rowmap, double* W, int64_t ldw, double flops, hipStream_t s);
+// the segment solves of the tiles of one level (in place in W, and compact into xbuf)
+void blk_trsm(int vs, int32_t ntile, const TrsmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
+              const int32_t* rowmap, double* W, int64_t ldw, double* xbuf, double flops, hipStream_t s);
+// the rectangles below, subtracted from W through rowmap (atomically: tasks of one level may share target rows)
+void blk_gemm(int vs, int32_t ntile, const GemmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
+              const int32_t* rowmap, double* W, int64_t ldw, const double* xbuf, double flops, hipStream_t s);
 // dense partial-pivot LU of the panel's lower section (nr x P, column-major, ld), sub-panel [a, a + nb)
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
            double pivot_rel, int32_t* piv, PanelStatus* st, hipStream_t s);

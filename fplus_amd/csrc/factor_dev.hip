@@ -4,9 +4,10 @@
 // columns touch -- first the rows already pivotal (the U segments), then the rows not yet pivotal -- found by the host
 // search; rowmap[row] gives a row's position.  The kernels here
 //   * scatter the panel's columns of A into W (scatter_cols);
-//   * apply the earlier supernodes to the panel, slab by slab: the unit-lower solve of a slab's pivot rows
-//     (slab_solve, the TRSM) and the rectangle below subtracted through rowmap (slab_gemm, the GEMM with the
-//     relative-index scatter) -- one launch pair per dependency level, every slab of a level in one launch;
+//   * apply the earlier supernodes to the panel, block by block: the unit-lower solve of the block's pivot rows the
+//     panel's columns reach (blk_trsm, the TRSM) and the rectangle below subtracted through rowmap (blk_gemm, the
+//     GEMM on the FP64 matrix cores with the relative-index scatter) -- one launch pair per dependency level, every
+//     block of a level in one launch;
 //   * factor the rows not yet pivotal by dense partial pivoting (getf2 per sub-panel of nb columns, then the
 //     trailing TRSM and GEMM), LAPACK-style with whole-row swaps, so the pivot rows end at the top in column order.
 // Entries of W outside a column's sparse pattern are exact zeros and stay so, which makes the dense panel LU produce
@@ -52,113 +53,156 @@ __global__ void k_scatter_cols(int32_t ncol, const int64_t* __restrict__ Bp, con
     if (lane == 0) colmax[ci] = cm;
 }
 
-constexpr int SLAB = FPA_SLAB;   // columns per slab (the K of the update GEMM)
 constexpr int RT = FPA_ROW_TILE; // rows per GEMM tile
-// panel columns per tile: 64 real, 32 complex (LDS within 64 KB)
+// panel columns per tile of the panel-LU GEMM: 64 real, 32 complex (LDS within 64 KB)
 template <class T> struct CTile { static constexpr int v = 64; };
 template <> struct CTile<Zd> { static constexpr int v = 32; };
+template <class T> struct TCols { static constexpr int v = 8; };   // columns per blk_trsm workgroup
+template <> struct TCols<Zd> { static constexpr int v = 4; };
 constexpr int PNB = FPA_PANEL_NB;  // sub-panel width of the dense panel LU
 
-// the unit-lower solve of a slab's pivot rows, in place in W, for up to CT panel columns
+// the unit-lower solve of a block's segment [k0, w) for up to TCols panel columns: thread t owns segment row t (the
+// segment is at most 256 rows: a block is at most one panel wide), the rows below the current one subtract its
+// multiple; the result goes back to W and, compact, to the task's slice of xbuf
 template <class T>
-__global__ __launch_bounds__(TB) void k_slab_solve(const SlabTask* __restrict__ tasks, const int32_t* __restrict__ cols,
-                                                    const BlockDesc* __restrict__ blocks, const int32_t* __restrict__ rowmap,
-                                                    double* __restrict__ W, int64_t ldw, int32_t max_ctiles)
+__global__ __launch_bounds__(TB) void k_blk_trsm(const TrsmTile* __restrict__ tiles, const UpdTask* __restrict__ tasks,
+                                                  const int32_t* __restrict__ cols, const BlockDesc* __restrict__ blocks,
+                                                  const int32_t* __restrict__ rowmap, double* __restrict__ W, int64_t ldw,
+                                                  double* __restrict__ xbuf)
 {
-    constexpr int CT = CTile<T>::v;
-    __shared__ T Ls[SLAB][SLAB + 1];
-    __shared__ T Xs[SLAB][CT + 1];
-    __shared__ int32_t rpos[SLAB];
+    constexpr int CT = TCols<T>::v;
+    __shared__ T xs[2][CT];
     __shared__ int32_t cidx[CT];
-    const int32_t t = blockIdx.x / max_ctiles, ct = blockIdx.x % max_ctiles;
-    const SlabTask tk = tasks[t];
-    const int32_t c0 = ct * CT;
-    if (c0 >= tk.ncol) return;
-    const int32_t nc = min(CT, tk.ncol - c0);
+    const TrsmTile tt = tiles[blockIdx.x];
+    const UpdTask tk = tasks[tt.task];
     const BlockDesc B = blocks[tk.block];
-    const int32_t kk = tk.k1 - tk.k0;
+    const int32_t m = B.w - tk.k0, nc = min(CT, tk.ncol - tt.col0), t = threadIdx.x;
     const T* V = reinterpret_cast<const T*>(B.vals);
-    for (int i = threadIdx.x; i < kk; i += TB) rpos[i] = rowmap[B.rows[tk.k0 + i]];
-    for (int i = threadIdx.x; i < nc; i += TB) cidx[i] = cols[tk.col_off + c0 + i];
-    for (int i = threadIdx.x; i < kk * kk; i += TB) {
-        const int r = i % kk, c = i / kk;
-        Ls[r][c] = V[(int64_t)(tk.k0 + c) * B.nrows + tk.k0 + r];
-    }
+    if (t < nc) cidx[t] = cols[tk.col_off + tt.col0 + t];
     __syncthreads();
-    for (int i = threadIdx.x; i < kk * nc; i += TB) {
-        const int r = i % kk, c = i / kk;
-        Xs[r][c] = ld_val<T>(W, (int64_t)rpos[r] + (int64_t)cidx[c] * ldw);
+    const bool own = t < m;
+    const int64_t pos = own ? (int64_t)rowmap[B.rows[tk.k0 + t]] : 0;
+    T x[CT];
+#pragma unroll
+    for (int c = 0; c < CT; c++) x[c] = (own && c < nc) ? ld_val<T>(W, pos + (int64_t)cidx[c] * ldw) : Sc<T>::zero();
+    const T* Vc = V + (int64_t)tk.k0 * B.nrows + tk.k0 + t;   // column k0 of the segment, row t
+    T lnext = (own && t > 0) ? Vc[0] : Sc<T>::zero();
+    for (int32_t k = 0; k < m; k++) {
+        const T lcur = lnext;
+        if (own && t > k + 1) lnext = Vc[(int64_t)(k + 1) * B.nrows];
+        if (t == k) {
+#pragma unroll
+            for (int c = 0; c < CT; c++) xs[k & 1][c] = x[c];
+        }
+        __syncthreads();
+        if (own && t > k) {
+#pragma unroll
+            for (int c = 0; c < CT; c++) x[c] = Sc<T>::fms(x[c], lcur, xs[k & 1][c]);
+        }
     }
-    __syncthreads();
-    // forward substitution: column c of X handled by the threads with tid % CT == c, rows split four ways
-    const int c = threadIdx.x % CT, part = threadIdx.x / CT;
-    for (int a = 0; a < kk - 1; a++) {
+    if (!own) return;
+    T* X = reinterpret_cast<T*>(xbuf) + tk.xoff;
+#pragma unroll
+    for (int c = 0; c < CT; c++)
         if (c < nc) {
-            const T xa = Xs[a][c];
-            for (int r = a + 1 + part; r < kk; r += TB / CT) Xs[r][c] = Sc<T>::fms(Xs[r][c], Ls[r][a], xa);
+            st_val<T>(W, pos + (int64_t)cidx[c] * ldw, x[c]);
+            X[t + (int64_t)(tt.col0 + c) * m] = x[c];
+        }
+}
+
+// FP64 matrix cores: D(16x16) += A(16x4) B(4x16) per wavefront; lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16] and holds D[l / 16 + 4 v][l % 16], v < 4 (checked on gfx950)
+typedef double dx4 __attribute__((ext_vector_type(4)));
+__device__ inline dx4 mfma16(double a, double b, dx4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+// the rectangle below a block's segment for a 64 x 64 tile: W[rows] -= L[rows, seg] X[seg, cols] through rowmap, with
+// the product on the matrix cores (four real products per complex one), k in chunks of 16 staged in LDS
+template <class T>
+__global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ tiles, const UpdTask* __restrict__ tasks,
+                                                  const int32_t* __restrict__ cols, const BlockDesc* __restrict__ blocks,
+                                                  const int32_t* __restrict__ rowmap, double* __restrict__ W, int64_t ldw,
+                                                  const double* __restrict__ xbuf)
+{
+    constexpr bool CPLX = Sc<T>::vs == 2;
+    constexpr int KC = 16, TM = 64, TN = 64;
+    __shared__ T As[KC][TM];     // A[k][row]
+    __shared__ T Bs[KC][TN];     // B[k][col]
+    __shared__ int32_t rpos[TM];
+    __shared__ int32_t cidx[TN];
+    const GemmTile tl = tiles[blockIdx.x];
+    const UpdTask tk = tasks[tl.task];
+    const BlockDesc B = blocks[tk.block];
+    const int32_t m = B.w - tk.k0;
+    const int32_t nr = min(TM, B.nrows - tl.row0), nc = min(TN, tk.ncol - tl.col0);
+    const T* V = reinterpret_cast<const T*>(B.vals);
+    const T* X = reinterpret_cast<const T*>(xbuf) + tk.xoff;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    if (tid < TM) rpos[tid] = tid < nr ? rowmap[B.rows[tl.row0 + tid]] : 0;
+    else if (tid < TM + TN) cidx[tid - TM] = (tid - TM) < nc ? cols[tk.col_off + tl.col0 + tid - TM] : 0;
+    const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;   // the wave's 32 x 32 quarter
+    dx4 accr[2][2], acci[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            accr[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
+            acci[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
+        }
+    for (int32_t kc = 0; kc < m; kc += KC) {
+        const int32_t kn = min(KC, m - kc);
+        // A: column k of the segment is contiguous in the block (rows), B: column c of X is contiguous in k
+        for (int i = tid; i < KC * TM; i += TB) {
+            const int r = i % TM, kk = i / TM;
+            As[kk][r] = (r < nr && kk < kn) ? V[(int64_t)(tk.k0 + kc + kk) * B.nrows + tl.row0 + r] : Sc<T>::zero();
+        }
+        for (int i = tid; i < KC * TN; i += TB) {
+            const int kk = i % KC, c = i / KC;
+            Bs[kk][c] = (c < nc && kk < kn) ? X[(kc + kk) + (int64_t)(tl.col0 + c) * m] : Sc<T>::zero();
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < KC; ks += 4) {
+            const int kk = ks + (lane >> 4);
+            T a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) a[i] = As[kk][wr + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < 2; j++) b[j] = Bs[kk][wc + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    if constexpr (CPLX) {
+                        accr[i][j] = mfma16(a[i].x, b[j].x, accr[i][j]);
+                        accr[i][j] = mfma16(-a[i].y, b[j].y, accr[i][j]);
+                        acci[i][j] = mfma16(a[i].x, b[j].y, acci[i][j]);
+                        acci[i][j] = mfma16(a[i].y, b[j].x, acci[i][j]);
+                    } else {
+                        accr[i][j] = mfma16(a[i], b[j], accr[i][j]);
+                    }
+                }
         }
         __syncthreads();
     }
-    for (int i = threadIdx.x; i < kk * nc; i += TB) {
-        const int r = i % kk, cc = i / kk;
-        st_val<T>(W, (int64_t)rpos[r] + (int64_t)cidx[cc] * ldw, Xs[r][cc]);
-    }
-}
-
-// the rectangle below a slab: W[rows below] -= L[rows below, slab] * X[slab pivot rows], atomically (several slabs of
-// one level may update the same rows)
-template <class T>
-__global__ __launch_bounds__(TB) void k_slab_gemm(const GemmTile* __restrict__ tiles, const SlabTask* __restrict__ tasks,
-                                                   const int32_t* __restrict__ cols, const BlockDesc* __restrict__ blocks,
-                                                   const int32_t* __restrict__ rowmap, double* __restrict__ W, int64_t ldw)
-{
-    constexpr int CT = CTile<T>::v, TC = CT / 16;
-    __shared__ T Ls[RT][SLAB + 1];
-    __shared__ T Xs[SLAB][CT + 1];
-    __shared__ int32_t rpos[RT];
-    __shared__ int32_t cidx[CT];
-    const GemmTile tl = tiles[blockIdx.x];
-    const SlabTask tk = tasks[tl.task];
-    const BlockDesc B = blocks[tk.block];
-    const int32_t kk = tk.k1 - tk.k0;
-    const int32_t nr = min(RT, B.nrows - tl.row0), nc = min(CT, tk.ncol - tl.col0);
-    const T* V = reinterpret_cast<const T*>(B.vals);
-    for (int i = threadIdx.x; i < nr; i += TB) rpos[i] = rowmap[B.rows[tl.row0 + i]];
-    for (int i = threadIdx.x; i < nc; i += TB) cidx[i] = cols[tk.col_off + tl.col0 + i];
-    for (int i = threadIdx.x; i < nr * kk; i += TB) {
-        const int r = i % nr, c = i / nr;
-        Ls[r][c] = V[(int64_t)(tk.k0 + c) * B.nrows + tl.row0 + r];
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kk * nc; i += TB) {
-        const int r = i % kk, c = i / kk;
-        Xs[r][c] = ld_val<T>(W, (int64_t)rowmap[B.rows[tk.k0 + r]] + (int64_t)cidx[c] * ldw);
-    }
-    __syncthreads();
-    // a 4 x TC register tile per thread over the RT x CT output tile
-    const int tr = (threadIdx.x % 16) * 4, tc = (threadIdx.x / 16) * TC;
-    T acc[4][TC];
 #pragma unroll
-    for (int i = 0; i < 4; i++)
+    for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < TC; j++) acc[i][j] = Sc<T>::zero();
-    for (int a = 0; a < kk; a++) {
-        T l[4], x[TC];
+        for (int j = 0; j < 2; j++) {
+            const int c = wc + j * 16 + (lane & 15);
+            if (c >= nc) continue;
 #pragma unroll
-        for (int i = 0; i < 4; i++) l[i] = Ls[tr + i][a];
-#pragma unroll
-        for (int j = 0; j < TC; j++) x[j] = Xs[a][tc + j];
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int j = 0; j < TC; j++) acc[i][j] = Sc<T>::fma_(acc[i][j], l[i], x[j]);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < TC; j++)
-            if (tr + i < nr && tc + j < nc)
-                atomic_sub(reinterpret_cast<T*>(W) + (int64_t)rpos[tr + i] + (int64_t)cidx[tc + j] * ldw, acc[i][j]);
+            for (int v = 0; v < 4; v++) {
+                const int r = wr + i * 16 + (lane >> 4) + 4 * v;
+                if (r >= nr) continue;
+                double* p = W + ((int64_t)rpos[r] + (int64_t)cidx[c] * ldw) * Sc<T>::vs;
+                if constexpr (CPLX) {
+                    atomicAdd(p, -accr[i][j][v]);
+                    atomicAdd(p + 1, -acci[i][j][v]);
+                } else {
+                    atomicAdd(p, -accr[i][j][v]);
+                }
+            }
+        }
 }
 
 // ---- dense partial-pivot LU of the panel's lower section
@@ -342,22 +386,21 @@ void scatter_cols(int vs, int32_t ncol, const int64_t* Bp, const int32_t* Bi, co
     if (vs == 1) hipLaunchKernelGGL(k_scatter_cols<double>, dim3(g), dim3(TB), 0, s, ncol, Bp, Bi, Bx, j0, rowmap, W, ldw, colmax);
     else hipLaunchKernelGGL(k_scatter_cols<Zd>, dim3(g), dim3(TB), 0, s, ncol, Bp, Bi, Bx, j0, rowmap, W, ldw, colmax);
 }
-void slab_solve(int vs, int32_t ntask, const SlabTask* tasks, const int32_t* cols, const BlockDesc* blocks,
-                const int32_t* rowmap, double* W, int64_t ldw, int32_t max_ctiles, hipStream_t s)
-{
-    if (ntask <= 0) return;
-    ProfScope ps("slab_solve", 0.0, s);
-    const unsigned g = (unsigned)(ntask * max_ctiles);
-    if (vs == 1) hipLaunchKernelGGL(k_slab_solve<double>, dim3(g), dim3(TB), 0, s, tasks, cols, blocks, rowmap, W, ldw, max_ctiles);
-    else hipLaunchKernelGGL(k_slab_solve<Zd>, dim3(g), dim3(TB), 0, s, tasks, cols, blocks, rowmap, W, ldw, max_ctiles);
-}
-void slab_gemm(int vs, int32_t ntile, const GemmTile* tiles, const SlabTask* tasks, const int32_t* cols,
-               const BlockDesc* blocks, const int32_t* rowmap, double* W, int64_t ldw, double flops, hipStream_t s)
+void blk_trsm(int vs, int32_t ntile, const TrsmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
+              const int32_t* rowmap, double* W, int64_t ldw, double* xbuf, double flops, hipStream_t s)
 {
     if (ntile <= 0) return;
-    ProfScope ps("slab_gemm", flops, s);
-    if (vs == 1) hipLaunchKernelGGL(k_slab_gemm<double>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw);
-    else hipLaunchKernelGGL(k_slab_gemm<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw);
+    ProfScope ps("blk_trsm", flops, s);
+    if (vs == 1) hipLaunchKernelGGL(k_blk_trsm<double>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf);
+    else hipLaunchKernelGGL(k_blk_trsm<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf);
+}
+void blk_gemm(int vs, int32_t ntile, const GemmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
+              const int32_t* rowmap, double* W, int64_t ldw, const double* xbuf, double flops, hipStream_t s)
+{
+    if (ntile <= 0) return;
+    ProfScope ps("blk_gemm", flops, s);
+    if (vs == 1) hipLaunchKernelGGL(k_blk_gemm<double>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf);
+    else hipLaunchKernelGGL(k_blk_gemm<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf);
 }
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
            double pivot_rel, int32_t* piv, PanelStatus* st, hipStream_t s)

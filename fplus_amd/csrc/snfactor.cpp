@@ -6,8 +6,8 @@
 //   1. host: each column's depth-first search over the supernodes complete before the panel (its reached segments and
 //      non-pivotal rows), the union search giving a topological order, the update tasks and their dependency levels,
 //      the rows the panel touches (the workspace rows);
-//   2. device: the panel's columns of A scattered into the workspace, every earlier supernode applied slab by slab
-//      (TRSM, then GEMM + relative-index scatter), one launch pair per level;
+//   2. device: the panel's columns of A scattered into the workspace, every earlier supernode applied block by block
+//      (TRSM of the reached segment, then GEMM + relative-index scatter), one launch pair per level;
 //   3. device: dense partial-pivot LU of the workspace rows not yet pivotal; the P pivot positions come back;
 //   4. host: per column, the within-panel search, the U pattern, the relaxed-join decision and the new supernodes,
 //      pruning -- the reference's logic, fed with the device's pivots;
@@ -171,8 +171,10 @@ private:
     DevBuf<double> d_W_;
     DevBuf<int32_t> d_rowmap_, d_univ_, d_cols_, d_piv_, d_pos_;
     DevBuf<double> d_colmax_;
-    DevBuf<SlabTask> d_tasks_;
+    DevBuf<UpdTask> d_tasks_;
+    DevBuf<TrsmTile> d_ttiles_;
     DevBuf<GemmTile> d_tiles_;
+    DevBuf<double> d_X_;
     DevBuf<PanelStatus> d_status_;
     DevBuf<int64_t> d_umap_;
     DevBuf<ExtractCol> d_ex_;
@@ -285,8 +287,10 @@ int Driver::panel(int64_t j0, int64_t j1)
     F_.rows_upper += Ru;
     F_.rows_lower += Rl;
     F_.rows_lower_max = std::max(F_.rows_lower_max, Rl);
-    // ---- update tasks per union supernode and block, with dependency levels
-    const int SL = FPA_SLAB, CT = col_tile(vs_), RT = FPA_ROW_TILE;
+    // ---- update tasks per union supernode and block, with dependency levels: a block's task solves the block's pivot
+    // rows the panel's columns reach (read: they wait for every task writing them) and writes the rows below
+    const int TC = trsm_cols(vs_), RT = FPA_ROW_TILE, CTG = FPA_COL_TILE;
+    const double fm = vs_ == 1 ? 2.0 : 8.0;
     std::vector<int64_t> ucnt(un.size() + 1, 0);
     for (size_t q = 0; q < ext_s.size(); q++) ucnt[uidx_[ext_s[q]] + 1]++;
     for (size_t u = 0; u < un.size(); u++) ucnt[u + 1] += ucnt[u];
@@ -300,7 +304,7 @@ int Driver::panel(int64_t j0, int64_t j1)
                 ukf[pos] = ext_kf[q];
             }
     }
-    struct TaskL { SlabTask t; int32_t level; };
+    struct TaskL { UpdTask t; int32_t level; };
     std::vector<TaskL> tl;
     std::vector<int32_t> cols;
     std::vector<int32_t> wl(Ru + Rl, 0);
@@ -319,52 +323,54 @@ int Driver::panel(int64_t j0, int64_t j1)
             const int64_t ncol = (int64_t)cols.size() - col_off;
             if (ncol == 0) continue;
             const std::vector<int32_t>& br = F_.block_rows[b];
-            for (int64_t k0 = std::max<int64_t>(kmin, bb0) - bb0; k0 < B.w; k0 += SL) {
-                const int64_t k1 = std::min<int64_t>(k0 + SL, B.w);
-                int32_t lev = 0;
-                for (int64_t r = k0; r < k1; r++) {
-                    const int32_t hp = hrowpos_[br[r]];
-                    if (hp < 0) { errf(err_, errlen_, "internal: update row outside the panel rows"); return FPLUS_ERR_INTERNAL; }
-                    lev = std::max(lev, wl[hp]);
-                }
-                lev += 1;
-                for (int64_t r = k1; r < B.nrows; r++) {
-                    const int32_t hp = hrowpos_[br[r]];
-                    if (hp < 0) { errf(err_, errlen_, "internal: update row outside the panel rows"); return FPLUS_ERR_INTERNAL; }
-                    wl[hp] = std::max(wl[hp], lev);
-                }
-                TaskL t;
-                t.t.block = b; t.t.k0 = (int32_t)k0; t.t.k1 = (int32_t)k1;
-                t.t.col_off = (int32_t)col_off; t.t.ncol = (int32_t)ncol; t.t.pad = 0;
-                t.level = lev;
-                tl.push_back(t);
-                maxlevel = std::max(maxlevel, lev);
-                F_.gemm_flops += (vs_ == 1 ? 2.0 : 8.0) * (double)ncol * (double)(k1 - k0) * (double)(B.nrows - k1);
+            const int64_t k0 = std::max<int64_t>(kmin, bb0) - bb0;
+            int32_t lev = 0;
+            for (int64_t r = k0; r < B.w; r++) {
+                const int32_t hp = hrowpos_[br[r]];
+                if (hp < 0) { errf(err_, errlen_, "internal: update row outside the panel rows"); return FPLUS_ERR_INTERNAL; }
+                lev = std::max(lev, wl[hp]);
             }
+            lev += 1;
+            for (int64_t r = B.w; r < B.nrows; r++) {
+                const int32_t hp = hrowpos_[br[r]];
+                if (hp < 0) { errf(err_, errlen_, "internal: update row outside the panel rows"); return FPLUS_ERR_INTERNAL; }
+                wl[hp] = std::max(wl[hp], lev);
+            }
+            TaskL t;
+            t.t.block = b; t.t.k0 = (int32_t)k0;
+            t.t.col_off = (int32_t)col_off; t.t.ncol = (int32_t)ncol; t.t.xoff = 0;
+            t.level = lev;
+            tl.push_back(t);
+            maxlevel = std::max(maxlevel, lev);
+            F_.gemm_flops += fm * (double)ncol * (double)(B.w - k0) * (double)(B.nrows - B.w);
         }
     }
     std::stable_sort(tl.begin(), tl.end(), [](const TaskL& a, const TaskL& b) { return a.level < b.level; });
-    std::vector<SlabTask> tasks(tl.size());
-    std::vector<int32_t> lev_task(maxlevel + 2, 0), lev_tile(maxlevel + 2, 0), lev_ct(maxlevel + 2, 0);
-    std::vector<double> lev_flops(maxlevel + 2, 0.0);
-    std::vector<GemmTile> tiles;
+    std::vector<UpdTask> tasks(tl.size());
+    std::vector<int32_t> lev_tt(maxlevel + 2, 0), lev_gt(maxlevel + 2, 0);
+    std::vector<double> lev_gflops(maxlevel + 2, 0.0), lev_tflops(maxlevel + 2, 0.0);
+    std::vector<TrsmTile> ttiles;
+    std::vector<GemmTile> gtiles;
+    int64_t xtot = 0;
     for (size_t i = 0; i < tl.size(); i++) {
-        tasks[i] = tl[i].t;
+        UpdTask& T = tasks[i];
+        T = tl[i].t;
         const int32_t L = tl[i].level;
-        lev_task[L + 1] = (int32_t)i + 1;
-        const BlockDesc& B = F_.blocks[tasks[i].block];
-        lev_ct[L] = std::max<int32_t>(lev_ct[L], (tasks[i].ncol + CT - 1) / CT);
-        lev_flops[L] += (vs_ == 1 ? 2.0 : 8.0) * (double)tasks[i].ncol * (double)(tasks[i].k1 - tasks[i].k0) * (double)(B.nrows - tasks[i].k1);
-        for (int32_t r0 = tasks[i].k1; r0 < B.nrows; r0 += RT)
-            for (int32_t cc = 0; cc < tasks[i].ncol; cc += CT) {
-                GemmTile g; g.task = (int32_t)i; g.row0 = r0; g.col0 = cc; g.pad = 0;
-                tiles.push_back(g);
-            }
-        lev_tile[L + 1] = (int32_t)tiles.size();
+        const BlockDesc& B = F_.blocks[T.block];
+        const int64_t m = B.w - T.k0;
+        T.xoff = xtot;
+        xtot += m * T.ncol;
+        lev_tflops[L] += fm / 2 * (double)T.ncol * (double)m * (double)(m - 1);
+        lev_gflops[L] += fm * (double)T.ncol * (double)m * (double)(B.nrows - B.w);
+        for (int32_t cc = 0; cc < T.ncol; cc += TC) ttiles.push_back(TrsmTile{(int32_t)i, cc});
+        lev_tt[L + 1] = (int32_t)ttiles.size();
+        for (int32_t r0 = B.w; r0 < B.nrows; r0 += RT)
+            for (int32_t cc = 0; cc < T.ncol; cc += CTG) gtiles.push_back(GemmTile{(int32_t)i, r0, cc, 0});
+        lev_gt[L + 1] = (int32_t)gtiles.size();
     }
     for (int32_t L = 1; L <= maxlevel + 1; L++) {
-        lev_task[L] = std::max(lev_task[L], lev_task[L - 1]);
-        lev_tile[L] = std::max(lev_tile[L], lev_tile[L - 1]);
+        lev_tt[L] = std::max(lev_tt[L], lev_tt[L - 1]);
+        lev_gt[L] = std::max(lev_gt[L], lev_gt[L - 1]);
     }
     F_.levels += maxlevel;
     F_.tasks += (int64_t)tasks.size();
@@ -383,14 +389,19 @@ int Driver::panel(int64_t j0, int64_t j1)
         d_tasks_.upload(tasks.data(), tasks.size(), s_);
         d_cols_.reserve(cols.size());
         d_cols_.upload(cols.data(), cols.size(), s_);
-        d_tiles_.reserve(tiles.size());
-        d_tiles_.upload(tiles.data(), tiles.size(), s_);
+        d_ttiles_.reserve(ttiles.size());
+        d_ttiles_.upload(ttiles.data(), ttiles.size(), s_);
+        d_tiles_.reserve(gtiles.size() > 0 ? gtiles.size() : 1);
+        d_tiles_.upload(gtiles.data(), gtiles.size(), s_);
+        d_X_.reserve((size_t)(xtot * vs_));
         for (int32_t L = 1; L <= maxlevel; L++) {
-            const int32_t t0 = lev_task[L], t1 = lev_task[L + 1];
+            const int32_t t0 = lev_tt[L], t1 = lev_tt[L + 1];
             if (t1 <= t0) continue;
-            k::slab_solve(vs_, t1 - t0, d_tasks_.p + t0, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, lev_ct[L], s_);
-            const int32_t g0 = lev_tile[L], g1 = lev_tile[L + 1];
-            k::slab_gemm(vs_, g1 - g0, d_tiles_.p + g0, d_tasks_.p, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, lev_flops[L], s_);
+            k::blk_trsm(vs_, t1 - t0, d_ttiles_.p + t0, d_tasks_.p, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, d_X_.p,
+                        lev_tflops[L], s_);
+            const int32_t g0 = lev_gt[L], g1 = lev_gt[L + 1];
+            k::blk_gemm(vs_, g1 - g0, d_tiles_.p + g0, d_tasks_.p, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, d_X_.p,
+                        lev_gflops[L], s_);
         }
     }
     double* Wl = d_W_.p + Ru * vs_;

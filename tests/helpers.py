@@ -30,6 +30,12 @@ def rel_diff(A, B):
     return float(np.max(num / den)) if len(num) else 0.0
 
 
+def norm_diff(A, B):
+    """||A - B||_F / ||B||_F (entries on a common pattern)"""
+    nb = np.sqrt(float(abs(B).power(2).sum()))
+    return float(np.sqrt(float(abs(A - B).power(2).sum())) / (nb if nb > 0 else 1.0))
+
+
 def parity_matrices(small=True):
     """(name, n, Ap, Ai, Ax, relax) cases the device path is checked on"""
     out = []

@@ -1,7 +1,9 @@
 """Parity of the HIP path (through the C interface) with the oracle, on the same seeded inputs.
 
 Bit-exact: Ruiz (Dr, Dc and the scaled values), the pivot order, the supernode partition, the L and U patterns.
-To 1e-12 relative: the L and U values (the summation order differs) and the solutions; backward errors below 1e-12.
+To 1e-12 relative (normwise over each factor) the L and U values, and to 1e-10 column by column (the summation order
+differs, and a column built by cancellation carries the round-off of its larger terms); the solutions to 1e-10 of the
+oracle's; backward errors below 1e-12.
 """
 import numpy as np
 import pytest
@@ -9,11 +11,12 @@ import scipy.sparse as sp
 
 import fplus_amd
 import oracle as O
-from helpers import parity_matrices, rel_diff, same_pattern, sorted_csc
+from helpers import norm_diff, parity_matrices, rel_diff, same_pattern, sorted_csc
 
 pytestmark = pytest.mark.gpu
 
-TOL_VALUES = 1e-12
+TOL_VALUES = 1e-12       # normwise, per factor
+TOL_COLUMNS = 1e-10      # per column
 
 
 @pytest.mark.parametrize("case", parity_matrices(), ids=lambda c: c[0])
@@ -44,8 +47,12 @@ def _compare(name, n, Ap, Ai, Ax, relax, sym):
     Ud, Uo = sorted_csc(f["Up"], f["Ui"], f["Ux"]), sorted_csc(ref.lu["Up"], ref.lu["Ui"], ref.lu["Ux"])
     assert same_pattern(Ld, Lo), f"{name}: L pattern differs"
     assert same_pattern(Ud, Uo), f"{name}: U pattern differs"
+    nl, nu = norm_diff(Ld, Lo), norm_diff(Ud, Uo)
     dl, du = rel_diff(Ld, Lo), rel_diff(Ud, Uo)
-    assert dl < TOL_VALUES and du < TOL_VALUES, f"{name}: L / U values differ by {dl:.2e} / {du:.2e}"
+    msg = f"{name}: L / U differ normwise by {nl:.2e} / {nu:.2e}, per column by {dl:.2e} / {du:.2e}"
+    print(msg)
+    assert nl < TOL_VALUES and nu < TOL_VALUES, msg
+    assert dl < TOL_COLUMNS and du < TOL_COLUMNS, msg
     cplx = np.iscomplexobj(Ax)
     rng = np.random.default_rng(7)
     b = rng.standard_normal((n, 3)) + (1j * rng.standard_normal((n, 3)) if cplx else 0)
