@@ -40,6 +40,12 @@ struct GemmTile {
     int32_t task, row0, col0, pad;   // rows [row0, row0 + 64) of the block, columns [col0, col0 + 64) of the task's list
 };
 
+// one run of U entries out of the panel workspace: Ui[uoff + t] = row0 + t, Ux[uoff + t] = W[woff + t], t < len
+struct URun {
+    int64_t uoff, woff;
+    int32_t row0, len;
+};
+
 struct PanelStatus {
     int32_t err;         // 0, or 1 + the panel-local column of a singular pivot
     int32_t pad;
@@ -59,6 +65,7 @@ void extract_cols(int vs, int32_t ncols, const ExtractCol* d, const int32_t* pos
 void rowmap_set(int64_t cnt, const int32_t* rows, int32_t* rowmap, int reset, hipStream_t s);
 void scatter_cols(int vs, int32_t ncol, const int64_t* Bp, const int32_t* Bi, const double* Bx, int64_t j0,
                   const int32_t* rowmap, double* W, int64_t ldw, double* colmax, hipStream_t s);
+void u_runs(int vs, int32_t nruns, const URun* runs, const double* W, double* Ux, int32_t* Ui, hipStream_t s);
 // the segment solves of the tiles of one level (in place in W, and compact into xbuf)
 void blk_trsm(int vs, int32_t ntile, const TrsmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
               const int32_t* rowmap, double* W, int64_t ldw, double* xbuf, double flops, hipStream_t s);

@@ -346,6 +346,21 @@ __global__ __launch_bounds__(TB) void k_gemm_sub(double* __restrict__ Wd, int64_
             }
 }
 
+// one wavefront per run
+template <class T>
+__global__ void k_u_runs(int32_t nruns, const URun* __restrict__ runs, const double* __restrict__ W, double* __restrict__ Ux,
+                         int32_t* __restrict__ Ui)
+{
+    const int lane = threadIdx.x & 63;
+    const int32_t r = blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
+    if (r >= nruns) return;
+    const URun u = runs[r];
+    for (int32_t t = lane; t < u.len; t += 64) {
+        Ui[u.uoff + t] = u.row0 + t;
+        st_val<T>(Ux, u.uoff + t, ld_val<T>(W, u.woff + t));
+    }
+}
+
 template <class T>
 __global__ void k_extract_cols(const ExtractCol* __restrict__ d, const int32_t* __restrict__ pos, const double* __restrict__ Wl,
                                int64_t ldw)
@@ -373,6 +388,13 @@ void extract_cols(int vs, int32_t ncols, const ExtractCol* d, const int32_t* pos
     if (ncols <= 0) return;
     if (vs == 1) hipLaunchKernelGGL(k_extract_cols<double>, dim3(ncols), dim3(TB), 0, s, d, pos, Wl, ldw);
     else hipLaunchKernelGGL(k_extract_cols<Zd>, dim3(ncols), dim3(TB), 0, s, d, pos, Wl, ldw);
+}
+void u_runs(int vs, int32_t nruns, const URun* runs, const double* W, double* Ux, int32_t* Ui, hipStream_t s)
+{
+    if (nruns <= 0) return;
+    const unsigned g = (unsigned)((nruns + TB / 64 - 1) / (TB / 64));
+    if (vs == 1) hipLaunchKernelGGL(k_u_runs<double>, dim3(g), dim3(TB), 0, s, nruns, runs, W, Ux, Ui);
+    else hipLaunchKernelGGL(k_u_runs<Zd>, dim3(g), dim3(TB), 0, s, nruns, runs, W, Ux, Ui);
 }
 void rowmap_set(int64_t cnt, const int32_t* rows, int32_t* rowmap, int reset, hipStream_t s)
 {

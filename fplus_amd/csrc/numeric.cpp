@@ -200,10 +200,11 @@ struct fplus_numeric {
     DevBuf<int2> d_lft, d_ubt;
     std::vector<double> bytes_ub_tri;
     std::vector<double> bytes_lf_pull, bytes_lf_push, bytes_ub, bytes_utf, bytes_ltb;
-    // U row-compressed (for U x = y): row pointers, column indices, values
-    DevBuf<int64_t> d_URp;
-    DevBuf<int32_t> d_URj;
-    DevBuf<double> d_URx;
+    // U x = y beyond the blocks: a block's pieces are the runs of later columns over its rows, each covering the block's
+    // rows a .. w-1; ordered by a, so row t of the block reads the block's first pcnt[c0 + t] pieces; piece p holds
+    // U(c0 + t, pj[p]) at Ux[pu[p] + t]
+    DevBuf<int64_t> d_pb, d_pu;
+    DevBuf<int32_t> d_pj, d_pcnt;
     // solve workspace, reused across calls (solves on one numeric object are serialized)
     mutable std::mutex solve_mu;
     mutable DevBuf<double> d_w, d_contrib, d_b;
@@ -258,28 +259,82 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
     }
     // lvl is rewritten in place: blocks after b were assigned before b reads them
     order_levels(lvl, num->blk_ltb, num->lev_ltb);
-    // U x = y: a block waits for every block whose columns have U entries in its rows
-    std::fill(rowlvl.begin(), rowlvl.end(), 0);
-    for (int32_t b = nb - 1; b >= 0; b--) {
-        const BlockDesc& B = F.blocks[b];
-        int32_t L = 0;
-        for (int32_t c = 0; c < B.w; c++) L = std::max(L, rowlvl[B.c0 + c]);
-        lvl[b] = L;
-        for (int32_t c = 0; c < B.w; c++)
-            for (int64_t qq = F.Up[B.c0 + c]; qq < F.Up[B.c0 + c + 1]; qq++) {
-                const int64_t i = F.Ui[qq];
-                if (i < B.c0) rowlvl[i] = std::max(rowlvl[i], L + 1);
+    // U x = y: the pieces of every block (counted, then placed in column order, then ordered by first row), and the
+    // levels: a block waits for the blocks of its pieces' columns
+    std::vector<int64_t> pb(nb + 1, 0);
+    std::vector<int32_t> pj, pa;
+    std::vector<int64_t> pu;
+    {
+        auto for_pieces = [&](auto&& fn) {
+            for (int64_t j = 0; j < n; j++) {
+                const int32_t bj = colblk[j];
+                int64_t uo = F.Up[j];
+                for (int64_t r = F.Urp[j]; r < F.Urp[j + 1]; r++) {
+                    const int64_t r0 = F.Ur_row[r], r1 = r0 + F.Ur_len[r];
+                    for (int64_t i = r0; i < r1;) {
+                        const int32_t b = colblk[i];
+                        const BlockDesc& B = F.blocks[b];
+                        const int64_t e = std::min<int64_t>(r1, (int64_t)B.c0 + B.w);
+                        if (b != bj) {
+                            if (e != (int64_t)B.c0 + B.w) throw std::logic_error("internal: a U run ends inside a block");
+                            fn(b, j, i - B.c0, uo + (i - r0));
+                        }
+                        i = e;
+                    }
+                    uo += F.Ur_len[r];
+                }
             }
+        };
+        for_pieces([&](int32_t b, int64_t, int64_t, int64_t) { pb[b + 1]++; });
+        for (int32_t b = 0; b < nb; b++) pb[b + 1] += pb[b];
+        const int64_t np = pb[nb];
+        std::vector<int32_t> tj(np > 0 ? np : 1), ta(np > 0 ? np : 1);
+        std::vector<int64_t> tu(np > 0 ? np : 1), fill(pb.begin(), pb.end() - 1);
+        for_pieces([&](int32_t b, int64_t j, int64_t a, int64_t u) {
+            const int64_t o = fill[b]++;
+            tj[o] = (int32_t)j; ta[o] = (int32_t)a; tu[o] = u - a;
+        });
+        // per block: a counting sort by first row (stable: columns ascending within one first row), the row counts
+        pj.resize(np > 0 ? np : 1); pa.resize(np > 0 ? np : 1); pu.resize(np > 0 ? np : 1);
+        std::vector<int32_t> pcnt(n > 0 ? n : 1, 0);
+        std::vector<int64_t> h;
+        for (int32_t b = 0; b < nb; b++) {
+            const BlockDesc& B = F.blocks[b];
+            h.assign(B.w + 1, 0);
+            for (int64_t o = pb[b]; o < pb[b + 1]; o++) h[ta[o] + 1]++;
+            for (int32_t t = 0; t < B.w; t++) h[t + 1] += h[t];
+            for (int32_t t = 0; t < B.w; t++) pcnt[B.c0 + t] = (int32_t)h[t + 1];
+            for (int64_t o = pb[b]; o < pb[b + 1]; o++) {
+                const int64_t d = pb[b] + h[ta[o]]++;
+                pj[d] = tj[o]; pa[d] = ta[o]; pu[d] = tu[o];
+            }
+        }
+        auto up32v = [s](DevBuf<int32_t>& d, const std::vector<int32_t>& hh) { d.alloc(hh.size()); d.upload(hh.data(), hh.size(), s); };
+        auto up64v = [s](DevBuf<int64_t>& d, const std::vector<int64_t>& hh) { d.alloc(hh.size()); d.upload(hh.data(), hh.size(), s); };
+        up64v(num->d_pb, pb);
+        up64v(num->d_pu, pu);
+        up32v(num->d_pj, pj);
+        up32v(num->d_pcnt, pcnt);
+        dev_sync(s);
+    }
+    for (int32_t b = nb - 1; b >= 0; b--) {
+        int32_t L = 0;
+        for (int64_t o = pb[b]; o < pb[b + 1]; o++) L = std::max(L, lvl[colblk[pj[o]]] + 1);
+        lvl[b] = L;
     }
     order_levels(lvl, num->blk_ub, num->lev_ub);
-    // U^T y = b: a block waits for the blocks holding the rows of its U columns
+    // U^T y = b: a block waits for the blocks holding the rows of its U columns above it
     for (int32_t b = 0; b < nb; b++) {
         const BlockDesc& B = F.blocks[b];
         int32_t L = 0;
-        for (int32_t c = 0; c < B.w; c++)
-            for (int64_t qq = F.Up[B.c0 + c]; qq < F.Up[B.c0 + c + 1]; qq++) {
-                const int64_t i = F.Ui[qq];
-                if (i < B.c0) L = std::max(L, lvl[colblk[i]] + 1);
+        for (int64_t j = B.c0; j < (int64_t)B.c0 + B.w; j++)
+            for (int64_t r = F.Urp[j]; r < F.Urp[j + 1]; r++) {
+                const int64_t r0 = F.Ur_row[r], r1 = std::min<int64_t>(r0 + F.Ur_len[r], B.c0);
+                for (int64_t i = r0; i < r1;) {
+                    const int32_t bi = colblk[i];
+                    L = std::max(L, lvl[bi] + 1);
+                    i = (int64_t)F.blocks[bi].c0 + F.blocks[bi].w;
+                }
             }
         lvl[b] = L;
     }
@@ -329,54 +384,7 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         num->d_lft.alloc(tiles.size() > 0 ? tiles.size() : 1);
         num->d_lft.upload(tiles.data(), tiles.size(), s);
     }
-    // U row-compressed: a counting sort of U's entries by row, columns ascending within a row; threads take column
-    // ranges and place their entries after the earlier ranges' (the same arrays at any thread count)
     {
-        const int64_t nnzU = (int64_t)F.Ui.size();
-        std::vector<int64_t> URp(n + 1, 0), map(nnzU > 0 ? nnzU : 1);
-        std::vector<int32_t> URj(nnzU > 0 ? nnzU : 1);
-        const int T = (int)std::max<int64_t>(1, std::min<int64_t>({16, (int64_t)std::thread::hardware_concurrency(), nnzU / 1000000 + 1}));
-        std::vector<int64_t> jb(T + 1);
-        for (int t = 0; t <= T; t++) jb[t] = n * t / T;
-        std::vector<std::vector<int64_t>> cnt(T, std::vector<int64_t>(n > 0 ? n : 1, 0));
-        {
-            std::vector<std::thread> th;
-            for (int t = 0; t < T; t++)
-                th.emplace_back([&, t]() {
-                    std::vector<int64_t>& c = cnt[t];
-                    for (int64_t q = F.Up[jb[t]]; q < F.Up[jb[t + 1]]; q++) c[F.Ui[q]]++;
-                });
-            for (auto& x : th) x.join();
-        }
-        for (int64_t i = 0; i < n; i++) {   // URp, and every thread's first position in every row
-            int64_t o = URp[i];
-            for (int t = 0; t < T; t++) { const int64_t c = cnt[t][i]; cnt[t][i] = o; o += c; }
-            URp[i + 1] = o;
-        }
-        {
-            std::vector<std::thread> th;
-            for (int t = 0; t < T; t++)
-                th.emplace_back([&, t]() {
-                    std::vector<int64_t>& pos = cnt[t];
-                    for (int64_t j = jb[t]; j < jb[t + 1]; j++)
-                        for (int64_t q = F.Up[j]; q < F.Up[j + 1]; q++) {
-                            const int64_t o = pos[F.Ui[q]]++;
-                            URj[o] = (int32_t)j;
-                            map[o] = q;
-                        }
-                });
-            for (auto& x : th) x.join();
-        }
-        cnt.clear();
-        num->d_URp.alloc(n + 1);
-        num->d_URp.upload(URp.data(), n + 1, s);
-        num->d_URj.alloc(URj.size());
-        num->d_URj.upload(URj.data(), URj.size(), s);
-        DevBuf<int64_t> d_map(map.size());
-        d_map.upload(map.data(), map.size(), s);
-        num->d_URx.alloc((size_t)(nnzU > 0 ? nnzU : 1) * F.vs);
-        k::gather_vals(F.vs, nnzU, d_map.p, F.d_Ux.p, num->d_URx.p, s);
-        dev_sync(s);
         const size_t nub = num->lev_ub.size() - 1;
         num->bytes_ub.assign(nub, 0.0);
         num->bytes_ub_tri.assign(nub, 0.0);
@@ -386,10 +394,12 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
             for (int32_t k2 = num->lev_ub[L]; k2 < num->lev_ub[L + 1]; k2++) {
                 const int32_t b = num->blk_ub[k2];
                 const BlockDesc& B = F.blocks[b];
-                const double ent = (double)(URp[B.c0 + B.w] - URp[B.c0]);
-                num->bytes_ub[L] += ent * (vb + 4.0) + 2.0 * B.w * R0 * vb;
+                double ent = 0;
+                for (int64_t o = pb[b]; o < pb[b + 1]; o++) ent += (double)(B.w - pa[o]);
+                num->bytes_ub[L] += ent * vb + (double)(pb[b + 1] - pb[b]) * (12.0 + R0 * vb) + 2.0 * B.w * R0 * vb;
                 num->bytes_ub_tri[L] += (double)B.w * (B.w + 1) / 2 * vb + 2.0 * B.w * R0 * vb;
-                for (int32_t t0 = 0; t0 < B.w; t0 += 4) ubt.push_back(make_int2(b, t0));
+                if (pb[b + 1] > pb[b])
+                    for (int32_t t0 = 0; t0 < B.w; t0 += 4) ubt.push_back(make_int2(b, t0));
             }
             num->lev_ubt[L + 1] = (int32_t)ubt.size();
         }
@@ -445,8 +455,8 @@ void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, hi
                          num->d_contrib.p, R, num->bytes_lf_push[L], s);
         }
         for (size_t L = 0; L + 1 < num->lev_ub.size(); L++) {
-            k::ub_dots(vs, num->lev_ubt[L + 1] - num->lev_ubt[L], num->d_ubt.p + num->lev_ubt[L], blocks, num->d_URp.p, num->d_URj.p,
-                       num->d_URx.p, w, R, num->bytes_ub[L], s);
+            k::ub_dots(vs, num->lev_ubt[L + 1] - num->lev_ubt[L], num->d_ubt.p + num->lev_ubt[L], blocks, num->d_pb.p, num->d_pcnt.p,
+                       num->d_pj.p, num->d_pu.p, F.d_Ux.p, w, R, num->bytes_ub[L], s);
             k::ub_tri(vs, num->lev_ub[L + 1] - num->lev_ub[L], num->d_blk_ub.p + num->lev_ub[L], blocks, w, R, num->bytes_ub_tri[L], s);
         }
     } else {
@@ -497,13 +507,15 @@ void export_factors(const DevFactor& F, int is_complex, fplus_factors* f)
     f->Li = (int64_t*)malloc((size_t)(nnzL > 0 ? nnzL : 1) * sizeof(int64_t));
     f->Lx = malloc((size_t)(nnzL > 0 ? nnzL : 1) * sz);
     f->Up = (int64_t*)malloc((size_t)(n + 1) * sizeof(int64_t));
-    f->Ui = (int64_t*)malloc((size_t)(F.Ui.size() > 0 ? F.Ui.size() : 1) * sizeof(int64_t));
+    f->Ui = (int64_t*)malloc((size_t)(F.nnzU > 0 ? F.nnzU : 1) * sizeof(int64_t));
     f->Ux = malloc((size_t)(F.nnzU > 0 ? F.nnzU : 1) * sz);
     f->p = (int64_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
     f->q = (int64_t*)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
     if (!f->Lp || !f->Li || !f->Lx || !f->Up || !f->Ui || !f->Ux || !f->p || !f->q) { fplus_factors_free(f); throw std::bad_alloc(); }
     std::memcpy(f->Up, F.Up.data(), (n + 1) * sizeof(int64_t));
-    for (size_t q = 0; q < F.Ui.size(); q++) f->Ui[q] = F.Ui[q];
+    for (int64_t j = 0, o = 0; j < n; j++)
+        for (int64_t r = F.Urp[j]; r < F.Urp[j + 1]; r++)
+            for (int32_t t = 0; t < F.Ur_len[r]; t++) f->Ui[o++] = F.Ur_row[r] + t;
     F.d_Ux.download((double*)f->Ux, (size_t)F.nnzU * vs, s);
     for (int64_t i = 0; i < n; i++) { f->p[i] = F.p[i]; f->q[i] = i; }
     // block values to the host, block by block

@@ -164,9 +164,8 @@ private:
     std::vector<int32_t> hrowpos_;    // workspace position of a row, -1 outside the panel
     std::vector<int32_t> lposmap_;    // final lower-section position of a row after the panel LU
     std::vector<int64_t> ustamp_, uidx_;
-    std::vector<int32_t> Ui_;
-    std::vector<int64_t> Up_;
-    std::vector<int64_t> umap_;       // workspace offsets of the panel's U values
+    std::vector<int64_t> Up_, Urp_;
+    std::vector<int32_t> Ur_row_, Ur_len_;   // U as runs of consecutive rows
     // device workspace
     DevBuf<double> d_W_;
     DevBuf<int32_t> d_rowmap_, d_univ_, d_cols_, d_piv_, d_pos_;
@@ -176,7 +175,7 @@ private:
     DevBuf<GemmTile> d_tiles_;
     DevBuf<double> d_X_;
     DevBuf<PanelStatus> d_status_;
-    DevBuf<int64_t> d_umap_;
+    DevBuf<URun> d_runs_;
     DevBuf<ExtractCol> d_ex_;
     Arena<double> vals_{nullptr, 0, 0, 0};
     Arena<int32_t> rows_{nullptr, 0, 0, 0};
@@ -259,17 +258,26 @@ int Driver::panel(int64_t j0, int64_t j1)
         }
         for (int64_t t = top; t < n; t++) { uidx_[topo_[t]] = (int64_t)un.size(); ustamp_[topo_[t]] = us; un.push_back(topo_[t]); }
     }
-    // ---- the workspace rows: U rows (pivotal before j0) first, then the rows not yet pivotal
+    // ---- the workspace rows: the U rows (pivotal before j0) first, each union supernode's reached rows as one run (the
+    // union of its columns' segments [kf, we) is the longest one), then the rows not yet pivotal
     std::vector<int32_t> univ;
-    for (int64_t ci = 0; ci < np; ci++)
-        for (int64_t q = ext_off[ci]; q < ext_off[ci + 1]; q++) {
-            const HSn& S = sn_[ext_s[q]];
-            const int64_t we = ext_s[q] == open0 ? we_open : S.w;
-            for (int64_t cc = ext_kf[q] - S.c0; cc < we; cc++) {
-                const int64_t row = S.ri[cc];
-                if (hrowpos_[row] < 0) { hrowpos_[row] = (int32_t)univ.size(); univ.push_back((int32_t)row); }
-            }
+    std::vector<int64_t> ukmin(un.size(), INT64_MAX), ubase(un.size(), 0);
+    for (size_t q = 0; q < ext_s.size(); q++) {
+        const int64_t u = uidx_[ext_s[q]];
+        ukmin[u] = std::min(ukmin[u], ext_kf[q]);
+    }
+    for (size_t u = 0; u < un.size(); u++) {
+        ubase[u] = (int64_t)univ.size();
+        if (ukmin[u] == INT64_MAX) continue;
+        const HSn& S = sn_[un[u]];
+        const int64_t we = un[u] == open0 ? we_open : S.w;
+        for (int64_t cc = ukmin[u] - S.c0; cc < we; cc++) {
+            const int64_t row = S.ri[cc];
+            if (hrowpos_[row] >= 0) { errf(err_, errlen_, "internal: a U row in two segments"); return FPLUS_ERR_INTERNAL; }
+            hrowpos_[row] = (int32_t)univ.size();
+            univ.push_back((int32_t)row);
         }
+    }
     const int64_t Ru = (int64_t)univ.size();
     for (int64_t row : npl)
         if (hrowpos_[row] < 0) { hrowpos_[row] = (int32_t)univ.size(); univ.push_back((int32_t)row); }
@@ -433,8 +441,8 @@ int Driver::panel(int64_t j0, int64_t j1)
     for (int64_t ci = 0; ci < np; ci++) std::swap(lrows[ci], lrows[piv[ci]]);
     for (int64_t i = 0; i < Rl; i++) lposmap_[lrows[i]] = (int32_t)i;
     // ---- phase 3 on the host, with the device's pivots (src/factor_sn_panel.inc:162-390)
-    const size_t u_begin = Ui_.size();
-    umap_.clear();
+    const int64_t u_begin = Up_[j0];
+    std::vector<URun> pruns;
     std::vector<std::pair<int64_t, int64_t>> touched;   // (supernode, first column in this panel)
     for (int64_t ci = 0; ci < np; ci++) {
         const int64_t j = j0 + ci, stamp2 = n + 1 + j;
@@ -453,26 +461,29 @@ int Driver::panel(int64_t j0, int64_t j1)
             top = dfs(s, k, top, stamp2, open_, &ncand);
         }
         const int64_t pivrow = lrows[ci];
-        // U(:, j): pre-panel segments, within-panel segments, the diagonal; the value of each entry from the workspace
+        // U(:, j): pre-panel segments, within-panel segments, the diagonal -- each a run of consecutive rows whose values
+        // are consecutive in the workspace column
+        int64_t uo = Up_[j];
+        auto put_run = [&](int64_t row0, int64_t len, int64_t wrow) {
+            if (len <= 0) return;
+            Ur_row_.push_back((int32_t)row0);
+            Ur_len_.push_back((int32_t)len);
+            pruns.push_back(URun{uo, wrow + ci * ldw, (int32_t)row0, (int32_t)len});
+            uo += len;
+        };
         for (int64_t q = ext_off[ci]; q < ext_off[ci + 1]; q++) {
             const HSn& S = sn_[ext_s[q]];
-            const int64_t we = ext_s[q] == open0 ? we_open : S.w;
-            for (int64_t cc = ext_kf[q] - S.c0; cc < we; cc++) {
-                Ui_.push_back((int32_t)(S.c0 + cc));
-                umap_.push_back((int64_t)hrowpos_[S.ri[cc]] + ci * ldw);
-            }
+            const int64_t we = ext_s[q] == open0 ? we_open : S.w, u = uidx_[ext_s[q]];
+            put_run(ext_kf[q], S.c0 + we - ext_kf[q], ubase[u] + (ext_kf[q] - ukmin[u]));
         }
         for (int64_t t = top; t < n; t++) {
             const HSn& S = sn_[topo_[t]];
-            for (int64_t cc = seg0_[topo_[t]] - S.c0; cc < S.w; cc++) {
-                const int64_t col = S.c0 + cc;
-                Ui_.push_back((int32_t)col);
-                umap_.push_back(Ru + (col - j0) + ci * ldw);
-            }
+            const int64_t k0 = seg0_[topo_[t]];
+            put_run(k0, S.c0 + S.w - k0, Ru + (k0 - j0));
         }
-        Ui_.push_back((int32_t)j);
-        umap_.push_back(Ru + ci + ci * ldw);
-        Up_[j + 1] = (int64_t)Ui_.size();
+        put_run(j, 1, Ru + ci);
+        Up_[j + 1] = uo;
+        Urp_[j + 1] = (int64_t)Ur_row_.size();
         // L(:, j): join the open supernode (exactly or relaxed), else open a new one
         std::vector<int64_t>& npv = miss_;   // reused: the candidates still non-pivotal, then the missing rows
         std::vector<int64_t> npl_l;
@@ -552,26 +563,25 @@ int Driver::panel(int64_t j0, int64_t j1)
         k::extract_cols(vs_, (int32_t)ex.size(), d_ex_.p, d_pos_.p, Wl, ldw, s_);
     }
     {
-        const int64_t cnt = (int64_t)umap_.size();
-        if (F_.nnzU + cnt > (int64_t)(F_.d_Ux.n / vs_)) {
-            DevBuf<double> nb((size_t)std::max<int64_t>(2 * (F_.nnzU + cnt), 1024) * vs_);
-            if (F_.nnzU > 0)
-                hip_check(hipMemcpyAsync(nb.p, F_.d_Ux.p, F_.nnzU * vs_ * sizeof(double), hipMemcpyDeviceToDevice, s_), "grow U");
+        const int64_t cnt = Up_[j1] - u_begin;
+        if (u_begin + cnt > (int64_t)(F_.d_Ux.n / vs_)) {
+            DevBuf<double> nb((size_t)std::max<int64_t>(2 * (u_begin + cnt), 1024) * vs_);
+            if (u_begin > 0)
+                hip_check(hipMemcpyAsync(nb.p, F_.d_Ux.p, u_begin * vs_ * sizeof(double), hipMemcpyDeviceToDevice, s_), "grow U");
             dev_sync(s_);
             F_.d_Ux = std::move(nb);
         }
-        if (F_.nnzU + cnt > (int64_t)F_.d_Ui.n) {
-            DevBuf<int32_t> nb((size_t)std::max<int64_t>(2 * (F_.nnzU + cnt), 1024));
-            if (F_.nnzU > 0)
-                hip_check(hipMemcpyAsync(nb.p, F_.d_Ui.p, F_.nnzU * sizeof(int32_t), hipMemcpyDeviceToDevice, s_), "grow U rows");
+        if (u_begin + cnt > (int64_t)F_.d_Ui.n) {
+            DevBuf<int32_t> nb((size_t)std::max<int64_t>(2 * (u_begin + cnt), 1024));
+            if (u_begin > 0)
+                hip_check(hipMemcpyAsync(nb.p, F_.d_Ui.p, u_begin * sizeof(int32_t), hipMemcpyDeviceToDevice, s_), "grow U rows");
             dev_sync(s_);
             F_.d_Ui = std::move(nb);
         }
-        F_.d_Ui.upload(Ui_.data() + u_begin, cnt, s_, F_.nnzU);
-        d_umap_.reserve(cnt);
-        d_umap_.upload(umap_.data(), cnt, s_);
-        k::gather_vals(vs_, cnt, d_umap_.p, d_W_.p, F_.d_Ux.p + F_.nnzU * vs_, s_);
-        F_.nnzU += cnt;
+        d_runs_.reserve(pruns.size());
+        d_runs_.upload(pruns.data(), pruns.size(), s_);
+        k::u_runs(vs_, (int32_t)pruns.size(), d_runs_.p, d_W_.p, F_.d_Ux.p, F_.d_Ui.p, s_);
+        F_.nnzU = u_begin + cnt;
     }
     k::rowmap_set((int64_t)univ.size(), d_univ_.p, d_rowmap_.p, 1, s_);
     // the uploads above read host buffers that go out of scope: finish them before returning
@@ -603,7 +613,9 @@ int Driver::run()
     hrowpos_.assign(nn, -1); lposmap_.assign(nn, -1);
     sn_.reserve(nn);
     Up_.assign(n + 1, 0);
-    Ui_.reserve((size_t)(4 * nnzA + n + 16));
+    Urp_.assign(n + 1, 0);
+    Ur_row_.reserve((size_t)(nnzA + n + 16));
+    Ur_len_.reserve((size_t)(nnzA + n + 16));
     d_rowmap_.alloc(nn);
     k::fill_i32(d_rowmap_.p, n, -1, s_);
     d_status_.alloc(1);
@@ -631,10 +643,14 @@ int Driver::run()
     F_.p = p_;
     F_.pinv = pinv_;
     F_.Up = std::move(Up_);
-    F_.Ui = std::move(Ui_);
-    // factor flops c * sum_k l_k u_k (l_k below the diagonal of L(:, k), u_k right of the diagonal in U(k, :))
-    std::vector<int64_t> urow(nn, 0);
-    for (int64_t q = 0; q < (int64_t)F_.Ui.size(); q++) urow[F_.Ui[q]]++;
+    F_.Urp = std::move(Urp_);
+    F_.Ur_row = std::move(Ur_row_);
+    F_.Ur_len = std::move(Ur_len_);
+    // factor flops c * sum_k l_k u_k (l_k below the diagonal of L(:, k), u_k right of the diagonal in U(k, :)); the
+    // row counts of U from its runs by differences
+    std::vector<int64_t> urow(nn + 1, 0);
+    for (size_t r = 0; r < F_.Ur_row.size(); r++) { urow[F_.Ur_row[r]]++; urow[F_.Ur_row[r] + F_.Ur_len[r]]--; }
+    for (int64_t i = 1; i < n; i++) urow[i] += urow[i - 1];
     double fl = 0;
     for (int64_t s = 0; s < F_.nsn; s++) {
         const int64_t c0 = F_.sn_first[s], c1 = F_.sn_first[s + 1], nr = (int64_t)F_.sn_rows[s].size();

@@ -22,13 +22,16 @@ struct DevFactor {
     std::vector<BlockDesc> blocks;       // host mirror of the block table (device pointers)
     std::vector<std::vector<int32_t>> block_rows;   // host copy of every block's rows
     std::vector<int64_t> Up;             // U in CSC, column j: earlier supernodes' segments, the diagonal last
-    std::vector<int32_t> Ui;             // (row indices in pivot order)
+    // U's row indices as runs of consecutive rows (pivot order): column j holds runs Urp[j] .. Urp[j+1]-1, run r the
+    // rows Ur_row[r] .. Ur_row[r] + Ur_len[r] - 1, in storage order (a segment is a run: a supernode's pivot rows)
+    std::vector<int64_t> Urp;
+    std::vector<int32_t> Ur_row, Ur_len;
     // device
     DevBuf<BlockDesc> d_blocks;
     std::vector<DevBuf<double>> val_chunks;
     std::vector<DevBuf<int32_t>> row_chunks;
     DevBuf<double> d_Ux;                 // U values, in Ui order
-    DevBuf<int32_t> d_Ui;                // U row indices on the device, filled panel by panel
+    DevBuf<int32_t> d_Ui;                // U row indices on the device, expanded from the runs panel by panel
     int64_t nnzU = 0;
     // statistics
     double flops = 0, gemm_flops = 0;

@@ -15,8 +15,10 @@ void lfwd_pull(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks
                const double* contrib, double* x, int32_t R, double bytes, hipStream_t s);
 void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* slot_off, const double* x,
                double* contrib, int32_t R, double bytes, hipStream_t s);
-void ub_dots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* URp, const int32_t* URj,
-             const double* URx, double* x, int32_t R, double bytes, hipStream_t s);
+// tiles (block, first row) of 4 rows: row c0 + t of a block subtracts the dot product over the block's first pcnt[c0 + t]
+// pieces p (from pb[block]): U(c0 + t, pj[p]) = Ux[pu[p] + t] times x[pj[p]]
+void ub_dots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* pb, const int32_t* pcnt,
+             const int32_t* pj, const int64_t* pu, const double* Ux, double* x, int32_t R, double bytes, hipStream_t s);
 void ub_tri(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, double* x, int32_t R, double bytes, hipStream_t s);
 void utfwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
            const double* Ux, double* x, int32_t R, int conj, double bytes, hipStream_t s);

@@ -1,14 +1,15 @@
 // solve_dev.hip -- the triangular solves on the device factors (gfx950).
 //
-// L is held as supernode blocks (factor_dev.h), U in CSC (rows in pivot order, the diagonal last in every column) and,
-// for the solves with A, also row-compressed.  The right-hand sides live row-major during the sweeps, x[i * R + r] for
+// L is held as supernode blocks (factor_dev.h), U in CSC (rows in pivot order, the diagonal last in every column); a
+// U column is a list of runs of consecutive rows, which the solves with A read per block.  The right-hand sides live row-major during the sweeps, x[i * R + r] for
 // R right-hand sides, so one entry of a factor meets all R values of a row in one contiguous read.  A sweep walks the
 // blocks level by level (a level's blocks are independent), one launch per level, and every sweep pulls: each value is
 // written by one thread, no atomics, so the results are the same on every run.
 //   lfwd_pull   L y = b   a block's pivot rows gather the contributions the blocks before them left in their slots,
 //                         then the block's unit-lower triangle;
 //   lfwd_push   the block's rows below: contribution[slot + r] = L(r, block) y(block) (the slot is the row's own);
-//   ubwd        U x = y   a block's pivot rows: dot products along their U rows beyond the block (row-compressed U),
+//   ubwd        U x = y   a block's pivot rows: dot products along their U rows beyond the block (the runs of later
+//                         U columns over the block's rows, ordered by first row),
 //                         then the block's upper triangle from its columns, last to first;
 //   utfwd       U^T y = b (U^H): dot products down U's columns, the block's own rows in order;
 //   ltbwd       L^T x = y (L^H): dot products down the block's columns, the triangle last to first.
@@ -100,28 +101,28 @@ __global__ __launch_bounds__(TB) void k_lfwd_push(const int2* __restrict__ tiles
         if (r < R) contrib[slot * R + r] = acc[r];
 }
 
-// U x = y, part 1: the dot products of a block's pivot rows along their U rows beyond the block (row-compressed U),
+// U x = y, part 1: the dot products of a block's pivot rows along their U rows beyond the block (its pieces),
 // one wavefront per row and every right-hand side at once, four rows per workgroup; x_i -= sum in place
 template <class T>
 __global__ __launch_bounds__(TB) void k_ub_dots(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
-                                                 const int64_t* __restrict__ URp, const int32_t* __restrict__ URj,
-                                                 const double* __restrict__ URxd, double* __restrict__ xd, int32_t R)
+                                                 const int64_t* __restrict__ pb, const int32_t* __restrict__ pcnt,
+                                                 const int32_t* __restrict__ pj, const int64_t* __restrict__ pu,
+                                                 const double* __restrict__ Uxd, double* __restrict__ xd, int32_t R)
 {
     const int2 tl = tiles[blockIdx.x];
     const BlockDesc B = blocks[tl.x];
-    const T* URx = reinterpret_cast<const T*>(URxd);
+    const T* Ux = reinterpret_cast<const T*>(Uxd);
     T* x = reinterpret_cast<T*>(xd);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int t = tl.y + wid;
     if (t >= B.w) return;
-    const int64_t i = (int64_t)B.c0 + t, c1 = (int64_t)B.c0 + B.w;
+    const int64_t i = (int64_t)B.c0 + t, p0 = pb[tl.x], p1 = p0 + pcnt[i];
     T acc[RM<T>::v];
 #pragma unroll
     for (int r = 0; r < RM<T>::v; r++) acc[r] = Sc<T>::zero();
-    for (int64_t q = URp[i] + lane; q < URp[i + 1]; q += 64) {
-        const int64_t j = URj[q];
-        if (j < c1) continue;
-        const T u = URx[q];
+    for (int64_t p = p0 + lane; p < p1; p += 64) {
+        const int64_t j = pj[p];
+        const T u = Ux[pu[p] + t];
 #pragma unroll
         for (int r = 0; r < RM<T>::v; r++)
             if (r < R) acc[r] = Sc<T>::fma_(acc[r], u, x[j * R + r]);
@@ -129,8 +130,8 @@ __global__ __launch_bounds__(TB) void k_ub_dots(const int2* __restrict__ tiles, 
 #pragma unroll
     for (int r = 0; r < RM<T>::v; r++) {
         if (r >= R) break;
-        const T s = wave_sum(acc[r]);
-        if (lane == 0) x[i * R + r] = Sc<T>::sub(x[i * R + r], s);
+        const T sm = wave_sum(acc[r]);
+        if (lane == 0) x[i * R + r] = Sc<T>::sub(x[i * R + r], sm);
     }
 }
 
@@ -311,13 +312,13 @@ void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks
     if (vs == 1) hipLaunchKernelGGL(k_lfwd_push<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, slot_off, x, contrib, R);
     else hipLaunchKernelGGL(k_lfwd_push<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, slot_off, x, contrib, R);
 }
-void ub_dots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* URp, const int32_t* URj,
-             const double* URx, double* x, int32_t R, double bytes, hipStream_t s)
+void ub_dots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* pb, const int32_t* pcnt,
+             const int32_t* pj, const int64_t* pu, const double* Ux, double* x, int32_t R, double bytes, hipStream_t s)
 {
     if (ntile <= 0) return;
     ProfScope ps("ub_dots", bytes, s);
-    if (vs == 1) hipLaunchKernelGGL(k_ub_dots<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, URp, URj, URx, x, R);
-    else hipLaunchKernelGGL(k_ub_dots<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, URp, URj, URx, x, R);
+    if (vs == 1) hipLaunchKernelGGL(k_ub_dots<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, pb, pcnt, pj, pu, Ux, x, R);
+    else hipLaunchKernelGGL(k_ub_dots<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, pb, pcnt, pj, pu, Ux, x, R);
 }
 void ub_tri(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, double* x, int32_t R, double bytes, hipStream_t s)
 {
