@@ -72,9 +72,12 @@ void blk_trsm(int vs, int32_t ntile, const TrsmTile* tiles, const UpdTask* tasks
 // the rectangles below, subtracted from W through rowmap (atomically: tasks of one level may share target rows)
 void blk_gemm(int vs, int32_t ntile, const GemmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
               const int32_t* rowmap, double* W, int64_t ldw, const double* xbuf, double flops, hipStream_t s);
-// dense partial-pivot LU of the panel's lower section (nr x P, column-major, ld), sub-panel [a, a + nb)
+// dense partial-pivot LU of the panel's lower section (nr x P, column-major, ld), sub-panel [a, a + nb): a cooperative
+// launch over the rows (coop: getf2_coop_bytes(vs) of scratch, counter: one int), one workgroup for very tall panels
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
-           double pivot_rel, int32_t* piv, PanelStatus* st, hipStream_t s);
+           double pivot_rel, int32_t* piv, PanelStatus* st, void* coop, int32_t* counter, hipStream_t s);
+size_t getf2_coop_bytes(int vs);
+int getf2_coop_max_wg();
 void trsm_sub(int vs, double* Wl, int64_t ld, int32_t P, int32_t a, int32_t nb, const PanelStatus* st, hipStream_t s);
 void gemm_sub(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const PanelStatus* st,
               hipStream_t s);

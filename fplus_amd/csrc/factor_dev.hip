@@ -63,14 +63,15 @@ constexpr int PNB = FPA_PANEL_NB;  // sub-panel width of the dense panel LU
 
 // the unit-lower solve of a block's segment [k0, w) for up to TCols panel columns: thread t owns segment row t (the
 // segment is at most 256 rows: a block is at most one panel wide), the rows below the current one subtract its
-// multiple; the result goes back to W and, compact, to the task's slice of xbuf
+// multiple.  The factor's values are read PF steps ahead (they come from HBM; a step is short).  The result goes back
+// to W and, compact, to the task's slice of xbuf.
 template <class T>
 __global__ __launch_bounds__(TB) void k_blk_trsm(const TrsmTile* __restrict__ tiles, const UpdTask* __restrict__ tasks,
                                                   const int32_t* __restrict__ cols, const BlockDesc* __restrict__ blocks,
                                                   const int32_t* __restrict__ rowmap, double* __restrict__ W, int64_t ldw,
                                                   double* __restrict__ xbuf)
 {
-    constexpr int CT = TCols<T>::v;
+    constexpr int CT = TCols<T>::v, PF = 8;
     __shared__ T xs[2][CT];
     __shared__ int32_t cidx[CT];
     const TrsmTile tt = tiles[blockIdx.x];
@@ -86,19 +87,31 @@ __global__ __launch_bounds__(TB) void k_blk_trsm(const TrsmTile* __restrict__ ti
 #pragma unroll
     for (int c = 0; c < CT; c++) x[c] = (own && c < nc) ? ld_val<T>(W, pos + (int64_t)cidx[c] * ldw) : Sc<T>::zero();
     const T* Vc = V + (int64_t)tk.k0 * B.nrows + tk.k0 + t;   // column k0 of the segment, row t
-    T lnext = (own && t > 0) ? Vc[0] : Sc<T>::zero();
-    for (int32_t k = 0; k < m; k++) {
-        const T lcur = lnext;
-        if (own && t > k + 1) lnext = Vc[(int64_t)(k + 1) * B.nrows];
-        if (t == k) {
+    T lcur[PF], lnxt[PF];
 #pragma unroll
-            for (int c = 0; c < CT; c++) xs[k & 1][c] = x[c];
-        }
-        __syncthreads();
-        if (own && t > k) {
+    for (int u = 0; u < PF; u++) lcur[u] = (own && t > u && u < m) ? Vc[(int64_t)u * B.nrows] : Sc<T>::zero();
+    for (int32_t kg = 0; kg < m; kg += PF) {
 #pragma unroll
-            for (int c = 0; c < CT; c++) x[c] = Sc<T>::fms(x[c], lcur, xs[k & 1][c]);
+        for (int u = 0; u < PF; u++) {
+            const int32_t k = kg + PF + u;
+            lnxt[u] = (own && t > k && k < m) ? Vc[(int64_t)k * B.nrows] : Sc<T>::zero();
         }
+#pragma unroll
+        for (int u = 0; u < PF; u++) {
+            const int32_t k = kg + u;
+            if (k >= m) break;
+            if (t == k) {
+#pragma unroll
+                for (int c = 0; c < CT; c++) xs[k & 1][c] = x[c];
+            }
+            __syncthreads();
+            if (own && t > k) {
+#pragma unroll
+                for (int c = 0; c < CT; c++) x[c] = Sc<T>::fms(x[c], lcur[u], xs[k & 1][c]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < PF; u++) lcur[u] = lnxt[u];
     }
     if (!own) return;
     T* X = reinterpret_cast<T*>(xbuf) + tk.xoff;
@@ -116,7 +129,8 @@ typedef double dx4 __attribute__((ext_vector_type(4)));
 __device__ inline dx4 mfma16(double a, double b, dx4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
 // the rectangle below a block's segment for a 64 x 64 tile: W[rows] -= L[rows, seg] X[seg, cols] through rowmap, with
-// the product on the matrix cores (four real products per complex one), k in chunks of 16 staged in LDS
+// the product on the matrix cores (four real products per complex one), k in chunks of 16 staged in LDS, the next
+// chunk loaded while the current one multiplies
 template <class T>
 __global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ tiles, const UpdTask* __restrict__ tasks,
                                                   const int32_t* __restrict__ cols, const BlockDesc* __restrict__ blocks,
@@ -124,9 +138,9 @@ __global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ ti
                                                   const double* __restrict__ xbuf)
 {
     constexpr bool CPLX = Sc<T>::vs == 2;
-    constexpr int KC = 16, TM = 64, TN = 64;
-    __shared__ T As[KC][TM];     // A[k][row]
-    __shared__ T Bs[KC][TN];     // B[k][col]
+    constexpr int KC = 16, TM = 64, TN = 64, LPT = KC * TM / TB;   // elements of A (and of B) each thread stages per chunk
+    __shared__ T As[2][KC][TM];     // A[k][row], two chunks in flight
+    __shared__ T Bs[2][KC][TN];     // B[k][col]
     __shared__ int32_t rpos[TM];
     __shared__ int32_t cidx[TN];
     const GemmTile tl = tiles[blockIdx.x];
@@ -148,26 +162,41 @@ __global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ ti
             accr[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
             acci[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
         }
-    for (int32_t kc = 0; kc < m; kc += KC) {
+    // staging: thread element e = tid + q * TB; A: row e % TM, k e / TM (rows contiguous); B: k e % KC, column e / KC
+    T ra[LPT], rb[LPT];
+    auto fetch = [&](int32_t kc) {
         const int32_t kn = min(KC, m - kc);
-        // A: column k of the segment is contiguous in the block (rows), B: column c of X is contiguous in k
-        for (int i = tid; i < KC * TM; i += TB) {
-            const int r = i % TM, kk = i / TM;
-            As[kk][r] = (r < nr && kk < kn) ? V[(int64_t)(tk.k0 + kc + kk) * B.nrows + tl.row0 + r] : Sc<T>::zero();
+#pragma unroll
+        for (int q = 0; q < LPT; q++) {
+            const int e = tid + q * TB;
+            const int r = e % TM, ka = e / TM, kb = e % KC, c = e / KC;
+            ra[q] = (r < nr && ka < kn) ? V[(int64_t)(tk.k0 + kc + ka) * B.nrows + tl.row0 + r] : Sc<T>::zero();
+            rb[q] = (c < nc && kb < kn) ? X[(kc + kb) + (int64_t)(tl.col0 + c) * m] : Sc<T>::zero();
         }
-        for (int i = tid; i < KC * TN; i += TB) {
-            const int kk = i % KC, c = i / KC;
-            Bs[kk][c] = (c < nc && kk < kn) ? X[(kc + kk) + (int64_t)(tl.col0 + c) * m] : Sc<T>::zero();
+    };
+    auto stage = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < LPT; q++) {
+            const int e = tid + q * TB;
+            As[buf][e / TM][e % TM] = ra[q];
+            Bs[buf][e % KC][e / KC] = rb[q];
         }
-        __syncthreads();
+    };
+    fetch(0);
+    stage(0);
+    __syncthreads();
+    int buf = 0;
+    for (int32_t kc = 0; kc < m; kc += KC) {
+        const bool more = kc + KC < m;
+        if (more) fetch(kc + KC);
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 4) {
             const int kk = ks + (lane >> 4);
             T a[2], b[2];
 #pragma unroll
-            for (int i = 0; i < 2; i++) a[i] = As[kk][wr + i * 16 + (lane & 15)];
+            for (int i = 0; i < 2; i++) a[i] = As[buf][kk][wr + i * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < 2; j++) b[j] = Bs[kk][wc + j * 16 + (lane & 15)];
+            for (int j = 0; j < 2; j++) b[j] = Bs[buf][kk][wc + j * 16 + (lane & 15)];
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -182,7 +211,9 @@ __global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ ti
                     }
                 }
         }
+        if (more) stage(buf ^ 1);
         __syncthreads();
+        buf ^= 1;
     }
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -269,6 +300,157 @@ __global__ __launch_bounds__(LU_TB) void k_getf2(double* __restrict__ Wd, int64_
             for (int32_t c = j + 1; c < jend; c++) Wl[rr + c * ld] = Sc<T>::fms(Wl[rr + c * ld], l, Wl[j + c * ld]);
         }
         __syncthreads();
+    }
+}
+
+// The same sub-panel LU on many workgroups at once (a cooperative launch: every workgroup is resident).  Workgroup g
+// holds the sub-panel's nb columns of the positions a + 256 g + t in registers, one row per thread; per column, every
+// workgroup posts its best candidate (modulus, row, the row's values) and the owner of position j posts row j, then
+// all meet at a barrier (a monotonic counter), pick the same pivot (largest modulus, lowest row on ties) and update
+// their rows.  Slots alternate between two buffers by column parity: a workgroup can be at most one barrier ahead.
+template <class T> struct Cand {
+    double val;
+    int32_t row, pad;
+    T v[PNB];
+};
+constexpr long COOP_SPIN_LIMIT = 1L << 24;   // a barrier not met after this many polls ends the kernel with an error
+
+template <class T>
+__global__ __launch_bounds__(TB) void k_getf2_coop(double* __restrict__ Wd, int64_t ld, int32_t nr, int32_t P, int32_t a,
+                                                    int32_t nb, const double* __restrict__ colmax, double pivot_rel,
+                                                    int32_t* __restrict__ piv, PanelStatus* __restrict__ st,
+                                                    Cand<T>* __restrict__ cand, T* __restrict__ rowj,
+                                                    int32_t* __restrict__ counter)
+{
+    __shared__ double rv[TB / 64];
+    __shared__ int32_t rr[TB / 64];
+    __shared__ T pvals[PNB];
+    __shared__ int32_t sh_row, sh_gi, sh_abort;
+    __shared__ double sh_val;
+    T* Wl = reinterpret_cast<T*>(Wd);
+    const int G = gridDim.x, g = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (st->err) return;
+    const int32_t r = a + g * TB + tid;
+    const bool have = r < nr;
+    T x[PNB];
+#pragma unroll
+    for (int c = 0; c < PNB; c++) x[c] = (have && c < nb) ? Wl[r + (int64_t)(a + c) * ld] : Sc<T>::zero();
+    for (int jj = 0; jj < nb; jj++) {
+        const int32_t j = a + jj;
+        const int par = jj & 1;
+        double bv = -1.0;
+        int32_t br = -1;
+        if (have && r >= j) {
+#pragma unroll
+            for (int c = 0; c < PNB; c++)
+                if (c == jj) bv = Sc<T>::absval(x[c]);
+            br = r;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(bv, off);
+            const int32_t orr = __shfl_xor(br, off);
+            if (ov > bv || (ov == bv && orr >= 0 && (br < 0 || orr < br))) { bv = ov; br = orr; }
+        }
+        if (lane == 0) { rv[wid] = bv; rr[wid] = br; }
+        __syncthreads();
+        if (tid == 0) {
+            double v = -1.0;
+            int32_t row = -1;
+            for (int w = 0; w < TB / 64; w++)
+                if (rv[w] > v || (rv[w] == v && rr[w] >= 0 && (row < 0 || rr[w] < row))) { v = rv[w]; row = rr[w]; }
+            cand[par * G + g].val = v;
+            cand[par * G + g].row = row;
+            sh_row = row;
+        }
+        __syncthreads();
+        if (have && r == sh_row) {
+#pragma unroll
+            for (int c = 0; c < PNB; c++) cand[par * G + g].v[c] = x[c];
+        }
+        if (have && r == j) {
+#pragma unroll
+            for (int c = 0; c < PNB; c++) rowj[par * PNB + c] = x[c];
+        }
+        __syncthreads();
+        // one thread per workgroup: the release (the workgroup's stores above are complete at the barrier), the arrival,
+        // the wait, the acquire
+        if (tid == 0) {
+            __threadfence();
+            __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int32_t target = (jj + 1) * G;
+            long spins = 0;
+            int ab = 0;
+            while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                if (++spins > COOP_SPIN_LIMIT) { ab = 1; break; }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __threadfence();
+            sh_abort = ab;
+        }
+        __syncthreads();
+        if (sh_abort) {
+            if (tid == 0) atomicExch(&st->err, -1);
+            return;
+        }
+        if (wid == 0) {
+            double v = -1.0;
+            int32_t row = -1, gi = -1;
+            for (int q = lane; q < G; q += 64) {
+                const double cv = cand[par * G + q].val;
+                const int32_t cr = cand[par * G + q].row;
+                if (cv > v || (cv == v && cr >= 0 && (row < 0 || cr < row))) { v = cv; row = cr; gi = q; }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double ov = __shfl_xor(v, off);
+                const int32_t orr = __shfl_xor(row, off), og = __shfl_xor(gi, off);
+                if (ov > v || (ov == v && orr >= 0 && (row < 0 || orr < row))) { v = ov; row = orr; gi = og; }
+            }
+            if (lane == 0) { sh_val = v; sh_row = row; sh_gi = gi; }
+        }
+        __syncthreads();
+        const double thr = pivot_rel * colmax[j];
+        const int32_t pr = sh_row;
+        if (pr < 0 || sh_val <= 0.0 || sh_val < thr) {
+            if (g == 0 && tid == 0) {
+                st->err = j + 1;
+                st->best = sh_val > 0.0 ? sh_val : 0.0;
+                st->thr = thr;
+            }
+            return;
+        }
+        if (tid < PNB) pvals[tid] = cand[par * G + sh_gi].v[tid];
+        __syncthreads();
+        if (g == 0 && tid == 0) piv[j] = pr;
+        if (pr != j) {
+            if (have && r == j) {
+#pragma unroll
+                for (int c = 0; c < PNB; c++) x[c] = pvals[c];
+            } else if (have && r == pr) {
+#pragma unroll
+                for (int c = 0; c < PNB; c++) x[c] = rowj[par * PNB + c];
+            }
+            if (g == 0)
+                for (int32_t c = tid; c < P; c += TB)
+                    if (c < a || c >= a + nb) {
+                        const T t0 = Wl[j + (int64_t)c * ld];
+                        Wl[j + (int64_t)c * ld] = Wl[pr + (int64_t)c * ld];
+                        Wl[pr + (int64_t)c * ld] = t0;
+                    }
+        }
+        if (have && r > j) {
+            T l = Sc<T>::zero();
+#pragma unroll
+            for (int c = 0; c < PNB; c++)
+                if (c == jj) { l = Sc<T>::div(x[c], pvals[c]); x[c] = l; }
+#pragma unroll
+            for (int c = 0; c < PNB; c++)
+                if (c > jj && c < nb) x[c] = Sc<T>::fms(x[c], l, pvals[c]);
+        }
+    }
+    if (have) {
+#pragma unroll
+        for (int c = 0; c < PNB; c++)
+            if (c < nb) Wl[r + (int64_t)(a + c) * ld] = x[c];
     }
 }
 
@@ -425,12 +607,27 @@ void blk_gemm(int vs, int32_t ntile, const GemmTile* tiles, const UpdTask* tasks
     else hipLaunchKernelGGL(k_blk_gemm<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf);
 }
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
-           double pivot_rel, int32_t* piv, PanelStatus* st, hipStream_t s)
+           double pivot_rel, int32_t* piv, PanelStatus* st, void* coop, int32_t* counter, hipStream_t s)
 {
     ProfScope ps("getf2", (vs == 1 ? 2.0 : 8.0) * (double)nr * nb * nb / 2, s);
+    const int32_t G = (int32_t)((nr - a + TB - 1) / TB);
+    if (G >= 1 && G <= getf2_coop_max_wg() && nb <= PNB) {
+        k::fill_i32(counter, 1, 0, s);
+        void* args[] = {&Wl, &ld, &nr, &P, &a, &nb, (void*)&colmax, &pivot_rel, &piv, &st, &coop, (void*)&coop, &counter};
+        // cand (2 G slots) first in coop, then rowj (2 PNB scalars)
+        char* base = static_cast<char*>(coop);
+        void* rowj = base + (size_t)2 * G * (vs == 1 ? sizeof(Cand<double>) : sizeof(Cand<Zd>));
+        args[11] = &rowj;
+        hipError_t e = vs == 1 ? hipLaunchCooperativeKernel((const void*)k_getf2_coop<double>, dim3(G), dim3(TB), args, 0, s)
+                               : hipLaunchCooperativeKernel((const void*)k_getf2_coop<Zd>, dim3(G), dim3(TB), args, 0, s);
+        hip_check(e, "cooperative panel LU launch");
+        return;
+    }
     if (vs == 1) hipLaunchKernelGGL(k_getf2<double>, dim3(1), dim3(LU_TB), 0, s, Wl, ld, nr, P, a, nb, colmax, pivot_rel, piv, st);
     else hipLaunchKernelGGL(k_getf2<Zd>, dim3(1), dim3(LU_TB), 0, s, Wl, ld, nr, P, a, nb, colmax, pivot_rel, piv, st);
 }
+size_t getf2_coop_bytes(int vs) { return (size_t)2 * getf2_coop_max_wg() * (vs == 1 ? sizeof(Cand<double>) : sizeof(Cand<Zd>)) + 2 * PNB * 16 + 256; }
+int getf2_coop_max_wg() { return 512; }
 void trsm_sub(int vs, double* Wl, int64_t ld, int32_t P, int32_t a, int32_t nb, const PanelStatus* st, hipStream_t s)
 {
     const int32_t nc = P - a - nb;

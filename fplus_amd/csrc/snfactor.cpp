@@ -175,6 +175,8 @@ private:
     DevBuf<GemmTile> d_tiles_;
     DevBuf<double> d_X_;
     DevBuf<PanelStatus> d_status_;
+    DevBuf<double> d_coop_;          // the cooperative panel LU's candidate slots
+    DevBuf<int32_t> d_coop_cnt_;     // and its barrier counter
     DevBuf<URun> d_runs_;
     DevBuf<ExtractCol> d_ex_;
     Arena<double> vals_{nullptr, 0, 0, 0};
@@ -419,7 +421,8 @@ int Driver::panel(int64_t j0, int64_t j1)
     d_piv_.reserve(np);
     for (int64_t a = 0; a < np; a += FPA_PANEL_NB) {
         const int32_t nb = (int32_t)std::min<int64_t>(FPA_PANEL_NB, np - a);
-        k::getf2(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_colmax_.p, opt_.pivot_rel, d_piv_.p, d_status_.p, s_);
+        k::getf2(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_colmax_.p, opt_.pivot_rel, d_piv_.p, d_status_.p, d_coop_.p,
+                 d_coop_cnt_.p, s_);
         k::trsm_sub(vs_, Wl, ldw, (int32_t)np, (int32_t)a, nb, d_status_.p, s_);
         k::gemm_sub(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_status_.p, s_);
     }
@@ -431,6 +434,10 @@ int Driver::panel(int64_t j0, int64_t j1)
     dev_sync(s_);
     F_.t_wait += now_s() - tw0;
     const double th1 = now_s();
+    if (st.err < 0) {
+        errf(err_, errlen_, "internal: the cooperative panel LU's workgroups did not meet at a barrier");
+        return FPLUS_ERR_INTERNAL;
+    }
     if (st.err) {
         const int64_t j = j0 + st.err - 1;
         errf(err_, errlen_, "Near-zero or zero pivot %.6e at step %ld (threshold %.6e). Matrix is singular.", st.best, (long)j, st.thr);
@@ -619,6 +626,8 @@ int Driver::run()
     d_rowmap_.alloc(nn);
     k::fill_i32(d_rowmap_.p, n, -1, s_);
     d_status_.alloc(1);
+    d_coop_.alloc(k::getf2_coop_bytes(vs_) / sizeof(double) + 1);
+    d_coop_cnt_.alloc(1);
     F_.d_blocks.alloc(nn);
     F_.val_chunks.clear();
     F_.row_chunks.clear();
