@@ -8,9 +8,10 @@ region (it is reused across refactorizations of one pattern).  Prints one JSON l
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
 
 The line carries the backward error of every right-hand side of the last step, the roofline of the dominant kernel
-(timed with HIP events on the library's stream over the timed region), and a CPU baseline: the reference's own column
-kernel (oracle/_ref, compiled from the reference's sources without a CBLAS) or the oracle, on one host thread, timed
-on a smaller member of the same matrix family and scaled to the workload by the ratio of the factor flops.
+(timed with HIP events on the library's stream over the timed region), and a CPU baseline: the reference's own
+supernodal kernel (oracle/_ref, compiled from the reference's sources; no CBLAS is installed, so its loop form) or the
+oracle, on one host thread, timed on a smaller member of the same matrix family and scaled to the workload by the
+ratio of the factor flops.
 """
 import argparse
 import ctypes as C
@@ -41,7 +42,7 @@ CPU_SAMPLE = {
     "convdiff2d": (gen.convdiff2d, dict(g=96)),
     "convdiff3d": (gen.convdiff3d, dict(g=16)),
 }
-DEFAULT_WORKLOAD = "shiftinvert-16x4x256"
+DEFAULT_WORKLOAD = "shiftinvert-64x4x1024"   # n = 262144, the plain FP64 GPU drop-in configuration
 PEAK_FP64_MFMA = 78.6e12      # MI355X dense FP64 matrix, FLOP/s
 PEAK_HBM = 8.0e12             # MI355X HBM3E, B/s
 NRHS = 16
@@ -82,7 +83,7 @@ def cpu_baseline(workload_flops, family):
     t0 = time.perf_counter()
     if kind == "reference":
         x, Dr, Dc = O.ref_ruiz(n, Bp, Bi, Bx)
-        f = O.ref_factor_sn(n, Bp, Bi, x, relax_rows=32)
+        f = O.ref_factor_sn(n, Bp, Bi, x, relax_rows=32, panel=512)
         for r in range(NRHS):
             xs = np.ascontiguousarray(b[:, r])
             O.ref().fplus_tri_solve(n, O._p64(f["Lp"]), O._p64(f["Li"]), f["Lx"].ctypes.data, O._p64(f["Up"]), O._p64(f["Ui"]),
@@ -94,7 +95,8 @@ def cpu_baseline(workload_flops, family):
     scaled = t * workload_flops / fl
     desc = ", ".join(f"{k}={v}" for k, v in kw.items())
     return {"value": scaled, "unit": "s/system", "cores": 1, "kind": kind,
-            "sample": f"{family}({desc}), n={n}: factor (Ruiz + column kernel, relax 32) + {NRHS} solves in {t:.2f} s on one "
+            "sample": f"{family}({desc}), n={n}: factor (Ruiz + supernodal panel kernel, panel 512, relax 32, no BLAS) + "
+                      f"{NRHS} solves in {t:.2f} s on one "
                       f"thread, scaled by the factor flops ratio {workload_flops:.3e}/{fl:.3e}; CPU {platform.processor() or 'x86_64'}, "
                       f"{os.cpu_count()} logical CPUs on the host"}
 
@@ -102,7 +104,7 @@ def cpu_baseline(workload_flops, family):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
