@@ -69,9 +69,10 @@ void u_runs(int vs, int32_t nruns, const URun* runs, const double* W, double* Ux
 // the segment solves of the tiles of one level (in place in W, and compact into xbuf)
 void blk_trsm(int vs, int32_t ntile, const TrsmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
               const int32_t* rowmap, double* W, int64_t ldw, double* xbuf, double flops, hipStream_t s);
-// the rectangles below, subtracted from W through rowmap (atomically: tasks of one level may share target rows)
+// the rectangles below, subtracted from W through rowmap: atomically when shared_rows (two tasks of the level write a
+// common row), else with plain loads and stores
 void blk_gemm(int vs, int32_t ntile, const GemmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
-              const int32_t* rowmap, double* W, int64_t ldw, const double* xbuf, double flops, hipStream_t s);
+              const int32_t* rowmap, double* W, int64_t ldw, const double* xbuf, int shared_rows, double flops, hipStream_t s);
 // dense partial-pivot LU of the panel's lower section (nr x P, column-major, ld), sub-panel [a, a + nb): a cooperative
 // launch over the rows (coop: getf2_coop_bytes(vs) of scratch, counter: one int), one workgroup for very tall panels
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,

@@ -135,12 +135,14 @@ template <class T>
 __global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ tiles, const UpdTask* __restrict__ tasks,
                                                   const int32_t* __restrict__ cols, const BlockDesc* __restrict__ blocks,
                                                   const int32_t* __restrict__ rowmap, double* __restrict__ W, int64_t ldw,
-                                                  const double* __restrict__ xbuf)
+                                                  const double* __restrict__ xbuf, int shared_rows)
 {
     constexpr bool CPLX = Sc<T>::vs == 2;
     constexpr int KC = 16, TM = 64, TN = 64, LPT = KC * TM / TB;   // elements of A (and of B) each thread stages per chunk
-    __shared__ T As[2][KC][TM];     // A[k][row], two chunks in flight
-    __shared__ T Bs[2][KC][TN];     // B[k][col]
+    // staging: A[k][row] and B[k][col] for two chunks in flight; at the end the output tile
+    __shared__ T stg[2 * KC * TM + 2 * KC * TN];
+    T* const As = stg;
+    T* const Bs = stg + 2 * KC * TM;
     __shared__ int32_t rpos[TM];
     __shared__ int32_t cidx[TN];
     const GemmTile tl = tiles[blockIdx.x];
@@ -178,8 +180,8 @@ __global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ ti
 #pragma unroll
         for (int q = 0; q < LPT; q++) {
             const int e = tid + q * TB;
-            As[buf][e / TM][e % TM] = ra[q];
-            Bs[buf][e % KC][e / KC] = rb[q];
+            As[(buf * KC + e / TM) * TM + e % TM] = ra[q];
+            Bs[(buf * KC + e % KC) * TN + e / KC] = rb[q];
         }
     };
     fetch(0);
@@ -194,9 +196,9 @@ __global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ ti
             const int kk = ks + (lane >> 4);
             T a[2], b[2];
 #pragma unroll
-            for (int i = 0; i < 2; i++) a[i] = As[buf][kk][wr + i * 16 + (lane & 15)];
+            for (int i = 0; i < 2; i++) a[i] = As[(buf * KC + kk) * TM + wr + i * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < 2; j++) b[j] = Bs[buf][kk][wc + j * 16 + (lane & 15)];
+            for (int j = 0; j < 2; j++) b[j] = Bs[(buf * KC + kk) * TN + wc + j * 16 + (lane & 15)];
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -215,25 +217,44 @@ __global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ ti
         __syncthreads();
         buf ^= 1;
     }
+    // the tile through LDS (the staging buffers are free now), then column by column with lanes along the rows: each
+    // wavefront's 64 rows of one column are contiguous in W wherever rowmap keeps the block's rows together.  A tile
+    // owned by its launch alone (no other task of the level writes its rows) updates W with plain loads and stores.
+    T* const Cs = stg;   // [TN][TM], row r of column c at (r + c) % TM (no bank conflicts either way)
+    static_assert(TN * TM <= 2 * KC * (TM + TN), "tile staging does not fit");
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             const int c = wc + j * 16 + (lane & 15);
-            if (c >= nc) continue;
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 const int r = wr + i * 16 + (lane >> 4) + 4 * v;
-                if (r >= nr) continue;
-                double* p = W + ((int64_t)rpos[r] + (int64_t)cidx[c] * ldw) * Sc<T>::vs;
-                if constexpr (CPLX) {
-                    atomicAdd(p, -accr[i][j][v]);
-                    atomicAdd(p + 1, -acci[i][j][v]);
-                } else {
-                    atomicAdd(p, -accr[i][j][v]);
-                }
+                T val;
+                if constexpr (CPLX) val = zmake(accr[i][j][v], acci[i][j][v]);
+                else val = accr[i][j][v];
+                Cs[c * TM + ((r + c) & (TM - 1))] = val;
             }
         }
+    __syncthreads();
+    const int r = tid & 63;
+    if (r < nr) {
+        const int64_t rp = rpos[r];
+        for (int c = tid >> 6; c < nc; c += TB / 64) {
+            T* p = reinterpret_cast<T*>(W) + rp + (int64_t)cidx[c] * ldw;
+            const T val = Cs[c * TM + ((r + c) & (TM - 1))];
+            if (shared_rows) {
+                if constexpr (CPLX) {
+                    atomicAdd(&p->x, -val.x);
+                    atomicAdd(&p->y, -val.y);
+                } else {
+                    atomicAdd(p, -val);
+                }
+            } else {
+                *p = Sc<T>::sub(*p, val);
+            }
+        }
+    }
 }
 
 // ---- dense partial-pivot LU of the panel's lower section
@@ -599,12 +620,13 @@ void blk_trsm(int vs, int32_t ntile, const TrsmTile* tiles, const UpdTask* tasks
     else hipLaunchKernelGGL(k_blk_trsm<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf);
 }
 void blk_gemm(int vs, int32_t ntile, const GemmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
-              const int32_t* rowmap, double* W, int64_t ldw, const double* xbuf, double flops, hipStream_t s)
+              const int32_t* rowmap, double* W, int64_t ldw, const double* xbuf, int shared_rows, double flops, hipStream_t s)
 {
     if (ntile <= 0) return;
     ProfScope ps("blk_gemm", flops, s);
-    if (vs == 1) hipLaunchKernelGGL(k_blk_gemm<double>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf);
-    else hipLaunchKernelGGL(k_blk_gemm<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf);
+    if (vs == 1)
+        hipLaunchKernelGGL(k_blk_gemm<double>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
+    else hipLaunchKernelGGL(k_blk_gemm<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
 }
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
            double pivot_rel, int32_t* piv, PanelStatus* st, void* coop, int32_t* counter, hipStream_t s)

@@ -17,6 +17,7 @@
 #include "snfactor.h"
 
 #include "host_common.h"
+#include "host_pool.h"
 
 #include <algorithm>
 #include <chrono>
@@ -80,30 +81,43 @@ private:
         *end = S.srch_end >= 0 ? S.srch_end : S.nr() - S.w;
         return S.srch.data();
     }
-    int64_t dfs(int64_t root, int64_t col, int64_t top, int64_t stamp, int64_t open, int64_t* ncand)
+    // the search's marks and stacks (one set per thread of the parallel per-column search)
+    struct SWork {
+        std::vector<int64_t> rmark, smark, seg0, topo, cand, stack, pos;
+        void init(size_t nn)
+        {
+            rmark.assign(nn, 0); smark.assign(nn, 0); seg0.assign(nn, 0); topo.assign(nn, 0);
+            cand.assign(nn, 0); stack.assign(nn, 0); pos.assign(nn, 0);
+        }
+    };
+    int64_t dfs(SWork& w, int64_t root, int64_t col, int64_t top, int64_t stamp, int64_t open, int64_t* ncand) const
     {
         int64_t sp = 0;
-        stack_[0] = root; smark_[root] = stamp; seg0_[root] = col; pos_[root] = 0;
+        w.stack[0] = root; w.smark[root] = stamp; w.seg0[root] = col; w.pos[root] = 0;
         while (sp >= 0) {
-            const int64_t s = stack_[sp];
+            const int64_t s = w.stack[sp];
             int64_t end;
             const int64_t* rows = srch_rows(s, open, &end);
-            int64_t q = pos_[s], t = -1;
+            int64_t q = w.pos[s], t = -1;
             for (; q < end; q++) {
                 const int64_t i = rows[q], k = pinv_[i];
                 if (k < 0) {
-                    if (rmark_[i] != stamp) { rmark_[i] = stamp; cand_[(*ncand)++] = i; }
+                    if (w.rmark[i] != stamp) { w.rmark[i] = stamp; w.cand[(*ncand)++] = i; }
                     continue;
                 }
                 const int64_t u = col_sn_[k];
-                if (smark_[u] == stamp) { if (k < seg0_[u]) seg0_[u] = k; continue; }
-                t = u; smark_[u] = stamp; seg0_[u] = k;
+                if (w.smark[u] == stamp) { if (k < w.seg0[u]) w.seg0[u] = k; continue; }
+                t = u; w.smark[u] = stamp; w.seg0[u] = k;
                 break;
             }
-            if (t >= 0) { pos_[s] = q + 1; pos_[t] = 0; stack_[++sp] = t; }
-            else { sp--; topo_[--top] = s; }
+            if (t >= 0) { w.pos[s] = q + 1; w.pos[t] = 0; w.stack[++sp] = t; }
+            else { sp--; w.topo[--top] = s; }
         }
         return top;
+    }
+    int64_t dfs(int64_t root, int64_t col, int64_t top, int64_t stamp, int64_t open, int64_t* ncand)
+    {
+        return dfs(ws0_, root, col, top, stamp, open, ncand);
     }
     bool relax_ok(const HSn& S, int64_t n_miss, int64_t extra) const
     {
@@ -156,7 +170,10 @@ private:
     size_t errlen_;
 
     std::vector<HSn> sn_;
-    std::vector<int64_t> col_sn_, pinv_, p_, bpos_, rmark_, smark_, seg0_, topo_, cand_, stack_, pos_, miss_;
+    std::vector<int64_t> col_sn_, pinv_, p_, bpos_, miss_;
+    SWork ws0_;                     // the sequential phases' marks
+    std::vector<SWork> wsT_;        // one per thread of the parallel search
+    HostPool pool_;
     int64_t relax_ = 0, rz_w_[3] = {4, 16, 48}, rz_cap_ = 32;
     double rz_z_[3] = {0.8, 0.1, 0.05};
     bool relax_auto_ = false;
@@ -224,25 +241,38 @@ int Driver::panel(int64_t j0, int64_t j1)
     const int64_t np = j1 - j0, n = n_;
     const int64_t open0 = open_, we_open = open0 >= 0 ? sn_[open0].w : 0;
     const double th0 = now_s();
-    // ---- phase 1: per-column search over the supernodes complete before j0 (src/factor_sn_panel.inc:34-58)
+    // ---- phase 1: per-column search over the supernodes complete before j0 (src/factor_sn_panel.inc:34-58); the
+    // columns' searches are independent (they read the structure as it stood at j0): threads take columns in turn,
+    // each with its own marks, and the results are joined in column order
     std::vector<int64_t> ext_s, ext_kf, ext_off(np + 1), npl, np_off(np + 1);
-    for (int64_t ci = 0; ci < np; ci++) {
-        const int64_t c = j0 + ci, stamp = c + 1;
-        int64_t top = n, ncand = 0;
-        for (int64_t q = Bp_[c]; q < Bp_[c + 1]; q++) {
-            const int64_t i = Bi_[q], k = pinv_[i];
-            if (k < 0) {
-                if (rmark_[i] != stamp) { rmark_[i] = stamp; cand_[ncand++] = i; }
-                continue;
+    {
+        std::vector<std::vector<int64_t>> c_s(np), c_kf(np), c_np(np);
+        auto search = [&](int64_t ci, SWork& w) {
+            const int64_t c = j0 + ci, stamp = c + 1;
+            int64_t top = n, ncand = 0;
+            for (int64_t q = Bp_[c]; q < Bp_[c + 1]; q++) {
+                const int64_t i = Bi_[q], k = pinv_[i];
+                if (k < 0) {
+                    if (w.rmark[i] != stamp) { w.rmark[i] = stamp; w.cand[ncand++] = i; }
+                    continue;
+                }
+                const int64_t sn = col_sn_[k];
+                if (w.smark[sn] == stamp) { if (k < w.seg0[sn]) w.seg0[sn] = k; continue; }
+                top = dfs(w, sn, k, top, stamp, open_, &ncand);
             }
-            const int64_t s = col_sn_[k];
-            if (smark_[s] == stamp) { if (k < seg0_[s]) seg0_[s] = k; continue; }
-            top = dfs(s, k, top, stamp, open_, &ncand);
+            c_s[ci].resize(n - top);
+            c_kf[ci].resize(n - top);
+            for (int64_t t = top; t < n; t++) { c_s[ci][t - top] = w.topo[t]; c_kf[ci][t - top] = w.seg0[w.topo[t]]; }
+            c_np[ci].assign(w.cand.begin(), w.cand.begin() + ncand);
+        };
+        pool_.run(np, [&](int64_t ci, int tid) { search(ci, wsT_[tid]); });
+        for (int64_t ci = 0; ci < np; ci++) {
+            ext_off[ci] = (int64_t)ext_s.size();
+            np_off[ci] = (int64_t)npl.size();
+            ext_s.insert(ext_s.end(), c_s[ci].begin(), c_s[ci].end());
+            ext_kf.insert(ext_kf.end(), c_kf[ci].begin(), c_kf[ci].end());
+            npl.insert(npl.end(), c_np[ci].begin(), c_np[ci].end());
         }
-        ext_off[ci] = (int64_t)ext_s.size();
-        np_off[ci] = (int64_t)npl.size();
-        for (int64_t t = top; t < n; t++) { ext_s.push_back(topo_[t]); ext_kf.push_back(seg0_[topo_[t]]); }
-        npl.insert(npl.end(), cand_.begin(), cand_.begin() + ncand);
     }
     ext_off[np] = (int64_t)ext_s.size();
     np_off[np] = (int64_t)npl.size();
@@ -254,11 +284,11 @@ int Driver::panel(int64_t j0, int64_t j1)
         const int64_t us = 2 * n + 1 + j0;
         int64_t top = n, dummy = 0;
         for (size_t q = 0; q < ext_s.size(); q++) {
-            if (smark_[ext_s[q]] == us) continue;
+            if (ws0_.smark[ext_s[q]] == us) continue;
             top = dfs(ext_s[q], ext_kf[q], top, us, open_, &dummy);
             dummy = 0;
         }
-        for (int64_t t = top; t < n; t++) { uidx_[topo_[t]] = (int64_t)un.size(); ustamp_[topo_[t]] = us; un.push_back(topo_[t]); }
+        for (int64_t t = top; t < n; t++) { uidx_[ws0_.topo[t]] = (int64_t)un.size(); ustamp_[ws0_.topo[t]] = us; un.push_back(ws0_.topo[t]); }
     }
     // ---- the workspace rows: the U rows (pivotal before j0) first, each union supernode's reached rows as one run (the
     // union of its columns' segments [kf, we) is the longest one), then the rows not yet pivotal
@@ -346,6 +376,7 @@ int Driver::panel(int64_t j0, int64_t j1)
                 if (hp < 0) { errf(err_, errlen_, "internal: update row outside the panel rows"); return FPLUS_ERR_INTERNAL; }
                 wl[hp] = std::max(wl[hp], lev);
             }
+
             TaskL t;
             t.t.block = b; t.t.k0 = (int32_t)k0;
             t.t.col_off = (int32_t)col_off; t.t.ncol = (int32_t)ncol; t.t.xoff = 0;
@@ -382,6 +413,10 @@ int Driver::panel(int64_t j0, int64_t j1)
         lev_tt[L] = std::max(lev_tt[L], lev_tt[L - 1]);
         lev_gt[L] = std::max(lev_gt[L], lev_gt[L - 1]);
     }
+    // a level of one task writes its rows alone (plain loads and stores); with several, rows may be shared (atomics)
+    std::vector<int> lev_shared(maxlevel + 2, 0), lev_ntask(maxlevel + 2, 0);
+    for (const TaskL& x : tl) lev_ntask[x.level]++;
+    for (int32_t L = 0; L <= maxlevel + 1; L++) lev_shared[L] = lev_ntask[L] > 1;
     F_.levels += maxlevel;
     F_.tasks += (int64_t)tasks.size();
     F_.t_tasks += now_s() - th_p1;
@@ -411,7 +446,7 @@ int Driver::panel(int64_t j0, int64_t j1)
                         lev_tflops[L], s_);
             const int32_t g0 = lev_gt[L], g1 = lev_gt[L + 1];
             k::blk_gemm(vs_, g1 - g0, d_tiles_.p + g0, d_tasks_.p, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, d_X_.p,
-                        lev_gflops[L], s_);
+                        lev_shared[L], lev_gflops[L], s_);
         }
     }
     double* Wl = d_W_.p + Ru * vs_;
@@ -453,21 +488,23 @@ int Driver::panel(int64_t j0, int64_t j1)
     std::vector<std::pair<int64_t, int64_t>> touched;   // (supernode, first column in this panel)
     for (int64_t ci = 0; ci < np; ci++) {
         const int64_t j = j0 + ci, stamp2 = n + 1 + j;
+        const double tq0 = now_s();
         int64_t ncand = 0, top = n;
-        for (int64_t q = np_off[ci]; q < np_off[ci + 1]; q++) { const int64_t i = npl[q]; cand_[ncand++] = i; rmark_[i] = stamp2; }
+        for (int64_t q = np_off[ci]; q < np_off[ci + 1]; q++) { const int64_t i = npl[q]; ws0_.cand[ncand++] = i; ws0_.rmark[i] = stamp2; }
         const int64_t n0 = ncand;
-        if (open0 >= 0 && sn_[open0].w > we_open && smark_[open0] != stamp2) {
+        if (open0 >= 0 && sn_[open0].w > we_open && ws0_.smark[open0] != stamp2) {
             for (int64_t q = ext_off[ci]; q < ext_off[ci + 1]; q++)
                 if (ext_s[q] == open0) { top = dfs(open0, sn_[open0].c0 + we_open, top, stamp2, open_, &ncand); break; }
         }
         for (int64_t q = 0; q < n0; q++) {
-            const int64_t i = cand_[q], k = pinv_[i];
+            const int64_t i = ws0_.cand[q], k = pinv_[i];
             if (k < 0) continue;
             const int64_t s = col_sn_[k];
-            if (smark_[s] == stamp2) { if (k < seg0_[s]) seg0_[s] = k; continue; }
+            if (ws0_.smark[s] == stamp2) { if (k < ws0_.seg0[s]) ws0_.seg0[s] = k; continue; }
             top = dfs(s, k, top, stamp2, open_, &ncand);
         }
         const int64_t pivrow = lrows[ci];
+        const double tq1 = now_s();
         // U(:, j): pre-panel segments, within-panel segments, the diagonal -- each a run of consecutive rows whose values
         // are consecutive in the workspace column
         int64_t uo = Up_[j];
@@ -484,18 +521,19 @@ int Driver::panel(int64_t j0, int64_t j1)
             put_run(ext_kf[q], S.c0 + we - ext_kf[q], ubase[u] + (ext_kf[q] - ukmin[u]));
         }
         for (int64_t t = top; t < n; t++) {
-            const HSn& S = sn_[topo_[t]];
-            const int64_t k0 = seg0_[topo_[t]];
+            const HSn& S = sn_[ws0_.topo[t]];
+            const int64_t k0 = ws0_.seg0[ws0_.topo[t]];
             put_run(k0, S.c0 + S.w - k0, Ru + (k0 - j0));
         }
         put_run(j, 1, Ru + ci);
         Up_[j + 1] = uo;
         Urp_[j + 1] = (int64_t)Ur_row_.size();
+        const double tq2 = now_s();
         // L(:, j): join the open supernode (exactly or relaxed), else open a new one
         std::vector<int64_t>& npv = miss_;   // reused: the candidates still non-pivotal, then the missing rows
         std::vector<int64_t> npl_l;
         npl_l.reserve(ncand);
-        for (int64_t q = 0; q < ncand; q++) if (pinv_[cand_[q]] < 0) npl_l.push_back(cand_[q]);
+        for (int64_t q = 0; q < ncand; q++) if (pinv_[ws0_.cand[q]] < 0) npl_l.push_back(ws0_.cand[q]);
         int join = 0;
         int64_t n_miss = 0;
         if (open_ >= 0) {
@@ -547,9 +585,15 @@ int Driver::panel(int64_t j0, int64_t j1)
             touched.push_back({s, j});
         }
         pinv_[pivrow] = j; p_[j] = pivrow;
+        const double tq3 = now_s();
         // prune the closed supernodes this column reached (pre-panel, then within-panel)
         for (int64_t q = ext_off[ci]; q < ext_off[ci + 1]; q++) if (ext_s[q] != open_) prune(ext_s[q], pivrow);
-        for (int64_t t = top; t < n; t++) if (topo_[t] != open_) prune(topo_[t], pivrow);
+        for (int64_t t = top; t < n; t++) if (ws0_.topo[t] != open_) prune(ws0_.topo[t], pivrow);
+        const double tq4 = now_s();
+        F_.t_p3_dfs += tq1 - tq0;
+        F_.t_p3_u += tq2 - tq1;
+        F_.t_p3_join += tq3 - tq2;
+        F_.t_p3_prune += tq4 - tq3;
     }
     const double th_out = now_s();
     F_.t_p3 += th_out - th1;
@@ -615,8 +659,10 @@ int Driver::run()
         rz_w_[0] = 2; rz_w_[1] = 8; rz_w_[2] = 32; rz_z_[0] = 0.3; rz_z_[1] = 0.03; rz_z_[2] = 0.01; rz_cap_ = 8;
     }
     col_sn_.assign(nn, -1); pinv_.assign(nn, -1); p_.assign(nn, -1); bpos_.assign(nn, -1);
-    rmark_.assign(nn, 0); smark_.assign(nn, 0); seg0_.assign(nn, 0); topo_.assign(nn, 0); cand_.assign(nn, 0);
-    stack_.assign(nn, 0); pos_.assign(nn, 0); miss_.assign(nn, 0); ustamp_.assign(nn, 0); uidx_.assign(nn, 0);
+    ws0_.init(nn);
+    wsT_.resize(pool_.size());
+    for (SWork& w : wsT_) w.init(nn);
+    miss_.assign(nn, 0); ustamp_.assign(nn, 0); uidx_.assign(nn, 0);
     hrowpos_.assign(nn, -1); lposmap_.assign(nn, -1);
     sn_.reserve(nn);
     Up_.assign(n + 1, 0);
