@@ -62,23 +62,29 @@ template <> struct TCols<Zd> { static constexpr int v = 4; };
 constexpr int PNB = FPA_PANEL_NB;  // sub-panel width of the dense panel LU
 
 // the unit-lower solve of a block's segment [k0, w) for up to TCols panel columns: thread t owns segment row t (the
-// segment is at most 256 rows: a block is at most one panel wide), the rows below the current one subtract its
-// multiple.  The factor's values are read PF steps ahead (they come from HBM; a step is short).  The result goes back
-// to W and, compact, to the task's slice of xbuf.
+// segment is at most 256 rows: a block is at most one panel wide).  The segment goes in chunks of 16 rows: a chunk's
+// triangle inside its wavefront (the row just finished broadcast by a lane shuffle), one barrier, then the rows below
+// subtract the chunk's 16 columns.  Every row receives its updates in column order, as in the column-by-column solve.
+// The factor's values of the next chunk are read while the current one is solved.  The result goes back to W and,
+// compact, to the task's slice of xbuf.
+template <class T> __device__ inline T shfl_val(T v, int src);
+template <> __device__ inline double shfl_val<double>(double v, int src) { return __shfl(v, src); }
+template <> __device__ inline Zd shfl_val<Zd>(Zd v, int src) { return zmake(__shfl(v.x, src), __shfl(v.y, src)); }
+
 template <class T>
 __global__ __launch_bounds__(TB) void k_blk_trsm(const TrsmTile* __restrict__ tiles, const UpdTask* __restrict__ tasks,
                                                   const int32_t* __restrict__ cols, const BlockDesc* __restrict__ blocks,
                                                   const int32_t* __restrict__ rowmap, double* __restrict__ W, int64_t ldw,
                                                   double* __restrict__ xbuf)
 {
-    constexpr int CT = TCols<T>::v, PF = 8;
-    __shared__ T xs[2][CT];
+    constexpr int CT = TCols<T>::v, CH = 16;
+    __shared__ T xs[TB][CT];
     __shared__ int32_t cidx[CT];
     const TrsmTile tt = tiles[blockIdx.x];
     const UpdTask tk = tasks[tt.task];
     const BlockDesc B = blocks[tk.block];
     const int32_t m = B.w - tk.k0, nc = min(CT, tk.ncol - tt.col0), t = threadIdx.x;
-    const T* V = reinterpret_cast<const T*>(B.vals);
+    const T* V = reinterpret_cast<const T*>(B.vals) + (int64_t)tk.k0 * B.nrows + tk.k0;   // V[k * nrows + r]: segment (r, k)
     if (t < nc) cidx[t] = cols[tk.col_off + tt.col0 + t];
     __syncthreads();
     const bool own = t < m;
@@ -86,32 +92,50 @@ __global__ __launch_bounds__(TB) void k_blk_trsm(const TrsmTile* __restrict__ ti
     T x[CT];
 #pragma unroll
     for (int c = 0; c < CT; c++) x[c] = (own && c < nc) ? ld_val<T>(W, pos + (int64_t)cidx[c] * ldw) : Sc<T>::zero();
-    const T* Vc = V + (int64_t)tk.k0 * B.nrows + tk.k0 + t;   // column k0 of the segment, row t
-    T lcur[PF], lnxt[PF];
+    // the factor's values row t needs from chunk ch: columns r0 .. min(r1, t) - 1
+    auto fetch = [&](int ch, T* lv) {
+        const int r0 = ch * CH;
 #pragma unroll
-    for (int u = 0; u < PF; u++) lcur[u] = (own && t > u && u < m) ? Vc[(int64_t)u * B.nrows] : Sc<T>::zero();
-    for (int32_t kg = 0; kg < m; kg += PF) {
+        for (int u = 0; u < CH; u++) {
+            const int k = r0 + u;
+            lv[u] = (own && k < t) ? V[(int64_t)k * B.nrows + t] : Sc<T>::zero();
+        }
+    };
+    T lcur[CH], lnxt[CH];
+    fetch(0, lcur);
+    const int nch = (m + CH - 1) / CH;
+    for (int ch = 0; ch < nch; ch++) {
+        const int r0 = ch * CH, r1 = min(m, r0 + CH);
+        if (ch + 1 < nch) fetch(ch + 1, lnxt);
+        if (t >= r0 && t < r1) {
+            // the chunk's triangle: its 16 rows are lanes of one wavefront
 #pragma unroll
-        for (int u = 0; u < PF; u++) {
-            const int32_t k = kg + PF + u;
-            lnxt[u] = (own && t > k && k < m) ? Vc[(int64_t)k * B.nrows] : Sc<T>::zero();
+            for (int u = 0; u < CH; u++) {
+                const int k = r0 + u;
+                if (k >= r1) break;
+                T xk[CT];
+#pragma unroll
+                for (int c = 0; c < CT; c++) xk[c] = shfl_val<T>(x[c], k & 63);
+                if (t > k) {
+#pragma unroll
+                    for (int c = 0; c < CT; c++) x[c] = Sc<T>::fms(x[c], lcur[u], xk[c]);
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CT; c++) xs[t][c] = x[c];
+        }
+        __syncthreads();
+        if (own && t >= r1) {
+#pragma unroll
+            for (int u = 0; u < CH; u++) {
+                const int k = r0 + u;
+                if (k >= r1) break;
+#pragma unroll
+                for (int c = 0; c < CT; c++) x[c] = Sc<T>::fms(x[c], lcur[u], xs[k][c]);
+            }
         }
 #pragma unroll
-        for (int u = 0; u < PF; u++) {
-            const int32_t k = kg + u;
-            if (k >= m) break;
-            if (t == k) {
-#pragma unroll
-                for (int c = 0; c < CT; c++) xs[k & 1][c] = x[c];
-            }
-            __syncthreads();
-            if (own && t > k) {
-#pragma unroll
-                for (int c = 0; c < CT; c++) x[c] = Sc<T>::fms(x[c], lcur[u], xs[k & 1][c]);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < PF; u++) lcur[u] = lnxt[u];
+        for (int u = 0; u < CH; u++) lcur[u] = lnxt[u];
     }
     if (!own) return;
     T* X = reinterpret_cast<T*>(xbuf) + tk.xoff;

@@ -171,6 +171,8 @@ private:
 
     std::vector<HSn> sn_;
     std::vector<int64_t> col_sn_, pinv_, p_, bpos_, miss_;
+    std::vector<int64_t> npl_l_, piv_c_;   // phase 3: a column's candidates not pivotal / pivotal in the panel
+    std::vector<std::vector<int64_t>> c_s_, c_kf_, c_np_;   // phase 1's results per panel column
     SWork ws0_;                     // the sequential phases' marks
     std::vector<SWork> wsT_;        // one per thread of the parallel search
     HostPool pool_;
@@ -244,9 +246,13 @@ int Driver::panel(int64_t j0, int64_t j1)
     // ---- phase 1: per-column search over the supernodes complete before j0 (src/factor_sn_panel.inc:34-58); the
     // columns' searches are independent (they read the structure as it stood at j0): threads take columns in turn,
     // each with its own marks, and the results are joined in column order
-    std::vector<int64_t> ext_s, ext_kf, ext_off(np + 1), npl, np_off(np + 1);
+    std::vector<int64_t> ext_s, ext_kf, ext_off(np + 1);
+    std::vector<std::vector<int64_t>>& c_np = c_np_;   // per column: its candidates (rows not pivotal), search order
+    if ((int64_t)c_np.size() < np) c_np.resize(np);
     {
-        std::vector<std::vector<int64_t>> c_s(np), c_kf(np), c_np(np);
+        std::vector<std::vector<int64_t>>& c_s = c_s_;
+        std::vector<std::vector<int64_t>>& c_kf = c_kf_;
+        if ((int64_t)c_s.size() < np) { c_s.resize(np); c_kf.resize(np); }
         auto search = [&](int64_t ci, SWork& w) {
             const int64_t c = j0 + ci, stamp = c + 1;
             int64_t top = n, ncand = 0;
@@ -268,14 +274,11 @@ int Driver::panel(int64_t j0, int64_t j1)
         pool_.run(np, [&](int64_t ci, int tid) { search(ci, wsT_[tid]); });
         for (int64_t ci = 0; ci < np; ci++) {
             ext_off[ci] = (int64_t)ext_s.size();
-            np_off[ci] = (int64_t)npl.size();
             ext_s.insert(ext_s.end(), c_s[ci].begin(), c_s[ci].end());
             ext_kf.insert(ext_kf.end(), c_kf[ci].begin(), c_kf[ci].end());
-            npl.insert(npl.end(), c_np[ci].begin(), c_np[ci].end());
         }
     }
     ext_off[np] = (int64_t)ext_s.size();
-    np_off[np] = (int64_t)npl.size();
     const double th_p1 = now_s();
     F_.t_p1 += th_p1 - th0;
     // ---- the union order (src/factor_sn_panel.inc:61-70)
@@ -311,8 +314,9 @@ int Driver::panel(int64_t j0, int64_t j1)
         }
     }
     const int64_t Ru = (int64_t)univ.size();
-    for (int64_t row : npl)
-        if (hrowpos_[row] < 0) { hrowpos_[row] = (int32_t)univ.size(); univ.push_back((int32_t)row); }
+    for (int64_t ci = 0; ci < np; ci++)
+        for (int64_t row : c_np[ci])
+            if (hrowpos_[row] < 0) { hrowpos_[row] = (int32_t)univ.size(); univ.push_back((int32_t)row); }
     if (open0 >= 0) {
         const HSn& S = sn_[open0];
         for (int64_t r = S.w; r < S.nr(); r++) {
@@ -489,20 +493,28 @@ int Driver::panel(int64_t j0, int64_t j1)
     for (int64_t ci = 0; ci < np; ci++) {
         const int64_t j = j0 + ci, stamp2 = n + 1 + j;
         const double tq0 = now_s();
+        // one pass over the column's candidates from the search before the panel: mark them, keep the ones still not
+        // pivotal (in order: the candidate order is the pivot tie-break order) and the ones pivotal in this panel (the
+        // within-panel search starts from them, in order); the search's new candidates follow, in its order
         int64_t ncand = 0, top = n;
-        for (int64_t q = np_off[ci]; q < np_off[ci + 1]; q++) { const int64_t i = npl[q]; ws0_.cand[ncand++] = i; ws0_.rmark[i] = stamp2; }
-        const int64_t n0 = ncand;
+        npl_l_.clear();
+        piv_c_.clear();
+        for (const int64_t i : c_np[ci]) {
+            ws0_.rmark[i] = stamp2;
+            if (pinv_[i] < 0) npl_l_.push_back(i);
+            else piv_c_.push_back(i);
+        }
         if (open0 >= 0 && sn_[open0].w > we_open && ws0_.smark[open0] != stamp2) {
             for (int64_t q = ext_off[ci]; q < ext_off[ci + 1]; q++)
                 if (ext_s[q] == open0) { top = dfs(open0, sn_[open0].c0 + we_open, top, stamp2, open_, &ncand); break; }
         }
-        for (int64_t q = 0; q < n0; q++) {
-            const int64_t i = ws0_.cand[q], k = pinv_[i];
-            if (k < 0) continue;
+        for (int64_t i : piv_c_) {
+            const int64_t k = pinv_[i];
             const int64_t s = col_sn_[k];
             if (ws0_.smark[s] == stamp2) { if (k < ws0_.seg0[s]) ws0_.seg0[s] = k; continue; }
             top = dfs(s, k, top, stamp2, open_, &ncand);
         }
+        npl_l_.insert(npl_l_.end(), ws0_.cand.begin(), ws0_.cand.begin() + ncand);
         const int64_t pivrow = lrows[ci];
         const double tq1 = now_s();
         // U(:, j): pre-panel segments, within-panel segments, the diagonal -- each a run of consecutive rows whose values
@@ -530,10 +542,8 @@ int Driver::panel(int64_t j0, int64_t j1)
         Urp_[j + 1] = (int64_t)Ur_row_.size();
         const double tq2 = now_s();
         // L(:, j): join the open supernode (exactly or relaxed), else open a new one
-        std::vector<int64_t>& npv = miss_;   // reused: the candidates still non-pivotal, then the missing rows
-        std::vector<int64_t> npl_l;
-        npl_l.reserve(ncand);
-        for (int64_t q = 0; q < ncand; q++) if (pinv_[ws0_.cand[q]] < 0) npl_l.push_back(ws0_.cand[q]);
+        std::vector<int64_t>& npv = miss_;   // reused: the missing rows
+        const std::vector<int64_t>& npl_l = npl_l_;   // the candidates still non-pivotal
         int join = 0;
         int64_t n_miss = 0;
         if (open_ >= 0) {
