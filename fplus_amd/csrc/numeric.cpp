@@ -195,6 +195,10 @@ struct fplus_numeric {
     std::vector<int32_t> blk_lf, blk_ub, blk_utf, blk_ltb;
     // L y = b: every block's rows below have one contribution slot each; a row gathers the slots that hold its updates
     DevBuf<int64_t> d_slot_off, d_incp, d_incs;
+    // the pivot rows of every L level's blocks (the rows the level gathers), level offsets
+    std::vector<int32_t> lev_gr;
+    DevBuf<int32_t> d_grows;
+    std::vector<double> bytes_lf_gather;
     int64_t nslots = 0;
     std::vector<int32_t> lev_lft, lev_ubt;                   // level offsets into the push / dot-product tiles
     DevBuf<int2> d_lft, d_ubt;
@@ -363,6 +367,25 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         up64(num->d_slot_off, slot_off);
         up64(num->d_incp, incp);
         up64(num->d_incs, incs);
+        {   // the rows each L level gathers: the pivot rows of its blocks
+            std::vector<int32_t> grows;
+            const size_t nlev0 = num->lev_lf.size() - 1;
+            num->lev_gr.assign(nlev0 + 1, 0);
+            num->bytes_lf_gather.assign(nlev0, 0.0);
+            for (size_t L = 0; L < nlev0; L++) {
+                for (int32_t k2 = num->lev_lf[L]; k2 < num->lev_lf[L + 1]; k2++) {
+                    const BlockDesc& B = F.blocks[num->blk_lf[k2]];
+                    for (int32_t c = 0; c < B.w; c++) {
+                        grows.push_back(B.c0 + c);
+                        num->bytes_lf_gather[L] += (double)(incp[B.c0 + c + 1] - incp[B.c0 + c]) * (8.0 + R0 * 8.0 * F.vs);
+                    }
+                }
+                num->lev_gr[L + 1] = (int32_t)grows.size();
+            }
+            num->d_grows.alloc(grows.size() > 0 ? grows.size() : 1);
+            num->d_grows.upload(grows.data(), grows.size(), s);
+            dev_sync(s);
+        }
         // push tiles per level (256 rows each), and the bytes of every level of the four sweeps
         std::vector<int2> tiles;
         const size_t nlev = num->lev_lf.size() - 1;
@@ -449,8 +472,10 @@ void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, hi
     if (trans == 0) {
         num->d_contrib.reserve((size_t)(num->nslots > 0 ? num->nslots : 1) * R * vs);
         for (size_t L = 0; L + 1 < num->lev_lf.size(); L++) {
-            k::lfwd_pull(vs, num->lev_lf[L + 1] - num->lev_lf[L], num->d_blk_lf.p + num->lev_lf[L], blocks, num->d_incp.p,
-                         num->d_incs.p, num->d_contrib.p, w, R, num->bytes_lf_pull[L], s);
+            k::lfwd_gather(vs, num->lev_gr[L + 1] - num->lev_gr[L], num->d_grows.p + num->lev_gr[L], num->d_incp.p, num->d_incs.p,
+                           num->d_contrib.p, w, R, num->bytes_lf_gather[L], s);
+            k::lfwd_pull(vs, num->lev_lf[L + 1] - num->lev_lf[L], num->d_blk_lf.p + num->lev_lf[L], blocks, w, R,
+                         num->bytes_lf_pull[L], s);
             k::lfwd_push(vs, num->lev_lft[L + 1] - num->lev_lft[L], num->d_lft.p + num->lev_lft[L], blocks, num->d_slot_off.p, w,
                          num->d_contrib.p, R, num->bytes_lf_push[L], s);
         }

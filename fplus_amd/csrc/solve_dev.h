@@ -11,8 +11,11 @@ namespace k {
 // x: n x R row-major (x[i * R + r]), indexed by pivot order; R <= solve_rhs_per_pass(vs).  blk: the blocks of one
 // level.  bytes: the algorithmic bytes of the launch (for the optional per-kernel timing).
 int solve_rhs_per_pass(int vs);
-void lfwd_pull(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int64_t* incp, const int64_t* incs,
-               const double* contrib, double* x, int32_t R, double bytes, hipStream_t s);
+// rows[k] (the pivot rows of a level's blocks) subtract the contributions in their slots: x_i -= sum over incs[incp[i] ..]
+void lfwd_gather(int vs, int32_t nrow, const int32_t* rows, const int64_t* incp, const int64_t* incs, const double* contrib,
+                 double* x, int32_t R, double bytes, hipStream_t s);
+// the unit-lower triangles of a level's blocks
+void lfwd_pull(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, double* x, int32_t R, double bytes, hipStream_t s);
 void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* slot_off, const double* x,
                double* contrib, int32_t R, double bytes, hipStream_t s);
 // tiles (block, first row) of 4 rows: row c0 + t of a block subtracts the dot product over the block's first pcnt[c0 + t]

@@ -5,8 +5,9 @@
 // R right-hand sides, so one entry of a factor meets all R values of a row in one contiguous read.  A sweep walks the
 // blocks level by level (a level's blocks are independent), one launch per level, and every sweep pulls: each value is
 // written by one thread, no atomics, so the results are the same on every run.
-//   lfwd_pull   L y = b   a block's pivot rows gather the contributions the blocks before them left in their slots,
-//                         then the block's unit-lower triangle;
+//   lfwd_gather L y = b   a level's pivot rows gather the contributions the blocks before them left in their slots
+//                         (one wavefront per row);
+//   lfwd_pull   the block's unit-lower triangle;
 //   lfwd_push   the block's rows below: contribution[slot + r] = L(r, block) y(block) (the slot is the row's own);
 //   ubwd        U x = y   a block's pivot rows: dot products along their U rows beyond the block (the runs of later
 //                         U columns over the block's rows, ordered by first row),
@@ -39,34 +40,116 @@ __device__ inline T wave_sum(T v)
 }
 
 // L y = b, part 1 (one workgroup per block of the level): pull the contributions into the pivot rows, solve the triangle
+// L y = b, part 0: the pivot rows of a level's blocks subtract the contributions waiting in their slots, one wavefront
+// per row: lane = 4 g + q, group g (of 16) summing the row's slots g, g + 16, ... for the right-hand sides q, q + 4,
+// ...; the groups' sums combined in a fixed tree
 template <class T>
-__global__ __launch_bounds__(TB) void k_lfwd_pull(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
-                                                   const int64_t* __restrict__ incp, const int64_t* __restrict__ incs,
-                                                   const double* __restrict__ contribd, double* __restrict__ xd, int32_t R)
+__global__ __launch_bounds__(TB) void k_lfwd_gather(int32_t nrow, const int32_t* __restrict__ rows, const int64_t* __restrict__ incp,
+                                                     const int64_t* __restrict__ incs, const double* __restrict__ contribd,
+                                                     double* __restrict__ xd, int32_t R)
 {
-    __shared__ T ys[TB * RM<T>::v];
-    const BlockDesc B = blocks[blk[blockIdx.x]];
-    const T* V = reinterpret_cast<const T*>(B.vals);
+    constexpr int RQ = RM<T>::v / 4;
     const T* contrib = reinterpret_cast<const T*>(contribd);
     T* x = reinterpret_cast<T*>(xd);
-    const int w = B.w;
-    const int64_t c0 = B.c0;
-    for (int idx = threadIdx.x; idx < w * R; idx += TB) {
-        const int t = idx / R, r = idx % R;
-        const int64_t i = c0 + t;
-        T acc = x[i * R + r];
-        for (int64_t q = incp[i]; q < incp[i + 1]; q++) acc = Sc<T>::sub(acc, contrib[incs[q] * R + r]);
-        ys[t * R + r] = acc;
+    const int lane = threadIdx.x & 63, g = lane >> 2, q = lane & 3;
+    const int32_t it = blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
+    if (it >= nrow) return;
+    const int64_t i = rows[it], q0 = incp[i], q1 = incp[i + 1];
+    T acc[RQ];
+#pragma unroll
+    for (int u = 0; u < RQ; u++) acc[u] = Sc<T>::zero();
+    for (int64_t p = q0 + g; p < q1; p += 16) {
+        const int64_t sl = incs[p] * R;
+#pragma unroll
+        for (int u = 0; u < RQ; u++) {
+            const int r = q + 4 * u;
+            if (r < R) acc[u] = Sc<T>::add(acc[u], contrib[sl + r]);
+        }
     }
-    __syncthreads();
-    for (int c = 0; c < w - 1; c++) {
-        for (int idx = threadIdx.x; idx < (w - c - 1) * R; idx += TB) {
-            const int t = c + 1 + idx / R, r = idx % R;
-            ys[t * R + r] = Sc<T>::fms(ys[t * R + r], V[(int64_t)c * B.nrows + t], ys[c * R + r]);
+#pragma unroll
+    for (int u = 0; u < RQ; u++) {
+        T v = acc[u];
+        for (int off = 32; off >= 4; off >>= 1) {
+            if constexpr (Sc<T>::vs == 1) v += __shfl_xor(v, off);
+            else { v.x += __shfl_xor(v.x, off); v.y += __shfl_xor(v.y, off); }
+        }
+        const int r = q + 4 * u;
+        if (g == 0 && r < R) x[i * R + r] = Sc<T>::sub(x[i * R + r], v);
+    }
+}
+
+// the diagonal triangles of the solves, one workgroup per block: thread t owns the block's pivot row t (at most 256:
+// a block is at most one panel wide) with all right-hand sides in registers.  The rows go in chunks of 16: a chunk's
+// triangle inside its wavefront (the row just finished broadcast by a lane shuffle), one barrier, then the rows after
+// the chunk apply its 16 columns.  Each row receives its updates in the column order of the column-by-column sweep.
+// The factor's values of the next chunk are read while the current one is solved.
+template <class T> __device__ inline T shfl_sc(T v, int src);
+template <> __device__ inline double shfl_sc<double>(double v, int src) { return __shfl(v, src); }
+template <> __device__ inline Zd shfl_sc<Zd>(Zd v, int src) { return zmake(__shfl(v.x, src), __shfl(v.y, src)); }
+
+// L y = b: the block's unit-lower triangle, chunks first to last
+template <class T>
+__global__ __launch_bounds__(TB) void k_lfwd_pull(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
+                                                   double* __restrict__ xd, int32_t R)
+{
+    constexpr int RC = RM<T>::v, CH = 16;
+    __shared__ T xs[TB][RC];
+    const BlockDesc B = blocks[blk[blockIdx.x]];
+    const T* V = reinterpret_cast<const T*>(B.vals);
+    T* x = reinterpret_cast<T*>(xd);
+    const int w = B.w, t = threadIdx.x;
+    const bool own = t < w;
+    T y[RC];
+#pragma unroll
+    for (int r = 0; r < RC; r++) y[r] = (own && r < R) ? x[((int64_t)B.c0 + t) * R + r] : Sc<T>::zero();
+    auto fetch = [&](int ch, T* lv) {
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int k = ch * CH + u;
+            lv[u] = (own && k < t) ? V[(int64_t)k * B.nrows + t] : Sc<T>::zero();
+        }
+    };
+    T lcur[CH], lnxt[CH];
+    fetch(0, lcur);
+    const int nch = (w + CH - 1) / CH;
+    for (int ch = 0; ch < nch; ch++) {
+        const int r0 = ch * CH, r1 = min(w, r0 + CH);
+        if (ch + 1 < nch) fetch(ch + 1, lnxt);
+        if (t >= r0 && t < r1) {
+#pragma unroll
+            for (int u = 0; u < CH; u++) {
+                const int k = r0 + u;
+                if (k >= r1) break;
+                T yk[RC];
+#pragma unroll
+                for (int r = 0; r < RC; r++) yk[r] = shfl_sc<T>(y[r], k & 63);
+                if (t > k) {
+#pragma unroll
+                    for (int r = 0; r < RC; r++)
+                        if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], yk[r]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < RC; r++) xs[t][r] = y[r];
         }
         __syncthreads();
+        if (own && t >= r1) {
+#pragma unroll
+            for (int u = 0; u < CH; u++) {
+                const int k = r0 + u;
+                if (k >= r1) break;
+#pragma unroll
+                for (int r = 0; r < RC; r++)
+                    if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], xs[k][r]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) lcur[u] = lnxt[u];
     }
-    for (int idx = threadIdx.x; idx < w * R; idx += TB) x[(c0 + idx / R) * R + idx % R] = ys[idx];
+    if (!own) return;
+#pragma unroll
+    for (int r = 0; r < RC; r++)
+        if (r < R) x[((int64_t)B.c0 + t) * R + r] = y[r];
 }
 
 // L y = b, part 2: the rows below a block, 256 rows per workgroup; slot_off[b] is the block's first slot
@@ -136,30 +219,78 @@ __global__ __launch_bounds__(TB) void k_ub_dots(const int2* __restrict__ tiles, 
 }
 
 // U x = y, part 2 (one workgroup per block): the block's upper triangle, dense in the block's pivot rows (the panel
-// LU left U(block, block) above the block's diagonal), thread t owning row t, last column first
+// LU left U(block, block) above the block's diagonal), chunks last to first; within a chunk the last row first, each
+// scaled by the reciprocal of its diagonal before it is broadcast
 template <class T>
 __global__ __launch_bounds__(TB) void k_ub_tri(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
                                                 double* __restrict__ xd, int32_t R)
 {
-    __shared__ T ys[TB * RM<T>::v];
+    constexpr int RC = RM<T>::v, CH = 16;
+    __shared__ T xs[TB][RC];
     const BlockDesc B = blocks[blk[blockIdx.x]];
     const T* V = reinterpret_cast<const T*>(B.vals);
     T* x = reinterpret_cast<T*>(xd);
-    const int w = B.w;
-    const int64_t c0 = B.c0;
-    for (int idx = threadIdx.x; idx < w * R; idx += TB) ys[idx] = x[(c0 + idx / R) * R + idx % R];
-    __syncthreads();
-    for (int c = w - 1; c >= 0; c--) {
-        const T d = V[(int64_t)c * B.nrows + c];
-        if ((int)threadIdx.x < R) ys[c * R + threadIdx.x] = Sc<T>::div(ys[c * R + threadIdx.x], d);
-        __syncthreads();
-        for (int idx = threadIdx.x; idx < c * R; idx += TB) {
-            const int t = idx / R, r = idx % R;
-            ys[t * R + r] = Sc<T>::fms(ys[t * R + r], V[(int64_t)c * B.nrows + t], ys[c * R + r]);
+    const int w = B.w, t = threadIdx.x;
+    const bool own = t < w;
+    T y[RC];
+#pragma unroll
+    for (int r = 0; r < RC; r++) y[r] = (own && r < R) ? x[((int64_t)B.c0 + t) * R + r] : Sc<T>::zero();
+    const T d = own ? V[(int64_t)t * B.nrows + t] : Sc<T>::one();
+    // U(t, k) for the chunk's columns k > t
+    auto fetch = [&](int ch, T* lv) {
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int k = ch * CH + u;
+            lv[u] = (own && k > t && k < w) ? V[(int64_t)k * B.nrows + t] : Sc<T>::zero();
+        }
+    };
+    const int nch = (w + CH - 1) / CH;
+    T lcur[CH], lnxt[CH];
+    fetch(nch - 1, lcur);
+    for (int ch = nch - 1; ch >= 0; ch--) {
+        const int r0 = ch * CH, r1 = min(w, r0 + CH);
+        if (ch > 0) fetch(ch - 1, lnxt);
+        if (t >= r0 && t < r1) {
+#pragma unroll
+            for (int u = CH - 1; u >= 0; u--) {
+                const int k = r0 + u;
+                if (k >= r1) continue;
+                if (t == k) {
+                    const T dinv = Sc<T>::div(Sc<T>::one(), d);
+#pragma unroll
+                    for (int r = 0; r < RC; r++)
+                        if (r < R) y[r] = Sc<T>::mul(y[r], dinv);
+                }
+                T yk[RC];
+#pragma unroll
+                for (int r = 0; r < RC; r++) yk[r] = shfl_sc<T>(y[r], k & 63);
+                if (t < k) {
+#pragma unroll
+                    for (int r = 0; r < RC; r++)
+                        if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], yk[r]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < RC; r++) xs[t][r] = y[r];
         }
         __syncthreads();
+        if (own && t < r0) {
+#pragma unroll
+            for (int u = CH - 1; u >= 0; u--) {
+                const int k = r0 + u;
+                if (k >= r1) continue;
+#pragma unroll
+                for (int r = 0; r < RC; r++)
+                    if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], xs[k][r]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) lcur[u] = lnxt[u];
     }
-    for (int idx = threadIdx.x; idx < w * R; idx += TB) x[(c0 + idx / R) * R + idx % R] = ys[idx];
+    if (!own) return;
+#pragma unroll
+    for (int r = 0; r < RC; r++)
+        if (r < R) x[((int64_t)B.c0 + t) * R + r] = y[r];
 }
 
 // U^T y = b (U^H with conj), one workgroup per block
@@ -296,13 +427,21 @@ inline unsigned gridn(int64_t n)
 namespace k {
 int solve_rhs_per_pass(int vs) { return vs == 1 ? RM<double>::v : RM<Zd>::v; }
 
-void lfwd_pull(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int64_t* incp, const int64_t* incs,
-               const double* contrib, double* x, int32_t R, double bytes, hipStream_t s)
+void lfwd_gather(int vs, int32_t nrow, const int32_t* rows, const int64_t* incp, const int64_t* incs, const double* contrib,
+                 double* x, int32_t R, double bytes, hipStream_t s)
+{
+    if (nrow <= 0) return;
+    ProfScope ps("lfwd_gather", bytes, s);
+    const unsigned g = (unsigned)((nrow + TB / 64 - 1) / (TB / 64));
+    if (vs == 1) hipLaunchKernelGGL(k_lfwd_gather<double>, dim3(g), dim3(TB), 0, s, nrow, rows, incp, incs, contrib, x, R);
+    else hipLaunchKernelGGL(k_lfwd_gather<Zd>, dim3(g), dim3(TB), 0, s, nrow, rows, incp, incs, contrib, x, R);
+}
+void lfwd_pull(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, double* x, int32_t R, double bytes, hipStream_t s)
 {
     if (nblk <= 0) return;
     ProfScope ps("lfwd_pull", bytes, s);
-    if (vs == 1) hipLaunchKernelGGL(k_lfwd_pull<double>, dim3(nblk), dim3(TB), 0, s, blk, blocks, incp, incs, contrib, x, R);
-    else hipLaunchKernelGGL(k_lfwd_pull<Zd>, dim3(nblk), dim3(TB), 0, s, blk, blocks, incp, incs, contrib, x, R);
+    if (vs == 1) hipLaunchKernelGGL(k_lfwd_pull<double>, dim3(nblk), dim3(TB), 0, s, blk, blocks, x, R);
+    else hipLaunchKernelGGL(k_lfwd_pull<Zd>, dim3(nblk), dim3(TB), 0, s, blk, blocks, x, R);
 }
 void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* slot_off, const double* x,
                double* contrib, int32_t R, double bytes, hipStream_t s)
