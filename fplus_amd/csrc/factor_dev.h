@@ -37,7 +37,7 @@ struct TrsmTile {
 };
 
 struct GemmTile {
-    int32_t task, row0, col0, pad;   // rows [row0, row0 + 64) of the block, columns [col0, col0 + 64) of the task's list
+    int32_t task, row0, col0, pad;   // rows [row0, row0 + tile rows) of the block, columns [col0, col0 + 64) of the task's list
 };
 
 // one run of U entries out of the panel workspace: Ui[uoff + t] = row0 + t, Ux[uoff + t] = W[woff + t], t < len
@@ -71,8 +71,10 @@ void blk_trsm(int vs, int32_t ntile, const TrsmTile* tiles, const UpdTask* tasks
               const int32_t* rowmap, double* W, int64_t ldw, double* xbuf, double flops, hipStream_t s);
 // the rectangles below, subtracted from W through rowmap: atomically when shared_rows (two tasks of the level write a
 // common row), else with plain loads and stores
+// tile_rows: 64, or 32 (tiles of half the rows, for levels with few tiles)
 void blk_gemm(int vs, int32_t ntile, const GemmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
-              const int32_t* rowmap, double* W, int64_t ldw, const double* xbuf, int shared_rows, double flops, hipStream_t s);
+              const int32_t* rowmap, double* W, int64_t ldw, const double* xbuf, int shared_rows, int tile_rows, double flops,
+              hipStream_t s);
 // dense partial-pivot LU of the panel's lower section (nr x P, column-major, ld), sub-panel [a, a + nb): a cooperative
 // launch over the rows (coop: getf2_coop_bytes(vs) of scratch, counter: one int), one workgroup for very tall panels
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,

@@ -155,14 +155,15 @@ __device__ inline dx4 mfma16(double a, double b, dx4 c) { return __builtin_amdgc
 // the rectangle below a block's segment for a 64 x 64 tile: W[rows] -= L[rows, seg] X[seg, cols] through rowmap, with
 // the product on the matrix cores (four real products per complex one), k in chunks of 16 staged in LDS, the next
 // chunk loaded while the current one multiplies
-template <class T>
+template <class T, int TM>
 __global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ tiles, const UpdTask* __restrict__ tasks,
                                                   const int32_t* __restrict__ cols, const BlockDesc* __restrict__ blocks,
                                                   const int32_t* __restrict__ rowmap, double* __restrict__ W, int64_t ldw,
                                                   const double* __restrict__ xbuf, int shared_rows)
 {
     constexpr bool CPLX = Sc<T>::vs == 2;
-    constexpr int KC = 16, TM = 64, TN = 64, LPT = KC * TM / TB;   // elements of A (and of B) each thread stages per chunk
+    constexpr int KC = 16, TN = 64, LPA = KC * TM / TB, LPB = KC * TN / TB;   // elements of A and of B each thread stages
+    constexpr int WR = TM / 32, WC = 4 / WR, SJ = TN / WC / 16;              // waves: WR x WC, each 32 x (16 SJ)
     // staging: A[k][row] and B[k][col] for two chunks in flight; at the end the output tile
     __shared__ T stg[2 * KC * TM + 2 * KC * TN];
     T* const As = stg;
@@ -179,32 +180,41 @@ __global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ ti
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     if (tid < TM) rpos[tid] = tid < nr ? rowmap[B.rows[tl.row0 + tid]] : 0;
     else if (tid < TM + TN) cidx[tid - TM] = (tid - TM) < nc ? cols[tk.col_off + tl.col0 + tid - TM] : 0;
-    const int wr = (wv >> 1) * 32, wc = (wv & 1) * 32;   // the wave's 32 x 32 quarter
-    dx4 accr[2][2], acci[2][2];
+    const int wr = (wv / WC) * 32, wc = (wv % WC) * (16 * SJ);   // the wave's part of the tile
+    dx4 accr[2][SJ], acci[2][SJ];
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
+        for (int j = 0; j < SJ; j++) {
             accr[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
             acci[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
         }
     // staging: thread element e = tid + q * TB; A: row e % TM, k e / TM (rows contiguous); B: k e % KC, column e / KC
-    T ra[LPT], rb[LPT];
+    T ra[LPA], rb[LPB];
     auto fetch = [&](int32_t kc) {
         const int32_t kn = min(KC, m - kc);
 #pragma unroll
-        for (int q = 0; q < LPT; q++) {
+        for (int q = 0; q < LPA; q++) {
             const int e = tid + q * TB;
-            const int r = e % TM, ka = e / TM, kb = e % KC, c = e / KC;
+            const int r = e % TM, ka = e / TM;
             ra[q] = (r < nr && ka < kn) ? V[(int64_t)(tk.k0 + kc + ka) * B.nrows + tl.row0 + r] : Sc<T>::zero();
+        }
+#pragma unroll
+        for (int q = 0; q < LPB; q++) {
+            const int e = tid + q * TB;
+            const int kb = e % KC, c = e / KC;
             rb[q] = (c < nc && kb < kn) ? X[(kc + kb) + (int64_t)(tl.col0 + c) * m] : Sc<T>::zero();
         }
     };
     auto stage = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < LPT; q++) {
+        for (int q = 0; q < LPA; q++) {
             const int e = tid + q * TB;
             As[(buf * KC + e / TM) * TM + e % TM] = ra[q];
+        }
+#pragma unroll
+        for (int q = 0; q < LPB; q++) {
+            const int e = tid + q * TB;
             Bs[(buf * KC + e % KC) * TN + e / KC] = rb[q];
         }
     };
@@ -218,15 +228,15 @@ __global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ ti
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 4) {
             const int kk = ks + (lane >> 4);
-            T a[2], b[2];
+            T a[2], b[SJ];
 #pragma unroll
             for (int i = 0; i < 2; i++) a[i] = As[(buf * KC + kk) * TM + wr + i * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < 2; j++) b[j] = Bs[(buf * KC + kk) * TN + wc + j * 16 + (lane & 15)];
+            for (int j = 0; j < SJ; j++) b[j] = Bs[(buf * KC + kk) * TN + wc + j * 16 + (lane & 15)];
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
-                for (int j = 0; j < 2; j++) {
+                for (int j = 0; j < SJ; j++) {
                     if constexpr (CPLX) {
                         accr[i][j] = mfma16(a[i].x, b[j].x, accr[i][j]);
                         accr[i][j] = mfma16(-a[i].y, b[j].y, accr[i][j]);
@@ -249,7 +259,7 @@ __global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ ti
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
+        for (int j = 0; j < SJ; j++) {
             const int c = wc + j * 16 + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; v++) {
@@ -261,10 +271,10 @@ __global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ ti
             }
         }
     __syncthreads();
-    const int r = tid & 63;
+    const int r = tid % TM;
     if (r < nr) {
         const int64_t rp = rpos[r];
-        for (int c = tid >> 6; c < nc; c += TB / 64) {
+        for (int c = tid / TM; c < nc; c += TB / TM) {
             T* p = reinterpret_cast<T*>(W) + rp + (int64_t)cidx[c] * ldw;
             const T val = Cs[c * TM + ((r + c) & (TM - 1))];
             if (shared_rows) {
@@ -644,13 +654,20 @@ void blk_trsm(int vs, int32_t ntile, const TrsmTile* tiles, const UpdTask* tasks
     else hipLaunchKernelGGL(k_blk_trsm<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf);
 }
 void blk_gemm(int vs, int32_t ntile, const GemmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
-              const int32_t* rowmap, double* W, int64_t ldw, const double* xbuf, int shared_rows, double flops, hipStream_t s)
+              const int32_t* rowmap, double* W, int64_t ldw, const double* xbuf, int shared_rows, int tile_rows, double flops,
+              hipStream_t s)
 {
     if (ntile <= 0) return;
     ProfScope ps("blk_gemm", flops, s);
-    if (vs == 1)
-        hipLaunchKernelGGL(k_blk_gemm<double>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
-    else hipLaunchKernelGGL(k_blk_gemm<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
+    if (tile_rows == 32) {
+        if (vs == 1)
+            hipLaunchKernelGGL((k_blk_gemm<double, 32>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
+        else hipLaunchKernelGGL((k_blk_gemm<Zd, 32>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
+    } else {
+        if (vs == 1)
+            hipLaunchKernelGGL((k_blk_gemm<double, 64>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
+        else hipLaunchKernelGGL((k_blk_gemm<Zd, 64>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
+    }
 }
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
            double pivot_rel, int32_t* piv, PanelStatus* st, void* coop, int32_t* counter, hipStream_t s)

@@ -391,6 +391,17 @@ int Driver::panel(int64_t j0, int64_t j1)
         }
     }
     std::stable_sort(tl.begin(), tl.end(), [](const TaskL& a, const TaskL& b) { return a.level < b.level; });
+    // tile rows per level: 64, or 32 where the level's 64-row tiles would not fill the device twice
+    std::vector<int32_t> lev_rt(maxlevel + 2, RT);
+    {
+        std::vector<int64_t> nt64(maxlevel + 2, 0);
+        for (const TaskL& x : tl) {
+            const BlockDesc& B = F_.blocks[x.t.block];
+            nt64[x.level] += (int64_t)((B.nrows - B.w + RT - 1) / RT) * ((x.t.ncol + CTG - 1) / CTG);
+        }
+        const int64_t fill = 2 * (int64_t)devctx().ncu;
+        for (int32_t L = 0; L <= maxlevel + 1; L++) lev_rt[L] = nt64[L] < fill ? RT / 2 : RT;
+    }
     std::vector<UpdTask> tasks(tl.size());
     std::vector<int32_t> lev_tt(maxlevel + 2, 0), lev_gt(maxlevel + 2, 0);
     std::vector<double> lev_gflops(maxlevel + 2, 0.0), lev_tflops(maxlevel + 2, 0.0);
@@ -409,7 +420,8 @@ int Driver::panel(int64_t j0, int64_t j1)
         lev_gflops[L] += fm * (double)T.ncol * (double)m * (double)(B.nrows - B.w);
         for (int32_t cc = 0; cc < T.ncol; cc += TC) ttiles.push_back(TrsmTile{(int32_t)i, cc});
         lev_tt[L + 1] = (int32_t)ttiles.size();
-        for (int32_t r0 = B.w; r0 < B.nrows; r0 += RT)
+        const int32_t rt = lev_rt[L];
+        for (int32_t r0 = B.w; r0 < B.nrows; r0 += rt)
             for (int32_t cc = 0; cc < T.ncol; cc += CTG) gtiles.push_back(GemmTile{(int32_t)i, r0, cc, 0});
         lev_gt[L + 1] = (int32_t)gtiles.size();
     }
@@ -450,7 +462,7 @@ int Driver::panel(int64_t j0, int64_t j1)
                         lev_tflops[L], s_);
             const int32_t g0 = lev_gt[L], g1 = lev_gt[L + 1];
             k::blk_gemm(vs_, g1 - g0, d_tiles_.p + g0, d_tasks_.p, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, d_X_.p,
-                        lev_shared[L], lev_gflops[L], s_);
+                        lev_shared[L], lev_rt[L], lev_gflops[L], s_);
         }
     }
     double* Wl = d_W_.p + Ru * vs_;
