@@ -11,7 +11,7 @@
 namespace fpa {
 
 inline int col_tile(int vs) { return vs == 1 ? 64 : 32; }   // panel columns per tile of the panel-LU GEMM
-inline int trsm_cols(int vs) { return vs == 1 ? 4 : 2; }    // panel columns per workgroup of blk_trsm
+inline int trsm_cols(int vs) { return vs == 1 ? 2 : 1; }    // panel columns per workgroup of blk_trsm
 
 // A block: the columns [c0, c0 + w) of one supernode computed in one panel, stored column-major over its rows (ld =
 // nrows): the pivot rows of its columns first, in column order, then the supernode's other rows as they stood at the

@@ -57,8 +57,8 @@ constexpr int RT = FPA_ROW_TILE; // rows per GEMM tile
 // panel columns per tile of the panel-LU GEMM: 64 real, 32 complex (LDS within 64 KB)
 template <class T> struct CTile { static constexpr int v = 64; };
 template <> struct CTile<Zd> { static constexpr int v = 32; };
-template <class T> struct TCols { static constexpr int v = 4; };   // columns per blk_trsm workgroup
-template <> struct TCols<Zd> { static constexpr int v = 2; };
+template <class T> struct TCols { static constexpr int v = 2; };   // columns per blk_trsm workgroup
+template <> struct TCols<Zd> { static constexpr int v = 1; };
 constexpr int PNB = FPA_PANEL_NB;  // sub-panel width of the dense panel LU
 
 // the unit-lower solve of a block's segment [k0, w) for up to TCols panel columns: thread t owns segment row t (the
