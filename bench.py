@@ -165,7 +165,7 @@ def main():
         L.fplus_numeric_free(nm)
     # the kernels' sums over the timed region
     kern = {}
-    for name in ("blk_gemm", "blk_trsm", "gemm_sub", "getf2", "trsm_sub", "lfwd_gather", "lfwd_pull", "lfwd_push", "ub_dots", "ub_tri", "utfwd",
+    for name in ("blk_gemm", "blk_trsm", "gemm_sub", "getf2", "trsm_sub", "lfwd_gather", "lfwd_pull", "lfwd_push", "ub_dots", "ub_tri", "utdots", "utfwd", "ltdots",
                  "ltbwd", "ruiz_rowmax", "ruiz_scale_colmax", "ruiz_scale_cols"):
         ms, cnt, work = C.c_double(), C.c_int64(), C.c_double()
         L.fplus_amd_profile_get(name.encode(), C.byref(ms), C.byref(cnt), C.byref(work))

@@ -200,8 +200,8 @@ struct fplus_numeric {
     DevBuf<int32_t> d_grows;
     std::vector<double> bytes_lf_gather;
     int64_t nslots = 0;
-    std::vector<int32_t> lev_lft, lev_ubt;                   // level offsets into the push / dot-product tiles
-    DevBuf<int2> d_lft, d_ubt;
+    std::vector<int32_t> lev_lft, lev_ubt, lev_utt, lev_ltt;   // level offsets into the push / dot-product tiles
+    DevBuf<int2> d_lft, d_ubt, d_utt, d_ltt;
     std::vector<double> bytes_ub_tri;
     std::vector<double> bytes_lf_pull, bytes_lf_push, bytes_ub, bytes_utf, bytes_ltb;
     // U x = y beyond the blocks: a block's pieces are the runs of later columns over its rows, each covering the block's
@@ -429,6 +429,22 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         num->d_ubt.alloc(ubt.size() > 0 ? ubt.size() : 1);
         num->d_ubt.upload(ubt.data(), ubt.size(), s);
         const size_t nut = num->lev_utf.size() - 1, nlt = num->lev_ltb.size() - 1;
+        {   // the transposed sweeps' dot-product tiles (4 columns each) per level
+            auto tiles_of = [&](const std::vector<int32_t>& lev, const std::vector<int32_t>& blk, std::vector<int32_t>& lt, DevBuf<int2>& d) {
+                std::vector<int2> tt;
+                lt.assign(lev.size(), 0);
+                for (size_t L = 0; L + 1 < lev.size(); L++) {
+                    for (int32_t k2 = lev[L]; k2 < lev[L + 1]; k2++)
+                        for (int32_t t0 = 0; t0 < F.blocks[blk[k2]].w; t0 += 4) tt.push_back(make_int2(blk[k2], t0));
+                    lt[L + 1] = (int32_t)tt.size();
+                }
+                d.alloc(tt.size() > 0 ? tt.size() : 1);
+                d.upload(tt.data(), tt.size(), s);
+            };
+            tiles_of(num->lev_utf, num->blk_utf, num->lev_utt, num->d_utt);
+            tiles_of(num->lev_ltb, num->blk_ltb, num->lev_ltt, num->d_ltt);
+            dev_sync(s);
+        }
         num->bytes_utf.assign(nut, 0.0);
         num->bytes_ltb.assign(nlt, 0.0);
         for (size_t L = 0; L < nut; L++)
@@ -486,12 +502,18 @@ void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, hi
         }
     } else {
         const int cj = trans == 2 && vs == 2;
-        for (size_t L = 0; L + 1 < num->lev_utf.size(); L++)
+        for (size_t L = 0; L + 1 < num->lev_utf.size(); L++) {
+            k::utdots(vs, num->lev_utt[L + 1] - num->lev_utt[L], num->d_utt.p + num->lev_utt[L], blocks, num->d_Up.p, num->F.d_Ui.p,
+                      F.d_Ux.p, w, R, cj, num->bytes_utf[L], s);
             k::utfwd(vs, num->lev_utf[L + 1] - num->lev_utf[L], num->d_blk_utf.p + num->lev_utf[L], blocks, num->d_Up.p,
-                     num->F.d_Ui.p, F.d_Ux.p, w, R, cj, num->bytes_utf[L], s);
-        for (size_t L = 0; L + 1 < num->lev_ltb.size(); L++)
+                     num->F.d_Ui.p, F.d_Ux.p, w, R, cj, 0.0, s);
+        }
+        for (size_t L = 0; L + 1 < num->lev_ltb.size(); L++) {
+            k::ltdots(vs, num->lev_ltt[L + 1] - num->lev_ltt[L], num->d_ltt.p + num->lev_ltt[L], blocks, num->d_pinv.p, w, R, cj,
+                      num->bytes_ltb[L], s);
             k::ltbwd(vs, num->lev_ltb[L + 1] - num->lev_ltb[L], num->d_blk_ltb.p + num->lev_ltb[L], blocks, num->d_pinv.p, w, R,
-                     cj, num->bytes_ltb[L], s);
+                     cj, 0.0, s);
+        }
     }
 }
 

@@ -23,6 +23,11 @@ void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks
 void ub_dots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* pb, const int32_t* pcnt,
              const int32_t* pj, const int64_t* pu, const double* Ux, double* x, int32_t R, double bytes, hipStream_t s);
 void ub_tri(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, double* x, int32_t R, double bytes, hipStream_t s);
+// the transposed sweeps' dot products (tiles: block, first of 4 columns), before the blocks' triangles
+void utdots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
+            const double* Ux, double* x, int32_t R, int conj, double bytes, hipStream_t s);
+void ltdots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int32_t* pinv_rows, double* x, int32_t R,
+            int conj, double bytes, hipStream_t s);
 void utfwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
            const double* Ux, double* x, int32_t R, int conj, double bytes, hipStream_t s);
 void ltbwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int32_t* pinv_rows, double* x, int32_t R,

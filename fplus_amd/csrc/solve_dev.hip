@@ -293,6 +293,69 @@ __global__ __launch_bounds__(TB) void k_ub_tri(const int32_t* __restrict__ blk, 
         if (r < R) x[((int64_t)B.c0 + t) * R + r] = y[r];
 }
 
+// U^T y = b, part 1: the dot products of a level's block columns over the rows before their block, tiles of 4 columns
+// (one wavefront per column): x_c -= sum_{i < c0} conj?(U(i, c)) x_i
+template <class T>
+__global__ __launch_bounds__(TB) void k_utdots(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
+                                                const int64_t* __restrict__ Up, const int32_t* __restrict__ Ui,
+                                                const double* __restrict__ Uxd, double* __restrict__ xd, int32_t R, int conj)
+{
+    const int2 tl = tiles[blockIdx.x];
+    const BlockDesc B = blocks[tl.x];
+    const T* Ux = reinterpret_cast<const T*>(Uxd);
+    T* x = reinterpret_cast<T*>(xd);
+    const int lane = threadIdx.x & 63, c = tl.y + (threadIdx.x >> 6);
+    if (c >= B.w) return;
+    const int64_t c0 = B.c0, q0 = Up[c0 + c], qd = Up[c0 + c + 1] - 1;
+    T acc[RM<T>::v];
+#pragma unroll
+    for (int r = 0; r < RM<T>::v; r++) acc[r] = Sc<T>::zero();
+    for (int64_t q = q0 + lane; q < qd; q += 64) {
+        const int64_t i = Ui[q];
+        if (i >= c0) continue;
+        const T u = cj(Ux[q], conj);
+#pragma unroll
+        for (int r = 0; r < RM<T>::v; r++)
+            if (r < R) acc[r] = Sc<T>::fma_(acc[r], u, x[i * R + r]);
+    }
+#pragma unroll
+    for (int r = 0; r < RM<T>::v; r++) {
+        if (r >= R) break;
+        const T sm = wave_sum(acc[r]);
+        if (lane == 0) x[(c0 + c) * R + r] = Sc<T>::sub(x[(c0 + c) * R + r], sm);
+    }
+}
+
+// L^T x = y, part 1: the dot products of a level's block columns over the block's rows below, tiles of 4 columns:
+// x_c -= sum_{rows i below} conj?(L(i, c)) x_pinv(i)
+template <class T>
+__global__ __launch_bounds__(TB) void k_ltdots(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
+                                                const int32_t* __restrict__ pinv_rows, double* __restrict__ xd, int32_t R, int conj)
+{
+    const int2 tl = tiles[blockIdx.x];
+    const BlockDesc B = blocks[tl.x];
+    const T* V = reinterpret_cast<const T*>(B.vals);
+    T* x = reinterpret_cast<T*>(xd);
+    const int lane = threadIdx.x & 63, c = tl.y + (threadIdx.x >> 6), w = B.w;
+    if (c >= w) return;
+    T acc[RM<T>::v];
+#pragma unroll
+    for (int r = 0; r < RM<T>::v; r++) acc[r] = Sc<T>::zero();
+    for (int32_t i = w + lane; i < B.nrows; i += 64) {
+        const T v = cj(V[(int64_t)c * B.nrows + i], conj);
+        const int64_t pi = pinv_rows[B.rows[i]];
+#pragma unroll
+        for (int r = 0; r < RM<T>::v; r++)
+            if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, x[pi * R + r]);
+    }
+#pragma unroll
+    for (int r = 0; r < RM<T>::v; r++) {
+        if (r >= R) break;
+        const T sm = wave_sum(acc[r]);
+        if (lane == 0) x[(B.c0 + c) * R + r] = Sc<T>::sub(x[(B.c0 + c) * R + r], sm);
+    }
+}
+
 // U^T y = b (U^H with conj), one workgroup per block
 template <class T>
 __global__ __launch_bounds__(TB) void k_utfwd(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
@@ -301,47 +364,78 @@ __global__ __launch_bounds__(TB) void k_utfwd(const int32_t* __restrict__ blk, c
 {
     __shared__ T ys[TB * RM<T>::v];
     const BlockDesc B = blocks[blk[blockIdx.x]];
-    const T* Ux = reinterpret_cast<const T*>(Uxd);
     T* x = reinterpret_cast<T*>(xd);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, w = B.w;
+    const int w = B.w;
     const int64_t c0 = B.c0;
-    // the dot products over rows before the block: one wavefront per column
-    for (int c = wid; c < w; c += TB / 64) {
-        const int64_t q0 = Up[c0 + c], qd = Up[c0 + c + 1] - 1;
-        T acc[RM<T>::v];
-#pragma unroll
-        for (int r = 0; r < RM<T>::v; r++) acc[r] = Sc<T>::zero();
-        for (int64_t q = q0 + lane; q < qd; q += 64) {
-            const int64_t i = Ui[q];
-            if (i >= c0) continue;
-            const T u = cj(Ux[q], conj);
-#pragma unroll
-            for (int r = 0; r < RM<T>::v; r++)
-                if (r < R) acc[r] = Sc<T>::fma_(acc[r], u, x[i * R + r]);
-        }
-#pragma unroll
-        for (int r = 0; r < RM<T>::v; r++) {
-            if (r >= R) break;
-            const T s = wave_sum(acc[r]);
-            if (lane == 0) ys[c * R + r] = Sc<T>::sub(x[(c0 + c) * R + r], s);
-        }
-    }
+    // (the dot products over the rows before the block are done: k_utdots)
+    for (int idx = threadIdx.x; idx < w * R; idx += TB) ys[idx] = x[c0 * R + idx];
     __syncthreads();
-    // the block's own rows, column by column; thread r handles right-hand side r
-    for (int c = 0; c < w; c++) {
-        const int64_t q0 = Up[c0 + c], qd = Up[c0 + c + 1] - 1;
-        if ((int)threadIdx.x < R) {
-            const int r = threadIdx.x;
-            T acc = ys[c * R + r];
-            for (int64_t q = q0; q < qd; q++) {
-                const int64_t i = Ui[q];
-                if (i >= c0) acc = Sc<T>::fms(acc, cj(Ux[q], conj), ys[(i - c0) * R + r]);
+    // the block's own rows: thread c owns column c, chunks of 16 first to last, within a chunk the first row first,
+    // each scaled by the reciprocal of its diagonal before it is broadcast; U(k, c) = V[c * nrows + k] for k < c (the
+    // block's upper part)
+    constexpr int RC = RM<T>::v, CH = 16;
+    const T* V = reinterpret_cast<const T*>(B.vals);
+    const int t = threadIdx.x;
+    const bool own = t < w;
+    T y[RC];
+#pragma unroll
+    for (int r = 0; r < RC; r++) y[r] = (own && r < R) ? ys[t * R + r] : Sc<T>::zero();
+    const T dinv = own ? Sc<T>::div(Sc<T>::one(), cj(V[(int64_t)t * B.nrows + t], conj)) : Sc<T>::one();
+    auto fetch = [&](int ch, T* lv) {
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int k = ch * CH + u;
+            lv[u] = (own && k < t) ? cj(V[(int64_t)t * B.nrows + k], conj) : Sc<T>::zero();
+        }
+    };
+    const int nch = (w + CH - 1) / CH;
+    T lcur[CH], lnxt[CH];
+    fetch(0, lcur);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ch++) {
+        const int r0 = ch * CH, r1 = min(w, r0 + CH);
+        if (ch + 1 < nch) fetch(ch + 1, lnxt);
+        if (t >= r0 && t < r1) {
+#pragma unroll
+            for (int u = 0; u < CH; u++) {
+                const int k = r0 + u;
+                if (k >= r1) break;
+                if (t == k) {
+#pragma unroll
+                    for (int r = 0; r < RC; r++)
+                        if (r < R) y[r] = Sc<T>::mul(y[r], dinv);
+                }
+                T yk[RC];
+#pragma unroll
+                for (int r = 0; r < RC; r++) yk[r] = shfl_sc<T>(y[r], k & 63);
+                if (t > k) {
+#pragma unroll
+                    for (int r = 0; r < RC; r++)
+                        if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], yk[r]);
+                }
             }
-            ys[c * R + r] = Sc<T>::div(acc, cj(Ux[qd], conj));
+#pragma unroll
+            for (int r = 0; r < RC; r++)
+                if (r < R) ys[t * R + r] = y[r];
         }
         __syncthreads();
+        if (own && t >= r1) {
+#pragma unroll
+            for (int u = 0; u < CH; u++) {
+                const int k = r0 + u;
+                if (k >= r1) break;
+#pragma unroll
+                for (int r = 0; r < RC; r++)
+                    if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], ys[k * R + r]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) lcur[u] = lnxt[u];
     }
-    for (int idx = threadIdx.x; idx < w * R; idx += TB) x[(c0 + idx / R) * R + idx % R] = ys[idx];
+    if (!own) return;
+#pragma unroll
+    for (int r = 0; r < RC; r++)
+        if (r < R) x[(c0 + t) * R + r] = y[r];
 }
 
 // L^T x = y (L^H with conj), one workgroup per block
@@ -353,37 +447,68 @@ __global__ __launch_bounds__(TB) void k_ltbwd(const int32_t* __restrict__ blk, c
     const BlockDesc B = blocks[blk[blockIdx.x]];
     const T* V = reinterpret_cast<const T*>(B.vals);
     T* x = reinterpret_cast<T*>(xd);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, w = B.w;
-    for (int c = wid; c < w; c += TB / 64) {
-        T acc[RM<T>::v];
-#pragma unroll
-        for (int r = 0; r < RM<T>::v; r++) acc[r] = Sc<T>::zero();
-        for (int32_t i = w + lane; i < B.nrows; i += 64) {
-            const T v = cj(V[(int64_t)c * B.nrows + i], conj);
-            const int64_t pi = pinv_rows[B.rows[i]];
-#pragma unroll
-            for (int r = 0; r < RM<T>::v; r++)
-                if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, x[pi * R + r]);
-        }
-#pragma unroll
-        for (int r = 0; r < RM<T>::v; r++) {
-            if (r >= R) break;
-            const T s = wave_sum(acc[r]);
-            if (lane == 0) ys[c * R + r] = Sc<T>::sub(x[(B.c0 + c) * R + r], s);
-        }
-    }
+    const int w = B.w;
+    // (the dot products over the rows below are done: k_ltdots)
+    for (int idx = threadIdx.x; idx < w * R; idx += TB) ys[idx] = x[B.c0 * R + idx];
     __syncthreads();
-    // the triangle, last column first: x_c = ys_c - sum_{rr > c} L(rr, c) x_rr
-    for (int c = w - 1; c >= 0; c--) {
-        if ((int)threadIdx.x < R) {
-            const int r = threadIdx.x;
-            T acc = ys[c * R + r];
-            for (int rr = c + 1; rr < w; rr++) acc = Sc<T>::fms(acc, cj(V[(int64_t)c * B.nrows + rr], conj), ys[rr * R + r]);
-            ys[c * R + r] = acc;
+    // the triangle: thread c owns column c, chunks of 16 last to first, within a chunk the last row first;
+    // x_c = ys_c - sum_{k > c} L(k, c) x_k with L(k, c) = V[c * nrows + k]
+    constexpr int RC = RM<T>::v, CH = 16;
+    const int t = threadIdx.x;
+    const bool own = t < w;
+    T y[RC];
+#pragma unroll
+    for (int r = 0; r < RC; r++) y[r] = (own && r < R) ? ys[t * R + r] : Sc<T>::zero();
+    auto fetch = [&](int ch, T* lv) {
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int k = ch * CH + u;
+            lv[u] = (own && k > t && k < w) ? cj(V[(int64_t)t * B.nrows + k], conj) : Sc<T>::zero();
+        }
+    };
+    const int nch = (w + CH - 1) / CH;
+    T lcur[CH], lnxt[CH];
+    fetch(nch - 1, lcur);
+    __syncthreads();
+    for (int ch = nch - 1; ch >= 0; ch--) {
+        const int r0 = ch * CH, r1 = min(w, r0 + CH);
+        if (ch > 0) fetch(ch - 1, lnxt);
+        if (t >= r0 && t < r1) {
+#pragma unroll
+            for (int u = CH - 1; u >= 0; u--) {
+                const int k = r0 + u;
+                if (k >= r1) continue;
+                T yk[RC];
+#pragma unroll
+                for (int r = 0; r < RC; r++) yk[r] = shfl_sc<T>(y[r], k & 63);
+                if (t < k) {
+#pragma unroll
+                    for (int r = 0; r < RC; r++)
+                        if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], yk[r]);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < RC; r++)
+                if (r < R) ys[t * R + r] = y[r];
         }
         __syncthreads();
+        if (own && t < r0) {
+#pragma unroll
+            for (int u = CH - 1; u >= 0; u--) {
+                const int k = r0 + u;
+                if (k >= r1) continue;
+#pragma unroll
+                for (int r = 0; r < RC; r++)
+                    if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], ys[k * R + r]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) lcur[u] = lnxt[u];
     }
-    for (int idx = threadIdx.x; idx < w * R; idx += TB) x[(B.c0 + idx / R) * R + idx % R] = ys[idx];
+    if (!own) return;
+#pragma unroll
+    for (int r = 0; r < RC; r++)
+        if (r < R) x[(B.c0 + t) * R + r] = y[r];
 }
 
 // w[k * R + r] = D[perm[k]] * b[perm[k] + r * ldb]
@@ -465,6 +590,22 @@ void ub_tri(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, d
     ProfScope ps("ub_tri", bytes, s);
     if (vs == 1) hipLaunchKernelGGL(k_ub_tri<double>, dim3(nblk), dim3(TB), 0, s, blk, blocks, x, R);
     else hipLaunchKernelGGL(k_ub_tri<Zd>, dim3(nblk), dim3(TB), 0, s, blk, blocks, x, R);
+}
+void utdots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
+            const double* Ux, double* x, int32_t R, int conj, double bytes, hipStream_t s)
+{
+    if (ntile <= 0) return;
+    ProfScope ps("utdots", bytes, s);
+    if (vs == 1) hipLaunchKernelGGL(k_utdots<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, Up, Ui, Ux, x, R, conj);
+    else hipLaunchKernelGGL(k_utdots<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, Up, Ui, Ux, x, R, conj);
+}
+void ltdots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int32_t* pinv_rows, double* x, int32_t R,
+            int conj, double bytes, hipStream_t s)
+{
+    if (ntile <= 0) return;
+    ProfScope ps("ltdots", bytes, s);
+    if (vs == 1) hipLaunchKernelGGL(k_ltdots<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, pinv_rows, x, R, conj);
+    else hipLaunchKernelGGL(k_ltdots<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, pinv_rows, x, R, conj);
 }
 void utfwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
            const double* Ux, double* x, int32_t R, int conj, double bytes, hipStream_t s)
