@@ -5,7 +5,8 @@ analysed order, Ruiz, the supernodal LU) plus one solve of 16 right-hand sides, 
 interface (fplus_amd_factor_device, fplus_amd_solve_device).  The symbolic analysis is done once, before the timed
 region (it is reused across refactorizations of one pattern).  Prints one JSON line.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu] [--verbose]
+    python bench.py --copy-probe      # 3 device-to-device copies of 1 GiB: the HBM counters' calibration run
 
 The line carries the backward error of every right-hand side of the last step, the roofline of the dominant kernel
 (timed with HIP events on the library's stream over the timed region), and a CPU baseline: the reference's own
@@ -46,6 +47,37 @@ DEFAULT_WORKLOAD = "shiftinvert-64x4x1024"   # n = 262144, the plain FP64 GPU dr
 PEAK_FP64_MFMA = 78.6e12      # MI355X dense FP64 matrix, FLOP/s
 PEAK_HBM = 8.0e12             # MI355X HBM3E, B/s
 NRHS = 16
+# the dominant kernel's HBM bytes per launch, from the rocprofv3 FETCH_SIZE and WRITE_SIZE passes summarized there
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blk_gemm-hbm_traffic-shiftinvert-64x4x1024.txt")
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed counter summary, or None"""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    vals = {}
+    for line in open(TRAFFIC_FILE):
+        if "=" in line and not line.startswith(" "):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    if vals.get("workload") != workload or vals.get("kernel") != kernel:
+        return None
+    return float(vals["hbm_bytes_per_launch"])
+
+
+def copy_probe():
+    """three 1 GiB device-to-device copies through the library (the HBM counters' known-size reference)"""
+    L = _lib.lib()
+    nb = 1 << 30
+    src, dst = C.c_void_p(), C.c_void_p()
+    if L.fplus_amd_dev_alloc(nb, C.byref(src)) != 0 or L.fplus_amd_dev_alloc(nb, C.byref(dst)) != 0:
+        raise RuntimeError("device allocation failed")
+    for _ in range(3):
+        if L.fplus_amd_dev_copy(dst, src, nb, 2) != 0:
+            raise RuntimeError("copy failed")
+    L.fplus_amd_dev_free(src)
+    L.fplus_amd_dev_free(dst)
+    print(json.dumps({"copy_probe": {"copies": 3, "bytes_each": nb}}))
 
 
 def dev_upload(a):
@@ -108,18 +140,23 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--verbose", action="store_true", help="the factorization's progress lines (for long profiled runs)")
+    ap.add_argument("--copy-probe", action="store_true", help="only the HBM counters' calibration copies")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the path would shard by "
               "independent systems or elimination subtrees: see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe:
+        copy_probe()
+        return
     fn, kw, desc = WORKLOADS[args.workload]
     n, Ap, Ai, Ax = fn(**kw)
     cplx = np.iscomplexobj(Ax)
     vs = 2 if cplx else 1
     sym = fplus_amd.Symbolic(n, Ap, Ai)
     L = _lib.lib()
-    opts = fplus_amd.default_options()
+    opts = fplus_amd.default_options(verbose=2 if args.verbose else 0)
     d_Ax = dev_upload(np.ascontiguousarray(Ax))
     b = gen.rhs(n, NRHS, cplx)
     bf = np.asfortranarray(b)
@@ -179,8 +216,10 @@ def main():
     peak = PEAK_FP64_MFMA if bound == "mfma" else PEAK_HBM
     roof = {"kernel": dom, "bound": bound, "achieved": achieved / (1e12 if bound == "mfma" else 1e9),
             "peak": peak / (1e12 if bound == "mfma" else 1e9), "unit": "TFLOP/s" if bound == "mfma" else "GB/s",
-            "frac": achieved / peak, "traffic": None, "launches": cnt_dom,
+            "frac": achieved / peak, "traffic": measured_traffic(args.workload, dom), "launches": cnt_dom,
             "avg_launch_ms": 1e3 * t_dom / cnt_dom, "work_per_launch": work_dom / cnt_dom}
+    if dom == "blk_gemm":
+        roof["algorithmic_bytes_per_launch"] = st.gemm_bytes / cnt_dom
     family = args.workload.split("-")[0]
     cpu = None if args.no_cpu else cpu_baseline(st.flops, family)
     out = {
@@ -191,7 +230,7 @@ def main():
         "config": {"workload": args.workload, "matrix": desc, **kw, "n": n, "nnz": int(Ap[-1]), "nrhs": NRHS,
                    "precision": "fp64 (plain GPU drop-in)", "ordering": "fplus_analyze defaults", "relax_rows": 32},
         "backward_error_max": max(bes),
-        "factor": {"flops": st.flops, "gemm_flops": st.gemm_flops, "nnzL": st.nnzL, "nnzU": st.nnzU,
+        "factor": {"flops": st.flops, "gemm_flops": st.gemm_flops, "gemm_bytes": st.gemm_bytes, "nnzL": st.nnzL, "nnzU": st.nnzU,
                    "supernodes": st.nsupernodes, "panels": st.panels, "levels": st.levels,
                    "t_ruiz": st.t_ruiz, "t_factor": st.t_factor, "t_host": st.t_host, "gflops": st.flops / sec / 1e9},
         "kernels": {k: {"s": v[0], "launches": v[1]} for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])},

@@ -44,7 +44,7 @@ class Stats(C.Structure):
     _fields_ = [("nnzL", C.c_int64), ("nnzU", C.c_int64), ("nsupernodes", C.c_int64), ("nblocks", C.c_int64),
                 ("panels", C.c_int64), ("levels", C.c_int64), ("flops", C.c_double), ("gemm_flops", C.c_double),
                 ("t_ruiz", C.c_double), ("t_factor", C.c_double), ("t_host", C.c_double), ("t_wait", C.c_double),
-                ("t_total", C.c_double), ("ruiz_iters", C.c_int)]
+                ("t_total", C.c_double), ("ruiz_iters", C.c_int), ("gemm_bytes", C.c_double)]
 
 
 # every entry point include/fplus.h and include/fplus_amd.h declare, with its ctypes signature

@@ -869,6 +869,7 @@ extern "C" int fplus_amd_numeric_stats(const fplus_numeric* num, fplus_amd_stats
     st->levels = F.levels;
     st->flops = F.flops;
     st->gemm_flops = F.gemm_flops;
+    st->gemm_bytes = F.gemm_bytes;
     st->t_ruiz = num->t_ruiz;
     st->t_factor = num->t_factor;
     st->t_host = F.t_host;

@@ -334,7 +334,7 @@ int Driver::panel(int64_t j0, int64_t j1)
     // ---- update tasks per union supernode and block, with dependency levels: a block's task solves the block's pivot
     // rows the panel's columns reach (read: they wait for every task writing them) and writes the rows below
     const int TC = trsm_cols(vs_), RT = FPA_ROW_TILE, CTG = FPA_COL_TILE;
-    const double fm = vs_ == 1 ? 2.0 : 8.0;
+    const double fm = vs_ == 1 ? 2.0 : 8.0, es = 8.0 * vs_;
     std::vector<int64_t> ucnt(un.size() + 1, 0);
     for (size_t q = 0; q < ext_s.size(); q++) ucnt[uidx_[ext_s[q]] + 1]++;
     for (size_t u = 0; u < un.size(); u++) ucnt[u + 1] += ucnt[u];
@@ -388,6 +388,8 @@ int Driver::panel(int64_t j0, int64_t j1)
             tl.push_back(t);
             maxlevel = std::max(maxlevel, lev);
             F_.gemm_flops += fm * (double)ncol * (double)(B.w - k0) * (double)(B.nrows - B.w);
+            F_.gemm_bytes += es * ((double)(B.nrows - B.w) * (double)(B.w - k0) + (double)(B.w - k0) * (double)ncol +
+                                   2.0 * (double)(B.nrows - B.w) * (double)ncol);
         }
     }
     std::stable_sort(tl.begin(), tl.end(), [](const TaskL& a, const TaskL& b) { return a.level < b.level; });

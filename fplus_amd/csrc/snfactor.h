@@ -34,7 +34,7 @@ struct DevFactor {
     DevBuf<int32_t> d_Ui;                // U row indices on the device, expanded from the runs panel by panel
     int64_t nnzU = 0;
     // statistics
-    double flops = 0, gemm_flops = 0;
+    double flops = 0, gemm_flops = 0, gemm_bytes = 0;
     int64_t panels = 0, levels = 0, tasks = 0;
     double t_host = 0, t_wait = 0;
     double t_p1 = 0, t_tasks = 0, t_p3 = 0, t_out = 0;   // host phases: search, tasks and levels, symbolic with the pivots, blocks and U out
