@@ -133,6 +133,24 @@ def cpu_baseline(workload_flops, family):
                       f"{os.cpu_count()} logical CPUs on the host"}
 
 
+FLOP_KERNELS = {"blk_gemm", "blk_trsm", "gemm_sub", "getf2"}
+BYTE_KERNELS = {"lfwd_gather", "lfwd_pull", "lfwd_push", "ub_dots", "ub_tri", "utdots", "utfwd", "ltdots", "ltbwd",
+                "ruiz_rowmax", "ruiz_scale_colmax", "ruiz_scale_cols"}
+
+
+def kernel_rate(name, sec, launches, work):
+    """seconds and launches of one kernel over the timed region, with its rate: TFLOP/s (and the fraction of the
+    FP64 matrix peak) for the factor's arithmetic kernels, GB/s of algorithmic bytes (and the fraction of HBM peak)
+    for Ruiz and the solves"""
+    r = {"s": sec, "launches": launches}
+    if work > 0 and sec > 0:
+        if name in FLOP_KERNELS:
+            r.update({"TFLOP/s": work / sec / 1e12, "frac_fp64_mfma": work / sec / PEAK_FP64_MFMA})
+        elif name in BYTE_KERNELS:
+            r.update({"GB/s": work / sec / 1e9, "frac_hbm": work / sec / PEAK_HBM})
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,8 +228,7 @@ def main():
             kern[name] = (ms.value / 1e3, cnt.value, work.value)
     dom = max(kern, key=lambda k: kern[k][0])
     t_dom, cnt_dom, work_dom = kern[dom]
-    mfma_kernels = {"blk_gemm", "blk_trsm", "gemm_sub", "getf2"}
-    bound = "mfma" if dom in mfma_kernels else "hbm"
+    bound = "mfma" if dom in FLOP_KERNELS else "hbm"
     achieved = work_dom / t_dom
     peak = PEAK_FP64_MFMA if bound == "mfma" else PEAK_HBM
     roof = {"kernel": dom, "bound": bound, "achieved": achieved / (1e12 if bound == "mfma" else 1e9),
@@ -233,7 +250,7 @@ def main():
         "factor": {"flops": st.flops, "gemm_flops": st.gemm_flops, "gemm_bytes": st.gemm_bytes, "nnzL": st.nnzL, "nnzU": st.nnzU,
                    "supernodes": st.nsupernodes, "panels": st.panels, "levels": st.levels,
                    "t_ruiz": st.t_ruiz, "t_factor": st.t_factor, "t_host": st.t_host, "gflops": st.flops / sec / 1e9},
-        "kernels": {k: {"s": v[0], "launches": v[1]} for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])},
+        "kernels": {k: kernel_rate(k, *v) for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
