@@ -61,6 +61,44 @@ template <class T> struct DevBuf {
 
 inline void dev_sync(hipStream_t s) { hip_check(hipStreamSynchronize(s), "hipStreamSynchronize"); }
 
+// page-locked host memory (hipHostMalloc): device-to-host copies into it run asynchronously on the stream
+template <class T> struct HostBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() { release(); }
+    void alloc(size_t cnt)
+    {
+        release();
+        n = cnt;
+        hip_check(hipHostMalloc((void**)&p, (cnt > 0 ? cnt : 1) * sizeof(T), hipHostMallocDefault), "hipHostMalloc");
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr; n = 0;
+    }
+};
+
+// events that mark points of a stream the host waits for one by one
+struct EventList {
+    std::vector<hipEvent_t> ev;
+    EventList() = default;
+    EventList(const EventList&) = delete;
+    EventList& operator=(const EventList&) = delete;
+    ~EventList() { for (hipEvent_t e : ev) (void)hipEventDestroy(e); }
+    void resize(size_t cnt)
+    {
+        while (ev.size() < cnt) {
+            hipEvent_t e;
+            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreateWithFlags");
+            ev.push_back(e);
+        }
+    }
+};
+
 // ---------------------------------------------------------------- kernels: fills, gathers, Ruiz (ruiz.hip)
 namespace k {
 void fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);

@@ -173,6 +173,9 @@ private:
     std::vector<int64_t> col_sn_, pinv_, p_, bpos_, miss_;
     std::vector<int64_t> npl_l_, piv_c_;   // phase 3: a column's candidates not pivotal / pivotal in the panel
     std::vector<std::vector<int64_t>> c_s_, c_kf_, c_np_;   // phase 1's results per panel column
+    std::vector<std::vector<int32_t>> ch_rows_;   // per chunk of columns: the rows not yet pivotal, first appearance
+    std::vector<int32_t> thp_;      // the workspace positions of the update tasks' rows
+    std::vector<int64_t> thpo_;
     SWork ws0_;                     // the sequential phases' marks
     std::vector<SWork> wsT_;        // one per thread of the parallel search
     HostPool pool_;
@@ -196,6 +199,9 @@ private:
     DevBuf<PanelStatus> d_status_;
     DevBuf<double> d_coop_;          // the cooperative panel LU's candidate slots
     DevBuf<int32_t> d_coop_cnt_;     // and its barrier counter
+    HostBuf<int32_t> h_piv_;         // the panel's pivot positions, copied back sub-panel by sub-panel
+    HostBuf<PanelStatus> h_st_;      // the panel LU's status after each sub-panel
+    EventList sp_ev_;                // the end of each sub-panel's copies
     DevBuf<URun> d_runs_;
     DevBuf<ExtractCol> d_ex_;
     Arena<double> vals_{nullptr, 0, 0, 0};
@@ -314,9 +320,25 @@ int Driver::panel(int64_t j0, int64_t j1)
         }
     }
     const int64_t Ru = (int64_t)univ.size();
-    for (int64_t ci = 0; ci < np; ci++)
-        for (int64_t row : c_np[ci])
-            if (hrowpos_[row] < 0) { hrowpos_[row] = (int32_t)univ.size(); univ.push_back((int32_t)row); }
+    {
+        // the rows not yet pivotal, in order of first appearance over the columns' candidate lists: chunks of columns
+        // list theirs in parallel (each with its own marks), then the lists are joined in chunk order -- the same order
+        // as one pass over all the lists
+        const int64_t nch = std::min<int64_t>(np, 4 * (int64_t)pool_.size()), per = (np + nch - 1) / nch;
+        if ((int64_t)ch_rows_.size() < nch) ch_rows_.resize(nch);
+        pool_.run(nch, [&](int64_t c, int tid) {
+            SWork& w = wsT_[tid];
+            const int64_t stamp = 2 * n + 2 + j0 + c;
+            std::vector<int32_t>& out = ch_rows_[c];
+            out.clear();
+            for (int64_t ci = c * per; ci < std::min(np, (c + 1) * per); ci++)
+                for (const int64_t row : c_np[ci])
+                    if (hrowpos_[row] < 0 && w.rmark[row] != stamp) { w.rmark[row] = stamp; out.push_back((int32_t)row); }
+        });
+        for (int64_t c = 0; c < nch; c++)
+            for (const int32_t row : ch_rows_[c])
+                if (hrowpos_[row] < 0) { hrowpos_[row] = (int32_t)univ.size(); univ.push_back(row); }
+    }
     if (open0 >= 0) {
         const HSn& S = sn_[open0];
         for (int64_t r = S.w; r < S.nr(); r++) {
@@ -325,6 +347,8 @@ int Driver::panel(int64_t j0, int64_t j1)
         }
     }
     const int64_t Rl = (int64_t)univ.size() - Ru;
+    const double th_ws = now_s();
+    F_.t_ws += th_ws - th_p1;
     for (int64_t q = Bp_[j0]; q < Bp_[j1]; q++)   // every entry of the panel's columns lands in the workspace
         if (hrowpos_[Bi_[q]] < 0) { errf(err_, errlen_, "internal: a column entry outside the panel rows"); return FPLUS_ERR_INTERNAL; }
     const int64_t ldw = std::max<int64_t>((Ru + Rl + 1) & ~(int64_t)1, 2);
@@ -351,8 +375,6 @@ int Driver::panel(int64_t j0, int64_t j1)
     struct TaskL { UpdTask t; int32_t level; };
     std::vector<TaskL> tl;
     std::vector<int32_t> cols;
-    std::vector<int32_t> wl(Ru + Rl, 0);
-    int32_t maxlevel = 0;
     for (size_t u = 0; u < un.size(); u++) {
         const int64_t s = un[u], c0 = ucnt[u], c1 = ucnt[u + 1];
         if (c1 == c0) continue;
@@ -366,32 +388,53 @@ int Driver::panel(int64_t j0, int64_t j1)
                 if (ukf[q] < bb1) { cols.push_back((int32_t)ucol[q]); kmin = std::min(kmin, ukf[q]); }
             const int64_t ncol = (int64_t)cols.size() - col_off;
             if (ncol == 0) continue;
-            const std::vector<int32_t>& br = F_.block_rows[b];
             const int64_t k0 = std::max<int64_t>(kmin, bb0) - bb0;
-            int32_t lev = 0;
-            for (int64_t r = k0; r < B.w; r++) {
-                const int32_t hp = hrowpos_[br[r]];
-                if (hp < 0) { errf(err_, errlen_, "internal: update row outside the panel rows"); return FPLUS_ERR_INTERNAL; }
-                lev = std::max(lev, wl[hp]);
-            }
-            lev += 1;
-            for (int64_t r = B.w; r < B.nrows; r++) {
-                const int32_t hp = hrowpos_[br[r]];
-                if (hp < 0) { errf(err_, errlen_, "internal: update row outside the panel rows"); return FPLUS_ERR_INTERNAL; }
-                wl[hp] = std::max(wl[hp], lev);
-            }
-
             TaskL t;
             t.t.block = b; t.t.k0 = (int32_t)k0;
             t.t.col_off = (int32_t)col_off; t.t.ncol = (int32_t)ncol; t.t.xoff = 0;
-            t.level = lev;
+            t.level = 0;
             tl.push_back(t);
-            maxlevel = std::max(maxlevel, lev);
             F_.gemm_flops += fm * (double)ncol * (double)(B.w - k0) * (double)(B.nrows - B.w);
             F_.gemm_bytes += es * ((double)(B.nrows - B.w) * (double)(B.w - k0) + (double)(B.w - k0) * (double)ncol +
                                    2.0 * (double)(B.nrows - B.w) * (double)ncol);
         }
     }
+    // the workspace positions of every task's rows (from its segment start on), gathered in parallel; then the levels
+    // in task order: a task is one past the highest level writing its segment rows and raises its rows below to its own
+    {
+        const int64_t nt = (int64_t)tl.size();
+        thpo_.assign(nt + 1, 0);
+        for (int64_t i = 0; i < nt; i++) thpo_[i + 1] = thpo_[i] + (F_.blocks[tl[i].t.block].nrows - tl[i].t.k0);
+        thp_.resize((size_t)thpo_[nt]);
+        const int64_t TPI = 32;
+        std::atomic<int> bad{0};
+        pool_.run((nt + TPI - 1) / TPI, [&](int64_t c, int) {
+            for (int64_t i = c * TPI; i < std::min(nt, (c + 1) * TPI); i++) {
+                const int32_t* br = F_.block_rows[tl[i].t.block].data();
+                const int64_t r0 = tl[i].t.k0, r1 = F_.blocks[tl[i].t.block].nrows;
+                int32_t* out = thp_.data() + thpo_[i];
+                for (int64_t r = r0; r < r1; r++) {
+                    const int32_t hp = hrowpos_[br[r]];
+                    if (hp < 0) bad.store(1, std::memory_order_relaxed);
+                    out[r - r0] = hp;
+                }
+            }
+        });
+        if (bad.load()) { errf(err_, errlen_, "internal: update row outside the panel rows"); return FPLUS_ERR_INTERNAL; }
+    }
+    std::vector<int32_t> wl(Ru + Rl, 0);
+    int32_t maxlevel = 0;
+    for (size_t i = 0; i < tl.size(); i++) {
+        const int32_t* hp = thp_.data() + thpo_[i];
+        const int64_t m = F_.blocks[tl[i].t.block].w - tl[i].t.k0, len = thpo_[i + 1] - thpo_[i];
+        int32_t lev = 0;
+        for (int64_t r = 0; r < m; r++) lev = std::max(lev, wl[hp[r]]);
+        lev += 1;
+        for (int64_t r = m; r < len; r++) wl[hp[r]] = std::max(wl[hp[r]], lev);
+        tl[i].level = lev;
+        maxlevel = std::max(maxlevel, lev);
+    }
+    F_.t_lev += now_s() - th_ws;
     std::stable_sort(tl.begin(), tl.end(), [](const TaskL& a, const TaskL& b) { return a.level < b.level; });
     // tile rows per level: 64, or 32 where the level's 64-row tiles would not fill the device twice
     std::vector<int32_t> lev_rt(maxlevel + 2, RT);
@@ -472,39 +515,56 @@ int Driver::panel(int64_t j0, int64_t j1)
     std::memset(&zero_st, 0, sizeof(zero_st));
     d_status_.upload(&zero_st, 1, s_);
     d_piv_.reserve(np);
-    for (int64_t a = 0; a < np; a += FPA_PANEL_NB) {
+    // each sub-panel's pivot positions and the status so far come back as soon as it is factored: the host works on
+    // a sub-panel's columns while the device factors the next ones
+    if ((int64_t)h_piv_.n < np) h_piv_.alloc(np);
+    const int64_t nsp = (np + FPA_PANEL_NB - 1) / FPA_PANEL_NB;
+    if ((int64_t)h_st_.n < nsp) h_st_.alloc(nsp);
+    sp_ev_.resize(nsp);
+    for (int64_t a = 0, sp = 0; a < np; a += FPA_PANEL_NB, sp++) {
         const int32_t nb = (int32_t)std::min<int64_t>(FPA_PANEL_NB, np - a);
         k::getf2(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_colmax_.p, opt_.pivot_rel, d_piv_.p, d_status_.p, d_coop_.p,
                  d_coop_cnt_.p, s_);
         k::trsm_sub(vs_, Wl, ldw, (int32_t)np, (int32_t)a, nb, d_status_.p, s_);
         k::gemm_sub(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_status_.p, s_);
+        hip_check(hipMemcpyAsync(h_piv_.p + a, d_piv_.p + a, nb * sizeof(int32_t), hipMemcpyDeviceToHost, s_), "pivots D2H");
+        hip_check(hipMemcpyAsync(h_st_.p + sp, d_status_.p, sizeof(PanelStatus), hipMemcpyDeviceToHost, s_), "status D2H");
+        hip_check(hipEventRecord(sp_ev_.ev[sp], s_), "hipEventRecord");
     }
-    std::vector<int32_t> piv(np);
-    PanelStatus st;
-    d_piv_.download(piv.data(), np, s_);
-    d_status_.download(&st, 1, s_);
-    const double tw0 = now_s();
-    dev_sync(s_);
-    F_.t_wait += now_s() - tw0;
     const double th1 = now_s();
-    if (st.err < 0) {
-        errf(err_, errlen_, "internal: the cooperative panel LU's workgroups did not meet at a barrier");
-        return FPLUS_ERR_INTERNAL;
-    }
-    if (st.err) {
-        const int64_t j = j0 + st.err - 1;
-        errf(err_, errlen_, "Near-zero or zero pivot %.6e at step %ld (threshold %.6e). Matrix is singular.", st.best, (long)j, st.thr);
-        return FPLUS_ERR_SINGULAR;
-    }
-    // ---- replay the row swaps: the pivot row of every panel column, the final positions of the lower rows
+    double waited = 0;
+    // the rows' order as the row swaps leave it: after column ci's swap, position ci holds its pivot row for good
     std::vector<int32_t> lrows(univ.begin() + Ru, univ.end());
-    for (int64_t ci = 0; ci < np; ci++) std::swap(lrows[ci], lrows[piv[ci]]);
-    for (int64_t i = 0; i < Rl; i++) lposmap_[lrows[i]] = (int32_t)i;
     // ---- phase 3 on the host, with the device's pivots (src/factor_sn_panel.inc:162-390)
     const int64_t u_begin = Up_[j0];
     std::vector<URun> pruns;
     std::vector<std::pair<int64_t, int64_t>> touched;   // (supernode, first column in this panel)
     for (int64_t ci = 0; ci < np; ci++) {
+        if (ci % FPA_PANEL_NB == 0) {
+            const int64_t sp = ci / FPA_PANEL_NB;
+            const double tw0 = now_s();
+            hip_check(hipEventSynchronize(sp_ev_.ev[sp]), "hipEventSynchronize");
+            waited += now_s() - tw0;
+            const PanelStatus st = h_st_.p[sp];
+            if (st.err < 0) {
+                dev_sync(s_);
+                errf(err_, errlen_, "internal: the cooperative panel LU's workgroups did not meet at a barrier");
+                return FPLUS_ERR_INTERNAL;
+            }
+            if (st.err) {
+                dev_sync(s_);
+                const int64_t j = j0 + st.err - 1;
+                errf(err_, errlen_, "Near-zero or zero pivot %.6e at step %ld (threshold %.6e). Matrix is singular.", st.best, (long)j, st.thr);
+                return FPLUS_ERR_SINGULAR;
+            }
+        }
+        const int32_t pv = h_piv_.p[ci];
+        if (pv < ci || pv >= Rl) {
+            dev_sync(s_);
+            errf(err_, errlen_, "internal: pivot position %d out of range for column %ld", (int)pv, (long)(j0 + ci));
+            return FPLUS_ERR_INTERNAL;
+        }
+        std::swap(lrows[ci], lrows[pv]);
         const int64_t j = j0 + ci, stamp2 = n + 1 + j;
         const double tq0 = now_s();
         // one pass over the column's candidates from the search before the panel: mark them, keep the ones still not
@@ -620,7 +680,10 @@ int Driver::panel(int64_t j0, int64_t j1)
         F_.t_p3_prune += tq4 - tq3;
     }
     const double th_out = now_s();
-    F_.t_p3 += th_out - th1;
+    F_.t_p3 += th_out - th1 - waited;
+    F_.t_wait += waited;
+    // the final positions of the lower rows
+    for (int64_t i = 0; i < Rl; i++) lposmap_[lrows[i]] = (int32_t)i;
     // ---- device: the new blocks and the U values out of the workspace
     std::vector<ExtractCol> ex;
     std::vector<int32_t> expos;
@@ -664,7 +727,7 @@ int Driver::panel(int64_t j0, int64_t j1)
     for (int32_t row : univ) hrowpos_[row] = -1;
     for (int32_t row : lrows) lposmap_[row] = -1;
     F_.t_out += now_s() - th_out;
-    F_.t_host += now_s() - th1;
+    F_.t_host += now_s() - th1 - waited;
     F_.panels++;
     return FPLUS_OK;
 }

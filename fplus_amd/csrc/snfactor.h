@@ -39,6 +39,7 @@ struct DevFactor {
     double t_host = 0, t_wait = 0;
     double t_p1 = 0, t_tasks = 0, t_p3 = 0, t_out = 0;   // host phases: search, tasks and levels, symbolic with the pivots, blocks and U out
     double t_p3_dfs = 0, t_p3_u = 0, t_p3_join = 0, t_p3_prune = 0;   // the symbolic with the pivots, by step
+    double t_ws = 0, t_lev = 0;   // within the tasks and levels: the union order and workspace rows, the tasks' levels
     int64_t rows_upper = 0, rows_lower = 0, rows_lower_max = 0;   // workspace rows summed over the panels
 };
 
