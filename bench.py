@@ -133,7 +133,7 @@ def cpu_baseline(workload_flops, family):
                       f"{os.cpu_count()} logical CPUs on the host"}
 
 
-FLOP_KERNELS = {"blk_gemm", "blk_trsm", "gemm_sub", "getf2"}
+FLOP_KERNELS = {"blk_upd", "blk_inverse", "gemm_sub", "getf2"}
 BYTE_KERNELS = {"lfwd_gather", "lfwd_pull", "lfwd_push", "ub_dots", "ub_tri", "utdots", "utfwd", "ltdots", "ltbwd",
                 "ruiz_rowmax", "ruiz_scale_colmax", "ruiz_scale_cols"}
 
@@ -220,7 +220,7 @@ def main():
         L.fplus_numeric_free(nm)
     # the kernels' sums over the timed region
     kern = {}
-    for name in ("blk_gemm", "blk_trsm", "gemm_sub", "getf2", "trsm_sub", "lfwd_gather", "lfwd_pull", "lfwd_push", "ub_dots", "ub_tri", "utdots", "utfwd", "ltdots",
+    for name in ("blk_upd", "blk_inverse", "gemm_sub", "getf2", "trsm_sub", "lfwd_gather", "lfwd_pull", "lfwd_push", "ub_dots", "ub_tri", "utdots", "utfwd", "ltdots",
                  "ltbwd", "ruiz_rowmax", "ruiz_scale_colmax", "ruiz_scale_cols"):
         ms, cnt, work = C.c_double(), C.c_int64(), C.c_double()
         L.fplus_amd_profile_get(name.encode(), C.byref(ms), C.byref(cnt), C.byref(work))
@@ -235,7 +235,7 @@ def main():
             "peak": peak / (1e12 if bound == "mfma" else 1e9), "unit": "TFLOP/s" if bound == "mfma" else "GB/s",
             "frac": achieved / peak, "traffic": measured_traffic(args.workload, dom), "launches": cnt_dom,
             "avg_launch_ms": 1e3 * t_dom / cnt_dom, "work_per_launch": work_dom / cnt_dom}
-    if dom == "blk_gemm":
+    if dom == "blk_upd":
         roof["algorithmic_bytes_per_launch"] = st.gemm_bytes / cnt_dom
     family = args.workload.split("-")[0]
     cpu = None if args.no_cpu else cpu_baseline(st.flops, family)

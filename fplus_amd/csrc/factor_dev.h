@@ -11,33 +11,33 @@
 namespace fpa {
 
 inline int col_tile(int vs) { return vs == 1 ? 64 : 32; }   // panel columns per tile of the panel-LU GEMM
-inline int trsm_cols(int vs) { return vs == 1 ? 2 : 1; }    // panel columns per workgroup of blk_trsm
 
 // A block: the columns [c0, c0 + w) of one supernode computed in one panel, stored column-major over its rows (ld =
 // nrows): the pivot rows of its columns first, in column order, then the supernode's other rows as they stood at the
 // end of that panel.  Values are the L factor below the unit diagonal (the entries above it are U values, unused by
-// the updates).  rows hold row indices of the permuted matrix.
+// the updates).  rows hold row indices of the permuted matrix.  ivals, same shape, is the inverse form the updates use:
+// [L11^-1 ; L21 L11^-1] with L11 the unit lower triangle of the pivot rows and L21 the rows below.
 struct BlockDesc {
     double* vals;      // nrows * w scalars (a scalar is 1 or 2 doubles)
+    double* ivals;     // nrows * w scalars
     int32_t* rows;     // nrows
     int32_t nrows, w;
     int32_t c0, sn;
 };
 
 // one block applied to some panel columns: the block's pivot rows [k0, w) (the segment the columns reach) are solved
-// with the block's unit lower triangle (blk_trsm), then the rows [w, nrows) updated by the rectangle below (blk_gemm)
+// with the block's unit lower triangle and the rows [w, nrows) updated by the rectangle below, both as one product with
+// the block's inverse form (blk_upd)
 struct UpdTask {
     int32_t block, k0;
     int32_t col_off, ncol;   // panel-local column indices at cols[col_off .. col_off + ncol)
     int64_t xoff;            // the solved segment, compact: X[(k - k0) + c * (w - k0)] at xbuf + xoff scalars
 };
 
-struct TrsmTile {
-    int32_t task, col0;      // columns [col0, col0 + trsm_cols(vs)) of the task's list
-};
-
+// a product tile: for blk_upd, rows [row0, row0 + tile rows) of the task's block and columns [col0, col0 + 64) of the
+// task's list; for blk_inverse, `task` is the block and rows [w + row0, ...), columns [col0, col0 + 64) of the block
 struct GemmTile {
-    int32_t task, row0, col0, pad;   // rows [row0, row0 + tile rows) of the block, columns [col0, col0 + 64) of the task's list
+    int32_t task, row0, col0, pad;
 };
 
 // one run of U entries out of the panel workspace: Ui[uoff + t] = row0 + t, Ux[uoff + t] = W[woff + t], t < len
@@ -66,19 +66,23 @@ void rowmap_set(int64_t cnt, const int32_t* rows, int32_t* rowmap, int reset, hi
 void scatter_cols(int vs, int32_t ncol, const int64_t* Bp, const int32_t* Bi, const double* Bx, int64_t j0,
                   const int32_t* rowmap, double* W, int64_t ldw, double* colmax, hipStream_t s);
 void u_runs(int vs, int32_t nruns, const URun* runs, const double* W, double* Ux, int32_t* Ui, hipStream_t s);
-// the segment solves of the tiles of one level (in place in W, and compact into xbuf)
-void blk_trsm(int vs, int32_t ntile, const TrsmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
-              const int32_t* rowmap, double* W, int64_t ldw, double* xbuf, double flops, hipStream_t s);
-// the rectangles below, subtracted from W through rowmap: atomically when shared_rows (two tasks of the level write a
-// common row), else with plain loads and stores
-// tile_rows: 64, or 32 (tiles of half the rows, for levels with few tiles)
-void blk_gemm(int vs, int32_t ntile, const GemmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
-              const int32_t* rowmap, double* W, int64_t ldw, const double* xbuf, int shared_rows, int tile_rows, double flops,
-              hipStream_t s);
+// the inverse forms of new blocks: jobs (block, column) for the triangles' inverses, then the tiles of the rows below
+void blk_inverse(int vs, int32_t njobs, const int2* jobs, int32_t ntile, const GemmTile* tiles, const BlockDesc* blocks,
+                 double flops, hipStream_t s);
+// the tiles of one dependency level: each task's solved segment into its slice of xbuf, its rows below subtracted from
+// W through rowmap -- atomically when shared_rows (two tasks of the level write a common row), else with plain loads
+// and stores; tile_rows: 64, or 32 (tiles of half the rows, for levels with few tiles)
+void blk_upd(int vs, int32_t ntile, const GemmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
+             const int32_t* rowmap, double* W, int64_t ldw, double* xbuf, int shared_rows, int tile_rows, double flops,
+             hipStream_t s);
+// the solved segments of all tasks back into W's rows
+void x_back(int vs, int32_t ntask, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks, const int32_t* rowmap,
+            double* W, int64_t ldw, const double* xbuf, hipStream_t s);
 // dense partial-pivot LU of the panel's lower section (nr x P, column-major, ld), sub-panel [a, a + nb): a cooperative
-// launch over the rows (coop: getf2_coop_bytes(vs) of scratch, counter: one int), one workgroup for very tall panels
+// launch over the rows (coop: getf2_coop_bytes(vs) of scratch; counter: one int, zero at first and then only touched
+// here, with *counted the arrivals it holds), one workgroup for very tall panels
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
-           double pivot_rel, int32_t* piv, PanelStatus* st, void* coop, int32_t* counter, hipStream_t s);
+           double pivot_rel, int32_t* piv, PanelStatus* st, void* coop, int32_t* counter, int64_t* counted, hipStream_t s);
 size_t getf2_coop_bytes(int vs);
 int getf2_coop_max_wg();
 void trsm_sub(int vs, double* Wl, int64_t ld, int32_t P, int32_t a, int32_t nb, const PanelStatus* st, hipStream_t s);

@@ -5,9 +5,10 @@
 // search; rowmap[row] gives a row's position.  The kernels here
 //   * scatter the panel's columns of A into W (scatter_cols);
 //   * apply the earlier supernodes to the panel, block by block: the unit-lower solve of the block's pivot rows the
-//     panel's columns reach (blk_trsm, the TRSM) and the rectangle below subtracted through rowmap (blk_gemm, the
-//     GEMM on the FP64 matrix cores with the relative-index scatter) -- one launch pair per dependency level, every
-//     block of a level in one launch;
+//     panel's columns reach (the TRSM) and the rectangle below subtracted through rowmap (the GEMM, with the
+//     relative-index scatter) as one product on the FP64 matrix cores with the block's inverse form (blk_upd) -- one
+//     launch per dependency level, every block of a level in it -- then the solved segments back into W (x_back);
+//   * make each new block's inverse form (blk_inverse: its triangle's inverse, then the rows below times it);
 //   * factor the rows not yet pivotal by dense partial pivoting (getf2 per sub-panel of nb columns, then the
 //     trailing TRSM and GEMM), LAPACK-style with whole-row swaps, so the pivot rows end at the top in column order.
 // Entries of W outside a column's sparse pattern are exact zeros and stay so, which makes the dense panel LU produce
@@ -57,72 +58,57 @@ constexpr int RT = FPA_ROW_TILE; // rows per GEMM tile
 // panel columns per tile of the panel-LU GEMM: 64 real, 32 complex (LDS within 64 KB)
 template <class T> struct CTile { static constexpr int v = 64; };
 template <> struct CTile<Zd> { static constexpr int v = 32; };
-template <class T> struct TCols { static constexpr int v = 2; };   // columns per blk_trsm workgroup
-template <> struct TCols<Zd> { static constexpr int v = 1; };
 constexpr int PNB = FPA_PANEL_NB;  // sub-panel width of the dense panel LU
 
-// the unit-lower solve of a block's segment [k0, w) for up to TCols panel columns: thread t owns segment row t (the
-// segment is at most 256 rows: a block is at most one panel wide).  The segment goes in chunks of 16 rows: a chunk's
-// triangle inside its wavefront (the row just finished broadcast by a lane shuffle), one barrier, then the rows below
-// subtract the chunk's 16 columns.  Every row receives its updates in column order, as in the column-by-column solve.
-// The factor's values of the next chunk are read while the current one is solved.  The result goes back to W and,
-// compact, to the task's slice of xbuf.
 template <class T> __device__ inline T shfl_val(T v, int src);
 template <> __device__ inline double shfl_val<double>(double v, int src) { return __shfl(v, src); }
 template <> __device__ inline Zd shfl_val<Zd>(Zd v, int src) { return zmake(__shfl(v.x, src), __shfl(v.y, src)); }
 
+// The updates use each block in an inverse form, computed once when the block is made:
+//     IV = [ L11^-1 ; L21 L11^-1 ]   (nrows x w, column-major like the block's values),
+// L11 the block's unit lower triangle (its pivot rows) and L21 the rows below.  The inverse of a trailing part
+// L11[k0:w, k0:w] is the trailing part of L11^-1, and (L21 L11^-1)[:, k0:w] = L21[:, k0:w] L11^-1[k0:w, k0:w], so a task
+// whose columns reach the segment [k0, w) is one product IV[k0:nrows, k0:w] S with S the segment's rows of W: its top
+// m = w - k0 rows are the solved segment X (the TRSM), the rest the update of the rows below (the GEMM).
+
+// column c of L11^-1 (one workgroup per (block, column); thread t owns row t, a block being at most 256 columns): the
+// unit-lower solve of e_c in chunks of 16 rows -- a chunk's triangle inside its wavefront (the row just finished
+// broadcast by a lane shuffle), one barrier, then the rows below subtract the chunk's 16 columns -- starting at the
+// chunk that holds row c (the rows above it stay zero); the next chunk's factor values are read during the current one
 template <class T>
-__global__ __launch_bounds__(TB) void k_blk_trsm(const TrsmTile* __restrict__ tiles, const UpdTask* __restrict__ tasks,
-                                                  const int32_t* __restrict__ cols, const BlockDesc* __restrict__ blocks,
-                                                  const int32_t* __restrict__ rowmap, double* __restrict__ W, int64_t ldw,
-                                                  double* __restrict__ xbuf)
+__global__ __launch_bounds__(TB) void k_blk_linv(const int2* __restrict__ jobs, const BlockDesc* __restrict__ blocks)
 {
-    constexpr int CT = TCols<T>::v, CH = 16;
-    __shared__ T xs[TB][CT];
-    __shared__ int32_t cidx[CT];
-    const TrsmTile tt = tiles[blockIdx.x];
-    const UpdTask tk = tasks[tt.task];
-    const BlockDesc B = blocks[tk.block];
-    const int32_t m = B.w - tk.k0, nc = min(CT, tk.ncol - tt.col0), t = threadIdx.x;
-    const T* V = reinterpret_cast<const T*>(B.vals) + (int64_t)tk.k0 * B.nrows + tk.k0;   // V[k * nrows + r]: segment (r, k)
-    if (t < nc) cidx[t] = cols[tk.col_off + tt.col0 + t];
-    __syncthreads();
-    const bool own = t < m;
-    const int64_t pos = own ? (int64_t)rowmap[B.rows[tk.k0 + t]] : 0;
-    T x[CT];
-#pragma unroll
-    for (int c = 0; c < CT; c++) x[c] = (own && c < nc) ? ld_val<T>(W, pos + (int64_t)cidx[c] * ldw) : Sc<T>::zero();
-    // the factor's values row t needs from chunk ch: columns r0 .. min(r1, t) - 1
+    constexpr int CH = 16;
+    __shared__ T xs[TB];
+    const int2 jb = jobs[blockIdx.x];
+    const BlockDesc B = blocks[jb.x];
+    const int32_t w = B.w, c = jb.y, t = threadIdx.x;
+    const T* L = reinterpret_cast<const T*>(B.vals);
+    const bool own = t < w;
+    T x = t == c ? Sc<T>::one() : Sc<T>::zero();
     auto fetch = [&](int ch, T* lv) {
         const int r0 = ch * CH;
 #pragma unroll
         for (int u = 0; u < CH; u++) {
             const int k = r0 + u;
-            lv[u] = (own && k < t) ? V[(int64_t)k * B.nrows + t] : Sc<T>::zero();
+            lv[u] = (own && k < t) ? L[(int64_t)k * B.nrows + t] : Sc<T>::zero();
         }
     };
     T lcur[CH], lnxt[CH];
-    fetch(0, lcur);
-    const int nch = (m + CH - 1) / CH;
-    for (int ch = 0; ch < nch; ch++) {
-        const int r0 = ch * CH, r1 = min(m, r0 + CH);
+    const int ch0 = c / CH, nch = (w + CH - 1) / CH;
+    fetch(ch0, lcur);
+    for (int ch = ch0; ch < nch; ch++) {
+        const int r0 = ch * CH, r1 = min(w, r0 + CH);
         if (ch + 1 < nch) fetch(ch + 1, lnxt);
         if (t >= r0 && t < r1) {
-            // the chunk's triangle: its 16 rows are lanes of one wavefront
 #pragma unroll
             for (int u = 0; u < CH; u++) {
                 const int k = r0 + u;
                 if (k >= r1) break;
-                T xk[CT];
-#pragma unroll
-                for (int c = 0; c < CT; c++) xk[c] = shfl_val<T>(x[c], k & 63);
-                if (t > k) {
-#pragma unroll
-                    for (int c = 0; c < CT; c++) x[c] = Sc<T>::fms(x[c], lcur[u], xk[c]);
-                }
+                const T xk = shfl_val<T>(x, k & 63);
+                if (t > k) x = Sc<T>::fms(x, lcur[u], xk);
             }
-#pragma unroll
-            for (int c = 0; c < CT; c++) xs[t][c] = x[c];
+            xs[t] = x;
         }
         __syncthreads();
         if (own && t >= r1) {
@@ -130,21 +116,13 @@ __global__ __launch_bounds__(TB) void k_blk_trsm(const TrsmTile* __restrict__ ti
             for (int u = 0; u < CH; u++) {
                 const int k = r0 + u;
                 if (k >= r1) break;
-#pragma unroll
-                for (int c = 0; c < CT; c++) x[c] = Sc<T>::fms(x[c], lcur[u], xs[k][c]);
+                x = Sc<T>::fms(x, lcur[u], xs[k]);
             }
         }
 #pragma unroll
         for (int u = 0; u < CH; u++) lcur[u] = lnxt[u];
     }
-    if (!own) return;
-    T* X = reinterpret_cast<T*>(xbuf) + tk.xoff;
-#pragma unroll
-    for (int c = 0; c < CT; c++)
-        if (c < nc) {
-            st_val<T>(W, pos + (int64_t)cidx[c] * ldw, x[c]);
-            X[t + (int64_t)(tt.col0 + c) * m] = x[c];
-        }
+    if (own) reinterpret_cast<T*>(B.ivals)[t + (int64_t)c * B.nrows] = x;
 }
 
 // FP64 matrix cores: D(16x16) += A(16x4) B(4x16) per wavefront; lane l supplies A[l % 16][l / 16] and
@@ -152,91 +130,82 @@ __global__ __launch_bounds__(TB) void k_blk_trsm(const TrsmTile* __restrict__ ti
 typedef double dx4 __attribute__((ext_vector_type(4)));
 __device__ inline dx4 mfma16(double a, double b, dx4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
-// the rectangle below a block's segment for a 64 x 64 tile: W[rows] -= L[rows, seg] X[seg, cols] through rowmap, with
-// the product on the matrix cores (four real products per complex one), k in chunks of 16 staged in LDS, the next
-// chunk loaded while the current one multiplies
-template <class T, int TM>
-__global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ tiles, const UpdTask* __restrict__ tasks,
-                                                  const int32_t* __restrict__ cols, const BlockDesc* __restrict__ blocks,
-                                                  const int32_t* __restrict__ rowmap, double* __restrict__ W, int64_t ldw,
-                                                  const double* __restrict__ xbuf, int shared_rows)
+constexpr int KC = 16, TN = 64;   // k per staged chunk, columns per product tile
+template <class T, int TM> struct TileCfg {
+    static constexpr int LPA = KC * TM / TB, LPB = KC * TN / TB;   // elements of A and of B each thread stages
+    static constexpr int WR = TM / 32, WC = 4 / WR, SJ = TN / WC / 16;   // waves: WR x WC, each 32 x (16 SJ)
+    static constexpr int STG = 2 * KC * TM + 2 * KC * TN;   // LDS scalars: two chunks of A[k][row] and B[k][col]
+    static_assert(TN * TM <= STG, "the product tile does not fit the staging buffer");
+};
+
+// C (TM x 64) = A (TM x [kb, ke)) B ([kb, ke) x 64) on the matrix cores (four real products per complex one): k in
+// chunks of 16 staged in LDS, the next chunk read while the current one multiplies.  la(r, k) and lb(k, c) give the
+// operands (zero outside the tile).  The product is left in LDS, C(r, c) at stg[c * TM + (r + c) % TM] (no bank
+// conflicts by rows or by columns), for the caller's write-out.
+template <class T, int TM, class LA, class LB>
+__device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB& lb, T* stg)
 {
+    using Cf = TileCfg<T, TM>;
     constexpr bool CPLX = Sc<T>::vs == 2;
-    constexpr int KC = 16, TN = 64, LPA = KC * TM / TB, LPB = KC * TN / TB;   // elements of A and of B each thread stages
-    constexpr int WR = TM / 32, WC = 4 / WR, SJ = TN / WC / 16;              // waves: WR x WC, each 32 x (16 SJ)
-    // staging: A[k][row] and B[k][col] for two chunks in flight; at the end the output tile
-    __shared__ T stg[2 * KC * TM + 2 * KC * TN];
     T* const As = stg;
     T* const Bs = stg + 2 * KC * TM;
-    __shared__ int32_t rpos[TM];
-    __shared__ int32_t cidx[TN];
-    const GemmTile tl = tiles[blockIdx.x];
-    const UpdTask tk = tasks[tl.task];
-    const BlockDesc B = blocks[tk.block];
-    const int32_t m = B.w - tk.k0;
-    const int32_t nr = min(TM, B.nrows - tl.row0), nc = min(TN, tk.ncol - tl.col0);
-    const T* V = reinterpret_cast<const T*>(B.vals);
-    const T* X = reinterpret_cast<const T*>(xbuf) + tk.xoff;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    if (tid < TM) rpos[tid] = tid < nr ? rowmap[B.rows[tl.row0 + tid]] : 0;
-    else if (tid < TM + TN) cidx[tid - TM] = (tid - TM) < nc ? cols[tk.col_off + tl.col0 + tid - TM] : 0;
-    const int wr = (wv / WC) * 32, wc = (wv % WC) * (16 * SJ);   // the wave's part of the tile
-    dx4 accr[2][SJ], acci[2][SJ];
+    const int wr = (wv / Cf::WC) * 32, wc = (wv % Cf::WC) * (16 * Cf::SJ);   // the wave's part of the tile
+    dx4 accr[2][Cf::SJ], acci[2][Cf::SJ];
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < SJ; j++) {
+        for (int j = 0; j < Cf::SJ; j++) {
             accr[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
             acci[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
         }
-    // staging: thread element e = tid + q * TB; A: row e % TM, k e / TM (rows contiguous); B: k e % KC, column e / KC
-    T ra[LPA], rb[LPB];
+    // thread element e = tid + q * TB; A: row e % TM, k e / TM (rows contiguous); B: k e % KC, column e / KC
+    T ra[Cf::LPA], rb[Cf::LPB];
     auto fetch = [&](int32_t kc) {
-        const int32_t kn = min(KC, m - kc);
 #pragma unroll
-        for (int q = 0; q < LPA; q++) {
+        for (int q = 0; q < Cf::LPA; q++) {
             const int e = tid + q * TB;
-            const int r = e % TM, ka = e / TM;
-            ra[q] = (r < nr && ka < kn) ? V[(int64_t)(tk.k0 + kc + ka) * B.nrows + tl.row0 + r] : Sc<T>::zero();
+            const int32_t k = kc + e / TM;
+            ra[q] = k < ke ? la(e % TM, k) : Sc<T>::zero();
         }
 #pragma unroll
-        for (int q = 0; q < LPB; q++) {
+        for (int q = 0; q < Cf::LPB; q++) {
             const int e = tid + q * TB;
-            const int kb = e % KC, c = e / KC;
-            rb[q] = (c < nc && kb < kn) ? X[(kc + kb) + (int64_t)(tl.col0 + c) * m] : Sc<T>::zero();
+            const int32_t k = kc + e % KC;
+            rb[q] = k < ke ? lb(k, e / KC) : Sc<T>::zero();
         }
     };
     auto stage = [&](int buf) {
 #pragma unroll
-        for (int q = 0; q < LPA; q++) {
+        for (int q = 0; q < Cf::LPA; q++) {
             const int e = tid + q * TB;
             As[(buf * KC + e / TM) * TM + e % TM] = ra[q];
         }
 #pragma unroll
-        for (int q = 0; q < LPB; q++) {
+        for (int q = 0; q < Cf::LPB; q++) {
             const int e = tid + q * TB;
             Bs[(buf * KC + e % KC) * TN + e / KC] = rb[q];
         }
     };
-    fetch(0);
+    fetch(kb);
     stage(0);
     __syncthreads();
     int buf = 0;
-    for (int32_t kc = 0; kc < m; kc += KC) {
-        const bool more = kc + KC < m;
+    for (int32_t kc = kb; kc < ke; kc += KC) {
+        const bool more = kc + KC < ke;
         if (more) fetch(kc + KC);
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 4) {
             const int kk = ks + (lane >> 4);
-            T a[2], b[SJ];
+            T a[2], b[Cf::SJ];
 #pragma unroll
             for (int i = 0; i < 2; i++) a[i] = As[(buf * KC + kk) * TM + wr + i * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < SJ; j++) b[j] = Bs[(buf * KC + kk) * TN + wc + j * 16 + (lane & 15)];
+            for (int j = 0; j < Cf::SJ; j++) b[j] = Bs[(buf * KC + kk) * TN + wc + j * 16 + (lane & 15)];
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
-                for (int j = 0; j < SJ; j++) {
+                for (int j = 0; j < Cf::SJ; j++) {
                     if constexpr (CPLX) {
                         accr[i][j] = mfma16(a[i].x, b[j].x, accr[i][j]);
                         accr[i][j] = mfma16(-a[i].y, b[j].y, accr[i][j]);
@@ -251,15 +220,11 @@ __global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ ti
         __syncthreads();
         buf ^= 1;
     }
-    // the tile through LDS (the staging buffers are free now), then column by column with lanes along the rows: each
-    // wavefront's 64 rows of one column are contiguous in W wherever rowmap keeps the block's rows together.  A tile
-    // owned by its launch alone (no other task of the level writes its rows) updates W with plain loads and stores.
-    T* const Cs = stg;   // [TN][TM], row r of column c at (r + c) % TM (no bank conflicts either way)
-    static_assert(TN * TM <= 2 * KC * (TM + TN), "tile staging does not fit");
+    T* const Cs = stg;   // the staging buffers are free now
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < SJ; j++) {
+        for (int j = 0; j < Cf::SJ; j++) {
             const int c = wc + j * 16 + (lane & 15);
 #pragma unroll
             for (int v = 0; v < 4; v++) {
@@ -271,23 +236,103 @@ __global__ __launch_bounds__(TB) void k_blk_gemm(const GemmTile* __restrict__ ti
             }
         }
     __syncthreads();
+}
+
+// the rows below a new block's triangle in inverse form, one 64 x 64 tile: IV[w + r, c] = sum_k L21[r, k] L11^-1[k, c]
+// for k from the tile's first column on (L11^-1 is lower triangular)
+template <class T>
+__global__ __launch_bounds__(TB) void k_blk_ivgemm(const GemmTile* __restrict__ tiles, const BlockDesc* __restrict__ blocks)
+{
+    constexpr int TM = 64;
+    __shared__ T stg[TileCfg<T, TM>::STG];
+    const GemmTile tl = tiles[blockIdx.x];
+    const BlockDesc B = blocks[tl.task];
+    const int32_t r0 = B.w + tl.row0;
+    const int32_t nr = min(TM, B.nrows - r0), nc = min(TN, B.w - tl.col0);
+    const T* L = reinterpret_cast<const T*>(B.vals);
+    T* IV = reinterpret_cast<T*>(B.ivals);
+    auto la = [&](int r, int32_t k) { return r < nr ? L[(int64_t)k * B.nrows + r0 + r] : Sc<T>::zero(); };
+    auto lb = [&](int32_t k, int c) { return c < nc ? IV[k + (int64_t)(tl.col0 + c) * B.nrows] : Sc<T>::zero(); };
+    tile_mfma<T, TM>(tl.col0, B.w, la, lb, stg);
+    const int r = threadIdx.x % TM;
+    if (r < nr)
+        for (int c = threadIdx.x / TM; c < nc; c += TB / TM) IV[r0 + r + (int64_t)(tl.col0 + c) * B.nrows] = stg[c * TM + ((r + c) & (TM - 1))];
+}
+
+// one tile of a task's update, rows [row0, row0 + TM) of the block from the segment start k0 on: the product
+// IV[rows, k0:w] S with S the segment's rows of W.  Rows of the segment give the solved segment X, stored compact in the
+// task's slice of xbuf (W's segment rows are still being read by the task's other tiles); rows below are subtracted
+// from W through rowmap, the relative-index scatter -- column by column with lanes along the rows, so each wavefront's
+// rows of one column are contiguous in W wherever rowmap keeps the block's rows together.  A level of one task writes
+// its rows alone (plain loads and stores); with several, rows may be shared (atomics).
+template <class T, int TM>
+__global__ __launch_bounds__(TB) void k_blk_upd(const GemmTile* __restrict__ tiles, const UpdTask* __restrict__ tasks,
+                                                 const int32_t* __restrict__ cols, const BlockDesc* __restrict__ blocks,
+                                                 const int32_t* __restrict__ rowmap, double* __restrict__ W, int64_t ldw,
+                                                 double* __restrict__ xbuf, int shared_rows)
+{
+    constexpr bool CPLX = Sc<T>::vs == 2;
+    __shared__ T stg[TileCfg<T, TM>::STG];
+    __shared__ int32_t spos[TB];   // W positions of the segment rows (a segment is at most 256 rows)
+    __shared__ int32_t rpos[TM];
+    __shared__ int32_t cidx[TN];
+    const GemmTile tl = tiles[blockIdx.x];
+    const UpdTask tk = tasks[tl.task];
+    const BlockDesc B = blocks[tk.block];
+    const int32_t m = B.w - tk.k0;
+    const int32_t nr = min(TM, B.nrows - tl.row0), nc = min(TN, tk.ncol - tl.col0);
+    const int tid = threadIdx.x;
+    if (tid < m) spos[tid] = rowmap[B.rows[tk.k0 + tid]];
+    if (tid < TM) rpos[tid] = (tid < nr && tl.row0 + tid >= B.w) ? rowmap[B.rows[tl.row0 + tid]] : 0;
+    if (tid < TN) cidx[tid] = tid < nc ? cols[tk.col_off + tl.col0 + tid] : 0;
+    __syncthreads();
+    const T* IV = reinterpret_cast<const T*>(B.ivals);
+    const T* Wt = reinterpret_cast<const T*>(W);
+    auto la = [&](int r, int32_t k) { return r < nr ? IV[(int64_t)k * B.nrows + tl.row0 + r] : Sc<T>::zero(); };
+    auto lb = [&](int32_t k, int c) { return c < nc ? Wt[spos[k - tk.k0] + (int64_t)cidx[c] * ldw] : Sc<T>::zero(); };
+    tile_mfma<T, TM>(tk.k0, B.w, la, lb, stg);
     const int r = tid % TM;
-    if (r < nr) {
-        const int64_t rp = rpos[r];
-        for (int c = tid / TM; c < nc; c += TB / TM) {
-            T* p = reinterpret_cast<T*>(W) + rp + (int64_t)cidx[c] * ldw;
-            const T val = Cs[c * TM + ((r + c) & (TM - 1))];
-            if (shared_rows) {
-                if constexpr (CPLX) {
-                    atomicAdd(&p->x, -val.x);
-                    atomicAdd(&p->y, -val.y);
-                } else {
-                    atomicAdd(p, -val);
-                }
+    if (r >= nr) return;
+    const int32_t row = tl.row0 + r;
+    if (row < B.w) {
+        T* X = reinterpret_cast<T*>(xbuf) + tk.xoff + (row - tk.k0);
+        for (int c = tid / TM; c < nc; c += TB / TM) X[(int64_t)(tl.col0 + c) * m] = stg[c * TM + ((r + c) & (TM - 1))];
+        return;
+    }
+    const int64_t rp = rpos[r];
+    for (int c = tid / TM; c < nc; c += TB / TM) {
+        T* p = reinterpret_cast<T*>(W) + rp + (int64_t)cidx[c] * ldw;
+        const T val = stg[c * TM + ((r + c) & (TM - 1))];
+        if (shared_rows) {
+            if constexpr (CPLX) {
+                atomicAdd(&p->x, -val.x);
+                atomicAdd(&p->y, -val.y);
             } else {
-                *p = Sc<T>::sub(*p, val);
+                atomicAdd(p, -val);
             }
+        } else {
+            *p = Sc<T>::sub(*p, val);
         }
+    }
+}
+
+// the solved segments back into W (one workgroup per task), for the U values copied out after the panel
+template <class T>
+__global__ __launch_bounds__(TB) void k_x_back(const UpdTask* __restrict__ tasks, const int32_t* __restrict__ cols,
+                                                const BlockDesc* __restrict__ blocks, const int32_t* __restrict__ rowmap,
+                                                double* __restrict__ W, int64_t ldw, const double* __restrict__ xbuf)
+{
+    __shared__ int32_t spos[TB];
+    const UpdTask tk = tasks[blockIdx.x];
+    const BlockDesc B = blocks[tk.block];
+    const int32_t m = B.w - tk.k0;
+    if ((int)threadIdx.x < m) spos[threadIdx.x] = rowmap[B.rows[tk.k0 + threadIdx.x]];
+    __syncthreads();
+    const T* X = reinterpret_cast<const T*>(xbuf) + tk.xoff;
+    const int64_t cnt = (int64_t)m * tk.ncol;
+    for (int64_t e = threadIdx.x; e < cnt; e += TB) {
+        const int32_t kk = (int32_t)(e % m), c = (int32_t)(e / m);
+        reinterpret_cast<T*>(W)[spos[kk] + (int64_t)cols[tk.col_off + c] * ldw] = X[e];
     }
 }
 
@@ -361,7 +406,7 @@ __global__ __launch_bounds__(LU_TB) void k_getf2(double* __restrict__ Wd, int64_
 // The same sub-panel LU on many workgroups at once (a cooperative launch: every workgroup is resident).  Workgroup g
 // holds the sub-panel's nb columns of the positions a + 256 g + t in registers, one row per thread; per column, every
 // workgroup posts its best candidate (modulus, row, the row's values) and the owner of position j posts row j, then
-// all meet at a barrier (a monotonic counter), pick the same pivot (largest modulus, lowest row on ties) and update
+// all meet at a barrier (a counter that only grows: this launch's arrivals start at `base`), pick the same pivot (largest modulus, lowest row on ties) and update
 // their rows.  Slots alternate between two buffers by column parity: a workgroup can be at most one barrier ahead.
 template <class T> struct Cand {
     double val;
@@ -375,7 +420,7 @@ __global__ __launch_bounds__(TB) void k_getf2_coop(double* __restrict__ Wd, int6
                                                     int32_t nb, const double* __restrict__ colmax, double pivot_rel,
                                                     int32_t* __restrict__ piv, PanelStatus* __restrict__ st,
                                                     Cand<T>* __restrict__ cand, T* __restrict__ rowj,
-                                                    int32_t* __restrict__ counter)
+                                                    int32_t* __restrict__ counter, int32_t base)
 {
     __shared__ double rv[TB / 64];
     __shared__ int32_t rr[TB / 64];
@@ -432,7 +477,7 @@ __global__ __launch_bounds__(TB) void k_getf2_coop(double* __restrict__ Wd, int6
         if (tid == 0) {
             __threadfence();
             __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const int32_t target = (jj + 1) * G;
+            const int32_t target = base + (jj + 1) * G;
             long spins = 0;
             int ab = 0;
             while (__hip_atomic_load(counter, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
@@ -645,38 +690,54 @@ void scatter_cols(int vs, int32_t ncol, const int64_t* Bp, const int32_t* Bi, co
     if (vs == 1) hipLaunchKernelGGL(k_scatter_cols<double>, dim3(g), dim3(TB), 0, s, ncol, Bp, Bi, Bx, j0, rowmap, W, ldw, colmax);
     else hipLaunchKernelGGL(k_scatter_cols<Zd>, dim3(g), dim3(TB), 0, s, ncol, Bp, Bi, Bx, j0, rowmap, W, ldw, colmax);
 }
-void blk_trsm(int vs, int32_t ntile, const TrsmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
-              const int32_t* rowmap, double* W, int64_t ldw, double* xbuf, double flops, hipStream_t s)
+void blk_inverse(int vs, int32_t njobs, const int2* jobs, int32_t ntile, const GemmTile* tiles, const BlockDesc* blocks,
+                 double flops, hipStream_t s)
 {
+    if (njobs <= 0) return;
+    ProfScope ps("blk_inverse", flops, s);
+    if (vs == 1) hipLaunchKernelGGL(k_blk_linv<double>, dim3(njobs), dim3(TB), 0, s, jobs, blocks);
+    else hipLaunchKernelGGL(k_blk_linv<Zd>, dim3(njobs), dim3(TB), 0, s, jobs, blocks);
     if (ntile <= 0) return;
-    ProfScope ps("blk_trsm", flops, s);
-    if (vs == 1) hipLaunchKernelGGL(k_blk_trsm<double>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf);
-    else hipLaunchKernelGGL(k_blk_trsm<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf);
+    if (vs == 1) hipLaunchKernelGGL(k_blk_ivgemm<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks);
+    else hipLaunchKernelGGL(k_blk_ivgemm<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks);
 }
-void blk_gemm(int vs, int32_t ntile, const GemmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
-              const int32_t* rowmap, double* W, int64_t ldw, const double* xbuf, int shared_rows, int tile_rows, double flops,
-              hipStream_t s)
+void blk_upd(int vs, int32_t ntile, const GemmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
+             const int32_t* rowmap, double* W, int64_t ldw, double* xbuf, int shared_rows, int tile_rows, double flops,
+             hipStream_t s)
 {
     if (ntile <= 0) return;
-    ProfScope ps("blk_gemm", flops, s);
+    ProfScope ps("blk_upd", flops, s);
     if (tile_rows == 32) {
         if (vs == 1)
-            hipLaunchKernelGGL((k_blk_gemm<double, 32>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
-        else hipLaunchKernelGGL((k_blk_gemm<Zd, 32>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
+            hipLaunchKernelGGL((k_blk_upd<double, 32>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
+        else hipLaunchKernelGGL((k_blk_upd<Zd, 32>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
     } else {
         if (vs == 1)
-            hipLaunchKernelGGL((k_blk_gemm<double, 64>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
-        else hipLaunchKernelGGL((k_blk_gemm<Zd, 64>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
+            hipLaunchKernelGGL((k_blk_upd<double, 64>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
+        else hipLaunchKernelGGL((k_blk_upd<Zd, 64>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
     }
 }
+void x_back(int vs, int32_t ntask, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks, const int32_t* rowmap,
+            double* W, int64_t ldw, const double* xbuf, hipStream_t s)
+{
+    if (ntask <= 0) return;
+    if (vs == 1) hipLaunchKernelGGL(k_x_back<double>, dim3(ntask), dim3(TB), 0, s, tasks, cols, blocks, rowmap, W, ldw, xbuf);
+    else hipLaunchKernelGGL(k_x_back<Zd>, dim3(ntask), dim3(TB), 0, s, tasks, cols, blocks, rowmap, W, ldw, xbuf);
+}
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
-           double pivot_rel, int32_t* piv, PanelStatus* st, void* coop, int32_t* counter, hipStream_t s)
+           double pivot_rel, int32_t* piv, PanelStatus* st, void* coop, int32_t* counter, int64_t* counted, hipStream_t s)
 {
     ProfScope ps("getf2", (vs == 1 ? 2.0 : 8.0) * (double)nr * nb * nb / 2, s);
     const int32_t G = (int32_t)((nr - a + TB - 1) / TB);
     if (G >= 1 && G <= getf2_coop_max_wg() && nb <= PNB) {
-        k::fill_i32(counter, 1, 0, s);
-        void* args[] = {&Wl, &ld, &nr, &P, &a, &nb, (void*)&colmax, &pivot_rel, &piv, &st, &coop, (void*)&coop, &counter};
+        // the barrier counter keeps growing across launches (nb G arrivals each); it is cleared before it could overflow
+        if (*counted + (int64_t)nb * G > ((int64_t)1 << 30)) {
+            k::fill_i32(counter, 1, 0, s);
+            *counted = 0;
+        }
+        int32_t arrived = (int32_t)*counted;
+        *counted += (int64_t)nb * G;
+        void* args[] = {&Wl, &ld, &nr, &P, &a, &nb, (void*)&colmax, &pivot_rel, &piv, &st, &coop, (void*)&coop, &counter, &arrived};
         // cand (2 G slots) first in coop, then rowj (2 PNB scalars)
         char* base = static_cast<char*>(coop);
         void* rowj = base + (size_t)2 * G * (vs == 1 ? sizeof(Cand<double>) : sizeof(Cand<Zd>));

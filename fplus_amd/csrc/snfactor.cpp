@@ -193,12 +193,15 @@ private:
     DevBuf<int32_t> d_rowmap_, d_univ_, d_cols_, d_piv_, d_pos_;
     DevBuf<double> d_colmax_;
     DevBuf<UpdTask> d_tasks_;
-    DevBuf<TrsmTile> d_ttiles_;
+    DevBuf<int2> d_ijobs_;           // the new blocks' inverse forms: (block, column) jobs and the product tiles
+    DevBuf<GemmTile> d_itiles_;
+    std::vector<int32_t> new_blocks_;   // the blocks made by the current panel
     DevBuf<GemmTile> d_tiles_;
     DevBuf<double> d_X_;
     DevBuf<PanelStatus> d_status_;
     DevBuf<double> d_coop_;          // the cooperative panel LU's candidate slots
     DevBuf<int32_t> d_coop_cnt_;     // and its barrier counter
+    int64_t coop_counted_ = 0;       // the arrivals the counter holds
     HostBuf<int32_t> h_piv_;         // the panel's pivot positions, copied back sub-panel by sub-panel
     HostBuf<PanelStatus> h_st_;      // the panel LU's status after each sub-panel
     EventList sp_ev_;                // the end of each sub-panel's copies
@@ -216,6 +219,7 @@ void Driver::add_block(int64_t s, int64_t b0, int64_t b1, int64_t j0, const std:
     const int64_t r0 = b0 - S.c0, nrows = S.nr() - r0, w = b1 - b0;
     BlockDesc B;
     B.vals = vals_.take((size_t)(nrows * w * vs_));
+    B.ivals = vals_.take((size_t)(nrows * w * vs_));
     B.rows = rows_.take((size_t)nrows);
     B.nrows = (int32_t)nrows;
     B.w = (int32_t)w;
@@ -239,6 +243,7 @@ void Driver::add_block(int64_t s, int64_t b0, int64_t b1, int64_t j0, const std:
         ex.push_back(e);
     }
     const int32_t id = (int32_t)F_.blocks.size();
+    new_blocks_.push_back(id);
     F_.blocks.push_back(B);
     F_.block_rows.push_back(std::move(hr));
     S.blocks.push_back(id);
@@ -357,7 +362,7 @@ int Driver::panel(int64_t j0, int64_t j1)
     F_.rows_lower_max = std::max(F_.rows_lower_max, Rl);
     // ---- update tasks per union supernode and block, with dependency levels: a block's task solves the block's pivot
     // rows the panel's columns reach (read: they wait for every task writing them) and writes the rows below
-    const int TC = trsm_cols(vs_), RT = FPA_ROW_TILE, CTG = FPA_COL_TILE;
+    const int RT = FPA_ROW_TILE, CTG = FPA_COL_TILE;
     const double fm = vs_ == 1 ? 2.0 : 8.0, es = 8.0 * vs_;
     std::vector<int64_t> ucnt(un.size() + 1, 0);
     for (size_t q = 0; q < ext_s.size(); q++) ucnt[uidx_[ext_s[q]] + 1]++;
@@ -395,7 +400,7 @@ int Driver::panel(int64_t j0, int64_t j1)
             t.level = 0;
             tl.push_back(t);
             F_.gemm_flops += fm * (double)ncol * (double)(B.w - k0) * (double)(B.nrows - B.w);
-            F_.gemm_bytes += es * ((double)(B.nrows - B.w) * (double)(B.w - k0) + (double)(B.w - k0) * (double)ncol +
+            F_.gemm_bytes += es * ((double)(B.nrows - k0) * (double)(B.w - k0) + 2.0 * (double)(B.w - k0) * (double)ncol +
                                    2.0 * (double)(B.nrows - B.w) * (double)ncol);
         }
     }
@@ -442,15 +447,14 @@ int Driver::panel(int64_t j0, int64_t j1)
         std::vector<int64_t> nt64(maxlevel + 2, 0);
         for (const TaskL& x : tl) {
             const BlockDesc& B = F_.blocks[x.t.block];
-            nt64[x.level] += (int64_t)((B.nrows - B.w + RT - 1) / RT) * ((x.t.ncol + CTG - 1) / CTG);
+            nt64[x.level] += (int64_t)((B.nrows - x.t.k0 + RT - 1) / RT) * ((x.t.ncol + CTG - 1) / CTG);
         }
         const int64_t fill = 2 * (int64_t)devctx().ncu;
         for (int32_t L = 0; L <= maxlevel + 1; L++) lev_rt[L] = nt64[L] < fill ? RT / 2 : RT;
     }
     std::vector<UpdTask> tasks(tl.size());
-    std::vector<int32_t> lev_tt(maxlevel + 2, 0), lev_gt(maxlevel + 2, 0);
-    std::vector<double> lev_gflops(maxlevel + 2, 0.0), lev_tflops(maxlevel + 2, 0.0);
-    std::vector<TrsmTile> ttiles;
+    std::vector<int32_t> lev_gt(maxlevel + 2, 0);
+    std::vector<double> lev_flops(maxlevel + 2, 0.0);   // the TRSM's and the GEMM's algorithmic flops
     std::vector<GemmTile> gtiles;
     int64_t xtot = 0;
     for (size_t i = 0; i < tl.size(); i++) {
@@ -461,19 +465,13 @@ int Driver::panel(int64_t j0, int64_t j1)
         const int64_t m = B.w - T.k0;
         T.xoff = xtot;
         xtot += m * T.ncol;
-        lev_tflops[L] += fm / 2 * (double)T.ncol * (double)m * (double)(m - 1);
-        lev_gflops[L] += fm * (double)T.ncol * (double)m * (double)(B.nrows - B.w);
-        for (int32_t cc = 0; cc < T.ncol; cc += TC) ttiles.push_back(TrsmTile{(int32_t)i, cc});
-        lev_tt[L + 1] = (int32_t)ttiles.size();
+        lev_flops[L] += fm * (double)T.ncol * (double)m * ((double)(m - 1) / 2 + (double)(B.nrows - B.w));
         const int32_t rt = lev_rt[L];
-        for (int32_t r0 = B.w; r0 < B.nrows; r0 += rt)
+        for (int32_t r0 = T.k0; r0 < B.nrows; r0 += rt)
             for (int32_t cc = 0; cc < T.ncol; cc += CTG) gtiles.push_back(GemmTile{(int32_t)i, r0, cc, 0});
         lev_gt[L + 1] = (int32_t)gtiles.size();
     }
-    for (int32_t L = 1; L <= maxlevel + 1; L++) {
-        lev_tt[L] = std::max(lev_tt[L], lev_tt[L - 1]);
-        lev_gt[L] = std::max(lev_gt[L], lev_gt[L - 1]);
-    }
+    for (int32_t L = 1; L <= maxlevel + 1; L++) lev_gt[L] = std::max(lev_gt[L], lev_gt[L - 1]);
     // a level of one task writes its rows alone (plain loads and stores); with several, rows may be shared (atomics)
     std::vector<int> lev_shared(maxlevel + 2, 0), lev_ntask(maxlevel + 2, 0);
     for (const TaskL& x : tl) lev_ntask[x.level]++;
@@ -495,20 +493,15 @@ int Driver::panel(int64_t j0, int64_t j1)
         d_tasks_.upload(tasks.data(), tasks.size(), s_);
         d_cols_.reserve(cols.size());
         d_cols_.upload(cols.data(), cols.size(), s_);
-        d_ttiles_.reserve(ttiles.size());
-        d_ttiles_.upload(ttiles.data(), ttiles.size(), s_);
         d_tiles_.reserve(gtiles.size() > 0 ? gtiles.size() : 1);
         d_tiles_.upload(gtiles.data(), gtiles.size(), s_);
         d_X_.reserve((size_t)(xtot * vs_));
         for (int32_t L = 1; L <= maxlevel; L++) {
-            const int32_t t0 = lev_tt[L], t1 = lev_tt[L + 1];
-            if (t1 <= t0) continue;
-            k::blk_trsm(vs_, t1 - t0, d_ttiles_.p + t0, d_tasks_.p, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, d_X_.p,
-                        lev_tflops[L], s_);
             const int32_t g0 = lev_gt[L], g1 = lev_gt[L + 1];
-            k::blk_gemm(vs_, g1 - g0, d_tiles_.p + g0, d_tasks_.p, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, d_X_.p,
-                        lev_shared[L], lev_rt[L], lev_gflops[L], s_);
+            k::blk_upd(vs_, g1 - g0, d_tiles_.p + g0, d_tasks_.p, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, d_X_.p,
+                       lev_shared[L], lev_rt[L], lev_flops[L], s_);
         }
+        k::x_back(vs_, (int32_t)tasks.size(), d_tasks_.p, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, d_X_.p, s_);
     }
     double* Wl = d_W_.p + Ru * vs_;
     PanelStatus zero_st;
@@ -524,7 +517,7 @@ int Driver::panel(int64_t j0, int64_t j1)
     for (int64_t a = 0, sp = 0; a < np; a += FPA_PANEL_NB, sp++) {
         const int32_t nb = (int32_t)std::min<int64_t>(FPA_PANEL_NB, np - a);
         k::getf2(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_colmax_.p, opt_.pivot_rel, d_piv_.p, d_status_.p, d_coop_.p,
-                 d_coop_cnt_.p, s_);
+                 d_coop_cnt_.p, &coop_counted_, s_);
         k::trsm_sub(vs_, Wl, ldw, (int32_t)np, (int32_t)a, nb, d_status_.p, s_);
         k::gemm_sub(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_status_.p, s_);
         hip_check(hipMemcpyAsync(h_piv_.p + a, d_piv_.p + a, nb * sizeof(int32_t), hipMemcpyDeviceToHost, s_), "pivots D2H");
@@ -687,6 +680,7 @@ int Driver::panel(int64_t j0, int64_t j1)
     // ---- device: the new blocks and the U values out of the workspace
     std::vector<ExtractCol> ex;
     std::vector<int32_t> expos;
+    new_blocks_.clear();
     for (size_t t = 0; t < touched.size(); t++) {
         const int64_t s = touched[t].first, b0 = touched[t].second;
         const int64_t b1 = std::min<int64_t>(sn_[s].c0 + sn_[s].w, j1);
@@ -699,6 +693,24 @@ int Driver::panel(int64_t j0, int64_t j1)
         d_pos_.reserve(expos.size());
         d_pos_.upload(expos.data(), expos.size(), s_);
         k::extract_cols(vs_, (int32_t)ex.size(), d_ex_.p, d_pos_.p, Wl, ldw, s_);
+    }
+    // the new blocks' inverse forms, for the updates of later panels
+    std::vector<int2> ijobs;
+    std::vector<GemmTile> itiles;
+    double iflops = 0;
+    for (const int32_t b : new_blocks_) {
+        const BlockDesc& B = F_.blocks[b];
+        for (int32_t c = 0; c < B.w; c++) ijobs.push_back(make_int2(b, c));
+        for (int32_t r0 = 0; r0 < B.nrows - B.w; r0 += 64)
+            for (int32_t c0 = 0; c0 < B.w; c0 += 64) itiles.push_back(GemmTile{b, r0, c0, 0});
+        iflops += (vs_ == 1 ? 2.0 : 8.0) * (double)B.w * (double)B.w * ((double)B.w / 6 + (double)(B.nrows - B.w) / 2);
+    }
+    if (!ijobs.empty()) {
+        d_ijobs_.reserve(ijobs.size());
+        d_ijobs_.upload(ijobs.data(), ijobs.size(), s_);
+        d_itiles_.reserve(itiles.size() > 0 ? itiles.size() : 1);
+        d_itiles_.upload(itiles.data(), itiles.size(), s_);
+        k::blk_inverse(vs_, (int32_t)ijobs.size(), d_ijobs_.p, (int32_t)itiles.size(), d_itiles_.p, d_blocks_p(), iflops, s_);
     }
     {
         const int64_t cnt = Up_[j1] - u_begin;
@@ -761,6 +773,8 @@ int Driver::run()
     d_status_.alloc(1);
     d_coop_.alloc(k::getf2_coop_bytes(vs_) / sizeof(double) + 1);
     d_coop_cnt_.alloc(1);
+    k::fill_i32(d_coop_cnt_.p, 1, 0, s_);
+    coop_counted_ = 0;
     F_.d_blocks.alloc(nn);
     F_.val_chunks.clear();
     F_.row_chunks.clear();

@@ -18,9 +18,9 @@ int fplus_amd_symbolic_from_orders(int64_t n, const int64_t *Ap, const int64_t *
 
 /* Statistics of a factorization: nnz(L), nnz(U), supernodes, device blocks, panels, dependency levels, factor flops
  * c * sum_k l_k u_k (c = 2 real, 8 complex), update GEMM flops, and the seconds spent in Ruiz, in the factorization,
- * on the host side of it, and waiting for the device, the Ruiz iterations, and the update GEMM's algorithmic bytes
- * (each update reads its L rows, its U segment and its target rows once and writes the target rows once).  For
- * benchmarks and logs. */
+ * on the host side of it, and waiting for the device, the Ruiz iterations, and the block updates' algorithmic bytes
+ * (each update reads its block's rows in inverse form and its segment once, writes the solved segment once, and reads
+ * and writes its target rows once).  For benchmarks and logs. */
 typedef struct {
     int64_t nnzL, nnzU, nsupernodes, nblocks, panels, levels;
     double flops, gemm_flops;
