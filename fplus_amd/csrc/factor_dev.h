@@ -32,13 +32,26 @@ struct UpdTask {
     int32_t block, k0;
     int32_t col_off, ncol;   // panel-local column indices at cols[col_off .. col_off + ncol)
     int64_t xoff;            // the solved segment, compact: X[(k - k0) + c * (w - k0)] at xbuf + xoff scalars
+    int32_t spos0, pad;      // the W row of the segment's first row (the segment's rows are consecutive in W)
 };
 
-// a product tile: for blk_upd, rows [row0, row0 + tile rows) of the task's block and columns [col0, col0 + 64) of the
-// task's list; for blk_inverse, `task` is the block and rows [w + row0, ...), columns [col0, col0 + 64) of the block
+// a tile of a new block's inverse form (blk_inverse): rows [w + row0, ...) and columns [col0, col0 + 64) of block `task`
 struct GemmTile {
     int32_t task, row0, col0, pad;
 };
+
+// a tile of a task's update (blk_upd) with all its workgroup needs in one 64-byte read: rows [row0, row0 + tile rows)
+// of the block, the task's columns cols[cfirst .. cfirst + nc) (nc <= 64)
+struct alignas(16) UpdTile {
+    const double* iv;      // the block's inverse form (ld = nrows)
+    const int32_t* rows;   // the block's rows
+    double* x;             // the task's solved segment from the tile's first column on: X[(row - k0) + c * (w - k0)]
+    int32_t nrows, w, k0, row0;
+    int32_t cfirst, nc;
+    int32_t spos0;         // the W row of the segment's first row (the segment's rows are consecutive in W)
+    int32_t shared;        // another task of the launch may write the tile's rows below the triangle: atomic updates
+};
+static_assert(sizeof(UpdTile) == 64, "UpdTile is one 64-byte read");
 
 // one run of U entries out of the panel workspace: Ui[uoff + t] = row0 + t, Ux[uoff + t] = W[woff + t], t < len
 struct URun {
@@ -69,15 +82,14 @@ void u_runs(int vs, int32_t nruns, const URun* runs, const double* W, double* Ux
 // the inverse forms of new blocks: jobs (block, column) for the triangles' inverses, then the tiles of the rows below
 void blk_inverse(int vs, int32_t njobs, const int2* jobs, int32_t ntile, const GemmTile* tiles, const BlockDesc* blocks,
                  double flops, hipStream_t s);
-// the tiles of one dependency level: each task's solved segment into its slice of xbuf, its rows below subtracted from
-// W through rowmap -- atomically when shared_rows (two tasks of the level write a common row), else with plain loads
-// and stores; tile_rows: 64, or 32 (tiles of half the rows, for levels with few tiles)
-void blk_upd(int vs, int32_t ntile, const GemmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
-             const int32_t* rowmap, double* W, int64_t ldw, double* xbuf, int shared_rows, int tile_rows, double flops,
-             hipStream_t s);
+// the tiles of one dependency level: each task's solved segment into its slice of X, its rows below subtracted from W
+// through rowmap (atomically for tiles marked shared); tile_rows: 64, or 32 (tiles of half the rows, for levels with
+// few tiles)
+void blk_upd(int vs, int32_t ntile, const UpdTile* tiles, const int32_t* cols, const int32_t* rowmap, double* W, int64_t ldw,
+             int tile_rows, double flops, hipStream_t s);
 // the solved segments of all tasks back into W's rows
-void x_back(int vs, int32_t ntask, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks, const int32_t* rowmap,
-            double* W, int64_t ldw, const double* xbuf, hipStream_t s);
+void x_back(int vs, int32_t ntask, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks, double* W, int64_t ldw,
+            const double* xbuf, hipStream_t s);
 // dense partial-pivot LU of the panel's lower section (nr x P, column-major, ld), sub-panel [a, a + nb): a cooperative
 // launch over the rows (coop: getf2_coop_bytes(vs) of scratch; counter: one int, zero at first and then only touched
 // here, with *counted the arrivals it holds), one workgroup for very tall panels

@@ -130,20 +130,20 @@ __global__ __launch_bounds__(TB) void k_blk_linv(const int2* __restrict__ jobs, 
 typedef double dx4 __attribute__((ext_vector_type(4)));
 __device__ inline dx4 mfma16(double a, double b, dx4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
-constexpr int KC = 16, TN = 64;   // k per staged chunk, columns per product tile
+constexpr int KC = 8, TN = 64;   // k per staged chunk, columns per product tile
 template <class T, int TM> struct TileCfg {
     static constexpr int LPA = KC * TM / TB, LPB = KC * TN / TB;   // elements of A and of B each thread stages
     static constexpr int WR = TM / 32, WC = 4 / WR, SJ = TN / WC / 16;   // waves: WR x WC, each 32 x (16 SJ)
     static constexpr int STG = 2 * KC * TM + 2 * KC * TN;   // LDS scalars: two chunks of A[k][row] and B[k][col]
-    static_assert(TN * TM <= STG, "the product tile does not fit the staging buffer");
 };
 
 // C (TM x 64) = A (TM x [kb, ke)) B ([kb, ke) x 64) on the matrix cores (four real products per complex one): k in
-// chunks of 16 staged in LDS, the next chunk read while the current one multiplies.  la(r, k) and lb(k, c) give the
-// operands (zero outside the tile).  The product is left in LDS, C(r, c) at stg[c * TM + (r + c) % TM] (no bank
-// conflicts by rows or by columns), for the caller's write-out.
-template <class T, int TM, class LA, class LB>
-__device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB& lb, T* stg)
+// chunks of 8 staged in LDS, the next chunk read while the current one multiplies.  la(r, k) and lb(k, c) give the
+// operands (zero outside the tile).  The products are formed transposed (B^T A^T), so each lane ends with rows of C
+// along lanes: out(r, c, v) is called for every element, and a column's 16 consecutive rows come from 16 lanes -- the
+// write-out needs no pass through LDS.
+template <class T, int TM, class LA, class LB, class OUT>
+__device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB& lb, const OUT& out, T* stg)
 {
     using Cf = TileCfg<T, TM>;
     constexpr bool CPLX = Sc<T>::vs == 2;
@@ -197,7 +197,7 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB&
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 4) {
             const int kk = ks + (lane >> 4);
-            T a[2], b[Cf::SJ];
+            T a[2], b[Cf::SJ];   // a: A[row lane % 16][k lane / 16]; b: B[k lane / 16][column lane % 16]
 #pragma unroll
             for (int i = 0; i < 2; i++) a[i] = As[(buf * KC + kk) * TM + wr + i * 16 + (lane & 15)];
 #pragma unroll
@@ -207,12 +207,12 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB&
 #pragma unroll
                 for (int j = 0; j < Cf::SJ; j++) {
                     if constexpr (CPLX) {
-                        accr[i][j] = mfma16(a[i].x, b[j].x, accr[i][j]);
-                        accr[i][j] = mfma16(-a[i].y, b[j].y, accr[i][j]);
-                        acci[i][j] = mfma16(a[i].x, b[j].y, acci[i][j]);
-                        acci[i][j] = mfma16(a[i].y, b[j].x, acci[i][j]);
+                        accr[i][j] = mfma16(b[j].x, a[i].x, accr[i][j]);
+                        accr[i][j] = mfma16(-b[j].y, a[i].y, accr[i][j]);
+                        acci[i][j] = mfma16(b[j].x, a[i].y, acci[i][j]);
+                        acci[i][j] = mfma16(b[j].y, a[i].x, acci[i][j]);
                     } else {
-                        accr[i][j] = mfma16(a[i], b[j], accr[i][j]);
+                        accr[i][j] = mfma16(b[j], a[i], accr[i][j]);
                     }
                 }
         }
@@ -220,22 +220,18 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB&
         __syncthreads();
         buf ^= 1;
     }
-    T* const Cs = stg;   // the staging buffers are free now
+    // lane l holds C[row l % 16][column l / 16 + 4 v] of each 16 x 16 part
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
-        for (int j = 0; j < Cf::SJ; j++) {
-            const int c = wc + j * 16 + (lane & 15);
+        for (int j = 0; j < Cf::SJ; j++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                const int r = wr + i * 16 + (lane >> 4) + 4 * v;
                 T val;
                 if constexpr (CPLX) val = zmake(accr[i][j][v], acci[i][j][v]);
                 else val = accr[i][j][v];
-                Cs[c * TM + ((r + c) & (TM - 1))] = val;
+                out(wr + i * 16 + (lane & 15), wc + j * 16 + (lane >> 4) + 4 * v, val);
             }
-        }
-    __syncthreads();
 }
 
 // the rows below a new block's triangle in inverse form, one 64 x 64 tile: IV[w + r, c] = sum_k L21[r, k] L11^-1[k, c]
@@ -253,86 +249,84 @@ __global__ __launch_bounds__(TB) void k_blk_ivgemm(const GemmTile* __restrict__ 
     T* IV = reinterpret_cast<T*>(B.ivals);
     auto la = [&](int r, int32_t k) { return r < nr ? L[(int64_t)k * B.nrows + r0 + r] : Sc<T>::zero(); };
     auto lb = [&](int32_t k, int c) { return c < nc ? IV[k + (int64_t)(tl.col0 + c) * B.nrows] : Sc<T>::zero(); };
-    tile_mfma<T, TM>(tl.col0, B.w, la, lb, stg);
-    const int r = threadIdx.x % TM;
-    if (r < nr)
-        for (int c = threadIdx.x / TM; c < nc; c += TB / TM) IV[r0 + r + (int64_t)(tl.col0 + c) * B.nrows] = stg[c * TM + ((r + c) & (TM - 1))];
+    T* out_base = IV + r0 + (int64_t)tl.col0 * B.nrows;
+    auto out = [&](int r, int c, T v) {
+        if (r < nr && c < nc) out_base[r + (int64_t)c * B.nrows] = v;
+    };
+    tile_mfma<T, TM>(tl.col0, B.w, la, lb, out, stg);
 }
 
 // one tile of a task's update, rows [row0, row0 + TM) of the block from the segment start k0 on: the product
-// IV[rows, k0:w] S with S the segment's rows of W.  Rows of the segment give the solved segment X, stored compact in the
-// task's slice of xbuf (W's segment rows are still being read by the task's other tiles); rows below are subtracted
-// from W through rowmap, the relative-index scatter -- column by column with lanes along the rows, so each wavefront's
-// rows of one column are contiguous in W wherever rowmap keeps the block's rows together.  A level of one task writes
-// its rows alone (plain loads and stores); with several, rows may be shared (atomics).
+// IV[rows, k0:w] S with S the segment's rows of W (consecutive in W from spos0).  Rows of the segment give the solved
+// segment X, stored compact in the task's slice (W's segment rows are still being read by the task's other tiles);
+// rows below are subtracted from W through rowmap, the relative-index scatter -- with atomics where another task of
+// the launch may write the same rows (tl.shared), else with plain loads and stores.
+
 template <class T, int TM>
-__global__ __launch_bounds__(TB) void k_blk_upd(const GemmTile* __restrict__ tiles, const UpdTask* __restrict__ tasks,
-                                                 const int32_t* __restrict__ cols, const BlockDesc* __restrict__ blocks,
-                                                 const int32_t* __restrict__ rowmap, double* __restrict__ W, int64_t ldw,
-                                                 double* __restrict__ xbuf, int shared_rows)
+__device__ inline void upd_tile(const UpdTile& tl, const int32_t* __restrict__ cols, const int32_t* __restrict__ rowmap,
+                                double* __restrict__ W, int64_t ldw, T* stg, int32_t* rpos, int32_t* cidx)
 {
     constexpr bool CPLX = Sc<T>::vs == 2;
-    __shared__ T stg[TileCfg<T, TM>::STG];
-    __shared__ int32_t spos[TB];   // W positions of the segment rows (a segment is at most 256 rows)
-    __shared__ int32_t rpos[TM];
-    __shared__ int32_t cidx[TN];
-    const GemmTile tl = tiles[blockIdx.x];
-    const UpdTask tk = tasks[tl.task];
-    const BlockDesc B = blocks[tk.block];
-    const int32_t m = B.w - tk.k0;
-    const int32_t nr = min(TM, B.nrows - tl.row0), nc = min(TN, tk.ncol - tl.col0);
+    const int32_t m = tl.w - tl.k0, nr = min(TM, tl.nrows - tl.row0), nc = tl.nc;
     const int tid = threadIdx.x;
-    if (tid < m) spos[tid] = rowmap[B.rows[tk.k0 + tid]];
-    if (tid < TM) rpos[tid] = (tid < nr && tl.row0 + tid >= B.w) ? rowmap[B.rows[tl.row0 + tid]] : 0;
-    if (tid < TN) cidx[tid] = tid < nc ? cols[tk.col_off + tl.col0 + tid] : 0;
+    if (tid < TM) rpos[tid] = (tid < nr && tl.row0 + tid >= tl.w) ? rowmap[tl.rows[tl.row0 + tid]] : 0;
+    if (tid < TN) cidx[tid] = tid < nc ? cols[tl.cfirst + tid] : 0;
     __syncthreads();
-    const T* IV = reinterpret_cast<const T*>(B.ivals);
-    const T* Wt = reinterpret_cast<const T*>(W);
-    auto la = [&](int r, int32_t k) { return r < nr ? IV[(int64_t)k * B.nrows + tl.row0 + r] : Sc<T>::zero(); };
-    auto lb = [&](int32_t k, int c) { return c < nc ? Wt[spos[k - tk.k0] + (int64_t)cidx[c] * ldw] : Sc<T>::zero(); };
-    tile_mfma<T, TM>(tk.k0, B.w, la, lb, stg);
-    const int r = tid % TM;
-    if (r >= nr) return;
-    const int32_t row = tl.row0 + r;
-    if (row < B.w) {
-        T* X = reinterpret_cast<T*>(xbuf) + tk.xoff + (row - tk.k0);
-        for (int c = tid / TM; c < nc; c += TB / TM) X[(int64_t)(tl.col0 + c) * m] = stg[c * TM + ((r + c) & (TM - 1))];
-        return;
-    }
-    const int64_t rp = rpos[r];
-    for (int c = tid / TM; c < nc; c += TB / TM) {
-        T* p = reinterpret_cast<T*>(W) + rp + (int64_t)cidx[c] * ldw;
-        const T val = stg[c * TM + ((r + c) & (TM - 1))];
-        if (shared_rows) {
+    const T* IV = reinterpret_cast<const T*>(tl.iv) + tl.row0;
+    const T* S = reinterpret_cast<const T*>(W) + (tl.spos0 - tl.k0);
+    auto la = [&](int r, int32_t k) { return r < nr ? IV[(int64_t)k * tl.nrows + r] : Sc<T>::zero(); };
+    auto lb = [&](int32_t k, int c) { return c < nc ? S[k + (int64_t)cidx[c] * ldw] : Sc<T>::zero(); };
+    T* X = reinterpret_cast<T*>(tl.x) + (tl.row0 - tl.k0);
+    T* Wt = reinterpret_cast<T*>(W);
+    const int32_t rw = tl.w - tl.row0;   // tile rows from here on are below the block's triangle
+    const bool shared = tl.shared != 0;
+    auto out = [&](int r, int c, T v) {
+        if (r >= nr || c >= nc) return;
+        if (r < rw) {
+            X[r + (int64_t)c * m] = v;
+            return;
+        }
+        T* p = Wt + rpos[r] + (int64_t)cidx[c] * ldw;
+        if (shared) {
             if constexpr (CPLX) {
-                atomicAdd(&p->x, -val.x);
-                atomicAdd(&p->y, -val.y);
+                atomicAdd(&p->x, -v.x);
+                atomicAdd(&p->y, -v.y);
             } else {
-                atomicAdd(p, -val);
+                atomicAdd(p, -v);
             }
         } else {
-            *p = Sc<T>::sub(*p, val);
+            *p = Sc<T>::sub(*p, v);
         }
-    }
+    };
+    tile_mfma<T, TM>(tl.k0, tl.w, la, lb, out, stg);
+}
+
+// one workgroup per tile of a dependency level (every task of the level in one launch)
+template <class T, int TM>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(4))) void k_blk_upd(
+    const UpdTile* __restrict__ tiles, const int32_t* __restrict__ cols, const int32_t* __restrict__ rowmap,
+    double* __restrict__ W, int64_t ldw)
+{
+    __shared__ T stg[TileCfg<T, TM>::STG];
+    __shared__ int32_t rpos[TM];
+    __shared__ int32_t cidx[TN];
+    upd_tile<T, TM>(tiles[blockIdx.x], cols, rowmap, W, ldw, stg, rpos, cidx);
 }
 
 // the solved segments back into W (one workgroup per task), for the U values copied out after the panel
 template <class T>
 __global__ __launch_bounds__(TB) void k_x_back(const UpdTask* __restrict__ tasks, const int32_t* __restrict__ cols,
-                                                const BlockDesc* __restrict__ blocks, const int32_t* __restrict__ rowmap,
-                                                double* __restrict__ W, int64_t ldw, const double* __restrict__ xbuf)
+                                                const BlockDesc* __restrict__ blocks, double* __restrict__ W, int64_t ldw,
+                                                const double* __restrict__ xbuf)
 {
-    __shared__ int32_t spos[TB];
     const UpdTask tk = tasks[blockIdx.x];
-    const BlockDesc B = blocks[tk.block];
-    const int32_t m = B.w - tk.k0;
-    if ((int)threadIdx.x < m) spos[threadIdx.x] = rowmap[B.rows[tk.k0 + threadIdx.x]];
-    __syncthreads();
+    const int32_t m = blocks[tk.block].w - tk.k0;
     const T* X = reinterpret_cast<const T*>(xbuf) + tk.xoff;
+    T* S = reinterpret_cast<T*>(W) + tk.spos0;
     const int64_t cnt = (int64_t)m * tk.ncol;
     for (int64_t e = threadIdx.x; e < cnt; e += TB) {
         const int32_t kk = (int32_t)(e % m), c = (int32_t)(e / m);
-        reinterpret_cast<T*>(W)[spos[kk] + (int64_t)cols[tk.col_off + c] * ldw] = X[e];
+        S[kk + (int64_t)cols[tk.col_off + c] * ldw] = X[e];
     }
 }
 
@@ -701,28 +695,25 @@ void blk_inverse(int vs, int32_t njobs, const int2* jobs, int32_t ntile, const G
     if (vs == 1) hipLaunchKernelGGL(k_blk_ivgemm<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks);
     else hipLaunchKernelGGL(k_blk_ivgemm<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks);
 }
-void blk_upd(int vs, int32_t ntile, const GemmTile* tiles, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks,
-             const int32_t* rowmap, double* W, int64_t ldw, double* xbuf, int shared_rows, int tile_rows, double flops,
-             hipStream_t s)
+void blk_upd(int vs, int32_t ntile, const UpdTile* tiles, const int32_t* cols, const int32_t* rowmap, double* W, int64_t ldw,
+             int tile_rows, double flops, hipStream_t s)
 {
     if (ntile <= 0) return;
     ProfScope ps("blk_upd", flops, s);
     if (tile_rows == 32) {
-        if (vs == 1)
-            hipLaunchKernelGGL((k_blk_upd<double, 32>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
-        else hipLaunchKernelGGL((k_blk_upd<Zd, 32>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
+        if (vs == 1) hipLaunchKernelGGL((k_blk_upd<double, 32>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw);
+        else hipLaunchKernelGGL((k_blk_upd<Zd, 32>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw);
     } else {
-        if (vs == 1)
-            hipLaunchKernelGGL((k_blk_upd<double, 64>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
-        else hipLaunchKernelGGL((k_blk_upd<Zd, 64>), dim3(ntile), dim3(TB), 0, s, tiles, tasks, cols, blocks, rowmap, W, ldw, xbuf, shared_rows);
+        if (vs == 1) hipLaunchKernelGGL((k_blk_upd<double, 64>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw);
+        else hipLaunchKernelGGL((k_blk_upd<Zd, 64>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw);
     }
 }
-void x_back(int vs, int32_t ntask, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks, const int32_t* rowmap,
-            double* W, int64_t ldw, const double* xbuf, hipStream_t s)
+void x_back(int vs, int32_t ntask, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks, double* W, int64_t ldw,
+            const double* xbuf, hipStream_t s)
 {
     if (ntask <= 0) return;
-    if (vs == 1) hipLaunchKernelGGL(k_x_back<double>, dim3(ntask), dim3(TB), 0, s, tasks, cols, blocks, rowmap, W, ldw, xbuf);
-    else hipLaunchKernelGGL(k_x_back<Zd>, dim3(ntask), dim3(TB), 0, s, tasks, cols, blocks, rowmap, W, ldw, xbuf);
+    if (vs == 1) hipLaunchKernelGGL(k_x_back<double>, dim3(ntask), dim3(TB), 0, s, tasks, cols, blocks, W, ldw, xbuf);
+    else hipLaunchKernelGGL(k_x_back<Zd>, dim3(ntask), dim3(TB), 0, s, tasks, cols, blocks, W, ldw, xbuf);
 }
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
            double pivot_rel, int32_t* piv, PanelStatus* st, void* coop, int32_t* counter, int64_t* counted, hipStream_t s)

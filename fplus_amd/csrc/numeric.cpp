@@ -678,6 +678,19 @@ int factor_core(fplus_numeric* num, const PatternCache& c, DevBuf<double>& d_Bx,
                "symbolic with pivots %.3fs (search %.3fs, U %.3fs, join %.3fs, prune %.3fs), out %.3fs\n", num->F.t_p1,
                num->F.t_tasks, num->F.t_ws, num->F.t_lev, num->F.t_p3, num->F.t_p3_dfs, num->F.t_p3_u, num->F.t_p3_join,
                num->F.t_p3_prune, num->F.t_out);
+    if (rc == FPLUS_OK && verbose >= 1) {
+        const double* h = num->F.upd_ncol_flops;
+        const double pf = std::max(num->F.upd_prod_flops, 1.0);
+        printf("[FPLUS]   block updates: product %.3e flops, issued on the matrix cores %.3e (%.2fx); by task width <=8 %.1f%%, "
+               "<=16 %.1f%%, <=32 %.1f%%, <=64 %.1f%%, more %.1f%%\n", num->F.upd_prod_flops, num->F.upd_mfma_flops,
+               num->F.upd_mfma_flops / pf, 100 * h[0] / pf, 100 * h[1] / pf, 100 * h[2] / pf, 100 * h[3] / pf, 100 * h[4] / pf);
+        for (int b = 0; b < 13; b++) {
+            const auto& H = num->F.lev_hist[b];
+            if (H.levels)
+                printf("[FPLUS]     levels with <= %5d tiles: %7ld levels, %9ld tiles, issued %.3e flops, mean longest segment %.0f\n",
+                       1 << b, (long)H.levels, (long)H.tiles, H.issued, H.maxm / (double)H.levels);
+        }
+    }
     return rc;
 }
 

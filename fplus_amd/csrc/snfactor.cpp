@@ -196,7 +196,7 @@ private:
     DevBuf<int2> d_ijobs_;           // the new blocks' inverse forms: (block, column) jobs and the product tiles
     DevBuf<GemmTile> d_itiles_;
     std::vector<int32_t> new_blocks_;   // the blocks made by the current panel
-    DevBuf<GemmTile> d_tiles_;
+    DevBuf<UpdTile> d_tiles_;
     DevBuf<double> d_X_;
     DevBuf<PanelStatus> d_status_;
     DevBuf<double> d_coop_;          // the cooperative panel LU's candidate slots
@@ -377,7 +377,7 @@ int Driver::panel(int64_t j0, int64_t j1)
                 ukf[pos] = ext_kf[q];
             }
     }
-    struct TaskL { UpdTask t; int32_t level; };
+    struct TaskL { UpdTask t; int32_t level; };   // t.spos0 is set with the levels
     std::vector<TaskL> tl;
     std::vector<int32_t> cols;
     for (size_t u = 0; u < un.size(); u++) {
@@ -396,7 +396,7 @@ int Driver::panel(int64_t j0, int64_t j1)
             const int64_t k0 = std::max<int64_t>(kmin, bb0) - bb0;
             TaskL t;
             t.t.block = b; t.t.k0 = (int32_t)k0;
-            t.t.col_off = (int32_t)col_off; t.t.ncol = (int32_t)ncol; t.t.xoff = 0;
+            t.t.col_off = (int32_t)col_off; t.t.ncol = (int32_t)ncol; t.t.xoff = 0; t.t.spos0 = 0; t.t.pad = 0;
             t.level = 0;
             tl.push_back(t);
             F_.gemm_flops += fm * (double)ncol * (double)(B.w - k0) * (double)(B.nrows - B.w);
@@ -437,6 +437,10 @@ int Driver::panel(int64_t j0, int64_t j1)
         lev += 1;
         for (int64_t r = m; r < len; r++) wl[hp[r]] = std::max(wl[hp[r]], lev);
         tl[i].level = lev;
+        // the segment's rows are consecutive in W (a union supernode's U rows are one run there)
+        for (int64_t r = 1; r < m; r++)
+            if (hp[r] != hp[0] + r) { errf(err_, errlen_, "internal: a segment's rows are not consecutive in the workspace"); return FPLUS_ERR_INTERNAL; }
+        tl[i].t.spos0 = hp[0];
         maxlevel = std::max(maxlevel, lev);
     }
     F_.t_lev += now_s() - th_ws;
@@ -453,29 +457,64 @@ int Driver::panel(int64_t j0, int64_t j1)
         for (int32_t L = 0; L <= maxlevel + 1; L++) lev_rt[L] = nt64[L] < fill ? RT / 2 : RT;
     }
     std::vector<UpdTask> tasks(tl.size());
-    std::vector<int32_t> lev_gt(maxlevel + 2, 0);
+    std::vector<int32_t> lev_gt(maxlevel + 2, 0);   // the first tile of each level's tasks
     std::vector<double> lev_flops(maxlevel + 2, 0.0);   // the TRSM's and the GEMM's algorithmic flops
-    std::vector<GemmTile> gtiles;
+    std::vector<double> lev_issued(maxlevel + 2, 0.0);
+    std::vector<int32_t> lev_maxm(maxlevel + 2, 0);
+    std::vector<UpdTile> gtiles;
     int64_t xtot = 0;
     for (size_t i = 0; i < tl.size(); i++) {
-        UpdTask& T = tasks[i];
-        T = tl[i].t;
+        tasks[i] = tl[i].t;
+        tasks[i].xoff = xtot;
+        xtot += (int64_t)(F_.blocks[tasks[i].block].w - tasks[i].k0) * tasks[i].ncol;
+    }
+    d_X_.reserve((size_t)std::max<int64_t>(xtot * vs_, 1));
+    // a level of one task writes its rows alone (plain loads and stores); with several, rows may be shared (atomics)
+    std::vector<int> lev_ntask(maxlevel + 2, 0);
+    for (const TaskL& x : tl) lev_ntask[x.level]++;
+    for (size_t i = 0; i < tl.size(); i++) {
+        const UpdTask& T = tasks[i];
         const int32_t L = tl[i].level;
         const BlockDesc& B = F_.blocks[T.block];
         const int64_t m = B.w - T.k0;
-        T.xoff = xtot;
-        xtot += m * T.ncol;
         lev_flops[L] += fm * (double)T.ncol * (double)m * ((double)(m - 1) / 2 + (double)(B.nrows - B.w));
+        {
+            // the product's size against what the tiles issue on the matrix cores (rows, columns and k padded)
+            const double prod = fm * (double)T.ncol * (double)m * (double)(B.nrows - T.k0);
+            const int64_t ntr = (B.nrows - T.k0 + lev_rt[L] - 1) / lev_rt[L], ntc = (T.ncol + CTG - 1) / CTG;
+            const double issued = fm * (double)(ntr * lev_rt[L]) * (double)(ntc * CTG) * (double)((m + 7) / 8 * 8);
+            F_.upd_prod_flops += prod;
+            F_.upd_mfma_flops += issued;
+            lev_issued[L] += issued;
+            lev_maxm[L] = std::max<int32_t>(lev_maxm[L], (int32_t)m);
+            const int bkt = T.ncol <= 8 ? 0 : T.ncol <= 16 ? 1 : T.ncol <= 32 ? 2 : T.ncol <= 64 ? 3 : 4;
+            F_.upd_ncol_flops[bkt] += prod;
+        }
         const int32_t rt = lev_rt[L];
         for (int32_t r0 = T.k0; r0 < B.nrows; r0 += rt)
-            for (int32_t cc = 0; cc < T.ncol; cc += CTG) gtiles.push_back(GemmTile{(int32_t)i, r0, cc, 0});
+            for (int32_t cc = 0; cc < T.ncol; cc += CTG) {
+                UpdTile u;
+                u.iv = B.ivals;
+                u.rows = B.rows;
+                u.x = d_X_.p + (T.xoff + (int64_t)cc * m) * vs_;
+                u.nrows = B.nrows; u.w = B.w; u.k0 = T.k0; u.row0 = r0;
+                u.cfirst = T.col_off + cc; u.nc = std::min<int32_t>(CTG, T.ncol - cc);
+                u.spos0 = T.spos0; u.shared = lev_ntask[L] > 1;
+                gtiles.push_back(u);
+            }
         lev_gt[L + 1] = (int32_t)gtiles.size();
     }
     for (int32_t L = 1; L <= maxlevel + 1; L++) lev_gt[L] = std::max(lev_gt[L], lev_gt[L - 1]);
-    // a level of one task writes its rows alone (plain loads and stores); with several, rows may be shared (atomics)
-    std::vector<int> lev_shared(maxlevel + 2, 0), lev_ntask(maxlevel + 2, 0);
-    for (const TaskL& x : tl) lev_ntask[x.level]++;
-    for (int32_t L = 0; L <= maxlevel + 1; L++) lev_shared[L] = lev_ntask[L] > 1;
+    for (int32_t L = 1; L <= maxlevel; L++) {   // levels by their tile count (powers of two), for the statistics
+        const int32_t nt = lev_gt[L + 1] - lev_gt[L];
+        if (nt <= 0) continue;
+        int bk = 0;
+        while (bk < 12 && (1 << bk) < nt) bk++;
+        F_.lev_hist[bk].levels++;
+        F_.lev_hist[bk].tiles += nt;
+        F_.lev_hist[bk].issued += lev_issued[L];
+        F_.lev_hist[bk].maxm += lev_maxm[L];
+    }
     F_.levels += maxlevel;
     F_.tasks += (int64_t)tasks.size();
     F_.t_tasks += now_s() - th_p1;
@@ -488,6 +527,9 @@ int Driver::panel(int64_t j0, int64_t j1)
     k::fill_f64(d_W_.p, ldw * np * vs_, 0.0, s_);
     d_colmax_.reserve(np);
     k::scatter_cols(vs_, (int32_t)np, d_Bp_, d_Bi_, d_Bx_, j0, d_rowmap_.p, d_W_.p, ldw, d_colmax_.p, s_);
+    PanelStatus zero_st;
+    std::memset(&zero_st, 0, sizeof(zero_st));
+    d_status_.upload(&zero_st, 1, s_);
     if (!tasks.empty()) {
         d_tasks_.reserve(tasks.size());
         d_tasks_.upload(tasks.data(), tasks.size(), s_);
@@ -495,18 +537,13 @@ int Driver::panel(int64_t j0, int64_t j1)
         d_cols_.upload(cols.data(), cols.size(), s_);
         d_tiles_.reserve(gtiles.size() > 0 ? gtiles.size() : 1);
         d_tiles_.upload(gtiles.data(), gtiles.size(), s_);
-        d_X_.reserve((size_t)(xtot * vs_));
         for (int32_t L = 1; L <= maxlevel; L++) {
             const int32_t g0 = lev_gt[L], g1 = lev_gt[L + 1];
-            k::blk_upd(vs_, g1 - g0, d_tiles_.p + g0, d_tasks_.p, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, d_X_.p,
-                       lev_shared[L], lev_rt[L], lev_flops[L], s_);
+            k::blk_upd(vs_, g1 - g0, d_tiles_.p + g0, d_cols_.p, d_rowmap_.p, d_W_.p, ldw, lev_rt[L], lev_flops[L], s_);
         }
-        k::x_back(vs_, (int32_t)tasks.size(), d_tasks_.p, d_cols_.p, d_blocks_p(), d_rowmap_.p, d_W_.p, ldw, d_X_.p, s_);
+        k::x_back(vs_, (int32_t)tasks.size(), d_tasks_.p, d_cols_.p, d_blocks_p(), d_W_.p, ldw, d_X_.p, s_);
     }
     double* Wl = d_W_.p + Ru * vs_;
-    PanelStatus zero_st;
-    std::memset(&zero_st, 0, sizeof(zero_st));
-    d_status_.upload(&zero_st, 1, s_);
     d_piv_.reserve(np);
     // each sub-panel's pivot positions and the status so far come back as soon as it is factored: the host works on
     // a sub-panel's columns while the device factors the next ones

@@ -40,6 +40,12 @@ struct DevFactor {
     double t_p1 = 0, t_tasks = 0, t_p3 = 0, t_out = 0;   // host phases: search, tasks and levels, symbolic with the pivots, blocks and U out
     double t_p3_dfs = 0, t_p3_u = 0, t_p3_join = 0, t_p3_prune = 0;   // the symbolic with the pivots, by step
     double t_ws = 0, t_lev = 0;   // within the tasks and levels: the union order and workspace rows, the tasks' levels
+    // the block updates' products: their flops, what the tiles issue on the matrix cores, the flops by task width
+    // (columns <= 8, 16, 32, 64, more)
+    double upd_prod_flops = 0, upd_mfma_flops = 0, upd_ncol_flops[5] = {0, 0, 0, 0, 0};
+    // the dependency levels by tile count (bucket b: at most 2^b tiles, the last one more): levels, tiles, flops issued
+    // on the matrix cores, and the sum of each level's longest segment
+    struct LevHist { int64_t levels = 0, tiles = 0; double issued = 0, maxm = 0; } lev_hist[13];
     int64_t rows_upper = 0, rows_lower = 0, rows_lower_max = 0;   // workspace rows summed over the panels
 };
 
