@@ -208,6 +208,8 @@ private:
     DevBuf<URun> d_runs_;
     DevBuf<ExtractCol> d_ex_;
     Arena<double> vals_{nullptr, 0, 0, 0};
+    std::vector<DevBuf<double>> iv_chunks_;   // the blocks' inverse forms: needed only while factoring, freed with the driver
+    Arena<double> ivals_{nullptr, 0, 0, 0};
     Arena<int32_t> rows_{nullptr, 0, 0, 0};
     hipStream_t s_ = nullptr;
 };
@@ -219,7 +221,7 @@ void Driver::add_block(int64_t s, int64_t b0, int64_t b1, int64_t j0, const std:
     const int64_t r0 = b0 - S.c0, nrows = S.nr() - r0, w = b1 - b0;
     BlockDesc B;
     B.vals = vals_.take((size_t)(nrows * w * vs_));
-    B.ivals = vals_.take((size_t)(nrows * w * vs_));
+    B.ivals = ivals_.take((size_t)(nrows * w * vs_));
     B.rows = rows_.take((size_t)nrows);
     B.nrows = (int32_t)nrows;
     B.w = (int32_t)w;
@@ -817,6 +819,8 @@ int Driver::run()
     F_.row_chunks.clear();
     vals_.chunks = &F_.val_chunks;
     vals_.min_chunk = (size_t)1 << 25;
+    ivals_.chunks = &iv_chunks_;
+    ivals_.min_chunk = (size_t)1 << 25;
     rows_.chunks = &F_.row_chunks;
     rows_.min_chunk = (size_t)1 << 24;
     const int64_t P = std::max<int64_t>(1, opt_.panel);
@@ -827,7 +831,8 @@ int Driver::run()
             printf("[FPLUS]   [%5.1f%%] column %ld/%ld  supernodes=%ld (panel %ld)\n", 100.0 * (double)std::min(j0 + P, n) / (double)n,
                    (long)std::min(j0 + P, n), (long)n, (long)sn_.size(), (long)P);
     }
-    // the final structure
+    // the final structure (the inverse forms go with the driver: the solves use the blocks' values only)
+    for (BlockDesc& B : F_.blocks) B.ivals = nullptr;
     F_.nsn = (int64_t)sn_.size();
     F_.sn_first.resize(F_.nsn + 1);
     F_.sn_rows.resize(F_.nsn);
