@@ -201,17 +201,20 @@ struct fplus_numeric {
     std::vector<double> bytes_lf_gather;
     int64_t nslots = 0;
     std::vector<int32_t> lev_lft, lev_ubt, lev_utt, lev_ltt;   // level offsets into the push / dot-product tiles
-    DevBuf<int2> d_lft, d_ubt, d_utt, d_ltt;
+    DevBuf<int2> d_lft, d_utt, d_ltt;
+    DevBuf<int4> d_ubt;        // U x = y dot-product tiles: block, first piece, pieces, partial slot
+    std::vector<int32_t> lev_ubr;
+    DevBuf<int4> d_ubr;        // and the partial sums' reduction tiles: block, first of 16 rows, first slot, slots
+    int64_t ub_slots_max = 0;  // the most partial slots of one level
     std::vector<double> bytes_ub_tri;
     std::vector<double> bytes_lf_pull, bytes_lf_push, bytes_ub, bytes_utf, bytes_ltb;
     // U x = y beyond the blocks: a block's pieces are the runs of later columns over its rows, each covering the block's
-    // rows a .. w-1; ordered by a, so row t of the block reads the block's first pcnt[c0 + t] pieces; piece p holds
-    // U(c0 + t, pj[p]) at Ux[pu[p] + t]
-    DevBuf<int64_t> d_pb, d_pu;
-    DevBuf<int32_t> d_pj, d_pcnt;
+    // rows a .. w-1 (a = pa[p]), ordered by a; piece p holds U(c0 + t, pj[p]) at Ux[pu[p] + t]
+    DevBuf<int64_t> d_pu;
+    DevBuf<int32_t> d_pj, d_pa;
     // solve workspace, reused across calls (solves on one numeric object are serialized)
     mutable std::mutex solve_mu;
-    mutable DevBuf<double> d_w, d_contrib, d_b;
+    mutable DevBuf<double> d_w, d_contrib, d_b, d_ub_part;
     // the CSC export, built on first access
     mutable std::mutex mu;
     mutable bool exported = false;
@@ -300,14 +303,12 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         });
         // per block: a counting sort by first row (stable: columns ascending within one first row), the row counts
         pj.resize(np > 0 ? np : 1); pa.resize(np > 0 ? np : 1); pu.resize(np > 0 ? np : 1);
-        std::vector<int32_t> pcnt(n > 0 ? n : 1, 0);
         std::vector<int64_t> h;
         for (int32_t b = 0; b < nb; b++) {
             const BlockDesc& B = F.blocks[b];
             h.assign(B.w + 1, 0);
             for (int64_t o = pb[b]; o < pb[b + 1]; o++) h[ta[o] + 1]++;
             for (int32_t t = 0; t < B.w; t++) h[t + 1] += h[t];
-            for (int32_t t = 0; t < B.w; t++) pcnt[B.c0 + t] = (int32_t)h[t + 1];
             for (int64_t o = pb[b]; o < pb[b + 1]; o++) {
                 const int64_t d = pb[b] + h[ta[o]]++;
                 pj[d] = tj[o]; pa[d] = ta[o]; pu[d] = tu[o];
@@ -315,10 +316,10 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         }
         auto up32v = [s](DevBuf<int32_t>& d, const std::vector<int32_t>& hh) { d.alloc(hh.size()); d.upload(hh.data(), hh.size(), s); };
         auto up64v = [s](DevBuf<int64_t>& d, const std::vector<int64_t>& hh) { d.alloc(hh.size()); d.upload(hh.data(), hh.size(), s); };
-        up64v(num->d_pb, pb);
+        if (np >= ((int64_t)1 << 31)) throw std::length_error("too many U pieces for the solve's 32-bit indices");
         up64v(num->d_pu, pu);
         up32v(num->d_pj, pj);
-        up32v(num->d_pcnt, pcnt);
+        up32v(num->d_pa, pa);
         dev_sync(s);
     }
     for (int32_t b = nb - 1; b >= 0; b--) {
@@ -412,8 +413,11 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         num->bytes_ub.assign(nub, 0.0);
         num->bytes_ub_tri.assign(nub, 0.0);
         num->lev_ubt.assign(nub + 1, 0);
-        std::vector<int2> ubt;
+        std::vector<int4> ubt, ubr;
+        num->lev_ubr.assign(nub + 1, 0);
+        num->ub_slots_max = 0;
         for (size_t L = 0; L < nub; L++) {
+            int32_t slot = 0;
             for (int32_t k2 = num->lev_ub[L]; k2 < num->lev_ub[L + 1]; k2++) {
                 const int32_t b = num->blk_ub[k2];
                 const BlockDesc& B = F.blocks[b];
@@ -421,13 +425,20 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
                 for (int64_t o = pb[b]; o < pb[b + 1]; o++) ent += (double)(B.w - pa[o]);
                 num->bytes_ub[L] += ent * vb + (double)(pb[b + 1] - pb[b]) * (12.0 + R0 * vb) + 2.0 * B.w * R0 * vb;
                 num->bytes_ub_tri[L] += (double)B.w * (B.w + 1) / 2 * vb + 2.0 * B.w * R0 * vb;
-                if (pb[b + 1] > pb[b])
-                    for (int32_t t0 = 0; t0 < B.w; t0 += 4) ubt.push_back(make_int2(b, t0));
+                const int32_t first = slot;
+                for (int64_t p = pb[b]; p < pb[b + 1]; p += k::UB_PC)
+                    ubt.push_back(make_int4(b, (int32_t)p, (int32_t)std::min<int64_t>(k::UB_PC, pb[b + 1] - p), slot++));
+                if (slot > first)
+                    for (int32_t row0 = 0; row0 < B.w; row0 += 16) ubr.push_back(make_int4(b, row0, first, slot - first));
             }
             num->lev_ubt[L + 1] = (int32_t)ubt.size();
+            num->lev_ubr[L + 1] = (int32_t)ubr.size();
+            num->ub_slots_max = std::max<int64_t>(num->ub_slots_max, slot);
         }
         num->d_ubt.alloc(ubt.size() > 0 ? ubt.size() : 1);
         num->d_ubt.upload(ubt.data(), ubt.size(), s);
+        num->d_ubr.alloc(ubr.size() > 0 ? ubr.size() : 1);
+        num->d_ubr.upload(ubr.data(), ubr.size(), s);
         const size_t nut = num->lev_utf.size() - 1, nlt = num->lev_ltb.size() - 1;
         {   // the transposed sweeps' dot-product tiles (4 columns each) per level
             auto tiles_of = [&](const std::vector<int32_t>& lev, const std::vector<int32_t>& blk, std::vector<int32_t>& lt, DevBuf<int2>& d) {
@@ -495,9 +506,12 @@ void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, hi
             k::lfwd_push(vs, num->lev_lft[L + 1] - num->lev_lft[L], num->d_lft.p + num->lev_lft[L], blocks, num->d_slot_off.p, w,
                          num->d_contrib.p, R, num->bytes_lf_push[L], s);
         }
+        num->d_ub_part.reserve((size_t)std::max<int64_t>(num->ub_slots_max, 1) * 256 * 16 * vs);   // a slot: 256 rows x 16
+        double* part = num->d_ub_part.p;
         for (size_t L = 0; L + 1 < num->lev_ub.size(); L++) {
-            k::ub_dots(vs, num->lev_ubt[L + 1] - num->lev_ubt[L], num->d_ubt.p + num->lev_ubt[L], blocks, num->d_pb.p, num->d_pcnt.p,
-                       num->d_pj.p, num->d_pu.p, F.d_Ux.p, w, R, num->bytes_ub[L], s);
+            k::ub_dots(vs, num->lev_ubt[L + 1] - num->lev_ubt[L], num->d_ubt.p + num->lev_ubt[L], blocks, num->d_pa.p, num->d_pj.p,
+                       num->d_pu.p, F.d_Ux.p, w, part, R, num->bytes_ub[L], s);
+            k::ub_red(vs, num->lev_ubr[L + 1] - num->lev_ubr[L], num->d_ubr.p + num->lev_ubr[L], blocks, part, w, R, s);
             k::ub_tri(vs, num->lev_ub[L + 1] - num->lev_ub[L], num->d_blk_ub.p + num->lev_ub[L], blocks, w, R, num->bytes_ub_tri[L], s);
         }
     } else {

@@ -18,10 +18,15 @@ void lfwd_gather(int vs, int32_t nrow, const int32_t* rows, const int64_t* incp,
 void lfwd_pull(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, double* x, int32_t R, double bytes, hipStream_t s);
 void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* slot_off, const double* x,
                double* contrib, int32_t R, double bytes, hipStream_t s);
-// tiles (block, first row) of 4 rows: row c0 + t of a block subtracts the dot product over the block's first pcnt[c0 + t]
-// pieces p (from pb[block]): U(c0 + t, pj[p]) = Ux[pu[p] + t] times x[pj[p]]
-void ub_dots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* pb, const int32_t* pcnt,
-             const int32_t* pj, const int64_t* pu, const double* Ux, double* x, int32_t R, double bytes, hipStream_t s);
+// U x = y per level.  ub_dots tiles (block, first piece, pieces, slot): block row t gets, in slot `slot` of part
+// (part[(slot * 256 + t) * 16 + r] scalars), the sum over the tile's pieces p with pa[p] <= t of U(t, pj[p]) =
+// Ux[pu[p] + t] times x[pj[p]].  ub_red tiles (block, first of 16 rows, first slot, slots): x minus the block's slots.
+// ub_tri: the blocks' upper triangles.
+constexpr int UB_PC = 32;   // pieces per ub_dots tile, staged at once
+void ub_dots(int vs, int32_t ntile, const int4* tiles, const BlockDesc* blocks, const int32_t* pa, const int32_t* pj,
+             const int64_t* pu, const double* Ux, const double* x, double* part, int32_t R, double bytes, hipStream_t s);
+void ub_red(int vs, int32_t ntile, const int4* tiles, const BlockDesc* blocks, const double* part, double* x, int32_t R,
+            hipStream_t s);
 void ub_tri(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, double* x, int32_t R, double bytes, hipStream_t s);
 // the transposed sweeps' dot products (tiles: block, first of 4 columns), before the blocks' triangles
 void utdots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,

@@ -184,38 +184,79 @@ __global__ __launch_bounds__(TB) void k_lfwd_push(const int2* __restrict__ tiles
         if (r < R) contrib[slot * R + r] = acc[r];
 }
 
-// U x = y, part 1: the dot products of a block's pivot rows along their U rows beyond the block (its pieces),
-// one wavefront per row and every right-hand side at once, four rows per workgroup; x_i -= sum in place
+// U x = y, part 1: the products of a block's pivot rows with their U rows beyond the block, one share of the block's
+// pieces per workgroup.  A piece is a later column j's run over the block's rows [a, w), the pieces sorted by a: thread t
+// owns block row t and adds U(t, j) x_j for every piece with a <= t, so consecutive rows read consecutive U entries; the
+// pieces go in batches of UB_PC, whose x_j for all right-hand sides sit in LDS.  A share's sums go to its own slot of
+// `part`; ub_red subtracts a block's slots from x in slot order (a fixed order, the same for any number of right-hand
+// sides).
 template <class T>
-__global__ __launch_bounds__(TB) void k_ub_dots(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
-                                                 const int64_t* __restrict__ pb, const int32_t* __restrict__ pcnt,
-                                                 const int32_t* __restrict__ pj, const int64_t* __restrict__ pu,
-                                                 const double* __restrict__ Uxd, double* __restrict__ xd, int32_t R)
+__global__ __launch_bounds__(TB) void k_ub_dots(const int4* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
+                                                 const int32_t* __restrict__ pa, const int32_t* __restrict__ pj,
+                                                 const int64_t* __restrict__ pu, const double* __restrict__ Uxd,
+                                                 const double* __restrict__ xd, double* __restrict__ partd, int32_t R)
 {
-    const int2 tl = tiles[blockIdx.x];
-    const BlockDesc B = blocks[tl.x];
+    constexpr int RC = RM<T>::v, PC = k::UB_PC;
+    __shared__ int32_t sa[PC];
+    __shared__ int64_t su[PC];
+    __shared__ T sx[PC * RC];
+    const int4 tl = tiles[blockIdx.x];   // block, first piece, pieces, slot
+    const int w = blocks[tl.x].w, t = threadIdx.x;
     const T* Ux = reinterpret_cast<const T*>(Uxd);
-    T* x = reinterpret_cast<T*>(xd);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const int t = tl.y + wid;
-    if (t >= B.w) return;
-    const int64_t i = (int64_t)B.c0 + t, p0 = pb[tl.x], p1 = p0 + pcnt[i];
-    T acc[RM<T>::v];
+    const T* x = reinterpret_cast<const T*>(xd);
+    T acc[RC];
 #pragma unroll
-    for (int r = 0; r < RM<T>::v; r++) acc[r] = Sc<T>::zero();
-    for (int64_t p = p0 + lane; p < p1; p += 64) {
-        const int64_t j = pj[p];
-        const T u = Ux[pu[p] + t];
+    for (int r = 0; r < RC; r++) acc[r] = Sc<T>::zero();
+    for (int32_t b0 = 0; b0 < tl.z; b0 += PC) {
+        const int npc = min(PC, tl.z - b0), p0 = tl.y + b0;
+        __syncthreads();   // the previous batch is done with the LDS
+        if (t < npc) {
+            sa[t] = pa[p0 + t];
+            su[t] = pu[p0 + t];
+        }
+        for (int idx = t; idx < npc * R; idx += TB) {
+            const int q = idx / R, r = idx - q * R;
+            sx[q * RC + r] = x[(int64_t)pj[p0 + q] * R + r];
+        }
+        __syncthreads();
+        if (t < w) {
+#pragma unroll 4
+            for (int q = 0; q < npc; q++) {
+                const T u = sa[q] <= t ? Ux[su[q] + t] : Sc<T>::zero();
 #pragma unroll
-        for (int r = 0; r < RM<T>::v; r++)
-            if (r < R) acc[r] = Sc<T>::fma_(acc[r], u, x[j * R + r]);
+                for (int r = 0; r < RC; r++)
+                    if (r < R) acc[r] = Sc<T>::fma_(acc[r], u, sx[q * RC + r]);
+            }
+        }
     }
+    if (t >= w) return;
+    T* part = reinterpret_cast<T*>(partd) + ((int64_t)tl.w * TB + t) * RC;
 #pragma unroll
-    for (int r = 0; r < RM<T>::v; r++) {
-        if (r >= R) break;
-        const T sm = wave_sum(acc[r]);
-        if (lane == 0) x[i * R + r] = Sc<T>::sub(x[i * R + r], sm);
+    for (int r = 0; r < RC; r++)
+        if (r < R) part[r] = acc[r];
+}
+
+// U x = y, between the parts: x minus a block's partial sums, 16 rows of the block per workgroup (thread: row
+// row0 + tid / 16, right-hand side tid % 16), the slots in order
+template <class T>
+__global__ __launch_bounds__(TB) void k_ub_red(const int4* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
+                                                const double* __restrict__ partd, double* __restrict__ xd, int32_t R)
+{
+    constexpr int RC = RM<T>::v;
+    const int4 tl = tiles[blockIdx.x];   // block, first row, first slot, slots
+    const int t = tl.y + (int)threadIdx.x / RC, r = (int)threadIdx.x % RC;
+    const BlockDesc B = blocks[tl.x];
+    if (t >= B.w || r >= R) return;
+    const T* part = reinterpret_cast<const T*>(partd) + ((int64_t)tl.z * TB + t) * RC + r;
+    T s0 = Sc<T>::zero(), s1 = Sc<T>::zero();
+    int c = 0;
+    for (; c + 1 < tl.w; c += 2) {   // two chains, joined at the end: the order is fixed by the slot count alone
+        s0 = Sc<T>::add(s0, part[(int64_t)c * TB * RC]);
+        s1 = Sc<T>::add(s1, part[(int64_t)(c + 1) * TB * RC]);
     }
+    if (c < tl.w) s0 = Sc<T>::add(s0, part[(int64_t)c * TB * RC]);
+    T* xp = reinterpret_cast<T*>(xd) + ((int64_t)B.c0 + t) * R + r;
+    *xp = Sc<T>::sub(*xp, Sc<T>::add(s0, s1));
 }
 
 // U x = y, part 2 (one workgroup per block): the block's upper triangle, dense in the block's pivot rows (the panel
@@ -576,13 +617,21 @@ void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks
     if (vs == 1) hipLaunchKernelGGL(k_lfwd_push<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, slot_off, x, contrib, R);
     else hipLaunchKernelGGL(k_lfwd_push<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, slot_off, x, contrib, R);
 }
-void ub_dots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* pb, const int32_t* pcnt,
-             const int32_t* pj, const int64_t* pu, const double* Ux, double* x, int32_t R, double bytes, hipStream_t s)
+void ub_dots(int vs, int32_t ntile, const int4* tiles, const BlockDesc* blocks, const int32_t* pa, const int32_t* pj,
+             const int64_t* pu, const double* Ux, const double* x, double* part, int32_t R, double bytes, hipStream_t s)
 {
     if (ntile <= 0) return;
     ProfScope ps("ub_dots", bytes, s);
-    if (vs == 1) hipLaunchKernelGGL(k_ub_dots<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, pb, pcnt, pj, pu, Ux, x, R);
-    else hipLaunchKernelGGL(k_ub_dots<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, pb, pcnt, pj, pu, Ux, x, R);
+    if (vs == 1) hipLaunchKernelGGL(k_ub_dots<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, pa, pj, pu, Ux, x, part, R);
+    else hipLaunchKernelGGL(k_ub_dots<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, pa, pj, pu, Ux, x, part, R);
+}
+void ub_red(int vs, int32_t ntile, const int4* tiles, const BlockDesc* blocks, const double* part, double* x, int32_t R,
+            hipStream_t s)
+{
+    if (ntile <= 0) return;
+    ProfScope ps("ub_red", 0.0, s);
+    if (vs == 1) hipLaunchKernelGGL(k_ub_red<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, part, x, R);
+    else hipLaunchKernelGGL(k_ub_red<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, part, x, R);
 }
 void ub_tri(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, double* x, int32_t R, double bytes, hipStream_t s)
 {
