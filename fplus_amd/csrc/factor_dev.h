@@ -16,10 +16,12 @@ inline int col_tile(int vs) { return vs == 1 ? 64 : 32; }   // panel columns per
 // nrows): the pivot rows of its columns first, in column order, then the supernode's other rows as they stood at the
 // end of that panel.  Values are the L factor below the unit diagonal (the entries above it are U values, unused by
 // the updates).  rows hold row indices of the permuted matrix.  ivals, same shape, is the inverse form the updates use:
-// [L11^-1 ; L21 L11^-1] with L11 the unit lower triangle of the pivot rows and L21 the rows below.
+// [L11^-1 ; L21 L11^-1] with L11 the unit lower triangle of the pivot rows and L21 the rows below (needed while
+// factoring only); linv keeps L11^-1 alone (w x w, ld = w) for the solves.
 struct BlockDesc {
     double* vals;      // nrows * w scalars (a scalar is 1 or 2 doubles)
     double* ivals;     // nrows * w scalars
+    double* linv;      // w * w scalars
     int32_t* rows;     // nrows
     int32_t nrows, w;
     int32_t c0, sn;
@@ -90,13 +92,14 @@ void blk_upd(int vs, int32_t ntile, const UpdTile* tiles, const int32_t* cols, c
 // the solved segments of all tasks back into W's rows
 void x_back(int vs, int32_t ntask, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks, double* W, int64_t ldw,
             const double* xbuf, hipStream_t s);
-// dense partial-pivot LU of the panel's lower section (nr x P, column-major, ld), sub-panel [a, a + nb): a cooperative
-// launch over the rows (coop: getf2_coop_bytes(vs) of scratch; counter: one int, zero at first and then only touched
+// dense partial-pivot LU of the panel's lower section (nr x P, column-major, ld), sub-panel [a, a + nb), and the
+// sub-panel's pivot rows in the columns right of it solved with its unit lower triangle: a cooperative launch over the rows (coop: getf2_coop_bytes(vs) of scratch; counter: one int, zero at first and then only touched
 // here, with *counted the arrivals it holds), one workgroup for very tall panels
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
            double pivot_rel, int32_t* piv, PanelStatus* st, void* coop, int32_t* counter, int64_t* counted, hipStream_t s);
 size_t getf2_coop_bytes(int vs);
 int getf2_coop_max_wg();
+// (getf2's own second step after its one-workgroup form)
 void trsm_sub(int vs, double* Wl, int64_t ld, int32_t P, int32_t a, int32_t nb, const PanelStatus* st, hipStream_t s);
 void gemm_sub(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const PanelStatus* st,
               hipStream_t s);

@@ -76,7 +76,7 @@ template <> __device__ inline Zd shfl_val<Zd>(Zd v, int src) { return zmake(__sh
 // broadcast by a lane shuffle), one barrier, then the rows below subtract the chunk's 16 columns -- starting at the
 // chunk that holds row c (the rows above it stay zero); the next chunk's factor values are read during the current one
 template <class T>
-__global__ __launch_bounds__(TB) void k_blk_linv(const int2* __restrict__ jobs, const BlockDesc* __restrict__ blocks)
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_blk_linv(const int2* __restrict__ jobs, const BlockDesc* __restrict__ blocks)
 {
     constexpr int CH = 16;
     __shared__ T xs[TB];
@@ -122,7 +122,10 @@ __global__ __launch_bounds__(TB) void k_blk_linv(const int2* __restrict__ jobs, 
 #pragma unroll
         for (int u = 0; u < CH; u++) lcur[u] = lnxt[u];
     }
-    if (own) reinterpret_cast<T*>(B.ivals)[t + (int64_t)c * B.nrows] = x;
+    if (own) {
+        reinterpret_cast<T*>(B.ivals)[t + (int64_t)c * B.nrows] = x;
+        reinterpret_cast<T*>(B.linv)[t + (int64_t)c * w] = x;
+    }
 }
 
 // FP64 matrix cores: D(16x16) += A(16x4) B(4x16) per wavefront; lane l supplies A[l % 16][l / 16] and
@@ -410,7 +413,7 @@ template <class T> struct Cand {
 constexpr long COOP_SPIN_LIMIT = 1L << 24;   // a barrier not met after this many polls ends the kernel with an error
 
 template <class T>
-__global__ __launch_bounds__(TB) void k_getf2_coop(double* __restrict__ Wd, int64_t ld, int32_t nr, int32_t P, int32_t a,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_getf2_coop(double* __restrict__ Wd, int64_t ld, int32_t nr, int32_t P, int32_t a,
                                                     int32_t nb, const double* __restrict__ colmax, double pivot_rel,
                                                     int32_t* __restrict__ piv, PanelStatus* __restrict__ st,
                                                     Cand<T>* __restrict__ cand, T* __restrict__ rowj,
@@ -419,6 +422,7 @@ __global__ __launch_bounds__(TB) void k_getf2_coop(double* __restrict__ Wd, int6
     __shared__ double rv[TB / 64];
     __shared__ int32_t rr[TB / 64];
     __shared__ T pvals[PNB];
+    __shared__ T Lsub[PNB][PNB + 1];
     __shared__ int32_t sh_row, sh_gi, sh_abort;
     __shared__ double sh_val;
     T* Wl = reinterpret_cast<T*>(Wd);
@@ -546,11 +550,37 @@ __global__ __launch_bounds__(TB) void k_getf2_coop(double* __restrict__ Wd, int6
         for (int c = 0; c < PNB; c++)
             if (c < nb) Wl[r + (int64_t)(a + c) * ld] = x[c];
     }
+    // the sub-panel's pivot rows in the columns right of it: the unit-lower solve with the sub-panel's pivot block, by
+    // workgroup 0, which holds those rows (positions a .. a + nb - 1) and alone has swapped the columns outside the
+    // sub-panel (the other workgroups never touch them)
+    if (g == 0) {
+        if (tid < PNB) {
+#pragma unroll
+            for (int c = 0; c < PNB; c++) Lsub[tid][c] = (have && tid < nb) ? x[c] : Sc<T>::zero();
+        }
+        __syncthreads();
+        const int32_t c = a + nb + tid;   // one column per thread: the panel is at most TB columns wide
+        if (c < P) {
+            T y[PNB];
+#pragma unroll
+            for (int i = 0; i < PNB; i++) y[i] = i < nb ? Wl[(a + i) + (int64_t)c * ld] : Sc<T>::zero();
+#pragma unroll
+            for (int i = 1; i < PNB; i++) {
+                __asm__ __volatile__("" ::: "memory");   // row i's factor values are read here, not all up front
+#pragma unroll
+                for (int k = 0; k < i; k++) y[i] = Sc<T>::fms(y[i], Lsub[i][k], y[k]);
+            }
+#pragma unroll
+            for (int i = 0; i < PNB; i++)
+                if (i < nb) Wl[(a + i) + (int64_t)c * ld] = y[i];
+        }
+    }
 }
 
-// rows [a, a + nb) of the columns right of the sub-panel: the unit-lower solve with the sub-panel's pivot block
+// rows [a, a + nb) of the columns right of the sub-panel: the unit-lower solve with the sub-panel's pivot block (after
+// the one-workgroup panel LU; the cooperative one does it itself)
 template <class T>
-__global__ __launch_bounds__(TB) void k_trsm_sub(double* __restrict__ Wd, int64_t ld, int32_t P, int32_t a, int32_t nb,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_trsm_sub(double* __restrict__ Wd, int64_t ld, int32_t P, int32_t a, int32_t nb,
                                                   const PanelStatus* __restrict__ st)
 {
     __shared__ T Ls[PNB][PNB + 1];
@@ -567,10 +597,14 @@ __global__ __launch_bounds__(TB) void k_trsm_sub(double* __restrict__ Wd, int64_
 #pragma unroll
     for (int i = 0; i < PNB; i++) x[i] = i < nb ? Wl[(a + i) + (int64_t)c * ld] : Sc<T>::zero();
 #pragma unroll
-    for (int i = 1; i < PNB; i++)
+    for (int i = 1; i < PNB; i++) {
+        __asm__ __volatile__("" ::: "memory");   // row i's factor values are read here, not all up front
 #pragma unroll
         for (int k = 0; k < i; k++) x[i] = Sc<T>::fms(x[i], Ls[i][k], x[k]);
-    for (int i = 0; i < nb; i++) Wl[(a + i) + (int64_t)c * ld] = x[i];
+    }
+#pragma unroll
+    for (int i = 0; i < PNB; i++)
+        if (i < nb) Wl[(a + i) + (int64_t)c * ld] = x[i];
 }
 
 // W[a+nb:nr, a+nb:P] -= W[a+nb:nr, a:a+nb] * W[a:a+nb, a+nb:P]
@@ -720,6 +754,7 @@ void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int
 {
     ProfScope ps("getf2", (vs == 1 ? 2.0 : 8.0) * (double)nr * nb * nb / 2, s);
     const int32_t G = (int32_t)((nr - a + TB - 1) / TB);
+    if (P > TB) throw std::invalid_argument("panel LU: the panel is wider than a workgroup");
     if (G >= 1 && G <= getf2_coop_max_wg() && nb <= PNB) {
         // the barrier counter keeps growing across launches (nb G arrivals each); it is cleared before it could overflow
         if (*counted + (int64_t)nb * G > ((int64_t)1 << 30)) {
@@ -740,6 +775,7 @@ void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int
     }
     if (vs == 1) hipLaunchKernelGGL(k_getf2<double>, dim3(1), dim3(LU_TB), 0, s, Wl, ld, nr, P, a, nb, colmax, pivot_rel, piv, st);
     else hipLaunchKernelGGL(k_getf2<Zd>, dim3(1), dim3(LU_TB), 0, s, Wl, ld, nr, P, a, nb, colmax, pivot_rel, piv, st);
+    trsm_sub(vs, Wl, ld, P, a, nb, st, s);
 }
 size_t getf2_coop_bytes(int vs) { return (size_t)2 * getf2_coop_max_wg() * (vs == 1 ? sizeof(Cand<double>) : sizeof(Cand<Zd>)) + 2 * PNB * 16 + 256; }
 int getf2_coop_max_wg() { return 512; }

@@ -400,7 +400,7 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
                 for (int32_t r0 = 0; r0 < B.nrows - B.w; r0 += 256) tiles.push_back(make_int2(b, r0));
                 double inc = 0;
                 for (int32_t c = 0; c < B.w; c++) inc += (double)(incp[B.c0 + c + 1] - incp[B.c0 + c]);
-                num->bytes_lf_pull[L] += (double)B.w * (B.w + 1) / 2 * vb + inc * (8.0 + R0 * vb) + 2.0 * B.w * R0 * vb;
+                num->bytes_lf_pull[L] += (double)B.w * B.w * vb + inc * (8.0 + R0 * vb) + 2.0 * B.w * R0 * vb;
                 num->bytes_lf_push[L] += (double)(B.nrows - B.w) * B.w * vb + (double)(B.nrows - B.w) * R0 * vb;
             }
             num->lev_lft[L + 1] = (int32_t)tiles.size();

@@ -90,10 +90,12 @@ private:
             cand.assign(nn, 0); stack.assign(nn, 0); pos.assign(nn, 0);
         }
     };
-    int64_t dfs(SWork& w, int64_t root, int64_t col, int64_t top, int64_t stamp, int64_t open, int64_t* ncand) const
+    // the search from supernode `root` entered at column `col`, its rows read from position pos0 of its search list
+    int64_t dfs(SWork& w, int64_t root, int64_t col, int64_t top, int64_t stamp, int64_t open, int64_t* ncand,
+                int64_t pos0 = 0) const
     {
         int64_t sp = 0;
-        w.stack[0] = root; w.smark[root] = stamp; w.seg0[root] = col; w.pos[root] = 0;
+        w.stack[0] = root; w.smark[root] = stamp; w.seg0[root] = col; w.pos[root] = pos0;
         while (sp >= 0) {
             const int64_t s = w.stack[sp];
             int64_t end;
@@ -115,9 +117,9 @@ private:
         }
         return top;
     }
-    int64_t dfs(int64_t root, int64_t col, int64_t top, int64_t stamp, int64_t open, int64_t* ncand)
+    int64_t dfs(int64_t root, int64_t col, int64_t top, int64_t stamp, int64_t open, int64_t* ncand, int64_t pos0 = 0)
     {
-        return dfs(ws0_, root, col, top, stamp, open, ncand);
+        return dfs(ws0_, root, col, top, stamp, open, ncand, pos0);
     }
     bool relax_ok(const HSn& S, int64_t n_miss, int64_t extra) const
     {
@@ -210,6 +212,7 @@ private:
     Arena<double> vals_{nullptr, 0, 0, 0};
     std::vector<DevBuf<double>> iv_chunks_;   // the blocks' inverse forms: needed only while factoring, freed with the driver
     Arena<double> ivals_{nullptr, 0, 0, 0};
+    Arena<double> linv_{nullptr, 0, 0, 0};
     Arena<int32_t> rows_{nullptr, 0, 0, 0};
     hipStream_t s_ = nullptr;
 };
@@ -222,6 +225,7 @@ void Driver::add_block(int64_t s, int64_t b0, int64_t b1, int64_t j0, const std:
     BlockDesc B;
     B.vals = vals_.take((size_t)(nrows * w * vs_));
     B.ivals = ivals_.take((size_t)(nrows * w * vs_));
+    B.linv = linv_.take((size_t)(w * w * vs_));
     B.rows = rows_.take((size_t)nrows);
     B.nrows = (int32_t)nrows;
     B.w = (int32_t)w;
@@ -254,7 +258,7 @@ void Driver::add_block(int64_t s, int64_t b0, int64_t b1, int64_t j0, const std:
 int Driver::panel(int64_t j0, int64_t j1)
 {
     const int64_t np = j1 - j0, n = n_;
-    const int64_t open0 = open_, we_open = open0 >= 0 ? sn_[open0].w : 0;
+    const int64_t open0 = open_, we_open = open0 >= 0 ? sn_[open0].w : 0, nr_open = open0 >= 0 ? sn_[open0].nr() : 0;
     const double th0 = now_s();
     // ---- phase 1: per-column search over the supernodes complete before j0 (src/factor_sn_panel.inc:34-58); the
     // columns' searches are independent (they read the structure as it stood at j0): threads take columns in turn,
@@ -557,7 +561,6 @@ int Driver::panel(int64_t j0, int64_t j1)
         const int32_t nb = (int32_t)std::min<int64_t>(FPA_PANEL_NB, np - a);
         k::getf2(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_colmax_.p, opt_.pivot_rel, d_piv_.p, d_status_.p, d_coop_.p,
                  d_coop_cnt_.p, &coop_counted_, s_);
-        k::trsm_sub(vs_, Wl, ldw, (int32_t)np, (int32_t)a, nb, d_status_.p, s_);
         k::gemm_sub(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_status_.p, s_);
         hip_check(hipMemcpyAsync(h_piv_.p + a, d_piv_.p + a, nb * sizeof(int32_t), hipMemcpyDeviceToHost, s_), "pivots D2H");
         hip_check(hipMemcpyAsync(h_st_.p + sp, d_status_.p, sizeof(PanelStatus), hipMemcpyDeviceToHost, s_), "status D2H");
@@ -611,8 +614,13 @@ int Driver::panel(int64_t j0, int64_t j1)
             else piv_c_.push_back(i);
         }
         if (open0 >= 0 && sn_[open0].w > we_open && ws0_.smark[open0] != stamp2) {
+            // while it stays open, the rows past its pivots that the supernode had when the panel began are all among
+            // the column's candidates (its search before the panel read them) and none has become pivotal (a pivot of
+            // the panel is either in it, moved above its rows past the pivots, or closed it): the walk finds nothing
+            // there and starts at the rows appended since
+            const int64_t skip = open_ == open0 ? std::max<int64_t>(0, nr_open - sn_[open0].w) : 0;
             for (int64_t q = ext_off[ci]; q < ext_off[ci + 1]; q++)
-                if (ext_s[q] == open0) { top = dfs(open0, sn_[open0].c0 + we_open, top, stamp2, open_, &ncand); break; }
+                if (ext_s[q] == open0) { top = dfs(open0, sn_[open0].c0 + we_open, top, stamp2, open_, &ncand, skip); break; }
         }
         for (int64_t i : piv_c_) {
             const int64_t k = pinv_[i];
@@ -821,6 +829,9 @@ int Driver::run()
     vals_.min_chunk = (size_t)1 << 25;
     ivals_.chunks = &iv_chunks_;
     ivals_.min_chunk = (size_t)1 << 25;
+    F_.linv_chunks.clear();
+    linv_.chunks = &F_.linv_chunks;
+    linv_.min_chunk = (size_t)1 << 24;
     rows_.chunks = &F_.row_chunks;
     rows_.min_chunk = (size_t)1 << 24;
     const int64_t P = std::max<int64_t>(1, opt_.panel);

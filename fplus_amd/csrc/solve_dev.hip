@@ -39,7 +39,6 @@ __device__ inline T wave_sum(T v)
     return v;
 }
 
-// L y = b, part 1 (one workgroup per block of the level): pull the contributions into the pivot rows, solve the triangle
 // L y = b, part 0: the pivot rows of a level's blocks subtract the contributions waiting in their slots, one wavefront
 // per row: lane = 4 g + q, group g (of 16) summing the row's slots g, g + 16, ... for the right-hand sides q, q + 4,
 // ...; the groups' sums combined in a fixed tree
@@ -87,69 +86,37 @@ template <class T> __device__ inline T shfl_sc(T v, int src);
 template <> __device__ inline double shfl_sc<double>(double v, int src) { return __shfl(v, src); }
 template <> __device__ inline Zd shfl_sc<Zd>(Zd v, int src) { return zmake(__shfl(v.x, src), __shfl(v.y, src)); }
 
-// L y = b: the block's unit-lower triangle, chunks first to last
+// L y = b, part 1: the block's unit-lower triangle, as the product with its inverse kept from the factorization (dense,
+// zero above the diagonal, column-major with ld = w): thread t owns row t, the block's x for all right-hand sides in LDS
 template <class T>
 __global__ __launch_bounds__(TB) void k_lfwd_pull(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
                                                    double* __restrict__ xd, int32_t R)
 {
-    constexpr int RC = RM<T>::v, CH = 16;
-    __shared__ T xs[TB][RC];
+    constexpr int RC = RM<T>::v;
+    __shared__ T xs[TB * RC];
     const BlockDesc B = blocks[blk[blockIdx.x]];
-    const T* V = reinterpret_cast<const T*>(B.vals);
+    const T* Li = reinterpret_cast<const T*>(B.linv);
     T* x = reinterpret_cast<T*>(xd);
     const int w = B.w, t = threadIdx.x;
-    const bool own = t < w;
-    T y[RC];
-#pragma unroll
-    for (int r = 0; r < RC; r++) y[r] = (own && r < R) ? x[((int64_t)B.c0 + t) * R + r] : Sc<T>::zero();
-    auto fetch = [&](int ch, T* lv) {
-#pragma unroll
-        for (int u = 0; u < CH; u++) {
-            const int k = ch * CH + u;
-            lv[u] = (own && k < t) ? V[(int64_t)k * B.nrows + t] : Sc<T>::zero();
-        }
-    };
-    T lcur[CH], lnxt[CH];
-    fetch(0, lcur);
-    const int nch = (w + CH - 1) / CH;
-    for (int ch = 0; ch < nch; ch++) {
-        const int r0 = ch * CH, r1 = min(w, r0 + CH);
-        if (ch + 1 < nch) fetch(ch + 1, lnxt);
-        if (t >= r0 && t < r1) {
-#pragma unroll
-            for (int u = 0; u < CH; u++) {
-                const int k = r0 + u;
-                if (k >= r1) break;
-                T yk[RC];
-#pragma unroll
-                for (int r = 0; r < RC; r++) yk[r] = shfl_sc<T>(y[r], k & 63);
-                if (t > k) {
-#pragma unroll
-                    for (int r = 0; r < RC; r++)
-                        if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], yk[r]);
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < RC; r++) xs[t][r] = y[r];
-        }
-        __syncthreads();
-        if (own && t >= r1) {
-#pragma unroll
-            for (int u = 0; u < CH; u++) {
-                const int k = r0 + u;
-                if (k >= r1) break;
-#pragma unroll
-                for (int r = 0; r < RC; r++)
-                    if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], xs[k][r]);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < CH; u++) lcur[u] = lnxt[u];
+    for (int idx = t; idx < w * R; idx += TB) {
+        const int c = idx / R, r = idx - c * R;
+        xs[c * RC + r] = x[((int64_t)B.c0 + c) * R + r];
     }
-    if (!own) return;
+    __syncthreads();
+    if (t >= w) return;
+    T acc[RC];
+#pragma unroll
+    for (int r = 0; r < RC; r++) acc[r] = Sc<T>::zero();
+#pragma unroll 8
+    for (int c = 0; c < w; c++) {
+        const T v = Li[t + (int64_t)c * w];
+#pragma unroll
+        for (int r = 0; r < RC; r++)
+            if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, xs[c * RC + r]);
+    }
 #pragma unroll
     for (int r = 0; r < RC; r++)
-        if (r < R) x[((int64_t)B.c0 + t) * R + r] = y[r];
+        if (r < R) x[((int64_t)B.c0 + t) * R + r] = acc[r];
 }
 
 // L y = b, part 2: the rows below a block, 256 rows per workgroup; slot_off[b] is the block's first slot
@@ -158,26 +125,38 @@ __global__ __launch_bounds__(TB) void k_lfwd_push(const int2* __restrict__ tiles
                                                    const int64_t* __restrict__ slot_off, const double* __restrict__ xd,
                                                    double* __restrict__ contribd, int32_t R)
 {
-    __shared__ T ys[TB * RM<T>::v];
+    constexpr int RC = RM<T>::v, CC = 64;   // the block's columns go through LDS CC at a time
+    __shared__ T ys[CC * RC];
     const int2 tl = tiles[blockIdx.x];
     const BlockDesc B = blocks[tl.x];
     const T* V = reinterpret_cast<const T*>(B.vals);
     const T* x = reinterpret_cast<const T*>(xd);
     T* contrib = reinterpret_cast<T*>(contribd);
     const int w = B.w;
-    for (int idx = threadIdx.x; idx < w * R; idx += TB) ys[idx] = x[(B.c0 + idx / R) * R + idx % R];
-    __syncthreads();
     const int32_t row = w + tl.y + threadIdx.x;
-    if (row >= B.nrows) return;
-    T acc[RM<T>::v];
+    const bool own = row < B.nrows;
+    T acc[RC];
 #pragma unroll
-    for (int r = 0; r < RM<T>::v; r++) acc[r] = Sc<T>::zero();
-    for (int c = 0; c < w; c++) {
-        const T v = V[(int64_t)c * B.nrows + row];
+    for (int r = 0; r < RC; r++) acc[r] = Sc<T>::zero();
+    for (int c0 = 0; c0 < w; c0 += CC) {
+        const int cn = min(CC, w - c0);
+        __syncthreads();   // the previous columns are done with the LDS
+        for (int idx = threadIdx.x; idx < cn * R; idx += TB) {
+            const int c = idx / R, r = idx - c * R;
+            ys[c * RC + r] = x[((int64_t)B.c0 + c0 + c) * R + r];
+        }
+        __syncthreads();
+        if (own) {
+#pragma unroll 8
+            for (int c = 0; c < cn; c++) {
+                const T v = V[(int64_t)(c0 + c) * B.nrows + row];
 #pragma unroll
-        for (int r = 0; r < RM<T>::v; r++)
-            if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, ys[c * R + r]);
+                for (int r = 0; r < RC; r++)
+                    if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, ys[c * RC + r]);
+            }
+        }
     }
+    if (!own) return;
     const int64_t slot = slot_off[tl.x] + row - w;
 #pragma unroll
     for (int r = 0; r < RM<T>::v; r++)
@@ -337,7 +316,7 @@ __global__ __launch_bounds__(TB) void k_ub_tri(const int32_t* __restrict__ blk, 
 // U^T y = b, part 1: the dot products of a level's block columns over the rows before their block, tiles of 4 columns
 // (one wavefront per column): x_c -= sum_{i < c0} conj?(U(i, c)) x_i
 template <class T>
-__global__ __launch_bounds__(TB) void k_utdots(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_utdots(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
                                                 const int64_t* __restrict__ Up, const int32_t* __restrict__ Ui,
                                                 const double* __restrict__ Uxd, double* __restrict__ xd, int32_t R, int conj)
 {
@@ -370,7 +349,7 @@ __global__ __launch_bounds__(TB) void k_utdots(const int2* __restrict__ tiles, c
 // L^T x = y, part 1: the dot products of a level's block columns over the block's rows below, tiles of 4 columns:
 // x_c -= sum_{rows i below} conj?(L(i, c)) x_pinv(i)
 template <class T>
-__global__ __launch_bounds__(TB) void k_ltdots(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_ltdots(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
                                                 const int32_t* __restrict__ pinv_rows, double* __restrict__ xd, int32_t R, int conj)
 {
     const int2 tl = tiles[blockIdx.x];
