@@ -50,7 +50,7 @@ PEAK_FP64_MFMA = 78.6e12      # MI355X dense FP64 matrix, FLOP/s
 PEAK_HBM = 8.0e12             # MI355X HBM3E, B/s
 NRHS = 16
 # the dominant kernel's HBM bytes per launch, from the rocprofv3 FETCH_SIZE and WRITE_SIZE passes summarized there
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blk_gemm-hbm_traffic-shiftinvert-64x4x1024.txt")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blk_upd-hbm_traffic-shiftinvert-64x4x1024.txt")
 
 
 def measured_traffic(workload, kernel):

@@ -67,6 +67,13 @@ struct PanelStatus {
     double best, thr;    // the failing column's best modulus and threshold
 };
 
+// the panel LU's status and pivot positions, together: one copy brings both back to the host
+#define FPA_PANEL_MAX 256   // the widest panel
+struct PanelOut {
+    PanelStatus st;
+    int32_t piv[FPA_PANEL_MAX];
+};
+
 // one block column copied out of the panel workspace: dst[r] = Wl[pos[pos_off + r] + ci * ldw], r < nrows
 struct ExtractCol {
     double* dst;

@@ -323,14 +323,13 @@ __global__ __launch_bounds__(TB) void k_x_back(const UpdTask* __restrict__ tasks
                                                 const double* __restrict__ xbuf)
 {
     const UpdTask tk = tasks[blockIdx.x];
-    const int32_t m = blocks[tk.block].w - tk.k0;
+    const int32_t m = blocks[tk.block].w - tk.k0;   // at most 256 = TB rows: one per thread
     const T* X = reinterpret_cast<const T*>(xbuf) + tk.xoff;
     T* S = reinterpret_cast<T*>(W) + tk.spos0;
-    const int64_t cnt = (int64_t)m * tk.ncol;
-    for (int64_t e = threadIdx.x; e < cnt; e += TB) {
-        const int32_t kk = (int32_t)(e % m), c = (int32_t)(e / m);
-        S[kk + (int64_t)cols[tk.col_off + c] * ldw] = X[e];
-    }
+    const int t = threadIdx.x;
+    if (t >= m) return;
+#pragma unroll 4
+    for (int32_t c = 0; c < tk.ncol; c++) S[t + (int64_t)cols[tk.col_off + c] * ldw] = X[t + (int64_t)c * m];
 }
 
 // ---- dense partial-pivot LU of the panel's lower section

@@ -21,6 +21,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstddef>
 #include <cstring>
 #include <numeric>
 #include <stdexcept>
@@ -192,7 +193,7 @@ private:
     std::vector<int32_t> Ur_row_, Ur_len_;   // U as runs of consecutive rows
     // device workspace
     DevBuf<double> d_W_;
-    DevBuf<int32_t> d_rowmap_, d_univ_, d_cols_, d_piv_, d_pos_;
+    DevBuf<int32_t> d_rowmap_, d_univ_, d_cols_, d_pos_;
     DevBuf<double> d_colmax_;
     DevBuf<UpdTask> d_tasks_;
     DevBuf<int2> d_ijobs_;           // the new blocks' inverse forms: (block, column) jobs and the product tiles
@@ -200,12 +201,11 @@ private:
     std::vector<int32_t> new_blocks_;   // the blocks made by the current panel
     DevBuf<UpdTile> d_tiles_;
     DevBuf<double> d_X_;
-    DevBuf<PanelStatus> d_status_;
+    DevBuf<PanelOut> d_pout_;        // the panel LU's status and pivots
     DevBuf<double> d_coop_;          // the cooperative panel LU's candidate slots
     DevBuf<int32_t> d_coop_cnt_;     // and its barrier counter
     int64_t coop_counted_ = 0;       // the arrivals the counter holds
-    HostBuf<int32_t> h_piv_;         // the panel's pivot positions, copied back sub-panel by sub-panel
-    HostBuf<PanelStatus> h_st_;      // the panel LU's status after each sub-panel
+    HostBuf<PanelOut> h_pout_;       // per sub-panel: the status and the pivots so far, copied back as it ends
     EventList sp_ev_;                // the end of each sub-panel's copies
     DevBuf<URun> d_runs_;
     DevBuf<ExtractCol> d_ex_;
@@ -535,7 +535,7 @@ int Driver::panel(int64_t j0, int64_t j1)
     k::scatter_cols(vs_, (int32_t)np, d_Bp_, d_Bi_, d_Bx_, j0, d_rowmap_.p, d_W_.p, ldw, d_colmax_.p, s_);
     PanelStatus zero_st;
     std::memset(&zero_st, 0, sizeof(zero_st));
-    d_status_.upload(&zero_st, 1, s_);
+    hip_check(hipMemcpyAsync(&d_pout_.p->st, &zero_st, sizeof(zero_st), hipMemcpyHostToDevice, s_), "status upload");
     if (!tasks.empty()) {
         d_tasks_.reserve(tasks.size());
         d_tasks_.upload(tasks.data(), tasks.size(), s_);
@@ -550,20 +550,21 @@ int Driver::panel(int64_t j0, int64_t j1)
         k::x_back(vs_, (int32_t)tasks.size(), d_tasks_.p, d_cols_.p, d_blocks_p(), d_W_.p, ldw, d_X_.p, s_);
     }
     double* Wl = d_W_.p + Ru * vs_;
-    d_piv_.reserve(np);
+    if (np > FPA_PANEL_MAX) { errf(err_, errlen_, "internal: a panel wider than %d columns", FPA_PANEL_MAX); return FPLUS_ERR_INTERNAL; }
+    PanelStatus* d_st = &d_pout_.p->st;
+    int32_t* d_piv = d_pout_.p->piv;
     // each sub-panel's pivot positions and the status so far come back as soon as it is factored: the host works on
     // a sub-panel's columns while the device factors the next ones
-    if ((int64_t)h_piv_.n < np) h_piv_.alloc(np);
     const int64_t nsp = (np + FPA_PANEL_NB - 1) / FPA_PANEL_NB;
-    if ((int64_t)h_st_.n < nsp) h_st_.alloc(nsp);
+    if ((int64_t)h_pout_.n < nsp) h_pout_.alloc(nsp);
     sp_ev_.resize(nsp);
     for (int64_t a = 0, sp = 0; a < np; a += FPA_PANEL_NB, sp++) {
         const int32_t nb = (int32_t)std::min<int64_t>(FPA_PANEL_NB, np - a);
-        k::getf2(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_colmax_.p, opt_.pivot_rel, d_piv_.p, d_status_.p, d_coop_.p,
+        k::getf2(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_colmax_.p, opt_.pivot_rel, d_piv, d_st, d_coop_.p,
                  d_coop_cnt_.p, &coop_counted_, s_);
-        k::gemm_sub(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_status_.p, s_);
-        hip_check(hipMemcpyAsync(h_piv_.p + a, d_piv_.p + a, nb * sizeof(int32_t), hipMemcpyDeviceToHost, s_), "pivots D2H");
-        hip_check(hipMemcpyAsync(h_st_.p + sp, d_status_.p, sizeof(PanelStatus), hipMemcpyDeviceToHost, s_), "status D2H");
+        k::gemm_sub(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_st, s_);
+        hip_check(hipMemcpyAsync(h_pout_.p + sp, d_pout_.p, offsetof(PanelOut, piv) + (a + nb) * sizeof(int32_t),
+                                 hipMemcpyDeviceToHost, s_), "panel status and pivots D2H");
         hip_check(hipEventRecord(sp_ev_.ev[sp], s_), "hipEventRecord");
     }
     const double th1 = now_s();
@@ -580,7 +581,7 @@ int Driver::panel(int64_t j0, int64_t j1)
             const double tw0 = now_s();
             hip_check(hipEventSynchronize(sp_ev_.ev[sp]), "hipEventSynchronize");
             waited += now_s() - tw0;
-            const PanelStatus st = h_st_.p[sp];
+            const PanelStatus st = h_pout_.p[sp].st;
             if (st.err < 0) {
                 dev_sync(s_);
                 errf(err_, errlen_, "internal: the cooperative panel LU's workgroups did not meet at a barrier");
@@ -593,7 +594,7 @@ int Driver::panel(int64_t j0, int64_t j1)
                 return FPLUS_ERR_SINGULAR;
             }
         }
-        const int32_t pv = h_piv_.p[ci];
+        const int32_t pv = h_pout_.p[ci / FPA_PANEL_NB].piv[ci];
         if (pv < ci || pv >= Rl) {
             dev_sync(s_);
             errf(err_, errlen_, "internal: pivot position %d out of range for column %ld", (int)pv, (long)(j0 + ci));
@@ -817,7 +818,7 @@ int Driver::run()
     Ur_len_.reserve((size_t)(nnzA + n + 16));
     d_rowmap_.alloc(nn);
     k::fill_i32(d_rowmap_.p, n, -1, s_);
-    d_status_.alloc(1);
+    d_pout_.alloc(1);
     d_coop_.alloc(k::getf2_coop_bytes(vs_) / sizeof(double) + 1);
     d_coop_cnt_.alloc(1);
     k::fill_i32(d_coop_cnt_.p, 1, 0, s_);
