@@ -2,7 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -80,6 +82,85 @@ template <class T> struct HostBuf {
         if (p) (void)hipHostFree(p);
         p = nullptr; n = 0;
     }
+};
+
+// page-locked staging for a stream's uploads, in two areas used in turn: what is staged in an area goes to the device
+// by asynchronous copies (no staging inside the runtime, no wait for the copy), and an area is reused only after the
+// event recorded behind its last copy has completed -- the host can go on with the next round of work while the copies
+// and the kernels reading them run
+class UploadStage {
+public:
+    UploadStage() = default;
+    UploadStage(const UploadStage&) = delete;
+    UploadStage& operator=(const UploadStage&) = delete;
+    ~UploadStage()
+    {
+        if (last_) (void)hipStreamSynchronize(last_);   // no copy still reads an area
+        for (int i = 0; i < 2; i++) {
+            if (ev_[i]) (void)hipEventDestroy(ev_[i]);
+            for (HostBuf<char>* b : area_[i].bufs) delete b;
+        }
+    }
+    // start a round of uploads in the other area, once its previous round's copies are done
+    void begin()
+    {
+        cur_ ^= 1;
+        Area& a = area_[cur_];
+        if (!ev_[cur_]) hip_check(hipEventCreateWithFlags(&ev_[cur_], hipEventDisableTiming), "hipEventCreateWithFlags");
+        if (a.recorded) hip_check(hipEventSynchronize(ev_[cur_]), "hipEventSynchronize");
+        a.recorded = false;
+        a.chunk = 0;
+        a.used = 0;
+    }
+    // room for cnt T's in the current area, 256-byte aligned, valid until the area's next round
+    template <class T> T* take(size_t cnt)
+    {
+        Area& a = area_[cur_];
+        const size_t need = ((cnt > 0 ? cnt : 1) * sizeof(T) + 255) & ~(size_t)255;
+        while (a.chunk < a.bufs.size() && a.used + need > a.bufs[a.chunk]->n) { a.chunk++; a.used = 0; }
+        if (a.chunk == a.bufs.size()) {
+            const size_t last = a.bufs.empty() ? 0 : a.bufs.back()->n;
+            HostBuf<char>* b = new HostBuf<char>();
+            b->alloc(std::max(need, std::max((size_t)4 << 20, 2 * last)));
+            a.bufs.push_back(b);
+            a.used = 0;
+        }
+        T* p = reinterpret_cast<T*>(a.bufs[a.chunk]->p + a.used);
+        a.used += need;
+        return p;
+    }
+    // cnt T's staged at h (from take) to the device at d, on stream s
+    template <class T> void send(T* d, const T* h, size_t cnt, hipStream_t s)
+    {
+        if (!cnt) return;
+        hip_check(hipMemcpyAsync(d, h, cnt * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
+        last_ = s;
+    }
+    // stage a copy of cnt T's and send it
+    template <class T> void upload(T* d, const T* src, size_t cnt, hipStream_t s)
+    {
+        if (!cnt) return;
+        T* h = take<T>(cnt);
+        std::memcpy(h, src, cnt * sizeof(T));
+        send(d, h, cnt, s);
+    }
+    // the round's copies are all issued: mark their end on the stream
+    void end(hipStream_t s)
+    {
+        hip_check(hipEventRecord(ev_[cur_], s), "hipEventRecord");
+        area_[cur_].recorded = true;
+    }
+
+private:
+    struct Area {
+        std::vector<HostBuf<char>*> bufs;
+        size_t chunk = 0, used = 0;
+        bool recorded = false;
+    };
+    Area area_[2];
+    hipEvent_t ev_[2] = {nullptr, nullptr};
+    hipStream_t last_ = nullptr;
+    int cur_ = 1;
 };
 
 // events that mark points of a stream the host waits for one by one

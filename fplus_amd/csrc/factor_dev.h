@@ -74,16 +74,18 @@ struct PanelOut {
     int32_t piv[FPA_PANEL_MAX];
 };
 
-// one block column copied out of the panel workspace: dst[r] = Wl[pos[pos_off + r] + ci * ldw], r < nrows
+// one block column copied out of the panel workspace: dst[r] = Wl[pos[pos_off + r] + ci * ldw], r < nrows; for a
+// block's first column also its row indices, rdst[r] = lrows[pos[pos_off + r]] (lrows: the row at each lower position)
 struct ExtractCol {
     double* dst;
+    int32_t* rdst;     // or null
     int64_t pos_off;
     int32_t nrows, ci;
 };
 
 namespace k {
-void extract_cols(int vs, int32_t ncols, const ExtractCol* d, const int32_t* pos, const double* Wl, int64_t ldw,
-                  hipStream_t s);
+void extract_cols(int vs, int32_t ncols, const ExtractCol* d, const int32_t* pos, const int32_t* lrows, const double* Wl,
+                  int64_t ldw, hipStream_t s);
 void rowmap_set(int64_t cnt, const int32_t* rows, int32_t* rowmap, int reset, hipStream_t s);
 void scatter_cols(int vs, int32_t ncol, const int64_t* Bp, const int32_t* Bi, const double* Bx, int64_t j0,
                   const int32_t* rowmap, double* W, int64_t ldw, double* colmax, hipStream_t s);

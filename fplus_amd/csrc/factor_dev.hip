@@ -144,9 +144,10 @@ template <class T, int TM> struct TileCfg {
 // chunks of 8 staged in LDS, the next chunk read while the current one multiplies.  la(r, k) and lb(k, c) give the
 // operands (zero outside the tile).  The products are formed transposed (B^T A^T), so each lane ends with rows of C
 // along lanes: out(r, c, v) is called for every element, and a column's 16 consecutive rows come from 16 lanes -- the
-// write-out needs no pass through LDS.
-template <class T, int TM, class LA, class LB, class OUT>
-__device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB& lb, const OUT& out, T* stg)
+// write-out needs no pass through LDS.  pre() runs once the first chunk is staged, before the barrier that publishes
+// it: LDS state the output needs is stored there, from loads that were in flight with the first chunk's.
+template <class T, int TM, class LA, class LB, class OUT, class PRE>
+__device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB& lb, const OUT& out, const PRE& pre, T* stg)
 {
     using Cf = TileCfg<T, TM>;
     constexpr bool CPLX = Sc<T>::vs == 2;
@@ -192,6 +193,7 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB&
     };
     fetch(kb);
     stage(0);
+    pre();
     __syncthreads();
     int buf = 0;
     for (int32_t kc = kb; kc < ke; kc += KC) {
@@ -256,7 +258,7 @@ __global__ __launch_bounds__(TB) void k_blk_ivgemm(const GemmTile* __restrict__ 
     auto out = [&](int r, int c, T v) {
         if (r < nr && c < nc) out_base[r + (int64_t)c * B.nrows] = v;
     };
-    tile_mfma<T, TM>(tl.col0, B.w, la, lb, out, stg);
+    tile_mfma<T, TM>(tl.col0, B.w, la, lb, out, [] {}, stg);
 }
 
 // one tile of a task's update, rows [row0, row0 + TM) of the block from the segment start k0 on: the product
@@ -272,9 +274,13 @@ __device__ inline void upd_tile(const UpdTile& tl, const int32_t* __restrict__ c
     constexpr bool CPLX = Sc<T>::vs == 2;
     const int32_t m = tl.w - tl.k0, nr = min(TM, tl.nrows - tl.row0), nc = tl.nc;
     const int tid = threadIdx.x;
-    if (tid < TM) rpos[tid] = (tid < nr && tl.row0 + tid >= tl.w) ? rowmap[tl.rows[tl.row0 + tid]] : 0;
+    // the columns are needed for the first chunk, the rows' W positions only for the output: their rowmap loads are
+    // issued here and land with the first chunk's
+    int32_t row = -1;
+    if (tid < TM && tid < nr && tl.row0 + tid >= tl.w) row = tl.rows[tl.row0 + tid];
     if (tid < TN) cidx[tid] = tid < nc ? cols[tl.cfirst + tid] : 0;
     __syncthreads();
+    const int32_t rp = row >= 0 ? rowmap[row] : 0;
     const T* IV = reinterpret_cast<const T*>(tl.iv) + tl.row0;
     const T* S = reinterpret_cast<const T*>(W) + (tl.spos0 - tl.k0);
     auto la = [&](int r, int32_t k) { return r < nr ? IV[(int64_t)k * tl.nrows + r] : Sc<T>::zero(); };
@@ -301,7 +307,7 @@ __device__ inline void upd_tile(const UpdTile& tl, const int32_t* __restrict__ c
             *p = Sc<T>::sub(*p, v);
         }
     };
-    tile_mfma<T, TM>(tl.k0, tl.w, la, lb, out, stg);
+    tile_mfma<T, TM>(tl.k0, tl.w, la, lb, out, [&] { if (tid < TM) rpos[tid] = rp; }, stg);
 }
 
 // one workgroup per tile of a dependency level (every task of the level in one launch)
@@ -671,13 +677,17 @@ __global__ void k_u_runs(int32_t nruns, const URun* __restrict__ runs, const dou
 }
 
 template <class T>
-__global__ void k_extract_cols(const ExtractCol* __restrict__ d, const int32_t* __restrict__ pos, const double* __restrict__ Wl,
-                               int64_t ldw)
+__global__ void k_extract_cols(const ExtractCol* __restrict__ d, const int32_t* __restrict__ pos,
+                               const int32_t* __restrict__ lrows, const double* __restrict__ Wl, int64_t ldw)
 {
     const ExtractCol e = d[blockIdx.x];
     const T* src = reinterpret_cast<const T*>(Wl) + (int64_t)e.ci * ldw;
     T* dst = reinterpret_cast<T*>(e.dst);
-    for (int32_t r = threadIdx.x; r < e.nrows; r += TB) dst[r] = src[pos[e.pos_off + r]];
+    for (int32_t r = threadIdx.x; r < e.nrows; r += TB) {
+        const int32_t q = pos[e.pos_off + r];
+        dst[r] = src[q];
+        if (e.rdst) e.rdst[r] = lrows[q];
+    }
 }
 
 inline unsigned gridn(int64_t n, int per)
@@ -691,12 +701,12 @@ inline unsigned gridn(int64_t n, int per)
 }  // namespace
 
 namespace k {
-void extract_cols(int vs, int32_t ncols, const ExtractCol* d, const int32_t* pos, const double* Wl, int64_t ldw,
-                  hipStream_t s)
+void extract_cols(int vs, int32_t ncols, const ExtractCol* d, const int32_t* pos, const int32_t* lrows, const double* Wl,
+                  int64_t ldw, hipStream_t s)
 {
     if (ncols <= 0) return;
-    if (vs == 1) hipLaunchKernelGGL(k_extract_cols<double>, dim3(ncols), dim3(TB), 0, s, d, pos, Wl, ldw);
-    else hipLaunchKernelGGL(k_extract_cols<Zd>, dim3(ncols), dim3(TB), 0, s, d, pos, Wl, ldw);
+    if (vs == 1) hipLaunchKernelGGL(k_extract_cols<double>, dim3(ncols), dim3(TB), 0, s, d, pos, lrows, Wl, ldw);
+    else hipLaunchKernelGGL(k_extract_cols<Zd>, dim3(ncols), dim3(TB), 0, s, d, pos, lrows, Wl, ldw);
 }
 void u_runs(int vs, int32_t nruns, const URun* runs, const double* W, double* Ux, int32_t* Ui, hipStream_t s)
 {

@@ -688,9 +688,10 @@ int factor_core(fplus_numeric* num, const PatternCache& c, DevBuf<double>& d_Bx,
                (double)num->F.rows_upper / std::max<int64_t>(1, num->F.panels), (double)num->F.rows_lower / std::max<int64_t>(1, num->F.panels),
                (long)num->F.rows_lower_max);
     if (rc == FPLUS_OK && verbose >= 1)
-        printf("[FPLUS]   host phases: search %.3fs, tasks/levels %.3fs (union order and workspace rows %.3fs, levels %.3fs), "
-               "symbolic with pivots %.3fs (search %.3fs, U %.3fs, join %.3fs, prune %.3fs), out %.3fs\n", num->F.t_p1,
-               num->F.t_tasks, num->F.t_ws, num->F.t_lev, num->F.t_p3, num->F.t_p3_dfs, num->F.t_p3_u, num->F.t_p3_join,
+        printf("[FPLUS]   host phases: search %.3fs, tasks/levels %.3fs (union order and workspace rows %.3fs, of which rows not "
+               "pivotal %.3fs; tasks and levels %.3fs, of which the level pass %.3fs), symbolic with pivots %.3fs (search %.3fs, "
+               "U %.3fs, join %.3fs, prune %.3fs), out %.3fs\n", num->F.t_p1, num->F.t_tasks, num->F.t_ws, num->F.t_ws_low,
+               num->F.t_lev, num->F.t_lev_pass, num->F.t_p3, num->F.t_p3_dfs, num->F.t_p3_u, num->F.t_p3_join,
                num->F.t_p3_prune, num->F.t_out);
     if (rc == FPLUS_OK && verbose >= 1) {
         const double* h = num->F.upd_ncol_flops;

@@ -155,7 +155,7 @@ private:
     void d_blocks_upload()
     {
         const size_t nb = F_.blocks.size();
-        if (nb > blocks_up_) F_.d_blocks.upload(F_.blocks.data() + blocks_up_, nb - blocks_up_, s_, blocks_up_);
+        if (nb > blocks_up_) up_.upload(F_.d_blocks.p + blocks_up_, F_.blocks.data() + blocks_up_, nb - blocks_up_, s_);
         blocks_up_ = nb;
     }
     size_t blocks_up_ = 0;
@@ -209,6 +209,9 @@ private:
     EventList sp_ev_;                // the end of each sub-panel's copies
     DevBuf<URun> d_runs_;
     DevBuf<ExtractCol> d_ex_;
+    DevBuf<int32_t> d_lrows_;        // the row at each lower workspace position after the panel LU
+    UploadStage up_;                 // every upload of a panel goes through page-locked staging (one area per panel,
+                                     // two in turn): the host goes on while the copies and the kernels run
     Arena<double> vals_{nullptr, 0, 0, 0};
     std::vector<DevBuf<double>> iv_chunks_;   // the blocks' inverse forms: needed only while factoring, freed with the driver
     Arena<double> ivals_{nullptr, 0, 0, 0};
@@ -239,10 +242,10 @@ void Driver::add_block(int64_t s, int64_t b0, int64_t b1, int64_t j0, const std:
         if (lpos[row] < 0) throw std::logic_error("internal: a block row outside the panel's lower rows");
         expos.push_back(lpos[row]);
     }
-    hip_check(hipMemcpyAsync(B.rows, hr.data(), nrows * sizeof(int32_t), hipMemcpyHostToDevice, s_), "upload block rows");
-    for (int64_t c = 0; c < w; c++) {
+    for (int64_t c = 0; c < w; c++) {   // the rows go to the device with the first column (the row at each position)
         ExtractCol e;
         e.dst = B.vals + (size_t)c * nrows * vs_;
+        e.rdst = c == 0 ? B.rows : nullptr;
         e.pos_off = pos_off;
         e.nrows = (int32_t)nrows;
         e.ci = (int32_t)(b0 - j0 + c);
@@ -260,6 +263,7 @@ int Driver::panel(int64_t j0, int64_t j1)
     const int64_t np = j1 - j0, n = n_;
     const int64_t open0 = open_, we_open = open0 >= 0 ? sn_[open0].w : 0, nr_open = open0 >= 0 ? sn_[open0].nr() : 0;
     const double th0 = now_s();
+    up_.begin();
     // ---- phase 1: per-column search over the supernodes complete before j0 (src/factor_sn_panel.inc:34-58); the
     // columns' searches are independent (they read the structure as it stood at j0): threads take columns in turn,
     // each with its own marks, and the results are joined in column order
@@ -298,17 +302,18 @@ int Driver::panel(int64_t j0, int64_t j1)
     ext_off[np] = (int64_t)ext_s.size();
     const double th_p1 = now_s();
     F_.t_p1 += th_p1 - th0;
-    // ---- the union order (src/factor_sn_panel.inc:61-70)
+    // ---- the union of the reached supernodes, in a topological order of the updates (the reference takes one from a
+    // search over the union, src/factor_sn_panel.inc:61-70): increasing supernode index is one -- a block's rows below
+    // its pivot rows were not pivotal when its supernode opened, so the supernodes whose pivot rows they become opened
+    // later, and a supernode's blocks follow in column order.  The order only places the U rows in the workspace and
+    // orders the tasks; the levels are the same for every topological order.
     std::vector<int64_t> un;
     {
         const int64_t us = 2 * n + 1 + j0;
-        int64_t top = n, dummy = 0;
-        for (size_t q = 0; q < ext_s.size(); q++) {
-            if (ws0_.smark[ext_s[q]] == us) continue;
-            top = dfs(ext_s[q], ext_kf[q], top, us, open_, &dummy);
-            dummy = 0;
-        }
-        for (int64_t t = top; t < n; t++) { uidx_[ws0_.topo[t]] = (int64_t)un.size(); ustamp_[ws0_.topo[t]] = us; un.push_back(ws0_.topo[t]); }
+        for (size_t q = 0; q < ext_s.size(); q++)
+            if (ustamp_[ext_s[q]] != us) { ustamp_[ext_s[q]] = us; un.push_back(ext_s[q]); }
+        std::sort(un.begin(), un.end());
+        for (size_t u = 0; u < un.size(); u++) uidx_[un[u]] = (int64_t)u;
     }
     // ---- the workspace rows: the U rows (pivotal before j0) first, each union supernode's reached rows as one run (the
     // union of its columns' segments [kf, we) is the longest one), then the rows not yet pivotal
@@ -335,7 +340,9 @@ int Driver::panel(int64_t j0, int64_t j1)
         // the rows not yet pivotal, in order of first appearance over the columns' candidate lists: chunks of columns
         // list theirs in parallel (each with its own marks), then the lists are joined in chunk order -- the same order
         // as one pass over all the lists
-        const int64_t nch = std::min<int64_t>(np, 4 * (int64_t)pool_.size()), per = (np + nch - 1) / nch;
+        // (one chunk per thread: the join reads every chunk's list, most rows are in all of them)
+        const double tl0 = now_s();
+        const int64_t nch = std::min<int64_t>(np, (int64_t)pool_.size()), per = (np + nch - 1) / nch;
         if ((int64_t)ch_rows_.size() < nch) ch_rows_.resize(nch);
         pool_.run(nch, [&](int64_t c, int tid) {
             SWork& w = wsT_[tid];
@@ -349,6 +356,7 @@ int Driver::panel(int64_t j0, int64_t j1)
         for (int64_t c = 0; c < nch; c++)
             for (const int32_t row : ch_rows_[c])
                 if (hrowpos_[row] < 0) { hrowpos_[row] = (int32_t)univ.size(); univ.push_back(row); }
+        F_.t_ws_low += now_s() - tl0;
     }
     if (open0 >= 0) {
         const HSn& S = sn_[open0];
@@ -435,6 +443,7 @@ int Driver::panel(int64_t j0, int64_t j1)
     }
     std::vector<int32_t> wl(Ru + Rl, 0);
     int32_t maxlevel = 0;
+    const double tlp0 = now_s();
     for (size_t i = 0; i < tl.size(); i++) {
         const int32_t* hp = thp_.data() + thpo_[i];
         const int64_t m = F_.blocks[tl[i].t.block].w - tl[i].t.k0, len = thpo_[i + 1] - thpo_[i];
@@ -450,6 +459,7 @@ int Driver::panel(int64_t j0, int64_t j1)
         maxlevel = std::max(maxlevel, lev);
     }
     F_.t_lev += now_s() - th_ws;
+    F_.t_lev_pass += now_s() - tlp0;
     std::stable_sort(tl.begin(), tl.end(), [](const TaskL& a, const TaskL& b) { return a.level < b.level; });
     // tile rows per level: 64, or 32 where the level's 64-row tiles would not fill the device twice
     std::vector<int32_t> lev_rt(maxlevel + 2, RT);
@@ -467,7 +477,13 @@ int Driver::panel(int64_t j0, int64_t j1)
     std::vector<double> lev_flops(maxlevel + 2, 0.0);   // the TRSM's and the GEMM's algorithmic flops
     std::vector<double> lev_issued(maxlevel + 2, 0.0);
     std::vector<int32_t> lev_maxm(maxlevel + 2, 0);
-    std::vector<UpdTile> gtiles;
+    int64_t ntile = 0;   // the tiles are written straight into the upload staging
+    for (const TaskL& x : tl) {
+        const int64_t rt = lev_rt[x.level];
+        ntile += (int64_t)((F_.blocks[x.t.block].nrows - x.t.k0 + rt - 1) / rt) * ((x.t.ncol + CTG - 1) / CTG);
+    }
+    UpdTile* gtiles = up_.take<UpdTile>((size_t)ntile);
+    int64_t gi = 0;
     int64_t xtot = 0;
     for (size_t i = 0; i < tl.size(); i++) {
         tasks[i] = tl[i].t;
@@ -499,16 +515,15 @@ int Driver::panel(int64_t j0, int64_t j1)
         const int32_t rt = lev_rt[L];
         for (int32_t r0 = T.k0; r0 < B.nrows; r0 += rt)
             for (int32_t cc = 0; cc < T.ncol; cc += CTG) {
-                UpdTile u;
+                UpdTile& u = gtiles[gi++];
                 u.iv = B.ivals;
                 u.rows = B.rows;
                 u.x = d_X_.p + (T.xoff + (int64_t)cc * m) * vs_;
                 u.nrows = B.nrows; u.w = B.w; u.k0 = T.k0; u.row0 = r0;
                 u.cfirst = T.col_off + cc; u.nc = std::min<int32_t>(CTG, T.ncol - cc);
                 u.spos0 = T.spos0; u.shared = lev_ntask[L] > 1;
-                gtiles.push_back(u);
             }
-        lev_gt[L + 1] = (int32_t)gtiles.size();
+        lev_gt[L + 1] = (int32_t)gi;
     }
     for (int32_t L = 1; L <= maxlevel + 1; L++) lev_gt[L] = std::max(lev_gt[L], lev_gt[L - 1]);
     for (int32_t L = 1; L <= maxlevel; L++) {   // levels by their tile count (powers of two), for the statistics
@@ -528,21 +543,21 @@ int Driver::panel(int64_t j0, int64_t j1)
     // ---- device: workspace, scatter, updates, panel LU
     d_W_.reserve((size_t)(ldw * np * vs_));
     d_univ_.reserve(univ.size());
-    d_univ_.upload(univ.data(), univ.size(), s_);
+    up_.upload(d_univ_.p, univ.data(), univ.size(), s_);
     k::rowmap_set((int64_t)univ.size(), d_univ_.p, d_rowmap_.p, 0, s_);
     k::fill_f64(d_W_.p, ldw * np * vs_, 0.0, s_);
     d_colmax_.reserve(np);
     k::scatter_cols(vs_, (int32_t)np, d_Bp_, d_Bi_, d_Bx_, j0, d_rowmap_.p, d_W_.p, ldw, d_colmax_.p, s_);
     PanelStatus zero_st;
     std::memset(&zero_st, 0, sizeof(zero_st));
-    hip_check(hipMemcpyAsync(&d_pout_.p->st, &zero_st, sizeof(zero_st), hipMemcpyHostToDevice, s_), "status upload");
+    up_.upload(&d_pout_.p->st, &zero_st, 1, s_);
     if (!tasks.empty()) {
         d_tasks_.reserve(tasks.size());
-        d_tasks_.upload(tasks.data(), tasks.size(), s_);
+        up_.upload(d_tasks_.p, tasks.data(), tasks.size(), s_);
         d_cols_.reserve(cols.size());
-        d_cols_.upload(cols.data(), cols.size(), s_);
-        d_tiles_.reserve(gtiles.size() > 0 ? gtiles.size() : 1);
-        d_tiles_.upload(gtiles.data(), gtiles.size(), s_);
+        up_.upload(d_cols_.p, cols.data(), cols.size(), s_);
+        d_tiles_.reserve(ntile > 0 ? (size_t)ntile : 1);
+        up_.send(d_tiles_.p, gtiles, (size_t)ntile, s_);
         for (int32_t L = 1; L <= maxlevel; L++) {
             const int32_t g0 = lev_gt[L], g1 = lev_gt[L + 1];
             k::blk_upd(vs_, g1 - g0, d_tiles_.p + g0, d_cols_.p, d_rowmap_.p, d_W_.p, ldw, lev_rt[L], lev_flops[L], s_);
@@ -737,10 +752,12 @@ int Driver::panel(int64_t j0, int64_t j1)
     d_blocks_upload();
     if (!ex.empty()) {
         d_ex_.reserve(ex.size());
-        d_ex_.upload(ex.data(), ex.size(), s_);
+        up_.upload(d_ex_.p, ex.data(), ex.size(), s_);
         d_pos_.reserve(expos.size());
-        d_pos_.upload(expos.data(), expos.size(), s_);
-        k::extract_cols(vs_, (int32_t)ex.size(), d_ex_.p, d_pos_.p, Wl, ldw, s_);
+        up_.upload(d_pos_.p, expos.data(), expos.size(), s_);
+        d_lrows_.reserve(lrows.size());
+        up_.upload(d_lrows_.p, lrows.data(), lrows.size(), s_);
+        k::extract_cols(vs_, (int32_t)ex.size(), d_ex_.p, d_pos_.p, d_lrows_.p, Wl, ldw, s_);
     }
     // the new blocks' inverse forms, for the updates of later panels
     std::vector<int2> ijobs;
@@ -755,9 +772,9 @@ int Driver::panel(int64_t j0, int64_t j1)
     }
     if (!ijobs.empty()) {
         d_ijobs_.reserve(ijobs.size());
-        d_ijobs_.upload(ijobs.data(), ijobs.size(), s_);
+        up_.upload(d_ijobs_.p, ijobs.data(), ijobs.size(), s_);
         d_itiles_.reserve(itiles.size() > 0 ? itiles.size() : 1);
-        d_itiles_.upload(itiles.data(), itiles.size(), s_);
+        up_.upload(d_itiles_.p, itiles.data(), itiles.size(), s_);
         k::blk_inverse(vs_, (int32_t)ijobs.size(), d_ijobs_.p, (int32_t)itiles.size(), d_itiles_.p, d_blocks_p(), iflops, s_);
     }
     {
@@ -777,13 +794,13 @@ int Driver::panel(int64_t j0, int64_t j1)
             F_.d_Ui = std::move(nb);
         }
         d_runs_.reserve(pruns.size());
-        d_runs_.upload(pruns.data(), pruns.size(), s_);
+        up_.upload(d_runs_.p, pruns.data(), pruns.size(), s_);
         k::u_runs(vs_, (int32_t)pruns.size(), d_runs_.p, d_W_.p, F_.d_Ux.p, F_.d_Ui.p, s_);
         F_.nnzU = u_begin + cnt;
     }
     k::rowmap_set((int64_t)univ.size(), d_univ_.p, d_rowmap_.p, 1, s_);
-    // the uploads above read host buffers that go out of scope: finish them before returning
-    dev_sync(s_);
+    // the device finishes the panel while the host starts the next one (the uploads read page-locked staging)
+    up_.end(s_);
     for (int32_t row : univ) hrowpos_[row] = -1;
     for (int32_t row : lrows) lposmap_[row] = -1;
     F_.t_out += now_s() - th_out;
@@ -843,6 +860,7 @@ int Driver::run()
             printf("[FPLUS]   [%5.1f%%] column %ld/%ld  supernodes=%ld (panel %ld)\n", 100.0 * (double)std::min(j0 + P, n) / (double)n,
                    (long)std::min(j0 + P, n), (long)n, (long)sn_.size(), (long)P);
     }
+    dev_sync(s_);   // the last panel's blocks and U values are out
     // the final structure (the inverse forms go with the driver: the solves use the blocks' values only)
     for (BlockDesc& B : F_.blocks) B.ivals = nullptr;
     F_.nsn = (int64_t)sn_.size();

@@ -41,6 +41,7 @@ struct DevFactor {
     double t_p1 = 0, t_tasks = 0, t_p3 = 0, t_out = 0;   // host phases: search, tasks and levels, symbolic with the pivots, blocks and U out
     double t_p3_dfs = 0, t_p3_u = 0, t_p3_join = 0, t_p3_prune = 0;   // the symbolic with the pivots, by step
     double t_ws = 0, t_lev = 0;   // within the tasks and levels: the union order and workspace rows, the tasks' levels
+    double t_ws_low = 0, t_lev_pass = 0;   // of which: the rows not yet pivotal; the sequential pass of the levels
     // the block updates' products: their flops, what the tiles issue on the matrix cores, the flops by task width
     // (columns <= 8, 16, 32, 64, more)
     double upd_prod_flops = 0, upd_mfma_flops = 0, upd_ncol_flops[5] = {0, 0, 0, 0, 0};
