@@ -134,18 +134,23 @@ typedef double dx4 __attribute__((ext_vector_type(4)));
 __device__ inline dx4 mfma16(double a, double b, dx4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
 constexpr int KC = 8, TN = 64;   // k per staged chunk, columns per product tile
+constexpr int BLD = TN + 1;      // LDS row length of B: one element of padding moves each k row to other banks
 template <class T, int TM> struct TileCfg {
     static constexpr int LPA = KC * TM / TB, LPB = KC * TN / TB;   // elements of A and of B each thread stages
     static constexpr int WR = TM / 32, WC = 4 / WR, SJ = TN / WC / 16;   // waves: WR x WC, each 32 x (16 SJ)
-    static constexpr int STG = 2 * KC * TM + 2 * KC * TN;   // LDS scalars: two chunks of A[k][row] and B[k][col]
+    static constexpr int STG = 2 * KC * TM + 2 * KC * BLD;   // LDS scalars: two chunks of A[k][row] and B[k][col]
+    // loads two chunks ahead, except for complex 64-row tiles: their accumulators leave no room for a second register set
+    static constexpr bool DEEP = !(Sc<T>::vs == 2 && TM == 64);
 };
 
 // C (TM x 64) = A (TM x [kb, ke)) B ([kb, ke) x 64) on the matrix cores (four real products per complex one): k in
-// chunks of 8 staged in LDS, the next chunk read while the current one multiplies.  la(r, k) and lb(k, c) give the
-// operands (zero outside the tile).  The products are formed transposed (B^T A^T), so each lane ends with rows of C
-// along lanes: out(r, c, v) is called for every element, and a column's 16 consecutive rows come from 16 lanes -- the
-// write-out needs no pass through LDS.  pre() runs once the first chunk is staged, before the barrier that publishes
-// it: LDS state the output needs is stored there, from loads that were in flight with the first chunk's.
+// chunks of 8 staged in LDS (two buffers), with the global loads two chunks ahead of the products where registers allow
+// (TileCfg::DEEP): chunk i + 2 is read into registers while chunk i multiplies and chunk i + 1 waits in the other
+// register set, so a load has two chunks' products to land; else one chunk ahead.  la(r, k) and lb(k, c) give the operands (zero outside the tile).  The products are formed
+// transposed (B^T A^T), so each lane ends with rows of C along lanes: out(r, c, v) is called for every element, and a
+// column's 16 consecutive rows come from 16 lanes -- the write-out needs no pass through LDS.  pre() runs once the
+// first chunk is staged, before the barrier that publishes it: LDS state the output needs is stored there, from loads
+// that were in flight with the first chunks'.
 template <class T, int TM, class LA, class LB, class OUT, class PRE>
 __device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB& lb, const OUT& out, const PRE& pre, T* stg)
 {
@@ -164,8 +169,8 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB&
             acci[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
         }
     // thread element e = tid + q * TB; A: row e % TM, k e / TM (rows contiguous); B: k e % KC, column e / KC
-    T ra[Cf::LPA], rb[Cf::LPB];
-    auto fetch = [&](int32_t kc) {
+    T ra0[Cf::LPA], rb0[Cf::LPB], ra1[Cf::LPA], rb1[Cf::LPB];
+    auto fetch = [&](int32_t kc, T (&ra)[Cf::LPA], T (&rb)[Cf::LPB]) {
 #pragma unroll
         for (int q = 0; q < Cf::LPA; q++) {
             const int e = tid + q * TB;
@@ -179,7 +184,7 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB&
             rb[q] = k < ke ? lb(k, e / KC) : Sc<T>::zero();
         }
     };
-    auto stage = [&](int buf) {
+    auto stage = [&](int buf, const T (&ra)[Cf::LPA], const T (&rb)[Cf::LPB]) {
 #pragma unroll
         for (int q = 0; q < Cf::LPA; q++) {
             const int e = tid + q * TB;
@@ -188,17 +193,10 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB&
 #pragma unroll
         for (int q = 0; q < Cf::LPB; q++) {
             const int e = tid + q * TB;
-            Bs[(buf * KC + e % KC) * TN + e / KC] = rb[q];
+            Bs[(buf * KC + e % KC) * BLD + e / KC] = rb[q];
         }
     };
-    fetch(kb);
-    stage(0);
-    pre();
-    __syncthreads();
-    int buf = 0;
-    for (int32_t kc = kb; kc < ke; kc += KC) {
-        const bool more = kc + KC < ke;
-        if (more) fetch(kc + KC);
+    auto mma = [&](int buf) {
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 4) {
             const int kk = ks + (lane >> 4);
@@ -206,7 +204,7 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB&
 #pragma unroll
             for (int i = 0; i < 2; i++) a[i] = As[(buf * KC + kk) * TM + wr + i * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < Cf::SJ; j++) b[j] = Bs[(buf * KC + kk) * TN + wc + j * 16 + (lane & 15)];
+            for (int j = 0; j < Cf::SJ; j++) b[j] = Bs[(buf * KC + kk) * BLD + wc + j * 16 + (lane & 15)];
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -221,9 +219,41 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB&
                     }
                 }
         }
-        if (more) stage(buf ^ 1);
+    };
+    fetch(kb, ra0, rb0);
+    if constexpr (!Cf::DEEP) {
+        stage(0, ra0, rb0);
+        pre();
         __syncthreads();
-        buf ^= 1;
+        int buf = 0;
+        for (int32_t kc = kb; kc < ke; kc += KC) {
+            const bool more = kc + KC < ke;
+            if (more) fetch(kc + KC, ra0, rb0);
+            mma(buf);
+            if (more) stage(buf ^ 1, ra0, rb0);
+            __syncthreads();
+            buf ^= 1;
+        }
+    } else {
+    if (kb + KC < ke) fetch(kb + KC, ra1, rb1);
+    stage(0, ra0, rb0);
+    pre();
+    __syncthreads();
+    // the exits depend on kb and ke only: every wave of the workgroup takes the same ones
+    for (int32_t kc = kb;; kc += 2 * KC) {
+        // chunk kc in LDS buffer 0, chunk kc + KC in registers 1
+        if (kc + 2 * KC < ke) fetch(kc + 2 * KC, ra0, rb0);
+        mma(0);
+        if (kc + KC >= ke) break;
+        stage(1, ra1, rb1);
+        __syncthreads();
+        // chunk kc + KC in LDS buffer 1, chunk kc + 2 KC in registers 0
+        if (kc + 3 * KC < ke) fetch(kc + 3 * KC, ra1, rb1);
+        mma(1);
+        if (kc + 2 * KC >= ke) break;
+        stage(0, ra0, rb0);
+        __syncthreads();
+    }
     }
     // lane l holds C[row l % 16][column l / 16 + 4 v] of each 16 x 16 part
 #pragma unroll
