@@ -133,14 +133,15 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
 typedef double dx4 __attribute__((ext_vector_type(4)));
 __device__ inline dx4 mfma16(double a, double b, dx4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 
-constexpr int KC = 8, TN = 64;   // k per staged chunk, columns per product tile
+constexpr int TN = 64;           // columns per product tile
 constexpr int BLD = TN + 1;      // LDS row length of B: one element of padding moves each k row to other banks
 template <class T, int TM> struct TileCfg {
+    // loads two chunks ahead, except for complex 64-row tiles: their accumulators leave no room for a second register set
+    static constexpr bool DEEP = !(Sc<T>::vs == 2 && TM == 64);
+    static constexpr int KC = 8;   // k per staged chunk (one barrier per chunk; 16 was measured slower)
     static constexpr int LPA = KC * TM / TB, LPB = KC * TN / TB;   // elements of A and of B each thread stages
     static constexpr int WR = TM / 32, WC = 4 / WR, SJ = TN / WC / 16;   // waves: WR x WC, each 32 x (16 SJ)
     static constexpr int STG = 2 * KC * TM + 2 * KC * BLD;   // LDS scalars: two chunks of A[k][row] and B[k][col]
-    // loads two chunks ahead, except for complex 64-row tiles: their accumulators leave no room for a second register set
-    static constexpr bool DEEP = !(Sc<T>::vs == 2 && TM == 64);
 };
 
 // C (TM x 64) = A (TM x [kb, ke)) B ([kb, ke) x 64) on the matrix cores (four real products per complex one): k in
@@ -156,6 +157,7 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB&
 {
     using Cf = TileCfg<T, TM>;
     constexpr bool CPLX = Sc<T>::vs == 2;
+    constexpr int KC = Cf::KC;
     T* const As = stg;
     T* const Bs = stg + 2 * KC * TM;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;

@@ -177,8 +177,9 @@ private:
     std::vector<int64_t> npl_l_, piv_c_;   // phase 3: a column's candidates not pivotal / pivotal in the panel
     std::vector<std::vector<int64_t>> c_s_, c_kf_, c_np_;   // phase 1's results per panel column
     std::vector<std::vector<int32_t>> ch_rows_;   // per chunk of columns: the rows not yet pivotal, first appearance
-    std::vector<int32_t> thp_;      // the workspace positions of the update tasks' rows
-    std::vector<int64_t> thpo_;
+    std::vector<int32_t> reader_;   // per workspace U row: the update task whose segment holds it
+    std::vector<std::vector<int32_t>> succ_;   // per update task: the later tasks whose segments its rows below reach
+    std::vector<std::vector<int64_t>> tmark_;   // one mark array per thread of the pool, indexed by task
     SWork ws0_;                     // the sequential phases' marks
     std::vector<SWork> wsT_;        // one per thread of the parallel search
     HostPool pool_;
@@ -418,45 +419,68 @@ int Driver::panel(int64_t j0, int64_t j1)
                                    2.0 * (double)(B.nrows - B.w) * (double)ncol);
         }
     }
-    // the workspace positions of every task's rows (from its segment start on), gathered in parallel; then the levels
-    // in task order: a task is one past the highest level writing its segment rows and raises its rows below to its own
+    // the levels.  Each workspace U row is read by one task (the task of the block whose pivot row it is), so a task's
+    // level is one past the highest level of the tasks writing into its segment.  The tasks' rows are mapped to their
+    // workspace positions in parallel: this checks that a segment's rows are consecutive there and lists, per task, the
+    // distinct tasks whose segments its rows below fall in (always later tasks: the task order is topological); then
+    // one pass in task order over those lists.
+    const int64_t nt = (int64_t)tl.size();
+    reader_.assign((size_t)std::max<int64_t>(Ru, 1), -1);
+    for (int64_t i = 0; i < nt; i++) {
+        const BlockDesc& B = F_.blocks[tl[i].t.block];
+        const int64_t k0 = tl[i].t.k0, m = B.w - k0;
+        const int32_t sp = hrowpos_[F_.block_rows[tl[i].t.block][k0]];
+        if (sp < 0 || sp + m > Ru) { errf(err_, errlen_, "internal: a segment outside the workspace U rows"); return FPLUS_ERR_INTERNAL; }
+        tl[i].t.spos0 = sp;
+        for (int64_t r = 0; r < m; r++) reader_[sp + r] = (int32_t)i;
+    }
+    if ((int64_t)succ_.size() < nt) succ_.resize(nt);
+    for (std::vector<int64_t>& t : tmark_) if ((int64_t)t.size() < nt) t.resize(nt, 0);
+    const int64_t pstamp = (F_.panels + 1) << 32;   // a task's mark: unique over the panels and the panel's tasks
+    std::atomic<int> bad{0};
+    const double tlp0 = now_s();
     {
-        const int64_t nt = (int64_t)tl.size();
-        thpo_.assign(nt + 1, 0);
-        for (int64_t i = 0; i < nt; i++) thpo_[i + 1] = thpo_[i] + (F_.blocks[tl[i].t.block].nrows - tl[i].t.k0);
-        thp_.resize((size_t)thpo_[nt]);
-        const int64_t TPI = 32;
-        std::atomic<int> bad{0};
-        pool_.run((nt + TPI - 1) / TPI, [&](int64_t c, int) {
+        const int64_t TPI = 8;
+        pool_.run((nt + TPI - 1) / TPI, [&](int64_t c, int tid) {
+            std::vector<int64_t>& mark = tmark_[tid];
             for (int64_t i = c * TPI; i < std::min(nt, (c + 1) * TPI); i++) {
                 const int32_t* br = F_.block_rows[tl[i].t.block].data();
-                const int64_t r0 = tl[i].t.k0, r1 = F_.blocks[tl[i].t.block].nrows;
-                int32_t* out = thp_.data() + thpo_[i];
-                for (int64_t r = r0; r < r1; r++) {
+                const BlockDesc& B = F_.blocks[tl[i].t.block];
+                const int64_t k0 = tl[i].t.k0, w = B.w, nr = B.nrows, stamp = pstamp | (i + 1);
+                const int32_t sp = tl[i].t.spos0;
+                std::vector<int32_t>& su = succ_[i];
+                su.clear();
+                for (int64_t r = k0; r < w; r++)
+                    if (hrowpos_[br[r]] != sp + (int32_t)(r - k0)) bad.store(1, std::memory_order_relaxed);
+                for (int64_t r = w; r < nr; r++) {
                     const int32_t hp = hrowpos_[br[r]];
-                    if (hp < 0) bad.store(1, std::memory_order_relaxed);
-                    out[r - r0] = hp;
+                    if (hp < 0) { bad.store(2, std::memory_order_relaxed); continue; }
+                    if (hp >= Ru) continue;   // a row not pivotal: no task reads it
+                    const int32_t t = reader_[hp];
+                    if (t < 0 || mark[t] == stamp) continue;
+                    mark[t] = stamp;
+                    if (t <= i) bad.store(3, std::memory_order_relaxed);
+                    su.push_back(t);
                 }
             }
         });
-        if (bad.load()) { errf(err_, errlen_, "internal: update row outside the panel rows"); return FPLUS_ERR_INTERNAL; }
     }
-    std::vector<int32_t> wl(Ru + Rl, 0);
+    if (bad.load()) {
+        const int b = bad.load();
+        errf(err_, errlen_, b == 1 ? "internal: a segment's rows are not consecutive in the workspace"
+                            : b == 2 ? "internal: update row outside the panel rows"
+                                     : "internal: an update task writes into an earlier task's segment");
+        return FPLUS_ERR_INTERNAL;
+    }
     int32_t maxlevel = 0;
-    const double tlp0 = now_s();
-    for (size_t i = 0; i < tl.size(); i++) {
-        const int32_t* hp = thp_.data() + thpo_[i];
-        const int64_t m = F_.blocks[tl[i].t.block].w - tl[i].t.k0, len = thpo_[i + 1] - thpo_[i];
-        int32_t lev = 0;
-        for (int64_t r = 0; r < m; r++) lev = std::max(lev, wl[hp[r]]);
-        lev += 1;
-        for (int64_t r = m; r < len; r++) wl[hp[r]] = std::max(wl[hp[r]], lev);
-        tl[i].level = lev;
-        // the segment's rows are consecutive in W (a union supernode's U rows are one run there)
-        for (int64_t r = 1; r < m; r++)
-            if (hp[r] != hp[0] + r) { errf(err_, errlen_, "internal: a segment's rows are not consecutive in the workspace"); return FPLUS_ERR_INTERNAL; }
-        tl[i].t.spos0 = hp[0];
-        maxlevel = std::max(maxlevel, lev);
+    {
+        std::vector<int32_t> lin(nt, 0);
+        for (int64_t i = 0; i < nt; i++) {
+            const int32_t lev = lin[i] + 1;
+            tl[i].level = lev;
+            maxlevel = std::max(maxlevel, lev);
+            for (const int32_t t : succ_[i]) lin[t] = std::max(lin[t], lev);
+        }
     }
     F_.t_lev += now_s() - th_ws;
     F_.t_lev_pass += now_s() - tlp0;
@@ -825,6 +849,7 @@ int Driver::run()
     col_sn_.assign(nn, -1); pinv_.assign(nn, -1); p_.assign(nn, -1); bpos_.assign(nn, -1);
     ws0_.init(nn);
     wsT_.resize(pool_.size());
+    tmark_.assign(pool_.size(), {});
     for (SWork& w : wsT_) w.init(nn);
     miss_.assign(nn, 0); ustamp_.assign(nn, 0); uidx_.assign(nn, 0);
     hrowpos_.assign(nn, -1); lposmap_.assign(nn, -1);
