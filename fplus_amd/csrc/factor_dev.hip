@@ -437,7 +437,7 @@ __global__ __launch_bounds__(LU_TB) void k_getf2(double* __restrict__ Wd, int64_
     }
 }
 
-// The same sub-panel LU on many workgroups at once (a cooperative launch: every workgroup is resident).  Workgroup g
+// The same sub-panel LU on many workgroups at once (a grid the device holds at once: every workgroup is resident).  Workgroup g
 // holds the sub-panel's nb columns of the positions a + 256 g + t in registers, one row per thread; per column, every
 // workgroup posts its best candidate (modulus, row, the row's values) and the owner of position j posts row j, then
 // all meet at a barrier (a counter that only grows: this launch's arrivals start at `base`), pick the same pivot (largest modulus, lowest row on ties) and update
@@ -796,7 +796,17 @@ void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int
     ProfScope ps("getf2", (vs == 1 ? 2.0 : 8.0) * (double)nr * nb * nb / 2, s);
     const int32_t G = (int32_t)((nr - a + TB - 1) / TB);
     if (P > TB) throw std::invalid_argument("panel LU: the panel is wider than a workgroup");
-    if (G >= 1 && G <= getf2_coop_max_wg() && nb <= PNB) {
+    // the workgroups the device holds at once (occupancy times compute units), per scalar type
+    static int resident[2] = {-1, -1};
+    int& cap = resident[vs == 1 ? 0 : 1];
+    if (cap < 0) {
+        int per = 0;
+        hip_check(vs == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_getf2_coop<double>, TB, 0)
+                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_getf2_coop<Zd>, TB, 0),
+                  "occupancy of the panel LU");
+        cap = per * devctx().ncu;
+    }
+    if (G >= 1 && G <= std::min(cap, getf2_coop_max_wg()) && nb <= PNB) {
         // the barrier counter keeps growing across launches (nb G arrivals each); it is cleared before it could overflow
         if (*counted + (int64_t)nb * G > ((int64_t)1 << 30)) {
             k::fill_i32(counter, 1, 0, s);
@@ -809,9 +819,13 @@ void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int
         char* base = static_cast<char*>(coop);
         void* rowj = base + (size_t)2 * G * (vs == 1 ? sizeof(Cand<double>) : sizeof(Cand<Zd>));
         args[11] = &rowj;
-        hipError_t e = vs == 1 ? hipLaunchCooperativeKernel((const void*)k_getf2_coop<double>, dim3(G), dim3(TB), args, 0, s)
-                               : hipLaunchCooperativeKernel((const void*)k_getf2_coop<Zd>, dim3(G), dim3(TB), args, 0, s);
-        hip_check(e, "cooperative panel LU launch");
+        // an ordinary launch: a grid the device holds at once, after the stream's previous kernel has ended, has all its
+        // workgroups resident together (the cooperative launch API guarantees the same, but brackets every launch with
+        // a queue switch that costs about 20 us); should a workgroup not be scheduled, the bounded wait at the barrier
+        // ends the kernel with an error status instead of waiting forever
+        hipError_t e = vs == 1 ? hipLaunchKernel((const void*)k_getf2_coop<double>, dim3(G), dim3(TB), args, 0, s)
+                               : hipLaunchKernel((const void*)k_getf2_coop<Zd>, dim3(G), dim3(TB), args, 0, s);
+        hip_check(e, "panel LU launch");
         return;
     }
     if (vs == 1) hipLaunchKernelGGL(k_getf2<double>, dim3(1), dim3(LU_TB), 0, s, Wl, ld, nr, P, a, nb, colmax, pivot_rel, piv, st);
