@@ -17,11 +17,13 @@ inline int col_tile(int vs) { return vs == 1 ? 64 : 32; }   // panel columns per
 // end of that panel.  Values are the L factor below the unit diagonal (the entries above it are U values, unused by
 // the updates).  rows hold row indices of the permuted matrix.  ivals, same shape, is the inverse form the updates use:
 // [L11^-1 ; L21 L11^-1] with L11 the unit lower triangle of the pivot rows and L21 the rows below (needed while
-// factoring only); linv keeps L11^-1 alone (w x w, ld = w) for the solves.
+// factoring only); linv keeps L11^-1 alone (w x w, ld = w) for the solves, and uinv U11^-1 (w x w, ld = w), U11 the
+// upper triangle of the pivot rows with the diagonal.
 struct BlockDesc {
     double* vals;      // nrows * w scalars (a scalar is 1 or 2 doubles)
     double* ivals;     // nrows * w scalars
     double* linv;      // w * w scalars
+    double* uinv;      // w * w scalars
     int32_t* rows;     // nrows
     int32_t nrows, w;
     int32_t c0, sn;

@@ -128,6 +128,63 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     }
 }
 
+// column c of U11^-1 (one workgroup per (block, column); thread t owns row t): the upper solve of e_c, rows from c
+// down, in chunks of 16 rows -- a chunk's triangle inside its wavefront (each row divided by its diagonal, then
+// broadcast by a lane shuffle), one barrier, then the rows above subtract the chunk's 16 columns; rows below c stay
+// zero.  The next chunk's factor values are read during the current one.
+template <class T>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_blk_uinv(const int2* __restrict__ jobs, const BlockDesc* __restrict__ blocks)
+{
+    constexpr int CH = 16;
+    __shared__ T xs[TB];
+    const int2 jb = jobs[blockIdx.x];
+    const BlockDesc B = blocks[jb.x];
+    const int32_t w = B.w, c = jb.y, t = threadIdx.x;
+    const T* U = reinterpret_cast<const T*>(B.vals);
+    const bool own = t <= c;
+    T x = t == c ? Sc<T>::one() : Sc<T>::zero();
+    const T d = own ? U[(int64_t)t * B.nrows + t] : Sc<T>::one();
+    // U(t, k) for the chunk's columns k, t < k <= c
+    auto fetch = [&](int ch, T* lv) {
+        const int r0 = ch * CH;
+#pragma unroll
+        for (int u = 0; u < CH; u++) {
+            const int k = r0 + u;
+            lv[u] = (own && k > t && k <= c) ? U[(int64_t)k * B.nrows + t] : Sc<T>::zero();
+        }
+    };
+    T lcur[CH], lnxt[CH];
+    const int ch0 = c / CH;
+    fetch(ch0, lcur);
+    for (int ch = ch0; ch >= 0; ch--) {
+        const int r0 = ch * CH, r1 = min(c + 1, r0 + CH);
+        if (ch > 0) fetch(ch - 1, lnxt);
+        if (t >= r0 && t < r1) {
+#pragma unroll
+            for (int u = CH - 1; u >= 0; u--) {
+                const int k = r0 + u;
+                if (k >= r1) continue;
+                if (t == k) x = Sc<T>::div(x, d);
+                const T xk = shfl_val<T>(x, k & 63);
+                if (t < k) x = Sc<T>::fms(x, lcur[u], xk);
+            }
+            xs[t] = x;
+        }
+        __syncthreads();
+        if (t < r0) {
+#pragma unroll
+            for (int u = 0; u < CH; u++) {
+                const int k = r0 + u;
+                if (k >= r1) break;
+                x = Sc<T>::fms(x, lcur[u], xs[k]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CH; u++) lcur[u] = lnxt[u];
+    }
+    if (t < w) reinterpret_cast<T*>(B.uinv)[t + (int64_t)c * w] = x;
+}
+
 // FP64 matrix cores: D(16x16) += A(16x4) B(4x16) per wavefront; lane l supplies A[l % 16][l / 16] and
 // B[l / 16][l % 16] and holds D[l / 16 + 4 v][l % 16], v < 4 (checked on gfx950)
 typedef double dx4 __attribute__((ext_vector_type(4)));
@@ -766,6 +823,8 @@ void blk_inverse(int vs, int32_t njobs, const int2* jobs, int32_t ntile, const G
     ProfScope ps("blk_inverse", flops, s);
     if (vs == 1) hipLaunchKernelGGL(k_blk_linv<double>, dim3(njobs), dim3(TB), 0, s, jobs, blocks);
     else hipLaunchKernelGGL(k_blk_linv<Zd>, dim3(njobs), dim3(TB), 0, s, jobs, blocks);
+    if (vs == 1) hipLaunchKernelGGL(k_blk_uinv<double>, dim3(njobs), dim3(TB), 0, s, jobs, blocks);
+    else hipLaunchKernelGGL(k_blk_uinv<Zd>, dim3(njobs), dim3(TB), 0, s, jobs, blocks);
     if (ntile <= 0) return;
     if (vs == 1) hipLaunchKernelGGL(k_blk_ivgemm<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks);
     else hipLaunchKernelGGL(k_blk_ivgemm<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks);

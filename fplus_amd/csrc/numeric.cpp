@@ -5,6 +5,7 @@
 // orders and scalings, the same error convention.  Every arithmetic step runs on the device; a missing device is
 // FPLUS_ERR_GPU.
 #include "host_common.h"
+#include "host_pool.h"
 #include "snfactor.h"
 #include "solve_dev.h"
 #include "symbolic.h"
@@ -206,21 +207,24 @@ struct fplus_numeric {
     std::vector<int32_t> lev_ubr;
     DevBuf<int4> d_ubr;        // and the partial sums' reduction tiles: block, first of 16 rows, first slot, slots
     int64_t ub_slots_max = 0;  // the most partial slots of one level
-    std::vector<double> bytes_ub_tri;
+    std::vector<double> bytes_ub_prod;
+    std::vector<int32_t> lev_lpt, lev_upt;   // level offsets into the triangle-product tiles (block, first of 64 rows)
+    DevBuf<int2> d_lpt, d_upt;
     std::vector<double> bytes_lf_pull, bytes_lf_push, bytes_ub, bytes_utf, bytes_ltb;
     // U x = y beyond the blocks: a block's pieces are the runs of later columns over its rows, each covering the block's
-    // rows a .. w-1 (a = pa[p]), ordered by a; piece p holds U(c0 + t, pj[p]) at Ux[pu[p] + t]
+    // rows a .. w-1 (a = pa[p]), in column order; piece p holds U(c0 + t, pj[p]) at Ux[pu[p] + t]
     DevBuf<int64_t> d_pu;
     DevBuf<int32_t> d_pj, d_pa;
     // solve workspace, reused across calls (solves on one numeric object are serialized)
     mutable std::mutex solve_mu;
-    mutable DevBuf<double> d_w, d_contrib, d_b, d_ub_part;
+    mutable DevBuf<double> d_w, d_y, d_contrib, d_b, d_ub_part;
     // the CSC export, built on first access
     mutable std::mutex mu;
     mutable bool exported = false;
     mutable fplus_factors f{};
     // timings of the last factorization
     double t_ruiz = 0, t_factor = 0, t_total = 0, t_prepare = 0;
+    double t_prep_part[6] = {0, 0, 0, 0, 0, 0};   // the preparation by step: L levels, U pieces, U levels, slots, tiles, uploads
 };
 
 namespace {
@@ -231,6 +235,8 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
     DevFactor& F = num->F;
     const int64_t n = F.n;
     const int32_t nb = (int32_t)F.blocks.size();
+    double tp = now_s();
+    auto lap = [&](int i) { const double t = now_s(); num->t_prep_part[i] = t - tp; tp = t; };
     std::vector<int32_t> colblk(n > 0 ? n : 1);
     for (int32_t b = 0; b < nb; b++)
         for (int32_t c = 0; c < F.blocks[b].w; c++) colblk[F.blocks[b].c0 + c] = b;
@@ -266,14 +272,17 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
     }
     // lvl is rewritten in place: blocks after b were assigned before b reads them
     order_levels(lvl, num->blk_ltb, num->lev_ltb);
-    // U x = y: the pieces of every block (counted, then placed in column order, then ordered by first row), and the
+    lap(0);
+    // U x = y: the pieces of every block (counted, then placed in column order), and the
     // levels: a block waits for the blocks of its pieces' columns
     std::vector<int64_t> pb(nb + 1, 0);
-    std::vector<int32_t> pj, pa;
-    std::vector<int64_t> pu;
+    // (filled in full: no zero-initialisation)
+    std::unique_ptr<int32_t[]> pj, pa;
+    std::unique_ptr<int64_t[]> pu;
     {
-        auto for_pieces = [&](auto&& fn) {
-            for (int64_t j = 0; j < n; j++) {
+        // column j's runs over the rows of blocks other than its own, in column order, over the columns [j0, j1)
+        auto for_pieces = [&](int64_t j0, int64_t j1, auto&& fn) {
+            for (int64_t j = j0; j < j1; j++) {
                 const int32_t bj = colblk[j];
                 int64_t uo = F.Up[j];
                 for (int64_t r = F.Urp[j]; r < F.Urp[j + 1]; r++) {
@@ -292,36 +301,42 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
                 }
             }
         };
-        for_pieces([&](int32_t b, int64_t, int64_t, int64_t) { pb[b + 1]++; });
-        for (int32_t b = 0; b < nb; b++) pb[b + 1] += pb[b];
-        const int64_t np = pb[nb];
-        std::vector<int32_t> tj(np > 0 ? np : 1), ta(np > 0 ? np : 1);
-        std::vector<int64_t> tu(np > 0 ? np : 1), fill(pb.begin(), pb.end() - 1);
-        for_pieces([&](int32_t b, int64_t j, int64_t a, int64_t u) {
-            const int64_t o = fill[b]++;
-            tj[o] = (int32_t)j; ta[o] = (int32_t)a; tu[o] = u - a;
+        // in parallel: chunks of columns count their pieces per block, the chunks' places within each block follow in
+        // chunk order (so every block's pieces are in column order: a fixed summation order), the chunks fill them
+        HostPool pool;
+        const int64_t nch = std::min<int64_t>(std::max<int64_t>(n, 1), 4 * (int64_t)pool.size()), per = (n + nch - 1) / nch;
+        std::vector<std::vector<int64_t>> cnt(nch, std::vector<int64_t>(nb + 1, 0));
+        pool.run(nch, [&](int64_t c, int) {
+            std::vector<int64_t>& h = cnt[c];
+            for_pieces(c * per, std::min(n, (c + 1) * per), [&](int32_t b, int64_t, int64_t, int64_t) { h[b]++; });
         });
-        // per block: a counting sort by first row (stable: columns ascending within one first row), the row counts
-        pj.resize(np > 0 ? np : 1); pa.resize(np > 0 ? np : 1); pu.resize(np > 0 ? np : 1);
-        std::vector<int64_t> h;
         for (int32_t b = 0; b < nb; b++) {
-            const BlockDesc& B = F.blocks[b];
-            h.assign(B.w + 1, 0);
-            for (int64_t o = pb[b]; o < pb[b + 1]; o++) h[ta[o] + 1]++;
-            for (int32_t t = 0; t < B.w; t++) h[t + 1] += h[t];
-            for (int64_t o = pb[b]; o < pb[b + 1]; o++) {
-                const int64_t d = pb[b] + h[ta[o]]++;
-                pj[d] = tj[o]; pa[d] = ta[o]; pu[d] = tu[o];
-            }
+            int64_t o = pb[b];
+            for (int64_t c = 0; c < nch; c++) { const int64_t k = cnt[c][b]; cnt[c][b] = o; o += k; }
+            pb[b + 1] = o;
         }
-        auto up32v = [s](DevBuf<int32_t>& d, const std::vector<int32_t>& hh) { d.alloc(hh.size()); d.upload(hh.data(), hh.size(), s); };
-        auto up64v = [s](DevBuf<int64_t>& d, const std::vector<int64_t>& hh) { d.alloc(hh.size()); d.upload(hh.data(), hh.size(), s); };
+        const int64_t np = pb[nb];
         if (np >= ((int64_t)1 << 31)) throw std::length_error("too many U pieces for the solve's 32-bit indices");
-        up64v(num->d_pu, pu);
-        up32v(num->d_pj, pj);
-        up32v(num->d_pa, pa);
+        const size_t npa = (size_t)(np > 0 ? np : 1);
+        pj.reset(new int32_t[npa]);
+        pa.reset(new int32_t[npa]);
+        pu.reset(new int64_t[npa]);
+        pool.run(nch, [&](int64_t c, int) {
+            std::vector<int64_t>& fill = cnt[c];
+            for_pieces(c * per, std::min(n, (c + 1) * per), [&](int32_t b, int64_t j, int64_t a, int64_t u) {
+                const int64_t o = fill[b]++;
+                pj[o] = (int32_t)j; pa[o] = (int32_t)a; pu[o] = u - a;
+            });
+        });
+        num->d_pu.alloc(npa);
+        num->d_pu.upload(pu.get(), (size_t)np, s);
+        num->d_pj.alloc(npa);
+        num->d_pj.upload(pj.get(), (size_t)np, s);
+        num->d_pa.alloc(npa);
+        num->d_pa.upload(pa.get(), (size_t)np, s);
         dev_sync(s);
     }
+    lap(1);
     for (int32_t b = nb - 1; b >= 0; b--) {
         int32_t L = 0;
         for (int64_t o = pb[b]; o < pb[b + 1]; o++) L = std::max(L, lvl[colblk[pj[o]]] + 1);
@@ -344,6 +359,7 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         lvl[b] = L;
     }
     order_levels(lvl, num->blk_utf, num->lev_utf);
+    lap(2);
     const double vb = 8.0 * F.vs;   // bytes per scalar
     const int R0 = k::solve_rhs_per_pass(F.vs);
     // contribution slots and the rows' lists of them (blocks in column order: a deterministic summation order)
@@ -368,6 +384,7 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         up64(num->d_slot_off, slot_off);
         up64(num->d_incp, incp);
         up64(num->d_incs, incs);
+        lap(3);
         {   // the rows each L level gathers: the pivot rows of its blocks
             std::vector<int32_t> grows;
             const size_t nlev0 = num->lev_lf.size() - 1;
@@ -387,33 +404,40 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
             num->d_grows.upload(grows.data(), grows.size(), s);
             dev_sync(s);
         }
-        // push tiles per level (256 rows each), and the bytes of every level of the four sweeps
-        std::vector<int2> tiles;
+        // push tiles (64 rows below a block each) and triangle tiles (64 of its rows) per level, and the bytes of every
+        // level of the four sweeps
+        std::vector<int2> tiles, ptiles;
         const size_t nlev = num->lev_lf.size() - 1;
         num->lev_lft.assign(nlev + 1, 0);
+        num->lev_lpt.assign(nlev + 1, 0);
         num->bytes_lf_pull.assign(nlev, 0.0);
         num->bytes_lf_push.assign(nlev, 0.0);
         for (size_t L = 0; L < nlev; L++) {
             for (int32_t k2 = num->lev_lf[L]; k2 < num->lev_lf[L + 1]; k2++) {
                 const int32_t b = num->blk_lf[k2];
                 const BlockDesc& B = F.blocks[b];
-                for (int32_t r0 = 0; r0 < B.nrows - B.w; r0 += 256) tiles.push_back(make_int2(b, r0));
-                double inc = 0;
-                for (int32_t c = 0; c < B.w; c++) inc += (double)(incp[B.c0 + c + 1] - incp[B.c0 + c]);
-                num->bytes_lf_pull[L] += (double)B.w * B.w * vb + inc * (8.0 + R0 * vb) + 2.0 * B.w * R0 * vb;
+                for (int32_t r0 = 0; r0 < B.nrows - B.w; r0 += 64) tiles.push_back(make_int2(b, r0));
+                for (int32_t r0 = 0; r0 < B.w; r0 += 64) ptiles.push_back(make_int2(b, r0));
+                num->bytes_lf_pull[L] += (double)B.w * (B.w + 1) / 2 * vb + 2.0 * B.w * R0 * vb;
                 num->bytes_lf_push[L] += (double)(B.nrows - B.w) * B.w * vb + (double)(B.nrows - B.w) * R0 * vb;
             }
             num->lev_lft[L + 1] = (int32_t)tiles.size();
+            num->lev_lpt[L + 1] = (int32_t)ptiles.size();
         }
         num->d_lft.alloc(tiles.size() > 0 ? tiles.size() : 1);
         num->d_lft.upload(tiles.data(), tiles.size(), s);
+        num->d_lpt.alloc(ptiles.size() > 0 ? ptiles.size() : 1);
+        num->d_lpt.upload(ptiles.data(), ptiles.size(), s);
+        dev_sync(s);
     }
     {
         const size_t nub = num->lev_ub.size() - 1;
         num->bytes_ub.assign(nub, 0.0);
-        num->bytes_ub_tri.assign(nub, 0.0);
+        num->bytes_ub_prod.assign(nub, 0.0);
         num->lev_ubt.assign(nub + 1, 0);
+        num->lev_upt.assign(nub + 1, 0);
         std::vector<int4> ubt, ubr;
+        std::vector<int2> upt;
         num->lev_ubr.assign(nub + 1, 0);
         num->ub_slots_max = 0;
         for (size_t L = 0; L < nub; L++) {
@@ -424,7 +448,8 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
                 double ent = 0;
                 for (int64_t o = pb[b]; o < pb[b + 1]; o++) ent += (double)(B.w - pa[o]);
                 num->bytes_ub[L] += ent * vb + (double)(pb[b + 1] - pb[b]) * (12.0 + R0 * vb) + 2.0 * B.w * R0 * vb;
-                num->bytes_ub_tri[L] += (double)B.w * (B.w + 1) / 2 * vb + 2.0 * B.w * R0 * vb;
+                num->bytes_ub_prod[L] += (double)B.w * (B.w + 1) / 2 * vb + 2.0 * B.w * R0 * vb;
+                for (int32_t r0 = 0; r0 < B.w; r0 += 64) upt.push_back(make_int2(b, r0));
                 const int32_t first = slot;
                 for (int64_t p = pb[b]; p < pb[b + 1]; p += k::UB_PC)
                     ubt.push_back(make_int4(b, (int32_t)p, (int32_t)std::min<int64_t>(k::UB_PC, pb[b + 1] - p), slot++));
@@ -433,12 +458,16 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
             }
             num->lev_ubt[L + 1] = (int32_t)ubt.size();
             num->lev_ubr[L + 1] = (int32_t)ubr.size();
+            num->lev_upt[L + 1] = (int32_t)upt.size();
             num->ub_slots_max = std::max<int64_t>(num->ub_slots_max, slot);
         }
         num->d_ubt.alloc(ubt.size() > 0 ? ubt.size() : 1);
         num->d_ubt.upload(ubt.data(), ubt.size(), s);
         num->d_ubr.alloc(ubr.size() > 0 ? ubr.size() : 1);
         num->d_ubr.upload(ubr.data(), ubr.size(), s);
+        num->d_upt.alloc(upt.size() > 0 ? upt.size() : 1);
+        num->d_upt.upload(upt.data(), upt.size(), s);
+        dev_sync(s);
         const size_t nut = num->lev_utf.size() - 1, nlt = num->lev_ltb.size() - 1;
         {   // the transposed sweeps' dot-product tiles (4 columns each) per level
             auto tiles_of = [&](const std::vector<int32_t>& lev, const std::vector<int32_t>& blk, std::vector<int32_t>& lt, DevBuf<int2>& d) {
@@ -469,6 +498,7 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
                 num->bytes_ltb[L] += (double)B.nrows * B.w * vb + (double)B.nrows * (4.0 + R0 * vb);
             }
     }
+    lap(4);
     auto up32 = [s](DevBuf<int32_t>& d, const std::vector<int32_t>& h) { d.alloc(h.size()); d.upload(h.data(), h.size(), s); };
     up32(num->d_blk_lf, num->blk_lf);
     up32(num->d_blk_ub, num->blk_ub);
@@ -488,6 +518,7 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
     num->d_Dc.alloc(n > 0 ? n : 1);
     num->d_Dc.upload(num->Dc.data(), n, s);
     dev_sync(s);
+    lap(5);
 }
 
 // the sweeps on w (n x R row-major, pivot order, R <= solve_rhs_per_pass) in place
@@ -497,13 +528,16 @@ void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, hi
     const int vs = F.vs;
     const BlockDesc* blocks = F.d_blocks.p;
     if (trans == 0) {
+        // L y = b from w into y, then U x = y from y into w
         num->d_contrib.reserve((size_t)(num->nslots > 0 ? num->nslots : 1) * R * vs);
+        num->d_y.reserve((size_t)std::max<int64_t>(F.n, 1) * R * vs);
+        double* y = num->d_y.p;
         for (size_t L = 0; L + 1 < num->lev_lf.size(); L++) {
             k::lfwd_gather(vs, num->lev_gr[L + 1] - num->lev_gr[L], num->d_grows.p + num->lev_gr[L], num->d_incp.p, num->d_incs.p,
                            num->d_contrib.p, w, R, num->bytes_lf_gather[L], s);
-            k::lfwd_pull(vs, num->lev_lf[L + 1] - num->lev_lf[L], num->d_blk_lf.p + num->lev_lf[L], blocks, w, R,
+            k::lfwd_pull(vs, num->lev_lpt[L + 1] - num->lev_lpt[L], num->d_lpt.p + num->lev_lpt[L], blocks, w, y, R,
                          num->bytes_lf_pull[L], s);
-            k::lfwd_push(vs, num->lev_lft[L + 1] - num->lev_lft[L], num->d_lft.p + num->lev_lft[L], blocks, num->d_slot_off.p, w,
+            k::lfwd_push(vs, num->lev_lft[L + 1] - num->lev_lft[L], num->d_lft.p + num->lev_lft[L], blocks, num->d_slot_off.p, y,
                          num->d_contrib.p, R, num->bytes_lf_push[L], s);
         }
         num->d_ub_part.reserve((size_t)std::max<int64_t>(num->ub_slots_max, 1) * 256 * 16 * vs);   // a slot: 256 rows x 16
@@ -511,8 +545,9 @@ void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, hi
         for (size_t L = 0; L + 1 < num->lev_ub.size(); L++) {
             k::ub_dots(vs, num->lev_ubt[L + 1] - num->lev_ubt[L], num->d_ubt.p + num->lev_ubt[L], blocks, num->d_pa.p, num->d_pj.p,
                        num->d_pu.p, F.d_Ux.p, w, part, R, num->bytes_ub[L], s);
-            k::ub_red(vs, num->lev_ubr[L + 1] - num->lev_ubr[L], num->d_ubr.p + num->lev_ubr[L], blocks, part, w, R, s);
-            k::ub_tri(vs, num->lev_ub[L + 1] - num->lev_ub[L], num->d_blk_ub.p + num->lev_ub[L], blocks, w, R, num->bytes_ub_tri[L], s);
+            k::ub_red(vs, num->lev_ubr[L + 1] - num->lev_ubr[L], num->d_ubr.p + num->lev_ubr[L], blocks, part, y, R, s);
+            k::ub_prod(vs, num->lev_upt[L + 1] - num->lev_upt[L], num->d_upt.p + num->lev_upt[L], blocks, y, w, R,
+                       num->bytes_ub_prod[L], s);
         }
     } else {
         const int cj = trans == 2 && vs == 2;
@@ -804,7 +839,10 @@ static int factor_entry(const fplus_symbolic* sym, const int64_t* Ap, const int6
         const double tps = now_s();
         prepare_solves(num.get(), s);
         num->t_prepare = now_s() - tps;
-        if (opt->verbose >= 1) printf("[FPLUS] solve preparation %.3fs\n", num->t_prepare);
+        if (opt->verbose >= 1)
+            printf("[FPLUS] solve preparation %.3fs (L levels %.3f, U pieces %.3f, U levels %.3f, slots %.3f, tiles %.3f, "
+                   "uploads %.3f)\n", num->t_prepare, num->t_prep_part[0], num->t_prep_part[1], num->t_prep_part[2],
+                   num->t_prep_part[3], num->t_prep_part[4], num->t_prep_part[5]);
         num->t_total = now_s() - t0;
     } catch (const GpuError& e) {
         return gpu_error(e, err, errlen);

@@ -217,6 +217,7 @@ private:
     std::vector<DevBuf<double>> iv_chunks_;   // the blocks' inverse forms: needed only while factoring, freed with the driver
     Arena<double> ivals_{nullptr, 0, 0, 0};
     Arena<double> linv_{nullptr, 0, 0, 0};
+    Arena<double> uinv_{nullptr, 0, 0, 0};
     Arena<int32_t> rows_{nullptr, 0, 0, 0};
     hipStream_t s_ = nullptr;
 };
@@ -230,6 +231,7 @@ void Driver::add_block(int64_t s, int64_t b0, int64_t b1, int64_t j0, const std:
     B.vals = vals_.take((size_t)(nrows * w * vs_));
     B.ivals = ivals_.take((size_t)(nrows * w * vs_));
     B.linv = linv_.take((size_t)(w * w * vs_));
+    B.uinv = uinv_.take((size_t)(w * w * vs_));
     B.rows = rows_.take((size_t)nrows);
     B.nrows = (int32_t)nrows;
     B.w = (int32_t)w;
@@ -875,6 +877,9 @@ int Driver::run()
     F_.linv_chunks.clear();
     linv_.chunks = &F_.linv_chunks;
     linv_.min_chunk = (size_t)1 << 24;
+    F_.uinv_chunks.clear();
+    uinv_.chunks = &F_.uinv_chunks;
+    uinv_.min_chunk = (size_t)1 << 24;
     rows_.chunks = &F_.row_chunks;
     rows_.min_chunk = (size_t)1 << 24;
     const int64_t P = std::max<int64_t>(1, opt_.panel);

@@ -31,6 +31,7 @@ struct DevFactor {
     std::vector<DevBuf<double>> val_chunks;
     std::vector<DevBuf<int32_t>> row_chunks;
     std::vector<DevBuf<double>> linv_chunks;   // the blocks' L11^-1, for the solves
+    std::vector<DevBuf<double>> uinv_chunks;   // and their U11^-1
     DevBuf<double> d_Ux;                 // U values, in Ui order
     DevBuf<int32_t> d_Ui;                // U row indices on the device, expanded from the runs panel by panel
     int64_t nnzU = 0;

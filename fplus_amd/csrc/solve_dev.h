@@ -14,20 +14,24 @@ int solve_rhs_per_pass(int vs);
 // rows[k] (the pivot rows of a level's blocks) subtract the contributions in their slots: x_i -= sum over incs[incp[i] ..]
 void lfwd_gather(int vs, int32_t nrow, const int32_t* rows, const int64_t* incp, const int64_t* incs, const double* contrib,
                  double* x, int32_t R, double bytes, hipStream_t s);
-// the unit-lower triangles of a level's blocks
-void lfwd_pull(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, double* x, int32_t R, double bytes, hipStream_t s);
-void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* slot_off, const double* x,
+// the unit-lower triangles of a level's blocks as products with L11^-1 (tiles: block, first of 64 rows): y = L11^-1 x on
+// the blocks' rows; then the rows below (tiles: block, first of 64 rows below) into the contribution slots from y
+void lfwd_pull(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const double* x, double* y, int32_t R,
+               double bytes, hipStream_t s);
+void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* slot_off, const double* y,
                double* contrib, int32_t R, double bytes, hipStream_t s);
 // U x = y per level.  ub_dots tiles (block, first piece, pieces, slot): block row t gets, in slot `slot` of part
 // (part[(slot * 256 + t) * 16 + r] scalars), the sum over the tile's pieces p with pa[p] <= t of U(t, pj[p]) =
-// Ux[pu[p] + t] times x[pj[p]].  ub_red tiles (block, first of 16 rows, first slot, slots): x minus the block's slots.
-// ub_tri: the blocks' upper triangles.
+// Ux[pu[p] + t] times x[pj[p]] (x: the solved rows of later blocks).  ub_red tiles (block, first of 16 rows, first slot,
+// slots): y (the lower sweep's result) minus the block's slots.  ub_prod (tiles: block, first of 64 rows): x = U11^-1 y
+// on the blocks' rows.
 constexpr int UB_PC = 32;   // pieces per ub_dots tile, staged at once
 void ub_dots(int vs, int32_t ntile, const int4* tiles, const BlockDesc* blocks, const int32_t* pa, const int32_t* pj,
              const int64_t* pu, const double* Ux, const double* x, double* part, int32_t R, double bytes, hipStream_t s);
 void ub_red(int vs, int32_t ntile, const int4* tiles, const BlockDesc* blocks, const double* part, double* x, int32_t R,
             hipStream_t s);
-void ub_tri(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, double* x, int32_t R, double bytes, hipStream_t s);
+void ub_prod(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const double* y, double* x, int32_t R,
+             double bytes, hipStream_t s);
 // the transposed sweeps' dot products (tiles: block, first of 4 columns), before the blocks' triangles
 void utdots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
             const double* Ux, double* x, int32_t R, int conj, double bytes, hipStream_t s);

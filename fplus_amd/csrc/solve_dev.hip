@@ -3,15 +3,16 @@
 // L is held as supernode blocks (factor_dev.h), U in CSC (rows in pivot order, the diagonal last in every column); a
 // U column is a list of runs of consecutive rows, which the solves with A read per block.  The right-hand sides live row-major during the sweeps, x[i * R + r] for
 // R right-hand sides, so one entry of a factor meets all R values of a row in one contiguous read.  A sweep walks the
-// blocks level by level (a level's blocks are independent), one launch per level, and every sweep pulls: each value is
-// written by one thread, no atomics, so the results are the same on every run.
+// blocks level by level (a level's blocks are independent), one launch per level and step, and every sweep pulls: each
+// value is written by one thread, no atomics, so the results are the same on every run.  The sweeps with A use a second
+// vector y: L y = b leaves its result in y, U x = y its result in x.
 //   lfwd_gather L y = b   a level's pivot rows gather the contributions the blocks before them left in their slots
-//                         (one wavefront per row);
-//   lfwd_pull   the block's unit-lower triangle;
+//                         (one wavefront per row, in x);
+//   lfwd_pull   y on the block's rows = L11^-1 x (the inverse kept from the factorization);
 //   lfwd_push   the block's rows below: contribution[slot + r] = L(r, block) y(block) (the slot is the row's own);
-//   ubwd        U x = y   a block's pivot rows: dot products along their U rows beyond the block (the runs of later
-//                         U columns over the block's rows, ordered by first row),
-//                         then the block's upper triangle from its columns, last to first;
+//   ub_dots     U x = y   a block's pivot rows: dot products along their U rows beyond the block (the runs of later
+//   ub_red                U columns over the block's rows, in column order) with x, subtracted from y;
+//   ub_prod     x on the block's rows = U11^-1 y;
 //   utfwd       U^T y = b (U^H): dot products down U's columns, the block's own rows in order;
 //   ltbwd       L^T x = y (L^H): dot products down the block's columns, the triangle last to first.
 // Blocks are at most 256 columns wide (the panel width): the triangles keep one row per thread.
@@ -86,85 +87,120 @@ template <class T> __device__ inline T shfl_sc(T v, int src);
 template <> __device__ inline double shfl_sc<double>(double v, int src) { return __shfl(v, src); }
 template <> __device__ inline Zd shfl_sc<Zd>(Zd v, int src) { return zmake(__shfl(v.x, src), __shfl(v.y, src)); }
 
-// L y = b, part 1: the block's unit-lower triangle, as the product with its inverse kept from the factorization (dense,
-// zero above the diagonal, column-major with ld = w): thread t owns row t, the block's x for all right-hand sides in LDS
-template <class T>
-__global__ __launch_bounds__(TB) void k_lfwd_pull(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
-                                                   double* __restrict__ xd, int32_t R)
+// The blocks' diagonal triangles in the sweeps with A, as products with the inverses kept from the factorization:
+// out(block rows) = M in(block rows), M = L11^-1 (lower, unit diagonal) or U11^-1 (upper), w x w column-major (ld = w).
+// One workgroup per 64 rows of a block: lane = row; the four waves take a quarter each of the columns the rows reach
+// (lower: up to the chunk's last row, upper: from its first row), with the block's input for those columns and all
+// right-hand sides in LDS; the quarters' sums are added in wave order -- a fixed order that does not depend on the
+// number of right-hand sides.  The output goes to another vector: the block's input is still read by its other row
+// chunks.
+template <class T, bool UPPER>
+__global__ __launch_bounds__(TB) void k_tri_prod(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
+                                                  const double* __restrict__ ind, double* __restrict__ outd, int32_t R)
 {
     constexpr int RC = RM<T>::v;
     __shared__ T xs[TB * RC];
-    const BlockDesc B = blocks[blk[blockIdx.x]];
-    const T* Li = reinterpret_cast<const T*>(B.linv);
-    T* x = reinterpret_cast<T*>(xd);
-    const int w = B.w, t = threadIdx.x;
-    for (int idx = t; idx < w * R; idx += TB) {
+    const int2 tl = tiles[blockIdx.x];   // block, first row
+    const BlockDesc B = blocks[tl.x];
+    const T* M = reinterpret_cast<const T*>(UPPER ? B.uinv : B.linv);
+    const T* in = reinterpret_cast<const T*>(ind);
+    T* out = reinterpret_cast<T*>(outd);
+    const int w = B.w, r0 = tl.y;
+    const int ca = UPPER ? r0 : 0, cb = UPPER ? w : min(w, r0 + 64), ncol = cb - ca;
+    for (int idx = threadIdx.x; idx < ncol * R; idx += TB) {
         const int c = idx / R, r = idx - c * R;
-        xs[c * RC + r] = x[((int64_t)B.c0 + c) * R + r];
+        xs[c * RC + r] = in[((int64_t)B.c0 + ca + c) * R + r];
     }
     __syncthreads();
-    if (t >= w) return;
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6, row = r0 + lane;
+    const int per = (ncol + 3) / 4, qa = ca + q * per, qb = min(cb, qa + per);
     T acc[RC];
 #pragma unroll
     for (int r = 0; r < RC; r++) acc[r] = Sc<T>::zero();
+    if (row < w) {
+        const int lo = UPPER ? max(qa, row) : qa, hi = UPPER ? qb : min(qb, row + 1);
 #pragma unroll 8
-    for (int c = 0; c < w; c++) {
-        const T v = Li[t + (int64_t)c * w];
+        for (int c = lo; c < hi; c++) {
+            const T v = M[row + (int64_t)c * w];
+#pragma unroll
+            for (int r = 0; r < RC; r++)
+                if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, xs[(c - ca) * RC + r]);
+        }
+    }
+    __syncthreads();   // the LDS now takes the partial sums of waves 1 .. 3
+    if (q > 0) {
+#pragma unroll
+        for (int r = 0; r < RC; r++) xs[((q - 1) * 64 + lane) * RC + r] = acc[r];
+    }
+    __syncthreads();
+    if (q == 0 && row < w) {
+        for (int qq = 0; qq < 3; qq++) {
+#pragma unroll
+            for (int r = 0; r < RC; r++) acc[r] = Sc<T>::add(acc[r], xs[(qq * 64 + lane) * RC + r]);
+        }
 #pragma unroll
         for (int r = 0; r < RC; r++)
-            if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, xs[c * RC + r]);
+            if (r < R) out[((int64_t)B.c0 + row) * R + r] = acc[r];
     }
-#pragma unroll
-    for (int r = 0; r < RC; r++)
-        if (r < R) x[((int64_t)B.c0 + t) * R + r] = acc[r];
 }
 
-// L y = b, part 2: the rows below a block, 256 rows per workgroup; slot_off[b] is the block's first slot
+// L y = b, part 2: the rows below a block, 64 rows per workgroup (lane = row), the four waves a quarter of the block's
+// columns each (the block's y for all columns and right-hand sides in LDS), the quarters' sums added in wave order;
+// slot_off[b] is the block's first slot
 template <class T>
 __global__ __launch_bounds__(TB) void k_lfwd_push(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
-                                                   const int64_t* __restrict__ slot_off, const double* __restrict__ xd,
+                                                   const int64_t* __restrict__ slot_off, const double* __restrict__ yd,
                                                    double* __restrict__ contribd, int32_t R)
 {
-    constexpr int RC = RM<T>::v, CC = 64;   // the block's columns go through LDS CC at a time
-    __shared__ T ys[CC * RC];
-    const int2 tl = tiles[blockIdx.x];
+    constexpr int RC = RM<T>::v;
+    __shared__ T ys[TB * RC];
+    const int2 tl = tiles[blockIdx.x];   // block, first row below the block (0-based)
     const BlockDesc B = blocks[tl.x];
     const T* V = reinterpret_cast<const T*>(B.vals);
-    const T* x = reinterpret_cast<const T*>(xd);
+    const T* y = reinterpret_cast<const T*>(yd);
     T* contrib = reinterpret_cast<T*>(contribd);
     const int w = B.w;
-    const int32_t row = w + tl.y + threadIdx.x;
+    for (int idx = threadIdx.x; idx < w * R; idx += TB) {
+        const int c = idx / R, r = idx - c * R;
+        ys[c * RC + r] = y[((int64_t)B.c0 + c) * R + r];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int32_t row = w + tl.y + lane;
     const bool own = row < B.nrows;
+    const int per = (w + 3) / 4, qa = q * per, qb = min(w, qa + per);
     T acc[RC];
 #pragma unroll
     for (int r = 0; r < RC; r++) acc[r] = Sc<T>::zero();
-    for (int c0 = 0; c0 < w; c0 += CC) {
-        const int cn = min(CC, w - c0);
-        __syncthreads();   // the previous columns are done with the LDS
-        for (int idx = threadIdx.x; idx < cn * R; idx += TB) {
-            const int c = idx / R, r = idx - c * R;
-            ys[c * RC + r] = x[((int64_t)B.c0 + c0 + c) * R + r];
-        }
-        __syncthreads();
-        if (own) {
+    if (own) {
 #pragma unroll 8
-            for (int c = 0; c < cn; c++) {
-                const T v = V[(int64_t)(c0 + c) * B.nrows + row];
+        for (int c = qa; c < qb; c++) {
+            const T v = V[(int64_t)c * B.nrows + row];
 #pragma unroll
-                for (int r = 0; r < RC; r++)
-                    if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, ys[c * RC + r]);
-            }
+            for (int r = 0; r < RC; r++)
+                if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, ys[c * RC + r]);
         }
     }
-    if (!own) return;
-    const int64_t slot = slot_off[tl.x] + row - w;
+    __syncthreads();
+    if (q > 0) {
 #pragma unroll
-    for (int r = 0; r < RM<T>::v; r++)
-        if (r < R) contrib[slot * R + r] = acc[r];
+        for (int r = 0; r < RC; r++) ys[((q - 1) * 64 + lane) * RC + r] = acc[r];
+    }
+    __syncthreads();
+    if (q == 0 && own) {
+        for (int qq = 0; qq < 3; qq++) {
+#pragma unroll
+            for (int r = 0; r < RC; r++) acc[r] = Sc<T>::add(acc[r], ys[(qq * 64 + lane) * RC + r]);
+        }
+        const int64_t slot = slot_off[tl.x] + row - w;
+#pragma unroll
+        for (int r = 0; r < RC; r++)
+            if (r < R) contrib[slot * R + r] = acc[r];
+    }
 }
 
 // U x = y, part 1: the products of a block's pivot rows with their U rows beyond the block, one share of the block's
-// pieces per workgroup.  A piece is a later column j's run over the block's rows [a, w), the pieces sorted by a: thread t
+// pieces per workgroup.  A piece is a later column j's run over the block's rows [a, w), the pieces in column order: thread t
 // owns block row t and adds U(t, j) x_j for every piece with a <= t, so consecutive rows read consecutive U entries; the
 // pieces go in batches of UB_PC, whose x_j for all right-hand sides sit in LDS.  A share's sums go to its own slot of
 // `part`; ub_red subtracts a block's slots from x in slot order (a fixed order, the same for any number of right-hand
@@ -236,81 +272,6 @@ __global__ __launch_bounds__(TB) void k_ub_red(const int4* __restrict__ tiles, c
     if (c < tl.w) s0 = Sc<T>::add(s0, part[(int64_t)c * TB * RC]);
     T* xp = reinterpret_cast<T*>(xd) + ((int64_t)B.c0 + t) * R + r;
     *xp = Sc<T>::sub(*xp, Sc<T>::add(s0, s1));
-}
-
-// U x = y, part 2 (one workgroup per block): the block's upper triangle, dense in the block's pivot rows (the panel
-// LU left U(block, block) above the block's diagonal), chunks last to first; within a chunk the last row first, each
-// scaled by the reciprocal of its diagonal before it is broadcast
-template <class T>
-__global__ __launch_bounds__(TB) void k_ub_tri(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
-                                                double* __restrict__ xd, int32_t R)
-{
-    constexpr int RC = RM<T>::v, CH = 16;
-    __shared__ T xs[TB][RC];
-    const BlockDesc B = blocks[blk[blockIdx.x]];
-    const T* V = reinterpret_cast<const T*>(B.vals);
-    T* x = reinterpret_cast<T*>(xd);
-    const int w = B.w, t = threadIdx.x;
-    const bool own = t < w;
-    T y[RC];
-#pragma unroll
-    for (int r = 0; r < RC; r++) y[r] = (own && r < R) ? x[((int64_t)B.c0 + t) * R + r] : Sc<T>::zero();
-    const T d = own ? V[(int64_t)t * B.nrows + t] : Sc<T>::one();
-    // U(t, k) for the chunk's columns k > t
-    auto fetch = [&](int ch, T* lv) {
-#pragma unroll
-        for (int u = 0; u < CH; u++) {
-            const int k = ch * CH + u;
-            lv[u] = (own && k > t && k < w) ? V[(int64_t)k * B.nrows + t] : Sc<T>::zero();
-        }
-    };
-    const int nch = (w + CH - 1) / CH;
-    T lcur[CH], lnxt[CH];
-    fetch(nch - 1, lcur);
-    for (int ch = nch - 1; ch >= 0; ch--) {
-        const int r0 = ch * CH, r1 = min(w, r0 + CH);
-        if (ch > 0) fetch(ch - 1, lnxt);
-        if (t >= r0 && t < r1) {
-#pragma unroll
-            for (int u = CH - 1; u >= 0; u--) {
-                const int k = r0 + u;
-                if (k >= r1) continue;
-                if (t == k) {
-                    const T dinv = Sc<T>::div(Sc<T>::one(), d);
-#pragma unroll
-                    for (int r = 0; r < RC; r++)
-                        if (r < R) y[r] = Sc<T>::mul(y[r], dinv);
-                }
-                T yk[RC];
-#pragma unroll
-                for (int r = 0; r < RC; r++) yk[r] = shfl_sc<T>(y[r], k & 63);
-                if (t < k) {
-#pragma unroll
-                    for (int r = 0; r < RC; r++)
-                        if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], yk[r]);
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < RC; r++) xs[t][r] = y[r];
-        }
-        __syncthreads();
-        if (own && t < r0) {
-#pragma unroll
-            for (int u = CH - 1; u >= 0; u--) {
-                const int k = r0 + u;
-                if (k >= r1) continue;
-#pragma unroll
-                for (int r = 0; r < RC; r++)
-                    if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], xs[k][r]);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < CH; u++) lcur[u] = lnxt[u];
-    }
-    if (!own) return;
-#pragma unroll
-    for (int r = 0; r < RC; r++)
-        if (r < R) x[((int64_t)B.c0 + t) * R + r] = y[r];
 }
 
 // U^T y = b, part 1: the dot products of a level's block columns over the rows before their block, tiles of 4 columns
@@ -581,20 +542,21 @@ void lfwd_gather(int vs, int32_t nrow, const int32_t* rows, const int64_t* incp,
     if (vs == 1) hipLaunchKernelGGL(k_lfwd_gather<double>, dim3(g), dim3(TB), 0, s, nrow, rows, incp, incs, contrib, x, R);
     else hipLaunchKernelGGL(k_lfwd_gather<Zd>, dim3(g), dim3(TB), 0, s, nrow, rows, incp, incs, contrib, x, R);
 }
-void lfwd_pull(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, double* x, int32_t R, double bytes, hipStream_t s)
+void lfwd_pull(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const double* x, double* y, int32_t R,
+               double bytes, hipStream_t s)
 {
-    if (nblk <= 0) return;
+    if (ntile <= 0) return;
     ProfScope ps("lfwd_pull", bytes, s);
-    if (vs == 1) hipLaunchKernelGGL(k_lfwd_pull<double>, dim3(nblk), dim3(TB), 0, s, blk, blocks, x, R);
-    else hipLaunchKernelGGL(k_lfwd_pull<Zd>, dim3(nblk), dim3(TB), 0, s, blk, blocks, x, R);
+    if (vs == 1) hipLaunchKernelGGL((k_tri_prod<double, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, x, y, R);
+    else hipLaunchKernelGGL((k_tri_prod<Zd, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, x, y, R);
 }
-void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* slot_off, const double* x,
+void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* slot_off, const double* y,
                double* contrib, int32_t R, double bytes, hipStream_t s)
 {
     if (ntile <= 0) return;
     ProfScope ps("lfwd_push", bytes, s);
-    if (vs == 1) hipLaunchKernelGGL(k_lfwd_push<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, slot_off, x, contrib, R);
-    else hipLaunchKernelGGL(k_lfwd_push<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, slot_off, x, contrib, R);
+    if (vs == 1) hipLaunchKernelGGL(k_lfwd_push<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, slot_off, y, contrib, R);
+    else hipLaunchKernelGGL(k_lfwd_push<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, slot_off, y, contrib, R);
 }
 void ub_dots(int vs, int32_t ntile, const int4* tiles, const BlockDesc* blocks, const int32_t* pa, const int32_t* pj,
              const int64_t* pu, const double* Ux, const double* x, double* part, int32_t R, double bytes, hipStream_t s)
@@ -612,12 +574,13 @@ void ub_red(int vs, int32_t ntile, const int4* tiles, const BlockDesc* blocks, c
     if (vs == 1) hipLaunchKernelGGL(k_ub_red<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, part, x, R);
     else hipLaunchKernelGGL(k_ub_red<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, part, x, R);
 }
-void ub_tri(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, double* x, int32_t R, double bytes, hipStream_t s)
+void ub_prod(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const double* y, double* x, int32_t R,
+             double bytes, hipStream_t s)
 {
-    if (nblk <= 0) return;
-    ProfScope ps("ub_tri", bytes, s);
-    if (vs == 1) hipLaunchKernelGGL(k_ub_tri<double>, dim3(nblk), dim3(TB), 0, s, blk, blocks, x, R);
-    else hipLaunchKernelGGL(k_ub_tri<Zd>, dim3(nblk), dim3(TB), 0, s, blk, blocks, x, R);
+    if (ntile <= 0) return;
+    ProfScope ps("ub_prod", bytes, s);
+    if (vs == 1) hipLaunchKernelGGL((k_tri_prod<double, true>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, y, x, R);
+    else hipLaunchKernelGGL((k_tri_prod<Zd, true>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, y, x, R);
 }
 void utdots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
             const double* Ux, double* x, int32_t R, int conj, double bytes, hipStream_t s)
