@@ -487,7 +487,8 @@ int Driver::panel(int64_t j0, int64_t j1)
     F_.t_lev += now_s() - th_ws;
     F_.t_lev_pass += now_s() - tlp0;
     std::stable_sort(tl.begin(), tl.end(), [](const TaskL& a, const TaskL& b) { return a.level < b.level; });
-    // tile rows per level: 64, or 32 where the level's 64-row tiles would not fill the device twice
+    // tile rows per level: 64, or 32 where the level has fewer than four 64-row tiles per compute unit (measured: the
+    // 32-row tiles, which read two k chunks ahead, are as fast or faster up to there)
     std::vector<int32_t> lev_rt(maxlevel + 2, RT);
     {
         std::vector<int64_t> nt64(maxlevel + 2, 0);
@@ -495,7 +496,7 @@ int Driver::panel(int64_t j0, int64_t j1)
             const BlockDesc& B = F_.blocks[x.t.block];
             nt64[x.level] += (int64_t)((B.nrows - x.t.k0 + RT - 1) / RT) * ((x.t.ncol + CTG - 1) / CTG);
         }
-        const int64_t fill = 2 * (int64_t)devctx().ncu;
+        const int64_t fill = 4 * (int64_t)devctx().ncu;
         for (int32_t L = 0; L <= maxlevel + 1; L++) lev_rt[L] = nt64[L] < fill ? RT / 2 : RT;
     }
     std::vector<UpdTask> tasks(tl.size());
