@@ -21,10 +21,15 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cstddef>
 #include <cstring>
+#include <exception>
+#include <functional>
+#include <mutex>
 #include <numeric>
 #include <stdexcept>
+#include <thread>
 
 namespace fpa {
 namespace {
@@ -62,6 +67,70 @@ template <class T> struct Arena {
         used += (cnt + 7) & ~(size_t)7;
         return p;
     }
+};
+
+// one helper thread running one job at a time: a panel's output (blocks and U out of the workspace) while the
+// next panel's search runs.  wait() returns when the job is done and rethrows what it threw.
+class SideWorker {
+public:
+    SideWorker() : th_([this] { loop(); }) {}
+    ~SideWorker()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        th_.join();
+    }
+    SideWorker(const SideWorker&) = delete;
+    SideWorker& operator=(const SideWorker&) = delete;
+    void submit(std::function<void()> fn)
+    {
+        std::lock_guard<std::mutex> lk(mu_);
+        job_ = std::move(fn);
+        busy_ = true;
+        cv_.notify_all();
+    }
+    void wait()
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [this] { return !busy_; });
+        if (err_) {
+            std::exception_ptr e = err_;
+            err_ = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
+
+private:
+    void loop()
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        for (;;) {
+            cv_.wait(lk, [this] { return stop_ || job_; });
+            if (!job_) return;
+            std::function<void()> fn = std::move(job_);
+            job_ = nullptr;
+            lk.unlock();
+            std::exception_ptr e;
+            try {
+                fn();
+            } catch (...) {
+                e = std::current_exception();
+            }
+            lk.lock();
+            err_ = e;
+            busy_ = false;
+            done_cv_.notify_all();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    std::function<void()> job_;
+    bool busy_ = false, stop_ = false;
+    std::exception_ptr err_;
+    std::thread th_;
 };
 
 class Driver {
@@ -151,6 +220,25 @@ private:
         S.srch_end = b;
     }
     int panel(int64_t j0, int64_t j1);
+    // the panel's output, on the helper thread; its inputs are moved into out_ by panel()
+    int panel_out();
+    struct OutState {
+        int64_t j0 = 0, j1 = 0, Rl = 0, ldw = 0, u_begin = 0;
+        double* Wl = nullptr;
+        std::vector<int32_t> univ, lrows;
+        std::vector<std::pair<int64_t, int64_t>> touched;
+        std::vector<URun> pruns;
+    } out_;
+    int out_rc_ = 0;
+    bool out_pending_ = false;
+    // wait for the previous panel's output; its status (or its exception)
+    int join_out()
+    {
+        if (!out_pending_) return FPLUS_OK;
+        out_pending_ = false;
+        side_.wait();
+        return out_rc_;
+    }
     const BlockDesc* d_blocks_p() const { return F_.d_blocks.p; }
     void d_blocks_upload()
     {
@@ -220,6 +308,7 @@ private:
     Arena<double> uinv_{nullptr, 0, 0, 0};
     Arena<int32_t> rows_{nullptr, 0, 0, 0};
     hipStream_t s_ = nullptr;
+    SideWorker side_;   // last: destroyed (joined) first, before the state its job uses
 };
 
 void Driver::add_block(int64_t s, int64_t b0, int64_t b1, int64_t j0, const std::vector<int32_t>& lpos,
@@ -266,7 +355,6 @@ int Driver::panel(int64_t j0, int64_t j1)
     const int64_t np = j1 - j0, n = n_;
     const int64_t open0 = open_, we_open = open0 >= 0 ? sn_[open0].w : 0, nr_open = open0 >= 0 ? sn_[open0].nr() : 0;
     const double th0 = now_s();
-    up_.begin();
     // ---- phase 1: per-column search over the supernodes complete before j0 (src/factor_sn_panel.inc:34-58); the
     // columns' searches are independent (they read the structure as it stood at j0): threads take columns in turn,
     // each with its own marks, and the results are joined in column order
@@ -305,6 +393,12 @@ int Driver::panel(int64_t j0, int64_t j1)
     ext_off[np] = (int64_t)ext_s.size();
     const double th_p1 = now_s();
     F_.t_p1 += th_p1 - th0;
+    // the previous panel's output ran meanwhile: the structure the next steps read is complete once it is done
+    {
+        const int orc = join_out();
+        if (orc != FPLUS_OK) return orc;
+    }
+    up_.begin();
     // ---- the union of the reached supernodes, in a topological order of the updates (the reference takes one from a
     // search over the union, src/factor_sn_panel.inc:61-70): increasing supernode index is one -- a block's rows below
     // its pivot rows were not pivotal when its supernode opened, so the supernodes whose pivot rows they become opened
@@ -356,7 +450,18 @@ int Driver::panel(int64_t j0, int64_t j1)
                 for (const int64_t row : c_np[ci])
                     if (hrowpos_[row] < 0 && w.rmark[row] != stamp) { w.rmark[row] = stamp; out.push_back((int32_t)row); }
         });
-        for (int64_t c = 0; c < nch; c++)
+        // the join: the first chunk's rows (most of them) placed, the other chunks' lists filtered against them in
+        // parallel, then the few rows left joined in chunk order
+        for (const int32_t row : ch_rows_[0]) { hrowpos_[row] = (int32_t)univ.size(); univ.push_back(row); }
+        if (nch > 1)
+            pool_.run(nch - 1, [&](int64_t c, int) {
+                std::vector<int32_t>& l = ch_rows_[c + 1];
+                size_t k = 0;
+                for (const int32_t row : l)
+                    if (hrowpos_[row] < 0) l[k++] = row;
+                l.resize(k);
+            });
+        for (int64_t c = 1; c < nch; c++)
             for (const int32_t row : ch_rows_[c])
                 if (hrowpos_[row] < 0) { hrowpos_[row] = (int32_t)univ.size(); univ.push_back(row); }
         F_.t_ws_low += now_s() - tl0;
@@ -762,9 +867,34 @@ int Driver::panel(int64_t j0, int64_t j1)
         F_.t_p3_join += tq3 - tq2;
         F_.t_p3_prune += tq4 - tq3;
     }
-    const double th_out = now_s();
-    F_.t_p3 += th_out - th1 - waited;
+    F_.t_p3 += now_s() - th1 - waited;
     F_.t_wait += waited;
+    F_.t_host += now_s() - th1 - waited;
+    // ---- the output on the helper thread, while the next panel's search runs
+    out_.j0 = j0;
+    out_.j1 = j1;
+    out_.Rl = Rl;
+    out_.ldw = ldw;
+    out_.u_begin = u_begin;
+    out_.Wl = Wl;
+    out_.univ = std::move(univ);
+    out_.lrows = std::move(lrows);
+    out_.touched = std::move(touched);
+    out_.pruns = std::move(pruns);
+    out_pending_ = true;
+    side_.submit([this] { out_rc_ = panel_out(); });
+    return FPLUS_OK;
+}
+
+int Driver::panel_out()
+{
+    const double th_out = now_s();
+    const int64_t j0 = out_.j0, j1 = out_.j1, Rl = out_.Rl, ldw = out_.ldw, u_begin = out_.u_begin;
+    double* Wl = out_.Wl;
+    const std::vector<int32_t>& univ = out_.univ;
+    const std::vector<int32_t>& lrows = out_.lrows;
+    const std::vector<std::pair<int64_t, int64_t>>& touched = out_.touched;
+    const std::vector<URun>& pruns = out_.pruns;
     // the final positions of the lower rows
     for (int64_t i = 0; i < Rl; i++) lposmap_[lrows[i]] = (int32_t)i;
     // ---- device: the new blocks and the U values out of the workspace
@@ -831,10 +961,11 @@ int Driver::panel(int64_t j0, int64_t j1)
     for (int32_t row : univ) hrowpos_[row] = -1;
     for (int32_t row : lrows) lposmap_[row] = -1;
     F_.t_out += now_s() - th_out;
-    F_.t_host += now_s() - th1 - waited;
+    F_.t_host += now_s() - th_out;
     F_.panels++;
     return FPLUS_OK;
 }
+
 
 int Driver::run()
 {
@@ -885,11 +1016,24 @@ int Driver::run()
     rows_.min_chunk = (size_t)1 << 24;
     const int64_t P = std::max<int64_t>(1, opt_.panel);
     for (int64_t j0 = 0; j0 < n; j0 += P) {
-        const int rc = panel(j0, std::min(j0 + P, n));
-        if (rc != FPLUS_OK) return rc;
+        int rc = FPLUS_OK;
+        try {
+            rc = panel(j0, std::min(j0 + P, n));
+        } catch (...) {
+            if (out_pending_) { out_pending_ = false; try { side_.wait(); } catch (...) {} }
+            throw;
+        }
+        if (rc != FPLUS_OK) {
+            if (out_pending_) { out_pending_ = false; try { side_.wait(); } catch (...) {} }
+            return rc;
+        }
         if (opt_.verbose >= 2 && ((j0 / P) % (n / (20 * P) + 1) == 0))
             printf("[FPLUS]   [%5.1f%%] column %ld/%ld  supernodes=%ld (panel %ld)\n", 100.0 * (double)std::min(j0 + P, n) / (double)n,
                    (long)std::min(j0 + P, n), (long)n, (long)sn_.size(), (long)P);
+    }
+    {
+        const int orc = join_out();
+        if (orc != FPLUS_OK) return orc;
     }
     dev_sync(s_);   // the last panel's blocks and U values are out
     // the final structure (the inverse forms go with the driver: the solves use the blocks' values only)
