@@ -237,6 +237,7 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
     const int32_t nb = (int32_t)F.blocks.size();
     double tp = now_s();
     auto lap = [&](int i) { const double t = now_s(); num->t_prep_part[i] = t - tp; tp = t; };
+    HostPool pool;
     std::vector<int32_t> colblk(n > 0 ? n : 1);
     for (int32_t b = 0; b < nb; b++)
         for (int32_t c = 0; c < F.blocks[b].w; c++) colblk[F.blocks[b].c0 + c] = b;
@@ -303,7 +304,6 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         };
         // in parallel: chunks of columns count their pieces per block, the chunks' places within each block follow in
         // chunk order (so every block's pieces are in column order: a fixed summation order), the chunks fill them
-        HostPool pool;
         const int64_t nch = std::min<int64_t>(std::max<int64_t>(n, 1), 4 * (int64_t)pool.size()), per = (n + nch - 1) / nch;
         std::vector<std::vector<int64_t>> cnt(nch, std::vector<int64_t>(nb + 1, 0));
         pool.run(nch, [&](int64_t c, int) {
@@ -337,25 +337,42 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         dev_sync(s);
     }
     lap(1);
+    // the levels of the two U sweeps from per-block lists of the distinct blocks they wait for, made in parallel
+    std::vector<std::vector<int32_t>> dep(nb > 0 ? nb : 1);
+    pool.run(nb, [&](int64_t b, int) {   // U x = y: the blocks of the pieces' columns (ascending: equal ones adjacent)
+        std::vector<int32_t>& d = dep[b];
+        d.clear();
+        for (int64_t o = pb[b]; o < pb[b + 1]; o++) {
+            const int32_t bj = colblk[pj[o]];
+            if (d.empty() || d.back() != bj) d.push_back(bj);
+        }
+    });
     for (int32_t b = nb - 1; b >= 0; b--) {
         int32_t L = 0;
-        for (int64_t o = pb[b]; o < pb[b + 1]; o++) L = std::max(L, lvl[colblk[pj[o]]] + 1);
+        for (const int32_t bj : dep[b]) L = std::max(L, lvl[bj] + 1);
         lvl[b] = L;
     }
     order_levels(lvl, num->blk_ub, num->lev_ub);
     // U^T y = b: a block waits for the blocks holding the rows of its U columns above it
-    for (int32_t b = 0; b < nb; b++) {
+    pool.run(nb, [&](int64_t b, int) {
         const BlockDesc& B = F.blocks[b];
-        int32_t L = 0;
+        std::vector<int32_t>& d = dep[b];
+        d.clear();
         for (int64_t j = B.c0; j < (int64_t)B.c0 + B.w; j++)
             for (int64_t r = F.Urp[j]; r < F.Urp[j + 1]; r++) {
                 const int64_t r0 = F.Ur_row[r], r1 = std::min<int64_t>(r0 + F.Ur_len[r], B.c0);
                 for (int64_t i = r0; i < r1;) {
                     const int32_t bi = colblk[i];
-                    L = std::max(L, lvl[bi] + 1);
+                    d.push_back(bi);
                     i = (int64_t)F.blocks[bi].c0 + F.blocks[bi].w;
                 }
             }
+        std::sort(d.begin(), d.end());
+        d.erase(std::unique(d.begin(), d.end()), d.end());
+    });
+    for (int32_t b = 0; b < nb; b++) {
+        int32_t L = 0;
+        for (const int32_t bi : dep[b]) L = std::max(L, lvl[bi] + 1);
         lvl[b] = L;
     }
     order_levels(lvl, num->blk_utf, num->lev_utf);
@@ -368,17 +385,45 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         int64_t ns = 0;
         for (int32_t b = 0; b < nb; b++) {
             slot_off[b] = ns;
-            const BlockDesc& B = F.blocks[b];
-            ns += B.nrows - B.w;
-            for (int32_t r = B.w; r < B.nrows; r++) incp[F.pinv[F.block_rows[b][r]] + 1]++;
+            ns += F.blocks[b].nrows - F.blocks[b].w;
         }
-        for (int64_t i = 0; i < n; i++) incp[i + 1] += incp[i];
+        // in parallel over chunks of blocks: each chunk counts its slots per row, every row's slots are placed chunk
+        // by chunk (so still in block order), then the chunks fill their places
+        const int32_t nch = (int32_t)std::min<int64_t>(std::max(nb, 1), pool.size()), per = (nb + nch - 1) / nch;
+        std::vector<std::vector<int32_t>> cnt(nch);
+        pool.run(nch, [&](int64_t c, int) {
+            std::vector<int32_t>& h = cnt[c];
+            h.assign(n > 0 ? n : 1, 0);
+            for (int32_t b = (int32_t)c * per; b < std::min<int32_t>(nb, ((int32_t)c + 1) * per); b++) {
+                const BlockDesc& B = F.blocks[b];
+                const int32_t* br = F.block_rows[b].data();
+                for (int32_t r = B.w; r < B.nrows; r++) h[F.pinv[br[r]]]++;
+            }
+        });
+        for (int64_t i = 0; i < n; i++) {
+            int64_t t = 0;
+            for (int32_t c = 0; c < nch; c++) t += cnt[c][i];
+            incp[i + 1] = incp[i] + t;
+        }
+        std::vector<std::vector<int64_t>> fill(nch);
+        pool.run(nch, [&](int64_t c, int) {   // chunk c's first place in every row
+            std::vector<int64_t>& f = fill[c];
+            f.resize(n > 0 ? n : 1);
+            for (int64_t i = 0; i < n; i++) {
+                int64_t o = incp[i];
+                for (int32_t c2 = 0; c2 < c; c2++) o += cnt[c2][i];
+                f[i] = o;
+            }
+        });
         incs.resize(ns > 0 ? ns : 1);
-        std::vector<int64_t> fill(incp.begin(), incp.end() - 1);
-        for (int32_t b = 0; b < nb; b++) {
-            const BlockDesc& B = F.blocks[b];
-            for (int32_t r = B.w; r < B.nrows; r++) incs[fill[F.pinv[F.block_rows[b][r]]]++] = slot_off[b] + r - B.w;
-        }
+        pool.run(nch, [&](int64_t c, int) {
+            std::vector<int64_t>& f = fill[c];
+            for (int32_t b = (int32_t)c * per; b < std::min<int32_t>(nb, ((int32_t)c + 1) * per); b++) {
+                const BlockDesc& B = F.blocks[b];
+                const int32_t* br = F.block_rows[b].data();
+                for (int32_t r = B.w; r < B.nrows; r++) incs[f[F.pinv[br[r]]]++] = slot_off[b] + r - B.w;
+            }
+        });
         num->nslots = ns;
         auto up64 = [s](DevBuf<int64_t>& d, const std::vector<int64_t>& h) { d.alloc(h.size()); d.upload(h.data(), h.size(), s); };
         up64(num->d_slot_off, slot_off);

@@ -270,7 +270,7 @@ private:
     std::vector<std::vector<int64_t>> tmark_;   // one mark array per thread of the pool, indexed by task
     SWork ws0_;                     // the sequential phases' marks
     std::vector<SWork> wsT_;        // one per thread of the parallel search
-    HostPool pool_;
+    HostPool pool_{15};             // one core left for the helper thread
     int64_t relax_ = 0, rz_w_[3] = {4, 16, 48}, rz_cap_ = 32;
     double rz_z_[3] = {0.8, 0.1, 0.05};
     bool relax_auto_ = false;
