@@ -208,8 +208,8 @@ struct fplus_numeric {
     DevBuf<int4> d_ubr;        // and the partial sums' reduction tiles: block, first of 16 rows, first slot, slots
     int64_t ub_slots_max = 0;  // the most partial slots of one level
     std::vector<double> bytes_ub_prod;
-    std::vector<int32_t> lev_lpt, lev_upt;   // level offsets into the triangle-product tiles (block, first of 64 rows)
-    DevBuf<int2> d_lpt, d_upt;
+    std::vector<int32_t> lev_lpt, lev_upt, lev_utp, lev_ltp;   // level offsets into the triangle-product tiles
+    DevBuf<int2> d_lpt, d_upt, d_utp, d_ltp;                    // (block, first of 64 rows), per sweep
     std::vector<double> bytes_lf_pull, bytes_lf_push, bytes_ub, bytes_utf, bytes_ltb;
     // U x = y beyond the blocks: a block's pieces are the runs of later columns over its rows, each covering the block's
     // rows a .. w-1 (a = pa[p]), in column order; piece p holds U(c0 + t, pj[p]) at Ux[pu[p] + t]
@@ -528,6 +528,20 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
             };
             tiles_of(num->lev_utf, num->blk_utf, num->lev_utt, num->d_utt);
             tiles_of(num->lev_ltb, num->blk_ltb, num->lev_ltt, num->d_ltt);
+            // and the triangle-product tiles (64 rows each) per level
+            auto ptiles_of = [&](const std::vector<int32_t>& lev, const std::vector<int32_t>& blk, std::vector<int32_t>& lt, DevBuf<int2>& d) {
+                std::vector<int2> tt;
+                lt.assign(lev.size(), 0);
+                for (size_t L = 0; L + 1 < lev.size(); L++) {
+                    for (int32_t k2 = lev[L]; k2 < lev[L + 1]; k2++)
+                        for (int32_t r0 = 0; r0 < F.blocks[blk[k2]].w; r0 += 64) tt.push_back(make_int2(blk[k2], r0));
+                    lt[L + 1] = (int32_t)tt.size();
+                }
+                d.alloc(tt.size() > 0 ? tt.size() : 1);
+                d.upload(tt.data(), tt.size(), s);
+            };
+            ptiles_of(num->lev_utf, num->blk_utf, num->lev_utp, num->d_utp);
+            ptiles_of(num->lev_ltb, num->blk_ltb, num->lev_ltp, num->d_ltp);
             dev_sync(s);
         }
         num->bytes_utf.assign(nut, 0.0);
@@ -595,18 +609,19 @@ void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, hi
                        num->bytes_ub_prod[L], s);
         }
     } else {
+        // U^T y = b from w into y, then L^T x = y from y into w
         const int cj = trans == 2 && vs == 2;
+        num->d_y.reserve((size_t)std::max<int64_t>(F.n, 1) * R * vs);
+        double* y = num->d_y.p;
         for (size_t L = 0; L + 1 < num->lev_utf.size(); L++) {
             k::utdots(vs, num->lev_utt[L + 1] - num->lev_utt[L], num->d_utt.p + num->lev_utt[L], blocks, num->d_Up.p, num->F.d_Ui.p,
-                      F.d_Ux.p, w, R, cj, num->bytes_utf[L], s);
-            k::utfwd(vs, num->lev_utf[L + 1] - num->lev_utf[L], num->d_blk_utf.p + num->lev_utf[L], blocks, num->d_Up.p,
-                     num->F.d_Ui.p, F.d_Ux.p, w, R, cj, 0.0, s);
+                      F.d_Ux.p, y, w, R, cj, num->bytes_utf[L], s);
+            k::ut_prod(vs, num->lev_utp[L + 1] - num->lev_utp[L], num->d_utp.p + num->lev_utp[L], blocks, w, y, R, cj, 0.0, s);
         }
         for (size_t L = 0; L + 1 < num->lev_ltb.size(); L++) {
-            k::ltdots(vs, num->lev_ltt[L + 1] - num->lev_ltt[L], num->d_ltt.p + num->lev_ltt[L], blocks, num->d_pinv.p, w, R, cj,
+            k::ltdots(vs, num->lev_ltt[L + 1] - num->lev_ltt[L], num->d_ltt.p + num->lev_ltt[L], blocks, num->d_pinv.p, w, y, R, cj,
                       num->bytes_ltb[L], s);
-            k::ltbwd(vs, num->lev_ltb[L + 1] - num->lev_ltb[L], num->d_blk_ltb.p + num->lev_ltb[L], blocks, num->d_pinv.p, w, R,
-                     cj, 0.0, s);
+            k::lt_prod(vs, num->lev_ltp[L + 1] - num->lev_ltp[L], num->d_ltp.p + num->lev_ltp[L], blocks, y, w, R, cj, 0.0, s);
         }
     }
 }

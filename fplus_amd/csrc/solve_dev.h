@@ -32,15 +32,18 @@ void ub_red(int vs, int32_t ntile, const int4* tiles, const BlockDesc* blocks, c
             hipStream_t s);
 void ub_prod(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const double* y, double* x, int32_t R,
              double bytes, hipStream_t s);
-// the transposed sweeps' dot products (tiles: block, first of 4 columns), before the blocks' triangles
+// the transposed sweeps per level: the dot products (tiles: block, first of 4 columns) with the solved rows of other
+// blocks (U^T: y of earlier blocks, subtracted in x; L^T: x of later blocks, subtracted in y), then the blocks' triangles
+// as products with the kept inverses, transposed and conjugated with conj (tiles: block, first of 64 rows):
+// ut_prod y = U11^-T x, lt_prod x = L11^-T y on the blocks' rows
 void utdots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
-            const double* Ux, double* x, int32_t R, int conj, double bytes, hipStream_t s);
-void ltdots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int32_t* pinv_rows, double* x, int32_t R,
-            int conj, double bytes, hipStream_t s);
-void utfwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
-           const double* Ux, double* x, int32_t R, int conj, double bytes, hipStream_t s);
-void ltbwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int32_t* pinv_rows, double* x, int32_t R,
-           int conj, double bytes, hipStream_t s);
+            const double* Ux, const double* y, double* x, int32_t R, int conj, double bytes, hipStream_t s);
+void ltdots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int32_t* pinv_rows, const double* x,
+            double* y, int32_t R, int conj, double bytes, hipStream_t s);
+void ut_prod(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const double* x, double* y, int32_t R, int conj,
+             double bytes, hipStream_t s);
+void lt_prod(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const double* y, double* x, int32_t R, int conj,
+             double bytes, hipStream_t s);
 // w[k * R + r] = D[perm[k]] * b[perm[k] + r * ldb]  /  b[perm[k] + r * ldb] = D[perm[k]] * w[k * R + r]
 void gather_scale(int vs, int64_t n, int32_t R, const int64_t* perm, const double* D, const double* b, int64_t ldb, double* w,
                   hipStream_t s);

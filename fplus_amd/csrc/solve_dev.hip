@@ -4,8 +4,8 @@
 // U column is a list of runs of consecutive rows, which the solves with A read per block.  The right-hand sides live row-major during the sweeps, x[i * R + r] for
 // R right-hand sides, so one entry of a factor meets all R values of a row in one contiguous read.  A sweep walks the
 // blocks level by level (a level's blocks are independent), one launch per level and step, and every sweep pulls: each
-// value is written by one thread, no atomics, so the results are the same on every run.  The sweeps with A use a second
-// vector y: L y = b leaves its result in y, U x = y its result in x.
+// value is written by one thread, no atomics, so the results are the same on every run.  The sweeps use a second vector
+// y: L y = b leaves its result in y, U x = y its result in x; U^T y = b leaves its result in y, L^T x = y in x.
 //   lfwd_gather L y = b   a level's pivot rows gather the contributions the blocks before them left in their slots
 //                         (one wavefront per row, in x);
 //   lfwd_pull   y on the block's rows = L11^-1 x (the inverse kept from the factorization);
@@ -13,9 +13,11 @@
 //   ub_dots     U x = y   a block's pivot rows: dot products along their U rows beyond the block (the runs of later
 //   ub_red                U columns over the block's rows, in column order) with x, subtracted from y;
 //   ub_prod     x on the block's rows = U11^-1 y;
-//   utfwd       U^T y = b (U^H): dot products down U's columns, the block's own rows in order;
-//   ltbwd       L^T x = y (L^H): dot products down the block's columns, the triangle last to first.
-// Blocks are at most 256 columns wide (the panel width): the triangles keep one row per thread.
+//   utdots      U^T y = b (U^H): dot products down U's columns with the solved rows in y, subtracted in x;
+//   ut_prod     y on the block's rows = U11^-T x (conjugated for U^H);
+//   ltdots      L^T x = y (L^H): dot products down the block's columns with the solved rows in x, subtracted in y;
+//   lt_prod     x on the block's rows = L11^-T y.
+// Blocks are at most 256 columns wide (the panel width): a block's input for all right-hand sides fits in LDS.
 #include "dev_common.h"
 #include "device_api.h"
 #include "prof.h"
@@ -78,31 +80,24 @@ __global__ __launch_bounds__(TB) void k_lfwd_gather(int32_t nrow, const int32_t*
     }
 }
 
-// the diagonal triangles of the solves, one workgroup per block: thread t owns the block's pivot row t (at most 256:
-// a block is at most one panel wide) with all right-hand sides in registers.  The rows go in chunks of 16: a chunk's
-// triangle inside its wavefront (the row just finished broadcast by a lane shuffle), one barrier, then the rows after
-// the chunk apply its 16 columns.  Each row receives its updates in the column order of the column-by-column sweep.
-// The factor's values of the next chunk are read while the current one is solved.
-template <class T> __device__ inline T shfl_sc(T v, int src);
-template <> __device__ inline double shfl_sc<double>(double v, int src) { return __shfl(v, src); }
-template <> __device__ inline Zd shfl_sc<Zd>(Zd v, int src) { return zmake(__shfl(v.x, src), __shfl(v.y, src)); }
-
-// The blocks' diagonal triangles in the sweeps with A, as products with the inverses kept from the factorization:
-// out(block rows) = M in(block rows), M = L11^-1 (lower, unit diagonal) or U11^-1 (upper), w x w column-major (ld = w).
-// One workgroup per 64 rows of a block: lane = row; the four waves take a quarter each of the columns the rows reach
-// (lower: up to the chunk's last row, upper: from its first row), with the block's input for those columns and all
-// right-hand sides in LDS; the quarters' sums are added in wave order -- a fixed order that does not depend on the
-// number of right-hand sides.  The output goes to another vector: the block's input is still read by its other row
-// chunks.
-template <class T, bool UPPER>
+// The blocks' diagonal triangles in the sweeps, as products with the inverses kept from the factorization:
+// out(block rows) = Op in(block rows), Op = L11^-1 (lower, unit diagonal) or U11^-1 (upper) for the sweeps with A, and
+// their transposes (conjugated for the sweeps with A^H: conj) for those with A^T; the inverses are w x w column-major
+// (ld = w), so Op(row, c) = M[row + c w], or M[c + row w] transposed.  UPPER: Op is upper triangular.  One workgroup
+// per 64 rows of a block: lane = row; the four waves take a quarter each of the columns the rows reach (lower: up to
+// the chunk's last row, upper: from its first row), with the block's input for those columns and all right-hand sides
+// in LDS; the quarters' sums are added in wave order -- a fixed order that does not depend on the number of right-hand
+// sides.  The output goes to another vector: the block's input is still read by its other row chunks.
+template <class T, bool UPPER, bool TRANS>
 __global__ __launch_bounds__(TB) void k_tri_prod(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
-                                                  const double* __restrict__ ind, double* __restrict__ outd, int32_t R)
+                                                  const double* __restrict__ ind, double* __restrict__ outd, int32_t R,
+                                                  int conj)
 {
     constexpr int RC = RM<T>::v;
     __shared__ T xs[TB * RC];
     const int2 tl = tiles[blockIdx.x];   // block, first row
     const BlockDesc B = blocks[tl.x];
-    const T* M = reinterpret_cast<const T*>(UPPER ? B.uinv : B.linv);
+    const T* M = reinterpret_cast<const T*>(UPPER != TRANS ? B.uinv : B.linv);
     const T* in = reinterpret_cast<const T*>(ind);
     T* out = reinterpret_cast<T*>(outd);
     const int w = B.w, r0 = tl.y;
@@ -121,7 +116,7 @@ __global__ __launch_bounds__(TB) void k_tri_prod(const int2* __restrict__ tiles,
         const int lo = UPPER ? max(qa, row) : qa, hi = UPPER ? qb : min(qb, row + 1);
 #pragma unroll 8
         for (int c = lo; c < hi; c++) {
-            const T v = M[row + (int64_t)c * w];
+            const T v = TRANS ? cj(M[c + (int64_t)row * w], conj) : M[row + (int64_t)c * w];
 #pragma unroll
             for (int r = 0; r < RC; r++)
                 if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, xs[(c - ca) * RC + r]);
@@ -275,15 +270,17 @@ __global__ __launch_bounds__(TB) void k_ub_red(const int4* __restrict__ tiles, c
 }
 
 // U^T y = b, part 1: the dot products of a level's block columns over the rows before their block, tiles of 4 columns
-// (one wavefront per column): x_c -= sum_{i < c0} conj?(U(i, c)) x_i
+// (one wavefront per column): x_c -= sum_{i < c0} conj?(U(i, c)) y_i, y the solved rows of earlier blocks
 template <class T>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_utdots(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
                                                 const int64_t* __restrict__ Up, const int32_t* __restrict__ Ui,
-                                                const double* __restrict__ Uxd, double* __restrict__ xd, int32_t R, int conj)
+                                                const double* __restrict__ Uxd, const double* __restrict__ yd,
+                                                double* __restrict__ xd, int32_t R, int conj)
 {
     const int2 tl = tiles[blockIdx.x];
     const BlockDesc B = blocks[tl.x];
     const T* Ux = reinterpret_cast<const T*>(Uxd);
+    const T* y = reinterpret_cast<const T*>(yd);
     T* x = reinterpret_cast<T*>(xd);
     const int lane = threadIdx.x & 63, c = tl.y + (threadIdx.x >> 6);
     if (c >= B.w) return;
@@ -297,7 +294,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         const T u = cj(Ux[q], conj);
 #pragma unroll
         for (int r = 0; r < RM<T>::v; r++)
-            if (r < R) acc[r] = Sc<T>::fma_(acc[r], u, x[i * R + r]);
+            if (r < R) acc[r] = Sc<T>::fma_(acc[r], u, y[i * R + r]);
     }
 #pragma unroll
     for (int r = 0; r < RM<T>::v; r++) {
@@ -308,15 +305,17 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
 }
 
 // L^T x = y, part 1: the dot products of a level's block columns over the block's rows below, tiles of 4 columns:
-// x_c -= sum_{rows i below} conj?(L(i, c)) x_pinv(i)
+// y_c -= sum_{rows i below} conj?(L(i, c)) x_pinv(i), x the solved rows of later blocks
 template <class T>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_ltdots(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
-                                                const int32_t* __restrict__ pinv_rows, double* __restrict__ xd, int32_t R, int conj)
+                                                const int32_t* __restrict__ pinv_rows, const double* __restrict__ xd,
+                                                double* __restrict__ yd, int32_t R, int conj)
 {
     const int2 tl = tiles[blockIdx.x];
     const BlockDesc B = blocks[tl.x];
     const T* V = reinterpret_cast<const T*>(B.vals);
-    T* x = reinterpret_cast<T*>(xd);
+    const T* x = reinterpret_cast<const T*>(xd);
+    T* y = reinterpret_cast<T*>(yd);
     const int lane = threadIdx.x & 63, c = tl.y + (threadIdx.x >> 6), w = B.w;
     if (c >= w) return;
     T acc[RM<T>::v];
@@ -333,163 +332,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     for (int r = 0; r < RM<T>::v; r++) {
         if (r >= R) break;
         const T sm = wave_sum(acc[r]);
-        if (lane == 0) x[(B.c0 + c) * R + r] = Sc<T>::sub(x[(B.c0 + c) * R + r], sm);
+        if (lane == 0) y[(B.c0 + c) * R + r] = Sc<T>::sub(y[(B.c0 + c) * R + r], sm);
     }
-}
-
-// U^T y = b (U^H with conj), one workgroup per block
-template <class T>
-__global__ __launch_bounds__(TB) void k_utfwd(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
-                                               const int64_t* __restrict__ Up, const int32_t* __restrict__ Ui,
-                                               const double* __restrict__ Uxd, double* __restrict__ xd, int32_t R, int conj)
-{
-    __shared__ T ys[TB * RM<T>::v];
-    const BlockDesc B = blocks[blk[blockIdx.x]];
-    T* x = reinterpret_cast<T*>(xd);
-    const int w = B.w;
-    const int64_t c0 = B.c0;
-    // (the dot products over the rows before the block are done: k_utdots)
-    for (int idx = threadIdx.x; idx < w * R; idx += TB) ys[idx] = x[c0 * R + idx];
-    __syncthreads();
-    // the block's own rows: thread c owns column c, chunks of 16 first to last, within a chunk the first row first,
-    // each scaled by the reciprocal of its diagonal before it is broadcast; U(k, c) = V[c * nrows + k] for k < c (the
-    // block's upper part)
-    constexpr int RC = RM<T>::v, CH = 16;
-    const T* V = reinterpret_cast<const T*>(B.vals);
-    const int t = threadIdx.x;
-    const bool own = t < w;
-    T y[RC];
-#pragma unroll
-    for (int r = 0; r < RC; r++) y[r] = (own && r < R) ? ys[t * R + r] : Sc<T>::zero();
-    const T dinv = own ? Sc<T>::div(Sc<T>::one(), cj(V[(int64_t)t * B.nrows + t], conj)) : Sc<T>::one();
-    auto fetch = [&](int ch, T* lv) {
-#pragma unroll
-        for (int u = 0; u < CH; u++) {
-            const int k = ch * CH + u;
-            lv[u] = (own && k < t) ? cj(V[(int64_t)t * B.nrows + k], conj) : Sc<T>::zero();
-        }
-    };
-    const int nch = (w + CH - 1) / CH;
-    T lcur[CH], lnxt[CH];
-    fetch(0, lcur);
-    __syncthreads();
-    for (int ch = 0; ch < nch; ch++) {
-        const int r0 = ch * CH, r1 = min(w, r0 + CH);
-        if (ch + 1 < nch) fetch(ch + 1, lnxt);
-        if (t >= r0 && t < r1) {
-#pragma unroll
-            for (int u = 0; u < CH; u++) {
-                const int k = r0 + u;
-                if (k >= r1) break;
-                if (t == k) {
-#pragma unroll
-                    for (int r = 0; r < RC; r++)
-                        if (r < R) y[r] = Sc<T>::mul(y[r], dinv);
-                }
-                T yk[RC];
-#pragma unroll
-                for (int r = 0; r < RC; r++) yk[r] = shfl_sc<T>(y[r], k & 63);
-                if (t > k) {
-#pragma unroll
-                    for (int r = 0; r < RC; r++)
-                        if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], yk[r]);
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < RC; r++)
-                if (r < R) ys[t * R + r] = y[r];
-        }
-        __syncthreads();
-        if (own && t >= r1) {
-#pragma unroll
-            for (int u = 0; u < CH; u++) {
-                const int k = r0 + u;
-                if (k >= r1) break;
-#pragma unroll
-                for (int r = 0; r < RC; r++)
-                    if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], ys[k * R + r]);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < CH; u++) lcur[u] = lnxt[u];
-    }
-    if (!own) return;
-#pragma unroll
-    for (int r = 0; r < RC; r++)
-        if (r < R) x[(c0 + t) * R + r] = y[r];
-}
-
-// L^T x = y (L^H with conj), one workgroup per block
-template <class T>
-__global__ __launch_bounds__(TB) void k_ltbwd(const int32_t* __restrict__ blk, const BlockDesc* __restrict__ blocks,
-                                               const int32_t* __restrict__ pinv_rows, double* __restrict__ xd, int32_t R, int conj)
-{
-    __shared__ T ys[TB * RM<T>::v];
-    const BlockDesc B = blocks[blk[blockIdx.x]];
-    const T* V = reinterpret_cast<const T*>(B.vals);
-    T* x = reinterpret_cast<T*>(xd);
-    const int w = B.w;
-    // (the dot products over the rows below are done: k_ltdots)
-    for (int idx = threadIdx.x; idx < w * R; idx += TB) ys[idx] = x[B.c0 * R + idx];
-    __syncthreads();
-    // the triangle: thread c owns column c, chunks of 16 last to first, within a chunk the last row first;
-    // x_c = ys_c - sum_{k > c} L(k, c) x_k with L(k, c) = V[c * nrows + k]
-    constexpr int RC = RM<T>::v, CH = 16;
-    const int t = threadIdx.x;
-    const bool own = t < w;
-    T y[RC];
-#pragma unroll
-    for (int r = 0; r < RC; r++) y[r] = (own && r < R) ? ys[t * R + r] : Sc<T>::zero();
-    auto fetch = [&](int ch, T* lv) {
-#pragma unroll
-        for (int u = 0; u < CH; u++) {
-            const int k = ch * CH + u;
-            lv[u] = (own && k > t && k < w) ? cj(V[(int64_t)t * B.nrows + k], conj) : Sc<T>::zero();
-        }
-    };
-    const int nch = (w + CH - 1) / CH;
-    T lcur[CH], lnxt[CH];
-    fetch(nch - 1, lcur);
-    __syncthreads();
-    for (int ch = nch - 1; ch >= 0; ch--) {
-        const int r0 = ch * CH, r1 = min(w, r0 + CH);
-        if (ch > 0) fetch(ch - 1, lnxt);
-        if (t >= r0 && t < r1) {
-#pragma unroll
-            for (int u = CH - 1; u >= 0; u--) {
-                const int k = r0 + u;
-                if (k >= r1) continue;
-                T yk[RC];
-#pragma unroll
-                for (int r = 0; r < RC; r++) yk[r] = shfl_sc<T>(y[r], k & 63);
-                if (t < k) {
-#pragma unroll
-                    for (int r = 0; r < RC; r++)
-                        if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], yk[r]);
-                }
-            }
-#pragma unroll
-            for (int r = 0; r < RC; r++)
-                if (r < R) ys[t * R + r] = y[r];
-        }
-        __syncthreads();
-        if (own && t < r0) {
-#pragma unroll
-            for (int u = CH - 1; u >= 0; u--) {
-                const int k = r0 + u;
-                if (k >= r1) continue;
-#pragma unroll
-                for (int r = 0; r < RC; r++)
-                    if (r < R) y[r] = Sc<T>::fms(y[r], lcur[u], ys[k * R + r]);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < CH; u++) lcur[u] = lnxt[u];
-    }
-    if (!own) return;
-#pragma unroll
-    for (int r = 0; r < RC; r++)
-        if (r < R) x[(B.c0 + t) * R + r] = y[r];
 }
 
 // w[k * R + r] = D[perm[k]] * b[perm[k] + r * ldb]
@@ -547,8 +391,8 @@ void lfwd_pull(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks
 {
     if (ntile <= 0) return;
     ProfScope ps("lfwd_pull", bytes, s);
-    if (vs == 1) hipLaunchKernelGGL((k_tri_prod<double, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, x, y, R);
-    else hipLaunchKernelGGL((k_tri_prod<Zd, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, x, y, R);
+    if (vs == 1) hipLaunchKernelGGL((k_tri_prod<double, false, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, x, y, R, 0);
+    else hipLaunchKernelGGL((k_tri_prod<Zd, false, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, x, y, R, 0);
 }
 void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* slot_off, const double* y,
                double* contrib, int32_t R, double bytes, hipStream_t s)
@@ -579,40 +423,40 @@ void ub_prod(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, 
 {
     if (ntile <= 0) return;
     ProfScope ps("ub_prod", bytes, s);
-    if (vs == 1) hipLaunchKernelGGL((k_tri_prod<double, true>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, y, x, R);
-    else hipLaunchKernelGGL((k_tri_prod<Zd, true>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, y, x, R);
+    if (vs == 1) hipLaunchKernelGGL((k_tri_prod<double, true, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, y, x, R, 0);
+    else hipLaunchKernelGGL((k_tri_prod<Zd, true, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, y, x, R, 0);
 }
 void utdots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
-            const double* Ux, double* x, int32_t R, int conj, double bytes, hipStream_t s)
+            const double* Ux, const double* y, double* x, int32_t R, int conj, double bytes, hipStream_t s)
 {
     if (ntile <= 0) return;
     ProfScope ps("utdots", bytes, s);
-    if (vs == 1) hipLaunchKernelGGL(k_utdots<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, Up, Ui, Ux, x, R, conj);
-    else hipLaunchKernelGGL(k_utdots<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, Up, Ui, Ux, x, R, conj);
+    if (vs == 1) hipLaunchKernelGGL(k_utdots<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, Up, Ui, Ux, y, x, R, conj);
+    else hipLaunchKernelGGL(k_utdots<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, Up, Ui, Ux, y, x, R, conj);
 }
-void ltdots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int32_t* pinv_rows, double* x, int32_t R,
-            int conj, double bytes, hipStream_t s)
+void ltdots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int32_t* pinv_rows, const double* x,
+            double* y, int32_t R, int conj, double bytes, hipStream_t s)
 {
     if (ntile <= 0) return;
     ProfScope ps("ltdots", bytes, s);
-    if (vs == 1) hipLaunchKernelGGL(k_ltdots<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, pinv_rows, x, R, conj);
-    else hipLaunchKernelGGL(k_ltdots<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, pinv_rows, x, R, conj);
+    if (vs == 1) hipLaunchKernelGGL(k_ltdots<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, pinv_rows, x, y, R, conj);
+    else hipLaunchKernelGGL(k_ltdots<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, pinv_rows, x, y, R, conj);
 }
-void utfwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
-           const double* Ux, double* x, int32_t R, int conj, double bytes, hipStream_t s)
+void ut_prod(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const double* x, double* y, int32_t R, int conj,
+             double bytes, hipStream_t s)
 {
-    if (nblk <= 0) return;
-    ProfScope ps("utfwd", bytes, s);
-    if (vs == 1) hipLaunchKernelGGL(k_utfwd<double>, dim3(nblk), dim3(TB), 0, s, blk, blocks, Up, Ui, Ux, x, R, conj);
-    else hipLaunchKernelGGL(k_utfwd<Zd>, dim3(nblk), dim3(TB), 0, s, blk, blocks, Up, Ui, Ux, x, R, conj);
+    if (ntile <= 0) return;
+    ProfScope ps("ut_prod", bytes, s);
+    if (vs == 1) hipLaunchKernelGGL((k_tri_prod<double, false, true>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, x, y, R, conj);
+    else hipLaunchKernelGGL((k_tri_prod<Zd, false, true>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, x, y, R, conj);
 }
-void ltbwd(int vs, int32_t nblk, const int32_t* blk, const BlockDesc* blocks, const int32_t* pinv_rows, double* x, int32_t R,
-           int conj, double bytes, hipStream_t s)
+void lt_prod(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const double* y, double* x, int32_t R, int conj,
+             double bytes, hipStream_t s)
 {
-    if (nblk <= 0) return;
-    ProfScope ps("ltbwd", bytes, s);
-    if (vs == 1) hipLaunchKernelGGL(k_ltbwd<double>, dim3(nblk), dim3(TB), 0, s, blk, blocks, pinv_rows, x, R, conj);
-    else hipLaunchKernelGGL(k_ltbwd<Zd>, dim3(nblk), dim3(TB), 0, s, blk, blocks, pinv_rows, x, R, conj);
+    if (ntile <= 0) return;
+    ProfScope ps("lt_prod", bytes, s);
+    if (vs == 1) hipLaunchKernelGGL((k_tri_prod<double, true, true>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, y, x, R, conj);
+    else hipLaunchKernelGGL((k_tri_prod<Zd, true, true>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, y, x, R, conj);
 }
 void gather_scale(int vs, int64_t n, int32_t R, const int64_t* perm, const double* D, const double* b, int64_t ldb, double* w,
                   hipStream_t s)
