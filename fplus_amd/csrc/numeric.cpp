@@ -514,13 +514,13 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         num->d_upt.upload(upt.data(), upt.size(), s);
         dev_sync(s);
         const size_t nut = num->lev_utf.size() - 1, nlt = num->lev_ltb.size() - 1;
-        {   // the transposed sweeps' dot-product tiles (4 columns each) per level
+        {   // the transposed sweeps' dot-product tiles (one column each) per level
             auto tiles_of = [&](const std::vector<int32_t>& lev, const std::vector<int32_t>& blk, std::vector<int32_t>& lt, DevBuf<int2>& d) {
                 std::vector<int2> tt;
                 lt.assign(lev.size(), 0);
                 for (size_t L = 0; L + 1 < lev.size(); L++) {
                     for (int32_t k2 = lev[L]; k2 < lev[L + 1]; k2++)
-                        for (int32_t t0 = 0; t0 < F.blocks[blk[k2]].w; t0 += 4) tt.push_back(make_int2(blk[k2], t0));
+                        for (int32_t t0 = 0; t0 < F.blocks[blk[k2]].w; t0++) tt.push_back(make_int2(blk[k2], t0));
                     lt[L + 1] = (int32_t)tt.size();
                 }
                 d.alloc(tt.size() > 0 ? tt.size() : 1);

@@ -32,7 +32,7 @@ void ub_red(int vs, int32_t ntile, const int4* tiles, const BlockDesc* blocks, c
             hipStream_t s);
 void ub_prod(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const double* y, double* x, int32_t R,
              double bytes, hipStream_t s);
-// the transposed sweeps per level: the dot products (tiles: block, first of 4 columns) with the solved rows of other
+// the transposed sweeps per level: the dot products (tiles: block, column) with the solved rows of other
 // blocks (U^T: y of earlier blocks, subtracted in x; L^T: x of later blocks, subtracted in y), then the blocks' triangles
 // as products with the kept inverses, transposed and conjugated with conj (tiles: block, first of 64 rows):
 // ut_prod y = U11^-T x, lt_prod x = L11^-T y on the blocks' rows

@@ -269,70 +269,96 @@ __global__ __launch_bounds__(TB) void k_ub_red(const int4* __restrict__ tiles, c
     *xp = Sc<T>::sub(*xp, Sc<T>::add(s0, s1));
 }
 
-// U^T y = b, part 1: the dot products of a level's block columns over the rows before their block, tiles of 4 columns
-// (one wavefront per column): x_c -= sum_{i < c0} conj?(U(i, c)) y_i, y the solved rows of earlier blocks
+// the 256 threads' sums for all right-hand sides added in a fixed order (lanes by a shuffle tree, then the four
+// wavefronts in order through LDS); thread 0 ends with the total
 template <class T>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_utdots(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
+__device__ inline void block_sum(T (&acc)[RM<T>::v], T* red)
+{
+    constexpr int RC = RM<T>::v;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int r = 0; r < RC; r++) acc[r] = wave_sum(acc[r]);
+    if (lane == 0 && wid > 0) {
+#pragma unroll
+        for (int r = 0; r < RC; r++) red[(wid - 1) * RC + r] = acc[r];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < TB / 64 - 1; q++) {
+#pragma unroll
+            for (int r = 0; r < RC; r++) acc[r] = Sc<T>::add(acc[r], red[q * RC + r]);
+        }
+    }
+}
+
+// U^T y = b, part 1: the dot products of a level's block columns over the rows before their block, one workgroup per
+// column (tiles: block, column): x_c -= sum_{i < c0} conj?(U(i, c)) y_i, y the solved rows of earlier blocks
+template <class T>
+__global__ __launch_bounds__(TB) void k_utdots(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
                                                 const int64_t* __restrict__ Up, const int32_t* __restrict__ Ui,
                                                 const double* __restrict__ Uxd, const double* __restrict__ yd,
                                                 double* __restrict__ xd, int32_t R, int conj)
 {
+    constexpr int RC = RM<T>::v;
+    __shared__ T red[(TB / 64 - 1) * RC];
     const int2 tl = tiles[blockIdx.x];
     const BlockDesc B = blocks[tl.x];
     const T* Ux = reinterpret_cast<const T*>(Uxd);
     const T* y = reinterpret_cast<const T*>(yd);
     T* x = reinterpret_cast<T*>(xd);
-    const int lane = threadIdx.x & 63, c = tl.y + (threadIdx.x >> 6);
-    if (c >= B.w) return;
+    const int c = tl.y;
     const int64_t c0 = B.c0, q0 = Up[c0 + c], qd = Up[c0 + c + 1] - 1;
-    T acc[RM<T>::v];
+    T acc[RC];
 #pragma unroll
-    for (int r = 0; r < RM<T>::v; r++) acc[r] = Sc<T>::zero();
-    for (int64_t q = q0 + lane; q < qd; q += 64) {
+    for (int r = 0; r < RC; r++) acc[r] = Sc<T>::zero();
+#pragma unroll 2
+    for (int64_t q = q0 + threadIdx.x; q < qd; q += TB) {
         const int64_t i = Ui[q];
         if (i >= c0) continue;
         const T u = cj(Ux[q], conj);
 #pragma unroll
-        for (int r = 0; r < RM<T>::v; r++)
+        for (int r = 0; r < RC; r++)
             if (r < R) acc[r] = Sc<T>::fma_(acc[r], u, y[i * R + r]);
     }
+    block_sum<T>(acc, red);
+    if (threadIdx.x == 0) {
 #pragma unroll
-    for (int r = 0; r < RM<T>::v; r++) {
-        if (r >= R) break;
-        const T sm = wave_sum(acc[r]);
-        if (lane == 0) x[(c0 + c) * R + r] = Sc<T>::sub(x[(c0 + c) * R + r], sm);
+        for (int r = 0; r < RC; r++)
+            if (r < R) x[(c0 + c) * R + r] = Sc<T>::sub(x[(c0 + c) * R + r], acc[r]);
     }
 }
 
-// L^T x = y, part 1: the dot products of a level's block columns over the block's rows below, tiles of 4 columns:
-// y_c -= sum_{rows i below} conj?(L(i, c)) x_pinv(i), x the solved rows of later blocks
+// L^T x = y, part 1: the dot products of a level's block columns over the block's rows below, one workgroup per
+// column: y_c -= sum_{rows i below} conj?(L(i, c)) x_pinv(i), x the solved rows of later blocks
 template <class T>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_ltdots(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
+__global__ __launch_bounds__(TB) void k_ltdots(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
                                                 const int32_t* __restrict__ pinv_rows, const double* __restrict__ xd,
                                                 double* __restrict__ yd, int32_t R, int conj)
 {
+    constexpr int RC = RM<T>::v;
+    __shared__ T red[(TB / 64 - 1) * RC];
     const int2 tl = tiles[blockIdx.x];
     const BlockDesc B = blocks[tl.x];
     const T* V = reinterpret_cast<const T*>(B.vals);
     const T* x = reinterpret_cast<const T*>(xd);
     T* y = reinterpret_cast<T*>(yd);
-    const int lane = threadIdx.x & 63, c = tl.y + (threadIdx.x >> 6), w = B.w;
-    if (c >= w) return;
-    T acc[RM<T>::v];
+    const int c = tl.y, w = B.w;
+    T acc[RC];
 #pragma unroll
-    for (int r = 0; r < RM<T>::v; r++) acc[r] = Sc<T>::zero();
-    for (int32_t i = w + lane; i < B.nrows; i += 64) {
+    for (int r = 0; r < RC; r++) acc[r] = Sc<T>::zero();
+#pragma unroll 2
+    for (int32_t i = w + threadIdx.x; i < B.nrows; i += TB) {
         const T v = cj(V[(int64_t)c * B.nrows + i], conj);
         const int64_t pi = pinv_rows[B.rows[i]];
 #pragma unroll
-        for (int r = 0; r < RM<T>::v; r++)
+        for (int r = 0; r < RC; r++)
             if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, x[pi * R + r]);
     }
+    block_sum<T>(acc, red);
+    if (threadIdx.x == 0) {
 #pragma unroll
-    for (int r = 0; r < RM<T>::v; r++) {
-        if (r >= R) break;
-        const T sm = wave_sum(acc[r]);
-        if (lane == 0) y[(B.c0 + c) * R + r] = Sc<T>::sub(y[(B.c0 + c) * R + r], sm);
+        for (int r = 0; r < RC; r++)
+            if (r < R) y[(B.c0 + c) * R + r] = Sc<T>::sub(y[(B.c0 + c) * R + r], acc[r]);
     }
 }
 
