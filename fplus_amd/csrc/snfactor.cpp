@@ -609,13 +609,15 @@ int Driver::panel(int64_t j0, int64_t j1)
     std::vector<double> lev_flops(maxlevel + 2, 0.0);   // the TRSM's and the GEMM's algorithmic flops
     std::vector<double> lev_issued(maxlevel + 2, 0.0);
     std::vector<int32_t> lev_maxm(maxlevel + 2, 0);
-    int64_t ntile = 0;   // the tiles are written straight into the upload staging
-    for (const TaskL& x : tl) {
-        const int64_t rt = lev_rt[x.level];
-        ntile += (int64_t)((F_.blocks[x.t.block].nrows - x.t.k0 + rt - 1) / rt) * ((x.t.ncol + CTG - 1) / CTG);
+    // the tiles are written straight into the upload staging, each task's from its offset (in parallel, below)
+    std::vector<int64_t> toff(tl.size() + 1, 0);
+    for (size_t i = 0; i < tl.size(); i++) {
+        const int64_t rt = lev_rt[tl[i].level];
+        toff[i + 1] = toff[i] + (int64_t)((F_.blocks[tl[i].t.block].nrows - tl[i].t.k0 + rt - 1) / rt) *
+                                    ((tl[i].t.ncol + CTG - 1) / CTG);
     }
+    const int64_t ntile = toff[tl.size()];
     UpdTile* gtiles = up_.take<UpdTile>((size_t)ntile);
-    int64_t gi = 0;
     int64_t xtot = 0;
     for (size_t i = 0; i < tl.size(); i++) {
         tasks[i] = tl[i].t;
@@ -644,18 +646,29 @@ int Driver::panel(int64_t j0, int64_t j1)
             const int bkt = T.ncol <= 8 ? 0 : T.ncol <= 16 ? 1 : T.ncol <= 32 ? 2 : T.ncol <= 64 ? 3 : 4;
             F_.upd_ncol_flops[bkt] += prod;
         }
-        const int32_t rt = lev_rt[L];
-        for (int32_t r0 = T.k0; r0 < B.nrows; r0 += rt)
-            for (int32_t cc = 0; cc < T.ncol; cc += CTG) {
-                UpdTile& u = gtiles[gi++];
-                u.iv = B.ivals;
-                u.rows = B.rows;
-                u.x = d_X_.p + (T.xoff + (int64_t)cc * m) * vs_;
-                u.nrows = B.nrows; u.w = B.w; u.k0 = T.k0; u.row0 = r0;
-                u.cfirst = T.col_off + cc; u.nc = std::min<int32_t>(CTG, T.ncol - cc);
-                u.spos0 = T.spos0; u.shared = lev_ntask[L] > 1;
+        lev_gt[L + 1] = (int32_t)toff[i + 1];
+    }
+    {
+        const int64_t nt = (int64_t)tl.size(), TPI = 16;
+        pool_.run((nt + TPI - 1) / TPI, [&](int64_t c, int) {
+            for (int64_t i = c * TPI; i < std::min(nt, (c + 1) * TPI); i++) {
+                const UpdTask& T = tasks[i];
+                const int32_t L = tl[i].level, rt = lev_rt[L];
+                const BlockDesc& B = F_.blocks[T.block];
+                const int64_t m = B.w - T.k0;
+                int64_t gi = toff[i];
+                for (int32_t r0 = T.k0; r0 < B.nrows; r0 += rt)
+                    for (int32_t cc = 0; cc < T.ncol; cc += CTG) {
+                        UpdTile& u = gtiles[gi++];
+                        u.iv = B.ivals;
+                        u.rows = B.rows;
+                        u.x = d_X_.p + (T.xoff + (int64_t)cc * m) * vs_;
+                        u.nrows = B.nrows; u.w = B.w; u.k0 = T.k0; u.row0 = r0;
+                        u.cfirst = T.col_off + cc; u.nc = std::min<int32_t>(CTG, T.ncol - cc);
+                        u.spos0 = T.spos0; u.shared = lev_ntask[L] > 1;
+                    }
             }
-        lev_gt[L + 1] = (int32_t)gi;
+        });
     }
     for (int32_t L = 1; L <= maxlevel + 1; L++) lev_gt[L] = std::max(lev_gt[L], lev_gt[L - 1]);
     for (int32_t L = 1; L <= maxlevel; L++) {   // levels by their tile count (powers of two), for the statistics
