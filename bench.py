@@ -165,7 +165,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the path would shard by "
-              "independent systems or elimination subtrees: see DESIGN.md)", file=sys.stderr)
+              "independent systems, or by elimination subtrees with separator updates exchanged between GPUs)", file=sys.stderr)
         sys.exit(2)
     if args.copy_probe:
         copy_probe()
