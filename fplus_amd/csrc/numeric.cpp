@@ -777,16 +777,18 @@ int factor_core(fplus_numeric* num, const PatternCache& c, DevBuf<double>& d_Bx,
     num->t_ruiz = t1 - t0;
     num->t_factor = now_s() - t1;
     if (rc == FPLUS_OK && verbose >= 1)
-        printf("[FPLUS] LU factorisation complete: nnz(U)=%ld, supernodes=%ld, panels=%ld, levels=%ld, time=%.3fs (host %.3fs, wait %.3fs); "
-               "workspace rows per panel: %.0f pivotal, %.0f not (max %ld)\n",
+        printf("[FPLUS] LU factorisation complete: nnz(U)=%ld, supernodes=%ld, panels=%ld, levels=%ld, time=%.3fs (host %.3fs, wait %.3fs, "
+               "of which %.3fs within the panel LU); workspace rows per panel: %.0f pivotal, %.0f not (max %ld)\n",
                (long)num->F.nnzU, (long)num->F.nsn, (long)num->F.panels, (long)num->F.levels, num->t_factor, num->F.t_host, num->F.t_wait,
+               num->F.t_wait_lu,
                (double)num->F.rows_upper / std::max<int64_t>(1, num->F.panels), (double)num->F.rows_lower / std::max<int64_t>(1, num->F.panels),
                (long)num->F.rows_lower_max);
     if (rc == FPLUS_OK && verbose >= 1)
         printf("[FPLUS]   host phases: search %.3fs, tasks/levels %.3fs (union order and workspace rows %.3fs, of which rows not "
-               "pivotal %.3fs; tasks and levels %.3fs, of which the level pass %.3fs), symbolic with pivots %.3fs (search %.3fs, "
-               "U %.3fs, join %.3fs, prune %.3fs), out %.3fs\n", num->F.t_p1, num->F.t_tasks, num->F.t_ws, num->F.t_ws_low,
-               num->F.t_lev, num->F.t_lev_pass, num->F.t_p3, num->F.t_p3_dfs, num->F.t_p3_u, num->F.t_p3_join,
+               "pivotal %.3fs; tasks and levels %.3fs, of which the level pass %.3fs), symbolic with pivots %.3fs (search %.3fs "
+               "[candidate pass %.3fs], U %.3fs, join %.3fs, prune %.3fs), out %.3fs\n", num->F.t_p1, num->F.t_tasks, num->F.t_ws,
+               num->F.t_ws_low, num->F.t_lev, num->F.t_lev_pass, num->F.t_p3, num->F.t_p3_dfs, num->F.t_p3_split, num->F.t_p3_u,
+               num->F.t_p3_join,
                num->F.t_p3_prune, num->F.t_out);
     if (rc == FPLUS_OK && verbose >= 1) {
         const double* h = num->F.upd_ncol_flops;

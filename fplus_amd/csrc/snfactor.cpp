@@ -160,9 +160,11 @@ private:
             cand.assign(nn, 0); stack.assign(nn, 0); pos.assign(nn, 0);
         }
     };
-    // the search from supernode `root` entered at column `col`, its rows read from position pos0 of its search list
-    int64_t dfs(SWork& w, int64_t root, int64_t col, int64_t top, int64_t stamp, int64_t open, int64_t* ncand,
-                int64_t pos0 = 0) const
+    // the search from supernode `root` entered at column `col`, its rows read from position pos0 of its search list;
+    // a row not pivotal is a new candidate unless marked or in_list(row) (the column's candidates known before)
+    template <class InList>
+    int64_t dfs_in(SWork& w, int64_t root, int64_t col, int64_t top, int64_t stamp, int64_t open, int64_t* ncand,
+                   int64_t pos0, const InList& in_list) const
     {
         int64_t sp = 0;
         w.stack[0] = root; w.smark[root] = stamp; w.seg0[root] = col; w.pos[root] = pos0;
@@ -174,7 +176,7 @@ private:
             for (; q < end; q++) {
                 const int64_t i = rows[q], k = pinv_[i];
                 if (k < 0) {
-                    if (w.rmark[i] != stamp) { w.rmark[i] = stamp; w.cand[(*ncand)++] = i; }
+                    if (w.rmark[i] != stamp && !in_list(i)) { w.rmark[i] = stamp; w.cand[(*ncand)++] = i; }
                     continue;
                 }
                 const int64_t u = col_sn_[k];
@@ -187,9 +189,9 @@ private:
         }
         return top;
     }
-    int64_t dfs(int64_t root, int64_t col, int64_t top, int64_t stamp, int64_t open, int64_t* ncand, int64_t pos0 = 0)
+    int64_t dfs(SWork& w, int64_t root, int64_t col, int64_t top, int64_t stamp, int64_t open, int64_t* ncand) const
     {
-        return dfs(ws0_, root, col, top, stamp, open, ncand, pos0);
+        return dfs_in(w, root, col, top, stamp, open, ncand, 0, [](int64_t) { return false; });
     }
     bool relax_ok(const HSn& S, int64_t n_miss, int64_t extra) const
     {
@@ -262,10 +264,15 @@ private:
 
     std::vector<HSn> sn_;
     std::vector<int64_t> col_sn_, pinv_, p_, bpos_, miss_;
-    std::vector<int64_t> npl_l_, piv_c_;   // phase 3: a column's candidates not pivotal / pivotal in the panel
+    std::vector<int64_t> piv_c_;   // phase 3: a column's candidates pivotal in the panel
     std::vector<std::vector<int64_t>> c_s_, c_kf_, c_np_;   // phase 1's results per panel column
     std::vector<std::vector<int32_t>> ch_rows_;   // per chunk of columns: the rows not yet pivotal, first appearance
     std::vector<int32_t> reader_;   // per workspace U row: the update task whose segment holds it
+    // phase 3's view of the columns' candidates: their lower workspace positions (as at the panel's start), the same as
+    // a bitset per column, and per lower position the panel column that pivoted it
+    std::vector<std::vector<int32_t>> c_lp_;
+    std::vector<uint64_t> cbits_;
+    std::vector<int32_t> pivcol_;
     std::vector<std::vector<int32_t>> succ_;   // per update task: the later tasks whose segments its rows below reach
     std::vector<std::vector<int64_t>> tmark_;   // one mark array per thread of the pool, indexed by task
     SWork ws0_;                     // the sequential phases' marks
@@ -727,6 +734,32 @@ int Driver::panel(int64_t j0, int64_t j1)
                                  hipMemcpyDeviceToHost, s_), "panel status and pivots D2H");
         hip_check(hipEventRecord(sp_ev_.ev[sp], s_), "hipEventRecord");
     }
+    // while the device runs the updates: the columns' candidates as lower positions and bitsets (in parallel)
+    const int64_t nw = (Rl + 63) / 64;
+    {
+        if ((int64_t)c_lp_.size() < np) c_lp_.resize(np);
+        cbits_.resize((size_t)std::max<int64_t>(np * nw, 1));
+        std::atomic<int> badlp{0};
+        pool_.run(np, [&](int64_t ci, int) {
+            std::vector<int32_t>& lp = c_lp_[ci];
+            const std::vector<int64_t>& cn = c_np[ci];
+            lp.resize(cn.size());
+            uint64_t* cb = cbits_.data() + ci * nw;
+            std::fill(cb, cb + nw, (uint64_t)0);
+            for (size_t k = 0; k < cn.size(); k++) {
+                const int64_t h = (int64_t)hrowpos_[cn[k]] - Ru;
+                if (h < 0 || h >= Rl) { badlp.store(1, std::memory_order_relaxed); lp[k] = 0; continue; }
+                lp[k] = (int32_t)h;
+                cb[h >> 6] |= (uint64_t)1 << (h & 63);
+            }
+        });
+        if (badlp.load()) {
+            dev_sync(s_);
+            errf(err_, errlen_, "internal: a candidate outside the panel's lower rows");
+            return FPLUS_ERR_INTERNAL;
+        }
+        pivcol_.assign((size_t)std::max<int64_t>(Rl, 1), INT32_MAX);
+    }
     const double th1 = now_s();
     double waited = 0;
     // the rows' order as the row swaps leave it: after column ci's swap, position ci holds its pivot row for good
@@ -741,6 +774,7 @@ int Driver::panel(int64_t j0, int64_t j1)
             const double tw0 = now_s();
             hip_check(hipEventSynchronize(sp_ev_.ev[sp]), "hipEventSynchronize");
             waited += now_s() - tw0;
+            if (sp > 0) F_.t_wait_lu += now_s() - tw0;
             const PanelStatus st = h_pout_.p[sp].st;
             if (st.err < 0) {
                 dev_sync(s_);
@@ -763,17 +797,33 @@ int Driver::panel(int64_t j0, int64_t j1)
         std::swap(lrows[ci], lrows[pv]);
         const int64_t j = j0 + ci, stamp2 = n + 1 + j;
         const double tq0 = now_s();
-        // one pass over the column's candidates from the search before the panel: mark them, keep the ones still not
-        // pivotal (in order: the candidate order is the pivot tie-break order) and the ones pivotal in this panel (the
-        // within-panel search starts from them, in order); the search's new candidates follow, in its order
+        // one pass over the column's candidates from the search before the panel: the ones pivotal in this panel (the
+        // within-panel search starts from them, in order) apart, and for the ones still not pivotal -- in order: the
+        // candidate order is the pivot tie-break order -- the count of the open supernode's rows among them and the
+        // others in order (the join's test, below); the search's new candidates follow, in its order (a row is a new one
+        // if it is not in the column's bitset)
         int64_t ncand = 0, top = n;
-        npl_l_.clear();
         piv_c_.clear();
-        for (const int64_t i : c_np[ci]) {
-            ws0_.rmark[i] = stamp2;
-            if (pinv_[i] < 0) npl_l_.push_back(i);
-            else piv_c_.push_back(i);
+        std::vector<int64_t>& npv = miss_;   // reused: the candidates missing from the open supernode, in order
+        int64_t in_S = 0, n_miss = 0;
+        const int64_t Sw = open_ >= 0 ? sn_[open_].w : INT64_MAX;
+        {
+            const int32_t* lp = c_lp_[ci].data();
+            const int64_t* cr = c_np[ci].data();
+            const size_t C = c_np[ci].size();
+            for (size_t k = 0; k < C; k++) {
+                const int64_t r = cr[k];
+                if (pivcol_[lp[k]] < (int32_t)ci) piv_c_.push_back(r);
+                else if (bpos_[r] >= Sw) in_S++;
+                else npv[n_miss++] = r;
+            }
         }
+        const uint64_t* cb = cbits_.data() + ci * nw;
+        auto in_list = [&](int64_t i) {
+            const int64_t h = (int64_t)hrowpos_[i] - Ru;
+            return h >= 0 && h < Rl && ((cb[h >> 6] >> (h & 63)) & 1);
+        };
+        F_.t_p3_split += now_s() - tq0;
         if (open0 >= 0 && sn_[open0].w > we_open && ws0_.smark[open0] != stamp2) {
             // while it stays open, the rows past its pivots that the supernode had when the panel began are all among
             // the column's candidates (its search before the panel read them) and none has become pivotal (a pivot of
@@ -781,16 +831,21 @@ int Driver::panel(int64_t j0, int64_t j1)
             // there and starts at the rows appended since
             const int64_t skip = open_ == open0 ? std::max<int64_t>(0, nr_open - sn_[open0].w) : 0;
             for (int64_t q = ext_off[ci]; q < ext_off[ci + 1]; q++)
-                if (ext_s[q] == open0) { top = dfs(open0, sn_[open0].c0 + we_open, top, stamp2, open_, &ncand, skip); break; }
+                if (ext_s[q] == open0) { top = dfs_in(ws0_, open0, sn_[open0].c0 + we_open, top, stamp2, open_, &ncand, skip, in_list); break; }
         }
         for (int64_t i : piv_c_) {
             const int64_t k = pinv_[i];
             const int64_t s = col_sn_[k];
             if (ws0_.smark[s] == stamp2) { if (k < ws0_.seg0[s]) ws0_.seg0[s] = k; continue; }
-            top = dfs(s, k, top, stamp2, open_, &ncand);
+            top = dfs_in(ws0_, s, k, top, stamp2, open_, &ncand, 0, in_list);
         }
-        npl_l_.insert(npl_l_.end(), ws0_.cand.begin(), ws0_.cand.begin() + ncand);
+        for (int64_t q = 0; q < ncand; q++) {
+            const int64_t r = ws0_.cand[q];
+            if (bpos_[r] >= Sw) in_S++;
+            else npv[n_miss++] = r;
+        }
         const int64_t pivrow = lrows[ci];
+        pivcol_[hrowpos_[pivrow] - Ru] = (int32_t)ci;
         const double tq1 = now_s();
         // U(:, j): pre-panel segments, within-panel segments, the diagonal -- each a run of consecutive rows whose values
         // are consecutive in the workspace column
@@ -817,17 +872,9 @@ int Driver::panel(int64_t j0, int64_t j1)
         Urp_[j + 1] = (int64_t)Ur_row_.size();
         const double tq2 = now_s();
         // L(:, j): join the open supernode (exactly or relaxed), else open a new one
-        std::vector<int64_t>& npv = miss_;   // reused: the missing rows
-        const std::vector<int64_t>& npl_l = npl_l_;   // the candidates still non-pivotal
         int join = 0;
-        int64_t n_miss = 0;
         if (open_ >= 0) {
             HSn& S = sn_[open_];
-            int64_t in_S = 0;
-            for (int64_t r : npl_l) {
-                if (bpos_[r] >= S.w) in_S++;
-                else npv[n_miss++] = r;
-            }
             const int64_t extra = (S.nr() - S.w) - in_S;
             if (n_miss + extra == 0) join = 1;
             else if (relax_ok(S, n_miss, extra)) join = 2;
@@ -858,13 +905,19 @@ int Driver::panel(int64_t j0, int64_t j1)
             sn_.emplace_back();
             HSn& S = sn_.back();
             S.c0 = j; S.w = 1;
-            S.ri.reserve(npl_l.size());
+            // its rows: the pivot row, then the candidates still not pivotal in order (those from before the panel,
+            // then the search's new ones)
+            S.ri.reserve(c_np[ci].size() + ncand);
             S.ri.push_back(pivrow); bpos_[pivrow] = 0;
-            for (int64_t r : npl_l) {
-                if (r == pivrow) continue;
+            auto add = [&](int64_t r) {
+                if (r == pivrow) return;
                 bpos_[r] = S.nr();
                 S.ri.push_back(r);
-            }
+            };
+            const int32_t* lp = c_lp_[ci].data();
+            for (size_t k = 0; k < c_np[ci].size(); k++)
+                if (pivcol_[lp[k]] >= (int32_t)ci) add(c_np[ci][k]);
+            for (int64_t q = 0; q < ncand; q++) add(ws0_.cand[q]);
             col_sn_[j] = s;
             open_ = s;
             touched.push_back({s, j});

@@ -39,8 +39,10 @@ struct DevFactor {
     double flops = 0, gemm_flops = 0, gemm_bytes = 0;
     int64_t panels = 0, levels = 0, tasks = 0;
     double t_host = 0, t_wait = 0;
+    double t_wait_lu = 0;   // of the wait: for the panel LU's sub-panels after the first (the rest is the updates)
     double t_p1 = 0, t_tasks = 0, t_p3 = 0, t_out = 0;   // host phases: search, tasks and levels, symbolic with the pivots, blocks and U out
     double t_p3_dfs = 0, t_p3_u = 0, t_p3_join = 0, t_p3_prune = 0;   // the symbolic with the pivots, by step
+    double t_p3_split = 0;   // of the search: the pass over the column's candidates
     double t_ws = 0, t_lev = 0;   // within the tasks and levels: the union order and workspace rows, the tasks' levels
     double t_ws_low = 0, t_lev_pass = 0;   // of which: the rows not yet pivotal; the sequential pass of the levels
     // the block updates' products: their flops, what the tiles issue on the matrix cores, the flops by task width
