@@ -263,9 +263,11 @@ private:
     size_t errlen_;
 
     std::vector<HSn> sn_;
-    std::vector<int64_t> col_sn_, pinv_, p_, bpos_, miss_;
+    std::vector<int64_t> col_sn_, pinv_, p_, miss_;
+    std::vector<int32_t> bpos_;   // a row's position in the open supernode's rows, -1 outside it
     std::vector<int64_t> piv_c_;   // phase 3: a column's candidates pivotal in the panel
-    std::vector<std::vector<int64_t>> c_s_, c_kf_, c_np_;   // phase 1's results per panel column
+    std::vector<std::vector<int64_t>> c_s_, c_kf_;   // phase 1's results per panel column
+    std::vector<std::vector<int32_t>> c_np_;         // and its candidates
     std::vector<std::vector<int32_t>> ch_rows_;   // per chunk of columns: the rows not yet pivotal, first appearance
     std::vector<int32_t> reader_;   // per workspace U row: the update task whose segment holds it
     // phase 3's view of the columns' candidates: their lower workspace positions (as at the panel's start), the same as
@@ -366,7 +368,7 @@ int Driver::panel(int64_t j0, int64_t j1)
     // columns' searches are independent (they read the structure as it stood at j0): threads take columns in turn,
     // each with its own marks, and the results are joined in column order
     std::vector<int64_t> ext_s, ext_kf, ext_off(np + 1);
-    std::vector<std::vector<int64_t>>& c_np = c_np_;   // per column: its candidates (rows not pivotal), search order
+    std::vector<std::vector<int32_t>>& c_np = c_np_;   // per column: its candidates (rows not pivotal), search order
     if ((int64_t)c_np.size() < np) c_np.resize(np);
     {
         std::vector<std::vector<int64_t>>& c_s = c_s_;
@@ -388,7 +390,8 @@ int Driver::panel(int64_t j0, int64_t j1)
             c_s[ci].resize(n - top);
             c_kf[ci].resize(n - top);
             for (int64_t t = top; t < n; t++) { c_s[ci][t - top] = w.topo[t]; c_kf[ci][t - top] = w.seg0[w.topo[t]]; }
-            c_np[ci].assign(w.cand.begin(), w.cand.begin() + ncand);
+            c_np[ci].resize(ncand);
+            for (int64_t q = 0; q < ncand; q++) c_np[ci][q] = (int32_t)w.cand[q];
         };
         pool_.run(np, [&](int64_t ci, int tid) { search(ci, wsT_[tid]); });
         for (int64_t ci = 0; ci < np; ci++) {
@@ -454,8 +457,8 @@ int Driver::panel(int64_t j0, int64_t j1)
             std::vector<int32_t>& out = ch_rows_[c];
             out.clear();
             for (int64_t ci = c * per; ci < std::min(np, (c + 1) * per); ci++)
-                for (const int64_t row : c_np[ci])
-                    if (hrowpos_[row] < 0 && w.rmark[row] != stamp) { w.rmark[row] = stamp; out.push_back((int32_t)row); }
+                for (const int32_t row : c_np[ci])
+                    if (hrowpos_[row] < 0 && w.rmark[row] != stamp) { w.rmark[row] = stamp; out.push_back(row); }
         });
         // the join: the first chunk's rows (most of them) placed, the other chunks' lists filtered against them in
         // parallel, then the few rows left joined in chunk order
@@ -742,7 +745,7 @@ int Driver::panel(int64_t j0, int64_t j1)
         std::atomic<int> badlp{0};
         pool_.run(np, [&](int64_t ci, int) {
             std::vector<int32_t>& lp = c_lp_[ci];
-            const std::vector<int64_t>& cn = c_np[ci];
+            const std::vector<int32_t>& cn = c_np[ci];
             lp.resize(cn.size());
             uint64_t* cb = cbits_.data() + ci * nw;
             std::fill(cb, cb + nw, (uint64_t)0);
@@ -809,7 +812,7 @@ int Driver::panel(int64_t j0, int64_t j1)
         const int64_t Sw = open_ >= 0 ? sn_[open_].w : INT64_MAX;
         {
             const int32_t* lp = c_lp_[ci].data();
-            const int64_t* cr = c_np[ci].data();
+            const int32_t* cr = c_np[ci].data();
             const size_t C = c_np[ci].size();
             for (size_t k = 0; k < C; k++) {
                 const int64_t r = cr[k];
@@ -880,7 +883,7 @@ int Driver::panel(int64_t j0, int64_t j1)
             else if (relax_ok(S, n_miss, extra)) join = 2;
             if (join == 2) {
                 S.nzero += (double)n_miss * (double)S.w + (double)extra;
-                for (int64_t t = 0; t < n_miss; t++) { bpos_[npv[t]] = S.nr(); S.ri.push_back(npv[t]); }
+                for (int64_t t = 0; t < n_miss; t++) { bpos_[npv[t]] = (int32_t)S.nr(); S.ri.push_back(npv[t]); }
             }
         }
         if (join) {
@@ -889,7 +892,7 @@ int Driver::panel(int64_t j0, int64_t j1)
             if (pp != w) {
                 const int64_t other = S.ri[w];
                 S.ri[w] = pivrow; S.ri[pp] = other;
-                bpos_[pivrow] = w; bpos_[other] = pp;
+                bpos_[pivrow] = (int32_t)w; bpos_[other] = (int32_t)pp;
             }
             S.w = w + 1;
             col_sn_[j] = open_;
@@ -911,7 +914,7 @@ int Driver::panel(int64_t j0, int64_t j1)
             S.ri.push_back(pivrow); bpos_[pivrow] = 0;
             auto add = [&](int64_t r) {
                 if (r == pivrow) return;
-                bpos_[r] = S.nr();
+                bpos_[r] = (int32_t)S.nr();
                 S.ri.push_back(r);
             };
             const int32_t* lp = c_lp_[ci].data();
