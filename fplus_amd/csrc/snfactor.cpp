@@ -403,12 +403,6 @@ int Driver::panel(int64_t j0, int64_t j1)
     ext_off[np] = (int64_t)ext_s.size();
     const double th_p1 = now_s();
     F_.t_p1 += th_p1 - th0;
-    // the previous panel's output ran meanwhile: the structure the next steps read is complete once it is done
-    {
-        const int orc = join_out();
-        if (orc != FPLUS_OK) return orc;
-    }
-    up_.begin();
     // ---- the union of the reached supernodes, in a topological order of the updates (the reference takes one from a
     // search over the union, src/factor_sn_panel.inc:61-70): increasing supernode index is one -- a block's rows below
     // its pivot rows were not pivotal when its supernode opened, so the supernodes whose pivot rows they become opened
@@ -492,6 +486,13 @@ int Driver::panel(int64_t j0, int64_t j1)
     F_.rows_upper += Ru;
     F_.rows_lower += Rl;
     F_.rows_lower_max = std::max(F_.rows_lower_max, Rl);
+    // the previous panel's output ran meanwhile (it makes the blocks the tasks read, and uses the other staging area):
+    // done by now in general
+    {
+        const int orc = join_out();
+        if (orc != FPLUS_OK) return orc;
+    }
+    up_.begin();
     // ---- update tasks per union supernode and block, with dependency levels: a block's task solves the block's pivot
     // rows the panel's columns reach (read: they wait for every task writing them) and writes the rows below
     const int RT = FPA_ROW_TILE, CTG = FPA_COL_TILE;
@@ -946,6 +947,7 @@ int Driver::panel(int64_t j0, int64_t j1)
     out_.ldw = ldw;
     out_.u_begin = u_begin;
     out_.Wl = Wl;
+    for (int32_t row : univ) hrowpos_[row] = -1;   // (the next panel's workspace rows are placed meanwhile)
     out_.univ = std::move(univ);
     out_.lrows = std::move(lrows);
     out_.touched = std::move(touched);
@@ -1027,7 +1029,6 @@ int Driver::panel_out()
     k::rowmap_set((int64_t)univ.size(), d_univ_.p, d_rowmap_.p, 1, s_);
     // the device finishes the panel while the host starts the next one (the uploads read page-locked staging)
     up_.end(s_);
-    for (int32_t row : univ) hrowpos_[row] = -1;
     for (int32_t row : lrows) lposmap_[row] = -1;
     F_.t_out += now_s() - th_out;
     F_.t_host += now_s() - th_out;
