@@ -106,8 +106,11 @@ void x_back(int vs, int32_t ntask, const UpdTask* tasks, const int32_t* cols, co
 // dense partial-pivot LU of the panel's lower section (nr x P, column-major, ld), sub-panel [a, a + nb), and the
 // sub-panel's pivot rows in the columns right of it solved with its unit lower triangle: a cooperative launch over the rows (coop: getf2_coop_bytes(vs) of scratch; counter: one int, zero at first and then only touched
 // here, with *counted the arrivals it holds), one workgroup for very tall panels
+// hout: the sub-panel's slot in page-locked host memory (a device address of it), where the kernel writes the pivots
+// of the sub-panel's columns and the status (err 0 once the sub-panel is factored): no copy back
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
-           double pivot_rel, int32_t* piv, PanelStatus* st, void* coop, int32_t* counter, int64_t* counted, hipStream_t s);
+           double pivot_rel, int32_t* piv, PanelStatus* st, PanelOut* hout, void* coop, int32_t* counter, int64_t* counted,
+           hipStream_t s);
 size_t getf2_coop_bytes(int vs);
 int getf2_coop_max_wg();
 // (getf2's own second step after its one-workgroup form)

@@ -434,7 +434,8 @@ constexpr int LU_TB = 1024;
 template <class T>
 __global__ __launch_bounds__(LU_TB) void k_getf2(double* __restrict__ Wd, int64_t ld, int32_t nr, int32_t P, int32_t a,
                                                   int32_t nb, const double* __restrict__ colmax, double pivot_rel,
-                                                  int32_t* __restrict__ piv, PanelStatus* __restrict__ st)
+                                                  int32_t* __restrict__ piv, PanelStatus* __restrict__ st,
+                                                  PanelOut* __restrict__ hout)
 {
     __shared__ double bestv[LU_TB / 64];
     __shared__ int32_t bestr[LU_TB / 64];
@@ -468,10 +469,12 @@ __global__ __launch_bounds__(LU_TB) void k_getf2(double* __restrict__ Wd, int64_
                 st->err = j + 1;
                 st->best = v > 0.0 ? v : 0.0;
                 st->thr = thr;
+                hout->st = *st;
                 pr = -1;
             } else {
                 pr = r;
                 piv[j] = r;
+                hout->piv[j] = r;
             }
         }
         __syncthreads();
@@ -492,6 +495,7 @@ __global__ __launch_bounds__(LU_TB) void k_getf2(double* __restrict__ Wd, int64_
         }
         __syncthreads();
     }
+    if (threadIdx.x == 0) hout->st.err = 0;   // the sub-panel is factored: its status for the host
 }
 
 // The same sub-panel LU on many workgroups at once (a grid the device holds at once: every workgroup is resident).  Workgroup g
@@ -511,7 +515,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                                                     int32_t nb, const double* __restrict__ colmax, double pivot_rel,
                                                     int32_t* __restrict__ piv, PanelStatus* __restrict__ st,
                                                     Cand<T>* __restrict__ cand, T* __restrict__ rowj,
-                                                    int32_t* __restrict__ counter, int32_t base)
+                                                    int32_t* __restrict__ counter, int32_t base, PanelOut* __restrict__ hout)
 {
     __shared__ double rv[TB / 64];
     __shared__ int32_t rr[TB / 64];
@@ -581,7 +585,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         }
         __syncthreads();
         if (sh_abort) {
-            if (tid == 0) atomicExch(&st->err, -1);
+            if (tid == 0) {
+                atomicExch(&st->err, -1);
+                hout->st.err = -1;
+            }
             return;
         }
         if (wid == 0) {
@@ -607,12 +614,18 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 st->err = j + 1;
                 st->best = sh_val > 0.0 ? sh_val : 0.0;
                 st->thr = thr;
+                hout->st.best = st->best;
+                hout->st.thr = thr;
+                hout->st.err = j + 1;
             }
             return;
         }
         if (tid < PNB) pvals[tid] = cand[par * G + sh_gi].v[tid];
         __syncthreads();
-        if (g == 0 && tid == 0) piv[j] = pr;
+        if (g == 0 && tid == 0) {
+            piv[j] = pr;
+            hout->piv[j] = pr;
+        }
         if (pr != j) {
             if (have && r == j) {
 #pragma unroll
@@ -639,6 +652,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 if (c > jj && c < nb) x[c] = Sc<T>::fms(x[c], l, pvals[c]);
         }
     }
+    if (g == 0 && tid == 0) hout->st.err = 0;   // the sub-panel is factored: its status for the host
     if (have) {
 #pragma unroll
         for (int c = 0; c < PNB; c++)
@@ -850,7 +864,8 @@ void x_back(int vs, int32_t ntask, const UpdTask* tasks, const int32_t* cols, co
     else hipLaunchKernelGGL(k_x_back<Zd>, dim3(ntask), dim3(TB), 0, s, tasks, cols, blocks, W, ldw, xbuf);
 }
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
-           double pivot_rel, int32_t* piv, PanelStatus* st, void* coop, int32_t* counter, int64_t* counted, hipStream_t s)
+           double pivot_rel, int32_t* piv, PanelStatus* st, PanelOut* hout, void* coop, int32_t* counter, int64_t* counted,
+           hipStream_t s)
 {
     ProfScope ps("getf2", (vs == 1 ? 2.0 : 8.0) * (double)nr * nb * nb / 2, s);
     const int32_t G = (int32_t)((nr - a + TB - 1) / TB);
@@ -873,7 +888,8 @@ void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int
         }
         int32_t arrived = (int32_t)*counted;
         *counted += (int64_t)nb * G;
-        void* args[] = {&Wl, &ld, &nr, &P, &a, &nb, (void*)&colmax, &pivot_rel, &piv, &st, &coop, (void*)&coop, &counter, &arrived};
+        void* args[] = {&Wl, &ld, &nr, &P, &a, &nb, (void*)&colmax, &pivot_rel, &piv, &st, &coop, (void*)&coop, &counter, &arrived,
+                        &hout};
         // cand (2 G slots) first in coop, then rowj (2 PNB scalars)
         char* base = static_cast<char*>(coop);
         void* rowj = base + (size_t)2 * G * (vs == 1 ? sizeof(Cand<double>) : sizeof(Cand<Zd>));
@@ -887,8 +903,8 @@ void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int
         hip_check(e, "panel LU launch");
         return;
     }
-    if (vs == 1) hipLaunchKernelGGL(k_getf2<double>, dim3(1), dim3(LU_TB), 0, s, Wl, ld, nr, P, a, nb, colmax, pivot_rel, piv, st);
-    else hipLaunchKernelGGL(k_getf2<Zd>, dim3(1), dim3(LU_TB), 0, s, Wl, ld, nr, P, a, nb, colmax, pivot_rel, piv, st);
+    if (vs == 1) hipLaunchKernelGGL(k_getf2<double>, dim3(1), dim3(LU_TB), 0, s, Wl, ld, nr, P, a, nb, colmax, pivot_rel, piv, st, hout);
+    else hipLaunchKernelGGL(k_getf2<Zd>, dim3(1), dim3(LU_TB), 0, s, Wl, ld, nr, P, a, nb, colmax, pivot_rel, piv, st, hout);
     trsm_sub(vs, Wl, ld, P, a, nb, st, s);
 }
 size_t getf2_coop_bytes(int vs) { return (size_t)2 * getf2_coop_max_wg() * (vs == 1 ? sizeof(Cand<double>) : sizeof(Cand<Zd>)) + 2 * PNB * 16 + 256; }

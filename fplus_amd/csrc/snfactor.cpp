@@ -303,7 +303,8 @@ private:
     DevBuf<double> d_coop_;          // the cooperative panel LU's candidate slots
     DevBuf<int32_t> d_coop_cnt_;     // and its barrier counter
     int64_t coop_counted_ = 0;       // the arrivals the counter holds
-    HostBuf<PanelOut> h_pout_;       // per sub-panel: the status and the pivots so far, copied back as it ends
+    HostBuf<PanelOut> h_pout_;       // per sub-panel: the status and its pivots, written by the panel LU kernel
+    PanelOut* h_pout_dev_ = nullptr; // the device's address of h_pout_
     EventList sp_ev_;                // the end of each sub-panel's copies
     DevBuf<URun> d_runs_;
     DevBuf<ExtractCol> d_ex_;
@@ -727,15 +728,17 @@ int Driver::panel(int64_t j0, int64_t j1)
     // each sub-panel's pivot positions and the status so far come back as soon as it is factored: the host works on
     // a sub-panel's columns while the device factors the next ones
     const int64_t nsp = (np + FPA_PANEL_NB - 1) / FPA_PANEL_NB;
-    if ((int64_t)h_pout_.n < nsp) h_pout_.alloc(nsp);
+    if ((int64_t)h_pout_.n < nsp) {
+        h_pout_.alloc(nsp);
+        hip_check(hipHostGetDevicePointer((void**)&h_pout_dev_, h_pout_.p, 0), "hipHostGetDevicePointer");
+    }
     sp_ev_.resize(nsp);
     for (int64_t a = 0, sp = 0; a < np; a += FPA_PANEL_NB, sp++) {
         const int32_t nb = (int32_t)std::min<int64_t>(FPA_PANEL_NB, np - a);
-        k::getf2(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_colmax_.p, opt_.pivot_rel, d_piv, d_st, d_coop_.p,
-                 d_coop_cnt_.p, &coop_counted_, s_);
+        h_pout_.p[sp].st.err = INT32_MIN;   // (the kernel writes the slot; this marks it not written yet)
+        k::getf2(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_colmax_.p, opt_.pivot_rel, d_piv, d_st,
+                 h_pout_dev_ + sp, d_coop_.p, d_coop_cnt_.p, &coop_counted_, s_);
         k::gemm_sub(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_st, s_);
-        hip_check(hipMemcpyAsync(h_pout_.p + sp, d_pout_.p, offsetof(PanelOut, piv) + (a + nb) * sizeof(int32_t),
-                                 hipMemcpyDeviceToHost, s_), "panel status and pivots D2H");
         hip_check(hipEventRecord(sp_ev_.ev[sp], s_), "hipEventRecord");
     }
     // while the device runs the updates: the columns' candidates as lower positions and bitsets (in parallel)
@@ -780,6 +783,11 @@ int Driver::panel(int64_t j0, int64_t j1)
             waited += now_s() - tw0;
             if (sp > 0) F_.t_wait_lu += now_s() - tw0;
             const PanelStatus st = h_pout_.p[sp].st;
+            if (st.err == INT32_MIN) {
+                dev_sync(s_);
+                errf(err_, errlen_, "internal: the panel LU left no status");
+                return FPLUS_ERR_INTERNAL;
+            }
             if (st.err < 0) {
                 dev_sync(s_);
                 errf(err_, errlen_, "internal: the cooperative panel LU's workgroups did not meet at a barrier");
