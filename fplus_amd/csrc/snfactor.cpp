@@ -151,9 +151,10 @@ private:
         *end = S.srch_end >= 0 ? S.srch_end : S.nr() - S.w;
         return S.srch.data();
     }
-    // the search's marks and stacks (one set per thread of the parallel per-column search)
+    // the search's marks and stacks (one set per thread of the parallel per-column search), 32-bit: rows, columns,
+    // supernodes and marks (below 3 n + P) all fit, and half the footprint keeps more of the marks in cache
     struct SWork {
-        std::vector<int64_t> rmark, smark, seg0, topo, cand, stack, pos;
+        std::vector<int32_t> rmark, smark, seg0, topo, cand, stack, pos;
         void init(size_t nn)
         {
             rmark.assign(nn, 0); smark.assign(nn, 0); seg0.assign(nn, 0); topo.assign(nn, 0);
@@ -167,7 +168,7 @@ private:
                    int64_t pos0, const InList& in_list) const
     {
         int64_t sp = 0;
-        w.stack[0] = root; w.smark[root] = stamp; w.seg0[root] = col; w.pos[root] = pos0;
+        w.stack[0] = (int32_t)root; w.smark[root] = (int32_t)stamp; w.seg0[root] = (int32_t)col; w.pos[root] = (int32_t)pos0;
         while (sp >= 0) {
             const int64_t s = w.stack[sp];
             int64_t end;
@@ -176,16 +177,16 @@ private:
             for (; q < end; q++) {
                 const int64_t i = rows[q], k = pinv_[i];
                 if (k < 0) {
-                    if (w.rmark[i] != stamp && !in_list(i)) { w.rmark[i] = stamp; w.cand[(*ncand)++] = i; }
+                    if (w.rmark[i] != stamp && !in_list(i)) { w.rmark[i] = (int32_t)stamp; w.cand[(*ncand)++] = (int32_t)i; }
                     continue;
                 }
                 const int64_t u = col_sn_[k];
-                if (w.smark[u] == stamp) { if (k < w.seg0[u]) w.seg0[u] = k; continue; }
-                t = u; w.smark[u] = stamp; w.seg0[u] = k;
+                if (w.smark[u] == stamp) { if (k < w.seg0[u]) w.seg0[u] = (int32_t)k; continue; }
+                t = u; w.smark[u] = (int32_t)stamp; w.seg0[u] = (int32_t)k;
                 break;
             }
-            if (t >= 0) { w.pos[s] = q + 1; w.pos[t] = 0; w.stack[++sp] = t; }
-            else { sp--; w.topo[--top] = s; }
+            if (t >= 0) { w.pos[s] = (int32_t)(q + 1); w.pos[t] = 0; w.stack[++sp] = (int32_t)t; }
+            else { sp--; w.topo[--top] = (int32_t)s; }
         }
         return top;
     }
@@ -381,11 +382,11 @@ int Driver::panel(int64_t j0, int64_t j1)
             for (int64_t q = Bp_[c]; q < Bp_[c + 1]; q++) {
                 const int64_t i = Bi_[q], k = pinv_[i];
                 if (k < 0) {
-                    if (w.rmark[i] != stamp) { w.rmark[i] = stamp; w.cand[ncand++] = i; }
+                    if (w.rmark[i] != stamp) { w.rmark[i] = (int32_t)stamp; w.cand[ncand++] = (int32_t)i; }
                     continue;
                 }
                 const int64_t sn = col_sn_[k];
-                if (w.smark[sn] == stamp) { if (k < w.seg0[sn]) w.seg0[sn] = k; continue; }
+                if (w.smark[sn] == stamp) { if (k < w.seg0[sn]) w.seg0[sn] = (int32_t)k; continue; }
                 top = dfs(w, sn, k, top, stamp, open_, &ncand);
             }
             c_s[ci].resize(n - top);
@@ -453,7 +454,7 @@ int Driver::panel(int64_t j0, int64_t j1)
             out.clear();
             for (int64_t ci = c * per; ci < std::min(np, (c + 1) * per); ci++)
                 for (const int32_t row : c_np[ci])
-                    if (hrowpos_[row] < 0 && w.rmark[row] != stamp) { w.rmark[row] = stamp; out.push_back(row); }
+                    if (hrowpos_[row] < 0 && w.rmark[row] != stamp) { w.rmark[row] = (int32_t)stamp; out.push_back(row); }
         });
         // the join: the first chunk's rows (most of them) placed, the other chunks' lists filtered against them in
         // parallel, then the few rows left joined in chunk order
@@ -848,7 +849,7 @@ int Driver::panel(int64_t j0, int64_t j1)
         for (int64_t i : piv_c_) {
             const int64_t k = pinv_[i];
             const int64_t s = col_sn_[k];
-            if (ws0_.smark[s] == stamp2) { if (k < ws0_.seg0[s]) ws0_.seg0[s] = k; continue; }
+            if (ws0_.smark[s] == stamp2) { if (k < ws0_.seg0[s]) ws0_.seg0[s] = (int32_t)k; continue; }
             top = dfs_in(ws0_, s, k, top, stamp2, open_, &ncand, 0, in_list);
         }
         for (int64_t q = 0; q < ncand; q++) {
@@ -1054,6 +1055,10 @@ int Driver::run()
     F_.vs = vs_;
     relax_ = opt_.relax_rows > 0 ? opt_.relax_rows : 0;
     relax_auto_ = opt_.relax_rows < 0;
+    if (n >= 700000000) {   // the search's 32-bit marks reach 3 n + P
+        errf(err_, errlen_, "matrix order %ld is beyond this build's supernodal factorization", (long)n);
+        return FPLUS_ERR_BADARG;
+    }
     const int64_t nnzA = Bp_[n];
     if (n <= 4096 && nnzA <= 24 * n) {   // the automatic rule's bounds for small sparse matrices (src/factor_sn.c:644-645)
         rz_w_[0] = 2; rz_w_[1] = 8; rz_w_[2] = 32; rz_z_[0] = 0.3; rz_z_[1] = 0.03; rz_z_[2] = 0.01; rz_cap_ = 8;
