@@ -605,10 +605,12 @@ int Driver::panel(int64_t j0, int64_t j1)
     F_.t_lev += now_s() - th_ws;
     F_.t_lev_pass += now_s() - tlp0;
     std::stable_sort(tl.begin(), tl.end(), [](const TaskL& a, const TaskL& b) { return a.level < b.level; });
-    // tile rows per level: 64, or 32 where the level has fewer than four 64-row tiles per compute unit (measured: the
-    // 32-row tiles, which read two k chunks ahead, are as fast or faster up to there)
+    // tile rows per level: real, 64, or 32 where the level has fewer than four 64-row tiles per compute unit (measured:
+    // the 32-row tiles, which read two k chunks ahead, are as fast or faster up to there); complex, always 32 (their
+    // tiles multiply with three real products instead of four, which the 64-row tiles' registers do not allow)
     std::vector<int32_t> lev_rt(maxlevel + 2, RT);
-    {
+    if (vs_ == 2) std::fill(lev_rt.begin(), lev_rt.end(), RT / 2);
+    else {
         std::vector<int64_t> nt64(maxlevel + 2, 0);
         for (const TaskL& x : tl) {
             const BlockDesc& B = F_.blocks[x.t.block];
@@ -650,8 +652,12 @@ int Driver::panel(int64_t j0, int64_t j1)
         {
             // the product's size against what the tiles issue on the matrix cores (rows, columns and k padded)
             const double prod = fm * (double)T.ncol * (double)m * (double)(B.nrows - T.k0);
-            const int64_t ntr = (B.nrows - T.k0 + lev_rt[L] - 1) / lev_rt[L], ntc = (T.ncol + CTG - 1) / CTG;
-            const double issued = fm * (double)(ntr * lev_rt[L]) * (double)(ntc * CTG) * (double)((m + 7) / 8 * 8);
+            const int32_t rt = lev_rt[L];
+            const int64_t ntc = (T.ncol + CTG - 1) / CTG, ntr = (B.nrows - T.k0 + rt - 1) / rt, q = m / rt;
+            // k padded to chunks of 8, summed over the row tiles: the q tiles inside the triangle stop at their last
+            // row (rt, 2 rt, ..), the others run to w
+            const double kr = (double)rt * (double)(q * (q + 1) / 2) + (double)(ntr - q) * (double)((m + 7) / 8 * 8);
+            const double issued = fm * (double)rt * kr * (double)(ntc * CTG);
             F_.upd_prod_flops += prod;
             F_.upd_mfma_flops += issued;
             lev_issued[L] += issued;
