@@ -135,7 +135,7 @@ def cpu_baseline(workload_flops, family):
                       f"{os.cpu_count()} logical CPUs on the host"}
 
 
-FLOP_KERNELS = {"blk_upd", "blk_inverse", "gemm_sub", "getf2"}
+FLOP_KERNELS = {"blk_upd", "blk_inverse", "blk_uinv", "gemm_sub", "getf2"}
 BYTE_KERNELS = {"lfwd_gather", "lfwd_pull", "lfwd_push", "ub_dots", "ub_red", "ub_prod", "utdots", "ut_prod", "ltdots", "lt_prod",
                 "ruiz_rowmax", "ruiz_scale_colmax", "ruiz_scale_cols"}
 
@@ -230,7 +230,7 @@ def main():
         L.fplus_numeric_free(nm)
     # the kernels' sums over the timed region
     kern = {}
-    for name in ("blk_upd", "blk_inverse", "gemm_sub", "getf2", "trsm_sub", "lfwd_gather", "lfwd_pull", "lfwd_push", "ub_dots", "ub_red", "ub_prod", "utdots", "ut_prod", "ltdots",
+    for name in ("blk_upd", "blk_inverse", "blk_uinv", "gemm_sub", "getf2", "trsm_sub", "lfwd_gather", "lfwd_pull", "lfwd_push", "ub_dots", "ub_red", "ub_prod", "utdots", "ut_prod", "ltdots",
                  "lt_prod", "ruiz_rowmax", "ruiz_scale_colmax", "ruiz_scale_cols"):
         ms, cnt, work = C.c_double(), C.c_int64(), C.c_double()
         L.fplus_amd_profile_get(name.encode(), C.byref(ms), C.byref(cnt), C.byref(work))

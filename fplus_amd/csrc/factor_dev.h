@@ -92,9 +92,11 @@ void rowmap_set(int64_t cnt, const int32_t* rows, int32_t* rowmap, int reset, hi
 void scatter_cols(int vs, int32_t ncol, const int64_t* Bp, const int32_t* Bi, const double* Bx, int64_t j0,
                   const int32_t* rowmap, double* W, int64_t ldw, double* colmax, hipStream_t s);
 void u_runs(int vs, int32_t nruns, const URun* runs, const double* W, double* Ux, int32_t* Ui, hipStream_t s);
-// the inverse forms of new blocks: jobs (block, column) for the triangles' inverses, then the tiles of the rows below
+// the inverse forms of new blocks: jobs (block, column) for L11^-1, then the tiles of the rows below
 void blk_inverse(int vs, int32_t njobs, const int2* jobs, int32_t ntile, const GemmTile* tiles, const BlockDesc* blocks,
                  double flops, hipStream_t s);
+// U11^-1 of blocks, jobs (block, column): only the solves read it, so it is made for every block at once
+void blk_uinv(int vs, int64_t njobs, const int2* jobs, const BlockDesc* blocks, double flops, hipStream_t s);
 // the tiles of one dependency level: each task's solved segment into its slice of X, its rows below subtracted from W
 // through rowmap (atomically for tiles marked shared); tile_rows: 64, or 32 (tiles of half the rows, for levels with
 // few tiles)

@@ -748,8 +748,15 @@ __global__ __launch_bounds__(TB) void k_gemm_sub(double* __restrict__ Wd, int64_
         const int r = i % nb, c = i / nb;
         Bs[r][c] = Wl[(a + r) + (int64_t)(c0 + c) * ld];
     }
-    __syncthreads();
+    // the thread's part of the target, read with the operands (not after the product)
     const int tr = (threadIdx.x % 16) * 4, tc = (threadIdx.x / 16) * TC;
+    T cv[4][TC];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < TC; j++)
+            cv[i][j] = (tr + i < mr && tc + j < mc) ? Wl[(r0 + tr + i) + (int64_t)(c0 + tc + j) * ld] : Sc<T>::zero();
+    __syncthreads();
     T acc[4][TC];
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -770,10 +777,7 @@ __global__ __launch_bounds__(TB) void k_gemm_sub(double* __restrict__ Wd, int64_
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < TC; j++)
-            if (tr + i < mr && tc + j < mc) {
-                T* p = Wl + (r0 + tr + i) + (int64_t)(c0 + tc + j) * ld;
-                *p = Sc<T>::sub(*p, acc[i][j]);
-            }
+            if (tr + i < mr && tc + j < mc) Wl[(r0 + tr + i) + (int64_t)(c0 + tc + j) * ld] = Sc<T>::sub(cv[i][j], acc[i][j]);
 }
 
 // one wavefront per run
@@ -849,11 +853,17 @@ void blk_inverse(int vs, int32_t njobs, const int2* jobs, int32_t ntile, const G
     ProfScope ps("blk_inverse", flops, s);
     if (vs == 1) hipLaunchKernelGGL(k_blk_linv<double>, dim3(njobs), dim3(TB), 0, s, jobs, blocks);
     else hipLaunchKernelGGL(k_blk_linv<Zd>, dim3(njobs), dim3(TB), 0, s, jobs, blocks);
-    if (vs == 1) hipLaunchKernelGGL(k_blk_uinv<double>, dim3(njobs), dim3(TB), 0, s, jobs, blocks);
-    else hipLaunchKernelGGL(k_blk_uinv<Zd>, dim3(njobs), dim3(TB), 0, s, jobs, blocks);
     if (ntile <= 0) return;
     if (vs == 1) hipLaunchKernelGGL(k_blk_ivgemm<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks);
     else hipLaunchKernelGGL(k_blk_ivgemm<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks);
+}
+void blk_uinv(int vs, int64_t njobs, const int2* jobs, const BlockDesc* blocks, double flops, hipStream_t s)
+{
+    if (njobs <= 0) return;
+    if (njobs > INT32_MAX) throw std::invalid_argument("blk_uinv: too many jobs for one launch");
+    ProfScope ps("blk_uinv", flops, s);
+    if (vs == 1) hipLaunchKernelGGL(k_blk_uinv<double>, dim3((unsigned)njobs), dim3(TB), 0, s, jobs, blocks);
+    else hipLaunchKernelGGL(k_blk_uinv<Zd>, dim3((unsigned)njobs), dim3(TB), 0, s, jobs, blocks);
 }
 void blk_upd(int vs, int32_t ntile, const UpdTile* tiles, const int32_t* cols, const int32_t* rowmap, double* W, int64_t ldw,
              int tile_rows, double flops, hipStream_t s)

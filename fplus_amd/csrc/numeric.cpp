@@ -203,6 +203,7 @@ struct fplus_numeric {
     int64_t nslots = 0;
     std::vector<int32_t> lev_lft, lev_ubt, lev_utt, lev_ltt;   // level offsets into the push / dot-product tiles
     DevBuf<int2> d_lft, d_utt, d_ltt;
+    DevBuf<int2> d_ujobs;                                       // (block, column) jobs of the blocks' U11^-1
     DevBuf<int4> d_ubt;        // U x = y dot-product tiles: block, first piece, pieces, partial slot
     std::vector<int32_t> lev_ubr;
     DevBuf<int4> d_ubr;        // and the partial sums' reduction tiles: block, first of 16 rows, first slot, slots
@@ -239,8 +240,24 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
     auto lap = [&](int i) { const double t = now_s(); num->t_prep_part[i] = t - tp; tp = t; };
     HostPool pool;
     std::vector<int32_t> colblk(n > 0 ? n : 1);
-    for (int32_t b = 0; b < nb; b++)
-        for (int32_t c = 0; c < F.blocks[b].w; c++) colblk[F.blocks[b].c0 + c] = b;
+    // every block's U11^-1, one job per column, on the device while the host prepares the rest (kept until the
+    // final synchronization below: the upload reads the host list)
+    std::vector<int2> ujobs;
+    ujobs.reserve(n > 0 ? n : 1);
+    double uflops = 0;
+    for (int32_t b = 0; b < nb; b++) {
+        const double w = F.blocks[b].w;
+        uflops += (F.vs == 1 ? 2.0 : 8.0) * w * w * w / 6;
+        for (int32_t c = 0; c < F.blocks[b].w; c++) {
+            colblk[F.blocks[b].c0 + c] = b;
+            ujobs.push_back(make_int2(b, c));
+        }
+    }
+    if (!ujobs.empty()) {
+        num->d_ujobs.reserve(ujobs.size());
+        num->d_ujobs.upload(ujobs.data(), ujobs.size(), s);
+        k::blk_uinv(F.vs, (int64_t)ujobs.size(), num->d_ujobs.p, F.d_blocks.p, uflops, s);
+    }
     auto order_levels = [nb](const std::vector<int32_t>& lvl, std::vector<int32_t>& blk, std::vector<int32_t>& off) {
         int32_t mx = 0;
         for (int32_t b = 0; b < nb; b++) mx = std::max(mx, lvl[b]);
