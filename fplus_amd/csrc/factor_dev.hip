@@ -213,16 +213,23 @@ template <class T, int TM> struct TileCfg {
 // column's 16 consecutive rows come from 16 lanes -- the write-out needs no pass through LDS.  pre() runs once the
 // first chunk is staged, before the barrier that publishes it: LDS state the output needs is stored there, from loads
 // that were in flight with the first chunks'.
+template <int N> struct IC { static constexpr int value = N; };
+
 template <class T, int TM, class LA, class LB, class OUT, class PRE>
-__device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB& lb, const OUT& out, const PRE& pre, T* stg)
+__device__ inline void tile_mfma(int32_t kb, int32_t ke, int32_t mrows, int32_t ncols, const LA& la, const LB& lb, const OUT& out,
+                                 const PRE& pre, T* stg)
 {
     using Cf = TileCfg<T, TM>;
     constexpr bool CPLX = Sc<T>::vs == 2;
     constexpr int KC = Cf::KC;
     T* const As = stg;
     T* const Bs = stg + 2 * KC * TM;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wr = (wv / Cf::WC) * 32, wc = (wv % Cf::WC) * (16 * Cf::SJ);   // the wave's part of the tile
+    // a wave whose part of the tile holds no used row or column (of mrows x ncols) skips its products: they are zero
+    // and never written, and its matrix-core cycles go to other waves (finer skips, per 16 rows or columns inside a
+    // wave's part, cost the complex tiles their registers)
+    const bool on = mrows > wr && ncols > wc;
     dx4 accr[2][Cf::SJ], acci[2][Cf::SJ], acc3[2][Cf::SJ];
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -260,19 +267,20 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB&
             Bs[(buf * KC + e % KC) * BLD + e / KC] = rb[q];
         }
     };
-    auto mma = [&](int buf) {
+    auto mma_n = [&](auto ni_c, auto nj_c, int buf) {
+        constexpr int NI = decltype(ni_c)::value, NJ = decltype(nj_c)::value;
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 4) {
             const int kk = ks + (lane >> 4);
             T a[2], b[Cf::SJ];   // a: A[row lane % 16][k lane / 16]; b: B[k lane / 16][column lane % 16]
 #pragma unroll
-            for (int i = 0; i < 2; i++) a[i] = As[(buf * KC + kk) * TM + wr + i * 16 + (lane & 15)];
+            for (int i = 0; i < NI; i++) a[i] = As[(buf * KC + kk) * TM + wr + i * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < Cf::SJ; j++) b[j] = Bs[(buf * KC + kk) * BLD + wc + j * 16 + (lane & 15)];
+            for (int j = 0; j < NJ; j++) b[j] = Bs[(buf * KC + kk) * BLD + wc + j * 16 + (lane & 15)];
 #pragma unroll
-            for (int i = 0; i < 2; i++)
+            for (int i = 0; i < NI; i++)
 #pragma unroll
-                for (int j = 0; j < Cf::SJ; j++) {
+                for (int j = 0; j < NJ; j++) {
                     if constexpr (Cf::G3) {
                         accr[i][j] = mfma16(b[j].x, a[i].x + a[i].y, accr[i][j]);
                         acci[i][j] = mfma16(b[j].y - b[j].x, a[i].x, acci[i][j]);
@@ -287,6 +295,9 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, const LA& la, const LB&
                     }
                 }
         }
+    };
+    auto mma = [&](int buf) {
+        if (on) mma_n(IC<2>(), IC<Cf::SJ>(), buf);
     };
     fetch(kb, ra0, rb0);
     if constexpr (!Cf::DEEP) {
@@ -357,7 +368,7 @@ __global__ __launch_bounds__(TB) void k_blk_ivgemm(const GemmTile* __restrict__ 
     auto out = [&](int r, int c, T v) {
         if (r < nr && c < nc) out_base[r + (int64_t)c * B.nrows] = v;
     };
-    tile_mfma<T, TM>(tl.col0, B.w, la, lb, out, [] {}, stg);
+    tile_mfma<T, TM>(tl.col0, B.w, nr, nc, la, lb, out, [] {}, stg);
 }
 
 // one tile of a task's update, rows [row0, row0 + TM) of the block from the segment start k0 on: the product
@@ -408,7 +419,7 @@ __device__ inline void upd_tile(const UpdTile& tl, const int32_t* __restrict__ c
     };
     // IV's rows inside the triangle are rows of L11^-1, zero right of the diagonal: a tile whose rows all lie above w
     // needs k only up to its last row
-    tile_mfma<T, TM>(tl.k0, min(tl.w, tl.row0 + TM), la, lb, out, [&] { if (tid < TM) rpos[tid] = rp; }, stg);
+    tile_mfma<T, TM>(tl.k0, min(tl.w, tl.row0 + TM), nr, nc, la, lb, out, [&] { if (tid < TM) rpos[tid] = rp; }, stg);
 }
 
 // one workgroup per tile of a dependency level (every task of the level in one launch)

@@ -653,11 +653,12 @@ int Driver::panel(int64_t j0, int64_t j1)
             // the product's size against what the tiles issue on the matrix cores (rows, columns and k padded)
             const double prod = fm * (double)T.ncol * (double)m * (double)(B.nrows - T.k0);
             const int32_t rt = lev_rt[L];
-            const int64_t ntc = (T.ncol + CTG - 1) / CTG, ntr = (B.nrows - T.k0 + rt - 1) / rt, q = m / rt;
-            // k padded to chunks of 8, summed over the row tiles: the q tiles inside the triangle stop at their last
-            // row (rt, 2 rt, ..), the others run to w
-            const double kr = (double)rt * (double)(q * (q + 1) / 2) + (double)(ntr - q) * (double)((m + 7) / 8 * 8);
-            const double issued = fm * (double)rt * kr * (double)(ntc * CTG);
+            // rows and k over the row tiles: the q tiles inside the triangle stop at their last row (k rt, 2 rt, ..),
+            // the others run to w (k padded to chunks of 8); a wave skips a part of a tile with no used row or
+            // column, so rows count in 32s and columns in a wave's width (16 for 32-row tiles, 32 for 64-row tiles)
+            const int64_t q = m / rt, rows_rest = (B.nrows - T.k0 - q * rt + 31) / 32 * 32, cg = rt == 32 ? 16 : 32;
+            const double kr = (double)rt * rt * (double)(q * (q + 1) / 2) + (double)rows_rest * (double)((m + 7) / 8 * 8);
+            const double issued = fm * kr * (double)((T.ncol + cg - 1) / cg * cg);
             F_.upd_prod_flops += prod;
             F_.upd_mfma_flops += issued;
             lev_issued[L] += issued;
