@@ -8,8 +8,8 @@ region (it is reused across refactorizations of one pattern).  Prints one JSON l
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu] [--verbose]
     python bench.py --copy-probe      # 3 device-to-device copies of 1 GiB: the HBM counters' calibration run
 
-The line carries the backward error of every right-hand side of the last step, the roofline of the dominant kernel,
-and a CPU baseline.  The kernels are timed with HIP events on the library's stream around every launch, in one more
+The line carries the backward error of every right-hand side of the last step, the solve alone on that step's
+factorization with 1 and with 16 right-hand sides (solve_s), the roofline of the dominant kernel, and a CPU baseline.  The kernels are timed with HIP events on the library's stream around every launch, in one more
 step run right after the timed ones: events on the ~175k launches of a step slow it by about a tenth, so the timed
 steps run without them (the profiled step's own time is in the line as profiled_step_s).  The CPU baseline is the reference's own
 supernodal kernel (oracle/_ref, compiled from the reference's sources; no CBLAS is installed, so its loop form) or the
@@ -214,6 +214,15 @@ def main():
     L.fplus_amd_dev_sync()
     t3 = time.perf_counter()
     L.fplus_amd_profile(0)
+    # the solve alone on the last factorization, with one and with 16 right-hand sides (wall clock, vectors in HBM)
+    solve_s = {}
+    for k in (1, NRHS):
+        L.fplus_amd_dev_copy(d_b, C.c_void_p(bf.ctypes.data), bf.nbytes, 0)
+        L.fplus_amd_dev_sync()
+        ts = time.perf_counter()
+        _lib.check(L.fplus_amd_solve_device(nums[-1], d_b, k, n, 0, err, 512), err)
+        L.fplus_amd_dev_sync()
+        solve_s[f"nrhs{k}"] = time.perf_counter() - ts
     # correctness of the last step: solve the 16 right-hand sides again from b and check the backward errors
     L.fplus_amd_dev_copy(d_b, C.c_void_p(bf.ctypes.data), bf.nbytes, 0)
     _lib.check(L.fplus_amd_solve_device(nums[-1], d_b, NRHS, n, 0, err, 512), err)
@@ -257,6 +266,7 @@ def main():
         "config": {"workload": args.workload, "matrix": desc, **kw, "n": n, "nnz": int(Ap[-1]), "nrhs": NRHS,
                    "precision": "fp64 (plain GPU drop-in)", "ordering": "fplus_analyze defaults", "relax_rows": 32},
         "backward_error_max": max(bes),
+        "solve_s": solve_s,
         "factor": {"flops": st.flops, "gemm_flops": st.gemm_flops, "gemm_bytes": st.gemm_bytes, "nnzL": st.nnzL, "nnzU": st.nnzU,
                    "supernodes": st.nsupernodes, "panels": st.panels, "levels": st.levels,
                    "t_ruiz": st.t_ruiz, "t_factor": st.t_factor, "t_host": st.t_host, "gflops": st.flops / sec / 1e9},
