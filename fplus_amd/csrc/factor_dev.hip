@@ -13,6 +13,8 @@
 //     trailing TRSM and GEMM), LAPACK-style with whole-row swaps, so the pivot rows end at the top in column order.
 // Entries of W outside a column's sparse pattern are exact zeros and stay so, which makes the dense panel LU produce
 // the sparse factorization's values and pivots (a row outside the pattern has modulus 0 and cannot win).
+#include <atomic>
+
 #include "dev_common.h"
 #include "device_api.h"
 #include "factor_dev.h"
@@ -904,14 +906,16 @@ void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int
     const int32_t G = (int32_t)((nr - a + TB - 1) / TB);
     if (P > TB) throw std::invalid_argument("panel LU: the panel is wider than a workgroup");
     // the workgroups the device holds at once (occupancy times compute units), per scalar type
-    static int resident[2] = {-1, -1};
-    int& cap = resident[vs == 1 ? 0 : 1];
+    // (found by the first call of each type; calls on other threads may find it too, with the same result)
+    static std::atomic<int> resident[2] = {-1, -1};
+    int cap = resident[vs == 1 ? 0 : 1].load(std::memory_order_relaxed);
     if (cap < 0) {
         int per = 0;
         hip_check(vs == 1 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_getf2_coop<double>, TB, 0)
                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)k_getf2_coop<Zd>, TB, 0),
                   "occupancy of the panel LU");
         cap = per * devctx().ncu;
+        resident[vs == 1 ? 0 : 1].store(cap, std::memory_order_relaxed);
     }
     if (G >= 1 && G <= std::min(cap, getf2_coop_max_wg()) && nb <= PNB) {
         // the barrier counter keeps growing across launches (nb G arrivals each); it is cleared before it could overflow

@@ -247,20 +247,21 @@ KernelProf& kprof()
 
 DevCtx& devctx()
 {
-    static DevCtx ctx;
-    static bool inited = false;
-    if (!inited) {
+    // made once, by the first caller, under the language's guarantee for local statics (callers on other threads wait
+    // for it); a failed set-up throws and leaves it to the next call
+    static DevCtx ctx = [] {
+        DevCtx c;
         int cnt = 0;
         hipError_t e = hipGetDeviceCount(&cnt);
         if (e != hipSuccess || cnt < 1) throw GpuError("no HIP device available (the FPLUS MI355X build runs on the GPU only)");
         hip_check(hipSetDevice(0), "hipSetDevice");
         hipDeviceProp_t prop;
         hip_check(hipGetDeviceProperties(&prop, 0), "hipGetDeviceProperties");
-        ctx.device = 0;
-        ctx.ncu = prop.multiProcessorCount;
-        hip_check(hipStreamCreateWithFlags(&ctx.stream, hipStreamNonBlocking), "hipStreamCreate");
-        inited = true;
-    }
+        c.device = 0;
+        c.ncu = prop.multiProcessorCount;
+        hip_check(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "hipStreamCreate");
+        return c;
+    }();
     return ctx;
 }
 

@@ -8,7 +8,8 @@
   message, as the oracle does, and the next factorization in the same process is unaffected;
 - `fplus_solve` with ldb > n and more right-hand sides than one device pass holds (the reference solves in groups of
   32, src/numeric.c:86-125): every column equals its own one-column solve bit for bit and the rows past n are untouched;
-- the reference's high-level case with unscaled rows (`test_high_level(1, 0.0, PARTIAL)`, test_fplus.c:283).
+- the reference's high-level case with unscaled rows (`test_high_level(1, 0.0, PARTIAL)`, test_fplus.c:283);
+- factorizations and solves on two host threads at once (include/fplus.h:4-5).
 """
 import ctypes as C
 
@@ -148,3 +149,41 @@ def test_high_level_unscaled_complex():
         x = num.solve(b, trans)
         for k in range(3):
             assert _bwd(A, x[:, k], b[:, k], trans) < 1e-12
+
+
+def test_concurrent_factorizations():
+    """the library keeps no per-call state outside its handles (reference include/fplus.h:4-5, separate calls may run
+    on separate threads): two host threads factoring and solving at once, three times each, get the oracle's pivots
+    and backward errors at round-off"""
+    import threading
+
+    cases = [(gen.reference_test_matrix(300, True, 6.0), 4), (gen.convdiff3d(10), 32)]
+    results = [None, None]
+    errors = []
+
+    def work(i):
+        try:
+            (n, Ap, Ai, Ax), relax = cases[i]
+            A = sp.csc_matrix((Ax, Ai, Ap), shape=(n, n))
+            out = []
+            for rep in range(3):
+                sym, num = _factor_device(n, Ap, Ai, Ax, relax)
+                b = np.random.default_rng(20 + rep).standard_normal(n)
+                out.append((sym.row_perm.copy(), sym.col_perm.copy(), num.p.copy(), _bwd(A, num.solve(b), b, 0)))
+            results[i] = out
+        except Exception as e:   # reported below, in the test's own thread
+            errors.append(e)
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(2)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    assert not errors, errors
+    for i, ((n, Ap, Ai, Ax), relax) in enumerate(cases):
+        rp, cp = results[i][0][0], results[i][0][1]
+        ref = O.Numeric(n, Ap, Ai, Ax, rp, cp, relax_rows=relax)
+        for rp_k, cp_k, p_k, be in results[i]:
+            assert np.array_equal(rp_k, rp) and np.array_equal(cp_k, cp)
+            assert np.array_equal(p_k, ref.p)
+            assert be < 1e-12
