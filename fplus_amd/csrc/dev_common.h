@@ -13,6 +13,22 @@ struct Zd {
 
 __host__ __device__ inline Zd zmake(double a, double b) { Zd r; r.x = a; r.y = b; return r; }
 
+// A pointer the compiler cannot place (read from a descriptor in memory) is generic, and a generic access counts
+// against the LDS counter as well as the vector-memory one: every wait for an LDS read then also waits for it, which
+// undoes a prefetch.  Device buffers reached that way are made global pointers with as_global.
+template <class T> using gptr = T __attribute__((address_space(1)))*;
+template <class T> __device__ inline gptr<T> as_global(T* p) { return (gptr<T>)p; }
+// element i through a global pointer (a complex one as one 16-byte access)
+typedef double gdv2 __attribute__((ext_vector_type(2)));
+__device__ inline double gload(gptr<const double> p, int64_t i) { return p[i]; }
+__device__ inline Zd gload(gptr<const Zd> p, int64_t i)
+{
+    const gdv2 v = ((gptr<const gdv2>)p)[i];
+    return zmake(v.x, v.y);
+}
+__device__ inline void gstore(gptr<double> p, int64_t i, double v) { p[i] = v; }
+__device__ inline void gstore(gptr<Zd> p, int64_t i, Zd v) { ((gptr<gdv2>)p)[i] = gdv2{v.x, v.y}; }
+
 template <class T> struct Sc;
 template <> struct Sc<double> {
     static constexpr int vs = 1;

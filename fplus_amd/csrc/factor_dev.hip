@@ -243,30 +243,33 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, int32_t mrows, int32_t 
         }
     // thread element e = tid + q * TB; A: row e % TM, k e / TM (rows contiguous); B: k e % KC, column e / KC
     T ra0[Cf::LPA], rb0[Cf::LPB], ra1[Cf::LPA], rb1[Cf::LPB];
+    // Every load of a chunk is issued, also past the end (k clamped to ke - 1 < ke, so the address stays inside the
+    // operand), and the elements past ke are zeroed when the chunk is staged, where the data are waited for anyway:
+    // loads under branches, or values used right after their load, make the compiler wait for every load in flight --
+    // the prefetched chunks' too -- at each LDS store.  la and lb read rows past the tile's and columns past its width
+    // from valid places too; the products of those rows and columns are not written out.
     auto fetch = [&](int32_t kc, T (&ra)[Cf::LPA], T (&rb)[Cf::LPB]) {
 #pragma unroll
         for (int q = 0; q < Cf::LPA; q++) {
             const int e = tid + q * TB;
-            const int32_t k = kc + e / TM;
-            ra[q] = k < ke ? la(e % TM, k) : Sc<T>::zero();
+            ra[q] = la(e % TM, min(kc + e / TM, ke - 1));
         }
 #pragma unroll
         for (int q = 0; q < Cf::LPB; q++) {
             const int e = tid + q * TB;
-            const int32_t k = kc + e % KC;
-            rb[q] = k < ke ? lb(k, e / KC) : Sc<T>::zero();
+            rb[q] = lb(min(kc + e % KC, ke - 1), e / KC, q);
         }
     };
-    auto stage = [&](int buf, const T (&ra)[Cf::LPA], const T (&rb)[Cf::LPB]) {
+    auto stage = [&](int buf, int32_t kc, const T (&ra)[Cf::LPA], const T (&rb)[Cf::LPB]) {
 #pragma unroll
         for (int q = 0; q < Cf::LPA; q++) {
             const int e = tid + q * TB;
-            As[(buf * KC + e / TM) * TM + e % TM] = ra[q];
+            As[(buf * KC + e / TM) * TM + e % TM] = kc + e / TM < ke ? ra[q] : Sc<T>::zero();
         }
 #pragma unroll
         for (int q = 0; q < Cf::LPB; q++) {
             const int e = tid + q * TB;
-            Bs[(buf * KC + e % KC) * BLD + e / KC] = rb[q];
+            Bs[(buf * KC + e % KC) * BLD + e / KC] = kc + e % KC < ke ? rb[q] : Sc<T>::zero();
         }
     };
     auto mma_n = [&](auto ni_c, auto nj_c, int buf) {
@@ -303,36 +306,36 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, int32_t mrows, int32_t 
     };
     fetch(kb, ra0, rb0);
     if constexpr (!Cf::DEEP) {
-        stage(0, ra0, rb0);
+        stage(0, kb, ra0, rb0);
         pre();
         __syncthreads();
         int buf = 0;
         for (int32_t kc = kb; kc < ke; kc += KC) {
             const bool more = kc + KC < ke;
-            if (more) fetch(kc + KC, ra0, rb0);
+            fetch(kc + KC, ra0, rb0);
             mma(buf);
-            if (more) stage(buf ^ 1, ra0, rb0);
+            if (more) stage(buf ^ 1, kc + KC, ra0, rb0);
             __syncthreads();
             buf ^= 1;
         }
     } else {
-    if (kb + KC < ke) fetch(kb + KC, ra1, rb1);
-    stage(0, ra0, rb0);
+    fetch(kb + KC, ra1, rb1);
+    stage(0, kb, ra0, rb0);
     pre();
     __syncthreads();
     // the exits depend on kb and ke only: every wave of the workgroup takes the same ones
     for (int32_t kc = kb;; kc += 2 * KC) {
         // chunk kc in LDS buffer 0, chunk kc + KC in registers 1
-        if (kc + 2 * KC < ke) fetch(kc + 2 * KC, ra0, rb0);
+        fetch(kc + 2 * KC, ra0, rb0);
         mma(0);
         if (kc + KC >= ke) break;
-        stage(1, ra1, rb1);
+        stage(1, kc + KC, ra1, rb1);
         __syncthreads();
         // chunk kc + KC in LDS buffer 1, chunk kc + 2 KC in registers 0
-        if (kc + 3 * KC < ke) fetch(kc + 3 * KC, ra1, rb1);
+        fetch(kc + 3 * KC, ra1, rb1);
         mma(1);
         if (kc + 2 * KC >= ke) break;
-        stage(0, ra0, rb0);
+        stage(0, kc + 2 * KC, ra0, rb0);
         __syncthreads();
     }
     }
@@ -362,13 +365,14 @@ __global__ __launch_bounds__(TB) void k_blk_ivgemm(const GemmTile* __restrict__ 
     const BlockDesc B = blocks[tl.task];
     const int32_t r0 = B.w + tl.row0;
     const int32_t nr = min(TM, B.nrows - r0), nc = min(TN, B.w - tl.col0);
-    const T* L = reinterpret_cast<const T*>(B.vals);
-    T* IV = reinterpret_cast<T*>(B.ivals);
-    auto la = [&](int r, int32_t k) { return r < nr ? L[(int64_t)k * B.nrows + r0 + r] : Sc<T>::zero(); };
-    auto lb = [&](int32_t k, int c) { return c < nc ? IV[k + (int64_t)(tl.col0 + c) * B.nrows] : Sc<T>::zero(); };
-    T* out_base = IV + r0 + (int64_t)tl.col0 * B.nrows;
+    const gptr<const T> L = as_global(reinterpret_cast<const T*>(B.vals));
+    const gptr<const T> IVr = as_global(reinterpret_cast<const T*>(B.ivals));
+    // rows past nr and columns past nc read the tile's last ones (nr, nc >= 1): their products are not written out
+    auto la = [&](int r, int32_t k) { return gload(L, (int64_t)k * B.nrows + r0 + min(r, nr - 1)); };
+    auto lb = [&](int32_t k, int c, int) { return gload(IVr, k + (int64_t)(tl.col0 + min(c, nc - 1)) * B.nrows); };
+    const gptr<T> out_base = as_global(reinterpret_cast<T*>(B.ivals) + r0 + (int64_t)tl.col0 * B.nrows);
     auto out = [&](int r, int c, T v) {
-        if (r < nr && c < nc) out_base[r + (int64_t)c * B.nrows] = v;
+        if (r < nr && c < nc) gstore(out_base, r + (int64_t)c * B.nrows, v);
     };
     tile_mfma<T, TM>(tl.col0, B.w, nr, nc, la, lb, out, [] {}, stg);
 }
@@ -393,18 +397,25 @@ __device__ inline void upd_tile(const UpdTile& tl, const int32_t* __restrict__ c
     if (tid < TN) cidx[tid] = tid < nc ? cols[tl.cfirst + tid] : 0;
     __syncthreads();
     const int32_t rp = row >= 0 ? rowmap[row] : 0;
-    const T* IV = reinterpret_cast<const T*>(tl.iv) + tl.row0;
+    const gptr<const T> IV = as_global(reinterpret_cast<const T*>(tl.iv) + tl.row0);
     const T* S = reinterpret_cast<const T*>(W) + (tl.spos0 - tl.k0);
-    auto la = [&](int r, int32_t k) { return r < nr ? IV[(int64_t)k * tl.nrows + r] : Sc<T>::zero(); };
-    auto lb = [&](int32_t k, int c) { return c < nc ? S[k + (int64_t)cidx[c] * ldw] : Sc<T>::zero(); };
-    T* X = reinterpret_cast<T*>(tl.x) + (tl.row0 - tl.k0);
+    // rows past nr read the tile's last row (nr >= 1), columns past nc W's column 0 (cidx): their products are not
+    // written out.  A thread stages the same B columns in every chunk (column (tid + q TB) / KC of its q-th element):
+    // their W offsets are taken once
+    using Cf = TileCfg<T, TM>;
+    int64_t cofs[Cf::LPB];
+#pragma unroll
+    for (int q = 0; q < Cf::LPB; q++) cofs[q] = (int64_t)cidx[(tid + q * TB) / Cf::KC] * ldw;
+    auto la = [&](int r, int32_t k) { return gload(IV, (int64_t)k * tl.nrows + min(r, nr - 1)); };
+    auto lb = [&](int32_t k, int, int q) { return S[k + cofs[q]]; };
+    const gptr<T> X = as_global(reinterpret_cast<T*>(tl.x) + (tl.row0 - tl.k0));
     T* Wt = reinterpret_cast<T*>(W);
     const int32_t rw = tl.w - tl.row0;   // tile rows from here on are below the block's triangle
     const bool shared = tl.shared != 0;
     auto out = [&](int r, int c, T v) {
         if (r >= nr || c >= nc) return;
         if (r < rw) {
-            X[r + (int64_t)c * m] = v;
+            gstore(X, r + (int64_t)c * m, v);
             return;
         }
         T* p = Wt + rpos[r] + (int64_t)cidx[c] * ldw;
