@@ -85,7 +85,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     const int2 jb = jobs[blockIdx.x];
     const BlockDesc B = blocks[jb.x];
     const int32_t w = B.w, c = jb.y, t = threadIdx.x;
-    const T* L = reinterpret_cast<const T*>(B.vals);
+    const gptr<const T> L = as_global(reinterpret_cast<const T*>(B.vals));   // global: generic loads would hold up the LDS waits
     const bool own = t < w;
     T x = t == c ? Sc<T>::one() : Sc<T>::zero();
     auto fetch = [&](int ch, T* lv) {
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
 #pragma unroll
         for (int u = 0; u < CH; u++) {
             const int k = r0 + u;
-            lv[u] = (own && k < t) ? L[(int64_t)k * B.nrows + t] : Sc<T>::zero();
+            lv[u] = (own && k < t) ? gload(L, (int64_t)k * B.nrows + t) : Sc<T>::zero();
         }
     };
     T lcur[CH], lnxt[CH];
@@ -125,8 +125,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         for (int u = 0; u < CH; u++) lcur[u] = lnxt[u];
     }
     if (own) {
-        reinterpret_cast<T*>(B.ivals)[t + (int64_t)c * B.nrows] = x;
-        reinterpret_cast<T*>(B.linv)[t + (int64_t)c * w] = x;
+        gstore(as_global(reinterpret_cast<T*>(B.ivals)), t + (int64_t)c * B.nrows, x);
+        gstore(as_global(reinterpret_cast<T*>(B.linv)), t + (int64_t)c * w, x);
     }
 }
 
@@ -142,17 +142,17 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     const int2 jb = jobs[blockIdx.x];
     const BlockDesc B = blocks[jb.x];
     const int32_t w = B.w, c = jb.y, t = threadIdx.x;
-    const T* U = reinterpret_cast<const T*>(B.vals);
+    const gptr<const T> U = as_global(reinterpret_cast<const T*>(B.vals));   // global, as in k_blk_linv
     const bool own = t <= c;
     T x = t == c ? Sc<T>::one() : Sc<T>::zero();
-    const T d = own ? U[(int64_t)t * B.nrows + t] : Sc<T>::one();
+    const T d = own ? gload(U, (int64_t)t * B.nrows + t) : Sc<T>::one();
     // U(t, k) for the chunk's columns k, t < k <= c
     auto fetch = [&](int ch, T* lv) {
         const int r0 = ch * CH;
 #pragma unroll
         for (int u = 0; u < CH; u++) {
             const int k = r0 + u;
-            lv[u] = (own && k > t && k <= c) ? U[(int64_t)k * B.nrows + t] : Sc<T>::zero();
+            lv[u] = (own && k > t && k <= c) ? gload(U, (int64_t)k * B.nrows + t) : Sc<T>::zero();
         }
     };
     T lcur[CH], lnxt[CH];
@@ -184,7 +184,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
 #pragma unroll
         for (int u = 0; u < CH; u++) lcur[u] = lnxt[u];
     }
-    if (t < w) reinterpret_cast<T*>(B.uinv)[t + (int64_t)c * w] = x;
+    if (t < w) gstore(as_global(reinterpret_cast<T*>(B.uinv)), t + (int64_t)c * w, x);
 }
 
 // FP64 matrix cores: D(16x16) += A(16x4) B(4x16) per wavefront; lane l supplies A[l % 16][l / 16] and

@@ -97,7 +97,8 @@ __global__ __launch_bounds__(TB) void k_tri_prod(const int2* __restrict__ tiles,
     __shared__ T xs[TB * RC];
     const int2 tl = tiles[blockIdx.x];   // block, first row
     const BlockDesc B = blocks[tl.x];
-    const T* M = reinterpret_cast<const T*>(UPPER != TRANS ? B.uinv : B.linv);
+    // (global pointers: a generic load would make each LDS wait below wait for the matrix loads in flight too)
+    const gptr<const T> M = as_global(reinterpret_cast<const T*>(UPPER != TRANS ? B.uinv : B.linv));
     const T* in = reinterpret_cast<const T*>(ind);
     T* out = reinterpret_cast<T*>(outd);
     const int w = B.w, r0 = tl.y;
@@ -116,7 +117,7 @@ __global__ __launch_bounds__(TB) void k_tri_prod(const int2* __restrict__ tiles,
         const int lo = UPPER ? max(qa, row) : qa, hi = UPPER ? qb : min(qb, row + 1);
 #pragma unroll 8
         for (int c = lo; c < hi; c++) {
-            const T v = TRANS ? cj(M[c + (int64_t)row * w], conj) : M[row + (int64_t)c * w];
+            const T v = TRANS ? cj(gload(M, c + (int64_t)row * w), conj) : gload(M, row + (int64_t)c * w);
 #pragma unroll
             for (int r = 0; r < RC; r++)
                 if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, xs[(c - ca) * RC + r]);
@@ -151,7 +152,7 @@ __global__ __launch_bounds__(TB) void k_lfwd_push(const int2* __restrict__ tiles
     __shared__ T ys[TB * RC];
     const int2 tl = tiles[blockIdx.x];   // block, first row below the block (0-based)
     const BlockDesc B = blocks[tl.x];
-    const T* V = reinterpret_cast<const T*>(B.vals);
+    const gptr<const T> V = as_global(reinterpret_cast<const T*>(B.vals));   // a global pointer, as in k_tri_prod
     const T* y = reinterpret_cast<const T*>(yd);
     T* contrib = reinterpret_cast<T*>(contribd);
     const int w = B.w;
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(TB) void k_lfwd_push(const int2* __restrict__ tiles
     if (own) {
 #pragma unroll 8
         for (int c = qa; c < qb; c++) {
-            const T v = V[(int64_t)c * B.nrows + row];
+            const T v = gload(V, (int64_t)c * B.nrows + row);
 #pragma unroll
             for (int r = 0; r < RC; r++)
                 if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, ys[c * RC + r]);
