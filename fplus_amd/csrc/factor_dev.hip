@@ -323,20 +323,30 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, int32_t mrows, int32_t 
     stage(0, kb, ra0, rb0);
     pre();
     __syncthreads();
-    // the exits depend on kb and ke only: every wave of the workgroup takes the same ones
-    for (int32_t kc = kb;; kc += 2 * KC) {
+    // chunk i = kb + i KC; the loop runs while both of its halves stage a next chunk, and the last one or two chunks
+    // are multiplied after it: a loop with no exit in its body lets the compiler keep the second register set's loads
+    // in flight across the loop head (an exit in the body merges with the back edge and forces a full wait there).
+    // The trip count depends on kb and ke only: every wave takes the same path.
+    const int32_t nch = (ke - kb + KC - 1) / KC;
+    int32_t kc = kb;
+    for (int32_t i = 0; i + 2 < nch; i += 2, kc += 2 * KC) {
         // chunk kc in LDS buffer 0, chunk kc + KC in registers 1
         fetch(kc + 2 * KC, ra0, rb0);
         mma(0);
-        if (kc + KC >= ke) break;
         stage(1, kc + KC, ra1, rb1);
         __syncthreads();
         // chunk kc + KC in LDS buffer 1, chunk kc + 2 KC in registers 0
         fetch(kc + 3 * KC, ra1, rb1);
         mma(1);
-        if (kc + 2 * KC >= ke) break;
         stage(0, kc + 2 * KC, ra0, rb0);
         __syncthreads();
+    }
+    // one or two chunks left: kc in LDS buffer 0, kc + KC (if any) in registers 1
+    mma(0);
+    if (kc + KC < ke) {
+        stage(1, kc + KC, ra1, rb1);
+        __syncthreads();
+        mma(1);
     }
     }
     // lane l holds C[row l % 16][column l / 16 + 4 v] of each 16 x 16 part
