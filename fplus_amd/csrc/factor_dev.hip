@@ -403,7 +403,8 @@ __device__ inline void upd_tile(const UpdTile& tl, const int32_t* __restrict__ c
     // the columns are needed for the first chunk, the rows' W positions only for the output: their rowmap loads are
     // issued here and land with the first chunk's
     int32_t row = -1;
-    if (tid < TM && tid < nr && tl.row0 + tid >= tl.w) row = tl.rows[tl.row0 + tid];
+    // (a global load: a generic one would wait with the descriptor's scalar loads and hold back the column load)
+    if (tid < TM && tid < nr && tl.row0 + tid >= tl.w) row = as_global(tl.rows)[tl.row0 + tid];
     if (tid < TN) cidx[tid] = tid < nc ? cols[tl.cfirst + tid] : 0;
     __syncthreads();
     const int32_t rp = row >= 0 ? rowmap[row] : 0;
