@@ -4,7 +4,7 @@ The numeric factorization (Ruiz equilibration, the supernodal left-looking LU wi
 solves run as HIP kernels on one GPU behind the FPLUS C interface (include/fplus.h, libfplus_amd.so).  This module
 binds that interface with ctypes and provides ``SparseLU``, the drop-in for ``scipy.sparse.linalg.splu`` of the
 reference (reference sparse_lu_full_pivot.py:630-836): ``solve(b, trans)``, ``L``, ``U``, ``perm_r``, ``perm_c``,
-``shape``, ``nnz``.
+``shape``, ``nnz`` (fplus_amd.splu, with the reference module's other functions).
 """
 import ctypes as C
 
@@ -14,7 +14,9 @@ import scipy.sparse as sp
 from ._lib import (FPLUS_ORDER_AMD_ATA, FPLUS_ORDER_AUTO, FPLUS_ORDER_COLAMD, FPLUS_PIVOT_PARTIAL,  # noqa: F401
                    FPLUS_PIVOT_REL_DEFAULT, Factors, FplusError, Options, Stats, check, lib)
 
-__all__ = ["SparseLU", "Symbolic", "Numeric", "SolvePlan", "default_options", "FplusError", "ruiz", "factor_sn_perm", "tri_solve"]
+__all__ = ["SparseLU", "sparse_lu_full_pivot", "lu_solve", "lu_residual", "clear_ordering_cache", "Symbolic", "Numeric",
+           "SolvePlan", "default_options", "FplusError", "ruiz", "factor_sn", "factor_sn_perm", "factor_sn_sym", "tri_solve",
+           "tune"]
 
 _I64P = C.POINTER(C.c_int64)
 _DP = C.POINTER(C.c_double)
@@ -140,68 +142,6 @@ class Numeric:
             self.ptr = None
 
 
-class SparseLU:
-    """Drop-in for the object ``scipy.sparse.linalg.splu`` returns (reference sparse_lu_full_pivot.py:630-836).
-
-    ``SparseLU(A, **kw)`` analyses and factors A (canonical CSC int64 inside) on the GPU; keyword arguments are the
-    fplus_options fields (``relax_rows``, ``equilibrate``, ``pivot_rel``, ...) plus the analysis settings
-    ``pin_dense_rows``, ``tau_row_mask``, ``tau_col_mask``, ``postorder`` and ``colamd`` (False: the identity orders).
-    Attributes: ``L``, ``U`` (CSC, of the scaled permuted matrix), ``perm_r``, ``perm_c``, ``shape``, ``nnz``.
-    ``solve(b, trans='N'|'T'|'H')`` for b of shape (n,) or (n, k)."""
-
-    def __init__(self, A, colamd=True, pin_dense_rows=True, tau_density=0.5, tau_min_nnz=10, tau_row_mask=None,
-                 tau_col_mask=None, postorder=True, ordering="colamd", **opts):
-        n, Ap, Ai, Ax = _canonical_csc(A)
-        self._n = n
-        ords = {"colamd": FPLUS_ORDER_COLAMD, "amd_ata": FPLUS_ORDER_AMD_ATA, "auto": FPLUS_ORDER_AUTO}
-        if colamd:
-            self._sym = Symbolic(n, Ap, Ai, pin_dense=pin_dense_rows, tau_density=tau_density, tau_min_nnz=tau_min_nnz,
-                                 tau_row_mask=tau_row_mask, tau_col_mask=tau_col_mask, postorder=postorder,
-                                 ordering=ords[ordering])
-        else:
-            self._sym = _IdentitySymbolic(n, Ap, Ai)
-        o = default_options(**{k: v for k, v in opts.items() if k in dict(Options._fields_)})
-        self._num = Numeric(self._sym, Ap, Ai, Ax, o)
-        self._f = None
-        self._is_complex = self._num.is_complex
-
-    def _factors(self):
-        if self._f is None:
-            self._f = self._num.factors()
-        return self._f
-
-    @property
-    def L(self):
-        return self._factors()["L"]
-
-    @property
-    def U(self):
-        return self._factors()["U"]
-
-    @property
-    def perm_r(self):
-        return self._num.p
-
-    @property
-    def perm_c(self):
-        return self._num.q
-
-    @property
-    def shape(self):
-        return (self._n, self._n)
-
-    @property
-    def nnz(self):
-        f = self._factors()
-        return f["L"].nnz + f["U"].nnz
-
-    def solve(self, b, trans="N"):
-        if trans not in ("N", "T", "H"):
-            raise ValueError(f"trans must be 'N', 'T', or 'H'; got {trans!r}")
-        tr = 0 if trans == "N" else (1 if trans == "T" or not self._is_complex else 2)
-        return self._num.solve(b, tr)
-
-
 class _IdentitySymbolic(Symbolic):
     """the identity orders (colamd=False), through fplus_amd_symbolic_from_orders"""
 
@@ -257,9 +197,39 @@ def ruiz(n, indptr, indices, data, is_csc=True, max_iter=20, tol=1e-2):
     return x, Dr, Dc
 
 
+def _factors_dict(f, n, cplx):
+    """an fplus_factors as numpy arrays (copies); the struct is freed"""
+    try:
+        nL, nU = f.Lp[n], f.Up[n]
+        dt = np.complex128 if cplx else np.float64
+        vs = 2 if cplx else 1
+        return dict(n=n, nsn=f.nsupernodes, Lp=np.ctypeslib.as_array(f.Lp, (n + 1,)).copy(),
+                    Li=np.ctypeslib.as_array(f.Li, (max(nL, 1),))[:nL].copy(),
+                    Lx=np.ctypeslib.as_array(C.cast(f.Lx, _DP), (max(nL * vs, 1),))[:nL * vs].copy().view(dt),
+                    Up=np.ctypeslib.as_array(f.Up, (n + 1,)).copy(),
+                    Ui=np.ctypeslib.as_array(f.Ui, (max(nU, 1),))[:nU].copy(),
+                    Ux=np.ctypeslib.as_array(C.cast(f.Ux, _DP), (max(nU * vs, 1),))[:nU * vs].copy().view(dt),
+                    p=np.ctypeslib.as_array(f.p, (n,)).copy())
+    finally:
+        lib().fplus_factors_free(C.byref(f))
+
+
+def factor_sn(n, Ap, Ai, Ax, relax_rows=32, pivot_rel=FPLUS_PIVOT_REL_DEFAULT):
+    """fplus_factor_sn: the supernodal factorization of the matrix as given (duplicates summed, no equilibration)"""
+    Ax = np.ascontiguousarray(Ax)
+    cplx = int(np.iscomplexobj(Ax))
+    f = Factors()
+    err = C.create_string_buffer(512)
+    rc = lib().fplus_factor_sn(n, _p64(np.ascontiguousarray(Ap, np.int64)), _p64(np.ascontiguousarray(Ai, np.int64)), _vp(Ax),
+                               cplx, 0, relax_rows, 512, 1, 5e7, pivot_rel, C.byref(f), err, 512)
+    check(rc, err)
+    return _factors_dict(f, n, cplx)
+
+
 def factor_sn_perm(n, Ap, Ai, Ax, row_perm, col_perm, equilibrate=True, relax_rows=32, pivot_rel=FPLUS_PIVOT_REL_DEFAULT,
                    ruiz_max_iter=20, ruiz_tol=1e-2):
-    """fplus_factor_sn_perm: the factors of the permuted, equilibrated matrix as numpy arrays, and Dr, Dc"""
+    """fplus_factor_sn_perm: the factors of the permuted, equilibrated matrix as numpy arrays, and Dr, Dc (in the
+    permuted order)"""
     Ax = np.ascontiguousarray(Ax)
     cplx = int(np.iscomplexobj(Ax))
     f = Factors()
@@ -272,20 +242,33 @@ def factor_sn_perm(n, Ap, Ai, Ax, row_perm, col_perm, equilibrate=True, relax_ro
                                     relax_rows, 512, 1, 5e7, pivot_rel, C.byref(f), Dr.ctypes.data_as(_DP),
                                     Dc.ctypes.data_as(_DP), err, 512)
     check(rc, err)
-    try:
-        nL, nU = f.Lp[n], f.Up[n]
-        dt = np.complex128 if cplx else np.float64
-        vs = 2 if cplx else 1
-        out = dict(n=n, nsn=f.nsupernodes, Lp=np.ctypeslib.as_array(f.Lp, (n + 1,)).copy(),
-                   Li=np.ctypeslib.as_array(f.Li, (max(nL, 1),))[:nL].copy(),
-                   Lx=np.ctypeslib.as_array(C.cast(f.Lx, _DP), (max(nL * vs, 1),))[:nL * vs].copy().view(dt),
-                   Up=np.ctypeslib.as_array(f.Up, (n + 1,)).copy(),
-                   Ui=np.ctypeslib.as_array(f.Ui, (max(nU, 1),))[:nU].copy(),
-                   Ux=np.ctypeslib.as_array(C.cast(f.Ux, _DP), (max(nU * vs, 1),))[:nU * vs].copy().view(dt),
-                   p=np.ctypeslib.as_array(f.p, (n,)).copy(), Dr=Dr, Dc=Dc)
-        return out
-    finally:
-        lib().fplus_factors_free(C.byref(f))
+    out = _factors_dict(f, n, cplx)
+    out.update(Dr=Dr, Dc=Dc)
+    return out
+
+
+def factor_sn_sym(sym, Ap, Ai, Ax, equilibrate=True, relax_rows=32, pivot_rel=FPLUS_PIVOT_REL_DEFAULT, ruiz_max_iter=20,
+                  ruiz_tol=1e-2):
+    """fplus_factor_sn_sym: factor_sn_perm with the orders of a Symbolic"""
+    Ax = np.ascontiguousarray(Ax)
+    cplx = int(np.iscomplexobj(Ax))
+    n = sym.n
+    f = Factors()
+    Dr = np.empty(n)
+    Dc = np.empty(n)
+    err = C.create_string_buffer(512)
+    rc = lib().fplus_factor_sn_sym(sym.ptr, _p64(np.ascontiguousarray(Ap, np.int64)), _p64(np.ascontiguousarray(Ai, np.int64)),
+                                   _vp(Ax), cplx, int(equilibrate), ruiz_max_iter, ruiz_tol, 0, relax_rows, 512, 1, 5e7,
+                                   pivot_rel, C.byref(f), Dr.ctypes.data_as(_DP), Dc.ctypes.data_as(_DP), err, 512)
+    check(rc, err)
+    out = _factors_dict(f, n, cplx)
+    out.update(Dr=Dr, Dc=Dc)
+    return out
+
+
+def tune(name, value):
+    """fplus_amd_tune: a process-wide tuning knob ("panel_lu_max_workgroups")"""
+    check(lib().fplus_amd_tune(name.encode(), int(value)), None)
 
 
 class SolvePlan:
@@ -339,3 +322,6 @@ def tri_solve(L, U, b, trans=0, plan=None):
                                         int(cplx)), None)
             X[:, r] = col
     return X[:, 0] if one else X
+
+
+from .splu import SparseLU, clear_ordering_cache, lu_residual, lu_solve, sparse_lu_full_pivot  # noqa: E402,F401

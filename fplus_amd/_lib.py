@@ -44,7 +44,8 @@ class Stats(C.Structure):
     _fields_ = [("nnzL", C.c_int64), ("nnzU", C.c_int64), ("nsupernodes", C.c_int64), ("nblocks", C.c_int64),
                 ("panels", C.c_int64), ("levels", C.c_int64), ("flops", C.c_double), ("gemm_flops", C.c_double),
                 ("t_ruiz", C.c_double), ("t_factor", C.c_double), ("t_host", C.c_double), ("t_wait", C.c_double),
-                ("t_total", C.c_double), ("ruiz_iters", C.c_int), ("gemm_bytes", C.c_double)]
+                ("t_total", C.c_double), ("ruiz_iters", C.c_int), ("gemm_bytes", C.c_double),
+                ("shared_levels", C.c_int64), ("max_block_w", C.c_int64), ("coop_retried", C.c_int)]
 
 
 # every entry point include/fplus.h and include/fplus_amd.h declare, with its ctypes signature
@@ -106,6 +107,7 @@ SIGNATURES = {
     "fplus_amd_dev_sync": (C.c_int, []),
     "fplus_amd_profile": (C.c_int, [C.c_int]),
     "fplus_amd_profile_get": (C.c_int, [C.c_char_p, DP, I64P, DP]),
+    "fplus_amd_tune": (C.c_int, [C.c_char_p, C.c_int64]),
 }
 
 _lib = None

@@ -28,6 +28,11 @@ struct DevCtx {
 };
 DevCtx& devctx();   // initialises on first use; throws GpuError when no device is usable
 
+// copies between device memory and pageable host memory, through page-locked bounce buffers (hostcopy.cpp): copy_h2d
+// returns once h may be reused (the device copy is ordered on s), copy_d2h once h holds the data
+void copy_h2d(void* d, const void* h, size_t bytes, hipStream_t s);
+void copy_d2h(void* h, const void* d, size_t bytes, hipStream_t s);
+
 template <class T> struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
@@ -51,14 +56,9 @@ template <class T> struct DevBuf {
         if (p) (void)hipFree(p);
         p = nullptr; n = 0;
     }
-    void upload(const T* h, size_t cnt, hipStream_t s, size_t off = 0)
-    {
-        if (cnt) hip_check(hipMemcpyAsync(p + off, h, cnt * sizeof(T), hipMemcpyHostToDevice, s), "hipMemcpyAsync H2D");
-    }
-    void download(T* h, size_t cnt, hipStream_t s, size_t off = 0) const
-    {
-        if (cnt) hip_check(hipMemcpyAsync(h, p + off, cnt * sizeof(T), hipMemcpyDeviceToHost, s), "hipMemcpyAsync D2H");
-    }
+    // from / into pageable host memory (through the bounce buffers): h is free again, resp. filled, on return
+    void upload(const T* h, size_t cnt, hipStream_t s, size_t off = 0) { copy_h2d(p + off, h, cnt * sizeof(T), s); }
+    void download(T* h, size_t cnt, hipStream_t s, size_t off = 0) const { copy_d2h(h, p + off, cnt * sizeof(T), s); }
 };
 
 inline void dev_sync(hipStream_t s) { hip_check(hipStreamSynchronize(s), "hipStreamSynchronize"); }

@@ -110,11 +110,14 @@ void x_back(int vs, int32_t ntask, const UpdTask* tasks, const int32_t* cols, co
 // here, with *counted the arrivals it holds), one workgroup for very tall panels
 // hout: the sub-panel's slot in page-locked host memory (a device address of it), where the kernel writes the pivots
 // of the sub-panel's columns and the status (err 0 once the sub-panel is factored): no copy back
+// counter: two ints, zero at first: [0] the barrier counter, [1] set (and never cleared) when a workgroup gave up at a
+// barrier; allow_coop false: the one-workgroup kernel for every sub-panel
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
            double pivot_rel, int32_t* piv, PanelStatus* st, PanelOut* hout, void* coop, int32_t* counter, int64_t* counted,
-           hipStream_t s);
+           bool allow_coop, hipStream_t s);
 size_t getf2_coop_bytes(int vs);
 int getf2_coop_max_wg();
+void set_getf2_coop_max_wg(int cap);   // at most 512; 0: the one-workgroup kernel for every sub-panel
 // (getf2's own second step after its one-workgroup form)
 void trsm_sub(int vs, double* Wl, int64_t ld, int32_t P, int32_t a, int32_t nb, const PanelStatus* st, hipStream_t s);
 void gemm_sub(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const PanelStatus* st,

@@ -635,6 +635,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             if (tid == 0) {
                 atomicExch(&st->err, -1);
                 hout->st.err = -1;
+                // counter[1]: a flag that is never cleared while the factorization runs -- the host reads it at the end,
+                // so a barrier given up anywhere fails the factorization even when the other workgroups went on
+                atomicExch(counter + 1, 1);
             }
             return;
         }
@@ -699,7 +702,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                 if (c > jj && c < nb) x[c] = Sc<T>::fms(x[c], l, pvals[c]);
         }
     }
-    if (g == 0 && tid == 0) hout->st.err = 0;   // the sub-panel is factored: its status for the host
+    // the sub-panel is factored: its status for the host (unless a workgroup gave up at a barrier meanwhile)
+    if (g == 0 && tid == 0 && __hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) hout->st.err = 0;
     if (have) {
 #pragma unroll
         for (int c = 0; c < PNB; c++)
@@ -922,7 +926,7 @@ void x_back(int vs, int32_t ntask, const UpdTask* tasks, const int32_t* cols, co
 }
 void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
            double pivot_rel, int32_t* piv, PanelStatus* st, PanelOut* hout, void* coop, int32_t* counter, int64_t* counted,
-           hipStream_t s)
+           bool allow_coop, hipStream_t s)
 {
     ProfScope ps("getf2", (vs == 1 ? 2.0 : 8.0) * (double)nr * nb * nb / 2, s);
     const int32_t G = (int32_t)((nr - a + TB - 1) / TB);
@@ -939,7 +943,7 @@ void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int
         cap = per * devctx().ncu;
         resident[vs == 1 ? 0 : 1].store(cap, std::memory_order_relaxed);
     }
-    if (G >= 1 && G <= std::min(cap, getf2_coop_max_wg()) && nb <= PNB) {
+    if (allow_coop && G >= 1 && G <= std::min(cap, getf2_coop_max_wg()) && nb <= PNB) {
         // the barrier counter keeps growing across launches (nb G arrivals each); it is cleared before it could overflow
         if (*counted + (int64_t)nb * G > ((int64_t)1 << 30)) {
             k::fill_i32(counter, 1, 0, s);
@@ -966,8 +970,12 @@ void getf2(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int
     else hipLaunchKernelGGL(k_getf2<Zd>, dim3(1), dim3(LU_TB), 0, s, Wl, ld, nr, P, a, nb, colmax, pivot_rel, piv, st, hout);
     trsm_sub(vs, Wl, ld, P, a, nb, st, s);
 }
-size_t getf2_coop_bytes(int vs) { return (size_t)2 * getf2_coop_max_wg() * (vs == 1 ? sizeof(Cand<double>) : sizeof(Cand<Zd>)) + 2 * PNB * 16 + 256; }
-int getf2_coop_max_wg() { return 512; }
+constexpr int COOP_MAX_WG = 512;   // the most workgroups of the cooperative panel LU (its candidate slots are sized for it)
+// the cap in use (a tuning knob: 0 forces the one-workgroup kernel)
+static std::atomic<int> coop_cap{COOP_MAX_WG};
+size_t getf2_coop_bytes(int vs) { return (size_t)2 * COOP_MAX_WG * (vs == 1 ? sizeof(Cand<double>) : sizeof(Cand<Zd>)) + 2 * PNB * 16 + 256; }
+int getf2_coop_max_wg() { return coop_cap.load(std::memory_order_relaxed); }
+void set_getf2_coop_max_wg(int cap) { coop_cap.store(std::max(0, std::min(cap, COOP_MAX_WG)), std::memory_order_relaxed); }
 void trsm_sub(int vs, double* Wl, int64_t ld, int32_t P, int32_t a, int32_t nb, const PanelStatus* st, hipStream_t s)
 {
     const int32_t nc = P - a - nb;

@@ -97,10 +97,10 @@ int ruiz_device(int64_t n, int64_t nnz, const int32_t* d_erow, const int32_t* d_
     int k4 = 0;
     {
         const unsigned long long init[2] = {0ULL, (unsigned long long)0x7FF0000000000000ULL};   // 0, +inf
-        hip_check(hipMemcpyAsync(amax, init, sizeof(init), hipMemcpyHostToDevice, s), "upload");
+        copy_h2d(amax, init, sizeof(init), s);
         k::absminmax(nnz * vs, d_x, amax, amin, s);
         unsigned long long hv[2];
-        hip_check(hipMemcpyAsync(hv, amax, sizeof(hv), hipMemcpyDeviceToHost, s), "download");
+        copy_d2h(hv, amax, sizeof(hv), s);
         dev_sync(s);
         double vmax, vmin;
         std::memcpy(&vmax, &hv[0], 8);
@@ -111,9 +111,9 @@ int ruiz_device(int64_t n, int64_t nnz, const int32_t* d_erow, const int32_t* d_
                 k::fill_u64(d_m.p, 2 * n, 0ULL, s);
                 k::line_absmax(vs, nnz, d_erow, d_ecol, d_x, d_m.p, n, s);
                 unsigned long long lo = ULLONG_MAX;
-                hip_check(hipMemcpyAsync(amin, &lo, 8, hipMemcpyHostToDevice, s), "upload");
+                copy_h2d(amin, &lo, 8, s);
                 k::min_positive(2 * n, d_m.p, amin, s);
-                hip_check(hipMemcpyAsync(&lo, amin, 8, hipMemcpyDeviceToHost, s), "download");
+                copy_d2h(&lo, amin, 8, s);
                 dev_sync(s);
                 double lmin;
                 std::memcpy(&lmin, &lo, 8);
@@ -696,7 +696,7 @@ void export_factors(const DevFactor& F, int is_complex, fplus_factors* f)
     for (size_t b = 0; b < F.blocks.size(); b++) {
         const BlockDesc& B = F.blocks[b];
         bv[b].resize((size_t)B.nrows * B.w * vs);
-        hip_check(hipMemcpyAsync(bv[b].data(), B.vals, bv[b].size() * sizeof(double), hipMemcpyDeviceToHost, s), "download L");
+        copy_d2h(bv[b].data(), B.vals, bv[b].size() * sizeof(double), s);
     }
     dev_sync(s);
     std::vector<int32_t> colblk(n > 0 ? n : 1);
@@ -791,6 +791,16 @@ int factor_core(fplus_numeric* num, const PatternCache& c, DevBuf<double>& d_Bx,
         printf("[FPLUS] LU factorisation start: n=%ld, nnz(A)=%ld, %s, supernodal left-looking (device), pivot=partial, relax_rows=%ld, panel=%ld\n",
                (long)n, (long)c.nnz, vs == 2 ? "complex128" : "float64", (long)relax_rows, (long)so.panel);
     rc = sn_factor_device(n, c.Bp.data(), c.Bi.data(), c.d_Bp.p, c.d_Bi.p, d_Bx.p, vs, so, num->F, err, errlen);
+    if (rc == FPLUS_ERR_INTERNAL && num->F.coop_aborted) {
+        // the cooperative panel LU needs its whole grid resident at once; when another process holds compute units it
+        // can give up at a barrier: the factorization runs again with the one-workgroup panel LU
+        if (verbose >= 1) printf("[FPLUS] the cooperative panel LU gave up at a barrier: factoring again with the one-workgroup panel LU\n");
+        dev_sync(s);
+        num->F = DevFactor();
+        so.coop_panel_lu = false;
+        rc = sn_factor_device(n, c.Bp.data(), c.Bi.data(), c.d_Bp.p, c.d_Bi.p, d_Bx.p, vs, so, num->F, err, errlen);
+        num->F.coop_retried = true;
+    }
     num->t_ruiz = t1 - t0;
     num->t_factor = now_s() - t1;
     if (rc == FPLUS_OK && verbose >= 1)
@@ -1022,7 +1032,21 @@ extern "C" int fplus_amd_numeric_stats(const fplus_numeric* num, fplus_amd_stats
     st->t_wait = F.t_wait;
     st->t_total = num->t_total;
     st->ruiz_iters = num->ruiz_iters;
+    st->shared_levels = F.shared_levels;
+    st->max_block_w = F.max_block_w;
+    st->coop_retried = F.coop_retried ? 1 : 0;
     return FPLUS_OK;
+}
+
+extern "C" int fplus_amd_tune(const char* name, int64_t value)
+{
+    if (!name) return FPLUS_ERR_BADARG;
+    if (std::strcmp(name, "panel_lu_max_workgroups") == 0) {
+        if (value < 0 || value > 512) return FPLUS_ERR_BADARG;
+        k::set_getf2_coop_max_wg((int)value);
+        return FPLUS_OK;
+    }
+    return FPLUS_ERR_BADARG;
 }
 
 extern "C" {
@@ -1064,8 +1088,9 @@ int fplus_amd_dev_copy(void* dst, const void* src, size_t bytes, int kind)
 {
     try {
         hipStream_t s = devctx().stream;
-        const hipMemcpyKind k2 = kind == 0 ? hipMemcpyHostToDevice : (kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice);
-        if (hipMemcpyAsync(dst, src, bytes, k2, s) != hipSuccess) return FPLUS_ERR_GPU;
+        if (kind == 0) copy_h2d(dst, src, bytes, s);
+        else if (kind == 1) copy_d2h(dst, src, bytes, s);
+        else if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) return FPLUS_ERR_GPU;
         return hipStreamSynchronize(s) == hipSuccess ? FPLUS_OK : FPLUS_ERR_GPU;
     } catch (const GpuError&) {
         return FPLUS_ERR_GPU;

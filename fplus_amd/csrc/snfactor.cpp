@@ -337,6 +337,7 @@ void Driver::add_block(int64_t s, int64_t b0, int64_t b1, int64_t j0, const std:
     B.w = (int32_t)w;
     B.c0 = (int32_t)b0;
     B.sn = (int32_t)s;
+    F_.max_block_w = std::max<int64_t>(F_.max_block_w, w);
     std::vector<int32_t> hr(nrows);
     const int64_t pos_off = (int64_t)expos.size();
     for (int64_t r = 0; r < nrows; r++) {
@@ -701,6 +702,7 @@ int Driver::panel(int64_t j0, int64_t j1)
         F_.lev_hist[bk].issued += lev_issued[L];
         F_.lev_hist[bk].maxm += lev_maxm[L];
     }
+    for (int32_t L = 1; L <= maxlevel; L++) F_.shared_levels += lev_ntask[L] > 1;
     F_.levels += maxlevel;
     F_.tasks += (int64_t)tasks.size();
     F_.t_tasks += now_s() - th_p1;
@@ -745,7 +747,7 @@ int Driver::panel(int64_t j0, int64_t j1)
         const int32_t nb = (int32_t)std::min<int64_t>(FPA_PANEL_NB, np - a);
         h_pout_.p[sp].st.err = INT32_MIN;   // (the kernel writes the slot; this marks it not written yet)
         k::getf2(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_colmax_.p, opt_.pivot_rel, d_piv, d_st,
-                 h_pout_dev_ + sp, d_coop_.p, d_coop_cnt_.p, &coop_counted_, s_);
+                 h_pout_dev_ + sp, d_coop_.p, d_coop_cnt_.p, &coop_counted_, opt_.coop_panel_lu, s_);
         k::gemm_sub(vs_, Wl, ldw, (int32_t)Rl, (int32_t)np, (int32_t)a, nb, d_st, s_);
         hip_check(hipEventRecord(sp_ev_.ev[sp], s_), "hipEventRecord");
     }
@@ -798,6 +800,7 @@ int Driver::panel(int64_t j0, int64_t j1)
             }
             if (st.err < 0) {
                 dev_sync(s_);
+                F_.coop_aborted = true;
                 errf(err_, errlen_, "internal: the cooperative panel LU's workgroups did not meet at a barrier");
                 return FPLUS_ERR_INTERNAL;
             }
@@ -1086,8 +1089,8 @@ int Driver::run()
     k::fill_i32(d_rowmap_.p, n, -1, s_);
     d_pout_.alloc(1);
     d_coop_.alloc(k::getf2_coop_bytes(vs_) / sizeof(double) + 1);
-    d_coop_cnt_.alloc(1);
-    k::fill_i32(d_coop_cnt_.p, 1, 0, s_);
+    d_coop_cnt_.alloc(2);
+    k::fill_i32(d_coop_cnt_.p, 2, 0, s_);
     coop_counted_ = 0;
     F_.d_blocks.alloc(nn);
     F_.val_chunks.clear();
@@ -1126,6 +1129,17 @@ int Driver::run()
         if (orc != FPLUS_OK) return orc;
     }
     dev_sync(s_);   // the last panel's blocks and U values are out
+    {
+        // a workgroup of the cooperative panel LU that gave up at a barrier leaves this flag, whatever the others did
+        int32_t gave_up = 0;
+        d_coop_cnt_.download(&gave_up, 1, s_, 1);
+        dev_sync(s_);
+        if (gave_up) {
+            F_.coop_aborted = true;
+            errf(err_, errlen_, "internal: the cooperative panel LU's workgroups did not meet at a barrier");
+            return FPLUS_ERR_INTERNAL;
+        }
+    }
     // the final structure (the inverse forms go with the driver: the solves use the blocks' values only)
     for (BlockDesc& B : F_.blocks) B.ivals = nullptr;
     F_.nsn = (int64_t)sn_.size();

@@ -52,6 +52,10 @@ struct DevFactor {
     // on the matrix cores, and the sum of each level's longest segment
     struct LevHist { int64_t levels = 0, tiles = 0; double issued = 0, maxm = 0; } lev_hist[13];
     int64_t rows_upper = 0, rows_lower = 0, rows_lower_max = 0;   // workspace rows summed over the panels
+    int64_t shared_levels = 0;   // dependency levels whose tasks can write the same rows (their updates are atomic)
+    int64_t max_block_w = 0;     // the widest block
+    bool coop_aborted = false;   // the cooperative panel LU gave up at a barrier (the factorization failed; retry without it)
+    bool coop_retried = false;   // this factorization is the retry with the one-workgroup panel LU
 };
 
 struct SnOptions {
@@ -59,6 +63,7 @@ struct SnOptions {
     double pivot_rel = 1.4901161193847656e-08;
     int64_t panel = 256;
     int verbose = 0;
+    bool coop_panel_lu = true;   // false: the one-workgroup panel LU for every sub-panel
 };
 
 // factor the n x n CSC matrix whose values are on the device (d_Bx, vs doubles per entry); the host holds its

@@ -328,8 +328,8 @@ int plan_solve(const fplus_solve_plan* pl, int64_t n, const int64_t* Lp, const v
     }
     // the right-hand sides: to the device (columns of n), passes of TS_RHS
     pl->d_b.reserve((size_t)(n * nrhs) * vs);
-    hip_check(hipMemcpy2DAsync(pl->d_b.p, (size_t)n * sz, xv, (size_t)ldx * sz, (size_t)n * sz, (size_t)nrhs, hipMemcpyHostToDevice, s),
-              "upload x");
+    for (int64_t r = 0; r < nrhs; r++)
+        copy_h2d(reinterpret_cast<char*>(pl->d_b.p) + (size_t)r * n * sz, static_cast<const char*>(xv) + (size_t)r * ldx * sz, (size_t)n * sz, s);
     pl->d_w.reserve((size_t)(n * TS_RHS) * vs);
     if (trans == 0 && pl->nup_rows > 0) pl->d_tmp.reserve((size_t)(pl->nup_rows * TS_RHS) * vs);
     for (int64_t r0 = 0; r0 < nrhs; r0 += TS_RHS) {
@@ -349,8 +349,8 @@ int plan_solve(const fplus_solve_plan* pl, int64_t n, const int64_t* Lp, const v
         }
         k::rhs_out(vs, n, R, pl->d_w.p, br, n, s);
     }
-    hip_check(hipMemcpy2DAsync(xv, (size_t)ldx * sz, pl->d_b.p, (size_t)n * sz, (size_t)n * sz, (size_t)nrhs, hipMemcpyDeviceToHost, s),
-              "download x");
+    for (int64_t r = 0; r < nrhs; r++)
+        copy_d2h(static_cast<char*>(xv) + (size_t)r * ldx * sz, reinterpret_cast<const char*>(pl->d_b.p) + (size_t)r * n * sz, (size_t)n * sz, s);
     dev_sync(s);
     return FPLUS_OK;
 }

@@ -27,6 +27,9 @@ typedef struct {
     double t_ruiz, t_factor, t_host, t_wait, t_total;
     int ruiz_iters;
     double gemm_bytes;
+    int64_t shared_levels;   /* dependency levels whose update tasks can write the same rows */
+    int64_t max_block_w;     /* the widest device block (columns) */
+    int coop_retried;        /* 1: the cooperative panel LU gave up at a barrier and the factorization ran again without it */
 } fplus_amd_stats;
 int fplus_amd_numeric_stats(const fplus_numeric *num, fplus_amd_stats *st);
 
@@ -43,6 +46,11 @@ int fplus_amd_dev_alloc(size_t bytes, void **ptr);
 int fplus_amd_dev_free(void *ptr);
 int fplus_amd_dev_copy(void *dst, const void *src, size_t bytes, int kind);
 int fplus_amd_dev_sync(void);
+
+/* Tuning knobs (process-wide; tests and benchmarks).  "panel_lu_max_workgroups": the most workgroups of the
+ * cooperative panel LU (at most 512, the default); 0 runs every sub-panel on the one-workgroup kernel.  Unknown names
+ * are FPLUS_ERR_BADARG. */
+int fplus_amd_tune(const char *name, int64_t value);
 
 /* Per-kernel timing with HIP events on the library's stream (benchmarks): on = 1 starts recording and clears the sums,
  * on = 0 stops.  fplus_amd_profile_get returns, for a kernel name, the summed milliseconds, the launches and the summed

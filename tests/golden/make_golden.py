@@ -8,7 +8,8 @@ reference tree is not on the GPU box):
     make -C oracle ref && python tests/golden/make_golden.py
 
 Large outputs are stored as digests (the pivot order, the partition, the patterns' sizes and per-column checksums)
-rather than in full.
+rather than in full; the mid-size case stores no matrix either (it is regenerated from its seed, and its SHA-256 is
+kept), only the orders it was factored with.  Name cases on the command line to write only those.
 """
 import os
 import sys
@@ -59,6 +60,11 @@ def cases():
     out.append(("shiftinvert_4x4x32", n, Ap, Ai, Ax, 32, "full"))
     n, Ap, Ai, Ax = gen.convdiff3d(8)
     out.append(("convdiff3d_8_auto", n, Ap, Ai, Ax, -1, "full"))
+    # a mid-size member of the complex shift-invert family (n = 16384, 64 panels of 256 columns on the device), with the
+    # orders of this build's own analysis stored beside the digests (the test factors with exactly these orders); the
+    # matrix itself is regenerated from the seeded generator and identified by its SHA-256
+    n, Ap, Ai, Ax = gen.shift_invert(16, 4, 256)
+    out.append(("shiftinvert_16x4x256", n, Ap, Ai, Ax, 32, "orders"))
     for name in ("relaxed_panel_case3", "relaxed_panel_case50"):
         z = np.load(os.path.join(HERE, name + ".npz"))
         A = sp.csr_matrix((z["data"], z["indices"], z["indptr"]), shape=(1200, 1200)).tocsc()
@@ -67,16 +73,28 @@ def cases():
     return out
 
 
-def main():
+def main(only=None):
     if not O.ref_available():
         sys.exit("oracle/_ref/libfplus_ref.so is not built: make -C oracle ref")
     for name, n, Ap, Ai, Ax, relax, kind in cases():
+        if only and name not in only:
+            continue
+        A0 = (Ap, Ai, Ax)
+        if kind == "orders":
+            import fplus_amd
+            sym = fplus_amd.Symbolic(n, Ap, Ai)
+            rp, cp = sym.row_perm.copy(), sym.col_perm.copy()
+            Ap, Ai, Ax = permuted(n, Ap, Ai, Ax, rp, cp)
         x, Dr, Dc = O.ref_ruiz(n, Ap, Ai, Ax)
         f = O.ref_factor_sn(n, Ap, Ai, x, relax_rows=relax)
         rng = np.random.default_rng(3)
         cplx = np.iscomplexobj(Ax)
         b = rng.standard_normal(n) + (1j * rng.standard_normal(n) if cplx else 0)
-        out = dict(n=n, Ap=Ap, Ai=Ai, Ax=Ax, relax=relax, Dr=Dr, Dc=Dc, p=f["p"], nsn=f["nsn"], Lp=f["Lp"], Up=f["Up"], b=b)
+        out = dict(n=n, relax=relax, Dr=Dr, Dc=Dc, p=f["p"], nsn=f["nsn"], Lp=f["Lp"], Up=f["Up"], b=b)
+        if kind == "orders":   # the matrix is regenerated by the test: its digest, and the orders used
+            out.update(digest=gen.digest(n, *A0), row_perm=rp, col_perm=cp)
+        else:
+            out.update(Ap=Ap, Ai=Ai, Ax=Ax)
         for tr in ((0, 1, 2) if cplx else (0, 1)):
             xs = np.array(b, copy=True)
             rc = O.ref().fplus_tri_solve(n, O._p64(f["Lp"]), O._p64(f["Li"]), f["Lx"].ctypes.data, O._p64(f["Up"]),
@@ -85,6 +103,18 @@ def main():
             out[f"x_trans{tr}"] = xs
         if kind == "full":
             out.update(xscaled=x, Li=f["Li"], Lx=f["Lx"], Ui=f["Ui"], Ux=f["Ux"])
+        elif kind == "orders":
+            # the solution of the original system A x = b through the reference's composition (src/numeric.c:88-131):
+            # c = Dr b[row_perm] in pivot order, z = U^-1 L^-1 c, x[col_perm] = Dc z
+            c = (Dr * b[rp])[f["p"]]
+            rc = O.ref().fplus_tri_solve(n, O._p64(f["Lp"]), O._p64(f["Li"]), f["Lx"].ctypes.data, O._p64(f["Up"]),
+                                        O._p64(f["Ui"]), f["Ux"].ctypes.data, c.ctypes.data, 0, int(cplx))
+            assert rc == 0
+            xf = np.empty_like(c)
+            xf[cp] = Dc * c
+            out.update(xfull=xf)
+            out.update(Lsum=col_checksums(f["Lp"], f["Lx"])[0], Labs=col_checksums(f["Lp"], f["Lx"])[1],
+                       Usum=col_checksums(f["Up"], f["Ux"])[0], Uabs=col_checksums(f["Up"], f["Ux"])[1])
         else:
             out.update(Lsum=col_checksums(f["Lp"], f["Lx"])[0], Labs=col_checksums(f["Lp"], f["Lx"])[1],
                        Usum=col_checksums(f["Up"], f["Ux"])[0], Uabs=col_checksums(f["Up"], f["Ux"])[1],
@@ -95,4 +125,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:] or None)   # names of cases to write (default: all)

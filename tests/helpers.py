@@ -10,9 +10,10 @@ REF_TEST_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden
 
 
 def sorted_csc(Lp, Li, Lx):
-    """per column: rows ascending, values along"""
+    """per column: rows ascending, values along (on copies: scipy keeps the caller's value array as its own data but may
+    convert the indices, so sorting in place would permute the caller's values against its indices)"""
     n = len(Lp) - 1
-    A = sp.csc_matrix((Lx, Li, Lp), shape=(n, n))
+    A = sp.csc_matrix((np.array(Lx, copy=True), np.array(Li, copy=True), np.array(Lp, copy=True)), shape=(n, n))
     A.sort_indices()
     return A
 

@@ -1,12 +1,19 @@
 """The path at the benchmark's own configurations on the device.
 
 - the CPU-runnable configuration (2D convection-diffusion on a 96 x 96 grid, n = 9216): full parity with the oracle
-  (pivots, supernodes, patterns bit-exact, values normwise to 1e-12, backward error < 1e-12 for both solutions;
-  the matrix's scaling over 12 orders makes the solutions themselves agree only to its conditioning);
+  (pivots, supernodes, patterns bit-exact, values normwise to 1e-12), and the solution of b = A x_true: backward errors
+  by the scale-invariant measures (fplus_amd.accuracy) below 1e-14 (equilibrated) and 1e-8 (componentwise; the
+  oracle's own scores 1.4e-16 and 2.0e-11), and the solution within 1e-2 of the
+  oracle's and of x_true (the matrix's scaling over 12 orders leaves it so ill-conditioned that two correct
+  factorizations -- the oracle's with relax 0 and with relax 32 -- give solutions 1e-3 apart, while scipy's splu,
+  without equilibration, is off by 0.5: that one fails every gate here but the reference tests' normwise measure,
+  1.3e-18);
 - the plain FP64 drop-in configuration (complex shift-invert pencil, 64 modes x 4 fields x 1024 coefficients,
   n = 262144) and the 3D convection-diffusion configuration (64^3, n = 262144): size-independent properties -- the
-  pivot order is a permutation, the factors reproduce every right-hand side to backward error < 1e-12 (with A, A^T and
-  A^H), and several right-hand sides at once equal one at a time bit for bit;
+  pivot order is a permutation, the factors reproduce every right-hand side (with A, A^T and A^H) to an equilibrated
+  backward error below 1e-14 and a componentwise one below 1e-6 (the reference's normwise measure is reported and kept
+  below 1e-12, but with row and column scales over 16 orders it would pass a 1% error too), and several right-hand
+  sides at once equal one at a time bit for bit;
 - the splu drop-in surface (SparseLU) against the oracle.
 """
 import time
@@ -17,16 +24,26 @@ import scipy.sparse as sp
 
 import fplus_amd
 import oracle as O
+from fplus_amd import accuracy as acc
 from fplus_amd import gen
 from helpers import norm_diff, rel_diff, same_pattern, sorted_csc
 
 pytestmark = pytest.mark.gpu
 
+# the gates of a direct FP64 solve: the oracle's own solutions of these families score at most 3.8e-16 equilibrated
+# and 6.7e-11 componentwise at n <= 9216 (partial pivoting is normwise, not componentwise, stable: the device's solves
+# of the n = 262144 pencil score 1.4e-7 componentwise, 3.6e-17 equilibrated)
+TOL_EQ = 1e-14
+TOL_COMP = 1e-6
 
-def _bwd(A, x, b, op):
-    nA = abs(A).sum(axis=0).max()
-    M = {"N": A, "T": A.T, "H": A.conj().T}[op]
-    return float(np.linalg.norm(M @ x - b) / (nA * np.linalg.norm(x) + np.linalg.norm(b)))
+
+def _check(A, x, b, Dr, Dc, trans, what):
+    m = acc.all_measures(A, x, b, Dr, Dc, trans)
+    print(f"{what}: normwise {m['normwise']:.2e}, componentwise {m['componentwise']:.2e}, equilibrated {m['equilibrated']:.2e}")
+    assert m["equilibrated"] < TOL_EQ, (what, m)
+    assert m["componentwise"] < TOL_COMP, (what, m)
+    assert m["normwise"] < 1e-12, (what, m)
+    return m
 
 
 def test_convdiff2d_96_parity():
@@ -49,9 +66,15 @@ def test_convdiff2d_96_parity():
     b = A @ xt                                  # one right-hand side b = A x_true, x_true ~ N(0, 1), seed 0
     x = num.solve(b)
     xo = ref.solve(b)
-    # rows and columns scaled over 12 orders: the solutions agree to the problem's conditioning, the backward errors
-    # of both are at round-off
-    assert _bwd(A, x, b, "N") < 1e-12 and _bwd(A, xo, b, "N") < 1e-12
+    mo = acc.all_measures(A, xo, b, ref.Dr, ref.Dc)
+    m = _check(A, x, b, num.Dr, num.Dc, 0, "convdiff2d-96")
+    print(f"convdiff2d-96, the oracle's solution: {mo}")
+    assert m["componentwise"] < 1e-8 and m["equilibrated"] < 1e-14
+    # the solutions agree to the problem's conditioning (the oracle's relax 0 and relax 32 solutions differ by 1e-3)
+    d = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    ft = np.linalg.norm(x - xt) / np.linalg.norm(xt)
+    print(f"convdiff2d-96: |x - x_oracle| / |x_oracle| = {d:.2e}, |x - x_true| / |x_true| = {ft:.2e}")
+    assert d < 1e-2 and ft < 1e-2
 
 
 @pytest.mark.parametrize("name", ["shiftinvert-64x4x1024", "convdiff3d-64"])
@@ -76,27 +99,26 @@ def test_full_size_properties(name):
     B = gen.rhs(n, 16, cplx)
     X = num.solve(B)
     step("solved 16 right-hand sides")
-    for k in range(16):
-        assert _bwd(A, X[:, k], B[:, k], "N") < 1e-12, f"right-hand side {k}"
+    _check(A, X, B, num.Dr, num.Dc, 0, f"{name} 16 right-hand sides")
     x3 = num.solve(B[:, 3])
     assert np.array_equal(x3.view(np.float64), np.ascontiguousarray(X[:, 3]).view(np.float64))   # 1 rhs = 16 at once
     for op, tr in (("T", 1), ("H", 2)):
         x = num.solve(B[:, 0], tr)
         step(f"solved with A^{op}")
-        assert _bwd(A, x, B[:, 0], op) < 1e-12, op
+        _check(A, x, B[:, 0], num.Dr, num.Dc, tr, f"{name} A^{op}")
 
 
 def test_sparselu_drop_in():
     n, Ap, Ai, Ax = gen.shift_invert(4, 4, 48, seed=9)
     A = sp.csc_matrix((Ax, Ai, Ap), shape=(n, n))
-    lu = fplus_amd.SparseLU(A, relax_rows=32)
+    lu = fplus_amd.SparseLU(A, relax_rows=32, colamd=True)
     assert lu.shape == (n, n) and lu.nnz == lu.L.nnz + lu.U.nnz
     assert sorted(lu.perm_r) == list(range(n)) and sorted(lu.perm_c) == list(range(n))
     b = gen.rhs(n, 2, True)
     for op in ("N", "T", "H"):
         x = lu.solve(b, trans=op)
         assert x.shape == (n, 2)
-        for k in range(2):
-            assert _bwd(A, x[:, k], b[:, k], op) < 1e-12, op
+        m = acc.all_measures(A, x, b, lu.Dr, lu.Dc, op)
+        assert m["equilibrated"] < TOL_EQ and m["componentwise"] < TOL_COMP, (op, m)
         x0 = lu.solve(b[:, 0], trans=op)
         assert x0.shape == (n,) and np.array_equal(x0, x[:, 0])
