@@ -12,11 +12,12 @@ import numpy as np
 import scipy.sparse as sp
 
 from ._lib import (FPLUS_ORDER_AMD_ATA, FPLUS_ORDER_AUTO, FPLUS_ORDER_COLAMD, FPLUS_PIVOT_PARTIAL,  # noqa: F401
-                   FPLUS_PIVOT_REL_DEFAULT, Factors, FplusError, Options, Stats, check, lib)
+                   FPLUS_PIVOT_REL_DEFAULT, AmdOptions, Factors, FplusError, Options, RefineInfo, RefineOptions, Stats,
+                   check, lib)
 
 __all__ = ["SparseLU", "sparse_lu_full_pivot", "lu_solve", "lu_residual", "clear_ordering_cache", "Symbolic", "Numeric",
-           "SolvePlan", "default_options", "FplusError", "ruiz", "factor_sn", "factor_sn_perm", "factor_sn_sym", "tri_solve",
-           "tune"]
+           "SolvePlan", "default_options", "amd_options", "FplusError", "ruiz", "factor_sn", "factor_sn_perm", "factor_sn_sym",
+           "tri_solve", "tune"]
 
 _I64P = C.POINTER(C.c_int64)
 _DP = C.POINTER(C.c_double)
@@ -78,10 +79,23 @@ class Symbolic:
             self.ptr = None
 
 
-class Numeric:
-    """fplus_factor on a symbolic analysis: the device factorization, its composed orders and scalings"""
+def amd_options(**kw):
+    """fplus_amd_options with the given fields (precision, refactor, static_pivots)"""
+    o = AmdOptions()
+    lib().fplus_amd_default_options(C.byref(o))
+    for k, v in kw.items():
+        if not hasattr(o, k):
+            raise TypeError(f"unknown option {k}")
+        setattr(o, k, v)
+    return o
 
-    def __init__(self, sym, Ap, Ai, Ax, options=None):
+
+class Numeric:
+    """fplus_factor on a symbolic analysis: the device factorization, its composed orders and scalings.  With
+    ``amd_options`` (fplus_amd_options: update precision, keeping what a refactorization replays) it is made by
+    fplus_amd_factor_ex."""
+
+    def __init__(self, sym, Ap, Ai, Ax, options=None, amd_options=None):
         self.sym = sym
         self.n = sym.n
         self.is_complex = bool(np.iscomplexobj(Ax))
@@ -91,14 +105,45 @@ class Numeric:
         self.ptr = C.c_void_p()
         err = C.create_string_buffer(512)
         o = options if options is not None else default_options()
-        rc = lib().fplus_factor(sym.ptr, _p64(Ap), _p64(Ai), _vp(self._Ax), int(self.is_complex), C.byref(o), C.byref(self.ptr),
-                                err, 512)
+        if amd_options is None:
+            rc = lib().fplus_factor(sym.ptr, _p64(Ap), _p64(Ai), _vp(self._Ax), int(self.is_complex), C.byref(o),
+                                    C.byref(self.ptr), err, 512)
+        else:
+            rc = lib().fplus_amd_factor_ex(sym.ptr, _p64(Ap), _p64(Ai), _vp(self._Ax), 0, int(self.is_complex), C.byref(o),
+                                           C.byref(amd_options), C.byref(self.ptr), err, 512)
         check(rc, err)
+        self._refresh()
+
+    def _refresh(self):
         n = self.n
         self.p = np.ctypeslib.as_array(lib().fplus_numeric_p(self.ptr), (n,)).copy()
         self.q = np.ctypeslib.as_array(lib().fplus_numeric_q(self.ptr), (n,)).copy()
         self.Dr = np.ctypeslib.as_array(lib().fplus_numeric_Dr(self.ptr), (n,)).copy()
         self.Dc = np.ctypeslib.as_array(lib().fplus_numeric_Dc(self.ptr), (n,)).copy()
+
+    def refactor(self, Ax, precision=0, static_pivots=0):
+        """fplus_amd_refactor: new values of the same pattern, factored in place in the recorded pivot order"""
+        Ax = np.ascontiguousarray(Ax, dtype=np.complex128 if self.is_complex else np.float64)
+        err = C.create_string_buffer(512)
+        check(lib().fplus_amd_refactor(self.ptr, _vp(Ax), 0, int(precision), int(static_pivots), err, 512), err)
+        self._Ax = Ax
+        self._refresh()
+
+    def solve_refined(self, b, max_outer=10, restart=8, tol=1e-14, inner_tol=1e-4, verbose=0):
+        """fplus_amd_solve_refined: (x, info dict) for A x = b, b of shape (n,) or (n, k)"""
+        b = np.asarray(b)
+        one = b.ndim == 1
+        dt = np.complex128 if self.is_complex else np.float64
+        x = np.array(b.reshape(self.n, -1), dtype=dt, order="F", copy=True)
+        ro = RefineOptions()
+        lib().fplus_amd_default_refine_options(C.byref(ro))
+        ro.max_outer, ro.restart, ro.tol, ro.inner_tol, ro.verbose = max_outer, restart, tol, inner_tol, verbose
+        info = RefineInfo()
+        err = C.create_string_buffer(512)
+        check(lib().fplus_amd_solve_refined(self.ptr, C.c_void_p(x.ctypes.data), 0, x.shape[1], self.n, C.byref(ro),
+                                            C.byref(info), err, 512), err)
+        out = {k: getattr(info, k) for k, _ in RefineInfo._fields_ if k != "reserved"}
+        return (x[:, 0] if one else x), out
 
     def solve(self, b, trans=0):
         """solve in place on a copy: trans 0 A x = b, 1 A^T x = b, 2 A^H x = b; b of shape (n,) or (n, k)"""

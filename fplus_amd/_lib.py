@@ -45,7 +45,25 @@ class Stats(C.Structure):
                 ("panels", C.c_int64), ("levels", C.c_int64), ("flops", C.c_double), ("gemm_flops", C.c_double),
                 ("t_ruiz", C.c_double), ("t_factor", C.c_double), ("t_host", C.c_double), ("t_wait", C.c_double),
                 ("t_total", C.c_double), ("ruiz_iters", C.c_int), ("gemm_bytes", C.c_double),
-                ("shared_levels", C.c_int64), ("max_block_w", C.c_int64), ("coop_retried", C.c_int)]
+                ("shared_levels", C.c_int64), ("max_block_w", C.c_int64), ("coop_retried", C.c_int), ("precision", C.c_int),
+                ("refactor_replaced", C.c_int64), ("refactor_min_ratio", C.c_double), ("replay_bytes", C.c_double)]
+
+
+class AmdOptions(C.Structure):
+    """fplus_amd_options (include/fplus_amd.h)"""
+    _fields_ = [("precision", C.c_int), ("refactor", C.c_int), ("static_pivots", C.c_int), ("reserved", C.c_int)]
+
+
+class RefineOptions(C.Structure):
+    """fplus_amd_refine_options (include/fplus_amd.h)"""
+    _fields_ = [("max_outer", C.c_int), ("restart", C.c_int), ("tol", C.c_double), ("inner_tol", C.c_double),
+                ("verbose", C.c_int), ("reserved", C.c_int)]
+
+
+class RefineInfo(C.Structure):
+    """fplus_amd_refine_info (include/fplus_amd.h)"""
+    _fields_ = [("outer", C.c_int), ("inner", C.c_int), ("converged", C.c_int), ("reserved", C.c_int),
+                ("bwd_first", C.c_double), ("bwd_last", C.c_double)]
 
 
 # every entry point include/fplus.h and include/fplus_amd.h declare, with its ctypes signature
@@ -108,6 +126,12 @@ SIGNATURES = {
     "fplus_amd_profile": (C.c_int, [C.c_int]),
     "fplus_amd_profile_get": (C.c_int, [C.c_char_p, DP, I64P, DP]),
     "fplus_amd_tune": (C.c_int, [C.c_char_p, C.c_int64]),
+    "fplus_amd_default_options": (None, [VP]),
+    "fplus_amd_factor_ex": (C.c_int, [VP, I64P, I64P, VP, C.c_int, C.c_int, C.POINTER(Options), VP, C.POINTER(VP), C.c_char_p,
+                                      C.c_size_t]),
+    "fplus_amd_refactor": (C.c_int, [VP, VP, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
+    "fplus_amd_default_refine_options": (None, [VP]),
+    "fplus_amd_solve_refined": (C.c_int, [VP, VP, C.c_int, C.c_int64, C.c_int64, VP, VP, C.c_char_p, C.c_size_t]),
 }
 
 _lib = None

@@ -185,6 +185,7 @@ namespace k {
 void fill_u64(unsigned long long* p, int64_t n, unsigned long long v, hipStream_t s);
 void fill_f64(double* p, int64_t n, double v, hipStream_t s);
 void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+void fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
 // dst[o] = src[map[o]] for entries of vs doubles
 void gather_vals(int vs, int64_t cnt, const int64_t* map, const double* src, double* dst, hipStream_t s);
 // Ruiz, one iteration in pieces (erow / ecol: row and column of every entry, int32)

@@ -46,14 +46,18 @@ struct GemmTile {
 
 // a tile of a task's update (blk_upd) with all its workgroup needs in one 64-byte read: rows [row0, row0 + tile rows)
 // of the block, the task's columns cols[cfirst .. cfirst + nc) (nc <= 64)
+#define FPA_TILE_SHARED 1   // another task of the launch may write the tile's rows below the triangle: atomic updates
 struct alignas(16) UpdTile {
     const double* iv;      // the block's inverse form (ld = nrows)
     const int32_t* rows;   // the block's rows
-    double* x;             // the task's solved segment from the tile's first column on: X[(row - k0) + c * (w - k0)]
+    int64_t xo;            // the task's solved segment from the tile's first column on, in scalars from the launch's X:
+                           // X[xo + (row - k0) + c * (w - k0)]
     int32_t nrows, w, k0, row0;
     int32_t cfirst, nc;
     int32_t spos0;         // the W row of the segment's first row (the segment's rows are consecutive in W)
-    int32_t shared;        // another task of the launch may write the tile's rows below the triangle: atomic updates
+    int32_t flags;         // FPA_TILE_SHARED
+    int32_t block;         // the block (its entry of the per-block precision marks)
+    int32_t pad;
 };
 static_assert(sizeof(UpdTile) == 64, "UpdTile is one 64-byte read");
 
@@ -74,6 +78,14 @@ struct PanelStatus {
 struct PanelOut {
     PanelStatus st;
     int32_t piv[FPA_PANEL_MAX];
+};
+
+// a refactorization's status: the first column (1 + global index) whose pivot fell below pivot_rel * colmax with
+// pivots not replaced, the pivots replaced (static pivoting), and min |pivot| / colmax over the columns (double bits)
+struct RefactorStatus {
+    int32_t err_col;
+    int32_t nreplaced;
+    unsigned long long min_ratio_bits;
 };
 
 // one block column copied out of the panel workspace: dst[r] = Wl[pos[pos_off + r] + ci * ldw], r < nrows; for a
@@ -100,8 +112,10 @@ void blk_uinv(int vs, int64_t njobs, const int2* jobs, const BlockDesc* blocks, 
 // the tiles of one dependency level: each task's solved segment into its slice of X, its rows below subtracted from W
 // through rowmap (atomically for tiles marked shared); tile_rows: 64, or 32 (tiles of half the rows, for levels with
 // few tiles)
+// X: the solved segments' buffer the tiles' xo index; prec_mode 0: every tile's product in FP64, 1: every tile's in
+// FP32, 2: by its block's mark blk_f32[tile.block] (nonzero: FP32), set on the device (e.g. by the precision selector)
 void blk_upd(int vs, int32_t ntile, const UpdTile* tiles, const int32_t* cols, const int32_t* rowmap, double* W, int64_t ldw,
-             int tile_rows, double flops, hipStream_t s);
+             int tile_rows, double flops, double* X, int prec_mode, const uint8_t* blk_f32, hipStream_t s);
 // the solved segments of all tasks back into W's rows
 void x_back(int vs, int32_t ntask, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks, double* W, int64_t ldw,
             const double* xbuf, hipStream_t s);
@@ -122,6 +136,10 @@ void set_getf2_coop_max_wg(int cap);   // at most 512; 0: the one-workgroup kern
 void trsm_sub(int vs, double* Wl, int64_t ld, int32_t P, int32_t a, int32_t nb, const PanelStatus* st, hipStream_t s);
 void gemm_sub(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const PanelStatus* st,
               hipStream_t s);
+// sub-panel [a, a + nb) of the panel LU with the rows already in pivot order (no exchanges), and the sub-panel's rows in
+// the columns right of it; mode 0: a small pivot is recorded in st, 1: replaced and counted (j0: the panel's first column)
+void lu_nopiv(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
+              double pivot_rel, int mode, int64_t j0, RefactorStatus* st, hipStream_t s);
 }  // namespace k
 
 }  // namespace fpa

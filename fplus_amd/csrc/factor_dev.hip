@@ -187,10 +187,42 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
     if (t < w) gstore(as_global(reinterpret_cast<T*>(B.uinv)), t + (int64_t)c * w, x);
 }
 
-// FP64 matrix cores: D(16x16) += A(16x4) B(4x16) per wavefront; lane l supplies A[l % 16][l / 16] and
-// B[l / 16][l % 16] and holds D[l / 16 + 4 v][l % 16], v < 4 (checked on gfx950)
+// Matrix cores: D(16x16) += A(16x4) B(4x16) per wavefront; lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16].
+// FP64 (v_mfma_f64_16x16x4f64): lane l holds D[l / 16 + 4 v][l % 16], v < 4; FP32 (v_mfma_f32_16x16x4f32): lane l holds
+// D[4 (l / 16) + v][l % 16] (both checked on gfx950)
 typedef double dx4 __attribute__((ext_vector_type(4)));
+typedef float fx4 __attribute__((ext_vector_type(4)));
 __device__ inline dx4 mfma16(double a, double b, dx4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ inline fx4 mfma16(float a, float b, fx4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+struct Zf {
+    float x, y;
+};
+
+// the arithmetic of a product tile: FP64 operands and accumulation, or FP32 (the operands rounded to FP32 when they are
+// staged in LDS, FP32 accumulation, the result widened to FP64 for the update of W)
+template <class T, bool F32> struct Ar {
+    using S = T;
+    using acc = dx4;
+    __device__ static S cvt(T v) { return v; }
+    __device__ static S zero() { return Sc<T>::zero(); }
+    // the column of its 16 x 16 part that lane `lane`'s accumulator element v holds (its row is lane % 16)
+    __device__ static int col(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <> struct Ar<double, true> {
+    using S = float;
+    using acc = fx4;
+    __device__ static S cvt(double v) { return (float)v; }
+    __device__ static S zero() { return 0.0f; }
+    __device__ static int col(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+template <> struct Ar<Zd, true> {
+    using S = Zf;
+    using acc = fx4;
+    __device__ static S cvt(Zd v) { Zf r; r.x = (float)v.x; r.y = (float)v.y; return r; }
+    __device__ static S zero() { Zf r; r.x = 0.0f; r.y = 0.0f; return r; }
+    __device__ static int col(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
 
 constexpr int TN = 64;           // columns per product tile
 constexpr int BLD = TN + 1;      // LDS row length of B: one element of padding moves each k row to other banks
@@ -207,39 +239,43 @@ template <class T, int TM> struct TileCfg {
     static constexpr int STG = 2 * KC * TM + 2 * KC * BLD;   // LDS scalars: two chunks of A[k][row] and B[k][col]
 };
 
-// C (TM x 64) = A (TM x [kb, ke)) B ([kb, ke) x 64) on the matrix cores (four real products per complex one): k in
-// chunks of 8 staged in LDS (two buffers), with the global loads two chunks ahead of the products where registers allow
-// (TileCfg::DEEP): chunk i + 2 is read into registers while chunk i multiplies and chunk i + 1 waits in the other
-// register set, so a load has two chunks' products to land; else one chunk ahead.  la(r, k) and lb(k, c) give the operands (zero outside the tile).  The products are formed
+// C (TM x 64) = A (TM x [kb, ke)) B ([kb, ke) x 64) on the matrix cores (four real products per complex one, or three):
+// k in chunks of 8 staged in LDS (two buffers), with the global loads two chunks ahead of the products where registers
+// allow (TileCfg::DEEP): chunk i + 2 is read into registers while chunk i multiplies and chunk i + 1 waits in the other
+// register set, so a load has two chunks' products to land; else one chunk ahead.  la(r, k) and lb(k, c) give the
+// operands (zero outside the tile) in FP64; F32 rounds them to FP32 when they are staged.  The products are formed
 // transposed (B^T A^T), so each lane ends with rows of C along lanes: out(r, c, v) is called for every element, and a
 // column's 16 consecutive rows come from 16 lanes -- the write-out needs no pass through LDS.  pre() runs once the
 // first chunk is staged, before the barrier that publishes it: LDS state the output needs is stored there, from loads
-// that were in flight with the first chunks'.
+// that were in flight with the first chunks'.  stg: LDS of TileCfg<T, TM>::STG scalars T.
 template <int N> struct IC { static constexpr int value = N; };
 
-template <class T, int TM, class LA, class LB, class OUT, class PRE>
+template <class T, int TM, bool F32, class LA, class LB, class OUT, class PRE>
 __device__ inline void tile_mfma(int32_t kb, int32_t ke, int32_t mrows, int32_t ncols, const LA& la, const LB& lb, const OUT& out,
                                  const PRE& pre, T* stg)
 {
     using Cf = TileCfg<T, TM>;
+    using AR = Ar<T, F32>;
+    using S = typename AR::S;
+    using ACC = typename AR::acc;
     constexpr bool CPLX = Sc<T>::vs == 2;
     constexpr int KC = Cf::KC;
-    T* const As = stg;
-    T* const Bs = stg + 2 * KC * TM;
+    S* const As = reinterpret_cast<S*>(stg);
+    S* const Bs = As + 2 * KC * TM;
     const int tid = threadIdx.x, lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wr = (wv / Cf::WC) * 32, wc = (wv % Cf::WC) * (16 * Cf::SJ);   // the wave's part of the tile
     // a wave whose part of the tile holds no used row or column (of mrows x ncols) skips its products: they are zero
     // and never written, and its matrix-core cycles go to other waves (finer skips, per 16 rows or columns inside a
     // wave's part, cost the complex tiles their registers)
     const bool on = mrows > wr && ncols > wc;
-    dx4 accr[2][Cf::SJ], acci[2][Cf::SJ], acc3[2][Cf::SJ];
+    ACC accr[2][Cf::SJ], acci[2][Cf::SJ], acc3[2][Cf::SJ];
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
         for (int j = 0; j < Cf::SJ; j++) {
-            accr[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
-            acci[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
-            acc3[i][j] = dx4{0.0, 0.0, 0.0, 0.0};
+            accr[i][j] = ACC{0, 0, 0, 0};
+            acci[i][j] = ACC{0, 0, 0, 0};
+            acc3[i][j] = ACC{0, 0, 0, 0};
         }
     // thread element e = tid + q * TB; A: row e % TM, k e / TM (rows contiguous); B: k e % KC, column e / KC
     T ra0[Cf::LPA], rb0[Cf::LPB], ra1[Cf::LPA], rb1[Cf::LPB];
@@ -264,12 +300,12 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, int32_t mrows, int32_t 
 #pragma unroll
         for (int q = 0; q < Cf::LPA; q++) {
             const int e = tid + q * TB;
-            As[(buf * KC + e / TM) * TM + e % TM] = kc + e / TM < ke ? ra[q] : Sc<T>::zero();
+            As[(buf * KC + e / TM) * TM + e % TM] = kc + e / TM < ke ? AR::cvt(ra[q]) : AR::zero();
         }
 #pragma unroll
         for (int q = 0; q < Cf::LPB; q++) {
             const int e = tid + q * TB;
-            Bs[(buf * KC + e % KC) * BLD + e / KC] = kc + e % KC < ke ? rb[q] : Sc<T>::zero();
+            Bs[(buf * KC + e % KC) * BLD + e / KC] = kc + e % KC < ke ? AR::cvt(rb[q]) : AR::zero();
         }
     };
     auto mma_n = [&](auto ni_c, auto nj_c, int buf) {
@@ -277,7 +313,7 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, int32_t mrows, int32_t 
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 4) {
             const int kk = ks + (lane >> 4);
-            T a[2], b[Cf::SJ];   // a: A[row lane % 16][k lane / 16]; b: B[k lane / 16][column lane % 16]
+            S a[2], b[Cf::SJ];   // a: A[row lane % 16][k lane / 16]; b: B[k lane / 16][column lane % 16]
 #pragma unroll
             for (int i = 0; i < NI; i++) a[i] = As[(buf * KC + kk) * TM + wr + i * 16 + (lane & 15)];
 #pragma unroll
@@ -349,7 +385,7 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, int32_t mrows, int32_t 
         mma(1);
     }
     }
-    // lane l holds C[row l % 16][column l / 16 + 4 v] of each 16 x 16 part
+    // lane l holds C[row l % 16][column AR::col(l, v)] of each 16 x 16 part
 #pragma unroll
     for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -357,10 +393,11 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, int32_t mrows, int32_t 
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 T val;
-                if constexpr (Cf::G3) val = zmake(accr[i][j][v] - acc3[i][j][v], accr[i][j][v] + acci[i][j][v]);
+                if constexpr (Cf::G3)
+                    val = zmake((double)accr[i][j][v] - (double)acc3[i][j][v], (double)accr[i][j][v] + (double)acci[i][j][v]);
                 else if constexpr (CPLX) val = zmake(accr[i][j][v], acci[i][j][v]);
                 else val = accr[i][j][v];
-                out(wr + i * 16 + (lane & 15), wc + j * 16 + (lane >> 4) + 4 * v, val);
+                out(wr + i * 16 + (lane & 15), wc + j * 16 + AR::col(lane, v), val);
             }
 }
 
@@ -384,7 +421,7 @@ __global__ __launch_bounds__(TB) void k_blk_ivgemm(const GemmTile* __restrict__ 
     auto out = [&](int r, int c, T v) {
         if (r < nr && c < nc) gstore(out_base, r + (int64_t)c * B.nrows, v);
     };
-    tile_mfma<T, TM>(tl.col0, B.w, nr, nc, la, lb, out, [] {}, stg);
+    tile_mfma<T, TM, false>(tl.col0, B.w, nr, nc, la, lb, out, [] {}, stg);
 }
 
 // one tile of a task's update, rows [row0, row0 + TM) of the block from the segment start k0 on: the product
@@ -393,9 +430,9 @@ __global__ __launch_bounds__(TB) void k_blk_ivgemm(const GemmTile* __restrict__ 
 // rows below are subtracted from W through rowmap, the relative-index scatter -- with atomics where another task of
 // the launch may write the same rows (tl.shared), else with plain loads and stores.
 
-template <class T, int TM>
+template <class T, int TM, bool F32>
 __device__ inline void upd_tile(const UpdTile& tl, const int32_t* __restrict__ cols, const int32_t* __restrict__ rowmap,
-                                double* __restrict__ W, int64_t ldw, T* stg, int32_t* rpos, int32_t* cidx)
+                                double* __restrict__ W, int64_t ldw, double* __restrict__ Xb, T* stg, int32_t* rpos, int32_t* cidx)
 {
     constexpr bool CPLX = Sc<T>::vs == 2;
     const int32_t m = tl.w - tl.k0, nr = min(TM, tl.nrows - tl.row0), nc = tl.nc;
@@ -419,10 +456,10 @@ __device__ inline void upd_tile(const UpdTile& tl, const int32_t* __restrict__ c
     for (int q = 0; q < Cf::LPB; q++) cofs[q] = (int64_t)cidx[(tid + q * TB) / Cf::KC] * ldw;
     auto la = [&](int r, int32_t k) { return gload(IV, (int64_t)k * tl.nrows + min(r, nr - 1)); };
     auto lb = [&](int32_t k, int, int q) { return S[k + cofs[q]]; };
-    const gptr<T> X = as_global(reinterpret_cast<T*>(tl.x) + (tl.row0 - tl.k0));
+    const gptr<T> X = as_global(reinterpret_cast<T*>(Xb) + tl.xo + (tl.row0 - tl.k0));
     T* Wt = reinterpret_cast<T*>(W);
     const int32_t rw = tl.w - tl.row0;   // tile rows from here on are below the block's triangle
-    const bool shared = tl.shared != 0;
+    const bool shared = (tl.flags & FPA_TILE_SHARED) != 0;
     auto out = [&](int r, int c, T v) {
         if (r >= nr || c >= nc) return;
         if (r < rw) {
@@ -443,19 +480,28 @@ __device__ inline void upd_tile(const UpdTile& tl, const int32_t* __restrict__ c
     };
     // IV's rows inside the triangle are rows of L11^-1, zero right of the diagonal: a tile whose rows all lie above w
     // needs k only up to its last row
-    tile_mfma<T, TM>(tl.k0, min(tl.w, tl.row0 + TM), nr, nc, la, lb, out, [&] { if (tid < TM) rpos[tid] = rp; }, stg);
+    tile_mfma<T, TM, F32>(tl.k0, min(tl.w, tl.row0 + TM), nr, nc, la, lb, out, [&] { if (tid < TM) rpos[tid] = rp; }, stg);
 }
 
-// one workgroup per tile of a dependency level (every task of the level in one launch)
-template <class T, int TM>
+// one workgroup per tile of a dependency level (every task of the level in one launch).  MODE 0: FP64 products; 1: FP32
+// products; 2: each tile takes the FP32 or the FP64 product by its block's mark (uniform over the workgroup)
+template <class T, int TM, int MODE>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(4))) void k_blk_upd(
     const UpdTile* __restrict__ tiles, const int32_t* __restrict__ cols, const int32_t* __restrict__ rowmap,
-    double* __restrict__ W, int64_t ldw)
+    double* __restrict__ W, int64_t ldw, double* __restrict__ X, const uint8_t* __restrict__ blk_f32)
 {
     __shared__ T stg[TileCfg<T, TM>::STG];
     __shared__ int32_t rpos[TM];
     __shared__ int32_t cidx[TN];
-    upd_tile<T, TM>(tiles[blockIdx.x], cols, rowmap, W, ldw, stg, rpos, cidx);
+    const UpdTile tl = tiles[blockIdx.x];
+    if constexpr (MODE == 1) {
+        upd_tile<T, TM, true>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx);
+    } else if constexpr (MODE == 2) {
+        if (blk_f32[tl.block]) upd_tile<T, TM, true>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx);
+        else upd_tile<T, TM, false>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx);
+    } else {
+        upd_tile<T, TM, false>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx);
+    }
 }
 
 // the solved segments back into W (one workgroup per task), for the U values copied out after the panel
@@ -774,7 +820,7 @@ __global__ __launch_bounds__(TB) void k_gemm_sub(double* __restrict__ Wd, int64_
     constexpr int CT = CTile<T>::v, TC = CT / 16;
     __shared__ T As[RT][PNB + 1];
     __shared__ T Bs[PNB][CT + 1];
-    if (st->err) return;
+    if (st && st->err) return;   // (st null: a refactorization, which records small pivots itself)
     T* Wl = reinterpret_cast<T*>(Wd);
     const int32_t r0 = a + nb + blockIdx.x * RT, c0 = a + nb + blockIdx.y * CT;
     if (r0 >= nr || c0 >= P) return;
@@ -817,6 +863,91 @@ __global__ __launch_bounds__(TB) void k_gemm_sub(double* __restrict__ Wd, int64_
 #pragma unroll
         for (int j = 0; j < TC; j++)
             if (tr + i < mr && tc + j < mc) Wl[(r0 + tr + i) + (int64_t)(c0 + tc + j) * ld] = Sc<T>::sub(cv[i][j], acc[i][j]);
+}
+
+// ---- the panel LU with the pivots known (a refactorization in the pivot order of an earlier factorization)
+//
+// The lower section's rows are already in pivot order, so sub-panel [a, a + nb) is an LU without row exchanges: every
+// workgroup factors the sub-panel's nb x nb diagonal block in LDS (the same operations in every workgroup, so the same
+// values), then its own rows below (L21 = A21 U11^-1, one row per thread, the sub-panel's nb values in registers);
+// workgroup 0 writes the diagonal block back and solves the block's rows in the columns right of the sub-panel
+// (U12 = L11^-1 A12, one column per thread).  The operations per entry are those of the pivoting kernels (divide by
+// the pivot, then subtract the multiple of the pivot row column by column).  A pivot whose modulus is below
+// pivot_rel * colmax of its column: mode 0 records the first such column in st (the refactorization fails and the
+// caller factors with pivoting); mode 1 replaces the pivot by pivot_rel * colmax with the pivot's phase (static
+// pivoting; an exact zero becomes that real value) and counts it.
+template <class T>
+__global__ __launch_bounds__(TB) void k_lu_nopiv(double* __restrict__ Wd, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb,
+                                                  const double* __restrict__ colmax, double pivot_rel, int mode, int64_t j0,
+                                                  RefactorStatus* __restrict__ st)
+{
+    __shared__ T D[PNB][PNB + 1];
+    T* Wl = reinterpret_cast<T*>(Wd);
+    const int tid = threadIdx.x, g = blockIdx.x;
+    {
+        const int r = tid % PNB, c = tid / PNB;   // TB = PNB * PNB: one element per thread
+        D[r][c] = (r < nb && c < nb) ? Wl[(a + r) + (int64_t)(a + c) * ld] : Sc<T>::zero();
+    }
+    __syncthreads();
+    for (int jj = 0; jj < nb; jj++) {
+        if (tid == 0) {
+            const T pv = D[jj][jj];
+            const double m = Sc<T>::absval(pv), thr = pivot_rel * colmax[a + jj];
+            if (!(m >= thr) || m == 0.0) {
+                if (mode == 1) {
+                    D[jj][jj] = m > 0.0 ? Sc<T>::scale(pv, thr / m) : Sc<T>::scale(Sc<T>::one(), thr > 0.0 ? thr : 1.0);
+                    if (g == 0) atomicAdd(&st->nreplaced, 1);
+                } else if (g == 0) {
+                    atomicCAS(&st->err_col, 0, (int32_t)(j0 + a + jj + 1));
+                }
+            }
+            if (g == 0 && m > 0.0 && colmax[a + jj] > 0.0) atomicMin(&st->min_ratio_bits, (unsigned long long)__double_as_longlong(m / colmax[a + jj]));
+        }
+        __syncthreads();
+        const int r = tid % PNB, c = tid / PNB;
+        if (r > jj && r < nb && c == jj) D[r][jj] = Sc<T>::div(D[r][jj], D[jj][jj]);
+        __syncthreads();
+        if (r > jj && r < nb && c > jj && c < nb) D[r][c] = Sc<T>::fms(D[r][c], D[r][jj], D[jj][c]);
+        __syncthreads();
+    }
+    // the rows below the sub-panel: workgroup g takes rows a + nb + g TB + tid
+    const int32_t row = a + nb + g * TB + tid;
+    if (row < nr) {
+        T x[PNB];
+#pragma unroll
+        for (int c = 0; c < PNB; c++) x[c] = c < nb ? Wl[row + (int64_t)(a + c) * ld] : Sc<T>::zero();
+#pragma unroll
+        for (int jj = 0; jj < PNB; jj++) {
+            if (jj >= nb) break;
+            const T l = Sc<T>::div(x[jj], D[jj][jj]);
+            x[jj] = l;
+#pragma unroll
+            for (int c = jj + 1; c < PNB; c++)
+                if (c < nb) x[c] = Sc<T>::fms(x[c], l, D[jj][c]);
+        }
+#pragma unroll
+        for (int c = 0; c < PNB; c++)
+            if (c < nb) Wl[row + (int64_t)(a + c) * ld] = x[c];
+    }
+    if (g != 0) return;
+    {
+        const int r = tid % PNB, c = tid / PNB;
+        if (r < nb && c < nb) Wl[(a + r) + (int64_t)(a + c) * ld] = D[r][c];
+    }
+    const int32_t c = a + nb + tid;   // one column per thread: the panel is at most TB columns wide
+    if (c < P) {
+        T y[PNB];
+#pragma unroll
+        for (int i = 0; i < PNB; i++) y[i] = i < nb ? Wl[(a + i) + (int64_t)c * ld] : Sc<T>::zero();
+#pragma unroll
+        for (int i = 1; i < PNB; i++) {
+#pragma unroll
+            for (int k = 0; k < i; k++) y[i] = Sc<T>::fms(y[i], D[i][k], y[k]);
+        }
+#pragma unroll
+        for (int i = 0; i < PNB; i++)
+            if (i < nb) Wl[(a + i) + (int64_t)c * ld] = y[i];
+    }
 }
 
 // one wavefront per run
@@ -904,17 +1035,26 @@ void blk_uinv(int vs, int64_t njobs, const int2* jobs, const BlockDesc* blocks, 
     if (vs == 1) hipLaunchKernelGGL(k_blk_uinv<double>, dim3((unsigned)njobs), dim3(TB), 0, s, jobs, blocks);
     else hipLaunchKernelGGL(k_blk_uinv<Zd>, dim3((unsigned)njobs), dim3(TB), 0, s, jobs, blocks);
 }
+template <class T, int TM>
+static void blk_upd_t(int mode, int32_t ntile, const UpdTile* tiles, const int32_t* cols, const int32_t* rowmap, double* W,
+                      int64_t ldw, double* X, const uint8_t* bf, hipStream_t s)
+{
+    if (mode == 1) hipLaunchKernelGGL((k_blk_upd<T, TM, 1>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw, X, bf);
+    else if (mode == 2) hipLaunchKernelGGL((k_blk_upd<T, TM, 2>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw, X, bf);
+    else hipLaunchKernelGGL((k_blk_upd<T, TM, 0>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw, X, bf);
+}
 void blk_upd(int vs, int32_t ntile, const UpdTile* tiles, const int32_t* cols, const int32_t* rowmap, double* W, int64_t ldw,
-             int tile_rows, double flops, hipStream_t s)
+             int tile_rows, double flops, double* X, int prec_mode, const uint8_t* blk_f32, hipStream_t s)
 {
     if (ntile <= 0) return;
-    ProfScope ps("blk_upd", flops, s);
+    if (prec_mode == 2 && !blk_f32) throw std::invalid_argument("blk_upd: per-block precision without the block marks");
+    ProfScope ps(prec_mode == 0 ? "blk_upd" : (prec_mode == 1 ? "blk_upd_f32" : "blk_upd_mix"), flops, s);
     if (tile_rows == 32) {
-        if (vs == 1) hipLaunchKernelGGL((k_blk_upd<double, 32>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw);
-        else hipLaunchKernelGGL((k_blk_upd<Zd, 32>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw);
+        if (vs == 1) blk_upd_t<double, 32>(prec_mode, ntile, tiles, cols, rowmap, W, ldw, X, blk_f32, s);
+        else blk_upd_t<Zd, 32>(prec_mode, ntile, tiles, cols, rowmap, W, ldw, X, blk_f32, s);
     } else {
-        if (vs == 1) hipLaunchKernelGGL((k_blk_upd<double, 64>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw);
-        else hipLaunchKernelGGL((k_blk_upd<Zd, 64>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw);
+        if (vs == 1) blk_upd_t<double, 64>(prec_mode, ntile, tiles, cols, rowmap, W, ldw, X, blk_f32, s);
+        else blk_upd_t<Zd, 64>(prec_mode, ntile, tiles, cols, rowmap, W, ldw, X, blk_f32, s);
     }
 }
 void x_back(int vs, int32_t ntask, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks, double* W, int64_t ldw,
@@ -976,6 +1116,15 @@ static std::atomic<int> coop_cap{COOP_MAX_WG};
 size_t getf2_coop_bytes(int vs) { return (size_t)2 * COOP_MAX_WG * (vs == 1 ? sizeof(Cand<double>) : sizeof(Cand<Zd>)) + 2 * PNB * 16 + 256; }
 int getf2_coop_max_wg() { return coop_cap.load(std::memory_order_relaxed); }
 void set_getf2_coop_max_wg(int cap) { coop_cap.store(std::max(0, std::min(cap, COOP_MAX_WG)), std::memory_order_relaxed); }
+void lu_nopiv(int vs, double* Wl, int64_t ld, int32_t nr, int32_t P, int32_t a, int32_t nb, const double* colmax,
+              double pivot_rel, int mode, int64_t j0, RefactorStatus* st, hipStream_t s)
+{
+    if (nb > PNB || P > TB || a + nb > nr) throw std::invalid_argument("lu_nopiv: sub-panel larger than the kernel takes");
+    ProfScope ps("lu_nopiv", (vs == 1 ? 2.0 : 8.0) * (double)nr * nb * nb / 2, s);
+    const unsigned g = (unsigned)std::max<int64_t>(1, ((int64_t)nr - a - nb + TB - 1) / TB);
+    if (vs == 1) hipLaunchKernelGGL(k_lu_nopiv<double>, dim3(g), dim3(TB), 0, s, Wl, ld, nr, P, a, nb, colmax, pivot_rel, mode, j0, st);
+    else hipLaunchKernelGGL(k_lu_nopiv<Zd>, dim3(g), dim3(TB), 0, s, Wl, ld, nr, P, a, nb, colmax, pivot_rel, mode, j0, st);
+}
 void trsm_sub(int vs, double* Wl, int64_t ld, int32_t P, int32_t a, int32_t nb, const PanelStatus* st, hipStream_t s)
 {
     const int32_t nc = P - a - nb;

@@ -10,6 +10,7 @@
 #include "solve_dev.h"
 #include "symbolic.h"
 #include "prof.h"
+#include "fplus_amd.h"
 
 #include <algorithm>
 #include <chrono>
@@ -179,54 +180,7 @@ int gpu_error(const std::exception& e, char* err, size_t errlen)
 
 }  // namespace
 
-// ---------------------------------------------------------------- the numeric object
-struct fplus_numeric {
-    int64_t n = 0;
-    int is_complex = 0;
-    std::vector<int64_t> p, q;
-    std::vector<double> Dr, Dc;
-    DevFactor F;
-    int ruiz_iters = 0;
-    // solve data on the device
-    DevBuf<int64_t> d_p, d_q, d_Up;
-    DevBuf<int32_t> d_pinv;
-    DevBuf<double> d_Dr, d_Dc;
-    std::vector<int32_t> lev_lf, lev_ub, lev_utf, lev_ltb;   // level offsets into the block lists
-    DevBuf<int32_t> d_blk_lf, d_blk_ub, d_blk_utf, d_blk_ltb;
-    std::vector<int32_t> blk_lf, blk_ub, blk_utf, blk_ltb;
-    // L y = b: every block's rows below have one contribution slot each; a row gathers the slots that hold its updates
-    DevBuf<int64_t> d_slot_off, d_incp, d_incs;
-    // the pivot rows of every L level's blocks (the rows the level gathers), level offsets
-    std::vector<int32_t> lev_gr;
-    DevBuf<int32_t> d_grows;
-    std::vector<double> bytes_lf_gather;
-    int64_t nslots = 0;
-    std::vector<int32_t> lev_lft, lev_ubt, lev_utt, lev_ltt;   // level offsets into the push / dot-product tiles
-    DevBuf<int2> d_lft, d_utt, d_ltt;
-    DevBuf<int2> d_ujobs;                                       // (block, column) jobs of the blocks' U11^-1
-    DevBuf<int4> d_ubt;        // U x = y dot-product tiles: block, first piece, pieces, partial slot
-    std::vector<int32_t> lev_ubr;
-    DevBuf<int4> d_ubr;        // and the partial sums' reduction tiles: block, first of 16 rows, first slot, slots
-    int64_t ub_slots_max = 0;  // the most partial slots of one level
-    std::vector<double> bytes_ub_prod;
-    std::vector<int32_t> lev_lpt, lev_upt, lev_utp, lev_ltp;   // level offsets into the triangle-product tiles
-    DevBuf<int2> d_lpt, d_upt, d_utp, d_ltp;                    // (block, first of 64 rows), per sweep
-    std::vector<double> bytes_lf_pull, bytes_lf_push, bytes_ub, bytes_utf, bytes_ltb;
-    // U x = y beyond the blocks: a block's pieces are the runs of later columns over its rows, each covering the block's
-    // rows a .. w-1 (a = pa[p]), in column order; piece p holds U(c0 + t, pj[p]) at Ux[pu[p] + t]
-    DevBuf<int64_t> d_pu;
-    DevBuf<int32_t> d_pj, d_pa;
-    // solve workspace, reused across calls (solves on one numeric object are serialized)
-    mutable std::mutex solve_mu;
-    mutable DevBuf<double> d_w, d_y, d_contrib, d_b, d_ub_part;
-    // the CSC export, built on first access
-    mutable std::mutex mu;
-    mutable bool exported = false;
-    mutable fplus_factors f{};
-    // timings of the last factorization
-    double t_ruiz = 0, t_factor = 0, t_total = 0, t_prepare = 0;
-    double t_prep_part[6] = {0, 0, 0, 0, 0, 0};   // the preparation by step: L levels, U pieces, U levels, slots, tiles, uploads
-};
+#include "numeric_obj.h"
 
 namespace {
 
@@ -253,6 +207,8 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
             ujobs.push_back(make_int2(b, c));
         }
     }
+    num->n_ujobs = (int64_t)ujobs.size();
+    num->uinv_flops = uflops;
     if (!ujobs.empty()) {
         num->d_ujobs.reserve(ujobs.size());
         num->d_ujobs.upload(ujobs.data(), ujobs.size(), s);
@@ -597,6 +553,9 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
     lap(5);
 }
 
+}  // namespace
+
+namespace fpa {
 // the sweeps on w (n x R row-major, pivot order, R <= solve_rhs_per_pass) in place
 void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, hipStream_t s)
 {
@@ -660,6 +619,10 @@ void solve_on_device(const fplus_numeric* num, double* b, int64_t nrhs, int64_t 
         else k::scatter_scale(vs, n, R, num->d_p.p, num->d_Dr.p, num->d_w.p, br, ldb, s);
     }
 }
+
+}  // namespace fpa
+
+namespace {
 
 // the factors as CSC arrays on the host (src/factor_sn.c:741-808 layout): L with the unit diagonal first, then the
 // supernode's later pivot rows, then its other rows ascending by pivot index; U as stored (segments, diagonal last)
@@ -769,7 +732,7 @@ int64_t panel_width(int64_t panel)
 // the common core: values on the device (d_Bx, permuted, canonical pattern cache c), Ruiz, the factorization
 int factor_core(fplus_numeric* num, const PatternCache& c, DevBuf<double>& d_Bx, int vs, int equilibrate, int ruiz_max_iter,
                 double ruiz_tol, int64_t relax_rows, int64_t panel, double pivot_rel, int verbose, DevBuf<double>& d_Drp,
-                DevBuf<double>& d_Dcp, char* err, size_t errlen)
+                DevBuf<double>& d_Dcp, const fplus_amd_options& ao, char* err, size_t errlen)
 {
     hipStream_t s = devctx().stream;
     const int64_t n = c.n;
@@ -787,6 +750,8 @@ int factor_core(fplus_numeric* num, const PatternCache& c, DevBuf<double>& d_Bx,
     so.pivot_rel = pivot_rel;
     so.panel = panel_width(panel);
     so.verbose = verbose;
+    so.precision = ao.precision;
+    so.record = ao.refactor != 0;
     if (verbose >= 1)
         printf("[FPLUS] LU factorisation start: n=%ld, nnz(A)=%ld, %s, supernodal left-looking (device), pivot=partial, relax_rows=%ld, panel=%ld\n",
                (long)n, (long)c.nnz, vs == 2 ? "complex128" : "float64", (long)relax_rows, (long)so.panel);
@@ -869,13 +834,24 @@ void fplus_numeric_free(fplus_numeric* num)
     delete num;
 }
 
+static int check_amd_options(const fplus_amd_options* ao, char* err, size_t errlen)
+{
+    if (ao->precision < 0 || ao->precision > 2) { errf(err, errlen, "precision must be 0 (FP64), 1 (FP32) or 2 (per supernode)"); return FPLUS_ERR_BADARG; }
+    if (ao->static_pivots < 0 || ao->static_pivots > 1) { errf(err, errlen, "static_pivots must be 0 or 1"); return FPLUS_ERR_BADARG; }
+    return FPLUS_OK;
+}
+
 static int factor_entry(const fplus_symbolic* sym, const int64_t* Ap, const int64_t* Ai, const void* Ax, bool device_values,
-                        int is_complex, const fplus_options* opt, fplus_numeric** out, char* err, size_t errlen)
+                        int is_complex, const fplus_options* opt, const fplus_amd_options* ao, fplus_numeric** out, char* err,
+                        size_t errlen)
 {
     *out = nullptr;
     fplus_options d;
     if (!opt) { fplus_default_options(&d); opt = &d; }
+    fplus_amd_options ad;
+    if (!ao) { fplus_amd_default_options(&ad); ao = &ad; }
     int rc = check_options(opt, err, errlen);
+    if (rc == FPLUS_OK) rc = check_amd_options(ao, err, errlen);
     if (rc != FPLUS_OK) return rc;
     const int64_t n = sym->n;
     if (Ap[n] != sym->nnz) {
@@ -908,10 +884,19 @@ static int factor_entry(const fplus_symbolic* sym, const int64_t* Ap, const int6
             src = d_Ax.p;
         }
         k::gather_vals(vs, nnz, c->d_src.p, src, d_Bx.p, s);
-        DevBuf<double> d_Drp, d_Dcp;
+        num->cache = c;
+        num->opt = *opt;
+        num->aopt = *ao;
+        num->rp = sym->row_perm;
+        num->cp = sym->col_perm;
+        DevBuf<double>& d_Drp = num->d_Drp;
+        DevBuf<double>& d_Dcp = num->d_Dcp;
         rc = factor_core(num.get(), *c, d_Bx, vs, opt->equilibrate, opt->ruiz_max_iter, opt->ruiz_tol, opt->relax_rows, opt->panel,
-                         opt->pivot_rel, opt->verbose, d_Drp, d_Dcp, err, errlen);
+                         opt->pivot_rel, opt->verbose, d_Drp, d_Dcp, *ao, err, errlen);
         if (rc != FPLUS_OK) return rc;
+        num->d_Abar = std::move(d_Bx);   // (equilibrated in place by Ruiz)
+        num->d_plu.alloc(n > 0 ? n : 1);
+        num->d_plu.upload(num->F.p.data(), n, s);
         std::vector<double> Drp(n), Dcp(n);
         d_Drp.download(Drp.data(), n, s);
         d_Dcp.download(Dcp.data(), n, s);
@@ -949,12 +934,13 @@ static int factor_entry(const fplus_symbolic* sym, const int64_t* Ap, const int6
 int fplus_factor(const fplus_symbolic* sym, const int64_t* Ap, const int64_t* Ai, const void* Ax, int is_complex,
                  const fplus_options* opt, fplus_numeric** out, char* err, size_t errlen)
 {
-    return factor_entry(sym, Ap, Ai, Ax, false, is_complex, opt, out, err, errlen);
+    return factor_entry(sym, Ap, Ai, Ax, false, is_complex, opt, nullptr, out, err, errlen);
 }
 
 int fplus_solve(const fplus_numeric* num, void* b, int64_t nrhs, int64_t ldb, int trans, char* err, size_t errlen)
 {
     const int64_t n = num->n;
+    if (!num->valid) { errf(err, errlen, "the last refactorization of this factorization failed: factor again"); return FPLUS_ERR_BADARG; }
     if (trans < 0 || trans > 2 || nrhs < 0 || ldb < n) { errf(err, errlen, "fplus_solve: bad trans, nrhs or ldb"); return FPLUS_ERR_BADARG; }
     if (nrhs == 0 || n == 0) return FPLUS_OK;
     const int vs = num->F.vs;
@@ -1035,6 +1021,104 @@ extern "C" int fplus_amd_numeric_stats(const fplus_numeric* num, fplus_amd_stats
     st->shared_levels = F.shared_levels;
     st->max_block_w = F.max_block_w;
     st->coop_retried = F.coop_retried ? 1 : 0;
+    st->precision = F.precision;
+    st->refactor_replaced = num->refactor_replaced;
+    st->refactor_min_ratio = num->refactor_min_ratio;
+    st->replay_bytes = F.replay ? (double)F.replay->device_bytes() : 0.0;
+    return FPLUS_OK;
+}
+
+extern "C" void fplus_amd_default_options(fplus_amd_options* o)
+{
+    std::memset(o, 0, sizeof(*o));
+}
+
+extern "C" int fplus_amd_factor_ex(const fplus_symbolic* sym, const int64_t* Ap, const int64_t* Ai, const void* Ax, int ax_on_device,
+                                   int is_complex, const fplus_options* opt, const fplus_amd_options* aopt, fplus_numeric** num,
+                                   char* err, size_t errlen)
+{
+    return factor_entry(sym, Ap, Ai, Ax, ax_on_device != 0, is_complex, opt, aopt, num, err, errlen);
+}
+
+extern "C" int fplus_amd_refactor(fplus_numeric* num, const void* Ax, int ax_on_device, int precision, int static_pivots, char* err,
+                                  size_t errlen)
+{
+    if (!num) { errf(err, errlen, "no factorization"); return FPLUS_ERR_BADARG; }
+    if (!num->F.replay) {
+        errf(err, errlen, "this factorization was made without fplus_amd_options.refactor = 1: nothing to replay");
+        return FPLUS_ERR_BADARG;
+    }
+    if (precision < 0 || precision > 2 || static_pivots < 0 || static_pivots > 1) {
+        errf(err, errlen, "precision must be 0, 1 or 2 and static_pivots 0 or 1");
+        return FPLUS_ERR_BADARG;
+    }
+    const PatternCache& c = *num->cache;
+    const int vs = num->F.vs;
+    const int64_t n = num->n, nnz = c.nnz;
+    if (!ax_on_device) {
+        const int rc = check_finite(nnz, Ax, num->is_complex, err, errlen);
+        if (rc != FPLUS_OK) return rc;
+    }
+    try {
+        std::lock_guard<std::mutex> g1(num->solve_mu);
+        std::lock_guard<std::mutex> g2(num->mu);
+        const double t0 = now_s();
+        hipStream_t s = devctx().stream;
+        DevBuf<double> d_Ax, d_Bx((size_t)(nnz > 0 ? nnz : 1) * vs);
+        const double* src = static_cast<const double*>(Ax);
+        if (!ax_on_device) {
+            d_Ax.alloc((size_t)(nnz > 0 ? nnz : 1) * vs);
+            d_Ax.upload(static_cast<const double*>(Ax), (size_t)nnz * vs, s);
+            src = d_Ax.p;
+        }
+        k::gather_vals(vs, nnz, c.d_src.p, src, d_Bx.p, s);
+        num->valid = false;
+        int rc = FPLUS_OK;
+        if (num->opt.equilibrate)
+            rc = ruiz_device(n, nnz, c.d_Bi.p, c.d_ecol.p, d_Bx.p, vs, num->opt.ruiz_max_iter, num->opt.ruiz_tol, num->d_Drp.p,
+                             num->d_Dcp.p, num->opt.verbose, &num->ruiz_iters, err, errlen);
+        if (rc != FPLUS_OK) return rc;
+        const double t1 = now_s();
+        int64_t replaced = 0;
+        double minr = 0;
+        rc = sn_refactor_device(num->F, c.d_Bp.p, c.d_Bi.p, d_Bx.p, num->opt.pivot_rel, static_pivots, precision, &replaced, &minr,
+                                err, errlen);
+        num->refactor_replaced = replaced;
+        num->refactor_min_ratio = minr;
+        if (rc != FPLUS_OK) return rc;
+        num->d_Abar = std::move(d_Bx);
+        std::vector<double> Drp(n), Dcp(n);
+        num->d_Drp.download(Drp.data(), n, s);
+        num->d_Dcp.download(Dcp.data(), n, s);
+        for (int64_t k2 = 0; k2 < n; k2++) {
+            num->Dr[num->rp[k2]] = Drp[k2];
+            num->Dc[num->cp[k2]] = Dcp[k2];
+        }
+        num->d_Dr.upload(num->Dr.data(), n, s);
+        num->d_Dc.upload(num->Dc.data(), n, s);
+        if (num->n_ujobs > 0) k::blk_uinv(vs, num->n_ujobs, num->d_ujobs.p, num->F.d_blocks.p, num->uinv_flops, s);
+        dev_sync(s);
+        if (num->exported) {
+            fplus_factors_free(&num->f);
+            num->exported = false;
+        }
+        num->valid = true;
+        num->t_ruiz = t1 - t0;
+        num->t_factor = now_s() - t1;
+        num->t_total = now_s() - t0;
+        if (num->opt.verbose >= 1)
+            printf("[FPLUS] refactorization in the recorded pivot order: %.3fs (Ruiz %.3fs), %s updates, %ld pivots replaced, "
+                   "smallest |pivot| / colmax %.3e\n", num->t_total, num->t_ruiz, precision == 0 ? "FP64" : (precision == 1 ? "FP32" : "per-supernode"),
+                   (long)replaced, minr);
+    } catch (const GpuError& e) {
+        return gpu_error(e, err, errlen);
+    } catch (const std::bad_alloc&) {
+        errf(err, errlen, "out of memory");
+        return FPLUS_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        errf(err, errlen, "%s", e.what());
+        return FPLUS_ERR_INTERNAL;
+    }
     return FPLUS_OK;
 }
 
@@ -1053,12 +1137,13 @@ extern "C" {
 int fplus_amd_factor_device(const fplus_symbolic* sym, const int64_t* Ap, const int64_t* Ai, const void* d_Ax, int is_complex,
                             const fplus_options* opt, fplus_numeric** num, char* err, size_t errlen)
 {
-    return factor_entry(sym, Ap, Ai, d_Ax, true, is_complex, opt, num, err, errlen);
+    return factor_entry(sym, Ap, Ai, d_Ax, true, is_complex, opt, nullptr, num, err, errlen);
 }
 
 int fplus_amd_solve_device(const fplus_numeric* num, void* d_b, int64_t nrhs, int64_t ldb, int trans, char* err, size_t errlen)
 {
     const int64_t n = num->n;
+    if (!num->valid) { errf(err, errlen, "the last refactorization of this factorization failed: factor again"); return FPLUS_ERR_BADARG; }
     if (trans < 0 || trans > 2 || nrhs < 0 || ldb < n) { errf(err, errlen, "fplus_amd_solve_device: bad trans, nrhs or ldb"); return FPLUS_ERR_BADARG; }
     if (nrhs == 0 || n == 0) return FPLUS_OK;
     const int vs = num->F.vs;

@@ -37,6 +37,10 @@ __global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
+__global__ void k_fill_u8(uint8_t* p, int64_t n, uint8_t v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
 
 template <int VS>
 __global__ void k_gather_vals(int64_t cnt, const int64_t* __restrict__ map, const double* __restrict__ src, double* __restrict__ dst)
@@ -181,6 +185,10 @@ void fill_f64(double* p, int64_t n, double v, hipStream_t s)
 void fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s)
 {
     if (n > 0) hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n)), dim3(TB), 0, s, p, n, v);
+}
+void fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_fill_u8, dim3(grid_for(n)), dim3(TB), 0, s, p, n, v);
 }
 void gather_vals(int vs, int64_t cnt, const int64_t* map, const double* src, double* dst, hipStream_t s)
 {

@@ -319,6 +319,8 @@ private:
     Arena<double> uinv_{nullptr, 0, 0, 0};
     Arena<int32_t> rows_{nullptr, 0, 0, 0};
     hipStream_t s_ = nullptr;
+    Replay* rec_ = nullptr;          // recording what a refactorization replays (F_.replay), or null
+    ReplayPanel rec_panel_{};        // the panel being recorded
     SideWorker side_;   // last: destroyed (joined) first, before the state its job uses
 };
 
@@ -683,10 +685,11 @@ int Driver::panel(int64_t j0, int64_t j1)
                         UpdTile& u = gtiles[gi++];
                         u.iv = B.ivals;
                         u.rows = B.rows;
-                        u.x = d_X_.p + (T.xoff + (int64_t)cc * m) * vs_;
+                        u.xo = T.xoff + (int64_t)cc * m;
                         u.nrows = B.nrows; u.w = B.w; u.k0 = T.k0; u.row0 = r0;
                         u.cfirst = T.col_off + cc; u.nc = std::min<int32_t>(CTG, T.ncol - cc);
-                        u.spos0 = T.spos0; u.shared = lev_ntask[L] > 1;
+                        u.spos0 = T.spos0; u.flags = lev_ntask[L] > 1 ? FPA_TILE_SHARED : 0;
+                        u.block = T.block; u.pad = 0;
                     }
             }
         });
@@ -703,6 +706,25 @@ int Driver::panel(int64_t j0, int64_t j1)
         F_.lev_hist[bk].maxm += lev_maxm[L];
     }
     for (int32_t L = 1; L <= maxlevel; L++) F_.shared_levels += lev_ntask[L] > 1;
+    if (rec_) {   // what a refactorization replays: the tasks, the tiles and the levels (in the order they run)
+        ReplayPanel P{};
+        P.j0 = j0; P.np = np; P.Ru = Ru; P.Rl = Rl; P.ldw = ldw; P.xtot = xtot;
+        P.cols_off = (int64_t)rec_->h_cols.size();
+        rec_->h_cols.insert(rec_->h_cols.end(), cols.begin(), cols.end());
+        P.tasks_off = (int64_t)rec_->h_tasks.size();
+        P.ntask = (int64_t)tasks.size();
+        rec_->h_tasks.insert(rec_->h_tasks.end(), tasks.begin(), tasks.end());
+        const int64_t tb = (int64_t)rec_->h_tiles.size();
+        rec_->h_tiles.insert(rec_->h_tiles.end(), gtiles, gtiles + ntile);
+        P.lev_off = (int64_t)rec_->levels.size();
+        for (int32_t L = 1; L <= maxlevel; L++)
+            if (lev_gt[L + 1] > lev_gt[L]) rec_->levels.push_back(ReplayLevel{tb + lev_gt[L], lev_gt[L + 1] - lev_gt[L], lev_rt[L], lev_flops[L]});
+        P.nlev = (int64_t)rec_->levels.size() - P.lev_off;
+        rec_->max_w = std::max(rec_->max_w, ldw * np * vs_);
+        rec_->max_x = std::max(rec_->max_x, xtot * vs_);
+        rec_->max_np = std::max(rec_->max_np, np);
+        rec_panel_ = P;
+    }
     F_.levels += maxlevel;
     F_.tasks += (int64_t)tasks.size();
     F_.t_tasks += now_s() - th_p1;
@@ -727,7 +749,8 @@ int Driver::panel(int64_t j0, int64_t j1)
         up_.send(d_tiles_.p, gtiles, (size_t)ntile, s_);
         for (int32_t L = 1; L <= maxlevel; L++) {
             const int32_t g0 = lev_gt[L], g1 = lev_gt[L + 1];
-            k::blk_upd(vs_, g1 - g0, d_tiles_.p + g0, d_cols_.p, d_rowmap_.p, d_W_.p, ldw, lev_rt[L], lev_flops[L], s_);
+            k::blk_upd(vs_, g1 - g0, d_tiles_.p + g0, d_cols_.p, d_rowmap_.p, d_W_.p, ldw, lev_rt[L], lev_flops[L], d_X_.p,
+                       opt_.precision, F_.d_blk_f32.p, s_);
         }
         k::x_back(vs_, (int32_t)tasks.size(), d_tasks_.p, d_cols_.p, d_blocks_p(), d_W_.p, ldw, d_X_.p, s_);
     }
@@ -966,6 +989,11 @@ int Driver::panel(int64_t j0, int64_t j1)
     out_.ldw = ldw;
     out_.u_begin = u_begin;
     out_.Wl = Wl;
+    if (rec_) {   // the refactorization's workspace rows: the U rows, then the lower rows in their final order
+        rec_panel_.rows_off = (int64_t)rec_->h_rows.size();
+        rec_->h_rows.insert(rec_->h_rows.end(), univ.begin(), univ.begin() + Ru);
+        rec_->h_rows.insert(rec_->h_rows.end(), lrows.begin(), lrows.end());
+    }
     for (int32_t row : univ) hrowpos_[row] = -1;   // (the next panel's workspace rows are placed meanwhile)
     out_.univ = std::move(univ);
     out_.lrows = std::move(lrows);
@@ -1048,6 +1076,25 @@ int Driver::panel_out()
     k::rowmap_set((int64_t)univ.size(), d_univ_.p, d_rowmap_.p, 1, s_);
     // the device finishes the panel while the host starts the next one (the uploads read page-locked staging)
     up_.end(s_);
+    if (rec_) {   // (rec_panel_ is the main thread's again only after it joined this job)
+        ReplayPanel& P = rec_panel_;
+        P.ex_off = (int64_t)rec_->h_ex.size();
+        P.nex = (int64_t)ex.size();
+        rec_->h_ex.insert(rec_->h_ex.end(), ex.begin(), ex.end());
+        P.expos_off = (int64_t)rec_->h_expos.size();
+        rec_->h_expos.insert(rec_->h_expos.end(), expos.begin(), expos.end());
+        P.ij_off = (int64_t)rec_->h_ij.size();
+        P.nij = (int64_t)ijobs.size();
+        rec_->h_ij.insert(rec_->h_ij.end(), ijobs.begin(), ijobs.end());
+        P.it_off = (int64_t)rec_->h_it.size();
+        P.nit = (int64_t)itiles.size();
+        rec_->h_it.insert(rec_->h_it.end(), itiles.begin(), itiles.end());
+        P.iflops = iflops;
+        P.runs_off = (int64_t)rec_->h_runs.size();
+        P.nruns = (int64_t)pruns.size();
+        rec_->h_runs.insert(rec_->h_runs.end(), pruns.begin(), pruns.end());
+        rec_->panels.push_back(P);
+    }
     for (int32_t row : lrows) lposmap_[row] = -1;
     F_.t_out += now_s() - th_out;
     F_.t_host += now_s() - th_out;
@@ -1093,6 +1140,13 @@ int Driver::run()
     k::fill_i32(d_coop_cnt_.p, 2, 0, s_);
     coop_counted_ = 0;
     F_.d_blocks.alloc(nn);
+    F_.d_blk_f32.alloc(nn);   // (a block has at least one column)
+    k::fill_u8(F_.d_blk_f32.p, (int64_t)nn, opt_.precision == 1 ? 1 : 0, s_);
+    F_.precision = opt_.precision;
+    if (opt_.record) {
+        F_.replay.reset(new Replay());
+        rec_ = F_.replay.get();
+    }
     F_.val_chunks.clear();
     F_.row_chunks.clear();
     vals_.chunks = &F_.val_chunks;
@@ -1140,8 +1194,14 @@ int Driver::run()
             return FPLUS_ERR_INTERNAL;
         }
     }
-    // the final structure (the inverse forms go with the driver: the solves use the blocks' values only)
-    for (BlockDesc& B : F_.blocks) B.ivals = nullptr;
+    // the final structure (the inverse forms go with the driver unless a refactorization will replay this one: the
+    // solves use the blocks' values only)
+    if (rec_) {
+        rec_->iv_chunks = std::move(iv_chunks_);
+        rec_->upload(s_);
+    } else {
+        for (BlockDesc& B : F_.blocks) B.ivals = nullptr;
+    }
     F_.nsn = (int64_t)sn_.size();
     F_.sn_first.resize(F_.nsn + 1);
     F_.sn_rows.resize(F_.nsn);
@@ -1174,6 +1234,109 @@ int sn_factor_device(int64_t n, const int64_t* Bp, const int64_t* Bi, const int6
 {
     Driver d(n, Bp, Bi, d_Bp, d_Bi, d_Bx, vs, opt, F, err, errlen);
     return d.run();
+}
+
+namespace {
+template <class T> void up_vec(DevBuf<T>& d, std::vector<T>& h, hipStream_t s)
+{
+    d.alloc(h.empty() ? 1 : h.size());
+    d.upload(h.data(), h.size(), s);
+    std::vector<T>().swap(h);
+}
+}  // namespace
+
+void Replay::upload(hipStream_t s)
+{
+    up_vec(d_rows, h_rows, s);
+    up_vec(d_cols, h_cols, s);
+    up_vec(d_expos, h_expos, s);
+    up_vec(d_tasks, h_tasks, s);
+    up_vec(d_tiles, h_tiles, s);
+    up_vec(d_ex, h_ex, s);
+    up_vec(d_ij, h_ij, s);
+    up_vec(d_it, h_it, s);
+    up_vec(d_runs, h_runs, s);
+}
+
+size_t Replay::device_bytes() const
+{
+    size_t b = d_rows.n * 4 + d_cols.n * 4 + d_expos.n * 4 + d_tasks.n * sizeof(UpdTask) + d_tiles.n * sizeof(UpdTile) +
+               d_ex.n * sizeof(ExtractCol) + d_ij.n * sizeof(int2) + d_it.n * sizeof(GemmTile) + d_runs.n * sizeof(URun);
+    for (const DevBuf<double>& c : iv_chunks) b += c.n * sizeof(double);
+    return b;
+}
+
+// The recorded factorization replayed on new values: per panel the workspace rows (lower ones in final pivot order),
+// the scatter of the panel's columns, the update levels, the solved segments back, the panel LU without row exchanges,
+// the new blocks and their inverse forms, U out -- the same launches as the factorization with pivoting, with none of
+// its host work and no wait for the device until the end.
+int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, const double* d_Bx, double pivot_rel,
+                       int static_pivots, int precision, int64_t* replaced, double* min_ratio, char* err, size_t errlen)
+{
+    if (!F.replay) {
+        errf(err, errlen, "the factorization was not recorded for refactorization");
+        return FPLUS_ERR_BADARG;
+    }
+    Replay& R = *F.replay;
+    const int vs = F.vs;
+    const int64_t n = F.n;
+    hipStream_t s = devctx().stream;
+    R.d_W.reserve((size_t)R.max_w);
+    R.d_X.reserve((size_t)R.max_x);
+    R.d_colmax.reserve((size_t)R.max_np);
+    if (R.d_rowmap.n < (size_t)std::max<int64_t>(n, 1)) {
+        R.d_rowmap.alloc((size_t)std::max<int64_t>(n, 1));
+        k::fill_i32(R.d_rowmap.p, n, -1, s);
+    }
+    if (!R.d_st.p) R.d_st.alloc(1);
+    RefactorStatus st0;
+    st0.err_col = 0;
+    st0.nreplaced = 0;
+    st0.min_ratio_bits = 0x7FF0000000000000ULL;   // +inf
+    R.d_st.upload(&st0, 1, s);
+    if (precision == 1) k::fill_u8(F.d_blk_f32.p, (int64_t)F.blocks.size(), 1, s);
+    else if (precision == 0) k::fill_u8(F.d_blk_f32.p, (int64_t)F.blocks.size(), 0, s);
+    const BlockDesc* blocks = F.d_blocks.p;
+    for (const ReplayPanel& P : R.panels) {
+        const int32_t* rows = R.d_rows.p + P.rows_off;
+        const int64_t nrow = P.Ru + P.Rl, ldw = P.ldw;
+        k::rowmap_set(nrow, rows, R.d_rowmap.p, 0, s);
+        k::fill_f64(R.d_W.p, ldw * P.np * vs, 0.0, s);
+        k::scatter_cols(vs, (int32_t)P.np, d_Bp, d_Bi, d_Bx, P.j0, R.d_rowmap.p, R.d_W.p, ldw, R.d_colmax.p, s);
+        const int32_t* cols = R.d_cols.p + P.cols_off;
+        for (int64_t L = P.lev_off; L < P.lev_off + P.nlev; L++) {
+            const ReplayLevel& V = R.levels[L];
+            k::blk_upd(vs, V.ntile, R.d_tiles.p + V.tile0, cols, R.d_rowmap.p, R.d_W.p, ldw, V.rt, V.flops, R.d_X.p, precision,
+                       F.d_blk_f32.p, s);
+        }
+        if (P.ntask) k::x_back(vs, (int32_t)P.ntask, R.d_tasks.p + P.tasks_off, cols, blocks, R.d_W.p, ldw, R.d_X.p, s);
+        double* Wl = R.d_W.p + P.Ru * vs;
+        for (int64_t a = 0; a < P.np; a += FPA_PANEL_NB) {
+            const int32_t nb = (int32_t)std::min<int64_t>(FPA_PANEL_NB, P.np - a);
+            k::lu_nopiv(vs, Wl, ldw, (int32_t)P.Rl, (int32_t)P.np, (int32_t)a, nb, R.d_colmax.p, pivot_rel, static_pivots, P.j0,
+                        R.d_st.p, s);
+            k::gemm_sub(vs, Wl, ldw, (int32_t)P.Rl, (int32_t)P.np, (int32_t)a, nb, nullptr, s);
+        }
+        if (P.nex) k::extract_cols(vs, (int32_t)P.nex, R.d_ex.p + P.ex_off, R.d_expos.p + P.expos_off, rows + P.Ru, Wl, ldw, s);
+        if (P.nij)
+            k::blk_inverse(vs, (int32_t)P.nij, R.d_ij.p + P.ij_off, (int32_t)P.nit, R.d_it.p + P.it_off, blocks, P.iflops, s);
+        if (P.nruns) k::u_runs(vs, (int32_t)P.nruns, R.d_runs.p + P.runs_off, R.d_W.p, F.d_Ux.p, F.d_Ui.p, s);
+        k::rowmap_set(nrow, rows, R.d_rowmap.p, 1, s);
+    }
+    RefactorStatus st;
+    R.d_st.download(&st, 1, s);
+    dev_sync(s);
+    F.precision = precision;
+    if (replaced) *replaced = st.nreplaced;
+    double mr;
+    std::memcpy(&mr, &st.min_ratio_bits, 8);
+    if (min_ratio) *min_ratio = mr;
+    if (st.err_col) {
+        errf(err, errlen, "refactorization in the recorded pivot order: the pivot of column %ld fell below pivot_rel times its "
+             "column's largest entry; factor with pivoting", (long)(st.err_col - 1));
+        return FPLUS_ERR_SINGULAR;
+    }
+    return FPLUS_OK;
 }
 
 }  // namespace fpa

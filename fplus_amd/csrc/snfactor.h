@@ -9,6 +9,52 @@
 
 namespace fpa {
 
+// What a same-pattern refactorization replays: every panel's device work of a factorization with pivoting, recorded as
+// it ran (the workspace rows with the lower ones in their final pivot order, the update tasks and tiles by dependency
+// level, the new blocks' columns and inverse-form jobs, the runs of U), kept on the device with the blocks' inverse
+// forms, so that new values with the same pattern factor in the same pivot order with no host work.
+struct ReplayLevel {
+    int64_t tile0;       // into the tiles
+    int32_t ntile, rt;   // tiles, tile rows
+    double flops;
+};
+struct ReplayPanel {
+    int64_t j0, np, Ru, Rl, ldw, xtot;
+    int64_t rows_off;              // Ru + Rl workspace rows: the U rows, then the lower rows in final (pivot) order
+    int64_t cols_off, tasks_off, ntask, lev_off, nlev;
+    int64_t ex_off, nex, expos_off;
+    int64_t ij_off, nij, it_off, nit;
+    double iflops;
+    int64_t runs_off, nruns;
+};
+struct Replay {
+    std::vector<ReplayPanel> panels;
+    std::vector<ReplayLevel> levels;
+    // filled while recording, uploaded once at the end of the factorization, then released
+    std::vector<int32_t> h_rows, h_cols, h_expos;
+    std::vector<UpdTask> h_tasks;
+    std::vector<UpdTile> h_tiles;
+    std::vector<ExtractCol> h_ex;
+    std::vector<int2> h_ij;
+    std::vector<GemmTile> h_it;
+    std::vector<URun> h_runs;
+    DevBuf<int32_t> d_rows, d_cols, d_expos;
+    DevBuf<UpdTask> d_tasks;
+    DevBuf<UpdTile> d_tiles;
+    DevBuf<ExtractCol> d_ex;
+    DevBuf<int2> d_ij;
+    DevBuf<GemmTile> d_it;
+    DevBuf<URun> d_runs;
+    std::vector<DevBuf<double>> iv_chunks;   // the blocks' inverse forms (rewritten by every refactorization)
+    int64_t max_w = 1, max_x = 1, max_np = 1;  // workspace, solved-segment buffer (scalars), widest panel
+    // a refactorization's workspace, kept across refactorizations
+    DevBuf<double> d_W, d_X, d_colmax;
+    DevBuf<int32_t> d_rowmap;
+    DevBuf<RefactorStatus> d_st;
+    void upload(hipStream_t s);
+    size_t device_bytes() const;
+};
+
 // the factorization as the device holds it, plus the host structure the solves and the CSC export need
 struct DevFactor {
     int64_t n = 0;
@@ -56,6 +102,9 @@ struct DevFactor {
     int64_t max_block_w = 0;     // the widest block
     bool coop_aborted = false;   // the cooperative panel LU gave up at a barrier (the factorization failed; retry without it)
     bool coop_retried = false;   // this factorization is the retry with the one-workgroup panel LU
+    DevBuf<uint8_t> d_blk_f32;   // per block: its updates of later columns on the FP32 matrix cores (precision mode 2)
+    std::unique_ptr<Replay> replay;   // kept when the factorization records what a refactorization replays
+    int precision = 0;           // the updates' precision of the last (re)factorization: 0 FP64, 1 FP32, 2 per block
 };
 
 struct SnOptions {
@@ -64,7 +113,16 @@ struct SnOptions {
     int64_t panel = 256;
     int verbose = 0;
     bool coop_panel_lu = true;   // false: the one-workgroup panel LU for every sub-panel
+    int precision = 0;           // the block updates: 0 FP64, 1 FP32, 2 per block by DevFactor::d_blk_f32
+    bool record = false;         // keep what a same-pattern refactorization replays (DevFactor::replay)
 };
+
+// a same-pattern refactorization: the values d_Bx (the recorded factorization's analysed, equilibrated pattern) factored
+// in the recorded pivot order by replaying its device work; static_pivots 0: a pivot below pivot_rel * colmax fails it
+// (FPLUS_ERR_SINGULAR, the caller factors with pivoting), 1: such pivots are replaced (counted in *replaced).
+// precision as in SnOptions.  min_ratio: the smallest |pivot| / colmax.
+int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, const double* d_Bx, double pivot_rel,
+                       int static_pivots, int precision, int64_t* replaced, double* min_ratio, char* err, size_t errlen);
 
 // factor the n x n CSC matrix whose values are on the device (d_Bx, vs doubles per entry); the host holds its
 // pattern (Bp, Bi); d_Bp / d_Bi are the same pattern on the device.  Throws GpuError / std::bad_alloc; returns

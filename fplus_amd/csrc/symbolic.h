@@ -16,6 +16,12 @@ struct PatternCache {
     std::vector<int64_t> Bp, Bi;
     DevBuf<int64_t> d_src, d_Bp;
     DevBuf<int32_t> d_Bi, d_ecol;
+    // the same pattern by rows (for the residuals of the refined solves), made on first use: row pointers, the column of
+    // each entry, and its position in the CSC arrays
+    std::mutex csr_mu;
+    bool have_csr = false;
+    DevBuf<int64_t> d_Rp, d_Rpos;
+    DevBuf<int32_t> d_Rj;
 };
 }  // namespace fpa
 

@@ -30,6 +30,10 @@ typedef struct {
     int64_t shared_levels;   /* dependency levels whose update tasks can write the same rows */
     int64_t max_block_w;     /* the widest device block (columns) */
     int coop_retried;        /* 1: the cooperative panel LU gave up at a barrier and the factorization ran again without it */
+    int precision;           /* the block updates of the last (re)factorization: 0 FP64, 1 FP32, 2 per supernode */
+    int64_t refactor_replaced;    /* the last refactorization: pivots replaced (static pivoting) */
+    double refactor_min_ratio;    /* and its smallest |pivot| / (largest entry of the pivot's column) */
+    double replay_bytes;          /* device bytes kept for refactorizations (schedule and inverse forms) */
 } fplus_amd_stats;
 int fplus_amd_numeric_stats(const fplus_numeric *num, fplus_amd_stats *st);
 
@@ -39,6 +43,65 @@ int fplus_amd_numeric_stats(const fplus_numeric *num, fplus_amd_stats *st);
 int fplus_amd_factor_device(const fplus_symbolic *sym, const int64_t *Ap, const int64_t *Ai, const void *d_Ax, int is_complex,
                             const fplus_options *opt, fplus_numeric **num, char *err, size_t errlen);
 int fplus_amd_solve_device(const fplus_numeric *num, void *d_b, int64_t nrhs, int64_t ldb, int trans, char *err, size_t errlen);
+
+/* Options of this build's factorization beyond fplus_options (fplus_amd_default_options: all zero).
+ *   precision      the block updates (the Schur-complement products of the supernodal LU): 0 FP64 (the drop-in's);
+ *                  1 the FP32 matrix cores (operands rounded to FP32, FP32 accumulation, the result subtracted in FP64);
+ *                  2 per supernode, by the marks the precision selector sets (fplus_amd_select).  Pivot selection, the
+ *                  panels' dense LU, Ruiz and the solves stay FP64.  A factorization made with FP32 updates is meant for
+ *                  fplus_amd_solve_refined, which wins the accuracy back.
+ *   refactor       1: keep what fplus_amd_refactor replays -- the device schedule of this factorization and the blocks'
+ *                  inverse forms (fplus_amd_stats.replay_bytes).
+ *   static_pivots  (used by fplus_amd_refactor) */
+typedef struct {
+    int precision;
+    int refactor;
+    int static_pivots;
+    int reserved;
+} fplus_amd_options;
+void fplus_amd_default_options(fplus_amd_options *o);
+
+/* fplus_factor with the options above; Ax in device memory when ax_on_device is 1 (the pattern Ap, Ai on the host). */
+int fplus_amd_factor_ex(const fplus_symbolic *sym, const int64_t *Ap, const int64_t *Ai, const void *Ax, int ax_on_device,
+                        int is_complex, const fplus_options *opt, const fplus_amd_options *aopt, fplus_numeric **num,
+                        char *err, size_t errlen);
+
+/* A same-pattern refactorization (new values Ax of the pattern num was factored with, e.g. the next shift of a
+ * shift-invert sweep), in place: Ruiz on the new values, then the recorded factorization replayed in its pivot order
+ * -- no pivot search, no host work per panel.  precision: as fplus_amd_options.precision.  static_pivots 0: a pivot whose
+ * modulus falls below pivot_rel times its column's largest entry fails the refactorization (FPLUS_ERR_SINGULAR; factor
+ * again with pivoting); 1: it is replaced by that bound with its phase (static pivoting; fplus_amd_stats counts them) and
+ * fplus_amd_solve_refined recovers the accuracy.  After a failure num is unusable until a refactorization succeeds.
+ * Needs a factorization made with fplus_amd_options.refactor = 1. */
+int fplus_amd_refactor(fplus_numeric *num, const void *Ax, int ax_on_device, int precision, int static_pivots, char *err,
+                       size_t errlen);
+
+/* Refined solves: A x = b by iterative refinement in FP64 around the factorization, each correction solved by GMRES
+ * preconditioned with the factors (GMRES-IR), in the equilibrated system Dr A Dc of the factorization -- for
+ * factorizations with FP32 updates, replaced pivots (static pivoting) or a refactorization in an earlier pivot order.  The
+ * residuals use the values the factorization was given (kept on the device).  Every right-hand side is refined until its
+ * equilibrated normwise backward error ||Dr (b - A x)|| / (||Dr A Dc||_1 ||x / Dc|| + ||Dr b||) is at most tol, or
+ * max_outer steps were taken.  b: n x nrhs column-major (leading dimension ldb), in host or device memory, overwritten
+ * with x.  Defaults (fplus_amd_default_refine_options): 10 steps, GMRES restart 8, tol 1e-14, inner_tol 1e-4. */
+typedef struct {
+    int max_outer;      /* refinement steps at most */
+    int restart;        /* GMRES iterations per step at most (1 to 32) */
+    double tol;         /* the target equilibrated backward error */
+    double inner_tol;   /* a step's GMRES stops once the preconditioned residual fell by this factor */
+    int verbose;
+    int reserved;
+} fplus_amd_refine_options;
+typedef struct {
+    int outer;          /* refinement steps taken, over all batches of right-hand sides (0: the first solve met tol) */
+    int inner;          /* GMRES iterations, each one triangular solve and one product with A */
+    int converged;      /* 1: every right-hand side met tol */
+    int reserved;
+    double bwd_first;   /* the largest equilibrated backward error after the first solve */
+    double bwd_last;    /* and at the end */
+} fplus_amd_refine_info;
+void fplus_amd_default_refine_options(fplus_amd_refine_options *o);
+int fplus_amd_solve_refined(const fplus_numeric *num, void *b, int b_on_device, int64_t nrhs, int64_t ldb,
+                            const fplus_amd_refine_options *ro, fplus_amd_refine_info *info, char *err, size_t errlen);
 
 /* Device memory and the library's stream for those callers: allocate, free, copy (kind 0 host to device, 1 device to
  * host, 2 device to device), wait for the library's stream. */
