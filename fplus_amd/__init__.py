@@ -17,7 +17,7 @@ from ._lib import (FPLUS_ORDER_AMD_ATA, FPLUS_ORDER_AUTO, FPLUS_ORDER_COLAMD, FP
 
 __all__ = ["SparseLU", "sparse_lu_full_pivot", "lu_solve", "lu_residual", "clear_ordering_cache", "Symbolic", "Numeric",
            "SolvePlan", "default_options", "amd_options", "FplusError", "ruiz", "factor_sn", "factor_sn_perm", "factor_sn_sym",
-           "tri_solve", "tune"]
+           "tri_solve", "tune", "Selector", "log_blocks"]
 
 _I64P = C.POINTER(C.c_int64)
 _DP = C.POINTER(C.c_double)
@@ -80,12 +80,16 @@ class Symbolic:
 
 
 def amd_options(**kw):
-    """fplus_amd_options with the given fields (precision, refactor, static_pivots)"""
+    """fplus_amd_options with the given fields (precision, refactor, static_pivots, selector: a Selector)"""
     o = AmdOptions()
     lib().fplus_amd_default_options(C.byref(o))
+    o._selector = None
     for k, v in kw.items():
         if not hasattr(o, k):
             raise TypeError(f"unknown option {k}")
+        if k == "selector":
+            o._selector = v          # kept alive with the options (and the factorization made with them)
+            v = v.ptr if v is not None else None
         setattr(o, k, v)
     return o
 
@@ -112,6 +116,7 @@ class Numeric:
             rc = lib().fplus_amd_factor_ex(sym.ptr, _p64(Ap), _p64(Ai), _vp(self._Ax), 0, int(self.is_complex), C.byref(o),
                                            C.byref(amd_options), C.byref(self.ptr), err, 512)
         check(rc, err)
+        self._selector = getattr(amd_options, "_selector", None)
         self._refresh()
 
     def _refresh(self):
@@ -143,6 +148,28 @@ class Numeric:
         check(lib().fplus_amd_solve_refined(self.ptr, C.c_void_p(x.ctypes.data), 0, x.shape[1], self.n, C.byref(ro),
                                             C.byref(info), err, 512), err)
         out = {k: getattr(info, k) for k, _ in RefineInfo._fields_ if k != "reserved"}
+        return (x[:, 0] if one else x), out
+
+    def refactor_solve(self, Ax, b, precision=2, max_outer=10, restart=8, tol=1e-14, inner_tol=1e-4):
+        """fplus_amd_refactor_solve: refactorization at `precision` (static pivoting) and a refined solve, again with
+        FP64 updates when the refinement stalls; returns (x, info dict with "fallback")"""
+        Ax = np.ascontiguousarray(Ax, dtype=np.complex128 if self.is_complex else np.float64)
+        b = np.asarray(b)
+        one = b.ndim == 1
+        dt = np.complex128 if self.is_complex else np.float64
+        x = np.array(b.reshape(self.n, -1), dtype=dt, order="F", copy=True)
+        ro = RefineOptions()
+        lib().fplus_amd_default_refine_options(C.byref(ro))
+        ro.max_outer, ro.restart, ro.tol, ro.inner_tol = max_outer, restart, tol, inner_tol
+        info = RefineInfo()
+        fb = C.c_int(0)
+        err = C.create_string_buffer(512)
+        check(lib().fplus_amd_refactor_solve(self.ptr, _vp(Ax), 0, C.c_void_p(x.ctypes.data), 0, x.shape[1], self.n,
+                                             int(precision), C.byref(ro), C.byref(info), C.byref(fb), err, 512), err)
+        self._Ax = Ax
+        self._refresh()
+        out = {k: getattr(info, k) for k, _ in RefineInfo._fields_ if k != "reserved"}
+        out["fallback"] = fb.value
         return (x[:, 0] if one else x), out
 
     def solve(self, b, trans=0):
@@ -369,4 +396,5 @@ def tri_solve(L, U, b, trans=0, plan=None):
     return X[:, 0] if one else X
 
 
+from .selector import Selector, log_blocks  # noqa: E402,F401
 from .splu import SparseLU, clear_ordering_cache, lu_residual, lu_solve, sparse_lu_full_pivot  # noqa: E402,F401

@@ -21,6 +21,7 @@ FPLUS_ORDER_COLAMD, FPLUS_ORDER_AMD_ATA, FPLUS_ORDER_AUTO = 0, 1, 2
 
 I64P = C.POINTER(C.c_int64)
 DP = C.POINTER(C.c_double)
+FP = C.POINTER(C.c_float)
 VP = C.c_void_p
 
 
@@ -46,12 +47,14 @@ class Stats(C.Structure):
                 ("t_ruiz", C.c_double), ("t_factor", C.c_double), ("t_host", C.c_double), ("t_wait", C.c_double),
                 ("t_total", C.c_double), ("ruiz_iters", C.c_int), ("gemm_bytes", C.c_double),
                 ("shared_levels", C.c_int64), ("max_block_w", C.c_int64), ("coop_retried", C.c_int), ("precision", C.c_int),
-                ("refactor_replaced", C.c_int64), ("refactor_min_ratio", C.c_double), ("replay_bytes", C.c_double)]
+                ("refactor_replaced", C.c_int64), ("refactor_min_ratio", C.c_double), ("replay_bytes", C.c_double),
+                ("f32_blocks", C.c_int64), ("f32_flop_share", C.c_double)]
 
 
 class AmdOptions(C.Structure):
     """fplus_amd_options (include/fplus_amd.h)"""
-    _fields_ = [("precision", C.c_int), ("refactor", C.c_int), ("static_pivots", C.c_int), ("reserved", C.c_int)]
+    _fields_ = [("precision", C.c_int), ("refactor", C.c_int), ("static_pivots", C.c_int), ("reserved", C.c_int),
+                ("selector", C.c_void_p)]
 
 
 class RefineOptions(C.Structure):
@@ -132,6 +135,16 @@ SIGNATURES = {
     "fplus_amd_refactor": (C.c_int, [VP, VP, C.c_int, C.c_int, C.c_int, C.c_char_p, C.c_size_t]),
     "fplus_amd_default_refine_options": (None, [VP]),
     "fplus_amd_solve_refined": (C.c_int, [VP, VP, C.c_int, C.c_int64, C.c_int64, VP, VP, C.c_char_p, C.c_size_t]),
+    "fplus_amd_refactor_solve": (C.c_int, [VP, VP, C.c_int, VP, C.c_int, C.c_int64, C.c_int64, C.c_int, VP, VP,
+                                           C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
+    "fplus_amd_selector_create": (C.c_int, [C.c_uint64, C.POINTER(VP)]),
+    "fplus_amd_selector_free": (None, [VP]),
+    "fplus_amd_selector_train": (C.c_int, [VP, FP, FP, C.c_int64, C.c_int, C.c_float, C.c_uint64, FP]),
+    "fplus_amd_selector_predict": (C.c_int, [VP, FP, C.c_int64, FP]),
+    "fplus_amd_selector_set_threshold": (C.c_int, [VP, C.c_float]),
+    "fplus_amd_selector_get_weights": (C.c_int, [VP, FP, C.c_int64]),
+    "fplus_amd_selector_set_weights": (C.c_int, [VP, FP, C.c_int64]),
+    "fplus_amd_selector_log": (C.c_int, [VP, FP, DP, DP, C.c_int64, C.POINTER(C.c_int64)]),
 }
 
 _lib = None

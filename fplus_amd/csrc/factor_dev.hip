@@ -430,9 +430,10 @@ __global__ __launch_bounds__(TB) void k_blk_ivgemm(const GemmTile* __restrict__ 
 // rows below are subtracted from W through rowmap, the relative-index scatter -- with atomics where another task of
 // the launch may write the same rows (tl.shared), else with plain loads and stores.
 
-template <class T, int TM, bool F32>
+template <class T, int TM, bool F32, bool LOG = false>
 __device__ inline void upd_tile(const UpdTile& tl, const int32_t* __restrict__ cols, const int32_t* __restrict__ rowmap,
-                                double* __restrict__ W, int64_t ldw, double* __restrict__ Xb, T* stg, int32_t* rpos, int32_t* cidx)
+                                double* __restrict__ W, int64_t ldw, double* __restrict__ Xb, T* stg, int32_t* rpos, int32_t* cidx,
+                                typename Ar<T, true>::S* p32 = nullptr, unsigned long long* blk_err = nullptr)
 {
     constexpr bool CPLX = Sc<T>::vs == 2;
     const int32_t m = tl.w - tl.k0, nr = min(TM, tl.nrows - tl.row0), nc = tl.nc;
@@ -480,21 +481,59 @@ __device__ inline void upd_tile(const UpdTile& tl, const int32_t* __restrict__ c
     };
     // IV's rows inside the triangle are rows of L11^-1, zero right of the diagonal: a tile whose rows all lie above w
     // needs k only up to its last row
-    tile_mfma<T, TM, F32>(tl.k0, min(tl.w, tl.row0 + TM), nr, nc, la, lb, out, [&] { if (tid < TM) rpos[tid] = rp; }, stg);
+    const int32_t ke = min(tl.w, tl.row0 + TM);
+    if constexpr (!LOG) {
+        tile_mfma<T, TM, F32>(tl.k0, ke, nr, nc, la, lb, out, [&] { if (tid < TM) rpos[tid] = rp; }, stg);
+    } else {
+        // the logging form, for training the precision selector: the FP32 product first (kept in LDS, not applied),
+        // then the FP64 one (applied); the largest difference and the largest FP64 value go to the block's entries of
+        // blk_err (non-negative doubles as bits, max-accumulated)
+        using AR = Ar<T, true>;
+        auto out32 = [&](int r, int c, T v) {
+            if (r < nr && c < nc) p32[r * TN + c] = AR::cvt(v);
+        };
+        tile_mfma<T, TM, true>(tl.k0, ke, nr, nc, la, lb, out32, [&] { if (tid < TM) rpos[tid] = rp; }, stg);
+        __syncthreads();
+        double dmax = 0.0, vmax = 0.0;
+        auto out64 = [&](int r, int c, T v) {
+            if (r >= nr || c >= nc) return;
+            const typename AR::S q = p32[r * TN + c];
+            T qd;
+            if constexpr (CPLX) qd = zmake(q.x, q.y);
+            else qd = (double)q;
+            dmax = fmax(dmax, Sc<T>::absval(Sc<T>::sub(v, qd)));
+            vmax = fmax(vmax, Sc<T>::absval(v));
+            out(r, c, v);
+        };
+        tile_mfma<T, TM, false>(tl.k0, ke, nr, nc, la, lb, out64, [] {}, stg);
+        for (int off = 32; off > 0; off >>= 1) {
+            dmax = fmax(dmax, __shfl_xor(dmax, off));
+            vmax = fmax(vmax, __shfl_xor(vmax, off));
+        }
+        if ((tid & 63) == 0) {
+            atomic_max_nonneg(blk_err + 2 * (int64_t)tl.block, dmax);
+            atomic_max_nonneg(blk_err + 2 * (int64_t)tl.block + 1, vmax);
+        }
+    }
 }
 
 // one workgroup per tile of a dependency level (every task of the level in one launch).  MODE 0: FP64 products; 1: FP32
-// products; 2: each tile takes the FP32 or the FP64 product by its block's mark (uniform over the workgroup)
+// products; 2: each tile takes the FP32 or the FP64 product by its block's mark (uniform over the workgroup); 3: FP64
+// products applied, the FP32 ones computed beside them and their differences logged per block
 template <class T, int TM, int MODE>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(4))) void k_blk_upd(
     const UpdTile* __restrict__ tiles, const int32_t* __restrict__ cols, const int32_t* __restrict__ rowmap,
-    double* __restrict__ W, int64_t ldw, double* __restrict__ X, const uint8_t* __restrict__ blk_f32)
+    double* __restrict__ W, int64_t ldw, double* __restrict__ X, const uint8_t* __restrict__ blk_f32,
+    unsigned long long* __restrict__ blk_err)
 {
     __shared__ T stg[TileCfg<T, TM>::STG];
     __shared__ int32_t rpos[TM];
     __shared__ int32_t cidx[TN];
     const UpdTile tl = tiles[blockIdx.x];
-    if constexpr (MODE == 1) {
+    if constexpr (MODE == 3) {
+        __shared__ typename Ar<T, true>::S p32[TM * TN];
+        upd_tile<T, TM, false, true>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx, p32, blk_err);
+    } else if constexpr (MODE == 1) {
         upd_tile<T, TM, true>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx);
     } else if constexpr (MODE == 2) {
         if (blk_f32[tl.block]) upd_tile<T, TM, true>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx);
@@ -882,33 +921,47 @@ __global__ __launch_bounds__(TB) void k_lu_nopiv(double* __restrict__ Wd, int64_
                                                   RefactorStatus* __restrict__ st)
 {
     __shared__ T D[PNB][PNB + 1];
+    __shared__ double cmx[PNB];
     T* Wl = reinterpret_cast<T*>(Wd);
     const int tid = threadIdx.x, g = blockIdx.x;
-    {
-        const int r = tid % PNB, c = tid / PNB;   // TB = PNB * PNB: one element per thread
-        D[r][c] = (r < nb && c < nb) ? Wl[(a + r) + (int64_t)(a + c) * ld] : Sc<T>::zero();
-    }
+    const int dr = tid % PNB, dc = tid / PNB;   // TB = PNB * PNB: one element of the diagonal block per thread
+    D[dr][dc] = (dr < nb && dc < nb) ? Wl[(a + dr) + (int64_t)(a + dc) * ld] : Sc<T>::zero();
+    if (tid < PNB) cmx[tid] = tid < nb ? colmax[a + tid] : 0.0;
     __syncthreads();
+    // workgroup 0's statistics, kept in thread 0 and posted once after the loop (a barrier waits for a thread's
+    // memory operations in flight: an atomic per column would put its round trip on every step)
+    int32_t n_small = 0, first_small = -1;
+    double min_ratio = INFINITY;
     for (int jj = 0; jj < nb; jj++) {
+        // every thread takes the pivot (replaced by the same rule in all of them) and its operands before any write
+        T pv = D[jj][jj];
+        const double m = Sc<T>::absval(pv), thr = pivot_rel * cmx[jj];
+        const bool small = !(m >= thr) || m == 0.0;
+        if (small && mode == 1) pv = m > 0.0 ? Sc<T>::scale(pv, thr / m) : Sc<T>::scale(Sc<T>::one(), thr > 0.0 ? thr : 1.0);
+        const bool below = dr > jj && dr < nb, right = dc > jj && dc < nb;
+        const T drj = D[dr][jj], djc = D[jj][dc];
+        __syncthreads();
         if (tid == 0) {
-            const T pv = D[jj][jj];
-            const double m = Sc<T>::absval(pv), thr = pivot_rel * colmax[a + jj];
-            if (!(m >= thr) || m == 0.0) {
-                if (mode == 1) {
-                    D[jj][jj] = m > 0.0 ? Sc<T>::scale(pv, thr / m) : Sc<T>::scale(Sc<T>::one(), thr > 0.0 ? thr : 1.0);
-                    if (g == 0) atomicAdd(&st->nreplaced, 1);
-                } else if (g == 0) {
-                    atomicCAS(&st->err_col, 0, (int32_t)(j0 + a + jj + 1));
-                }
+            if (small) {
+                n_small++;
+                if (first_small < 0) first_small = jj;
             }
-            if (g == 0 && m > 0.0 && colmax[a + jj] > 0.0) atomicMin(&st->min_ratio_bits, (unsigned long long)__double_as_longlong(m / colmax[a + jj]));
+            if (m > 0.0 && cmx[jj] > 0.0) min_ratio = fmin(min_ratio, m / cmx[jj]);
+            D[jj][jj] = pv;
+        }
+        if (below) {
+            const T l = Sc<T>::div(drj, pv);
+            if (dc == jj) D[dr][jj] = l;
+            else if (right) D[dr][dc] = Sc<T>::fms(D[dr][dc], l, djc);
         }
         __syncthreads();
-        const int r = tid % PNB, c = tid / PNB;
-        if (r > jj && r < nb && c == jj) D[r][jj] = Sc<T>::div(D[r][jj], D[jj][jj]);
-        __syncthreads();
-        if (r > jj && r < nb && c > jj && c < nb) D[r][c] = Sc<T>::fms(D[r][c], D[r][jj], D[jj][c]);
-        __syncthreads();
+    }
+    if (g == 0 && tid == 0) {
+        if (n_small > 0) {
+            if (mode == 1) atomicAdd(&st->nreplaced, n_small);
+            else atomicCAS(&st->err_col, 0, (int32_t)(j0 + a + first_small + 1));
+        }
+        if (min_ratio < INFINITY) atomicMin(&st->min_ratio_bits, (unsigned long long)__double_as_longlong(min_ratio));
     }
     // the rows below the sub-panel: workgroup g takes rows a + nb + g TB + tid
     const int32_t row = a + nb + g * TB + tid;
@@ -1037,24 +1090,28 @@ void blk_uinv(int vs, int64_t njobs, const int2* jobs, const BlockDesc* blocks, 
 }
 template <class T, int TM>
 static void blk_upd_t(int mode, int32_t ntile, const UpdTile* tiles, const int32_t* cols, const int32_t* rowmap, double* W,
-                      int64_t ldw, double* X, const uint8_t* bf, hipStream_t s)
+                      int64_t ldw, double* X, const uint8_t* bf, unsigned long long* be, hipStream_t s)
 {
-    if (mode == 1) hipLaunchKernelGGL((k_blk_upd<T, TM, 1>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw, X, bf);
-    else if (mode == 2) hipLaunchKernelGGL((k_blk_upd<T, TM, 2>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw, X, bf);
-    else hipLaunchKernelGGL((k_blk_upd<T, TM, 0>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw, X, bf);
+    if (mode == 1) hipLaunchKernelGGL((k_blk_upd<T, TM, 1>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw, X, bf, be);
+    else if (mode == 2) hipLaunchKernelGGL((k_blk_upd<T, TM, 2>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw, X, bf, be);
+    else if (mode == 3) hipLaunchKernelGGL((k_blk_upd<T, TM, 3>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw, X, bf, be);
+    else hipLaunchKernelGGL((k_blk_upd<T, TM, 0>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw, X, bf, be);
 }
 void blk_upd(int vs, int32_t ntile, const UpdTile* tiles, const int32_t* cols, const int32_t* rowmap, double* W, int64_t ldw,
-             int tile_rows, double flops, double* X, int prec_mode, const uint8_t* blk_f32, hipStream_t s)
+             int tile_rows, double flops, double* X, int prec_mode, const uint8_t* blk_f32, unsigned long long* blk_err,
+             hipStream_t s)
 {
     if (ntile <= 0) return;
     if (prec_mode == 2 && !blk_f32) throw std::invalid_argument("blk_upd: per-block precision without the block marks");
-    ProfScope ps(prec_mode == 0 ? "blk_upd" : (prec_mode == 1 ? "blk_upd_f32" : "blk_upd_mix"), flops, s);
+    if (prec_mode == 3 && !blk_err) throw std::invalid_argument("blk_upd: logging without the per-block error array");
+    static const char* names[4] = {"blk_upd", "blk_upd_f32", "blk_upd_mix", "blk_upd_log"};
+    ProfScope ps(names[prec_mode & 3], flops, s);
     if (tile_rows == 32) {
-        if (vs == 1) blk_upd_t<double, 32>(prec_mode, ntile, tiles, cols, rowmap, W, ldw, X, blk_f32, s);
-        else blk_upd_t<Zd, 32>(prec_mode, ntile, tiles, cols, rowmap, W, ldw, X, blk_f32, s);
+        if (vs == 1) blk_upd_t<double, 32>(prec_mode, ntile, tiles, cols, rowmap, W, ldw, X, blk_f32, blk_err, s);
+        else blk_upd_t<Zd, 32>(prec_mode, ntile, tiles, cols, rowmap, W, ldw, X, blk_f32, blk_err, s);
     } else {
-        if (vs == 1) blk_upd_t<double, 64>(prec_mode, ntile, tiles, cols, rowmap, W, ldw, X, blk_f32, s);
-        else blk_upd_t<Zd, 64>(prec_mode, ntile, tiles, cols, rowmap, W, ldw, X, blk_f32, s);
+        if (vs == 1) blk_upd_t<double, 64>(prec_mode, ntile, tiles, cols, rowmap, W, ldw, X, blk_f32, blk_err, s);
+        else blk_upd_t<Zd, 64>(prec_mode, ntile, tiles, cols, rowmap, W, ldw, X, blk_f32, blk_err, s);
     }
 }
 void x_back(int vs, int32_t ntask, const UpdTask* tasks, const int32_t* cols, const BlockDesc* blocks, double* W, int64_t ldw,

@@ -752,6 +752,7 @@ int factor_core(fplus_numeric* num, const PatternCache& c, DevBuf<double>& d_Bx,
     so.verbose = verbose;
     so.precision = ao.precision;
     so.record = ao.refactor != 0;
+    so.sel = selector_args(ao.precision >= 2 ? ao.selector : nullptr, d_Drp.p, d_Dcp.p);
     if (verbose >= 1)
         printf("[FPLUS] LU factorisation start: n=%ld, nnz(A)=%ld, %s, supernodal left-looking (device), pivot=partial, relax_rows=%ld, panel=%ld\n",
                (long)n, (long)c.nnz, vs == 2 ? "complex128" : "float64", (long)relax_rows, (long)so.panel);
@@ -836,7 +837,11 @@ void fplus_numeric_free(fplus_numeric* num)
 
 static int check_amd_options(const fplus_amd_options* ao, char* err, size_t errlen)
 {
-    if (ao->precision < 0 || ao->precision > 2) { errf(err, errlen, "precision must be 0 (FP64), 1 (FP32) or 2 (per supernode)"); return FPLUS_ERR_BADARG; }
+    if (ao->precision < 0 || ao->precision > 3) {
+        errf(err, errlen, "precision must be 0 (FP64), 1 (FP32), 2 (per block, by the selector) or 3 (FP64, FP32 differences logged)");
+        return FPLUS_ERR_BADARG;
+    }
+    if (ao->precision == 2 && !ao->selector) { errf(err, errlen, "precision 2 needs a selector (fplus_amd_options.selector)"); return FPLUS_ERR_BADARG; }
     if (ao->static_pivots < 0 || ao->static_pivots > 1) { errf(err, errlen, "static_pivots must be 0 or 1"); return FPLUS_ERR_BADARG; }
     return FPLUS_OK;
 }
@@ -1025,6 +1030,29 @@ extern "C" int fplus_amd_numeric_stats(const fplus_numeric* num, fplus_amd_stats
     st->refactor_replaced = num->refactor_replaced;
     st->refactor_min_ratio = num->refactor_min_ratio;
     st->replay_bytes = F.replay ? (double)F.replay->device_bytes() : 0.0;
+    if (F.precision == 1) {
+        st->f32_blocks = st->nblocks;
+        st->f32_flop_share = 1.0;
+    } else if (F.precision == 2 && F.d_blk_f32.p && !F.blocks.empty()) {
+        // the selector's marks, from the device
+        std::vector<uint8_t> mk(F.blocks.size());
+        try {
+            F.d_blk_f32.download(mk.data(), mk.size(), devctx().stream);
+            dev_sync(devctx().stream);
+        } catch (const GpuError&) {
+            return FPLUS_ERR_GPU;
+        }
+        double tot = 0, f32 = 0;
+        for (size_t b = 0; b < mk.size(); b++) {
+            const double fl = b < F.blk_upd_flops.size() ? F.blk_upd_flops[b] : 0.0;
+            tot += fl;
+            if (mk[b]) {
+                st->f32_blocks++;
+                f32 += fl;
+            }
+        }
+        st->f32_flop_share = tot > 0 ? f32 / tot : 0.0;
+    }
     return FPLUS_OK;
 }
 
@@ -1048,8 +1076,12 @@ extern "C" int fplus_amd_refactor(fplus_numeric* num, const void* Ax, int ax_on_
         errf(err, errlen, "this factorization was made without fplus_amd_options.refactor = 1: nothing to replay");
         return FPLUS_ERR_BADARG;
     }
-    if (precision < 0 || precision > 2 || static_pivots < 0 || static_pivots > 1) {
-        errf(err, errlen, "precision must be 0, 1 or 2 and static_pivots 0 or 1");
+    if (precision < 0 || precision > 3 || static_pivots < 0 || static_pivots > 1) {
+        errf(err, errlen, "precision must be 0, 1, 2 or 3 and static_pivots 0 or 1");
+        return FPLUS_ERR_BADARG;
+    }
+    if (precision == 2 && !num->aopt.selector) {
+        errf(err, errlen, "precision 2 needs the selector given with the factorization (fplus_amd_options.selector)");
         return FPLUS_ERR_BADARG;
     }
     const PatternCache& c = *num->cache;
@@ -1081,8 +1113,9 @@ extern "C" int fplus_amd_refactor(fplus_numeric* num, const void* Ax, int ax_on_
         const double t1 = now_s();
         int64_t replaced = 0;
         double minr = 0;
-        rc = sn_refactor_device(num->F, c.d_Bp.p, c.d_Bi.p, d_Bx.p, num->opt.pivot_rel, static_pivots, precision, &replaced, &minr,
-                                err, errlen);
+        const SelectorArgs sa = selector_args(precision == 2 ? num->aopt.selector : nullptr, num->d_Drp.p, num->d_Dcp.p);
+        rc = sn_refactor_device(num->F, c.d_Bp.p, c.d_Bi.p, d_Bx.p, num->opt.pivot_rel, static_pivots, precision, sa, &replaced,
+                                &minr, err, errlen);
         num->refactor_replaced = replaced;
         num->refactor_min_ratio = minr;
         if (rc != FPLUS_OK) return rc;
@@ -1108,7 +1141,8 @@ extern "C" int fplus_amd_refactor(fplus_numeric* num, const void* Ax, int ax_on_
         num->t_total = now_s() - t0;
         if (num->opt.verbose >= 1)
             printf("[FPLUS] refactorization in the recorded pivot order: %.3fs (Ruiz %.3fs), %s updates, %ld pivots replaced, "
-                   "smallest |pivot| / colmax %.3e\n", num->t_total, num->t_ruiz, precision == 0 ? "FP64" : (precision == 1 ? "FP32" : "per-supernode"),
+                   "smallest |pivot| / colmax %.3e\n", num->t_total, num->t_ruiz,
+                   precision == 0 ? "FP64" : (precision == 1 ? "FP32" : (precision == 2 ? "per-block (selector)" : "FP64 (logged)")),
                    (long)replaced, minr);
     } catch (const GpuError& e) {
         return gpu_error(e, err, errlen);
@@ -1128,6 +1162,11 @@ extern "C" int fplus_amd_tune(const char* name, int64_t value)
     if (std::strcmp(name, "panel_lu_max_workgroups") == 0) {
         if (value < 0 || value > 512) return FPLUS_ERR_BADARG;
         k::set_getf2_coop_max_wg((int)value);
+        return FPLUS_OK;
+    }
+    if (std::strcmp(name, "xcd_tile_order") == 0) {
+        if (value != 0 && value != 1) return FPLUS_ERR_BADARG;
+        set_xcd_tile_order(value != 0);
         return FPLUS_OK;
     }
     return FPLUS_ERR_BADARG;

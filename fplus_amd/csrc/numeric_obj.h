@@ -81,6 +81,8 @@ struct fplus_numeric {
 
 
 namespace fpa {
+// the selector's device model and threshold with the scalings its features read (selector.cpp)
+SelectorArgs selector_args(const fplus_amd_selector* sel, const double* d_Dr, const double* d_Dc);
 // the sweeps on w (n x R row-major, pivot order, R <= solve_rhs_per_pass) in place: trans 0 L U x = w, the result in
 // column order; 1 / 2 with U^T L^T (conjugated for 2)
 void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, hipStream_t s);

@@ -301,3 +301,81 @@ extern "C" int fplus_amd_solve_refined(const fplus_numeric* num, void* b, int b_
     if (info) *info = inf;
     return FPLUS_OK;
 }
+
+extern "C" int fplus_amd_refactor_solve(fplus_numeric* num, const void* Ax, int ax_on_device, void* b, int b_on_device, int64_t nrhs,
+                                        int64_t ldb, int precision, const fplus_amd_refine_options* ro, fplus_amd_refine_info* info,
+                                        int* fallback, char* err, size_t errlen)
+{
+    if (fallback) *fallback = 0;
+    if (!num) { errf(err, errlen, "no factorization"); return FPLUS_ERR_BADARG; }
+    const int64_t n = num->n;
+    if (nrhs < 0 || ldb < n) { errf(err, errlen, "fplus_amd_refactor_solve: bad nrhs or ldb"); return FPLUS_ERR_BADARG; }
+    fplus_amd_refine_options d;
+    if (!ro) { fplus_amd_default_refine_options(&d); ro = &d; }
+    const int vs = num->F.vs;
+    // b's columns kept on the device for a second attempt
+    DevBuf<double> keep;
+    try {
+        hipStream_t s = devctx().stream;
+        keep.alloc((size_t)std::max<int64_t>(n * nrhs, 1) * vs);
+        for (int64_t r = 0; r < nrhs; r++) {
+            const double* col = static_cast<const double*>(b) + (size_t)(r * ldb) * vs;
+            if (b_on_device)
+                hip_check(hipMemcpyAsync(keep.p + (size_t)(r * n) * vs, col, (size_t)n * vs * sizeof(double), hipMemcpyDeviceToDevice, s),
+                          "hipMemcpyAsync");
+            else keep.upload(col, (size_t)n * vs, s, (size_t)(r * n) * vs);
+        }
+    } catch (const GpuError& e) {
+        errf(err, errlen, "GPU: %s", e.what());
+        return FPLUS_ERR_GPU;
+    } catch (const std::bad_alloc&) {
+        errf(err, errlen, "out of memory");
+        return FPLUS_ERR_NOMEM;
+    }
+    int rc = fplus_amd_refactor(num, Ax, ax_on_device, precision, 1, err, errlen);
+    fplus_amd_refine_info inf;
+    std::memset(&inf, 0, sizeof(inf));
+    if (rc == FPLUS_OK) rc = fplus_amd_solve_refined(num, keep.p, 1, nrhs, n, ro, &inf, err, errlen);
+    if (rc == FPLUS_OK && !inf.converged && precision != 0) {
+        // the refinement stalled: the factorization again with FP64 updates, b solved again from the start
+        if (fallback) *fallback = 1;
+        try {
+            hipStream_t s = devctx().stream;
+            for (int64_t r = 0; r < nrhs; r++) {
+                const double* col = static_cast<const double*>(b) + (size_t)(r * ldb) * vs;
+                if (b_on_device)
+                    hip_check(hipMemcpyAsync(keep.p + (size_t)(r * n) * vs, col, (size_t)n * vs * sizeof(double), hipMemcpyDeviceToDevice, s),
+                              "hipMemcpyAsync");
+                else keep.upload(col, (size_t)n * vs, s, (size_t)(r * n) * vs);
+            }
+        } catch (const GpuError& e) {
+            errf(err, errlen, "GPU: %s", e.what());
+            return FPLUS_ERR_GPU;
+        }
+        fplus_amd_refine_info inf2;
+        std::memset(&inf2, 0, sizeof(inf2));
+        rc = fplus_amd_refactor(num, Ax, ax_on_device, 0, 1, err, errlen);
+        if (rc == FPLUS_OK) rc = fplus_amd_solve_refined(num, keep.p, 1, nrhs, n, ro, &inf2, err, errlen);
+        inf2.outer += inf.outer;
+        inf2.inner += inf.inner;
+        inf2.bwd_first = inf.bwd_first;
+        inf = inf2;
+    }
+    if (rc != FPLUS_OK) return rc;
+    try {
+        hipStream_t s = devctx().stream;
+        for (int64_t r = 0; r < nrhs; r++) {
+            double* col = static_cast<double*>(b) + (size_t)(r * ldb) * vs;
+            if (b_on_device)
+                hip_check(hipMemcpyAsync(col, keep.p + (size_t)(r * n) * vs, (size_t)n * vs * sizeof(double), hipMemcpyDeviceToDevice, s),
+                          "hipMemcpyAsync");
+            else keep.download(col, (size_t)n * vs, s, (size_t)(r * n) * vs);
+        }
+        dev_sync(s);
+    } catch (const GpuError& e) {
+        errf(err, errlen, "GPU: %s", e.what());
+        return FPLUS_ERR_GPU;
+    }
+    if (info) *info = inf;
+    return FPLUS_OK;
+}

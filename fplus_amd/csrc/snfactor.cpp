@@ -18,8 +18,10 @@
 
 #include "host_common.h"
 #include "host_pool.h"
+#include "selector_dev.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstddef>
@@ -32,6 +34,11 @@
 #include <thread>
 
 namespace fpa {
+
+static std::atomic<bool> g_xcd_order{true};
+bool xcd_tile_order() { return g_xcd_order.load(std::memory_order_relaxed); }
+void set_xcd_tile_order(bool on) { g_xcd_order.store(on, std::memory_order_relaxed); }
+
 namespace {
 
 inline double now_s()
@@ -309,6 +316,7 @@ private:
     EventList sp_ev_;                // the end of each sub-panel's copies
     DevBuf<URun> d_runs_;
     DevBuf<ExtractCol> d_ex_;
+    DevBuf<int32_t> d_newblk_;       // the panel's new blocks (features, selector)
     DevBuf<int32_t> d_lrows_;        // the row at each lower workspace position after the panel LU
     UploadStage up_;                 // every upload of a panel goes through page-locked staging (one area per panel,
                                      // two in turn): the host goes on while the copies and the kernels run
@@ -651,7 +659,10 @@ int Driver::panel(int64_t j0, int64_t j1)
         const int32_t L = tl[i].level;
         const BlockDesc& B = F_.blocks[T.block];
         const int64_t m = B.w - T.k0;
-        lev_flops[L] += fm * (double)T.ncol * (double)m * ((double)(m - 1) / 2 + (double)(B.nrows - B.w));
+        const double tfl = fm * (double)T.ncol * (double)m * ((double)(m - 1) / 2 + (double)(B.nrows - B.w));
+        lev_flops[L] += tfl;
+        if ((size_t)T.block >= F_.blk_upd_flops.size()) F_.blk_upd_flops.resize((size_t)T.block + 1, 0.0);
+        F_.blk_upd_flops[T.block] += tfl;
         {
             // the product's size against what the tiles issue on the matrix cores (rows, columns and k padded)
             const double prod = fm * (double)T.ncol * (double)m * (double)(B.nrows - T.k0);
@@ -695,6 +706,54 @@ int Driver::panel(int64_t j0, int64_t j1)
         });
     }
     for (int32_t L = 1; L <= maxlevel + 1; L++) lev_gt[L] = std::max(lev_gt[L], lev_gt[L - 1]);
+    if (xcd_tile_order()) {
+        // XCD-aware order of each level's tiles: a launch's workgroup i runs on XCD i mod 8 (round-robin dispatch), and
+        // the row tiles of one task and column tile all read the same slice of the segment S; giving them positions of
+        // one residue mod 8 keeps that slice in one XCD's L2, where otherwise every XCD fetches it from HBM.  Groups go to
+        // the least loaded XCD, largest first; an XCD whose list runs out lends its positions to the fullest one.
+        constexpr int NX = 8;
+        std::vector<UpdTile> tmp;
+        std::vector<std::pair<int64_t, int64_t>> grp;   // (size, first tile); a group's tiles are ncc apart
+        std::vector<int64_t> gstride;
+        size_t ti = 0;
+        for (int32_t L = 1; L <= maxlevel; L++) {
+            const int64_t g0 = lev_gt[L], g1 = lev_gt[L + 1];
+            const size_t t0 = ti;
+            while (ti < tl.size() && tl[ti].level == L) ti++;
+            if (g1 - g0 < 2 * NX) continue;
+            grp.clear();
+            gstride.clear();
+            for (size_t i = t0; i < ti; i++) {
+                const int64_t rt = lev_rt[L];
+                const int64_t nrt = (F_.blocks[tasks[i].block].nrows - tasks[i].k0 + rt - 1) / rt;
+                const int64_t ncc = (tasks[i].ncol + CTG - 1) / CTG;
+                for (int64_t cc = 0; cc < ncc; cc++) { grp.push_back({nrt, toff[i] + cc}); gstride.push_back(ncc); }
+            }
+            std::vector<int64_t> order(grp.size());
+            std::iota(order.begin(), order.end(), 0);
+            std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return grp[x].first > grp[y].first; });
+            std::vector<std::vector<int64_t>> bin(NX);
+            std::vector<int64_t> load(NX, 0);
+            for (const int64_t gi : order) {
+                const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                for (int64_t r = 0; r < grp[gi].first; r++) bin[x].push_back(grp[gi].second + r * gstride[gi]);
+                load[x] += grp[gi].first;
+            }
+            tmp.resize((size_t)(g1 - g0));
+            std::vector<size_t> nxt(NX, 0);
+            for (int64_t p2 = 0; p2 < g1 - g0; p2++) {
+                int x = (int)(p2 % NX);
+                if (nxt[x] >= bin[x].size()) {
+                    int best = -1;
+                    for (int y = 0; y < NX; y++)
+                        if (nxt[y] < bin[y].size() && (best < 0 || bin[y].size() - nxt[y] > bin[best].size() - nxt[best])) best = y;
+                    x = best;
+                }
+                tmp[(size_t)p2] = gtiles[bin[x][nxt[x]++]];
+            }
+            std::copy(tmp.begin(), tmp.end(), gtiles + g0);
+        }
+    }
     for (int32_t L = 1; L <= maxlevel; L++) {   // levels by their tile count (powers of two), for the statistics
         const int32_t nt = lev_gt[L + 1] - lev_gt[L];
         if (nt <= 0) continue;
@@ -750,7 +809,7 @@ int Driver::panel(int64_t j0, int64_t j1)
         for (int32_t L = 1; L <= maxlevel; L++) {
             const int32_t g0 = lev_gt[L], g1 = lev_gt[L + 1];
             k::blk_upd(vs_, g1 - g0, d_tiles_.p + g0, d_cols_.p, d_rowmap_.p, d_W_.p, ldw, lev_rt[L], lev_flops[L], d_X_.p,
-                       opt_.precision, F_.d_blk_f32.p, s_);
+                       opt_.precision, F_.d_blk_f32.p, F_.d_blk_err.p, s_);
         }
         k::x_back(vs_, (int32_t)tasks.size(), d_tasks_.p, d_cols_.p, d_blocks_p(), d_W_.p, ldw, d_X_.p, s_);
     }
@@ -1052,6 +1111,17 @@ int Driver::panel_out()
         up_.upload(d_itiles_.p, itiles.data(), itiles.size(), s_);
         k::blk_inverse(vs_, (int32_t)ijobs.size(), d_ijobs_.p, (int32_t)itiles.size(), d_itiles_.p, d_blocks_p(), iflops, s_);
     }
+    if (opt_.precision >= 2 && !new_blocks_.empty()) {
+        // the new blocks' features and (precision 2) the selector's marks, on the device: later panels' update tiles read
+        // the marks, so the decisions need no trip to the host
+        d_newblk_.reserve(new_blocks_.size());
+        up_.upload(d_newblk_.p, new_blocks_.data(), new_blocks_.size(), s_);
+        k::block_features(vs_, (int32_t)new_blocks_.size(), d_newblk_.p, d_blocks_p(), d_colmax_.p, j0, opt_.sel.d_Dr, opt_.sel.d_Dc,
+                          n_, F_.d_feat.p, s_);
+        if (opt_.precision == 2)
+            k::sel_infer((int64_t)new_blocks_.size(), d_newblk_.p, F_.d_feat.p, opt_.sel.d_model, opt_.sel.threshold, nullptr,
+                         F_.d_blk_f32.p, s_);
+    }
     {
         const int64_t cnt = Up_[j1] - u_begin;
         if (u_begin + cnt > (int64_t)(F_.d_Ux.n / vs_)) {
@@ -1093,6 +1163,9 @@ int Driver::panel_out()
         P.runs_off = (int64_t)rec_->h_runs.size();
         P.nruns = (int64_t)pruns.size();
         rec_->h_runs.insert(rec_->h_runs.end(), pruns.begin(), pruns.end());
+        P.nb_off = (int64_t)rec_->h_newblk.size();
+        P.nnb = (int64_t)new_blocks_.size();
+        rec_->h_newblk.insert(rec_->h_newblk.end(), new_blocks_.begin(), new_blocks_.end());
         rec_->panels.push_back(P);
     }
     for (int32_t row : lrows) lposmap_[row] = -1;
@@ -1143,6 +1216,15 @@ int Driver::run()
     F_.d_blk_f32.alloc(nn);   // (a block has at least one column)
     k::fill_u8(F_.d_blk_f32.p, (int64_t)nn, opt_.precision == 1 ? 1 : 0, s_);
     F_.precision = opt_.precision;
+    if (opt_.precision >= 2) {
+        if (opt_.precision == 2 && !opt_.sel.d_model) throw std::invalid_argument("per-block precision without a selector model");
+        if (!opt_.sel.d_Dr || !opt_.sel.d_Dc) throw std::invalid_argument("the selector's features need the scalings");
+        F_.d_feat.alloc(nn * SEL_NF);
+        if (opt_.precision == 3) {
+            F_.d_blk_err.alloc(2 * nn);
+            k::fill_u64(F_.d_blk_err.p, (int64_t)(2 * nn), 0ULL, s_);
+        }
+    }
     if (opt_.record) {
         F_.replay.reset(new Replay());
         rec_ = F_.replay.get();
@@ -1256,6 +1338,7 @@ void Replay::upload(hipStream_t s)
     up_vec(d_ij, h_ij, s);
     up_vec(d_it, h_it, s);
     up_vec(d_runs, h_runs, s);
+    up_vec(d_newblk, h_newblk, s);
 }
 
 size_t Replay::device_bytes() const
@@ -1271,7 +1354,8 @@ size_t Replay::device_bytes() const
 // the new blocks and their inverse forms, U out -- the same launches as the factorization with pivoting, with none of
 // its host work and no wait for the device until the end.
 int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, const double* d_Bx, double pivot_rel,
-                       int static_pivots, int precision, int64_t* replaced, double* min_ratio, char* err, size_t errlen)
+                       int static_pivots, int precision, const SelectorArgs& sel, int64_t* replaced, double* min_ratio, char* err,
+                       size_t errlen)
 {
     if (!F.replay) {
         errf(err, errlen, "the factorization was not recorded for refactorization");
@@ -1295,7 +1379,16 @@ int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, c
     st0.min_ratio_bits = 0x7FF0000000000000ULL;   // +inf
     R.d_st.upload(&st0, 1, s);
     if (precision == 1) k::fill_u8(F.d_blk_f32.p, (int64_t)F.blocks.size(), 1, s);
-    else if (precision == 0) k::fill_u8(F.d_blk_f32.p, (int64_t)F.blocks.size(), 0, s);
+    else if (precision == 0 || precision == 3) k::fill_u8(F.d_blk_f32.p, (int64_t)F.blocks.size(), 0, s);
+    if (precision >= 2) {
+        if (precision == 2 && !sel.d_model) { errf(err, errlen, "per-block precision needs a selector"); return FPLUS_ERR_BADARG; }
+        const size_t nn = (size_t)std::max<int64_t>(n, 1);
+        if (F.d_feat.n < nn * SEL_NF) F.d_feat.alloc(nn * SEL_NF);
+        if (precision == 3) {
+            if (F.d_blk_err.n < 2 * nn) F.d_blk_err.alloc(2 * nn);
+            k::fill_u64(F.d_blk_err.p, (int64_t)(2 * nn), 0ULL, s);
+        }
+    }
     const BlockDesc* blocks = F.d_blocks.p;
     for (const ReplayPanel& P : R.panels) {
         const int32_t* rows = R.d_rows.p + P.rows_off;
@@ -1307,7 +1400,7 @@ int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, c
         for (int64_t L = P.lev_off; L < P.lev_off + P.nlev; L++) {
             const ReplayLevel& V = R.levels[L];
             k::blk_upd(vs, V.ntile, R.d_tiles.p + V.tile0, cols, R.d_rowmap.p, R.d_W.p, ldw, V.rt, V.flops, R.d_X.p, precision,
-                       F.d_blk_f32.p, s);
+                       F.d_blk_f32.p, F.d_blk_err.p, s);
         }
         if (P.ntask) k::x_back(vs, (int32_t)P.ntask, R.d_tasks.p + P.tasks_off, cols, blocks, R.d_W.p, ldw, R.d_X.p, s);
         double* Wl = R.d_W.p + P.Ru * vs;
@@ -1320,6 +1413,11 @@ int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, c
         if (P.nex) k::extract_cols(vs, (int32_t)P.nex, R.d_ex.p + P.ex_off, R.d_expos.p + P.expos_off, rows + P.Ru, Wl, ldw, s);
         if (P.nij)
             k::blk_inverse(vs, (int32_t)P.nij, R.d_ij.p + P.ij_off, (int32_t)P.nit, R.d_it.p + P.it_off, blocks, P.iflops, s);
+        if (precision >= 2 && P.nnb) {
+            k::block_features(vs, (int32_t)P.nnb, R.d_newblk.p + P.nb_off, blocks, R.d_colmax.p, P.j0, sel.d_Dr, sel.d_Dc, n, F.d_feat.p, s);
+            if (precision == 2)
+                k::sel_infer(P.nnb, R.d_newblk.p + P.nb_off, F.d_feat.p, sel.d_model, sel.threshold, nullptr, F.d_blk_f32.p, s);
+        }
         if (P.nruns) k::u_runs(vs, (int32_t)P.nruns, R.d_runs.p + P.runs_off, R.d_W.p, F.d_Ux.p, F.d_Ui.p, s);
         k::rowmap_set(nrow, rows, R.d_rowmap.p, 1, s);
     }

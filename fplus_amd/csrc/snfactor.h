@@ -26,6 +26,7 @@ struct ReplayPanel {
     int64_t ij_off, nij, it_off, nit;
     double iflops;
     int64_t runs_off, nruns;
+    int64_t nb_off, nnb;           // the panel's new blocks (their features and precision marks)
 };
 struct Replay {
     std::vector<ReplayPanel> panels;
@@ -38,7 +39,8 @@ struct Replay {
     std::vector<int2> h_ij;
     std::vector<GemmTile> h_it;
     std::vector<URun> h_runs;
-    DevBuf<int32_t> d_rows, d_cols, d_expos;
+    std::vector<int32_t> h_newblk;
+    DevBuf<int32_t> d_rows, d_cols, d_expos, d_newblk;
     DevBuf<UpdTask> d_tasks;
     DevBuf<UpdTile> d_tiles;
     DevBuf<ExtractCol> d_ex;
@@ -94,6 +96,7 @@ struct DevFactor {
     // the block updates' products: their flops, what the tiles issue on the matrix cores, the flops by task width
     // (columns <= 8, 16, 32, 64, more)
     double upd_prod_flops = 0, upd_mfma_flops = 0, upd_ncol_flops[5] = {0, 0, 0, 0, 0};
+    std::vector<double> blk_upd_flops;   // per block (source of updates): its update products' algorithmic flops
     // the dependency levels by tile count (bucket b: at most 2^b tiles, the last one more): levels, tiles, flops issued
     // on the matrix cores, and the sum of each level's longest segment
     struct LevHist { int64_t levels = 0, tiles = 0; double issued = 0, maxm = 0; } lev_hist[13];
@@ -103,8 +106,19 @@ struct DevFactor {
     bool coop_aborted = false;   // the cooperative panel LU gave up at a barrier (the factorization failed; retry without it)
     bool coop_retried = false;   // this factorization is the retry with the one-workgroup panel LU
     DevBuf<uint8_t> d_blk_f32;   // per block: its updates of later columns on the FP32 matrix cores (precision mode 2)
+    DevBuf<float> d_feat;        // per block: the selector's features (precision 2 and 3)
+    DevBuf<unsigned long long> d_blk_err;   // per block: the logged FP32 differences and FP64 magnitudes (precision 3)
     std::unique_ptr<Replay> replay;   // kept when the factorization records what a refactorization replays
     int precision = 0;           // the updates' precision of the last (re)factorization: 0 FP64, 1 FP32, 2 per block
+};
+
+// what the precision selector needs while a factorization runs: the model (SEL_NW floats on the device) and its
+// threshold, and the scalings of the analysed matrix (for the blocks' features)
+struct SelectorArgs {
+    const float* d_model = nullptr;
+    float threshold = 0.5f;
+    const double* d_Dr = nullptr;
+    const double* d_Dc = nullptr;
 };
 
 struct SnOptions {
@@ -113,16 +127,24 @@ struct SnOptions {
     int64_t panel = 256;
     int verbose = 0;
     bool coop_panel_lu = true;   // false: the one-workgroup panel LU for every sub-panel
-    int precision = 0;           // the block updates: 0 FP64, 1 FP32, 2 per block by DevFactor::d_blk_f32
+    int precision = 0;           // the block updates: 0 FP64, 1 FP32, 2 per block by DevFactor::d_blk_f32 (set by the
+                                 // precision selector as the blocks are made), 3 FP64 with the FP32 differences logged
     bool record = false;         // keep what a same-pattern refactorization replays (DevFactor::replay)
+    SelectorArgs sel;            // precision 2 and 3
 };
+
+// the order of the update tiles within a level: XCD-aware (the row tiles reading one segment slice on one XCD; the
+// default) or as built (tuning knob "xcd_tile_order")
+bool xcd_tile_order();
+void set_xcd_tile_order(bool on);
 
 // a same-pattern refactorization: the values d_Bx (the recorded factorization's analysed, equilibrated pattern) factored
 // in the recorded pivot order by replaying its device work; static_pivots 0: a pivot below pivot_rel * colmax fails it
 // (FPLUS_ERR_SINGULAR, the caller factors with pivoting), 1: such pivots are replaced (counted in *replaced).
 // precision as in SnOptions.  min_ratio: the smallest |pivot| / colmax.
 int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, const double* d_Bx, double pivot_rel,
-                       int static_pivots, int precision, int64_t* replaced, double* min_ratio, char* err, size_t errlen);
+                       int static_pivots, int precision, const SelectorArgs& sel, int64_t* replaced, double* min_ratio, char* err,
+                       size_t errlen);
 
 // factor the n x n CSC matrix whose values are on the device (d_Bx, vs doubles per entry); the host holds its
 // pattern (Bp, Bi); d_Bp / d_Bi are the same pattern on the device.  Throws GpuError / std::bad_alloc; returns

@@ -34,6 +34,8 @@ typedef struct {
     int64_t refactor_replaced;    /* the last refactorization: pivots replaced (static pivoting) */
     double refactor_min_ratio;    /* and its smallest |pivot| / (largest entry of the pivot's column) */
     double replay_bytes;          /* device bytes kept for refactorizations (schedule and inverse forms) */
+    int64_t f32_blocks;           /* blocks whose updates ran in FP32 in the last (re)factorization */
+    double f32_flop_share;        /* their share of the update flops */
 } fplus_amd_stats;
 int fplus_amd_numeric_stats(const fplus_numeric *num, fplus_amd_stats *st);
 
@@ -47,17 +49,22 @@ int fplus_amd_solve_device(const fplus_numeric *num, void *d_b, int64_t nrhs, in
 /* Options of this build's factorization beyond fplus_options (fplus_amd_default_options: all zero).
  *   precision      the block updates (the Schur-complement products of the supernodal LU): 0 FP64 (the drop-in's);
  *                  1 the FP32 matrix cores (operands rounded to FP32, FP32 accumulation, the result subtracted in FP64);
- *                  2 per supernode, by the marks the precision selector sets (fplus_amd_select).  Pivot selection, the
- *                  panels' dense LU, Ruiz and the solves stay FP64.  A factorization made with FP32 updates is meant for
- *                  fplus_amd_solve_refined, which wins the accuracy back.
+ *                  2 per block (a supernode's columns within a panel), by the learned precision selector: as each block
+ *                  is made its features go through the selector's model on the device, which marks whether the block's
+ *                  updates of later columns may run in FP32; 3 FP64 updates with the FP32 products computed beside them
+ *                  and their differences logged per block (fplus_amd_selector_log: the selector's training data).  Pivot
+ *                  selection, the panels' dense LU, Ruiz and the solves stay FP64.  A factorization with FP32 updates
+ *                  is meant for fplus_amd_solve_refined, which wins the accuracy back.
  *   refactor       1: keep what fplus_amd_refactor replays -- the device schedule of this factorization and the blocks'
  *                  inverse forms (fplus_amd_stats.replay_bytes).
  *   static_pivots  (used by fplus_amd_refactor) */
+typedef struct fplus_amd_selector fplus_amd_selector;
 typedef struct {
     int precision;
     int refactor;
     int static_pivots;
     int reserved;
+    const fplus_amd_selector *selector;   /* precision 2 (here or in a refactorization): the model that marks the blocks */
 } fplus_amd_options;
 void fplus_amd_default_options(fplus_amd_options *o);
 
@@ -75,6 +82,30 @@ int fplus_amd_factor_ex(const fplus_symbolic *sym, const int64_t *Ap, const int6
  * Needs a factorization made with fplus_amd_options.refactor = 1. */
 int fplus_amd_refactor(fplus_numeric *num, const void *Ax, int ax_on_device, int precision, int static_pivots, char *err,
                        size_t errlen);
+
+/* The learned precision selector: an MLP 12 -> 64 -> 64 -> 2 in FP32 over per-block features (the block's width and
+ * rows, the largest entries of its inverse form, of L21 and of L11^-1, the spread and size of its pivots against their
+ * columns' largest entries, the spread of the equilibration scales over its rows and columns, its position), trained
+ * on logs that factorizations with precision 3 produce (each block's updates computed both ways), with batched training
+ * and inference kernels on the device.  Class 1 (the block's FP32 updates keep its products within the training
+ * threshold) above `threshold` marks the block FP32.
+ *   create: a model with seeded random weights (untrained: set weights or train it before use).
+ *   train: nsamples feature rows (12 floats each) with labels (1: FP32 safe, 0: not), SGD with momentum over
+ *          minibatches of 256 for `epochs` passes; the features' means and scales are taken from the samples.
+ *   predict: P(class 1) per feature row.
+ *   get_weights / set_weights: the SEL_NW = 5146 floats (W1, b1, W2, b2, W3, b3, feature means, scales).
+ *   log: after a factorization (or refactorization) with precision 3, per block its features, the largest difference
+ *        |P32 - P64| of its update products computed in FP32 and in FP64, and the largest |P64| (both in the
+ *        equilibrated scale of the factorization, Dr A Dc; 0 for a block that updated nothing). */
+int fplus_amd_selector_create(uint64_t seed, fplus_amd_selector **sel);
+void fplus_amd_selector_free(fplus_amd_selector *sel);
+int fplus_amd_selector_train(fplus_amd_selector *sel, const float *features, const float *labels, int64_t nsamples, int epochs,
+                             float lr, uint64_t seed, float *final_loss);
+int fplus_amd_selector_predict(const fplus_amd_selector *sel, const float *features, int64_t nsamples, float *prob);
+int fplus_amd_selector_set_threshold(fplus_amd_selector *sel, float threshold);
+int fplus_amd_selector_get_weights(const fplus_amd_selector *sel, float *w, int64_t cap);
+int fplus_amd_selector_set_weights(fplus_amd_selector *sel, const float *w, int64_t n);
+int fplus_amd_selector_log(const fplus_numeric *num, float *features, double *diff, double *pmax, int64_t cap, int64_t *nblocks);
 
 /* Refined solves: A x = b by iterative refinement in FP64 around the factorization, each correction solved by GMRES
  * preconditioned with the factors (GMRES-IR), in the equilibrated system Dr A Dc of the factorization -- for
@@ -103,6 +134,14 @@ void fplus_amd_default_refine_options(fplus_amd_refine_options *o);
 int fplus_amd_solve_refined(const fplus_numeric *num, void *b, int b_on_device, int64_t nrhs, int64_t ldb,
                             const fplus_amd_refine_options *ro, fplus_amd_refine_info *info, char *err, size_t errlen);
 
+/* The refactor + solve sequence with the precision fallback: refactorization at `precision` (static pivoting), refined
+ * solve; when the refinement stalls (not every right-hand side meets tol within max_outer steps), the factorization is
+ * made again with FP64 updates and b solved again from the start.  *fallback: 0 none, 1 the FP64 refactorization ran.
+ * b is kept in a device copy for the second attempt. */
+int fplus_amd_refactor_solve(fplus_numeric *num, const void *Ax, int ax_on_device, void *b, int b_on_device, int64_t nrhs,
+                             int64_t ldb, int precision, const fplus_amd_refine_options *ro, fplus_amd_refine_info *info,
+                             int *fallback, char *err, size_t errlen);
+
 /* Device memory and the library's stream for those callers: allocate, free, copy (kind 0 host to device, 1 device to
  * host, 2 device to device), wait for the library's stream. */
 int fplus_amd_dev_alloc(size_t bytes, void **ptr);
@@ -111,8 +150,10 @@ int fplus_amd_dev_copy(void *dst, const void *src, size_t bytes, int kind);
 int fplus_amd_dev_sync(void);
 
 /* Tuning knobs (process-wide; tests and benchmarks).  "panel_lu_max_workgroups": the most workgroups of the
- * cooperative panel LU (at most 512, the default); 0 runs every sub-panel on the one-workgroup kernel.  Unknown names
- * are FPLUS_ERR_BADARG. */
+ * cooperative panel LU (at most 512, the default); 0 runs every sub-panel on the one-workgroup kernel.
+ * "xcd_tile_order": 1 (default) orders each update level's tiles so that those reading one segment slice run on one
+ * XCD, 0 keeps them in task order (for measurements; applies to factorizations made afterwards).  Unknown names are
+ * FPLUS_ERR_BADARG. */
 int fplus_amd_tune(const char *name, int64_t value);
 
 /* Per-kernel timing with HIP events on the library's stream (benchmarks): on = 1 starts recording and clears the sums,
