@@ -122,6 +122,7 @@ SIGNATURES = {
     "fplus_amd_factor_device": (C.c_int, [VP, I64P, I64P, VP, C.c_int, C.POINTER(Options), C.POINTER(VP), C.c_char_p,
                                           C.c_size_t]),
     "fplus_amd_solve_device": (C.c_int, [VP, VP, C.c_int64, C.c_int64, C.c_int, C.c_char_p, C.c_size_t]),
+    "fplus_amd_solve_device_noalloc": (C.c_int, [VP, VP, C.c_int64, C.c_int64, C.c_int, C.c_char_p, C.c_size_t]),
     "fplus_amd_dev_alloc": (C.c_int, [C.c_size_t, C.POINTER(VP)]),
     "fplus_amd_dev_free": (C.c_int, [VP]),
     "fplus_amd_dev_copy": (C.c_int, [VP, VP, C.c_size_t, C.c_int]),

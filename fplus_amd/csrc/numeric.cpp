@@ -556,27 +556,50 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
 }  // namespace
 
 namespace fpa {
-// the sweeps on w (n x R row-major, pivot order, R <= solve_rhs_per_pass) in place
+SweepScratch sweep_scratch_size(const fplus_numeric* num, int32_t R)
+{
+    const int vs = num->F.vs;
+    SweepScratch z;
+    z.n_y = (size_t)std::max<int64_t>(num->F.n, 1) * R * vs;
+    z.n_contrib = (size_t)(num->nslots > 0 ? num->nslots : 1) * R * vs;
+    z.n_part = (size_t)std::max<int64_t>(num->ub_slots_max, 1) * 256 * k::solve_rhs_per_pass(vs) * vs;   // a slot: 256 rows
+    return z;
+}
+
+// the sweeps on w (n x R row-major, pivot order, R <= solve_rhs_per_pass) in place, with the factorization's own
+// scratch buffers (grown here when too small)
 void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, hipStream_t s)
+{
+    SweepScratch z = sweep_scratch_size(num, R);
+    num->d_y.reserve(z.n_y);
+    z.y = num->d_y.p;
+    if (trans == 0) {
+        num->d_contrib.reserve(z.n_contrib);
+        num->d_ub_part.reserve(z.n_part);
+        z.contrib = num->d_contrib.p;
+        z.part = num->d_ub_part.p;
+    }
+    device_sweeps(num, w, R, trans, z, s);
+}
+
+// the same with caller-given scratch (z's pointers, of at least z's sizes for R; transposed sweeps use y only)
+void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, const SweepScratch& z, hipStream_t s)
 {
     const DevFactor& F = num->F;
     const int vs = F.vs;
     const BlockDesc* blocks = F.d_blocks.p;
     if (trans == 0) {
         // L y = b from w into y, then U x = y from y into w
-        num->d_contrib.reserve((size_t)(num->nslots > 0 ? num->nslots : 1) * R * vs);
-        num->d_y.reserve((size_t)std::max<int64_t>(F.n, 1) * R * vs);
-        double* y = num->d_y.p;
+        double* y = z.y;
         for (size_t L = 0; L + 1 < num->lev_lf.size(); L++) {
             k::lfwd_gather(vs, num->lev_gr[L + 1] - num->lev_gr[L], num->d_grows.p + num->lev_gr[L], num->d_incp.p, num->d_incs.p,
-                           num->d_contrib.p, w, R, num->bytes_lf_gather[L], s);
+                           z.contrib, w, R, num->bytes_lf_gather[L], s);
             k::lfwd_pull(vs, num->lev_lpt[L + 1] - num->lev_lpt[L], num->d_lpt.p + num->lev_lpt[L], blocks, w, y, R,
                          num->bytes_lf_pull[L], s);
             k::lfwd_push(vs, num->lev_lft[L + 1] - num->lev_lft[L], num->d_lft.p + num->lev_lft[L], blocks, num->d_slot_off.p, y,
-                         num->d_contrib.p, R, num->bytes_lf_push[L], s);
+                         z.contrib, R, num->bytes_lf_push[L], s);
         }
-        num->d_ub_part.reserve((size_t)std::max<int64_t>(num->ub_slots_max, 1) * 256 * 16 * vs);   // a slot: 256 rows x 16
-        double* part = num->d_ub_part.p;
+        double* part = z.part;
         for (size_t L = 0; L + 1 < num->lev_ub.size(); L++) {
             k::ub_dots(vs, num->lev_ubt[L + 1] - num->lev_ubt[L], num->d_ubt.p + num->lev_ubt[L], blocks, num->d_pa.p, num->d_pj.p,
                        num->d_pu.p, F.d_Ux.p, w, part, R, num->bytes_ub[L], s);
@@ -587,8 +610,7 @@ void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, hi
     } else {
         // U^T y = b from w into y, then L^T x = y from y into w
         const int cj = trans == 2 && vs == 2;
-        num->d_y.reserve((size_t)std::max<int64_t>(F.n, 1) * R * vs);
-        double* y = num->d_y.p;
+        double* y = z.y;
         for (size_t L = 0; L + 1 < num->lev_utf.size(); L++) {
             k::utdots(vs, num->lev_utt[L + 1] - num->lev_utt[L], num->d_utt.p + num->lev_utt[L], blocks, num->d_Up.p, num->F.d_Ui.p,
                       F.d_Ux.p, y, w, R, cj, num->bytes_utf[L], s);
@@ -1162,6 +1184,10 @@ extern "C" int fplus_amd_tune(const char* name, int64_t value)
     if (std::strcmp(name, "panel_lu_max_workgroups") == 0) {
         if (value < 0 || value > 512) return FPLUS_ERR_BADARG;
         k::set_getf2_coop_max_wg((int)value);
+        return FPLUS_OK;
+    }
+    if (std::strcmp(name, "solve_noalloc_limit_bytes") == 0) {
+        if (value < 0 || !set_noalloc_limit((size_t)value)) return FPLUS_ERR_BADARG;
         return FPLUS_OK;
     }
     if (std::strcmp(name, "xcd_tile_order") == 0) {

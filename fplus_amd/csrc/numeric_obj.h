@@ -86,6 +86,20 @@ SelectorArgs selector_args(const fplus_amd_selector* sel, const double* d_Dr, co
 // the sweeps on w (n x R row-major, pivot order, R <= solve_rhs_per_pass) in place: trans 0 L U x = w, the result in
 // column order; 1 / 2 with U^T L^T (conjugated for 2)
 void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, hipStream_t s);
+// the sweeps' scratch: the intermediate vector y (n x R), the forward sweep's contribution slots and the backward
+// sweep's partial sums (sizes in doubles); their contents need no clearing between calls (every kernel writes what it
+// reads before reading it)
+struct SweepScratch {
+    double* y = nullptr;
+    double* contrib = nullptr;
+    double* part = nullptr;
+    size_t n_y = 0, n_contrib = 0, n_part = 0;
+};
+SweepScratch sweep_scratch_size(const fplus_numeric* num, int32_t R);
+// the no-allocation solve's array (noalloc.hip): its size, and the bytes a call may use (0: all of it)
+size_t noalloc_scratch_bytes();
+bool set_noalloc_limit(size_t bytes);
+void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, const SweepScratch& z, hipStream_t s);
 // solve for nrhs right-hand sides in device memory b (column-major, ld), in passes of solve_rhs_per_pass columns
 void solve_on_device(const fplus_numeric* num, double* b, int64_t nrhs, int64_t ldb, int tr, hipStream_t s);
 }  // namespace fpa

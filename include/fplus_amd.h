@@ -46,6 +46,18 @@ int fplus_amd_factor_device(const fplus_symbolic *sym, const int64_t *Ap, const 
                             const fplus_options *opt, fplus_numeric **num, char *err, size_t errlen);
 int fplus_amd_solve_device(const fplus_numeric *num, void *d_b, int64_t nrhs, int64_t ldb, int trans, char *err, size_t errlen);
 
+/* fplus_amd_solve_device without allocating: no counterpart in the reference; for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  The same kernels and results
+ * as fplus_amd_solve_device; their scratch (the permuted right-hand sides, the sweeps' intermediate vector, contribution
+ * slots and partial sums) lives in a 64 MiB array of device memory that the library holds on each GPU until the process
+ * ends, instead of buffers it allocates.  Right-hand sides go in passes sized to fit the array; a factorization whose
+ * solve does not fit for one right-hand side (about n > 2 M unknowns) is rejected with FPLUS_ERR_BADARG.  Every call on
+ * a GPU shares that GPU's array, which keeps the last call's contents (no kernel reads what the call did not write);
+ * the library runs one such call at a time per process, and work of the caller's that overlaps a call on the same
+ * array (e.g. from a graph replayed on another stream) races with it.  Waits for the library's stream. */
+int fplus_amd_solve_device_noalloc(const fplus_numeric *num, void *d_b, int64_t nrhs, int64_t ldb, int trans, char *err,
+                                   size_t errlen);
+
 /* Options of this build's factorization beyond fplus_options (fplus_amd_default_options: all zero).
  *   precision      the block updates (the Schur-complement products of the supernodal LU): 0 FP64 (the drop-in's);
  *                  1 the FP32 matrix cores (operands rounded to FP32, FP32 accumulation, the result subtracted in FP64);
