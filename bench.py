@@ -1,20 +1,34 @@
-"""Benchmark of the FP64 GPU factor + solve path: seconds per system.
+"""Benchmark: factor + solve seconds per system at backward error <= 1e-13 -- learned mixed precision vs the plain FP64
+GPU path vs the CPU.
 
-A step is one factorization of the workload's matrix from values already resident in HBM (the permutation into the
-analysed order, Ruiz, the supernodal LU) plus one solve of 16 right-hand sides, also resident in HBM, through the C
-interface (fplus_amd_factor_device, fplus_amd_solve_device).  The symbolic analysis is done once, before the timed
-region (it is reused across refactorizations of one pattern).  Prints one JSON line.
+Workload: the complex shift-invert pencil A = L - sigma M, 64 modes x 4 fields x 1024 coefficients (n = 262144),
+sigma = 0.3 + 0.7i, row and column scales 10^U(-8, 8), 16 right-hand sides N(0,1), values and right-hand sides
+resident in HBM when a timed step starts.  The symbolic analysis (orders, supernode partition) is made once, before the
+timed region.  Legs, each a whole system per step:
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu] [--verbose]
+  fp64_gpu             the plain FP64 drop-in: the pivoting factorization (fplus_amd_factor_device) + the direct solve
+                       of the 16 right-hand sides (fplus_amd_solve_device).
+  fp64_refactor        fplus_amd_refactor_solve with FP64 updates: the factorization replayed in the pivot order of a
+                       factorization of the pencil at the neighbouring shift sigma - 0.01i (made once, before the timed
+                       region, as in a shift sweep), then the refined solve (which stops at once when the direct solve
+                       already meets the target).
+  fp32_refactor_ir     the same with every block update on the FP32 matrix cores, GMRES-IR to the target.
+  learned_refactor_ir  the same with the learned selector deciding per block (fplus_amd/data/selector_shiftinvert.npz,
+                       trained by fplus_amd/train_selector.py), GMRES-IR, and the FP64 refactorization as the fallback
+                       when the refinement stalls.  This is `value`.
+
+The target is the equilibrated normwise backward error ||Dr (b - A x)|| / (||Dr A Dc||_1 ||x / Dc|| + ||Dr b||) <= 1e-13 for
+every right-hand side (the refined solves run to it); every leg's result is checked on the host with it, the normwise
+one (||A x - b|| / (||A|| ||x|| + ||b||), which the 16 orders of scaling make vacuous) and the componentwise one.
+
+The line also carries the kernels of one FP64 step timed with HIP events on the library's stream (an untimed step run
+after the timed ones: events around the ~170k launches slow a step by about a tenth), the roofline of the dominant
+kernel, and a CPU baseline: oracle/cpu_panel.c (the same numeric path, panel-blocked, OpenMP on the host threads, no
+BLAS is installed) timed directly on a smaller member of the same family, median of 3 runs, scaled to the workload by
+the factor-flops ratio (at most 50x).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--legs a,b,..] [--no-cpu] [--verbose]
     python bench.py --copy-probe      # 3 device-to-device copies of 1 GiB: the HBM counters' calibration run
-
-The line carries the backward error of every right-hand side of the last step, the solve alone on that step's
-factorization with 1 and with 16 right-hand sides (solve_s), the roofline of the dominant kernel, and a CPU baseline.  The kernels are timed with HIP events on the library's stream around every launch, in one more
-step run right after the timed ones: events on the ~175k launches of a step slow it by about a tenth, so the timed
-steps run without them (the profiled step's own time is in the line as profiled_step_s).  The CPU baseline is the reference's own
-supernodal kernel (oracle/_ref, compiled from the reference's sources; no CBLAS is installed, so its loop form) or the
-oracle, on one host thread, timed on a smaller member of the same matrix family and scaled to the workload by the
-ratio of the factor flops.
 """
 import argparse
 import ctypes as C
@@ -32,23 +46,21 @@ sys.path.insert(0, ROOT)
 import fplus_amd  # noqa: E402
 from fplus_amd import _lib, gen  # noqa: E402
 
-# name: (generator, keyword arguments, description); "sample" is the CPU baseline's smaller member
 WORKLOADS = {
-    "shiftinvert-16x4x256": (gen.shift_invert, dict(modes=16, nfields=4, ncoef=256), "complex shift-invert pencil"),
-    "shiftinvert-32x4x512": (gen.shift_invert, dict(modes=32, nfields=4, ncoef=512), "complex shift-invert pencil"),
-    "shiftinvert-64x4x1024": (gen.shift_invert, dict(modes=64, nfields=4, ncoef=1024), "complex shift-invert pencil"),
-    "convdiff2d-96": (gen.convdiff2d, dict(g=96), "2D convection-diffusion"),
-    "convdiff3d-32": (gen.convdiff3d, dict(g=32), "3D convection-diffusion"),
-}
-CPU_SAMPLE = {
-    "shiftinvert": (gen.shift_invert, dict(modes=8, nfields=4, ncoef=128)),
-    "convdiff2d": (gen.convdiff2d, dict(g=96)),
-    "convdiff3d": (gen.convdiff3d, dict(g=16)),
+    "shiftinvert-64x4x1024": dict(modes=64, nfields=4, ncoef=1024),
+    "shiftinvert-16x4x1024": dict(modes=16, nfields=4, ncoef=1024),
+    "shiftinvert-8x4x256": dict(modes=8, nfields=4, ncoef=256),
 }
 DEFAULT_WORKLOAD = "shiftinvert-64x4x1024"   # n = 262144, the plain FP64 GPU drop-in configuration
+SIGMA = 0.3 + 0.7j
+SIGMA_PREV = 0.3 + 0.69j                     # the neighbouring shift whose pivot order the refactorizations replay
+CPU_SAMPLE = dict(modes=8, nfields=4, ncoef=1024)   # n = 32768, the CPU baseline's member of the family
 PEAK_FP64_MFMA = 78.6e12      # MI355X dense FP64 matrix, FLOP/s
 PEAK_HBM = 8.0e12             # MI355X HBM3E, B/s
 NRHS = 16
+TOL = 1e-13
+LEGS = ("fp64_gpu", "fp64_refactor", "fp32_refactor_ir", "learned_refactor_ir")
+SELECTOR = os.path.join(ROOT, "fplus_amd", "data", "selector_shiftinvert.npz")
 # the dominant kernel's HBM bytes per launch, from the rocprofv3 FETCH_SIZE and WRITE_SIZE passes summarized there
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blk_upd-hbm_traffic-shiftinvert-64x4x1024.txt")
 
@@ -97,53 +109,69 @@ def dev_download(p, a):
         raise RuntimeError("download failed")
 
 
-def factor_flops(n, Ap, Ai, Ax, sym):
-    """c * sum_k l_k u_k of the structure (the same count for any device or host factorization of these orders)"""
-    num = fplus_amd.Numeric(sym, Ap, Ai, Ax)
-    return fplus_amd.stats(num)["flops"], num
+def dev_copy(dst, src, nbytes):
+    if _lib.lib().fplus_amd_dev_copy(dst, src, nbytes, 2) != 0:
+        raise RuntimeError("device copy failed")
 
 
-def cpu_baseline(workload_flops, family):
-    """the reference column kernel (or the oracle) on one thread, on the family's CPU sample, scaled by flops"""
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(workload_flops, runs=3):
+    """oracle/cpu_panel.c on the family's CPU member, all host threads of the box's share (at most 16), median of `runs`"""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
-    fn, kw = CPU_SAMPLE[family]
-    n, Ap, Ai, Ax = fn(**kw)
+    n, Ap, Ai, Ax = gen.shift_invert(**CPU_SAMPLE, sigma=SIGMA)
     sym = fplus_amd.Symbolic(n, Ap, Ai)
-    fl, _ = factor_flops(n, Ap, Ai, Ax, sym)
-    Bp, Bi, Bx = O.permute(n, Ap, Ai, Ax, sym.row_perm, sym.col_perm)
-    kind = "reference" if O.ref_available() else "port"
-    b = gen.rhs(n, NRHS, np.iscomplexobj(Ax))
-    t0 = time.perf_counter()
-    if kind == "reference":
-        x, Dr, Dc = O.ref_ruiz(n, Bp, Bi, Bx)
-        f = O.ref_factor_sn(n, Bp, Bi, x, relax_rows=32, panel=512)
-        for r in range(NRHS):
-            xs = np.ascontiguousarray(b[:, r])
-            O.ref().fplus_tri_solve(n, O._p64(f["Lp"]), O._p64(f["Li"]), f["Lx"].ctypes.data, O._p64(f["Up"]), O._p64(f["Ui"]),
-                                    f["Ux"].ctypes.data, xs.ctypes.data, 0, int(np.iscomplexobj(Ax)))
-    else:
-        num = O.Numeric(n, Ap, Ai, Ax, sym.row_perm, sym.col_perm)
-        num.solve(b)
-    t = time.perf_counter() - t0
-    scaled = t * workload_flops / fl
-    desc = ", ".join(f"{k}={v}" for k, v in kw.items())
-    return {"value": scaled, "unit": "s/system", "cores": 1, "kind": kind,
-            "sample": f"{family}({desc}), n={n}: factor (Ruiz + supernodal panel kernel, panel 512, relax 32, no BLAS) + "
-                      f"{NRHS} solves in {t:.2f} s on one "
-                      f"thread, scaled by the factor flops ratio {workload_flops:.3e}/{fl:.3e}; CPU {platform.processor() or 'x86_64'}, "
-                      f"{os.cpu_count()} logical CPUs on the host"}
+    b = gen.rhs(n, NRHS, True)
+    threads = min(16, os.cpu_count() or 1)
+    # the sample's factor flops as the GPU path counts them (the same structure as the workload's count; the blocked
+    # CPU path's own count is a little lower: it leaves out rows its updates leave exactly zero)
+    num = fplus_amd.Numeric(sym, Ap, Ai, Ax)
+    sample_flops = fplus_amd.stats(num)["flops"]
+    del num
+    times, info = [], None
+    for r in range(runs):
+        t0 = time.perf_counter()
+        x, info = O.cpu_panel_factor_solve(n, Ap, Ai, Ax, sym.row_perm, sym.col_perm, b, panel=256, threads=threads)
+        times.append(time.perf_counter() - t0)
+        log(f"cpu baseline run {r + 1}/{runs}: {times[-1]:.2f} s")
+    import scipy.sparse as sp
+    from fplus_amd import accuracy as acc
+    A = sp.csc_matrix((Ax, Ai, Ap), shape=(n, n))
+    comp = acc.componentwise(A, x, b)
+    med = float(np.median(times))
+    ratio = workload_flops / sample_flops
+    model = platform.processor() or "x86_64"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    desc = ", ".join(f"{k}={v}" for k, v in CPU_SAMPLE.items())
+    return {"value": med * ratio, "unit": "s/system", "cores": threads, "kind": "port",
+            "sample": f"shift_invert({desc}), n={n}, sigma {SIGMA}: permutation + Ruiz + panel-blocked supernodal LU "
+                      f"(oracle/cpu_panel.c, panel 256, relax 32, OpenMP on {threads} threads, own blocked loops: no BLAS "
+                      f"installed) + {NRHS} solves, median {med:.2f} s of {runs} runs ({', '.join(f'{t:.2f}' for t in times)}; "
+                      f"factor {info['factor_s']:.2f} s at {info['flops'] / info['factor_s'] / 1e9:.1f} GFLOP/s, componentwise "
+                      f"backward error {comp:.1e}), scaled by the factor-flops ratio {workload_flops:.3e}/{sample_flops:.3e} = "
+                      f"{ratio:.1f}; CPU {model}, {os.cpu_count()} logical CPUs visible",
+            "sample_s": med, "flops_ratio": ratio, "sample_flops": sample_flops, "sample_flops_cpu_count": info["flops"]}
 
 
 FLOP_KERNELS = {"blk_upd", "blk_inverse", "blk_uinv", "gemm_sub", "getf2"}
 BYTE_KERNELS = {"lfwd_gather", "lfwd_pull", "lfwd_push", "ub_dots", "ub_red", "ub_prod", "utdots", "ut_prod", "ltdots", "lt_prod",
                 "ruiz_rowmax", "ruiz_scale_colmax", "ruiz_scale_cols"}
+PROFILED = ("blk_upd", "blk_inverse", "blk_uinv", "gemm_sub", "getf2", "trsm_sub", "lfwd_gather", "lfwd_pull", "lfwd_push", "ub_dots",
+            "ub_red", "ub_prod", "utdots", "ut_prod", "ltdots", "lt_prod", "ruiz_rowmax", "ruiz_scale_colmax", "ruiz_scale_cols")
 
 
 def kernel_rate(name, sec, launches, work):
-    """seconds and launches of one kernel over the timed region, with its rate: TFLOP/s (and the fraction of the
-    FP64 matrix peak) for the factor's arithmetic kernels, GB/s of algorithmic bytes (and the fraction of HBM peak)
-    for Ruiz and the solves"""
+    """seconds and launches of one kernel, with its rate: TFLOP/s (and the fraction of the FP64 matrix peak) for the
+    factor's arithmetic kernels, GB/s of algorithmic bytes (and the fraction of HBM peak) for Ruiz and the solves"""
     r = {"s": sec, "launches": launches}
     if work > 0 and sec > 0:
         if name in FLOP_KERNELS:
@@ -159,7 +187,9 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
+    ap.add_argument("--legs", default=",".join(LEGS))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--verbose", action="store_true", help="the factorization's progress lines (for long profiled runs)")
     ap.add_argument("--copy-probe", action="store_true", help="only the HBM counters' calibration copies")
     args = ap.parse_args()
@@ -170,114 +200,192 @@ def main():
     if args.copy_probe:
         copy_probe()
         return
-    fn, kw, desc = WORKLOADS[args.workload]
-    n, Ap, Ai, Ax = fn(**kw)
-    cplx = np.iscomplexobj(Ax)
-    vs = 2 if cplx else 1
+    legs = [x for x in args.legs.split(",") if x]
+    for x in legs:
+        if x not in LEGS:
+            raise SystemExit(f"unknown leg {x}")
+    kw = WORKLOADS[args.workload]
+    import scipy.sparse as sp
+    from fplus_amd import accuracy as acc
+    n, Ap, Ai, Ax = gen.shift_invert(**kw, sigma=SIGMA)
+    _, _, _, Ax_prev = gen.shift_invert(**kw, sigma=SIGMA_PREV)
+    A = sp.csc_matrix((Ax, Ai, Ap), shape=(n, n))
+    log(f"workload {args.workload}: n {n}, nnz {Ap[-1]}")
     sym = fplus_amd.Symbolic(n, Ap, Ai)
     L = _lib.lib()
     opts = fplus_amd.default_options(verbose=2 if args.verbose else 0)
     d_Ax = dev_upload(np.ascontiguousarray(Ax))
-    b = gen.rhs(n, NRHS, cplx)
+    b = gen.rhs(n, NRHS, True)
     bf = np.asfortranarray(b)
-    d_b = dev_upload(bf)
+    d_b0 = dev_upload(bf)              # the right-hand sides, kept
+    d_b = dev_upload(bf)               # overwritten with x by each step
     err = C.create_string_buffer(512)
+    P64 = _lib.I64P
 
-    def step():
-        num = C.c_void_p()
-        rc = L.fplus_amd_factor_device(sym.ptr, Ap.ctypes.data_as(_lib.I64P), Ai.ctypes.data_as(_lib.I64P), d_Ax, int(cplx),
-                                       C.byref(opts), C.byref(num), err, 512)
-        _lib.check(rc, err)
-        rc = L.fplus_amd_solve_device(num, d_b, NRHS, n, 0, err, 512)
-        _lib.check(rc, err)
-        return num
+    def reset_b():
+        dev_copy(d_b, d_b0, bf.nbytes)
+        L.fplus_amd_dev_sync()
 
-    for _ in range(args.warmup):
-        num = step()
+    def check_x(Dr, Dc):
+        x = np.empty_like(bf)
+        dev_download(d_b, x)
+        return acc.all_measures(A, x, b, Dr, Dc)
+
+    out_legs = {}
+    kern, st64, profiled_s = {}, None, None
+    if "fp64_gpu" in legs:
+        def step():
+            num = C.c_void_p()
+            _lib.check(L.fplus_amd_factor_device(sym.ptr, Ap.ctypes.data_as(P64), Ai.ctypes.data_as(P64), d_Ax, 1, C.byref(opts),
+                                                 C.byref(num), err, 512), err)
+            _lib.check(L.fplus_amd_solve_device(num, d_b, NRHS, n, 0, err, 512), err)
+            return num
+
+        for _ in range(args.warmup):
+            reset_b()
+            L.fplus_numeric_free(step())
+        times, num = [], None
+        for _ in range(args.steps):
+            if num is not None:
+                L.fplus_numeric_free(num)
+            reset_b()
+            t0 = time.perf_counter()
+            num = step()
+            L.fplus_amd_dev_sync()
+            times.append(time.perf_counter() - t0)
+        Dr = np.ctypeslib.as_array(L.fplus_numeric_Dr(num), (n,)).copy()
+        Dc = np.ctypeslib.as_array(L.fplus_numeric_Dc(num), (n,)).copy()
+        meas = check_x(Dr, Dc)
+        # the solve alone on this factorization, with one and with 16 right-hand sides
+        solve_s = {}
+        for k in (1, NRHS):
+            reset_b()
+            ts = time.perf_counter()
+            _lib.check(L.fplus_amd_solve_device(num, d_b, k, n, 0, err, 512), err)
+            L.fplus_amd_dev_sync()
+            solve_s[f"nrhs{k}"] = time.perf_counter() - ts
         L.fplus_numeric_free(num)
-        L.fplus_amd_dev_copy(d_b, C.c_void_p(bf.ctypes.data), bf.nbytes, 0)
-    L.fplus_amd_dev_sync()
-    t0 = time.perf_counter()
-    nums = []
-    for _ in range(args.steps):
-        if nums:   # a refactorization loop keeps one factorization: the previous one goes before the next is made
-            L.fplus_numeric_free(nums.pop())
-        nums.append(step())
-    L.fplus_amd_dev_sync()
-    t1 = time.perf_counter()
-    sec = (t1 - t0) / args.steps
-    # one more step with HIP events around every launch: the kernels' times for the breakdown and the roofline
-    L.fplus_amd_dev_copy(d_b, C.c_void_p(bf.ctypes.data), bf.nbytes, 0)
-    L.fplus_amd_profile(1)
-    t2 = time.perf_counter()
-    nums.append(step())
-    L.fplus_amd_dev_sync()
-    t3 = time.perf_counter()
-    L.fplus_amd_profile(0)
-    # the solve alone on the last factorization, with one and with 16 right-hand sides (wall clock, vectors in HBM)
-    solve_s = {}
-    for k in (1, NRHS):
-        L.fplus_amd_dev_copy(d_b, C.c_void_p(bf.ctypes.data), bf.nbytes, 0)
+        # one more step with HIP events around every launch: the kernels' times for the breakdown and the roofline
+        reset_b()
+        L.fplus_amd_profile(1)
+        t2 = time.perf_counter()
+        num = step()
         L.fplus_amd_dev_sync()
-        ts = time.perf_counter()
-        _lib.check(L.fplus_amd_solve_device(nums[-1], d_b, k, n, 0, err, 512), err)
+        profiled_s = time.perf_counter() - t2
+        L.fplus_amd_profile(0)
+        st64 = _lib.Stats()
+        L.fplus_amd_numeric_stats(num, C.byref(st64))
+        L.fplus_numeric_free(num)
+        for name in PROFILED:
+            ms, cnt, work = C.c_double(), C.c_int64(), C.c_double()
+            L.fplus_amd_profile_get(name.encode(), C.byref(ms), C.byref(cnt), C.byref(work))
+            if cnt.value:
+                kern[name] = (ms.value / 1e3, cnt.value, work.value)
+        out_legs["fp64_gpu"] = {"s_per_system": float(np.median(times)), "steps_s": times, "solve_s": solve_s,
+                                "backward_error": meas, "meets_1e-13": meas["equilibrated"] <= TOL}
+        log(f"fp64_gpu: {out_legs['fp64_gpu']['s_per_system']:.3f} s/system, {meas}")
+
+    ref_legs = [x for x in legs if x != "fp64_gpu"]
+    setup = None
+    if ref_legs:
+        # the pivot order the refactorizations replay: a factorization of the pencil at the neighbouring shift, made once
+        sel = fplus_amd.Selector.load(SELECTOR)
+        d_Ax_prev = dev_upload(np.ascontiguousarray(Ax_prev))
+        aopt = fplus_amd.amd_options(refactor=1, selector=sel)
+        num = C.c_void_p()
+        t0 = time.perf_counter()
+        _lib.check(L.fplus_amd_factor_ex(sym.ptr, Ap.ctypes.data_as(P64), Ai.ctypes.data_as(P64), d_Ax_prev, 1, 1, C.byref(opts),
+                                         C.byref(aopt), C.byref(num), err, 512), err)
         L.fplus_amd_dev_sync()
-        solve_s[f"nrhs{k}"] = time.perf_counter() - ts
-    # correctness of the last step: solve the 16 right-hand sides again from b and check the backward errors
-    L.fplus_amd_dev_copy(d_b, C.c_void_p(bf.ctypes.data), bf.nbytes, 0)
-    _lib.check(L.fplus_amd_solve_device(nums[-1], d_b, NRHS, n, 0, err, 512), err)
-    x = np.empty_like(bf)
-    dev_download(d_b, x)
-    import scipy.sparse as sp
-    from fplus_amd import accuracy as acc
-    A = sp.csc_matrix((Ax, Ai, Ap), shape=(n, n))
-    Dr = np.ctypeslib.as_array(L.fplus_numeric_Dr(nums[-1]), (n,)).copy()
-    Dc = np.ctypeslib.as_array(L.fplus_numeric_Dc(nums[-1]), (n,)).copy()
-    bwd = acc.all_measures(A, x, b, Dr, Dc)
-    st = _lib.Stats()
-    L.fplus_amd_numeric_stats(nums[-1], C.byref(st))
-    for nm in nums:
-        L.fplus_numeric_free(nm)
-    # the kernels' sums over the timed region
-    kern = {}
-    for name in ("blk_upd", "blk_inverse", "blk_uinv", "gemm_sub", "getf2", "trsm_sub", "lfwd_gather", "lfwd_pull", "lfwd_push", "ub_dots", "ub_red", "ub_prod", "utdots", "ut_prod", "ltdots",
-                 "lt_prod", "ruiz_rowmax", "ruiz_scale_colmax", "ruiz_scale_cols"):
-        ms, cnt, work = C.c_double(), C.c_int64(), C.c_double()
-        L.fplus_amd_profile_get(name.encode(), C.byref(ms), C.byref(cnt), C.byref(work))
-        if cnt.value:
-            kern[name] = (ms.value / 1e3, cnt.value, work.value)
-    dom = max(kern, key=lambda k: kern[k][0])
-    t_dom, cnt_dom, work_dom = kern[dom]
-    bound = "mfma" if dom in FLOP_KERNELS else "hbm"
-    achieved = work_dom / t_dom
-    peak = PEAK_FP64_MFMA if bound == "mfma" else PEAK_HBM
-    roof = {"kernel": dom, "bound": bound, "achieved": achieved / (1e12 if bound == "mfma" else 1e9),
-            "peak": peak / (1e12 if bound == "mfma" else 1e9), "unit": "TFLOP/s" if bound == "mfma" else "GB/s",
-            "frac": achieved / peak, "traffic": measured_traffic(args.workload, dom), "launches": cnt_dom,
-            "avg_launch_ms": 1e3 * t_dom / cnt_dom, "work_per_launch": work_dom / cnt_dom}
-    if dom == "blk_upd":
-        roof["algorithmic_bytes_per_launch"] = st.gemm_bytes / cnt_dom
-    family = args.workload.split("-")[0]
-    cpu = None if args.no_cpu else cpu_baseline(st.flops, family)
+        setup = {"pivoting_factorization_with_record_s": time.perf_counter() - t0, "sigma": str(SIGMA_PREV)}
+        L.fplus_amd_dev_free(d_Ax_prev)
+        st = _lib.Stats()
+        L.fplus_amd_numeric_stats(num, C.byref(st))
+        setup["replay_bytes"] = st.replay_bytes
+        log(f"refactorization setup: {setup}")
+        ro = _lib.RefineOptions()
+        L.fplus_amd_default_refine_options(C.byref(ro))
+        ro.tol = TOL
+        prec = {"fp64_refactor": 0, "fp32_refactor_ir": 1, "learned_refactor_ir": 2}
+        for leg in ref_legs:
+            def step():
+                info = _lib.RefineInfo()
+                fb = C.c_int(0)
+                _lib.check(L.fplus_amd_refactor_solve(num, d_Ax, 1, d_b, 1, NRHS, n, prec[leg], C.byref(ro), C.byref(info),
+                                                      C.byref(fb), err, 512), err)
+                return info, fb.value
+
+            for _ in range(args.warmup):
+                reset_b()
+                step()
+            times = []
+            for _ in range(args.steps):
+                reset_b()
+                t0 = time.perf_counter()
+                info, fb = step()
+                L.fplus_amd_dev_sync()
+                times.append(time.perf_counter() - t0)
+            Dr = np.ctypeslib.as_array(L.fplus_numeric_Dr(num), (n,)).copy()
+            Dc = np.ctypeslib.as_array(L.fplus_numeric_Dc(num), (n,)).copy()
+            meas = check_x(Dr, Dc)
+            st = _lib.Stats()
+            L.fplus_amd_numeric_stats(num, C.byref(st))
+            r = {"s_per_system": float(np.median(times)), "steps_s": times, "backward_error": meas,
+                 "meets_1e-13": meas["equilibrated"] <= TOL,
+                 "refine": {"outer": info.outer, "inner": info.inner, "converged": info.converged, "bwd_first": info.bwd_first,
+                            "bwd_last": info.bwd_last},
+                 "fallback_to_fp64": fb, "refactor_s": st.t_total, "pivots_replaced": st.refactor_replaced,
+                 "min_pivot_ratio": st.refactor_min_ratio, "f32_blocks": st.f32_blocks, "blocks": st.nblocks,
+                 "f32_update_flop_share": st.f32_flop_share}
+            out_legs[leg] = r
+            log(f"{leg}: {r['s_per_system']:.3f} s/system, refine {r['refine']}, fallback {fb}, {meas}")
+        L.fplus_numeric_free(num)
+        del sel
+
+    roof = None
+    if kern:
+        dom = max(kern, key=lambda k: kern[k][0])
+        t_dom, cnt_dom, work_dom = kern[dom]
+        bound = "mfma" if dom in FLOP_KERNELS else "hbm"
+        achieved = work_dom / t_dom
+        peak = PEAK_FP64_MFMA if bound == "mfma" else PEAK_HBM
+        roof = {"kernel": dom, "bound": bound, "achieved": achieved / (1e12 if bound == "mfma" else 1e9),
+                "peak": peak / (1e12 if bound == "mfma" else 1e9), "unit": "TFLOP/s" if bound == "mfma" else "GB/s",
+                "frac": achieved / peak, "traffic": measured_traffic(args.workload, dom), "launches": cnt_dom,
+                "avg_launch_ms": 1e3 * t_dom / cnt_dom, "work_per_launch": work_dom / cnt_dom}
+        if dom == "blk_upd":
+            roof["algorithmic_bytes_per_launch"] = st64.gemm_bytes / cnt_dom
+    cpu = None
+    if not args.no_cpu and st64 is not None and args.workload == DEFAULT_WORKLOAD:
+        cpu = cpu_baseline(st64.flops, args.cpu_runs)
+    head = "learned_refactor_ir" if "learned_refactor_ir" in out_legs else (legs[0] if legs else None)
+    value = out_legs[head]["s_per_system"] if head else None
+    base = out_legs.get("fp64_gpu", {}).get("s_per_system")
     out = {
         "metric": "factor+solve s/system at bwd err <=1e-13: mixed/learned vs FP64 GPU vs CPU",
-        "value": sec, "unit": "s/system", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": sec * 1e3, "profiled_step_s": t3 - t2, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c128" if cplx else "f64", "data": "synthetic",
-        "config": {"workload": args.workload, "matrix": desc, **kw, "n": n, "nnz": int(Ap[-1]), "nrhs": NRHS,
-                   "precision": "fp64 (plain GPU drop-in)", "ordering": "fplus_analyze defaults", "relax_rows": 32},
-        # the headline's stated measure (the reference tests' normwise one) and the two scale-invariant ones, each the
-        # maximum over the right-hand sides of the last step
-        "backward_error_max": bwd["normwise"],
-        "backward_error": bwd,
-        "solve_s": solve_s,
-        "factor": {"flops": st.flops, "gemm_flops": st.gemm_flops, "gemm_bytes": st.gemm_bytes, "nnzL": st.nnzL, "nnzU": st.nnzU,
-                   "supernodes": st.nsupernodes, "panels": st.panels, "levels": st.levels,
-                   "t_ruiz": st.t_ruiz, "t_factor": st.t_factor, "t_host": st.t_host, "gflops": st.flops / sec / 1e9},
+        "value": value, "unit": "s/system", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": value * 1e3 if value else None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128", "data": "synthetic",
+        "config": {"workload": args.workload, "matrix": "complex shift-invert pencil", **kw, "n": n, "nnz": int(Ap[-1]),
+                   "sigma": str(SIGMA), "scales": "10^U(-8,8)", "nrhs": NRHS, "value_leg": head,
+                   "backward_error_target": TOL, "backward_error_measure": "equilibrated normwise, every right-hand side",
+                   "ordering": "fplus_analyze defaults", "relax_rows": 32},
+        "legs": out_legs,
+        "speedup_vs_fp64_gpu": {k: base / v["s_per_system"] for k, v in out_legs.items()} if base else None,
+        "refactor_setup": setup,
+        "backward_error_max": max((v["backward_error"]["equilibrated"] for v in out_legs.values()), default=None),
+        "profiled_step_s": profiled_s,
+        "factor": None if st64 is None else {
+            "flops": st64.flops, "gemm_flops": st64.gemm_flops, "gemm_bytes": st64.gemm_bytes, "nnzL": st64.nnzL,
+            "nnzU": st64.nnzU, "supernodes": st64.nsupernodes, "panels": st64.panels, "levels": st64.levels,
+            "t_ruiz": st64.t_ruiz, "t_factor": st64.t_factor, "t_host": st64.t_host,
+            "gflops_fp64_gpu": st64.flops / out_legs["fp64_gpu"]["s_per_system"] / 1e9},
         "kernels": {k: kernel_rate(k, *v) for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    for p in (d_Ax, d_b0, d_b):
+        L.fplus_amd_dev_free(p)
     print(json.dumps(out))
 
 

@@ -20,9 +20,9 @@
 namespace fpa {
 namespace k {
 // features of the blocks blks[0 .. nb) into feat[block * SEL_NF ..]; colmax: the panel's column maxima (panel-local from
-// column j0); Dr, Dc: the scalings of the analysed matrix
+// column j0); Dr, Dc: the scalings of the analysed matrix; fmax: 3 per block, zero on entry and left zero
 void block_features(int vs, int32_t nb, const int32_t* blks, const BlockDesc* blocks, const double* colmax, int64_t j0,
-                    const double* Dr, const double* Dc, int64_t n, float* feat, hipStream_t s);
+                    const double* Dr, const double* Dc, int64_t n, unsigned long long* fmax, float* feat, hipStream_t s);
 // P(class 1) of samples idx[t] (or t when idx is null) of feat: into prob[t] and/or mark[idx[t]] = P > thr
 void sel_infer(int64_t ns, const int32_t* idx, const float* feat, const float* model, float thr, float* prob, uint8_t* mark,
                hipStream_t s);

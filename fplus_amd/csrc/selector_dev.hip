@@ -2,6 +2,7 @@
 // FP32) that decides per block whether its updates of later columns may run on the FP32 matrix cores, and its batched
 // training (forward, softmax cross-entropy, backward, SGD with momentum) in one workgroup per step.
 #include "dev_common.h"
+#include "prof.h"
 #include "selector_dev.h"
 
 #include <cfloat>
@@ -15,71 +16,99 @@ __device__ inline float lg10(double v) { return (float)fmin(40.0, fmax(-40.0, lo
 
 template <class T> __device__ inline double mag(T v) { return Sc<T>::absval(v); }
 
-// one workgroup per new block: the block's features (fields in selector_dev.h), from its values (L below the unit
-// diagonal, U11 on and above it), its inverse form, L11^-1, the columns' largest entries (colmax, panel-local from j0)
-// and the row and column scalings in the analysed order
+// the three largest moduli of a block -- its inverse form (nr x w), its L21 (rows w.. of its values) and L11^-1 (w x w)
+// -- over a grid of (block, column group): group g takes the columns g, g + FG, ...; the maxima meet in fmax[3 b ..]
+// (bmax: non-negative doubles as bits, max-accumulated; zero on entry)
+constexpr int FG = 16;
 template <class T>
-__global__ __launch_bounds__(TB) void k_features(int32_t nb, const int32_t* __restrict__ blks, const BlockDesc* __restrict__ blocks,
-                                                  const double* __restrict__ colmax, int64_t j0, const double* __restrict__ Dr,
-                                                  const double* __restrict__ Dc, int64_t n, float* __restrict__ feat)
+__global__ __launch_bounds__(TB) void k_feat_max(const int32_t* __restrict__ blks, const BlockDesc* __restrict__ blocks,
+                                                  unsigned long long* __restrict__ bmax)
 {
-    __shared__ double red[8][TB / 64];
     const int b = blks[blockIdx.x];
     const BlockDesc B = blocks[b];
     const T* V = reinterpret_cast<const T*>(B.vals);
     const T* IV = reinterpret_cast<const T*>(B.ivals);
     const T* LI = reinterpret_cast<const T*>(B.linv);
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int64_t nr = B.nrows, w = B.w;
-    // 0 max |IV|, 1 max |L21|, 2 max |U_jj|, 3 min |U_jj| (as its negation), 4 min |U_jj| / colmax (negated), 5 max
-    // |L11^-1|, 6 max Dc / min Dc (log), 7 max Dr / min Dr (log)
-    double m[8] = {0, 0, 0, -DBL_MAX, -DBL_MAX, 0, 0, 0};
-    double dcmax = 0, dcmin = DBL_MAX, drmax = 0, drmin = DBL_MAX;
-    for (int64_t e = tid; e < nr * w; e += TB) {
-        const int64_t c = e / nr, r = e - c * nr;
-        m[0] = fmax(m[0], mag(IV[e]));
-        if (r >= w) m[1] = fmax(m[1], mag(V[e]));
+    const int32_t nr = (int32_t)B.nrows, w = (int32_t)B.w;
+    double m0 = 0, m1 = 0, m2 = 0;
+    for (int32_t c = blockIdx.y; c < w; c += FG) {
+        const T* ivc = IV + (int64_t)c * nr;
+        const T* vc = V + (int64_t)c * nr;
+        for (int32_t r = threadIdx.x; r < nr; r += TB) {
+            m0 = fmax(m0, mag(ivc[r]));
+            if (r >= w) m1 = fmax(m1, mag(vc[r]));
+        }
+        for (int32_t r = threadIdx.x; r < w; r += TB) m2 = fmax(m2, mag(LI[(int64_t)c * w + r]));
     }
-    for (int64_t e = tid; e < w * w; e += TB) m[5] = fmax(m[5], mag(LI[e]));
-    for (int64_t c = tid; c < w; c += TB) {
-        const double d = mag(V[c * nr + c]);
-        m[2] = fmax(m[2], d);
-        m[3] = fmax(m[3], -d);
-        const double cm = colmax[B.c0 - j0 + c];
-        m[4] = fmax(m[4], -(cm > 0 ? d / cm : 1.0));
-        const double dc = Dc[B.c0 + c], dr = Dr[B.rows[c]];
-        dcmax = fmax(dcmax, dc); dcmin = fmin(dcmin, dc);
-        drmax = fmax(drmax, dr); drmin = fmin(drmin, dr);
-    }
-    m[6] = fmax(dcmax, 0.0);
-    m[7] = fmax(drmax, 0.0);
-    double mins[2] = {dcmin, drmin};
     for (int off = 32; off > 0; off >>= 1) {
-        for (int q = 0; q < 8; q++) m[q] = fmax(m[q], __shfl_xor(m[q], off));
-        for (int q = 0; q < 2; q++) mins[q] = fmin(mins[q], __shfl_xor(mins[q], off));
+        m0 = fmax(m0, __shfl_xor(m0, off));
+        m1 = fmax(m1, __shfl_xor(m1, off));
+        m2 = fmax(m2, __shfl_xor(m2, off));
     }
-    __shared__ double rmin[2][TB / 64];
-    if (lane == 0) {
-        for (int q = 0; q < 8; q++) red[q][wv] = m[q];
-        for (int q = 0; q < 2; q++) rmin[q][wv] = mins[q];
+    if ((threadIdx.x & 63) == 0) {
+        atomic_max_nonneg(bmax + 3 * (int64_t)b, m0);
+        atomic_max_nonneg(bmax + 3 * (int64_t)b + 1, m1);
+        atomic_max_nonneg(bmax + 3 * (int64_t)b + 2, m2);
     }
+}
+
+// one workgroup per new block: the block's features (fields in selector_dev.h) from the maxima of k_feat_max (read,
+// from bmax, then reset to zero for the next factorization), its pivots (U11's diagonal, on the diagonal of its values) against the
+// columns' largest entries (colmax, panel-local from j0), and the row and column scalings in the analysed order
+template <class T>
+__global__ __launch_bounds__(TB) void k_features(int32_t nb, const int32_t* __restrict__ blks, const BlockDesc* __restrict__ blocks,
+                                                  const double* __restrict__ colmax, int64_t j0, const double* __restrict__ Dr,
+                                                  const double* __restrict__ Dc, int64_t n, unsigned long long* __restrict__ bmax,
+                                                  float* __restrict__ feat)
+{
+    const int b = blks[blockIdx.x];
+    const BlockDesc B = blocks[b];
+    const T* V = reinterpret_cast<const T*>(B.vals);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int32_t nr = (int32_t)B.nrows, w = (int32_t)B.w;
+    // 0 max |U_jj|, 1 min |U_jj| (negated), 2 min |U_jj| / colmax (negated), 3 max Dc, 4 max Dr, 5 min Dc (negated),
+    // 6 min Dr (negated)
+    double m[7] = {0, -DBL_MAX, -DBL_MAX, 0, 0, -DBL_MAX, -DBL_MAX};
+    for (int32_t c = tid; c < w; c += TB) {
+        const double d = mag(V[(int64_t)c * nr + c]);
+        m[0] = fmax(m[0], d);
+        m[1] = fmax(m[1], -d);
+        const double cm = colmax[B.c0 - j0 + c];
+        m[2] = fmax(m[2], -(cm > 0 ? d / cm : 1.0));
+        const double dc = Dc[B.c0 + c], dr = Dr[B.rows[c]];
+        m[3] = fmax(m[3], dc);
+        m[4] = fmax(m[4], dr);
+        m[5] = fmax(m[5], -dc);
+        m[6] = fmax(m[6], -dr);
+    }
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < 7; q++) m[q] = fmax(m[q], __shfl_xor(m[q], off));
+    __shared__ double red[7][TB / 64];
+    if (lane == 0)
+        for (int q = 0; q < 7; q++) red[q][wv] = m[q];
     __syncthreads();
     if (tid == 0) {
-        for (int q = 0; q < 8; q++) for (int k = 1; k < TB / 64; k++) red[q][0] = fmax(red[q][0], red[q][k]);
-        for (int q = 0; q < 2; q++) for (int k = 1; k < TB / 64; k++) rmin[q][0] = fmin(rmin[q][0], rmin[q][k]);
+        for (int q = 0; q < 7; q++)
+            for (int k = 1; k < TB / 64; k++) red[q][0] = fmax(red[q][0], red[q][k]);
+        double mx[3];
+        for (int q = 0; q < 3; q++) {
+            const unsigned long long bits = bmax[3 * (int64_t)b + q];
+            mx[q] = __longlong_as_double((long long)bits);
+            bmax[3 * (int64_t)b + q] = 0ULL;
+        }
         float* f = feat + (int64_t)b * SEL_NF;
         f[0] = log2f((float)w);
         f[1] = log2f((float)(nr - w + 1));
         f[2] = log2f((float)nr);
-        f[3] = lg10(red[0][0]);
-        f[4] = lg10(red[1][0]);
-        f[5] = lg10(red[2][0] / fmax(-red[3][0], 1e-300));
-        f[6] = lg10(-red[4][0]);
-        f[7] = lg10(red[6][0] / fmax(rmin[0][0], 1e-300));
-        f[8] = lg10(red[7][0] / fmax(rmin[1][0], 1e-300));
-        f[9] = lg10(red[2][0]);
+        f[3] = lg10(mx[0]);
+        f[4] = lg10(mx[1]);
+        f[5] = lg10(red[0][0] / fmax(-red[1][0], 1e-300));
+        f[6] = lg10(-red[2][0]);
+        f[7] = lg10(red[3][0] / fmax(-red[5][0], 1e-300));
+        f[8] = lg10(red[4][0] / fmax(-red[6][0], 1e-300));
+        f[9] = lg10(red[0][0]);
         f[10] = (float)((double)B.c0 / (double)(n > 0 ? n : 1));
-        f[11] = lg10(red[5][0]);
+        f[11] = lg10(mx[2]);
     }
 }
 
@@ -220,16 +249,23 @@ __global__ __launch_bounds__(TB) void k_train_step(int64_t ns, const int32_t* __
 
 namespace k {
 void block_features(int vs, int32_t nb, const int32_t* blks, const BlockDesc* blocks, const double* colmax, int64_t j0,
-                    const double* Dr, const double* Dc, int64_t n, float* feat, hipStream_t s)
+                    const double* Dr, const double* Dc, int64_t n, unsigned long long* fmax, float* feat, hipStream_t s)
 {
     if (nb <= 0) return;
-    if (vs == 1) hipLaunchKernelGGL(k_features<double>, dim3(nb), dim3(TB), 0, s, nb, blks, blocks, colmax, j0, Dr, Dc, n, feat);
-    else hipLaunchKernelGGL(k_features<Zd>, dim3(nb), dim3(TB), 0, s, nb, blks, blocks, colmax, j0, Dr, Dc, n, feat);
+    ProfScope ps("sel_features", 0.0, s);
+    if (vs == 1) {
+        hipLaunchKernelGGL(k_feat_max<double>, dim3(nb, FG), dim3(TB), 0, s, blks, blocks, fmax);
+        hipLaunchKernelGGL(k_features<double>, dim3(nb), dim3(TB), 0, s, nb, blks, blocks, colmax, j0, Dr, Dc, n, fmax, feat);
+    } else {
+        hipLaunchKernelGGL(k_feat_max<Zd>, dim3(nb, FG), dim3(TB), 0, s, blks, blocks, fmax);
+        hipLaunchKernelGGL(k_features<Zd>, dim3(nb), dim3(TB), 0, s, nb, blks, blocks, colmax, j0, Dr, Dc, n, fmax, feat);
+    }
 }
 void sel_infer(int64_t ns, const int32_t* idx, const float* feat, const float* model, float thr, float* prob, uint8_t* mark,
                hipStream_t s)
 {
     if (ns <= 0) return;
+    ProfScope ps("sel_infer", 0.0, s);
     hipLaunchKernelGGL(k_infer, dim3((unsigned)((ns + TB - 1) / TB)), dim3(TB), 0, s, ns, idx, feat, model, thr, prob, mark);
 }
 void sel_train_step(int64_t ns, const int32_t* order, int64_t first, const float* feat, const float* label, float* model,

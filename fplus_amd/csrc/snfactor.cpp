@@ -1117,7 +1117,7 @@ int Driver::panel_out()
         d_newblk_.reserve(new_blocks_.size());
         up_.upload(d_newblk_.p, new_blocks_.data(), new_blocks_.size(), s_);
         k::block_features(vs_, (int32_t)new_blocks_.size(), d_newblk_.p, d_blocks_p(), d_colmax_.p, j0, opt_.sel.d_Dr, opt_.sel.d_Dc,
-                          n_, F_.d_feat.p, s_);
+                          n_, F_.d_featmax.p, F_.d_feat.p, s_);
         if (opt_.precision == 2)
             k::sel_infer((int64_t)new_blocks_.size(), d_newblk_.p, F_.d_feat.p, opt_.sel.d_model, opt_.sel.threshold, nullptr,
                          F_.d_blk_f32.p, s_);
@@ -1220,6 +1220,8 @@ int Driver::run()
         if (opt_.precision == 2 && !opt_.sel.d_model) throw std::invalid_argument("per-block precision without a selector model");
         if (!opt_.sel.d_Dr || !opt_.sel.d_Dc) throw std::invalid_argument("the selector's features need the scalings");
         F_.d_feat.alloc(nn * SEL_NF);
+        F_.d_featmax.alloc(3 * nn);
+        k::fill_u64(F_.d_featmax.p, (int64_t)(3 * nn), 0ULL, s_);
         if (opt_.precision == 3) {
             F_.d_blk_err.alloc(2 * nn);
             k::fill_u64(F_.d_blk_err.p, (int64_t)(2 * nn), 0ULL, s_);
@@ -1384,6 +1386,10 @@ int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, c
         if (precision == 2 && !sel.d_model) { errf(err, errlen, "per-block precision needs a selector"); return FPLUS_ERR_BADARG; }
         const size_t nn = (size_t)std::max<int64_t>(n, 1);
         if (F.d_feat.n < nn * SEL_NF) F.d_feat.alloc(nn * SEL_NF);
+        if (F.d_featmax.n < 3 * nn) {
+            F.d_featmax.alloc(3 * nn);
+            k::fill_u64(F.d_featmax.p, (int64_t)(3 * nn), 0ULL, s);
+        }
         if (precision == 3) {
             if (F.d_blk_err.n < 2 * nn) F.d_blk_err.alloc(2 * nn);
             k::fill_u64(F.d_blk_err.p, (int64_t)(2 * nn), 0ULL, s);
@@ -1414,7 +1420,8 @@ int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, c
         if (P.nij)
             k::blk_inverse(vs, (int32_t)P.nij, R.d_ij.p + P.ij_off, (int32_t)P.nit, R.d_it.p + P.it_off, blocks, P.iflops, s);
         if (precision >= 2 && P.nnb) {
-            k::block_features(vs, (int32_t)P.nnb, R.d_newblk.p + P.nb_off, blocks, R.d_colmax.p, P.j0, sel.d_Dr, sel.d_Dc, n, F.d_feat.p, s);
+            k::block_features(vs, (int32_t)P.nnb, R.d_newblk.p + P.nb_off, blocks, R.d_colmax.p, P.j0, sel.d_Dr, sel.d_Dc, n,
+                              F.d_featmax.p, F.d_feat.p, s);
             if (precision == 2)
                 k::sel_infer(P.nnb, R.d_newblk.p + P.nb_off, F.d_feat.p, sel.d_model, sel.threshold, nullptr, F.d_blk_f32.p, s);
         }

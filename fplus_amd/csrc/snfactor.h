@@ -107,6 +107,7 @@ struct DevFactor {
     bool coop_retried = false;   // this factorization is the retry with the one-workgroup panel LU
     DevBuf<uint8_t> d_blk_f32;   // per block: its updates of later columns on the FP32 matrix cores (precision mode 2)
     DevBuf<float> d_feat;        // per block: the selector's features (precision 2 and 3)
+    DevBuf<unsigned long long> d_featmax;   // per block: 3 maxima the features read (zero between uses)
     DevBuf<unsigned long long> d_blk_err;   // per block: the logged FP32 differences and FP64 magnitudes (precision 3)
     std::unique_ptr<Replay> replay;   // kept when the factorization records what a refactorization replays
     int precision = 0;           // the updates' precision of the last (re)factorization: 0 FP64, 1 FP32, 2 per block

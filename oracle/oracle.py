@@ -299,3 +299,45 @@ def ref_factor_sn(n, Ap, Ai, Ax, relax_rows=32, panel=1, pivot_rel=1.49011611938
                     Ui=arr(f.Ui, nU), Ux=Ux, p=arr(f.p, n), q=arr(f.q, n))
     finally:
         ref().fplus_factors_free(C.byref(f))
+
+
+# ---------------------------------------------------------------- the OpenMP panel-blocked CPU baseline (cpu_panel.c)
+_cp = None
+
+
+def cpu_panel_lib():
+    global _cp
+    if _cp is None:
+        path = os.path.join(HERE, "libcpupanel.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/libcpupanel.so is not built (make -C oracle)")
+        L = C.CDLL(path)
+        L.cpb_factor_solve.argtypes = [C.c_int64, _I64P, _I64P, _DP, C.c_int, _I64P, _I64P, C.c_int64, C.c_double, C.c_int64,
+                                       C.c_int, _DP, C.c_int64, _DP, C.c_char_p, C.c_size_t]
+        L.cpb_factor_solve.restype = C.c_int
+        _cp = L
+    return _cp
+
+
+def cpu_panel_factor_solve(n, Ap, Ai, Ax, row_perm, col_perm, b, relax_rows=32, pivot_rel=1.4901161193847656e-08, panel=256,
+                           threads=None):
+    """x = A^-1 b by the OpenMP panel-blocked CPU path, with its timings: (x, {"prep_s", "factor_s", "solve_s", "flops",
+    "supernodes", "threads"})"""
+    cplx = np.iscomplexobj(Ax)
+    Ap = np.ascontiguousarray(Ap, np.int64)
+    Ai = np.ascontiguousarray(Ai, np.int64)
+    xv = _f64view(np.ascontiguousarray(Ax, np.complex128 if cplx else np.float64))
+    rp = np.ascontiguousarray(row_perm, np.int64)
+    cp = np.ascontiguousarray(col_perm, np.int64)
+    X = np.array(np.asarray(b).reshape(n, -1), dtype=np.complex128 if cplx else np.float64, order="F", copy=True)
+    out = np.zeros(6)
+    err = C.create_string_buffer(512)
+    nth = int(threads or os.cpu_count() or 1)
+    rc = cpu_panel_lib().cpb_factor_solve(n, _p64(Ap), _p64(Ai), _pd(xv), int(cplx), _p64(rp), _p64(cp), int(relax_rows),
+                                          float(pivot_rel), int(panel), nth, C.cast(X.ctypes.data, _DP),
+                                          X.shape[1], _pd(out), err, 512)
+    if rc != 0:
+        raise OracleError(rc, err.value.decode(errors="replace"))
+    info = {"prep_s": out[0], "factor_s": out[1], "solve_s": out[2], "flops": out[3], "supernodes": int(out[4]),
+            "threads": int(out[5])}
+    return (X[:, 0] if np.asarray(b).ndim == 1 else X), info
