@@ -61,6 +61,8 @@ constexpr int RT = FPA_ROW_TILE; // rows per GEMM tile
 template <class T> struct CTile { static constexpr int v = 64; };
 template <> struct CTile<Zd> { static constexpr int v = 32; };
 constexpr int PNB = FPA_PANEL_NB;  // sub-panel width of the dense panel LU
+__device__ inline double shfl_t(double v, int src) { return __shfl(v, src); }
+__device__ inline Zd shfl_t(Zd v, int src) { return zmake(__shfl(v.x, src), __shfl(v.y, src)); }
 
 template <class T> __device__ inline T shfl_val(T v, int src);
 template <> __device__ inline double shfl_val<double>(double v, int src) { return __shfl(v, src); }
@@ -803,20 +805,17 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             for (int c = 0; c < PNB; c++) Lsub[tid][c] = (have && tid < nb) ? x[c] : Sc<T>::zero();
         }
         __syncthreads();
-        const int32_t c = a + nb + tid;   // one column per thread: the panel is at most TB columns wide
-        if (c < P) {
-            T y[PNB];
-#pragma unroll
-            for (int i = 0; i < PNB; i++) y[i] = i < nb ? Wl[(a + i) + (int64_t)c * ld] : Sc<T>::zero();
-#pragma unroll
-            for (int i = 1; i < PNB; i++) {
-                __asm__ __volatile__("" ::: "memory");   // row i's factor values are read here, not all up front
-#pragma unroll
-                for (int k = 0; k < i; k++) y[i] = Sc<T>::fms(y[i], Lsub[i][k], y[k]);
+        // L11 y = b, sixteen lanes per column, lane i holding y_i and taking y_k from lane k of its group
+        const int i = tid % PNB, lane = tid & 63;
+        for (int32_t c0 = a + nb; c0 < P; c0 += TB / PNB) {
+            const int32_t c = c0 + tid / PNB;
+            const bool mine = c < P && i < nb;
+            T y = mine ? Wl[(a + i) + (int64_t)c * ld] : Sc<T>::zero();
+            for (int kk = 0; kk < nb - 1; kk++) {
+                const T yk = shfl_t(y, (lane & ~(PNB - 1)) | kk);
+                if (i > kk) y = Sc<T>::fms(y, Lsub[i][kk], yk);
             }
-#pragma unroll
-            for (int i = 0; i < PNB; i++)
-                if (i < nb) Wl[(a + i) + (int64_t)c * ld] = y[i];
+            if (mine) Wl[(a + i) + (int64_t)c * ld] = y;
         }
     }
 }
@@ -835,20 +834,16 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         Ls[r][c] = Wl[(a + r) + (int64_t)(a + c) * ld];
     }
     __syncthreads();
-    const int32_t c = a + nb + blockIdx.x * TB + threadIdx.x;
-    if (c >= P) return;
-    T x[PNB];
-#pragma unroll
-    for (int i = 0; i < PNB; i++) x[i] = i < nb ? Wl[(a + i) + (int64_t)c * ld] : Sc<T>::zero();
-#pragma unroll
-    for (int i = 1; i < PNB; i++) {
-        __asm__ __volatile__("" ::: "memory");   // row i's factor values are read here, not all up front
-#pragma unroll
-        for (int k = 0; k < i; k++) x[i] = Sc<T>::fms(x[i], Ls[i][k], x[k]);
+    // sixteen lanes per column (TB / PNB columns per workgroup), lane i holding x_i and taking x_k from lane k
+    const int i = threadIdx.x % PNB, lane = threadIdx.x & 63;
+    const int32_t c = a + nb + blockIdx.x * (TB / PNB) + threadIdx.x / PNB;
+    const bool mine = c < P && i < nb;
+    T x = mine ? Wl[(a + i) + (int64_t)c * ld] : Sc<T>::zero();
+    for (int kk = 0; kk < nb - 1; kk++) {
+        const T xk = shfl_t(x, (lane & ~(PNB - 1)) | kk);
+        if (i > kk) x = Sc<T>::fms(x, Ls[i][kk], xk);
     }
-#pragma unroll
-    for (int i = 0; i < PNB; i++)
-        if (i < nb) Wl[(a + i) + (int64_t)c * ld] = x[i];
+    if (mine) Wl[(a + i) + (int64_t)c * ld] = x;
 }
 
 // W[a+nb:nr, a+nb:P] -= W[a+nb:nr, a:a+nb] * W[a:a+nb, a+nb:P]
@@ -963,6 +958,10 @@ __global__ __launch_bounds__(TB) void k_lu_nopiv(double* __restrict__ Wd, int64_
         }
         if (min_ratio < INFINITY) atomicMin(&st->min_ratio_bits, (unsigned long long)__double_as_longlong(min_ratio));
     }
+    // the pivots' reciprocals: the rows below multiply by them (one division per pivot, not one per row)
+    __shared__ T rcp[PNB];
+    if (tid < PNB) rcp[tid] = tid < nb ? Sc<T>::div(Sc<T>::one(), D[tid][tid]) : Sc<T>::zero();
+    __syncthreads();
     // the rows below the sub-panel: workgroup g takes rows a + nb + g TB + tid
     const int32_t row = a + nb + g * TB + tid;
     if (row < nr) {
@@ -971,35 +970,35 @@ __global__ __launch_bounds__(TB) void k_lu_nopiv(double* __restrict__ Wd, int64_
         for (int c = 0; c < PNB; c++) x[c] = c < nb ? Wl[row + (int64_t)(a + c) * ld] : Sc<T>::zero();
 #pragma unroll
         for (int jj = 0; jj < PNB; jj++) {
-            if (jj >= nb) break;
-            const T l = Sc<T>::div(x[jj], D[jj][jj]);
-            x[jj] = l;
+            if (jj < nb) {   // (no early exit: x stays in registers, indexed by constants only)
+                const T l = Sc<T>::mul(x[jj], rcp[jj]);
+                x[jj] = l;
 #pragma unroll
-            for (int c = jj + 1; c < PNB; c++)
-                if (c < nb) x[c] = Sc<T>::fms(x[c], l, D[jj][c]);
+                for (int c = jj + 1; c < PNB; c++)
+                    if (c < nb) x[c] = Sc<T>::fms(x[c], l, D[jj][c]);
+            }
         }
 #pragma unroll
         for (int c = 0; c < PNB; c++)
             if (c < nb) Wl[row + (int64_t)(a + c) * ld] = x[c];
     }
-    if (g != 0) return;
-    {
+    if (g == 0) {
         const int r = tid % PNB, c = tid / PNB;
         if (r < nb && c < nb) Wl[(a + r) + (int64_t)(a + c) * ld] = D[r][c];
     }
-    const int32_t c = a + nb + tid;   // one column per thread: the panel is at most TB columns wide
-    if (c < P) {
-        T y[PNB];
-#pragma unroll
-        for (int i = 0; i < PNB; i++) y[i] = i < nb ? Wl[(a + i) + (int64_t)c * ld] : Sc<T>::zero();
-#pragma unroll
-        for (int i = 1; i < PNB; i++) {
-#pragma unroll
-            for (int k = 0; k < i; k++) y[i] = Sc<T>::fms(y[i], D[i][k], y[k]);
+    // the sub-panel's pivot rows in the columns right of it, L11 y = b: sixteen lanes per column, lane i holding y_i and
+    // taking y_k from lane k of its group (a column per thread would keep the whole triangle of L11 in registers); the
+    // passes of TB / PNB columns go round the workgroups, each of which holds the factored pivot block
+    const int i = tid % PNB, lane = tid & 63;
+    for (int32_t c0 = a + nb + g * (TB / PNB); c0 < P; c0 += (int32_t)gridDim.x * (TB / PNB)) {
+        const int32_t c = c0 + tid / PNB;
+        const bool mine = c < P && i < nb;
+        T y = mine ? Wl[(a + i) + (int64_t)c * ld] : Sc<T>::zero();
+        for (int kk = 0; kk < nb - 1; kk++) {
+            const T yk = shfl_t(y, (lane & ~(PNB - 1)) | kk);
+            if (i > kk) y = Sc<T>::fms(y, D[i][kk], yk);
         }
-#pragma unroll
-        for (int i = 0; i < PNB; i++)
-            if (i < nb) Wl[(a + i) + (int64_t)c * ld] = y[i];
+        if (mine) Wl[(a + i) + (int64_t)c * ld] = y;
     }
 }
 
@@ -1187,7 +1186,7 @@ void trsm_sub(int vs, double* Wl, int64_t ld, int32_t P, int32_t a, int32_t nb, 
     const int32_t nc = P - a - nb;
     if (nc <= 0) return;
     ProfScope ps("trsm_sub", 0.0, s);
-    const unsigned g = (unsigned)((nc + TB - 1) / TB);
+    const unsigned g = (unsigned)((nc + TB / PNB - 1) / (TB / PNB));
     if (vs == 1) hipLaunchKernelGGL(k_trsm_sub<double>, dim3(g), dim3(TB), 0, s, Wl, ld, P, a, nb, st);
     else hipLaunchKernelGGL(k_trsm_sub<Zd>, dim3(g), dim3(TB), 0, s, Wl, ld, P, a, nb, st);
 }
