@@ -196,6 +196,15 @@ int fplus_analyze_ord(int64_t n, const int64_t* Ap, const int64_t* Ai, int pin_d
         fpa::errf(err, errlen, "ordering must be FPLUS_ORDER_COLAMD, _AMD_ATA or _AUTO (got %d)", ordering);
         return FPLUS_ERR_BADARG;
     }
+    // The reference's FPLUS_ORDER_AMD_ATA runs SuiteSparse AMD on the pattern of A^T A (reference src/order.c:335-352);
+    // no AMD is in this build, so it is refused rather than silently replaced.  FPLUS_ORDER_AUTO chooses between the two
+    // only for small sparse matrices (n <= 4096, nnz <= 24 n) and keeps the COLAMD-formulation order otherwise; here it
+    // always keeps it.
+    if (ordering == FPLUS_ORDER_AMD_ATA) {
+        fpa::errf(err, errlen, "FPLUS_ORDER_AMD_ATA (AMD on the pattern of A^T A) is not in this build: use FPLUS_ORDER_COLAMD or "
+                               "FPLUS_ORDER_AUTO");
+        return FPLUS_ERR_BADARG;
+    }
     try {
         const int64_t nnz = Ap[n];
         auto* sym = new fplus_symbolic();

@@ -103,6 +103,23 @@ def test_ruiz_converges(mk):
     assert sp.linalg.norm(sp.diags(Dr) @ A @ sp.diags(Dc) - sp.csc_matrix((x, Ai, Ap), shape=(n, n))) < 1e-13 * sp.linalg.norm(B)
 
 
+@pytest.mark.parametrize("mk", [lambda: gen.convdiff2d(16), lambda: gen.shift_invert(2, 4, 32)], ids=["cd2d", "si"])
+def test_ruiz_csr_route_equals_csc(mk):
+    """the reference's CSR route (src/equil.c:186-441, lines = rows) and its CSC route scale the same matrix the same way:
+    the same Dr and Dc bit for bit (maxima do not depend on the order they are taken in) and the same scaled entries"""
+    n, Ap, Ai, Ax = mk()
+    A = sp.csc_matrix((Ax, Ai, Ap), shape=(n, n))
+    R = A.tocsr()
+    R.sort_indices()
+    xc, Drc, Dcc, itc = O.ruiz(n, Ap, Ai, Ax)
+    xr, Drr, Dcr, itr = O.ruiz(n, R.indptr.astype(np.int64), R.indices.astype(np.int64), R.data, is_csc=False)
+    assert itc == itr
+    assert np.array_equal(Drc, Drr) and np.array_equal(Dcc, Dcr)
+    Bc = sp.csc_matrix((xc, Ai, Ap), shape=(n, n))
+    Br = sp.csr_matrix((xr, R.indices, R.indptr), shape=(n, n))
+    assert abs(Bc - Br).max() == 0.0
+
+
 def test_ruiz_zero_line_is_singular():
     n = 3
     p = np.array([0, 1, 2, 2], np.int64)

@@ -75,3 +75,16 @@ def test_lu_residual_closed_form():
     E[5, 2] = 1e-3
     r = splu.lu_residual(sp.csc_matrix(A), sp.csc_matrix(Lm + E), sp.csc_matrix(Um), p, q, Dr, Dc)
     assert np.isclose(r, np.linalg.norm(E @ Um) / np.linalg.norm(M))
+
+
+def test_ordering_modes():
+    """FPLUS_ORDER_AUTO keeps the COLAMD-formulation order (the reference's AMD alternative is not in this build) and
+    FPLUS_ORDER_AMD_ATA is refused with FPLUS_ERR_BADARG, not silently replaced"""
+    from fplus_amd._lib import FPLUS_ERR_BADARG
+    n, Ap, Ai, _ = gen.convdiff2d(12)
+    s0 = fplus_amd.Symbolic(n, Ap, Ai, ordering=fplus_amd.FPLUS_ORDER_COLAMD)
+    s2 = fplus_amd.Symbolic(n, Ap, Ai, ordering=fplus_amd.FPLUS_ORDER_AUTO)
+    assert np.array_equal(s0.col_perm, s2.col_perm) and np.array_equal(s0.row_perm, s2.row_perm)
+    with pytest.raises(fplus_amd.FplusError) as e:
+        fplus_amd.Symbolic(n, Ap, Ai, ordering=fplus_amd.FPLUS_ORDER_AMD_ATA)
+    assert e.value.code == FPLUS_ERR_BADARG and "AMD" in str(e.value)
