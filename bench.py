@@ -50,16 +50,24 @@ WORKLOADS = {
     "shiftinvert-64x4x1024": dict(modes=64, nfields=4, ncoef=1024),
     "shiftinvert-16x4x1024": dict(modes=16, nfields=4, ncoef=1024),
     "shiftinvert-8x4x256": dict(modes=8, nfields=4, ncoef=256),
+    # the precision configuration: 64^3 7-point convection-diffusion, Peclet 10, scales 10^U(-4, 4), real, 8 right-hand
+    # sides, FP32 and BF16 hi/lo updates against FP64, GMRES-IR to 1e-14
+    "convdiff3d-64": dict(g=64),
 }
+# the legs each workload runs (the default: the headline's four)
+WORKLOAD_LEGS = {"convdiff3d-64": ("fp64_gpu", "fp32_factor_ir", "bf16_factor_ir")}
+WORKLOAD_NRHS = {"convdiff3d-64": 8}
+WORKLOAD_TOL = {"convdiff3d-64": 1e-14}
 DEFAULT_WORKLOAD = "shiftinvert-64x4x1024"   # n = 262144, the plain FP64 GPU drop-in configuration
 SIGMA = 0.3 + 0.7j
 SIGMA_PREV = 0.3 + 0.69j                     # the neighbouring shift whose pivot order the refactorizations replay
 CPU_SAMPLE = dict(modes=8, nfields=4, ncoef=1024)   # n = 32768, the CPU baseline's member of the family
 PEAK_FP64_MFMA = 78.6e12      # MI355X dense FP64 matrix, FLOP/s
 PEAK_HBM = 8.0e12             # MI355X HBM3E, B/s
-NRHS = 16
-TOL = 1e-13
+NRHS_DEFAULT = 16
+TOL_DEFAULT = 1e-13
 LEGS = ("fp64_gpu", "fp64_refactor", "fp32_refactor_ir", "learned_refactor_ir")
+ALL_LEGS = LEGS + ("fp32_factor_ir", "bf16_factor_ir")
 SELECTOR = os.path.join(ROOT, "fplus_amd", "data", "selector_shiftinvert.npz")
 # the dominant kernel's HBM bytes per launch, from the rocprofv3 FETCH_SIZE and WRITE_SIZE passes summarized there
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blk_upd-hbm_traffic-shiftinvert-64x4x1024.txt")
@@ -124,7 +132,7 @@ def cpu_baseline(workload_flops, runs=3):
     import oracle as O
     n, Ap, Ai, Ax = gen.shift_invert(**CPU_SAMPLE, sigma=SIGMA)
     sym = fplus_amd.Symbolic(n, Ap, Ai)
-    b = gen.rhs(n, NRHS, True)
+    b = gen.rhs(n, NRHS_DEFAULT, True)
     threads = min(16, os.cpu_count() or 1)
     # the sample's factor flops as the GPU path counts them (the same structure as the workload's count; the blocked
     # CPU path's own count is a little lower: it leaves out rows its updates leave exactly zero)
@@ -155,7 +163,7 @@ def cpu_baseline(workload_flops, runs=3):
     return {"value": med * ratio, "unit": "s/system", "cores": threads, "kind": "port",
             "sample": f"shift_invert({desc}), n={n}, sigma {SIGMA}: permutation + Ruiz + panel-blocked supernodal LU "
                       f"(oracle/cpu_panel.c, panel 256, relax 32, OpenMP on {threads} threads, own blocked loops: no BLAS "
-                      f"installed) + {NRHS} solves, median {med:.2f} s of {runs} runs ({', '.join(f'{t:.2f}' for t in times)}; "
+                      f"installed) + {NRHS_DEFAULT} solves, median {med:.2f} s of {runs} runs ({', '.join(f'{t:.2f}' for t in times)}; "
                       f"factor {info['factor_s']:.2f} s at {info['flops'] / info['factor_s'] / 1e9:.1f} GFLOP/s, componentwise "
                       f"backward error {comp:.1e}), scaled by the factor-flops ratio {workload_flops:.3e}/{sample_flops:.3e} = "
                       f"{ratio:.1f}; CPU {model}, {os.cpu_count()} logical CPUs visible",
@@ -187,7 +195,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default=DEFAULT_WORKLOAD, choices=sorted(WORKLOADS))
-    ap.add_argument("--legs", default=",".join(LEGS))
+    ap.add_argument("--legs", default=None, help="comma-separated legs (default: the workload's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--verbose", action="store_true", help="the factorization's progress lines (for long profiled runs)")
@@ -200,22 +208,30 @@ def main():
     if args.copy_probe:
         copy_probe()
         return
-    legs = [x for x in args.legs.split(",") if x]
+    legs = [x for x in (args.legs or ",".join(WORKLOAD_LEGS.get(args.workload, LEGS))).split(",") if x]
     for x in legs:
-        if x not in LEGS:
+        if x not in ALL_LEGS:
             raise SystemExit(f"unknown leg {x}")
     kw = WORKLOADS[args.workload]
+    NRHS = WORKLOAD_NRHS.get(args.workload, NRHS_DEFAULT)
+    TOL = WORKLOAD_TOL.get(args.workload, TOL_DEFAULT)
     import scipy.sparse as sp
     from fplus_amd import accuracy as acc
-    n, Ap, Ai, Ax = gen.shift_invert(**kw, sigma=SIGMA)
-    _, _, _, Ax_prev = gen.shift_invert(**kw, sigma=SIGMA_PREV)
+    shift = args.workload.startswith("shiftinvert")
+    if shift:
+        n, Ap, Ai, Ax = gen.shift_invert(**kw, sigma=SIGMA)
+        _, _, _, Ax_prev = gen.shift_invert(**kw, sigma=SIGMA_PREV)
+    else:
+        n, Ap, Ai, Ax = gen.convdiff3d(**kw)
+        Ax_prev = Ax
+    cplx = bool(np.iscomplexobj(Ax))
     A = sp.csc_matrix((Ax, Ai, Ap), shape=(n, n))
     log(f"workload {args.workload}: n {n}, nnz {Ap[-1]}")
     sym = fplus_amd.Symbolic(n, Ap, Ai)
     L = _lib.lib()
     opts = fplus_amd.default_options(verbose=2 if args.verbose else 0)
     d_Ax = dev_upload(np.ascontiguousarray(Ax))
-    b = gen.rhs(n, NRHS, True)
+    b = gen.rhs(n, NRHS, cplx)
     bf = np.asfortranarray(b)
     d_b0 = dev_upload(bf)              # the right-hand sides, kept
     d_b = dev_upload(bf)               # overwritten with x by each step
@@ -236,8 +252,8 @@ def main():
     if "fp64_gpu" in legs:
         def step():
             num = C.c_void_p()
-            _lib.check(L.fplus_amd_factor_device(sym.ptr, Ap.ctypes.data_as(P64), Ai.ctypes.data_as(P64), d_Ax, 1, C.byref(opts),
-                                                 C.byref(num), err, 512), err)
+            _lib.check(L.fplus_amd_factor_device(sym.ptr, Ap.ctypes.data_as(P64), Ai.ctypes.data_as(P64), d_Ax, int(cplx),
+                                                 C.byref(opts), C.byref(num), err, 512), err)
             _lib.check(L.fplus_amd_solve_device(num, d_b, NRHS, n, 0, err, 512), err)
             return num
 
@@ -282,10 +298,52 @@ def main():
             if cnt.value:
                 kern[name] = (ms.value / 1e3, cnt.value, work.value)
         out_legs["fp64_gpu"] = {"s_per_system": float(np.median(times)), "steps_s": times, "solve_s": solve_s,
-                                "backward_error": meas, "meets_1e-13": meas["equilibrated"] <= TOL}
+                                "backward_error": meas, "meets_target": meas["equilibrated"] <= TOL}
         log(f"fp64_gpu: {out_legs['fp64_gpu']['s_per_system']:.3f} s/system, {meas}")
 
-    ref_legs = [x for x in legs if x != "fp64_gpu"]
+    # low-precision pivoting factorizations + GMRES-IR (the precision configuration)
+    ro_ir = _lib.RefineOptions()
+    L.fplus_amd_default_refine_options(C.byref(ro_ir))
+    ro_ir.tol = TOL
+    ro_ir.inner_tol = 1e-2
+    for leg, prec in (("fp32_factor_ir", 1), ("bf16_factor_ir", 4)):
+        if leg not in legs:
+            continue
+        aopt = fplus_amd.amd_options(precision=prec)
+
+        def step():
+            num = C.c_void_p()
+            _lib.check(L.fplus_amd_factor_ex(sym.ptr, Ap.ctypes.data_as(P64), Ai.ctypes.data_as(P64), d_Ax, 1, int(cplx),
+                                             C.byref(opts), C.byref(aopt), C.byref(num), err, 512), err)
+            info = _lib.RefineInfo()
+            _lib.check(L.fplus_amd_solve_refined(num, d_b, 1, NRHS, n, C.byref(ro_ir), C.byref(info), err, 512), err)
+            return num, info
+
+        for _ in range(args.warmup):
+            reset_b()
+            L.fplus_numeric_free(step()[0])
+        times, num = [], None
+        for _ in range(args.steps):
+            if num is not None:
+                L.fplus_numeric_free(num)
+            reset_b()
+            t0 = time.perf_counter()
+            num, info = step()
+            L.fplus_amd_dev_sync()
+            times.append(time.perf_counter() - t0)
+        Dr = np.ctypeslib.as_array(L.fplus_numeric_Dr(num), (n,)).copy()
+        Dc = np.ctypeslib.as_array(L.fplus_numeric_Dc(num), (n,)).copy()
+        meas = check_x(Dr, Dc)
+        st = _lib.Stats()
+        L.fplus_amd_numeric_stats(num, C.byref(st))
+        L.fplus_numeric_free(num)
+        out_legs[leg] = {"s_per_system": float(np.median(times)), "steps_s": times, "backward_error": meas,
+                         "meets_target": meas["equilibrated"] <= TOL, "factor_s": st.t_total,
+                         "refine": {"outer": info.outer, "inner": info.inner, "converged": info.converged,
+                                    "bwd_first": info.bwd_first, "bwd_last": info.bwd_last}}
+        log(f"{leg}: {out_legs[leg]['s_per_system']:.3f} s/system, refine {out_legs[leg]['refine']}, {meas}")
+
+    ref_legs = [x for x in legs if x in ("fp64_refactor", "fp32_refactor_ir", "learned_refactor_ir")]
     setup = None
     if ref_legs:
         # the pivot order the refactorizations replay: a factorization of the pencil at the neighbouring shift, made once
@@ -294,10 +352,11 @@ def main():
         aopt = fplus_amd.amd_options(refactor=1, selector=sel)
         num = C.c_void_p()
         t0 = time.perf_counter()
-        _lib.check(L.fplus_amd_factor_ex(sym.ptr, Ap.ctypes.data_as(P64), Ai.ctypes.data_as(P64), d_Ax_prev, 1, 1, C.byref(opts),
-                                         C.byref(aopt), C.byref(num), err, 512), err)
+        _lib.check(L.fplus_amd_factor_ex(sym.ptr, Ap.ctypes.data_as(P64), Ai.ctypes.data_as(P64), d_Ax_prev, 1, int(cplx),
+                                         C.byref(opts), C.byref(aopt), C.byref(num), err, 512), err)
         L.fplus_amd_dev_sync()
-        setup = {"pivoting_factorization_with_record_s": time.perf_counter() - t0, "sigma": str(SIGMA_PREV)}
+        setup = {"pivoting_factorization_with_record_s": time.perf_counter() - t0,
+                 "values": f"sigma {SIGMA_PREV}" if shift else "the workload's own"}
         L.fplus_amd_dev_free(d_Ax_prev)
         st = _lib.Stats()
         L.fplus_amd_numeric_stats(num, C.byref(st))
@@ -332,7 +391,7 @@ def main():
             st = _lib.Stats()
             L.fplus_amd_numeric_stats(num, C.byref(st))
             r = {"s_per_system": float(np.median(times)), "steps_s": times, "backward_error": meas,
-                 "meets_1e-13": meas["equilibrated"] <= TOL,
+                 "meets_target": meas["equilibrated"] <= TOL,
                  "refine": {"outer": info.outer, "inner": info.inner, "converged": info.converged, "bwd_first": info.bwd_first,
                             "bwd_last": info.bwd_last},
                  "fallback_to_fp64": fb, "refactor_s": st.t_total, "pivots_replaced": st.refactor_replaced,
@@ -359,16 +418,18 @@ def main():
     cpu = None
     if not args.no_cpu and st64 is not None and args.workload == DEFAULT_WORKLOAD:
         cpu = cpu_baseline(st64.flops, args.cpu_runs)
-    head = "learned_refactor_ir" if "learned_refactor_ir" in out_legs else (legs[0] if legs else None)
+    head = next((x for x in ("learned_refactor_ir", "bf16_factor_ir") if x in out_legs), legs[0] if legs else None)
     value = out_legs[head]["s_per_system"] if head else None
     base = out_legs.get("fp64_gpu", {}).get("s_per_system")
     out = {
         "metric": "factor+solve s/system at bwd err <=1e-13: mixed/learned vs FP64 GPU vs CPU",
         "value": value, "unit": "s/system", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": value * 1e3 if value else None, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "c128", "data": "synthetic",
-        "config": {"workload": args.workload, "matrix": "complex shift-invert pencil", **kw, "n": n, "nnz": int(Ap[-1]),
-                   "sigma": str(SIGMA), "scales": "10^U(-8,8)", "nrhs": NRHS, "value_leg": head,
+        "dtype": "c128" if cplx else "f64", "data": "synthetic",
+        "config": {"workload": args.workload,
+                   "matrix": "complex shift-invert pencil" if shift else "3D 7-point convection-diffusion, Peclet 10",
+                   **kw, "n": n, "nnz": int(Ap[-1]), "sigma": str(SIGMA) if shift else None,
+                   "scales": "10^U(-8,8)" if shift else "10^U(-4,4)", "nrhs": NRHS, "value_leg": head,
                    "backward_error_target": TOL, "backward_error_measure": "equilibrated normwise, every right-hand side",
                    "ordering": "fplus_analyze defaults", "relax_rows": 32},
         "legs": out_legs,
@@ -380,7 +441,7 @@ def main():
             "flops": st64.flops, "gemm_flops": st64.gemm_flops, "gemm_bytes": st64.gemm_bytes, "nnzL": st64.nnzL,
             "nnzU": st64.nnzU, "supernodes": st64.nsupernodes, "panels": st64.panels, "levels": st64.levels,
             "t_ruiz": st64.t_ruiz, "t_factor": st64.t_factor, "t_host": st64.t_host,
-            "gflops_fp64_gpu": st64.flops / out_legs["fp64_gpu"]["s_per_system"] / 1e9},
+            "gflops_fp64_gpu": st64.flops / out_legs["fp64_gpu"]["s_per_system"] / 1e9} if "fp64_gpu" in out_legs else None,
         "kernels": {k: kernel_rate(k, *v) for k, v in sorted(kern.items(), key=lambda kv: -kv[1][0])},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -115,7 +115,8 @@ void blk_uinv(int vs, int64_t njobs, const int2* jobs, const BlockDesc* blocks, 
 // X: the solved segments' buffer the tiles' xo index; prec_mode 0: every tile's product in FP64, 1: every tile's in
 // FP32, 2: by its block's mark blk_f32[tile.block] (nonzero: FP32), set on the device (e.g. by the precision selector);
 // 3: FP64 applied, and per block the largest difference of the FP32 product and the largest FP64 value max-accumulated
-// into blk_err[2 b], blk_err[2 b + 1] (the selector's training logs)
+// into blk_err[2 b], blk_err[2 b + 1] (the selector's training logs); 4: every tile's as three BF16 products of the
+// operands' hi/lo split with FP32 accumulation
 void blk_upd(int vs, int32_t ntile, const UpdTile* tiles, const int32_t* cols, const int32_t* rowmap, double* W, int64_t ldw,
              int tile_rows, double flops, double* X, int prec_mode, const uint8_t* blk_f32, unsigned long long* blk_err,
              hipStream_t s);

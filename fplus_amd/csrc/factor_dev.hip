@@ -201,9 +201,12 @@ struct Zf {
     float x, y;
 };
 
-// the arithmetic of a product tile: FP64 operands and accumulation, or FP32 (the operands rounded to FP32 when they are
-// staged in LDS, FP32 accumulation, the result widened to FP64 for the update of W)
-template <class T, bool F32> struct Ar {
+// the arithmetic of a product tile (PR): 0 FP64 operands and accumulation; 1 FP32 (the operands rounded to FP32 when
+// they are staged in LDS, FP32 accumulation, the result widened to FP64 for the update of W); 2 BF16 hi/lo split products
+// (staged in FP32 like 1; each FP32 operand x split into hi = bf16(x) and lo = bf16(x - hi), the product taken as
+// Ahi Bhi + Ahi Blo + Alo Bhi on the BF16 matrix cores with FP32 accumulation: about 16 bits of each operand)
+constexpr int PR64 = 0, PR32 = 1, PRBF = 2;
+template <class T, int PR> struct Ar {
     using S = T;
     using acc = dx4;
     __device__ static S cvt(T v) { return v; }
@@ -211,20 +214,49 @@ template <class T, bool F32> struct Ar {
     // the column of its 16 x 16 part that lane `lane`'s accumulator element v holds (its row is lane % 16)
     __device__ static int col(int lane, int v) { return (lane >> 4) + 4 * v; }
 };
-template <> struct Ar<double, true> {
+template <> struct Ar<double, PR32> {
     using S = float;
     using acc = fx4;
     __device__ static S cvt(double v) { return (float)v; }
     __device__ static S zero() { return 0.0f; }
     __device__ static int col(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
-template <> struct Ar<Zd, true> {
+template <> struct Ar<Zd, PR32> {
     using S = Zf;
     using acc = fx4;
     __device__ static S cvt(Zd v) { Zf r; r.x = (float)v.x; r.y = (float)v.y; return r; }
     __device__ static S zero() { Zf r; r.x = 0.0f; r.y = 0.0f; return r; }
     __device__ static int col(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
+
+template <> struct Ar<double, PRBF> : Ar<double, PR32> {};
+template <> struct Ar<Zd, PRBF> : Ar<Zd, PR32> {};
+
+// BF16 round to nearest even (finite inputs), and the three-product split multiply on 16 x 16 x 16 BF16 matrix cores:
+// lane l supplies A[l % 16][4 (l / 16) + t] and B[4 (l / 16) + t][l % 16], t < 4; the result lands as in the FP32 form
+typedef short s4v __attribute__((ext_vector_type(4)));
+__device__ inline short f2bf(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (short)(u >> 16);
+}
+__device__ inline float bf2f(short h) { return __uint_as_float(((uint32_t)(uint16_t)h) << 16); }
+__device__ inline fx4 mfma_bf3(const float (&a)[4], const float (&b)[4], fx4 c)
+{
+    s4v ah, al, bh, bl;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        ah[t] = f2bf(a[t]);
+        al[t] = f2bf(a[t] - bf2f(ah[t]));
+        bh[t] = f2bf(b[t]);
+        bl[t] = f2bf(b[t] - bf2f(bh[t]));
+    }
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl, c, 0, 0, 0);
+    c = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh, c, 0, 0, 0);
+    return c;
+}
 
 constexpr int TN = 64;           // columns per product tile
 constexpr int BLD = TN + 1;      // LDS row length of B: one element of padding moves each k row to other banks
@@ -245,19 +277,19 @@ template <class T, int TM> struct TileCfg {
 // k in chunks of 8 staged in LDS (two buffers), with the global loads two chunks ahead of the products where registers
 // allow (TileCfg::DEEP): chunk i + 2 is read into registers while chunk i multiplies and chunk i + 1 waits in the other
 // register set, so a load has two chunks' products to land; else one chunk ahead.  la(r, k) and lb(k, c) give the
-// operands (zero outside the tile) in FP64; F32 rounds them to FP32 when they are staged.  The products are formed
+// operands (zero outside the tile) in FP64; PR 1 and 2 round them to FP32 when they are staged.  The products are formed
 // transposed (B^T A^T), so each lane ends with rows of C along lanes: out(r, c, v) is called for every element, and a
 // column's 16 consecutive rows come from 16 lanes -- the write-out needs no pass through LDS.  pre() runs once the
 // first chunk is staged, before the barrier that publishes it: LDS state the output needs is stored there, from loads
 // that were in flight with the first chunks'.  stg: LDS of TileCfg<T, TM>::STG scalars T.
 template <int N> struct IC { static constexpr int value = N; };
 
-template <class T, int TM, bool F32, class LA, class LB, class OUT, class PRE>
+template <class T, int TM, int PR, class LA, class LB, class OUT, class PRE>
 __device__ inline void tile_mfma(int32_t kb, int32_t ke, int32_t mrows, int32_t ncols, const LA& la, const LB& lb, const OUT& out,
                                  const PRE& pre, T* stg)
 {
     using Cf = TileCfg<T, TM>;
-    using AR = Ar<T, F32>;
+    using AR = Ar<T, PR>;
     using S = typename AR::S;
     using ACC = typename AR::acc;
     constexpr bool CPLX = Sc<T>::vs == 2;
@@ -312,6 +344,57 @@ __device__ inline void tile_mfma(int32_t kb, int32_t ke, int32_t mrows, int32_t 
     };
     auto mma_n = [&](auto ni_c, auto nj_c, int buf) {
         constexpr int NI = decltype(ni_c)::value, NJ = decltype(nj_c)::value;
+        if constexpr (PR == PRBF) {
+            // the chunk's 8 k in one 16-deep BF16 product: lane groups 0 and 1 supply k 0-3 and 4-7, groups 2 and 3 zeros
+            static_assert(KC == 8, "the BF16 form takes chunks of 8");
+            const int kq = lane >> 4, kb4 = 4 * (kq & 1);
+            const float on4 = kq < 2 ? 1.0f : 0.0f;
+            S a[2][4], b[Cf::SJ][4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+#pragma unroll
+                for (int i = 0; i < NI; i++) a[i][t] = As[(buf * KC + kb4 + t) * TM + wr + i * 16 + (lane & 15)];
+#pragma unroll
+                for (int j = 0; j < NJ; j++) b[j][t] = Bs[(buf * KC + kb4 + t) * BLD + wc + j * 16 + (lane & 15)];
+            }
+#pragma unroll
+            for (int i = 0; i < NI; i++)
+#pragma unroll
+                for (int j = 0; j < NJ; j++) {
+                    float p[4], q[4];
+                    if constexpr (Cf::G3) {
+#pragma unroll
+                        for (int t = 0; t < 4; t++) { p[t] = on4 * b[j][t].x; q[t] = on4 * (a[i][t].x + a[i][t].y); }
+                        accr[i][j] = mfma_bf3(p, q, accr[i][j]);
+#pragma unroll
+                        for (int t = 0; t < 4; t++) { p[t] = on4 * (b[j][t].y - b[j][t].x); q[t] = on4 * a[i][t].x; }
+                        acci[i][j] = mfma_bf3(p, q, acci[i][j]);
+#pragma unroll
+                        for (int t = 0; t < 4; t++) { p[t] = on4 * (b[j][t].x + b[j][t].y); q[t] = on4 * a[i][t].y; }
+                        acc3[i][j] = mfma_bf3(p, q, acc3[i][j]);
+                    } else if constexpr (CPLX) {
+                        float py[4], pn[4];
+#pragma unroll
+                        for (int t = 0; t < 4; t++) {
+                            p[t] = on4 * b[j][t].x;
+                            py[t] = on4 * b[j][t].y;
+                            pn[t] = -py[t];
+                            q[t] = on4 * a[i][t].x;
+                        }
+                        accr[i][j] = mfma_bf3(p, q, accr[i][j]);    // re += b.x a.x
+                        acci[i][j] = mfma_bf3(py, q, acci[i][j]);   // im += b.y a.x
+#pragma unroll
+                        for (int t = 0; t < 4; t++) q[t] = on4 * a[i][t].y;
+                        accr[i][j] = mfma_bf3(pn, q, accr[i][j]);   // re -= b.y a.y
+                        acci[i][j] = mfma_bf3(p, q, acci[i][j]);    // im += b.x a.y
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < 4; t++) { p[t] = on4 * b[j][t]; q[t] = on4 * a[i][t]; }
+                        accr[i][j] = mfma_bf3(p, q, accr[i][j]);
+                    }
+                }
+            return;
+        }
 #pragma unroll
         for (int ks = 0; ks < KC; ks += 4) {
             const int kk = ks + (lane >> 4);
@@ -423,7 +506,7 @@ __global__ __launch_bounds__(TB) void k_blk_ivgemm(const GemmTile* __restrict__ 
     auto out = [&](int r, int c, T v) {
         if (r < nr && c < nc) gstore(out_base, r + (int64_t)c * B.nrows, v);
     };
-    tile_mfma<T, TM, false>(tl.col0, B.w, nr, nc, la, lb, out, [] {}, stg);
+    tile_mfma<T, TM, PR64>(tl.col0, B.w, nr, nc, la, lb, out, [] {}, stg);
 }
 
 // one tile of a task's update, rows [row0, row0 + TM) of the block from the segment start k0 on: the product
@@ -432,10 +515,10 @@ __global__ __launch_bounds__(TB) void k_blk_ivgemm(const GemmTile* __restrict__ 
 // rows below are subtracted from W through rowmap, the relative-index scatter -- with atomics where another task of
 // the launch may write the same rows (tl.shared), else with plain loads and stores.
 
-template <class T, int TM, bool F32, bool LOG = false>
+template <class T, int TM, int PR, bool LOG = false>
 __device__ inline void upd_tile(const UpdTile& tl, const int32_t* __restrict__ cols, const int32_t* __restrict__ rowmap,
                                 double* __restrict__ W, int64_t ldw, double* __restrict__ Xb, T* stg, int32_t* rpos, int32_t* cidx,
-                                typename Ar<T, true>::S* p32 = nullptr, unsigned long long* blk_err = nullptr)
+                                typename Ar<T, PR32>::S* p32 = nullptr, unsigned long long* blk_err = nullptr)
 {
     constexpr bool CPLX = Sc<T>::vs == 2;
     const int32_t m = tl.w - tl.k0, nr = min(TM, tl.nrows - tl.row0), nc = tl.nc;
@@ -485,16 +568,16 @@ __device__ inline void upd_tile(const UpdTile& tl, const int32_t* __restrict__ c
     // needs k only up to its last row
     const int32_t ke = min(tl.w, tl.row0 + TM);
     if constexpr (!LOG) {
-        tile_mfma<T, TM, F32>(tl.k0, ke, nr, nc, la, lb, out, [&] { if (tid < TM) rpos[tid] = rp; }, stg);
+        tile_mfma<T, TM, PR>(tl.k0, ke, nr, nc, la, lb, out, [&] { if (tid < TM) rpos[tid] = rp; }, stg);
     } else {
         // the logging form, for training the precision selector: the FP32 product first (kept in LDS, not applied),
         // then the FP64 one (applied); the largest difference and the largest FP64 value go to the block's entries of
         // blk_err (non-negative doubles as bits, max-accumulated)
-        using AR = Ar<T, true>;
+        using AR = Ar<T, PR32>;
         auto out32 = [&](int r, int c, T v) {
             if (r < nr && c < nc) p32[r * TN + c] = AR::cvt(v);
         };
-        tile_mfma<T, TM, true>(tl.k0, ke, nr, nc, la, lb, out32, [&] { if (tid < TM) rpos[tid] = rp; }, stg);
+        tile_mfma<T, TM, PR32>(tl.k0, ke, nr, nc, la, lb, out32, [&] { if (tid < TM) rpos[tid] = rp; }, stg);
         __syncthreads();
         double dmax = 0.0, vmax = 0.0;
         auto out64 = [&](int r, int c, T v) {
@@ -507,7 +590,7 @@ __device__ inline void upd_tile(const UpdTile& tl, const int32_t* __restrict__ c
             vmax = fmax(vmax, Sc<T>::absval(v));
             out(r, c, v);
         };
-        tile_mfma<T, TM, false>(tl.k0, ke, nr, nc, la, lb, out64, [] {}, stg);
+        tile_mfma<T, TM, PR64>(tl.k0, ke, nr, nc, la, lb, out64, [] {}, stg);
         for (int off = 32; off > 0; off >>= 1) {
             dmax = fmax(dmax, __shfl_xor(dmax, off));
             vmax = fmax(vmax, __shfl_xor(vmax, off));
@@ -521,7 +604,8 @@ __device__ inline void upd_tile(const UpdTile& tl, const int32_t* __restrict__ c
 
 // one workgroup per tile of a dependency level (every task of the level in one launch).  MODE 0: FP64 products; 1: FP32
 // products; 2: each tile takes the FP32 or the FP64 product by its block's mark (uniform over the workgroup); 3: FP64
-// products applied, the FP32 ones computed beside them and their differences logged per block
+// products applied, the FP32 ones computed beside them and their differences logged per block; 4: BF16 hi/lo split
+// products
 template <class T, int TM, int MODE>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(4))) void k_blk_upd(
     const UpdTile* __restrict__ tiles, const int32_t* __restrict__ cols, const int32_t* __restrict__ rowmap,
@@ -533,15 +617,17 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(4))) void k_
     __shared__ int32_t cidx[TN];
     const UpdTile tl = tiles[blockIdx.x];
     if constexpr (MODE == 3) {
-        __shared__ typename Ar<T, true>::S p32[TM * TN];
-        upd_tile<T, TM, false, true>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx, p32, blk_err);
+        __shared__ typename Ar<T, PR32>::S p32[TM * TN];
+        upd_tile<T, TM, PR64, true>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx, p32, blk_err);
     } else if constexpr (MODE == 1) {
-        upd_tile<T, TM, true>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx);
+        upd_tile<T, TM, PR32>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx);
     } else if constexpr (MODE == 2) {
-        if (blk_f32[tl.block]) upd_tile<T, TM, true>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx);
-        else upd_tile<T, TM, false>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx);
+        if (blk_f32[tl.block]) upd_tile<T, TM, PR32>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx);
+        else upd_tile<T, TM, PR64>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx);
+    } else if constexpr (MODE == 4) {
+        upd_tile<T, TM, PRBF>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx);
     } else {
-        upd_tile<T, TM, false>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx);
+        upd_tile<T, TM, PR64>(tl, cols, rowmap, W, ldw, X, stg, rpos, cidx);
     }
 }
 
@@ -1094,6 +1180,7 @@ static void blk_upd_t(int mode, int32_t ntile, const UpdTile* tiles, const int32
     if (mode == 1) hipLaunchKernelGGL((k_blk_upd<T, TM, 1>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw, X, bf, be);
     else if (mode == 2) hipLaunchKernelGGL((k_blk_upd<T, TM, 2>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw, X, bf, be);
     else if (mode == 3) hipLaunchKernelGGL((k_blk_upd<T, TM, 3>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw, X, bf, be);
+    else if (mode == 4) hipLaunchKernelGGL((k_blk_upd<T, TM, 4>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw, X, bf, be);
     else hipLaunchKernelGGL((k_blk_upd<T, TM, 0>), dim3(ntile), dim3(TB), 0, s, tiles, cols, rowmap, W, ldw, X, bf, be);
 }
 void blk_upd(int vs, int32_t ntile, const UpdTile* tiles, const int32_t* cols, const int32_t* rowmap, double* W, int64_t ldw,
@@ -1103,8 +1190,9 @@ void blk_upd(int vs, int32_t ntile, const UpdTile* tiles, const int32_t* cols, c
     if (ntile <= 0) return;
     if (prec_mode == 2 && !blk_f32) throw std::invalid_argument("blk_upd: per-block precision without the block marks");
     if (prec_mode == 3 && !blk_err) throw std::invalid_argument("blk_upd: logging without the per-block error array");
-    static const char* names[4] = {"blk_upd", "blk_upd_f32", "blk_upd_mix", "blk_upd_log"};
-    ProfScope ps(names[prec_mode & 3], flops, s);
+    if (prec_mode < 0 || prec_mode > 4) throw std::invalid_argument("blk_upd: precision mode 0 to 4");
+    static const char* names[5] = {"blk_upd", "blk_upd_f32", "blk_upd_mix", "blk_upd_log", "blk_upd_bf16"};
+    ProfScope ps(names[prec_mode], flops, s);
     if (tile_rows == 32) {
         if (vs == 1) blk_upd_t<double, 32>(prec_mode, ntile, tiles, cols, rowmap, W, ldw, X, blk_f32, blk_err, s);
         else blk_upd_t<Zd, 32>(prec_mode, ntile, tiles, cols, rowmap, W, ldw, X, blk_f32, blk_err, s);

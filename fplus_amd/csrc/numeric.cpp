@@ -859,8 +859,9 @@ void fplus_numeric_free(fplus_numeric* num)
 
 static int check_amd_options(const fplus_amd_options* ao, char* err, size_t errlen)
 {
-    if (ao->precision < 0 || ao->precision > 3) {
-        errf(err, errlen, "precision must be 0 (FP64), 1 (FP32), 2 (per block, by the selector) or 3 (FP64, FP32 differences logged)");
+    if (ao->precision < 0 || ao->precision > 4) {
+        errf(err, errlen, "precision must be 0 (FP64), 1 (FP32), 2 (per block, by the selector), 3 (FP64, FP32 differences logged) "
+                          "or 4 (BF16 hi/lo split)");
         return FPLUS_ERR_BADARG;
     }
     if (ao->precision == 2 && !ao->selector) { errf(err, errlen, "precision 2 needs a selector (fplus_amd_options.selector)"); return FPLUS_ERR_BADARG; }
@@ -1098,8 +1099,8 @@ extern "C" int fplus_amd_refactor(fplus_numeric* num, const void* Ax, int ax_on_
         errf(err, errlen, "this factorization was made without fplus_amd_options.refactor = 1: nothing to replay");
         return FPLUS_ERR_BADARG;
     }
-    if (precision < 0 || precision > 3 || static_pivots < 0 || static_pivots > 1) {
-        errf(err, errlen, "precision must be 0, 1, 2 or 3 and static_pivots 0 or 1");
+    if (precision < 0 || precision > 4 || static_pivots < 0 || static_pivots > 1) {
+        errf(err, errlen, "precision must be 0, 1, 2, 3 or 4 and static_pivots 0 or 1");
         return FPLUS_ERR_BADARG;
     }
     if (precision == 2 && !num->aopt.selector) {
@@ -1164,7 +1165,7 @@ extern "C" int fplus_amd_refactor(fplus_numeric* num, const void* Ax, int ax_on_
         if (num->opt.verbose >= 1)
             printf("[FPLUS] refactorization in the recorded pivot order: %.3fs (Ruiz %.3fs), %s updates, %ld pivots replaced, "
                    "smallest |pivot| / colmax %.3e\n", num->t_total, num->t_ruiz,
-                   precision == 0 ? "FP64" : (precision == 1 ? "FP32" : (precision == 2 ? "per-block (selector)" : "FP64 (logged)")),
+                   precision == 0 ? "FP64" : (precision == 1 ? "FP32" : (precision == 2 ? "per-block (selector)" : (precision == 3 ? "FP64 (logged)" : "BF16 hi/lo"))),
                    (long)replaced, minr);
     } catch (const GpuError& e) {
         return gpu_error(e, err, errlen);

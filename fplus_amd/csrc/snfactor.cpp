@@ -1111,7 +1111,7 @@ int Driver::panel_out()
         up_.upload(d_itiles_.p, itiles.data(), itiles.size(), s_);
         k::blk_inverse(vs_, (int32_t)ijobs.size(), d_ijobs_.p, (int32_t)itiles.size(), d_itiles_.p, d_blocks_p(), iflops, s_);
     }
-    if (opt_.precision >= 2 && !new_blocks_.empty()) {
+    if ((opt_.precision == 2 || opt_.precision == 3) && !new_blocks_.empty()) {
         // the new blocks' features and (precision 2) the selector's marks, on the device: later panels' update tiles read
         // the marks, so the decisions need no trip to the host
         d_newblk_.reserve(new_blocks_.size());
@@ -1216,7 +1216,7 @@ int Driver::run()
     F_.d_blk_f32.alloc(nn);   // (a block has at least one column)
     k::fill_u8(F_.d_blk_f32.p, (int64_t)nn, opt_.precision == 1 ? 1 : 0, s_);
     F_.precision = opt_.precision;
-    if (opt_.precision >= 2) {
+    if ((opt_.precision == 2 || opt_.precision == 3)) {
         if (opt_.precision == 2 && !opt_.sel.d_model) throw std::invalid_argument("per-block precision without a selector model");
         if (!opt_.sel.d_Dr || !opt_.sel.d_Dc) throw std::invalid_argument("the selector's features need the scalings");
         F_.d_feat.alloc(nn * SEL_NF);
@@ -1381,8 +1381,8 @@ int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, c
     st0.min_ratio_bits = 0x7FF0000000000000ULL;   // +inf
     R.d_st.upload(&st0, 1, s);
     if (precision == 1) k::fill_u8(F.d_blk_f32.p, (int64_t)F.blocks.size(), 1, s);
-    else if (precision == 0 || precision == 3) k::fill_u8(F.d_blk_f32.p, (int64_t)F.blocks.size(), 0, s);
-    if (precision >= 2) {
+    else k::fill_u8(F.d_blk_f32.p, (int64_t)F.blocks.size(), 0, s);
+    if ((precision == 2 || precision == 3)) {
         if (precision == 2 && !sel.d_model) { errf(err, errlen, "per-block precision needs a selector"); return FPLUS_ERR_BADARG; }
         const size_t nn = (size_t)std::max<int64_t>(n, 1);
         if (F.d_feat.n < nn * SEL_NF) F.d_feat.alloc(nn * SEL_NF);
@@ -1419,7 +1419,7 @@ int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, c
         if (P.nex) k::extract_cols(vs, (int32_t)P.nex, R.d_ex.p + P.ex_off, R.d_expos.p + P.expos_off, rows + P.Ru, Wl, ldw, s);
         if (P.nij)
             k::blk_inverse(vs, (int32_t)P.nij, R.d_ij.p + P.ij_off, (int32_t)P.nit, R.d_it.p + P.it_off, blocks, P.iflops, s);
-        if (precision >= 2 && P.nnb) {
+        if ((precision == 2 || precision == 3) && P.nnb) {
             k::block_features(vs, (int32_t)P.nnb, R.d_newblk.p + P.nb_off, blocks, R.d_colmax.p, P.j0, sel.d_Dr, sel.d_Dc, n,
                               F.d_featmax.p, F.d_feat.p, s);
             if (precision == 2)

@@ -64,7 +64,9 @@ int fplus_amd_solve_device_noalloc(const fplus_numeric *num, void *d_b, int64_t 
  *                  2 per block (a supernode's columns within a panel), by the learned precision selector: as each block
  *                  is made its features go through the selector's model on the device, which marks whether the block's
  *                  updates of later columns may run in FP32; 3 FP64 updates with the FP32 products computed beside them
- *                  and their differences logged per block (fplus_amd_selector_log: the selector's training data).  Pivot
+ *                  and their differences logged per block (fplus_amd_selector_log: the selector's training data); 4 BF16
+ *                  hi/lo split products (each FP32-rounded operand split into two BF16 parts, three BF16 matrix-core
+ *                  products with FP32 accumulation: about 16 bits per operand).  Pivot
  *                  selection, the panels' dense LU, Ruiz and the solves stay FP64.  A factorization with FP32 updates
  *                  is meant for fplus_amd_solve_refined, which wins the accuracy back.
  *   refactor       1: keep what fplus_amd_refactor replays -- the device schedule of this factorization and the blocks'

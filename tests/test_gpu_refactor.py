@@ -8,8 +8,9 @@
   matrix exactly: ||Dr A[p, q] Dc - L U|| / ||Dr A[p, q] Dc|| at round-off (lu_residual), and the refined solve reaches
   an equilibrated backward error below 1e-14.
 - FP32 updates (pivots, panel LU, Ruiz and solves in FP64): L and U differ from the FP64 factors by FP32 round-off
-  (1e-9 to 1e-3 normwise: not zero, so the FP32 products ran, and not more), and GMRES-IR brings every right-hand side
+  (1e-10 to 1e-3 normwise: not zero, so the FP32 products ran, and not more), and GMRES-IR brings every right-hand side
   to an equilibrated backward error below 1e-14 (and a componentwise one below 1e-6, the direct FP64 solves' bound).
+  The same for the BF16 hi/lo split products, whose factors lie 1e-8 to 1e-1 from the FP64 ones.
 - Static pivoting: a refactorization whose recorded pivot is exactly zero for the new values fails with
   FPLUS_ERR_SINGULAR, or, with static_pivots = 1, replaces it, counts it, and the refined solve still converges.
 """
@@ -81,8 +82,14 @@ def test_refactor_new_shift_and_refined_solve():
     assert info["converged"] == 1 and m["equilibrated"] < 1e-14, (info, m)
 
 
+# the low-precision update forms: FP32 (precision 1) and the BF16 hi/lo split products (precision 4, about 16 bits per
+# operand); the factors' distance from the FP64 ones lies in these bands
+LOWP = {1: (1e-10, 1e-3), 4: (1e-8, 1e-1)}
+
+
+@pytest.mark.parametrize("prec", [1, 4], ids=["fp32", "bf16x3"])
 @pytest.mark.parametrize("case", ["shiftinvert-8x4x64", "convdiff3d-16"])
-def test_fp32_updates_with_gmres_ir(case):
+def test_fp32_updates_with_gmres_ir(case, prec):
     if case.startswith("shift"):
         n, Ap, Ai, Ax = gen.shift_invert(8, 4, 64)
     else:
@@ -91,15 +98,16 @@ def test_fp32_updates_with_gmres_ir(case):
     A = sp.csc_matrix((Ax, Ai, Ap), shape=(n, n))
     sym = fplus_amd.Symbolic(n, Ap, Ai)
     num64 = fplus_amd.Numeric(sym, Ap, Ai, Ax)
-    num32 = fplus_amd.Numeric(sym, Ap, Ai, Ax, amd_options=fplus_amd.amd_options(precision=1))
-    assert fplus_amd.stats(num32)["precision"] == 1
+    num32 = fplus_amd.Numeric(sym, Ap, Ai, Ax, amd_options=fplus_amd.amd_options(precision=prec))
+    assert fplus_amd.stats(num32)["precision"] == prec
     L64, U64, _ = _lu(num64)
     L32, U32, _ = _lu(num32)
     same = np.array_equal(num32.p, num64.p)
     if same:
         dl, du = norm_diff(L32, L64), norm_diff(U32, U64)
-        print(f"{case}: FP32 updates vs FP64: L {dl:.2e}, U {du:.2e}")
-        assert 1e-10 < max(dl, du) < 1e-3, (dl, du)
+        print(f"{case}: precision {prec} updates vs FP64: L {dl:.2e}, U {du:.2e}")
+        lo, hi = LOWP[prec]
+        assert lo < max(dl, du) < hi, (dl, du)
     b = gen.rhs(n, 8, cplx)
     x0 = num32.solve(b)
     m0 = acc.all_measures(A, x0, b, num32.Dr, num32.Dc)
