@@ -53,10 +53,17 @@ WORKLOADS = {
     # the precision configuration: 64^3 7-point convection-diffusion, Peclet 10, scales 10^U(-4, 4), real, 8 right-hand
     # sides, FP32 and BF16 hi/lo updates against FP64, GMRES-IR to 1e-14
     "convdiff3d-64": dict(g=64),
+    # the learned selector's configuration: the shift-invert family with 17 orders of scaling, 10^U(-8, 9), n = 524288,
+    # 64 right-hand sides
+    "shiftinvert-128x4x1024-s17": dict(modes=128, nfields=4, ncoef=1024, scale_exp=8.0, scale_hi=9.0),
+    # the batching configuration's sweep: one n = 131072 pattern refactored at the 32 shifts sigma_k = 0.3 + i k / 32
+    # (k = 1 .. 32), 4 right-hand sides each, in the pivot order of one pivoting factorization at sigma = 0.3 + 0.5i
+    "sweep-32x4x1024": dict(modes=32, nfields=4, ncoef=1024),
 }
 # the legs each workload runs (the default: the headline's four)
-WORKLOAD_LEGS = {"convdiff3d-64": ("fp64_gpu", "fp32_factor_ir", "bf16_factor_ir")}
-WORKLOAD_NRHS = {"convdiff3d-64": 8}
+WORKLOAD_LEGS = {"convdiff3d-64": ("fp64_gpu", "fp32_factor_ir", "bf16_factor_ir"),
+                 "sweep-32x4x1024": ("sweep", "sweep_graph")}
+WORKLOAD_NRHS = {"convdiff3d-64": 8, "shiftinvert-128x4x1024-s17": 64, "sweep-32x4x1024": 4}
 WORKLOAD_TOL = {"convdiff3d-64": 1e-14}
 DEFAULT_WORKLOAD = "shiftinvert-64x4x1024"   # n = 262144, the plain FP64 GPU drop-in configuration
 SIGMA = 0.3 + 0.7j
@@ -67,7 +74,7 @@ PEAK_HBM = 8.0e12             # MI355X HBM3E, B/s
 NRHS_DEFAULT = 16
 TOL_DEFAULT = 1e-13
 LEGS = ("fp64_gpu", "fp64_refactor", "fp32_refactor_ir", "learned_refactor_ir")
-ALL_LEGS = LEGS + ("fp32_factor_ir", "bf16_factor_ir")
+ALL_LEGS = LEGS + ("fp32_factor_ir", "bf16_factor_ir", "sweep", "sweep_graph")
 SELECTOR = os.path.join(ROOT, "fplus_amd", "data", "selector_shiftinvert.npz")
 # the dominant kernel's HBM bytes per launch, from the rocprofv3 FETCH_SIZE and WRITE_SIZE passes summarized there
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blk_upd-hbm_traffic-shiftinvert-64x4x1024.txt")
@@ -189,6 +196,95 @@ def kernel_rate(name, sec, launches, work):
     return r
 
 
+def sweep_main(args, kw, legs, nrhs):
+    """the shift sweep: one pivoting factorization (untimed set-up), then the 32 shifts refactored and solved in one
+    fplus_amd_refactor_sweep call from device-resident values and right-hand sides, launched kernel by kernel ("sweep")
+    and through the captured graph ("sweep_graph"); seconds per shift, and per system with the set-up shared out"""
+    import scipy.sparse as sp
+    from fplus_amd import accuracy as acc
+    shifts = [0.3 + 1j * k / 32 for k in range(1, 33)]
+    n, Ap, Ai, Ax0 = gen.shift_invert(**kw, sigma=0.3 + 0.5j)
+    # the pencil's values are L - sigma M: the shifts' values from L and M (one generation at sigma 0 and one at 1)
+    _, _, _, AxL = gen.shift_invert(**kw, sigma=0.0)
+    _, _, _, AxL1 = gen.shift_invert(**kw, sigma=1.0)
+    AxM = AxL - AxL1
+    axs = np.ascontiguousarray(np.stack([AxL - sg * AxM for sg in shifts]))
+    nnz = int(Ap[-1])
+    b = np.stack([gen.rhs(n, nrhs, True, seed=100 + k).T for k in range(len(shifts))])   # shift, column, row
+    b = np.ascontiguousarray(b)
+    log(f"sweep: n {n}, nnz {nnz}, {len(shifts)} shifts, {nrhs} right-hand sides each")
+    L = _lib.lib()
+    sym = fplus_amd.Symbolic(n, Ap, Ai)
+    opts = fplus_amd.default_options()
+    err = C.create_string_buffer(512)
+    P64 = _lib.I64P
+    d_ax0 = dev_upload(np.ascontiguousarray(Ax0))
+    d_axs = dev_upload(axs)
+    d_b0 = dev_upload(b)
+    d_b = dev_upload(b)
+    aopt = fplus_amd.amd_options(refactor=1)
+    num = C.c_void_p()
+    t0 = time.perf_counter()
+    _lib.check(L.fplus_amd_factor_ex(sym.ptr, Ap.ctypes.data_as(P64), Ai.ctypes.data_as(P64), d_ax0, 1, 1, C.byref(opts),
+                                     C.byref(aopt), C.byref(num), err, 512), err)
+    L.fplus_amd_dev_sync()
+    setup_s = time.perf_counter() - t0
+    ro = _lib.RefineOptions()
+    L.fplus_amd_default_refine_options(C.byref(ro))
+    ro.tol = 1e-13
+    ro.inner_tol = 1e-2
+    ns = len(shifts)
+    out = {}
+    for leg in legs:
+        fplus_amd.tune("refactor_graph", 1 if leg == "sweep_graph" else 0)
+        info = (_lib.RefineInfo * ns)()
+        fb = (C.c_int * ns)()
+        times = []
+        CH = 8   # shifts per call: a progress line between calls
+        vb = nrhs * n * 16        # bytes of one shift's right-hand sides
+        va = nnz * 16             # bytes of one shift's values
+        for rep in range(args.warmup + args.steps):
+            dev_copy(d_b, d_b0, b.nbytes)
+            L.fplus_amd_dev_sync()
+            t1 = time.perf_counter()
+            for k0 in range(0, ns, CH):
+                kc = min(CH, ns - k0)
+                _lib.check(L.fplus_amd_refactor_sweep(num, C.c_void_p(d_axs.value + k0 * va), kc, C.c_void_p(d_b.value + k0 * vb),
+                                                      nrhs, n, 1, 0, C.byref(ro),
+                                                      C.c_void_p(C.addressof(info) + k0 * C.sizeof(_lib.RefineInfo)),
+                                                      C.cast(C.addressof(fb) + k0 * C.sizeof(C.c_int), C.POINTER(C.c_int)),
+                                                      err, 512), err)
+                L.fplus_amd_dev_sync()
+                log(f"{leg} run {rep + 1}: shifts {k0 + 1}-{k0 + kc} done, {time.perf_counter() - t1:.1f} s")
+            if rep >= args.warmup:
+                times.append(time.perf_counter() - t1)
+        x = np.empty_like(b)
+        dev_download(d_b, x)
+        Dr = np.ctypeslib.as_array(L.fplus_numeric_Dr(num), (n,)).copy()
+        Dc = np.ctypeslib.as_array(L.fplus_numeric_Dc(num), (n,)).copy()
+        A = sp.csc_matrix((axs[-1], Ai, Ap), shape=(n, n))
+        meas = acc.all_measures(A, x[-1].T, b[-1].T, Dr, Dc)   # the last shift (Dr, Dc are the last refactorization's)
+        sec = float(np.median(times))
+        out[leg] = {"sweep_s": sec, "s_per_shift": sec / ns, "s_per_system_with_setup": (sec + setup_s) / ns,
+                    "steps_s": times, "outer_max": max(i.outer for i in info), "converged": all(i.converged for i in info),
+                    "fallbacks": int(sum(fb)), "last_shift_backward_error": meas}
+        log(f"{leg}: {sec:.3f} s for {ns} shifts ({sec / ns:.3f} s/shift), last shift {meas}")
+    fplus_amd.tune("refactor_graph", 0)
+    L.fplus_numeric_free(num)
+    for p in (d_ax0, d_axs, d_b0, d_b):
+        L.fplus_amd_dev_free(p)
+    head = "sweep_graph" if "sweep_graph" in out else legs[0]
+    res = {"metric": "factor+solve s/system at bwd err <=1e-13: mixed/learned vs FP64 GPU vs CPU",
+           "value": out[head]["s_per_system_with_setup"], "unit": "s/system", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": out[head]["sweep_s"] * 1e3, "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+           "config": {"workload": args.workload, "matrix": "complex shift-invert pencil", **kw, "n": n, "nnz": nnz,
+                      "shifts": "0.3 + i k / 32, k = 1..32", "nrhs_per_shift": nrhs, "value_leg": head,
+                      "pivot_order_from": "one pivoting factorization at sigma 0.3 + 0.5i (set-up, shared out in value)"},
+           "setup_s": setup_s, "legs": out, "roofline": None, "cpu_baseline": None}
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -198,6 +294,7 @@ def main():
     ap.add_argument("--legs", default=None, help="comma-separated legs (default: the workload's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--max-outer", type=int, default=10, help="refinement steps at most in the refined legs")
     ap.add_argument("--verbose", action="store_true", help="the factorization's progress lines (for long profiled runs)")
     ap.add_argument("--copy-probe", action="store_true", help="only the HBM counters' calibration copies")
     args = ap.parse_args()
@@ -217,6 +314,8 @@ def main():
     TOL = WORKLOAD_TOL.get(args.workload, TOL_DEFAULT)
     import scipy.sparse as sp
     from fplus_amd import accuracy as acc
+    if args.workload.startswith("sweep"):
+        return sweep_main(args, kw, legs, NRHS)
     shift = args.workload.startswith("shiftinvert")
     if shift:
         n, Ap, Ai, Ax = gen.shift_invert(**kw, sigma=SIGMA)
@@ -306,6 +405,7 @@ def main():
     L.fplus_amd_default_refine_options(C.byref(ro_ir))
     ro_ir.tol = TOL
     ro_ir.inner_tol = 1e-2
+    ro_ir.max_outer = args.max_outer
     for leg, prec in (("fp32_factor_ir", 1), ("bf16_factor_ir", 4)):
         if leg not in legs:
             continue
@@ -366,6 +466,7 @@ def main():
         L.fplus_amd_default_refine_options(C.byref(ro))
         ro.tol = TOL
         ro.inner_tol = 1e-2     # GMRES-IR's inner solves need only modest accuracy: fewer triangular solves per step
+        ro.max_outer = args.max_outer
         prec = {"fp64_refactor": 0, "fp32_refactor_ir": 1, "learned_refactor_ir": 2}
         for leg in ref_legs:
             def step():

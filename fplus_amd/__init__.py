@@ -172,6 +172,36 @@ class Numeric:
         out["fallback"] = fb.value
         return (x[:, 0] if one else x), out
 
+    def refactor_sweep(self, Ax_list, b_list, precision=0, max_outer=10, restart=8, tol=1e-14, inner_tol=1e-4):
+        """fplus_amd_refactor_sweep: each value set of Ax_list refactored and its right-hand sides b_list[k] (n or n x r,
+        the same r for every shift) solved; returns ([x_k], [info dict per shift])"""
+        dt = np.complex128 if self.is_complex else np.float64
+        nsh = len(Ax_list)
+        Ax = np.ascontiguousarray(np.stack([np.asarray(a, dtype=dt) for a in Ax_list], axis=0))
+        bs = [np.asarray(x).reshape(self.n, -1) for x in b_list]
+        r = bs[0].shape[1] if bs else 0
+        B = np.empty((nsh, r, self.n), dtype=dt)   # shift-major, then column-major per shift
+        for k, x in enumerate(bs):
+            B[k] = x.T
+        info = (RefineInfo * max(nsh, 1))()
+        fb = (C.c_int * max(nsh, 1))()
+        ro = RefineOptions()
+        lib().fplus_amd_default_refine_options(C.byref(ro))
+        ro.max_outer, ro.restart, ro.tol, ro.inner_tol = max_outer, restart, tol, inner_tol
+        err = C.create_string_buffer(512)
+        check(lib().fplus_amd_refactor_sweep(self.ptr, _vp(Ax), nsh, C.c_void_p(B.ctypes.data), r, self.n, 0, int(precision),
+                                             C.byref(ro), info, fb, err, 512), err)
+        self._Ax = Ax[-1] if nsh else self._Ax
+        self._refresh()
+        xs = [B[k].T.copy() for k in range(nsh)]
+        xs = [x[:, 0] if np.asarray(b_list[k]).ndim == 1 else x for k, x in enumerate(xs)]
+        infos = []
+        for k in range(nsh):
+            d = {f: getattr(info[k], f) for f, _ in RefineInfo._fields_ if f != "reserved"}
+            d["fallback"] = fb[k]
+            infos.append(d)
+        return xs, infos
+
     def solve(self, b, trans=0):
         """solve in place on a copy: trans 0 A x = b, 1 A^T x = b, 2 A^H x = b; b of shape (n,) or (n, k)"""
         b = np.asarray(b)

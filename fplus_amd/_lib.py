@@ -138,6 +138,8 @@ SIGNATURES = {
     "fplus_amd_solve_refined": (C.c_int, [VP, VP, C.c_int, C.c_int64, C.c_int64, VP, VP, C.c_char_p, C.c_size_t]),
     "fplus_amd_refactor_solve": (C.c_int, [VP, VP, C.c_int, VP, C.c_int, C.c_int64, C.c_int64, C.c_int, VP, VP,
                                            C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
+    "fplus_amd_refactor_sweep": (C.c_int, [VP, VP, C.c_int64, VP, C.c_int64, C.c_int64, C.c_int, C.c_int, VP, VP,
+                                           C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
     "fplus_amd_selector_create": (C.c_int, [C.c_uint64, C.POINTER(VP)]),
     "fplus_amd_selector_free": (None, [VP]),
     "fplus_amd_selector_train": (C.c_int, [VP, FP, FP, C.c_int64, C.c_int, C.c_float, C.c_uint64, FP]),

@@ -904,7 +904,11 @@ static int factor_entry(const fplus_symbolic* sym, const int64_t* Ap, const int6
         num->n = n;
         num->is_complex = is_complex;
         const int64_t nnz = c->nnz;
-        DevBuf<double> d_Ax, d_Bx((size_t)(nnz > 0 ? nnz : 1) * vs);
+        // the new values' equilibrated, analysed form goes into the factorization's own buffer (d_Abar, which the
+        // refined solves read), so that a captured replay finds its input at the same address every time
+        DevBuf<double> d_Ax;
+        if (num->d_Abar.n < (size_t)(nnz > 0 ? nnz : 1) * vs) num->d_Abar.alloc((size_t)(nnz > 0 ? nnz : 1) * vs);
+        DevBuf<double>& d_Bx = num->d_Abar;
         const double* src = static_cast<const double*>(Ax);
         if (!device_values) {
             d_Ax.alloc((size_t)(nnz > 0 ? nnz : 1) * vs);
@@ -1119,7 +1123,11 @@ extern "C" int fplus_amd_refactor(fplus_numeric* num, const void* Ax, int ax_on_
         std::lock_guard<std::mutex> g2(num->mu);
         const double t0 = now_s();
         hipStream_t s = devctx().stream;
-        DevBuf<double> d_Ax, d_Bx((size_t)(nnz > 0 ? nnz : 1) * vs);
+        // the new values' equilibrated, analysed form goes into the factorization's own buffer (d_Abar, which the
+        // refined solves read), so that a captured replay finds its input at the same address every time
+        DevBuf<double> d_Ax;
+        if (num->d_Abar.n < (size_t)(nnz > 0 ? nnz : 1) * vs) num->d_Abar.alloc((size_t)(nnz > 0 ? nnz : 1) * vs);
+        DevBuf<double>& d_Bx = num->d_Abar;
         const double* src = static_cast<const double*>(Ax);
         if (!ax_on_device) {
             d_Ax.alloc((size_t)(nnz > 0 ? nnz : 1) * vs);
@@ -1142,7 +1150,6 @@ extern "C" int fplus_amd_refactor(fplus_numeric* num, const void* Ax, int ax_on_
         num->refactor_replaced = replaced;
         num->refactor_min_ratio = minr;
         if (rc != FPLUS_OK) return rc;
-        num->d_Abar = std::move(d_Bx);
         std::vector<double> Drp(n), Dcp(n);
         num->d_Drp.download(Drp.data(), n, s);
         num->d_Dcp.download(Dcp.data(), n, s);
@@ -1189,6 +1196,11 @@ extern "C" int fplus_amd_tune(const char* name, int64_t value)
     }
     if (std::strcmp(name, "solve_noalloc_limit_bytes") == 0) {
         if (value < 0 || !set_noalloc_limit((size_t)value)) return FPLUS_ERR_BADARG;
+        return FPLUS_OK;
+    }
+    if (std::strcmp(name, "refactor_graph") == 0) {
+        if (value != 0 && value != 1) return FPLUS_ERR_BADARG;
+        set_refactor_graph(value != 0);
         return FPLUS_OK;
     }
     if (std::strcmp(name, "xcd_tile_order") == 0) {

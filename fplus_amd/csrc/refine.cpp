@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -377,5 +378,33 @@ extern "C" int fplus_amd_refactor_solve(fplus_numeric* num, const void* Ax, int 
         return FPLUS_ERR_GPU;
     }
     if (info) *info = inf;
+    return FPLUS_OK;
+}
+
+extern "C" int fplus_amd_refactor_sweep(fplus_numeric* num, const void* Ax, int64_t nshift, void* b, int64_t nrhs, int64_t ldb,
+                                        int on_device, int precision, const fplus_amd_refine_options* ro,
+                                        fplus_amd_refine_info* info, int* fallback, char* err, size_t errlen)
+{
+    if (!num) { errf(err, errlen, "no factorization"); return FPLUS_ERR_BADARG; }
+    if (nshift < 0 || nrhs < 0 || ldb < num->n || !Ax || (nrhs > 0 && !b)) {
+        errf(err, errlen, "fplus_amd_refactor_sweep: bad nshift, nrhs, ldb, Ax or b");
+        return FPLUS_ERR_BADARG;
+    }
+    const int vs = num->F.vs;
+    const int64_t nnz = num->cache->nnz;
+    for (int64_t k = 0; k < nshift; k++) {
+        const double* ax = static_cast<const double*>(Ax) + (size_t)(k * nnz) * vs;
+        double* bk = static_cast<double*>(b) + (size_t)(k * ldb * nrhs) * vs;
+        fplus_amd_refine_info ik;
+        int fk = 0;
+        const int rc = fplus_amd_refactor_solve(num, ax, on_device, bk, on_device, nrhs, ldb, precision, ro, &ik, &fk, err, errlen);
+        if (info) info[k] = ik;
+        if (fallback) fallback[k] = fk;
+        if (rc != FPLUS_OK) {
+            const size_t used = std::strlen(err ? err : "");
+            if (err && used + 32 < errlen) std::snprintf(err + used, errlen - used, " (shift %ld)", (long)k);
+            return rc;
+        }
+    }
     return FPLUS_OK;
 }

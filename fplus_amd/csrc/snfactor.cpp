@@ -18,6 +18,7 @@
 
 #include "host_common.h"
 #include "host_pool.h"
+#include "prof.h"
 #include "selector_dev.h"
 
 #include <algorithm>
@@ -38,6 +39,9 @@ namespace fpa {
 static std::atomic<bool> g_xcd_order{true};
 bool xcd_tile_order() { return g_xcd_order.load(std::memory_order_relaxed); }
 void set_xcd_tile_order(bool on) { g_xcd_order.store(on, std::memory_order_relaxed); }
+static std::atomic<bool> g_refactor_graph{false};
+bool refactor_graph() { return g_refactor_graph.load(std::memory_order_relaxed); }
+void set_refactor_graph(bool on) { g_refactor_graph.store(on, std::memory_order_relaxed); }
 
 namespace {
 
@@ -1329,6 +1333,11 @@ template <class T> void up_vec(DevBuf<T>& d, std::vector<T>& h, hipStream_t s)
 }
 }  // namespace
 
+Replay::~Replay()
+{
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+}
+
 void Replay::upload(hipStream_t s)
 {
     up_vec(d_rows, h_rows, s);
@@ -1396,6 +1405,7 @@ int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, c
         }
     }
     const BlockDesc* blocks = F.d_blocks.p;
+    auto replay = [&] {
     for (const ReplayPanel& P : R.panels) {
         const int32_t* rows = R.d_rows.p + P.rows_off;
         const int64_t nrow = P.Ru + P.Rl, ldw = P.ldw;
@@ -1427,6 +1437,45 @@ int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, c
         }
         if (P.nruns) k::u_runs(vs, (int32_t)P.nruns, R.d_runs.p + P.runs_off, R.d_W.p, F.d_Ux.p, F.d_Ui.p, s);
         k::rowmap_set(nrow, rows, R.d_rowmap.p, 1, s);
+    }
+    };
+    // the panel loop as launched, or through the graph captured from it (no host work and no allocation inside it:
+    // every buffer is reserved above, the status reset and read around it)
+    const bool use_graph = refactor_graph() && !kprof().on;
+    if (use_graph) {
+        uint64_t thr_bits, pr_bits;
+        std::memcpy(&thr_bits, &sel.threshold, sizeof(float));
+        thr_bits &= 0xffffffffull;
+        std::memcpy(&pr_bits, &pivot_rel, 8);
+        const std::vector<uint64_t> key = {
+            (uint64_t)precision, (uint64_t)static_pivots, pr_bits, thr_bits, (uint64_t)sel.d_model, (uint64_t)sel.d_Dr,
+            (uint64_t)sel.d_Dc, (uint64_t)d_Bp, (uint64_t)d_Bi, (uint64_t)d_Bx, (uint64_t)R.d_W.p, (uint64_t)R.d_X.p,
+            (uint64_t)R.d_colmax.p, (uint64_t)R.d_rowmap.p, (uint64_t)R.d_st.p, (uint64_t)F.d_blk_f32.p,
+            (uint64_t)F.d_blk_err.p, (uint64_t)F.d_feat.p, (uint64_t)F.d_featmax.p, (uint64_t)F.d_Ux.p, (uint64_t)F.d_Ui.p,
+            (uint64_t)blocks};
+        if (!R.gexec || R.gkey != key) {
+            if (R.gexec) {
+                (void)hipGraphExecDestroy(R.gexec);
+                R.gexec = nullptr;
+            }
+            hipGraph_t graph = nullptr;
+            hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            try {
+                replay();
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(s, &graph), "hipStreamEndCapture");
+            const hipError_t e = hipGraphInstantiate(&R.gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            hip_check(e, "hipGraphInstantiate");
+            R.gkey = key;
+        }
+        hip_check(hipGraphLaunch(R.gexec, s), "hipGraphLaunch");
+    } else {
+        replay();
     }
     RefactorStatus st;
     R.d_st.download(&st, 1, s);

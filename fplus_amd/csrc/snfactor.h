@@ -53,6 +53,10 @@ struct Replay {
     DevBuf<double> d_W, d_X, d_colmax;
     DevBuf<int32_t> d_rowmap;
     DevBuf<RefactorStatus> d_st;
+    // the captured replay (refactor_graph) and what it was captured with
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uint64_t> gkey;
+    ~Replay();
     void upload(hipStream_t s);
     size_t device_bytes() const;
 };
@@ -138,6 +142,10 @@ struct SnOptions {
 // default) or as built (tuning knob "xcd_tile_order")
 bool xcd_tile_order();
 void set_xcd_tile_order(bool on);
+// refactorizations replayed through a captured HIP graph (tuning knob "refactor_graph"): the panel loop's launches are
+// captured once per set of buffers and arguments and the graph is launched for later refactorizations
+bool refactor_graph();
+void set_refactor_graph(bool on);
 
 // a same-pattern refactorization: the values d_Bx (the recorded factorization's analysed, equilibrated pattern) factored
 // in the recorded pivot order by replaying its device work; static_pivots 0: a pivot below pivot_rel * colmax fails it

@@ -156,6 +156,16 @@ int fplus_amd_refactor_solve(fplus_numeric *num, const void *Ax, int ax_on_devic
                              int64_t ldb, int precision, const fplus_amd_refine_options *ro, fplus_amd_refine_info *info,
                              int *fallback, char *err, size_t errlen);
 
+/* A same-pattern sweep (e.g. the shifts of a shift-invert pencil): nshift value sets in Ax (nnz x nshift, the values of
+ * shift k from Ax + k nnz, the pattern num was factored with) each refactored and solved as fplus_amd_refactor_solve
+ * does, for its own nrhs right-hand sides in b (shift k's from b + k ldb nrhs, column-major, leading dimension ldb),
+ * overwritten with x.  Ax and b in device memory when on_device is 1.  info[k] and fallback[k] (arrays of nshift, or
+ * NULL) per shift.  With the tuning knob "refactor_graph" on, the refactorizations replay through a HIP graph captured
+ * on the first one.  Stops at the first failing shift (the message names it). */
+int fplus_amd_refactor_sweep(fplus_numeric *num, const void *Ax, int64_t nshift, void *b, int64_t nrhs, int64_t ldb,
+                             int on_device, int precision, const fplus_amd_refine_options *ro, fplus_amd_refine_info *info,
+                             int *fallback, char *err, size_t errlen);
+
 /* Device memory and the library's stream for those callers: allocate, free, copy (kind 0 host to device, 1 device to
  * host, 2 device to device), wait for the library's stream. */
 int fplus_amd_dev_alloc(size_t bytes, void **ptr);
@@ -166,8 +176,10 @@ int fplus_amd_dev_sync(void);
 /* Tuning knobs (process-wide; tests and benchmarks).  "panel_lu_max_workgroups": the most workgroups of the
  * cooperative panel LU (at most 512, the default); 0 runs every sub-panel on the one-workgroup kernel.
  * "xcd_tile_order": 1 (default) orders each update level's tiles so that those reading one segment slice run on one
- * XCD, 0 keeps them in task order (for measurements; applies to factorizations made afterwards).  Unknown names are
- * FPLUS_ERR_BADARG. */
+ * XCD, 0 keeps them in task order (for measurements; applies to factorizations made afterwards).  "refactor_graph": 1
+ * replays refactorizations through a HIP graph captured once per factorization and set of arguments, 0 (default)
+ * launches them one by one.  "solve_noalloc_limit_bytes": the bytes fplus_amd_solve_device_noalloc may use of its
+ * array (0: all of it; for tests of the limit).  Unknown names are FPLUS_ERR_BADARG. */
 int fplus_amd_tune(const char *name, int64_t value);
 
 /* Per-kernel timing with HIP events on the library's stream (benchmarks): on = 1 starts recording and clears the sums,
