@@ -294,7 +294,7 @@ def main():
     ap.add_argument("--legs", default=None, help="comma-separated legs (default: the workload's)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-runs", type=int, default=3)
-    ap.add_argument("--max-outer", type=int, default=10, help="refinement steps at most in the refined legs")
+    ap.add_argument("--max-outer", type=int, default=4, help="refinement steps at most in the refined legs")
     ap.add_argument("--verbose", action="store_true", help="the factorization's progress lines (for long profiled runs)")
     ap.add_argument("--copy-probe", action="store_true", help="only the HBM counters' calibration copies")
     args = ap.parse_args()
