@@ -59,11 +59,15 @@ WORKLOADS = {
     # the batching configuration's sweep: one n = 131072 pattern refactored at the 32 shifts sigma_k = 0.3 + i k / 32
     # (k = 1 .. 32), 4 right-hand sides each, in the pivot order of one pivoting factorization at sigma = 0.3 + 0.5i
     "sweep-32x4x1024": dict(modes=32, nfields=4, ncoef=1024),
+    # the batching configuration's independent systems: 256 members of the shift-invert family, each its own seed, with
+    # mode counts drawn so that n is log-uniform in [4096, 65536], 4 right-hand sides each, factored and solved one
+    # after another (no cross-system batching in this build)
+    "batch-256": dict(systems=256, nfields=4, ncoef=1024),
 }
 # the legs each workload runs (the default: the headline's four)
 WORKLOAD_LEGS = {"convdiff3d-64": ("fp64_gpu", "fp32_factor_ir", "bf16_factor_ir"),
-                 "sweep-32x4x1024": ("sweep", "sweep_graph")}
-WORKLOAD_NRHS = {"convdiff3d-64": 8, "shiftinvert-128x4x1024-s17": 64, "sweep-32x4x1024": 4}
+                 "sweep-32x4x1024": ("sweep", "sweep_graph"), "batch-256": ("batch",)}
+WORKLOAD_NRHS = {"convdiff3d-64": 8, "shiftinvert-128x4x1024-s17": 64, "sweep-32x4x1024": 4, "batch-256": 4}
 WORKLOAD_TOL = {"convdiff3d-64": 1e-14}
 DEFAULT_WORKLOAD = "shiftinvert-64x4x1024"   # n = 262144, the plain FP64 GPU drop-in configuration
 SIGMA = 0.3 + 0.7j
@@ -74,7 +78,7 @@ PEAK_HBM = 8.0e12             # MI355X HBM3E, B/s
 NRHS_DEFAULT = 16
 TOL_DEFAULT = 1e-13
 LEGS = ("fp64_gpu", "fp64_refactor", "fp32_refactor_ir", "learned_refactor_ir")
-ALL_LEGS = LEGS + ("fp32_factor_ir", "bf16_factor_ir", "sweep", "sweep_graph")
+ALL_LEGS = LEGS + ("fp32_factor_ir", "bf16_factor_ir", "sweep", "sweep_graph", "batch")
 SELECTOR = os.path.join(ROOT, "fplus_amd", "data", "selector_shiftinvert.npz")
 # the dominant kernel's HBM bytes per launch, from the rocprofv3 FETCH_SIZE and WRITE_SIZE passes summarized there
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blk_upd-hbm_traffic-shiftinvert-64x4x1024.txt")
@@ -285,6 +289,64 @@ def sweep_main(args, kw, legs, nrhs):
     print(json.dumps(res))
 
 
+def batch_main(args, kw, nrhs):
+    """many independent systems: each one's symbolic analysis made before the timed region, then every system's
+    factorization and solve from device-resident values and right-hand sides, one after another"""
+    import scipy.sparse as sp
+    from fplus_amd import accuracy as acc
+    rng = np.random.default_rng(2024)
+    L = _lib.lib()
+    opts = fplus_amd.default_options()
+    err = C.create_string_buffer(512)
+    P64 = _lib.I64P
+    systems = []
+    t0 = time.perf_counter()
+    for i in range(kw["systems"]):
+        modes = int(np.clip(np.rint(2.0 ** rng.uniform(0.0, 4.0)), 1, 16))
+        n, Ap, Ai, Ax = gen.shift_invert(modes, kw["nfields"], kw["ncoef"], seed=5000 + i)
+        sym = fplus_amd.Symbolic(n, Ap, Ai)
+        b = np.asfortranarray(gen.rhs(n, nrhs, True, seed=6000 + i))
+        systems.append((n, Ap, Ai, Ax, sym, b, dev_upload(np.ascontiguousarray(Ax)), dev_upload(b)))
+        if (i + 1) % 32 == 0:
+            log(f"batch: {i + 1} systems analysed, {time.perf_counter() - t0:.1f} s")
+    prep_s = time.perf_counter() - t0
+    ns = [x[0] for x in systems]
+    times, worst = [], {}
+    t_all = time.perf_counter()
+    for i, (n, Ap, Ai, Ax, sym, b, d_ax, d_b) in enumerate(systems):
+        t1 = time.perf_counter()
+        num = C.c_void_p()
+        _lib.check(L.fplus_amd_factor_device(sym.ptr, Ap.ctypes.data_as(P64), Ai.ctypes.data_as(P64), d_ax, 1, C.byref(opts),
+                                             C.byref(num), err, 512), err)
+        _lib.check(L.fplus_amd_solve_device(num, d_b, nrhs, n, 0, err, 512), err)
+        L.fplus_amd_dev_sync()
+        times.append(time.perf_counter() - t1)
+        if i % 16 == 0:   # every 16th system's solutions checked (outside the per-system time)
+            x = np.empty_like(b)
+            dev_download(d_b, x)
+            Dr = np.ctypeslib.as_array(L.fplus_numeric_Dr(num), (n,)).copy()
+            Dc = np.ctypeslib.as_array(L.fplus_numeric_Dc(num), (n,)).copy()
+            m = acc.all_measures(sp.csc_matrix((Ax, Ai, Ap), shape=(n, n)), x, b, Dr, Dc)
+            worst = max(worst, m, key=lambda d: d.get("equilibrated", 0.0)) if worst else m
+        L.fplus_numeric_free(num)
+        if (i + 1) % 32 == 0:
+            log(f"batch: {i + 1} systems factored and solved, {time.perf_counter() - t_all:.1f} s")
+    total = time.perf_counter() - t_all
+    for x in systems:
+        L.fplus_amd_dev_free(x[6])
+        L.fplus_amd_dev_free(x[7])
+    res = {"metric": "factor+solve s/system at bwd err <=1e-13: mixed/learned vs FP64 GPU vs CPU",
+           "value": total / len(systems), "unit": "s/system", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": total * 1e3,
+           "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "c128", "data": "synthetic",
+           "config": {"workload": args.workload, "matrix": "complex shift-invert pencil family, own seed per system", **kw,
+                      "n_min": min(ns), "n_max": max(ns), "n_median": int(np.median(ns)), "nrhs": nrhs,
+                      "batching": "none: one system after another"},
+           "total_s": total, "analysis_s": prep_s, "per_system_s": {"min": min(times), "median": float(np.median(times)),
+                                                                     "max": max(times)},
+           "backward_error_worst_checked": worst, "roofline": None, "cpu_baseline": None}
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -316,6 +378,8 @@ def main():
     from fplus_amd import accuracy as acc
     if args.workload.startswith("sweep"):
         return sweep_main(args, kw, legs, NRHS)
+    if args.workload.startswith("batch"):
+        return batch_main(args, kw, NRHS)
     shift = args.workload.startswith("shiftinvert")
     if shift:
         n, Ap, Ai, Ax = gen.shift_invert(**kw, sigma=SIGMA)

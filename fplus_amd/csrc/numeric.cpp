@@ -1198,6 +1198,10 @@ extern "C" int fplus_amd_tune(const char* name, int64_t value)
         if (value < 0 || !set_noalloc_limit((size_t)value)) return FPLUS_ERR_BADARG;
         return FPLUS_OK;
     }
+    if (std::strcmp(name, "update_tile_policy") == 0) {
+        if (value < 0 || value > 4 || !set_update_tile_policy((int)value)) return FPLUS_ERR_BADARG;
+        return FPLUS_OK;
+    }
     if (std::strcmp(name, "refactor_graph") == 0) {
         if (value != 0 && value != 1) return FPLUS_ERR_BADARG;
         set_refactor_graph(value != 0);

@@ -39,6 +39,14 @@ namespace fpa {
 static std::atomic<bool> g_xcd_order{true};
 bool xcd_tile_order() { return g_xcd_order.load(std::memory_order_relaxed); }
 void set_xcd_tile_order(bool on) { g_xcd_order.store(on, std::memory_order_relaxed); }
+static std::atomic<int> g_tile_policy{0};
+int update_tile_policy() { return g_tile_policy.load(std::memory_order_relaxed); }
+bool set_update_tile_policy(int p)
+{
+    if (p < 0 || p > 4) return false;
+    g_tile_policy.store(p, std::memory_order_relaxed);
+    return true;
+}
 static std::atomic<bool> g_refactor_graph{false};
 bool refactor_graph() { return g_refactor_graph.load(std::memory_order_relaxed); }
 void set_refactor_graph(bool on) { g_refactor_graph.store(on, std::memory_order_relaxed); }
@@ -623,15 +631,19 @@ int Driver::panel(int64_t j0, int64_t j1)
     // tile rows per level: real, 64, or 32 where the level has fewer than four 64-row tiles per compute unit (measured:
     // the 32-row tiles, which read two k chunks ahead, are as fast or faster up to there); complex, always 32 (their
     // tiles multiply with three real products instead of four, which the 64-row tiles' registers do not allow)
+    // The policy (tuning knob "update_tile_policy", chosen by fplus_amd.autotune): 0 this rule; 1 every level 32-row
+    // tiles; 2 every level 64-row tiles; 3 and 4 the real rule for both scalar types with the bound at 4 and at 16 tiles
+    // per compute unit.
     std::vector<int32_t> lev_rt(maxlevel + 2, RT);
-    if (vs_ == 2) std::fill(lev_rt.begin(), lev_rt.end(), RT / 2);
-    else {
+    const int policy = update_tile_policy();
+    if (policy == 1 || (policy == 0 && vs_ == 2)) std::fill(lev_rt.begin(), lev_rt.end(), RT / 2);
+    else if (policy != 2) {
         std::vector<int64_t> nt64(maxlevel + 2, 0);
         for (const TaskL& x : tl) {
             const BlockDesc& B = F_.blocks[x.t.block];
             nt64[x.level] += (int64_t)((B.nrows - x.t.k0 + RT - 1) / RT) * ((x.t.ncol + CTG - 1) / CTG);
         }
-        const int64_t fill = 4 * (int64_t)devctx().ncu;
+        const int64_t fill = (policy == 4 ? 16 : 4) * (int64_t)devctx().ncu;
         for (int32_t L = 0; L <= maxlevel + 1; L++) lev_rt[L] = nt64[L] < fill ? RT / 2 : RT;
     }
     std::vector<UpdTask> tasks(tl.size());

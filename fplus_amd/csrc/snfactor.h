@@ -146,6 +146,9 @@ void set_xcd_tile_order(bool on);
 // captured once per set of buffers and arguments and the graph is launched for later refactorizations
 bool refactor_graph();
 void set_refactor_graph(bool on);
+// the update tile height per dependency level (tuning knob "update_tile_policy", 0 to 4; see snfactor.cpp)
+int update_tile_policy();
+bool set_update_tile_policy(int p);
 
 // a same-pattern refactorization: the values d_Bx (the recorded factorization's analysed, equilibrated pattern) factored
 // in the recorded pivot order by replaying its device work; static_pivots 0: a pivot below pivot_rel * colmax fails it

@@ -178,7 +178,10 @@ int fplus_amd_dev_sync(void);
  * "xcd_tile_order": 1 (default) orders each update level's tiles so that those reading one segment slice run on one
  * XCD, 0 keeps them in task order (for measurements; applies to factorizations made afterwards).  "refactor_graph": 1
  * replays refactorizations through a HIP graph captured once per factorization and set of arguments, 0 (default)
- * launches them one by one.  "solve_noalloc_limit_bytes": the bytes fplus_amd_solve_device_noalloc may use of its
+ * launches them one by one.  "update_tile_policy": the block updates' tile height per dependency level, for
+ * factorizations made afterwards -- 0 (default) 64-row tiles for real levels with at least 4 tiles per compute unit and
+ * 32-row tiles otherwise, 1 all 32-row, 2 all 64-row, 3 and 4 the real rule for real and complex with the bound at 4 and
+ * 16 tiles per compute unit (fplus_amd.autotune times them).  "solve_noalloc_limit_bytes": the bytes fplus_amd_solve_device_noalloc may use of its
  * array (0: all of it; for tests of the limit).  Unknown names are FPLUS_ERR_BADARG. */
 int fplus_amd_tune(const char *name, int64_t value);
 
