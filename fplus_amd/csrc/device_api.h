@@ -41,7 +41,12 @@ template <class T> struct DevBuf {
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
-    DevBuf& operator=(DevBuf&& o) noexcept { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; return *this; }
+    DevBuf& operator=(DevBuf&& o) noexcept
+    {
+        if (this == &o) return *this;
+        release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0;
+        return *this;
+    }
     ~DevBuf() { release(); }
     void alloc(size_t cnt)
     {

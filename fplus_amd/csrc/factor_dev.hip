@@ -891,17 +891,31 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
             for (int c = 0; c < PNB; c++) Lsub[tid][c] = (have && tid < nb) ? x[c] : Sc<T>::zero();
         }
         __syncthreads();
-        // L11 y = b, sixteen lanes per column, lane i holding y_i and taking y_k from lane k of its group
+        // L11 y = b, sixteen lanes per column, lane i holding y_i and taking y_k from lane k of its group; the passes of
+        // TB / PNB columns are interleaved (every pass's column loaded first, each elimination step applied to all of
+        // them), so the workgroup's steps overlap instead of running one pass after another
+        constexpr int NPASS = TB / (TB / PNB);   // the panel is at most TB columns wide
         const int i = tid % PNB, lane = tid & 63;
-        for (int32_t c0 = a + nb; c0 < P; c0 += TB / PNB) {
-            const int32_t c = c0 + tid / PNB;
-            const bool mine = c < P && i < nb;
-            T y = mine ? Wl[(a + i) + (int64_t)c * ld] : Sc<T>::zero();
-            for (int kk = 0; kk < nb - 1; kk++) {
-                const T yk = shfl_t(y, (lane & ~(PNB - 1)) | kk);
-                if (i > kk) y = Sc<T>::fms(y, Lsub[i][kk], yk);
+        const int32_t cg = a + nb + tid / PNB;
+        T y[NPASS];
+#pragma unroll
+        for (int q = 0; q < NPASS; q++) {
+            const int32_t c = cg + q * (TB / PNB);
+            y[q] = (c < P && i < nb) ? Wl[(a + i) + (int64_t)c * ld] : Sc<T>::zero();
+        }
+        for (int kk = 0; kk < nb - 1; kk++) {
+            const T l = Lsub[i][kk];
+            const bool below = i > kk;
+#pragma unroll
+            for (int q = 0; q < NPASS; q++) {
+                const T yk = shfl_t(y[q], (lane & ~(PNB - 1)) | kk);
+                if (below) y[q] = Sc<T>::fms(y[q], l, yk);
             }
-            if (mine) Wl[(a + i) + (int64_t)c * ld] = y;
+        }
+#pragma unroll
+        for (int q = 0; q < NPASS; q++) {
+            const int32_t c = cg + q * (TB / PNB);
+            if (c < P && i < nb) Wl[(a + i) + (int64_t)c * ld] = y[q];
         }
     }
 }

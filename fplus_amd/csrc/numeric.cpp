@@ -926,7 +926,7 @@ static int factor_entry(const fplus_symbolic* sym, const int64_t* Ap, const int6
         rc = factor_core(num.get(), *c, d_Bx, vs, opt->equilibrate, opt->ruiz_max_iter, opt->ruiz_tol, opt->relax_rows, opt->panel,
                          opt->pivot_rel, opt->verbose, d_Drp, d_Dcp, *ao, err, errlen);
         if (rc != FPLUS_OK) return rc;
-        num->d_Abar = std::move(d_Bx);   // (equilibrated in place by Ruiz)
+        // (d_Abar now holds the values equilibrated in place by Ruiz)
         num->d_plu.alloc(n > 0 ? n : 1);
         num->d_plu.upload(num->F.p.data(), n, s);
         std::vector<double> Drp(n), Dcp(n);

@@ -7,8 +7,9 @@ column scales 10^U(-8, 9)) with its own seed and a mode count drawn so that n is
 is factored once with precision 3 (a member whose pivot fails the threshold test is skipped and the next seed drawn): every block's update products run in FP64 (applied) and in FP32 beside them, and
 the factorization logs per block its 12 features and the largest |P32 - P64| (in the equilibrated scale).  A block is
 labelled FP32-safe when that difference is at most tau.  The MLP (12 -> 64 -> 64 -> 2, FP32) is trained on the device
-on all blocks of the training matrices and scored on held-out seeds; the weights and the threshold go to --out, the
-scores to stdout as one JSON line.
+on all blocks of the training matrices and scored on held-out seeds.  The decision threshold is then chosen on two more held-out members at the largest training
+size by the time of a refactorization + refined solve (calibrate_threshold).  The weights and the threshold go to
+--out, the scores to stdout as one JSON line.
 """
 import argparse
 import json
@@ -42,6 +43,33 @@ def log_matrix(seed, modes):
     return n, f, d, v
 
 
+def calibrate_threshold(sel, thresholds=(0.5, 0.9, 0.99, 0.999), seeds=(910000, 910001), modes=16, nrhs=16, tol=1e-13):
+    """the decision threshold by the time of a refactorization + refined solve (tol, nrhs right-hand sides) on held-out
+    members of the family at the largest training size: for each seed, a pivoting factorization at a neighbouring
+    shift is replayed at the member's own values under each threshold; the threshold with the least total time wins
+    (the lowest on ties).  Returns (best, {threshold: seconds})."""
+    tot = {t: 0.0 for t in thresholds}
+    for seed in seeds:
+        n, Ap, Ai, Ax = gen.shift_invert(modes, 4, 1024, seed=seed, scale_exp=8.0, scale_hi=9.0)
+        _, _, _, Axp = gen.shift_invert(modes, 4, 1024, seed=seed, scale_exp=8.0, scale_hi=9.0, sigma=0.3 + 0.69j)
+        sym = fplus_amd.Symbolic(n, Ap, Ai)
+        try:
+            num = fplus_amd.Numeric(sym, Ap, Ai, Axp, amd_options=fplus_amd.amd_options(refactor=1, selector=sel))
+        except fplus_amd.FplusError:
+            continue
+        b = gen.rhs(n, nrhs, True, seed=seed)
+        num.refactor_solve(Ax, b, precision=0, tol=tol, inner_tol=1e-2)   # warm the buffers
+        for t in thresholds:
+            sel.threshold = t
+            t0 = time.perf_counter()
+            num.refactor_solve(Ax, b, precision=2, tol=tol, inner_tol=1e-2)
+            tot[t] += time.perf_counter() - t0
+        del num
+    best = min(thresholds, key=lambda t: (round(tot[t], 3), t))
+    sel.threshold = best
+    return best, tot
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--matrices", type=int, default=200)
@@ -51,7 +79,15 @@ def main(argv=None):
     ap.add_argument("--lr", type=float, default=0.05)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--calibrate-only", action="store_true",
+                    help="keep the weights in --out and only choose the decision threshold (calibrate_threshold)")
     args = ap.parse_args(argv)
+    if args.calibrate_only:
+        sel = fplus_amd.Selector.load(args.out)
+        best, tot = calibrate_threshold(sel)
+        sel.save(args.out)
+        print(json.dumps({"threshold": best, "refactor_solve_s": {str(k): v for k, v in tot.items()}}))
+        return
     rng = np.random.default_rng(args.seed)
     t0 = time.perf_counter()
     F, D, sizes = [], [], []
@@ -92,6 +128,9 @@ def main(argv=None):
            "label_fp32_share": float(y.mean()), "loss": loss, "train_accuracy": acc_tr,
            "heldout_matrices": args.heldout, "heldout_blocks": int(len(yh)), "heldout_accuracy": float((ph == yh).mean()),
            "heldout_false_fp32": float((ph & ~yh).mean()), "log_s": t1 - t0, "train_s": t2 - t1}
+    best, tot = calibrate_threshold(sel)
+    rep["threshold"] = best
+    rep["calibration_refactor_solve_s"] = {str(k): v for k, v in tot.items()}
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     sel.save(args.out)
     print(json.dumps(rep))
