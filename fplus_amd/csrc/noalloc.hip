@@ -1,6 +1,6 @@
 // noalloc.hip -- fplus_amd_solve_device_noalloc: the device solve with its scratch (the permuted right-hand sides, the
-// sweeps' intermediate vector, contribution slots and partial sums) in a fixed array of device memory that the library
-// holds per GPU, for callers that may not allocate (code under graph capture, serving loops, harnesses that forbid
+// sweeps' intermediate vector, contribution slots, partial sums and the persistent sweeps' counters) in a fixed array
+// of device memory that the library holds per GPU, for callers that may not allocate (code under graph capture, serving loops, harnesses that forbid
 // allocation).  No hipMalloc, hipFree or allocating library routine runs on this path.
 #include "numeric_obj.h"
 #include "solve_dev.h"
@@ -67,7 +67,8 @@ extern "C" int fplus_amd_solve_device_noalloc(const fplus_numeric* num, void* d_
     for (;; R--) {
         z = sweep_scratch_size(num, R);
         const size_t n_w = (size_t)n * R * vs;
-        need = (up256(n_w) + up256(z.n_y) + (tr == 0 ? up256(z.n_contrib) + up256(z.n_part) : 0)) * sizeof(double);
+        need = (up256(n_w) + up256(z.n_y) + (tr == 0 ? up256(z.n_contrib) + up256(z.n_part) + up256((z.n_cnt + 1) / 2) : 0)) *
+               sizeof(double);
         if (need <= g_limit || R == 1) break;
     }
     if (need > g_limit) {
@@ -84,6 +85,7 @@ extern "C" int fplus_amd_solve_device_noalloc(const fplus_numeric* num, void* d_
         if (tr == 0) {
             z.contrib = z.y + up256(z.n_y);
             z.part = z.contrib + up256(z.n_contrib);
+            if (z.n_cnt) z.cnt = reinterpret_cast<int32_t*>(z.part + up256(z.n_part));
         }
         std::lock_guard<std::mutex> gs(num->solve_mu);
         for (int64_t r0 = 0; r0 < nrhs; r0 += R) {

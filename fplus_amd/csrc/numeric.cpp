@@ -326,6 +326,17 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         lvl[b] = L;
     }
     order_levels(lvl, num->blk_ub, num->lev_ub);
+    {   // the persistent U sweep waits per block for these blocks (ascending)
+        std::vector<int32_t> off(nb + 1, 0), all;
+        for (int32_t b = 0; b < nb; b++) off[b + 1] = off[b] + (int32_t)dep[b].size();
+        all.resize(off[nb] > 0 ? off[nb] : 1);
+        for (int32_t b = 0; b < nb; b++) std::copy(dep[b].begin(), dep[b].end(), all.begin() + off[b]);
+        num->d_udep_off.alloc(off.size());
+        num->d_udep_off.upload(off.data(), off.size(), s);
+        num->d_udep.alloc(all.size());
+        num->d_udep.upload(all.data(), all.size(), s);
+        dev_sync(s);
+    }
     // U^T y = b: a block waits for the blocks holding the rows of its U columns above it
     pool.run(nb, [&](int64_t b, int) {
         const BlockDesc& B = F.blocks[b];
@@ -447,6 +458,54 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         num->d_lpt.alloc(ptiles.size() > 0 ? ptiles.size() : 1);
         num->d_lpt.upload(ptiles.data(), ptiles.size(), s);
         dev_sync(s);
+        {   // the persistent L sweep: per block the blocks writing its pivot rows (those whose rows below hold them), and
+            // the items level by level: the blocks' gathers, their triangle tiles, their push tiles
+            std::vector<std::vector<int32_t>> tgt(nb > 0 ? nb : 1);
+            pool.run(nb, [&](int64_t b, int) {
+                const BlockDesc& B = F.blocks[b];
+                const int32_t* br = F.block_rows[b].data();
+                std::vector<int32_t>& t = tgt[b];
+                t.clear();
+                for (int32_t r = B.w; r < B.nrows; r++) t.push_back(colblk[F.pinv[br[r]]]);
+                std::sort(t.begin(), t.end());
+                t.erase(std::unique(t.begin(), t.end()), t.end());
+            });
+            std::vector<int32_t> off(nb + 1, 0);
+            for (int32_t b = 0; b < nb; b++)
+                for (const int32_t t : tgt[b]) off[t + 1]++;
+            for (int32_t b = 0; b < nb; b++) off[b + 1] += off[b];
+            std::vector<int32_t> all(off[nb] > 0 ? off[nb] : 1), fl(off.begin(), off.end() - 1);
+            for (int32_t b = 0; b < nb; b++)
+                for (const int32_t t : tgt[b]) all[fl[t]++] = b;
+            num->d_ldep_off.alloc(off.size());
+            num->d_ldep_off.upload(off.data(), off.size(), s);
+            num->d_ldep.alloc(all.size());
+            num->d_ldep.upload(all.data(), all.size(), s);
+            std::vector<int4> items;
+            double bytes = 0;
+            for (size_t L = 0; L < nlev; L++) {
+                const int32_t k0 = num->lev_lf[L], k1 = num->lev_lf[L + 1];
+                for (int32_t k2 = k0; k2 < k1; k2++) {
+                    const int32_t b = num->blk_lf[k2];
+                    for (int32_t r0 = 0; r0 < F.blocks[b].w; r0 += 64) items.push_back(make_int4(b, r0, 0, 0));
+                }
+                for (int32_t k2 = k0; k2 < k1; k2++) {
+                    const int32_t b = num->blk_lf[k2];
+                    for (int32_t r0 = 0; r0 < F.blocks[b].w; r0 += 64) items.push_back(make_int4((1 << 29) | b, r0, 0, 0));
+                }
+                for (int32_t k2 = k0; k2 < k1; k2++) {
+                    const int32_t b = num->blk_lf[k2];
+                    for (int32_t r0 = 0; r0 < F.blocks[b].nrows - F.blocks[b].w; r0 += 64) items.push_back(make_int4((2 << 29) | b, r0, 0, 0));
+                }
+                bytes += num->bytes_lf_gather[L] + num->bytes_lf_pull[L] + num->bytes_lf_push[L];
+            }
+            if (items.size() >= ((size_t)1 << 31)) throw std::length_error("too many items for the persistent L sweep");
+            num->dfl_nitems = (int32_t)items.size();
+            num->d_dfl_items.alloc(items.size() > 0 ? items.size() : 1);
+            num->d_dfl_items.upload(items.data(), items.size(), s);
+            num->df_bytes_l = bytes;
+            dev_sync(s);
+        }
     }
     {
         const size_t nub = num->lev_ub.size() - 1;
@@ -486,6 +545,45 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
         num->d_upt.alloc(upt.size() > 0 ? upt.size() : 1);
         num->d_upt.upload(upt.data(), upt.size(), s);
         dev_sync(s);
+        {   // the persistent U sweep's items, level by level: a block's dot-product groups (of at most 4 UB_PC pieces
+            // each: a few batches per item), its reduction, its triangle tiles
+            std::vector<int4> items;
+            std::vector<int64_t> ubase(nb > 0 ? nb : 1, 0);
+            int64_t pbase = 0;
+            double bytes = 0;
+            for (size_t L = 0; L < nub; L++) {
+                const int32_t k0 = num->lev_ub[L], k1 = num->lev_ub[L + 1];
+                std::vector<int32_t> ngrp(k1 - k0, 0);
+                for (int32_t k2 = k0; k2 < k1; k2++) {
+                    const int32_t b = num->blk_ub[k2];
+                    const int64_t np = pb[b + 1] - pb[b];
+                    if (np == 0) continue;
+                    const int64_t ng = (np + 4 * k::UB_PC - 1) / (4 * k::UB_PC), per = (np + ng - 1) / ng;
+                    int32_t g = 0;
+                    for (int64_t p = pb[b]; p < pb[b + 1]; p += per, g++)
+                        items.push_back(make_int4(b, (int32_t)p, (int32_t)std::min<int64_t>(per, pb[b + 1] - p), g));
+                    ngrp[k2 - k0] = g;
+                    ubase[b] = pbase;
+                    pbase += (int64_t)g * F.blocks[b].w;
+                }
+                for (int32_t k2 = k0; k2 < k1; k2++)
+                    if (ngrp[k2 - k0] > 0) items.push_back(make_int4((1 << 29) | num->blk_ub[k2], ngrp[k2 - k0], 0, 0));
+                for (int32_t k2 = k0; k2 < k1; k2++) {
+                    const int32_t b = num->blk_ub[k2];
+                    for (int32_t r0 = 0; r0 < F.blocks[b].w; r0 += 64) items.push_back(make_int4((2 << 29) | b, r0, ngrp[k2 - k0] > 0, 0));
+                }
+                bytes += num->bytes_ub[L] + num->bytes_ub_prod[L];
+            }
+            if (items.size() >= ((size_t)1 << 31)) throw std::length_error("too many items for the persistent U sweep");
+            num->dfu_nitems = (int32_t)items.size();
+            num->d_dfu_items.alloc(items.size() > 0 ? items.size() : 1);
+            num->d_dfu_items.upload(items.data(), items.size(), s);
+            num->d_ubase.alloc(ubase.size());
+            num->d_ubase.upload(ubase.data(), ubase.size(), s);
+            num->df_part = pbase;
+            num->df_bytes_u = bytes;
+            dev_sync(s);
+        }
         const size_t nut = num->lev_utf.size() - 1, nlt = num->lev_ltb.size() - 1;
         {   // the transposed sweeps' dot-product tiles (one column each) per level
             auto tiles_of = [&](const std::vector<int32_t>& lev, const std::vector<int32_t>& blk, std::vector<int32_t>& lt, DevBuf<int2>& d) {
@@ -563,8 +661,16 @@ SweepScratch sweep_scratch_size(const fplus_numeric* num, int32_t R)
     z.n_y = (size_t)std::max<int64_t>(num->F.n, 1) * R * vs;
     z.n_contrib = (size_t)(num->nslots > 0 ? num->nslots : 1) * R * vs;
     z.n_part = (size_t)std::max<int64_t>(num->ub_slots_max, 1) * 256 * k::solve_rhs_per_pass(vs) * vs;   // a slot: 256 rows
+    if (solve_dataflow()) {
+        z.n_part = std::max(z.n_part, (size_t)std::max<int64_t>(num->df_part, 1) * R * vs);
+        z.n_cnt = (size_t)k::df_counters((int64_t)num->F.blocks.size());
+    }
     return z;
 }
+
+static std::atomic<bool> g_solve_dataflow{false};
+bool solve_dataflow() { return g_solve_dataflow.load(std::memory_order_relaxed); }
+void set_solve_dataflow(bool on) { g_solve_dataflow.store(on, std::memory_order_relaxed); }
 
 // the sweeps on w (n x R row-major, pivot order, R <= solve_rhs_per_pass) in place, with the factorization's own
 // scratch buffers (grown here when too small)
@@ -578,6 +684,10 @@ void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, hi
         num->d_ub_part.reserve(z.n_part);
         z.contrib = num->d_contrib.p;
         z.part = num->d_ub_part.p;
+        if (z.n_cnt) {
+            num->d_dfcnt.reserve(z.n_cnt);
+            z.cnt = num->d_dfcnt.p;
+        }
     }
     device_sweeps(num, w, R, trans, z, s);
 }
@@ -588,6 +698,25 @@ void device_sweeps(const fplus_numeric* num, double* w, int32_t R, int trans, co
     const DevFactor& F = num->F;
     const int vs = F.vs;
     const BlockDesc* blocks = F.d_blocks.p;
+    if (trans == 0 && z.cnt && solve_dataflow()) {
+        // L y = b from w into y, then U x = y from y into w, each as one persistent launch
+        const int32_t nb = (int32_t)F.blocks.size();
+        k::fill_i32(z.cnt, k::df_counters(nb), 0, s);
+        k::lsweep_df(vs, num->dfl_nitems, num->d_dfl_items.p, blocks, num->d_ldep_off.p, num->d_ldep.p, num->d_incp.p, num->d_incs.p,
+                     num->d_slot_off.p, w, z.y, z.contrib, z.cnt, nb, R, num->df_bytes_l, s);
+        k::usweep_df(vs, num->dfu_nitems, num->d_dfu_items.p, blocks, num->d_udep_off.p, num->d_udep.p, num->d_pa.p, num->d_pj.p,
+                     num->d_pu.p, F.d_Ux.p, num->d_ubase.p, w, z.y, z.part, z.cnt, nb, R, num->df_bytes_u, s);
+        // a wait that gave up (which the ordering of the items rules out) leaves the results incomplete: the call fails
+        // and later solves take the level sweeps
+        int32_t ab = 0;
+        hip_check(hipMemcpyAsync(&ab, z.cnt + 2, sizeof(int32_t), hipMemcpyDeviceToHost, s), "persistent sweeps' status");
+        hip_check(hipStreamSynchronize(s), "persistent sweeps");
+        if (ab) {
+            set_solve_dataflow(false);
+            throw GpuError("internal: a persistent sweep gave up waiting for its inputs; later solves use the level sweeps");
+        }
+        return;
+    }
     if (trans == 0) {
         // L y = b from w into y, then U x = y from y into w
         double* y = z.y;
@@ -947,8 +1076,9 @@ static int factor_entry(const fplus_symbolic* sym, const int64_t* Ap, const int6
         num->t_prepare = now_s() - tps;
         if (opt->verbose >= 1)
             printf("[FPLUS] solve preparation %.3fs (L levels %.3f, U pieces %.3f, U levels %.3f, slots %.3f, tiles %.3f, "
-                   "uploads %.3f)\n", num->t_prepare, num->t_prep_part[0], num->t_prep_part[1], num->t_prep_part[2],
-                   num->t_prep_part[3], num->t_prep_part[4], num->t_prep_part[5]);
+                   "uploads %.3f); %zu lower and %zu upper sweep levels\n", num->t_prepare, num->t_prep_part[0],
+                   num->t_prep_part[1], num->t_prep_part[2], num->t_prep_part[3], num->t_prep_part[4], num->t_prep_part[5],
+                   num->lev_lf.size() - 1, num->lev_ub.size() - 1);
         num->t_total = now_s() - t0;
     } catch (const GpuError& e) {
         return gpu_error(e, err, errlen);
@@ -1205,6 +1335,11 @@ extern "C" int fplus_amd_tune(const char* name, int64_t value)
     if (std::strcmp(name, "refactor_graph") == 0) {
         if (value != 0 && value != 1) return FPLUS_ERR_BADARG;
         set_refactor_graph(value != 0);
+        return FPLUS_OK;
+    }
+    if (std::strcmp(name, "solve_dataflow") == 0) {
+        if (value != 0 && value != 1) return FPLUS_ERR_BADARG;
+        set_solve_dataflow(value != 0);
         return FPLUS_OK;
     }
     if (std::strcmp(name, "xcd_tile_order") == 0) {

@@ -61,9 +61,18 @@ struct fplus_numeric {
     // rows a .. w-1 (a = pa[p]), in column order; piece p holds U(c0 + t, pj[p]) at Ux[pu[p] + t]
     DevBuf<int64_t> d_pu;
     DevBuf<int32_t> d_pj, d_pa;
+    // the sweeps with A as persistent launches (solve_dev.h: lsweep_df / usweep_df): their item lists, per block the
+    // blocks each waits for, the U groups' first row, and the U groups' rows in all
+    DevBuf<int4> d_dfl_items, d_dfu_items;
+    int32_t dfl_nitems = 0, dfu_nitems = 0;
+    DevBuf<int32_t> d_ldep_off, d_ldep, d_udep_off, d_udep;
+    DevBuf<int64_t> d_ubase;
+    int64_t df_part = 0;
+    double df_bytes_l = 0, df_bytes_u = 0;
     // solve workspace, reused across calls (solves on one numeric object are serialized)
     mutable std::mutex solve_mu;
     mutable DevBuf<double> d_w, d_y, d_contrib, d_b, d_ub_part;
+    mutable DevBuf<int32_t> d_dfcnt;
     // the CSC export, built on first access
     mutable std::mutex mu;
     mutable bool exported = false;
@@ -93,8 +102,13 @@ struct SweepScratch {
     double* y = nullptr;
     double* contrib = nullptr;
     double* part = nullptr;
-    size_t n_y = 0, n_contrib = 0, n_part = 0;
+    int32_t* cnt = nullptr;   // the persistent sweeps' counters (cleared by the sweeps themselves)
+    size_t n_y = 0, n_contrib = 0, n_part = 0, n_cnt = 0;   // (n_cnt in ints)
 };
+// the sweeps with A as persistent launches (tuning knob "solve_dataflow"), else (the default) one launch per level and
+// step
+bool solve_dataflow();
+void set_solve_dataflow(bool on);
 SweepScratch sweep_scratch_size(const fplus_numeric* num, int32_t R);
 // the no-allocation solve's array (noalloc.hip): its size, and the bytes a call may use (0: all of it)
 size_t noalloc_scratch_bytes();

@@ -44,6 +44,20 @@ void ut_prod(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, 
              double bytes, hipStream_t s);
 void lt_prod(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const double* y, double* x, int32_t R, int conj,
              double bytes, hipStream_t s);
+// The sweeps with A as one persistent launch each (dataflow; solve_dev.hip describes the items and their waits).
+// items: (type << 29 | block, a, b, c); L: type 0 gather tile (block, first row), 1 triangle tile (block, first row),
+// 2 push tile (block, first row below); U: 0 dot-product group (block, first piece, pieces, group), 1 reduction (block,
+// groups), 2 triangle tile (block, first row, 1 if the block has groups).  dep_off / dep: per block the blocks it waits for (L: those writing
+// its pivot rows; U: those of its pieces' columns).  cnt: df_counters(nb) ints, zero at the start of a sweep pair;
+// cnt[2] nonzero afterwards when a wait gave up (the results are then incomplete).  ubase: per block its groups' first
+// row in part (groups of w rows of R scalars: group g's row t at (ubase[b] + g w + t) R).
+inline int64_t df_counters(int64_t nb) { return 4 + 6 * nb; }
+void lsweep_df(int vs, int32_t nitems, const int4* items, const BlockDesc* blocks, const int32_t* dep_off, const int32_t* dep,
+               const int64_t* incp, const int64_t* incs, const int64_t* slot_off, double* w, double* y, double* contrib,
+               int32_t* cnt, int32_t nb, int32_t R, double bytes, hipStream_t s);
+void usweep_df(int vs, int32_t nitems, const int4* items, const BlockDesc* blocks, const int32_t* dep_off, const int32_t* dep,
+               const int32_t* pa, const int32_t* pj, const int64_t* pu, const double* Ux, const int64_t* ubase, double* x,
+               double* y, double* part, int32_t* cnt, int32_t nb, int32_t R, double bytes, hipStream_t s);
 // w[k * R + r] = D[perm[k]] * b[perm[k] + r * ldb]  /  b[perm[k] + r * ldb] = D[perm[k]] * w[k * R + r]
 void gather_scale(int vs, int64_t n, int32_t R, const int64_t* perm, const double* D, const double* b, int64_t ldb, double* w,
                   hipStream_t s);

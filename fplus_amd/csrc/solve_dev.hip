@@ -18,6 +18,9 @@
 //   ltdots      L^T x = y (L^H): dot products down the block's columns with the solved rows in x, subtracted in y;
 //   lt_prod     x on the block's rows = L11^-T y.
 // Blocks are at most 256 columns wide (the panel width): a block's input for all right-hand sides fits in LDS.
+#include <algorithm>
+#include <atomic>
+
 #include "dev_common.h"
 #include "device_api.h"
 #include "prof.h"
@@ -46,17 +49,12 @@ __device__ inline T wave_sum(T v)
 // per row: lane = 4 g + q, group g (of 16) summing the row's slots g, g + 16, ... for the right-hand sides q, q + 4,
 // ...; the groups' sums combined in a fixed tree
 template <class T>
-__global__ __launch_bounds__(TB) void k_lfwd_gather(int32_t nrow, const int32_t* __restrict__ rows, const int64_t* __restrict__ incp,
-                                                     const int64_t* __restrict__ incs, const double* __restrict__ contribd,
-                                                     double* __restrict__ xd, int32_t R)
+__device__ __forceinline__ void gather_row(int64_t i, const int64_t* __restrict__ incp, const int64_t* __restrict__ incs,
+                                  const T* __restrict__ contrib, T* __restrict__ x, int32_t R)
 {
     constexpr int RQ = RM<T>::v / 4;
-    const T* contrib = reinterpret_cast<const T*>(contribd);
-    T* x = reinterpret_cast<T*>(xd);
     const int lane = threadIdx.x & 63, g = lane >> 2, q = lane & 3;
-    const int32_t it = blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
-    if (it >= nrow) return;
-    const int64_t i = rows[it], q0 = incp[i], q1 = incp[i + 1];
+    const int64_t q0 = incp[i], q1 = incp[i + 1];
     T acc[RQ];
 #pragma unroll
     for (int u = 0; u < RQ; u++) acc[u] = Sc<T>::zero();
@@ -79,6 +77,15 @@ __global__ __launch_bounds__(TB) void k_lfwd_gather(int32_t nrow, const int32_t*
         if (g == 0 && r < R) x[i * R + r] = Sc<T>::sub(x[i * R + r], v);
     }
 }
+template <class T>
+__global__ __launch_bounds__(TB) void k_lfwd_gather(int32_t nrow, const int32_t* __restrict__ rows, const int64_t* __restrict__ incp,
+                                                     const int64_t* __restrict__ incs, const double* __restrict__ contribd,
+                                                     double* __restrict__ xd, int32_t R)
+{
+    const int32_t it = blockIdx.x * (TB / 64) + (threadIdx.x >> 6);
+    if (it >= nrow) return;
+    gather_row<T>(rows[it], incp, incs, reinterpret_cast<const T*>(contribd), reinterpret_cast<T*>(xd), R);
+}
 
 // The blocks' diagonal triangles in the sweeps, as products with the inverses kept from the factorization:
 // out(block rows) = Op in(block rows), Op = L11^-1 (lower, unit diagonal) or U11^-1 (upper) for the sweeps with A, and
@@ -89,14 +96,11 @@ __global__ __launch_bounds__(TB) void k_lfwd_gather(int32_t nrow, const int32_t*
 // in LDS; the quarters' sums are added in wave order -- a fixed order that does not depend on the number of right-hand
 // sides.  The output goes to another vector: the block's input is still read by its other row chunks.
 template <class T, bool UPPER, bool TRANS>
-__global__ __launch_bounds__(TB) void k_tri_prod(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
-                                                  const double* __restrict__ ind, double* __restrict__ outd, int32_t R,
-                                                  int conj)
+__device__ __forceinline__ void tri_prod_tile(const int2 tl, const BlockDesc* __restrict__ blocks, const double* __restrict__ ind,
+                                     double* __restrict__ outd, int32_t R, int conj, T* xs)
 {
     constexpr int RC = RM<T>::v;
-    __shared__ T xs[TB * RC];
-    const int2 tl = tiles[blockIdx.x];   // block, first row
-    const BlockDesc B = blocks[tl.x];
+    const BlockDesc B = blocks[tl.x];   // tl: block, first row
     // (global pointers: a generic load would make each LDS wait below wait for the matrix loads in flight too)
     const gptr<const T> M = as_global(reinterpret_cast<const T*>(UPPER != TRANS ? B.uinv : B.linv));
     const T* in = reinterpret_cast<const T*>(ind);
@@ -139,19 +143,24 @@ __global__ __launch_bounds__(TB) void k_tri_prod(const int2* __restrict__ tiles,
             if (r < R) out[((int64_t)B.c0 + row) * R + r] = acc[r];
     }
 }
+template <class T, bool UPPER, bool TRANS>
+__global__ __launch_bounds__(TB) void k_tri_prod(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
+                                                  const double* __restrict__ ind, double* __restrict__ outd, int32_t R,
+                                                  int conj)
+{
+    __shared__ T xs[TB * RM<T>::v];
+    tri_prod_tile<T, UPPER, TRANS>(tiles[blockIdx.x], blocks, ind, outd, R, conj, xs);
+}
 
 // L y = b, part 2: the rows below a block, 64 rows per workgroup (lane = row), the four waves a quarter of the block's
 // columns each (the block's y for all columns and right-hand sides in LDS), the quarters' sums added in wave order;
 // slot_off[b] is the block's first slot
 template <class T>
-__global__ __launch_bounds__(TB) void k_lfwd_push(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
-                                                   const int64_t* __restrict__ slot_off, const double* __restrict__ yd,
-                                                   double* __restrict__ contribd, int32_t R)
+__device__ __forceinline__ void push_tile(const int2 tl, const BlockDesc* __restrict__ blocks, const int64_t* __restrict__ slot_off,
+                                 const double* __restrict__ yd, double* __restrict__ contribd, int32_t R, T* ys)
 {
     constexpr int RC = RM<T>::v;
-    __shared__ T ys[TB * RC];
-    const int2 tl = tiles[blockIdx.x];   // block, first row below the block (0-based)
-    const BlockDesc B = blocks[tl.x];
+    const BlockDesc B = blocks[tl.x];   // tl: block, first row below the block (0-based)
     const gptr<const T> V = as_global(reinterpret_cast<const T*>(B.vals));   // a global pointer, as in k_tri_prod
     const T* y = reinterpret_cast<const T*>(yd);
     T* contrib = reinterpret_cast<T*>(contribd);
@@ -194,6 +203,14 @@ __global__ __launch_bounds__(TB) void k_lfwd_push(const int2* __restrict__ tiles
             if (r < R) contrib[slot * R + r] = acc[r];
     }
 }
+template <class T>
+__global__ __launch_bounds__(TB) void k_lfwd_push(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
+                                                   const int64_t* __restrict__ slot_off, const double* __restrict__ yd,
+                                                   double* __restrict__ contribd, int32_t R)
+{
+    __shared__ T ys[TB * RM<T>::v];
+    push_tile<T>(tiles[blockIdx.x], blocks, slot_off, yd, contribd, R, ys);
+}
 
 // U x = y, part 1: the products of a block's pivot rows with their U rows beyond the block, one share of the block's
 // pieces per workgroup.  A piece is a later column j's run over the block's rows [a, w), the pieces in column order: thread t
@@ -201,17 +218,15 @@ __global__ __launch_bounds__(TB) void k_lfwd_push(const int2* __restrict__ tiles
 // pieces go in batches of UB_PC, whose x_j for all right-hand sides sit in LDS.  A share's sums go to its own slot of
 // `part`; ub_red subtracts a block's slots from x in slot order (a fixed order, the same for any number of right-hand
 // sides).
+// (the sums of pieces [p0, p0 + np) of block b go to dst[t * stride + r] for block row t)
 template <class T>
-__global__ __launch_bounds__(TB) void k_ub_dots(const int4* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
-                                                 const int32_t* __restrict__ pa, const int32_t* __restrict__ pj,
-                                                 const int64_t* __restrict__ pu, const double* __restrict__ Uxd,
-                                                 const double* __restrict__ xd, double* __restrict__ partd, int32_t R)
+__device__ __forceinline__ void dots_tile(int32_t b, int32_t p0t, int32_t np, const BlockDesc* __restrict__ blocks,
+                                 const int32_t* __restrict__ pa, const int32_t* __restrict__ pj, const int64_t* __restrict__ pu,
+                                 const double* __restrict__ Uxd, const double* __restrict__ xd, T* __restrict__ dst,
+                                 int64_t stride, int32_t R, T* sx, int32_t* sa, int64_t* su)
 {
     constexpr int RC = RM<T>::v, PC = k::UB_PC;
-    __shared__ int32_t sa[PC];
-    __shared__ int64_t su[PC];
-    __shared__ T sx[PC * RC];
-    const int4 tl = tiles[blockIdx.x];   // block, first piece, pieces, slot
+    const int4 tl = make_int4(b, p0t, np, 0);
     const int w = blocks[tl.x].w, t = threadIdx.x;
     const T* Ux = reinterpret_cast<const T*>(Uxd);
     const T* x = reinterpret_cast<const T*>(xd);
@@ -241,10 +256,24 @@ __global__ __launch_bounds__(TB) void k_ub_dots(const int4* __restrict__ tiles, 
         }
     }
     if (t >= w) return;
-    T* part = reinterpret_cast<T*>(partd) + ((int64_t)tl.w * TB + t) * RC;
+    T* part = dst + (int64_t)t * stride;
 #pragma unroll
     for (int r = 0; r < RC; r++)
         if (r < R) part[r] = acc[r];
+}
+template <class T>
+__global__ __launch_bounds__(TB) void k_ub_dots(const int4* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
+                                                 const int32_t* __restrict__ pa, const int32_t* __restrict__ pj,
+                                                 const int64_t* __restrict__ pu, const double* __restrict__ Uxd,
+                                                 const double* __restrict__ xd, double* __restrict__ partd, int32_t R)
+{
+    constexpr int RC = RM<T>::v, PC = k::UB_PC;
+    __shared__ int32_t sa[PC];
+    __shared__ int64_t su[PC];
+    __shared__ T sx[PC * RC];
+    const int4 tl = tiles[blockIdx.x];   // block, first piece, pieces, slot
+    dots_tile<T>(tl.x, tl.y, tl.z, blocks, pa, pj, pu, Uxd, xd, reinterpret_cast<T*>(partd) + (int64_t)tl.w * TB * RC, RC, R, sx,
+                 sa, su);
 }
 
 // U x = y, between the parts: x minus a block's partial sums, 16 rows of the block per workgroup (thread: row
@@ -391,6 +420,191 @@ __global__ void k_scatter_scale(int64_t n, int32_t R, const int64_t* __restrict_
     }
 }
 
+// ---- the sweeps with A as one persistent launch each (dataflow)
+//
+// Every level's work becomes items in one list, in level order: L y = b has per block gather tiles G (64 of its pivot
+// rows subtract their slots), triangle tiles T (64 rows: y = L11^-1 x) and push tiles P (64 rows below); U x = y has per
+// block dot-product groups D (a share of its pieces into the group's own partial rows), one reduction Rd (y on its rows
+// minus the groups' sums, in group order) and triangle tiles X (64 rows: x = U11^-1 y).  A workgroup takes the next item
+// by an atomic ticket and first waits until what the item reads is complete: per block counters of finished items,
+// raised after a release fence.  G waits for the push tiles of every block writing its pivot rows, T for its block's G, P for every
+// T of its block; D waits for the X tiles of every block its pieces' columns belong to, Rd for the block's D groups, X
+// for Rd.  An item waits only for items earlier in the list, and every item earlier in the list was taken by a
+// workgroup already running, so the waits end whatever the grid's size and residency.  A wait gives up after
+// DF_SPIN_LIMIT polls, raises the abort flag and the workgroup returns; the other waits see the flag and return too.
+// Counters (int32): [0] the L ticket, [1] the U ticket, [2] the abort flag, [3] spare, then per block G, T, P, D, Rd, X.
+// Every sum runs in a fixed order (a row's slots in slot order, a block's groups in group order): the same bits on every
+// run and for any number of right-hand sides.
+constexpr long DF_SPIN_LIMIT = 1L << 22;
+constexpr int DF_TYPE_SHIFT = 29;
+
+__device__ inline bool df_wait(const int32_t* c, int32_t target, int32_t* abortf)
+{
+    long spins = 0;
+    while (__hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        if ((++spins & 63) == 0 && __hip_atomic_load(abortf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return false;
+        if (spins > DF_SPIN_LIMIT) {
+            __hip_atomic_store(abortf, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+    return true;
+}
+__device__ inline int32_t df_tiles64(int32_t n) { return (n + 63) / 64; }
+
+// takes the next item and waits for its inputs (thread 0), then publishes it to the workgroup; -1: no more work
+template <class WAIT>
+__device__ inline int32_t df_next(int32_t* ticket, int32_t nitems, int32_t* abortf, const WAIT& wait_for, int32_t* sh)
+{
+    __syncthreads();   // the previous item is done with the LDS and sh
+    if (threadIdx.x == 0) {
+        const int32_t it = __hip_atomic_fetch_add(ticket, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        int32_t out = -1;
+        if (it < nitems && wait_for(it)) {
+            __threadfence();   // acquire: what the finished items wrote
+            out = it;
+        }
+        sh[0] = out;
+    }
+    __syncthreads();
+    return sh[0];
+}
+// the item is done: its writes released, then the block's counter raised
+__device__ inline void df_done(int32_t* c)
+{
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        __hip_atomic_fetch_add(c, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <class T>
+__global__ __launch_bounds__(TB) void k_lsweep_df(int32_t nitems, const int4* __restrict__ items, const BlockDesc* __restrict__ blocks,
+                                                   const int32_t* __restrict__ dep_off, const int32_t* __restrict__ dep,
+                                                   const int64_t* __restrict__ incp, const int64_t* __restrict__ incs,
+                                                   const int64_t* __restrict__ slot_off, double* __restrict__ wd,
+                                                   double* __restrict__ yd, double* __restrict__ contribd, int32_t* cnt,
+                                                   int32_t nb, int32_t R)
+{
+    __shared__ T lds[TB * RM<T>::v];
+    __shared__ int32_t sh[1];
+    int32_t* abortf = cnt + 2;
+    int32_t *cG = cnt + 4, *cT = cG + nb, *cP = cT + nb;
+    auto wait_for = [&](int32_t it) -> bool {
+        const int4 x = items[it];
+        const int type = x.x >> DF_TYPE_SHIFT, b = x.x & ((1 << DF_TYPE_SHIFT) - 1);
+        if (type == 0) {
+            for (int32_t k2 = dep_off[b]; k2 < dep_off[b + 1]; k2++) {
+                const int32_t bp = dep[k2];
+                if (!df_wait(cP + bp, df_tiles64(blocks[bp].nrows - blocks[bp].w), abortf)) return false;
+            }
+            return true;
+        }
+        if (type == 1) return df_wait(cG + b, df_tiles64(blocks[b].w), abortf);
+        return df_wait(cT + b, df_tiles64(blocks[b].w), abortf);
+    };
+    for (;;) {
+        const int32_t it = df_next(cnt, nitems, abortf, wait_for, sh);
+        if (it < 0) return;
+        const int4 x = items[it];
+        const int type = x.x >> DF_TYPE_SHIFT, b = x.x & ((1 << DF_TYPE_SHIFT) - 1);
+        if (type == 0) {
+            // G: rows [x.y, x.y + 64) of the block, a thread per (row, right-hand sides t % 4, t % 4 + 4, ..): the row's
+            // slots summed in slot order
+            const BlockDesc B = blocks[b];
+            const int row = x.y + (int)(threadIdx.x >> 2), q = threadIdx.x & 3;
+            if (row < B.w && q < R) {
+                const T* contrib = reinterpret_cast<const T*>(contribd);
+                T* wv = reinterpret_cast<T*>(wd);
+                const int64_t i = (int64_t)B.c0 + row, q0 = incp[i], q1 = incp[i + 1];
+                constexpr int RQ = RM<T>::v / 4;
+                T acc[RQ];
+#pragma unroll
+                for (int u = 0; u < RQ; u++) acc[u] = Sc<T>::zero();
+                for (int64_t p = q0; p < q1; p++) {
+                    const int64_t sl = incs[p] * R;
+#pragma unroll
+                    for (int u = 0; u < RQ; u++)
+                        if (q + 4 * u < R) acc[u] = Sc<T>::add(acc[u], contrib[sl + q + 4 * u]);
+                }
+#pragma unroll
+                for (int u = 0; u < RQ; u++)
+                    if (q + 4 * u < R) wv[i * R + q + 4 * u] = Sc<T>::sub(wv[i * R + q + 4 * u], acc[u]);
+            }
+            df_done(cG + b);
+        } else if (type == 1) {
+            tri_prod_tile<T, false, false>(make_int2(b, x.y), blocks, wd, yd, R, 0, lds);
+            df_done(cT + b);
+        } else {
+            push_tile<T>(make_int2(b, x.y), blocks, slot_off, yd, contribd, R, lds);
+            df_done(cP + b);
+        }
+    }
+}
+
+// part: per block its groups' partial sums, group g's row t right-hand side r at (ubase[b] + g w + t) R + r
+template <class T>
+__global__ __launch_bounds__(TB) void k_usweep_df(int32_t nitems, const int4* __restrict__ items, const BlockDesc* __restrict__ blocks,
+                                                   const int32_t* __restrict__ dep_off, const int32_t* __restrict__ dep,
+                                                   const int32_t* __restrict__ pa, const int32_t* __restrict__ pj,
+                                                   const int64_t* __restrict__ pu, const double* __restrict__ Uxd,
+                                                   const int64_t* __restrict__ ubase, double* __restrict__ xd,
+                                                   double* __restrict__ yd, double* __restrict__ partd, int32_t* cnt,
+                                                   int32_t nb, int32_t R)
+{
+    constexpr int RC = RM<T>::v, PC = k::UB_PC;
+    __shared__ T lds[TB * RC];
+    __shared__ int32_t sa[PC];
+    __shared__ int64_t su[PC];
+    __shared__ int32_t sh[1];
+    int32_t* abortf = cnt + 2;
+    int32_t *cD = cnt + 4 + 3 * nb, *cR = cD + nb, *cX = cR + nb;
+    T* part = reinterpret_cast<T*>(partd);
+    auto wait_for = [&](int32_t it) -> bool {
+        const int4 x = items[it];
+        const int type = x.x >> DF_TYPE_SHIFT, b = x.x & ((1 << DF_TYPE_SHIFT) - 1);
+        if (type == 0) {
+            for (int32_t k2 = dep_off[b]; k2 < dep_off[b + 1]; k2++) {
+                const int32_t bj = dep[k2];
+                if (!df_wait(cX + bj, df_tiles64(blocks[bj].w), abortf)) return false;
+            }
+            return true;
+        }
+        if (type == 1) return df_wait(cD + b, x.y, abortf);
+        return df_wait(cR + b, x.z, abortf);
+    };
+    for (;;) {
+        const int32_t it = df_next(cnt + 1, nitems, abortf, wait_for, sh);
+        if (it < 0) return;
+        const int4 x = items[it];
+        const int type = x.x >> DF_TYPE_SHIFT, b = x.x & ((1 << DF_TYPE_SHIFT) - 1);
+        if (type == 0) {
+            // D: pieces [x.y, x.y + x.z) into group x.w
+            const int32_t w = blocks[b].w;
+            dots_tile<T>(b, x.y, x.z, blocks, pa, pj, pu, Uxd, xd, part + (ubase[b] + (int64_t)x.w * w) * R, R, R, lds, sa, su);
+            df_done(cD + b);
+        } else if (type == 1) {
+            // Rd: y on the block's rows minus its x.y groups, in group order
+            const BlockDesc B = blocks[b];
+            T* y = reinterpret_cast<T*>(yd);
+            const T* pb = part + ubase[b] * R;
+            for (int idx = threadIdx.x; idx < B.w * R; idx += TB) {
+                const int t = idx / R, r = idx - t * R;
+                T acc = Sc<T>::zero();
+                for (int g = 0; g < x.y; g++) acc = Sc<T>::add(acc, pb[((int64_t)g * B.w + t) * R + r]);
+                T* yp = y + ((int64_t)B.c0 + t) * R + r;
+                *yp = Sc<T>::sub(*yp, acc);
+            }
+            df_done(cR + b);
+        } else {
+            tri_prod_tile<T, true, false>(make_int2(b, x.y), blocks, yd, xd, R, 0, lds);
+            df_done(cX + b);
+        }
+    }
+}
+
 inline unsigned gridn(int64_t n)
 {
     int64_t g = (n + TB - 1) / TB;
@@ -452,6 +666,50 @@ void ub_prod(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, 
     ProfScope ps("ub_prod", bytes, s);
     if (vs == 1) hipLaunchKernelGGL((k_tri_prod<double, true, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, y, x, R, 0);
     else hipLaunchKernelGGL((k_tri_prod<Zd, true, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, y, x, R, 0);
+}
+int df_grid(int vs, int upper)
+{
+    // the workgroups the device holds at once (the persistent sweeps' grid), per kernel
+    static std::atomic<int> cached[4] = {-1, -1, -1, -1};
+    const int i = (vs == 1 ? 0 : 2) + (upper ? 1 : 0);
+    int g = cached[i].load(std::memory_order_relaxed);
+    if (g < 0) {
+        int per = 0;
+        const void* f = vs == 1 ? (upper ? (const void*)k_usweep_df<double> : (const void*)k_lsweep_df<double>)
+                                : (upper ? (const void*)k_usweep_df<Zd> : (const void*)k_lsweep_df<Zd>);
+        hip_check(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, TB, 0), "occupancy of the persistent sweeps");
+        g = std::max(1, per) * std::max(1, devctx().ncu);
+        cached[i].store(g, std::memory_order_relaxed);
+    }
+    return g;
+}
+void lsweep_df(int vs, int32_t nitems, const int4* items, const BlockDesc* blocks, const int32_t* dep_off, const int32_t* dep,
+               const int64_t* incp, const int64_t* incs, const int64_t* slot_off, double* w, double* y, double* contrib,
+               int32_t* cnt, int32_t nb, int32_t R, double bytes, hipStream_t s)
+{
+    if (nitems <= 0) return;
+    ProfScope ps("lsweep_df", bytes, s);
+    const unsigned g = (unsigned)std::min(nitems, df_grid(vs, 0));
+    if (vs == 1)
+        hipLaunchKernelGGL(k_lsweep_df<double>, dim3(g), dim3(TB), 0, s, nitems, items, blocks, dep_off, dep, incp, incs, slot_off, w,
+                           y, contrib, cnt, nb, R);
+    else
+        hipLaunchKernelGGL(k_lsweep_df<Zd>, dim3(g), dim3(TB), 0, s, nitems, items, blocks, dep_off, dep, incp, incs, slot_off, w, y,
+                           contrib, cnt, nb, R);
+}
+void usweep_df(int vs, int32_t nitems, const int4* items, const BlockDesc* blocks, const int32_t* dep_off, const int32_t* dep,
+               const int32_t* pa, const int32_t* pj, const int64_t* pu, const double* Ux, const int64_t* ubase, double* x,
+               double* y, double* part, int32_t* cnt, int32_t nb, int32_t R, double bytes, hipStream_t s)
+{
+    if (nitems <= 0) return;
+    ProfScope ps("usweep_df", bytes, s);
+    const unsigned g = (unsigned)std::min(nitems, df_grid(vs, 1));
+    if (vs == 1)
+        hipLaunchKernelGGL(k_usweep_df<double>, dim3(g), dim3(TB), 0, s, nitems, items, blocks, dep_off, dep, pa, pj, pu, Ux, ubase, x,
+                           y, part, cnt, nb, R);
+    else
+        hipLaunchKernelGGL(k_usweep_df<Zd>, dim3(g), dim3(TB), 0, s, nitems, items, blocks, dep_off, dep, pa, pj, pu, Ux, ubase, x, y,
+                           part, cnt, nb, R);
 }
 void utdots(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* Up, const int32_t* Ui,
             const double* Ux, const double* y, double* x, int32_t R, int conj, double bytes, hipStream_t s)
