@@ -95,11 +95,10 @@ __global__ __launch_bounds__(TB) void k_lfwd_gather(int32_t nrow, const int32_t*
 // the chunk's last row, upper: from its first row), with the block's input for those columns and all right-hand sides
 // in LDS; the quarters' sums are added in wave order -- a fixed order that does not depend on the number of right-hand
 // sides.  The output goes to another vector: the block's input is still read by its other row chunks.
-template <class T, bool UPPER, bool TRANS>
+template <class T, bool UPPER, bool TRANS, int RC = RM<T>::v>
 __device__ __forceinline__ void tri_prod_tile(const int2 tl, const BlockDesc* __restrict__ blocks, const double* __restrict__ ind,
                                      double* __restrict__ outd, int32_t R, int conj, T* xs)
 {
-    constexpr int RC = RM<T>::v;
     const BlockDesc B = blocks[tl.x];   // tl: block, first row
     // (global pointers: a generic load would make each LDS wait below wait for the matrix loads in flight too)
     const gptr<const T> M = as_global(reinterpret_cast<const T*>(UPPER != TRANS ? B.uinv : B.linv));
@@ -119,12 +118,23 @@ __device__ __forceinline__ void tri_prod_tile(const int2 tl, const BlockDesc* __
     for (int r = 0; r < RC; r++) acc[r] = Sc<T>::zero();
     if (row < w) {
         const int lo = UPPER ? max(qa, row) : qa, hi = UPPER ? qb : min(qb, row + 1);
-#pragma unroll 8
-        for (int c = lo; c < hi; c++) {
-            const T v = TRANS ? cj(gload(M, c + (int64_t)row * w), conj) : gload(M, row + (int64_t)c * w);
+        // the column loads in batches of PF, every one of a batch issued before any is used (latency, not bandwidth,
+        // bounds a tile): clamped into the range, the entries past it not applied
+        constexpr int PF = RC == 1 ? 32 : 16;
+        for (int c0 = lo; c0 < hi; c0 += PF) {
+            T v[PF];
 #pragma unroll
-            for (int r = 0; r < RC; r++)
-                if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, xs[(c - ca) * RC + r]);
+            for (int j2 = 0; j2 < PF; j2++) {
+                const int c = min(c0 + j2, hi - 1);
+                v[j2] = TRANS ? cj(gload(M, c + (int64_t)row * w), conj) : gload(M, row + (int64_t)c * w);
+            }
+#pragma unroll
+            for (int j2 = 0; j2 < PF; j2++)
+                if (c0 + j2 < hi) {
+#pragma unroll
+                    for (int r = 0; r < RC; r++)
+                        if (r < R) acc[r] = Sc<T>::fma_(acc[r], v[j2], xs[(c0 + j2 - ca) * RC + r]);
+                }
         }
     }
     __syncthreads();   // the LDS now takes the partial sums of waves 1 .. 3
@@ -143,23 +153,22 @@ __device__ __forceinline__ void tri_prod_tile(const int2 tl, const BlockDesc* __
             if (r < R) out[((int64_t)B.c0 + row) * R + r] = acc[r];
     }
 }
-template <class T, bool UPPER, bool TRANS>
+template <class T, bool UPPER, bool TRANS, int RC = RM<T>::v>
 __global__ __launch_bounds__(TB) void k_tri_prod(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
                                                   const double* __restrict__ ind, double* __restrict__ outd, int32_t R,
                                                   int conj)
 {
-    __shared__ T xs[TB * RM<T>::v];
-    tri_prod_tile<T, UPPER, TRANS>(tiles[blockIdx.x], blocks, ind, outd, R, conj, xs);
+    __shared__ T xs[TB * RC];
+    tri_prod_tile<T, UPPER, TRANS, RC>(tiles[blockIdx.x], blocks, ind, outd, R, conj, xs);
 }
 
 // L y = b, part 2: the rows below a block, 64 rows per workgroup (lane = row), the four waves a quarter of the block's
 // columns each (the block's y for all columns and right-hand sides in LDS), the quarters' sums added in wave order;
 // slot_off[b] is the block's first slot
-template <class T>
+template <class T, int RC = RM<T>::v>
 __device__ __forceinline__ void push_tile(const int2 tl, const BlockDesc* __restrict__ blocks, const int64_t* __restrict__ slot_off,
                                  const double* __restrict__ yd, double* __restrict__ contribd, int32_t R, T* ys)
 {
-    constexpr int RC = RM<T>::v;
     const BlockDesc B = blocks[tl.x];   // tl: block, first row below the block (0-based)
     const gptr<const T> V = as_global(reinterpret_cast<const T*>(B.vals));   // a global pointer, as in k_tri_prod
     const T* y = reinterpret_cast<const T*>(yd);
@@ -178,12 +187,19 @@ __device__ __forceinline__ void push_tile(const int2 tl, const BlockDesc* __rest
 #pragma unroll
     for (int r = 0; r < RC; r++) acc[r] = Sc<T>::zero();
     if (own) {
-#pragma unroll 8
-        for (int c = qa; c < qb; c++) {
-            const T v = gload(V, (int64_t)c * B.nrows + row);
+        // loads in batches, as in tri_prod_tile
+        constexpr int PF = RC == 1 ? 32 : 16;
+        for (int c0 = qa; c0 < qb; c0 += PF) {
+            T v[PF];
 #pragma unroll
-            for (int r = 0; r < RC; r++)
-                if (r < R) acc[r] = Sc<T>::fma_(acc[r], v, ys[c * RC + r]);
+            for (int j2 = 0; j2 < PF; j2++) v[j2] = gload(V, (int64_t)min(c0 + j2, qb - 1) * B.nrows + row);
+#pragma unroll
+            for (int j2 = 0; j2 < PF; j2++)
+                if (c0 + j2 < qb) {
+#pragma unroll
+                    for (int r = 0; r < RC; r++)
+                        if (r < R) acc[r] = Sc<T>::fma_(acc[r], v[j2], ys[(c0 + j2) * RC + r]);
+                }
         }
     }
     __syncthreads();
@@ -203,13 +219,13 @@ __device__ __forceinline__ void push_tile(const int2 tl, const BlockDesc* __rest
             if (r < R) contrib[slot * R + r] = acc[r];
     }
 }
-template <class T>
+template <class T, int RC = RM<T>::v>
 __global__ __launch_bounds__(TB) void k_lfwd_push(const int2* __restrict__ tiles, const BlockDesc* __restrict__ blocks,
                                                    const int64_t* __restrict__ slot_off, const double* __restrict__ yd,
                                                    double* __restrict__ contribd, int32_t R)
 {
-    __shared__ T ys[TB * RM<T>::v];
-    push_tile<T>(tiles[blockIdx.x], blocks, slot_off, yd, contribd, R, ys);
+    __shared__ T ys[TB * RC];
+    push_tile<T, RC>(tiles[blockIdx.x], blocks, slot_off, yd, contribd, R, ys);
 }
 
 // U x = y, part 1: the products of a block's pivot rows with their U rows beyond the block, one share of the block's
@@ -246,13 +262,17 @@ __device__ __forceinline__ void dots_tile(int32_t b, int32_t p0t, int32_t np, co
         }
         __syncthreads();
         if (t < w) {
-#pragma unroll 4
-            for (int q = 0; q < npc; q++) {
-                const T u = sa[q] <= t ? Ux[su[q] + t] : Sc<T>::zero();
+            // the batch's U entries loaded before any is used (latency bounds the tile), the products in piece order
+            T u[PC];
 #pragma unroll
-                for (int r = 0; r < RC; r++)
-                    if (r < R) acc[r] = Sc<T>::fma_(acc[r], u, sx[q * RC + r]);
-            }
+            for (int q = 0; q < PC; q++) u[q] = (q < npc && sa[q] <= t) ? Ux[su[q] + t] : Sc<T>::zero();
+#pragma unroll
+            for (int q = 0; q < PC; q++)
+                if (q < npc) {
+#pragma unroll
+                    for (int r = 0; r < RC; r++)
+                        if (r < R) acc[r] = Sc<T>::fma_(acc[r], u[q], sx[q * RC + r]);
+                }
         }
     }
     if (t >= w) return;
@@ -289,12 +309,19 @@ __global__ __launch_bounds__(TB) void k_ub_red(const int4* __restrict__ tiles, c
     if (t >= B.w || r >= R) return;
     const T* part = reinterpret_cast<const T*>(partd) + ((int64_t)tl.z * TB + t) * RC + r;
     T s0 = Sc<T>::zero(), s1 = Sc<T>::zero();
-    int c = 0;
-    for (; c + 1 < tl.w; c += 2) {   // two chains, joined at the end: the order is fixed by the slot count alone
-        s0 = Sc<T>::add(s0, part[(int64_t)c * TB * RC]);
-        s1 = Sc<T>::add(s1, part[(int64_t)(c + 1) * TB * RC]);
+    // two chains, joined at the end: the order is fixed by the slot count alone; the slots' loads in batches of 16,
+    // each batch issued before its sums
+    constexpr int SB = 16;
+    for (int c0 = 0; c0 < tl.w; c0 += SB) {
+        T v[SB];
+#pragma unroll
+        for (int j = 0; j < SB; j++) v[j] = part[(int64_t)min(c0 + j, tl.w - 1) * TB * RC];
+#pragma unroll
+        for (int j = 0; j < SB; j += 2) {
+            if (c0 + j < tl.w) s0 = Sc<T>::add(s0, v[j]);
+            if (c0 + j + 1 < tl.w) s1 = Sc<T>::add(s1, v[j + 1]);
+        }
     }
-    if (c < tl.w) s0 = Sc<T>::add(s0, part[(int64_t)c * TB * RC]);
     T* xp = reinterpret_cast<T*>(xd) + ((int64_t)B.c0 + t) * R + r;
     *xp = Sc<T>::sub(*xp, Sc<T>::add(s0, s1));
 }
@@ -632,7 +659,10 @@ void lfwd_pull(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks
 {
     if (ntile <= 0) return;
     ProfScope ps("lfwd_pull", bytes, s);
-    if (vs == 1) hipLaunchKernelGGL((k_tri_prod<double, false, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, x, y, R, 0);
+    if (R == 1) {   // one right-hand side: one accumulator per thread
+        if (vs == 1) hipLaunchKernelGGL((k_tri_prod<double, false, false, 1>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, x, y, R, 0);
+        else hipLaunchKernelGGL((k_tri_prod<Zd, false, false, 1>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, x, y, R, 0);
+    } else if (vs == 1) hipLaunchKernelGGL((k_tri_prod<double, false, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, x, y, R, 0);
     else hipLaunchKernelGGL((k_tri_prod<Zd, false, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, x, y, R, 0);
 }
 void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, const int64_t* slot_off, const double* y,
@@ -640,7 +670,10 @@ void lfwd_push(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks
 {
     if (ntile <= 0) return;
     ProfScope ps("lfwd_push", bytes, s);
-    if (vs == 1) hipLaunchKernelGGL(k_lfwd_push<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, slot_off, y, contrib, R);
+    if (R == 1) {
+        if (vs == 1) hipLaunchKernelGGL((k_lfwd_push<double, 1>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, slot_off, y, contrib, R);
+        else hipLaunchKernelGGL((k_lfwd_push<Zd, 1>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, slot_off, y, contrib, R);
+    } else if (vs == 1) hipLaunchKernelGGL(k_lfwd_push<double>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, slot_off, y, contrib, R);
     else hipLaunchKernelGGL(k_lfwd_push<Zd>, dim3(ntile), dim3(TB), 0, s, tiles, blocks, slot_off, y, contrib, R);
 }
 void ub_dots(int vs, int32_t ntile, const int4* tiles, const BlockDesc* blocks, const int32_t* pa, const int32_t* pj,
@@ -664,7 +697,10 @@ void ub_prod(int vs, int32_t ntile, const int2* tiles, const BlockDesc* blocks, 
 {
     if (ntile <= 0) return;
     ProfScope ps("ub_prod", bytes, s);
-    if (vs == 1) hipLaunchKernelGGL((k_tri_prod<double, true, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, y, x, R, 0);
+    if (R == 1) {
+        if (vs == 1) hipLaunchKernelGGL((k_tri_prod<double, true, false, 1>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, y, x, R, 0);
+        else hipLaunchKernelGGL((k_tri_prod<Zd, true, false, 1>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, y, x, R, 0);
+    } else if (vs == 1) hipLaunchKernelGGL((k_tri_prod<double, true, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, y, x, R, 0);
     else hipLaunchKernelGGL((k_tri_prod<Zd, true, false>), dim3(ntile), dim3(TB), 0, s, tiles, blocks, y, x, R, 0);
 }
 int df_grid(int vs, int upper)
