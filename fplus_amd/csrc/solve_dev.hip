@@ -58,13 +58,23 @@ __device__ __forceinline__ void gather_row(int64_t i, const int64_t* __restrict_
     T acc[RQ];
 #pragma unroll
     for (int u = 0; u < RQ; u++) acc[u] = Sc<T>::zero();
-    for (int64_t p = q0 + g; p < q1; p += 16) {
-        const int64_t sl = incs[p] * R;
+    // the lane's slots g, g + 16, .. in batches of 4: the slot numbers, then the values, loaded before they are added
+    for (int64_t p0 = q0 + g; p0 < q1; p0 += 64) {
+        int64_t sl[4];
 #pragma unroll
-        for (int u = 0; u < RQ; u++) {
-            const int r = q + 4 * u;
-            if (r < R) acc[u] = Sc<T>::add(acc[u], contrib[sl + r]);
-        }
+        for (int j = 0; j < 4; j++) sl[j] = incs[min(p0 + 16 * j, q1 - 1)] * R;
+        T v[4][RQ];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int u = 0; u < RQ; u++) v[j][u] = (q + 4 * u < R) ? contrib[sl[j] + q + 4 * u] : Sc<T>::zero();
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (p0 + 16 * j < q1) {
+#pragma unroll
+                for (int u = 0; u < RQ; u++)
+                    if (q + 4 * u < R) acc[u] = Sc<T>::add(acc[u], v[j][u]);
+            }
     }
 #pragma unroll
     for (int u = 0; u < RQ; u++) {
