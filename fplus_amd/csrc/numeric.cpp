@@ -938,8 +938,9 @@ int factor_core(fplus_numeric* num, const PatternCache& c, DevBuf<double>& d_Bx,
         const double* h = num->F.upd_ncol_flops;
         const double pf = std::max(num->F.upd_prod_flops, 1.0);
         printf("[FPLUS]   block updates: product %.3e flops, issued on the matrix cores %.3e (%.2fx); by task width <=8 %.1f%%, "
-               "<=16 %.1f%%, <=32 %.1f%%, <=64 %.1f%%, more %.1f%%\n", num->F.upd_prod_flops, num->F.upd_mfma_flops,
-               num->F.upd_mfma_flops / pf, 100 * h[0] / pf, 100 * h[1] / pf, 100 * h[2] / pf, 100 * h[3] / pf, 100 * h[4] / pf);
+               "<=16 %.1f%%, <=32 %.1f%%, <=64 %.1f%%, more %.1f%%; %ld of the %ld tiles of shared levels atomic\n",
+               num->F.upd_prod_flops, num->F.upd_mfma_flops, num->F.upd_mfma_flops / pf, 100 * h[0] / pf, 100 * h[1] / pf,
+               100 * h[2] / pf, 100 * h[3] / pf, 100 * h[4] / pf, (long)num->F.atomic_tiles, (long)num->F.shared_level_tiles);
         for (int b = 0; b < 13; b++) {
             const auto& H = num->F.lev_hist[b];
             if (H.levels)
@@ -1335,6 +1336,11 @@ extern "C" int fplus_amd_tune(const char* name, int64_t value)
     if (std::strcmp(name, "refactor_graph") == 0) {
         if (value != 0 && value != 1) return FPLUS_ERR_BADARG;
         set_refactor_graph(value != 0);
+        return FPLUS_OK;
+    }
+    if (std::strcmp(name, "update_atomic_rows") == 0) {
+        if (value != 0 && value != 1) return FPLUS_ERR_BADARG;
+        set_update_atomic_rows(value != 0);
         return FPLUS_OK;
     }
     if (std::strcmp(name, "solve_dataflow") == 0) {

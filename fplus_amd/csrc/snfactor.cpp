@@ -47,6 +47,9 @@ bool set_update_tile_policy(int p)
     g_tile_policy.store(p, std::memory_order_relaxed);
     return true;
 }
+static std::atomic<bool> g_atomic_rows{true};
+bool update_atomic_rows() { return g_atomic_rows.load(std::memory_order_relaxed); }
+void set_update_atomic_rows(bool on) { g_atomic_rows.store(on, std::memory_order_relaxed); }
 static std::atomic<bool> g_refactor_graph{false};
 bool refactor_graph() { return g_refactor_graph.load(std::memory_order_relaxed); }
 void set_refactor_graph(bool on) { g_refactor_graph.store(on, std::memory_order_relaxed); }
@@ -297,6 +300,9 @@ private:
     std::vector<int32_t> pivcol_;
     std::vector<std::vector<int32_t>> succ_;   // per update task: the later tasks whose segments its rows below reach
     std::vector<std::vector<int64_t>> tmark_;   // one mark array per thread of the pool, indexed by task
+    // per thread of the pool, indexed by workspace row: the (panel, level) that last wrote the row and the row tile that
+    // wrote it (the rows two tasks of one level write)
+    std::vector<std::vector<int64_t>> wstamp_, wwho_;
     SWork ws0_;                     // the sequential phases' marks
     std::vector<SWork> wsT_;        // one per thread of the parallel search
     HostPool pool_{15};             // one core left for the helper thread
@@ -667,9 +673,53 @@ int Driver::panel(int64_t j0, int64_t j1)
         xtot += (int64_t)(F_.blocks[tasks[i].block].w - tasks[i].k0) * tasks[i].ncol;
     }
     d_X_.reserve((size_t)std::max<int64_t>(xtot * vs_, 1));
-    // a level of one task writes its rows alone (plain loads and stores); with several, rows may be shared (atomics)
+    // a level of one task writes its rows alone (plain loads and stores); with several, a row tile whose rows another
+    // task of the level writes too updates them with atomics, the others with plain loads and stores (tuning knob
+    // "update_atomic_rows": 0 every tile of such a level atomic).  The rows written twice are found per level, the
+    // levels in parallel (each thread its own row marks; a level's tasks stay on one thread).
     std::vector<int> lev_ntask(maxlevel + 2, 0);
     for (const TaskL& x : tl) lev_ntask[x.level]++;
+    std::vector<int64_t> rtoff(tl.size() + 1, 0);   // each task's first row tile in cmark
+    for (size_t i = 0; i < tl.size(); i++) {
+        const int64_t rt = lev_rt[tl[i].level];
+        rtoff[i + 1] = rtoff[i] + (F_.blocks[tl[i].t.block].nrows - tl[i].t.k0 + rt - 1) / rt;
+    }
+    std::vector<uint8_t> cmark((size_t)std::max<int64_t>(rtoff[tl.size()], 1), 1);
+    if (update_atomic_rows()) {
+        std::fill(cmark.begin(), cmark.end(), 0);
+        std::vector<int64_t> lfirst(maxlevel + 2, 0);
+        for (const TaskL& x : tl) lfirst[x.level + 1]++;
+        for (int32_t L = 0; L <= maxlevel; L++) lfirst[L + 1] += lfirst[L];
+        std::vector<int32_t> multi;
+        for (int32_t L = 1; L <= maxlevel; L++)
+            if (lev_ntask[L] > 1) multi.push_back(L);
+        for (int t = 0; t < pool_.size(); t++) {
+            if ((int64_t)wstamp_[t].size() < ldw) wstamp_[t].resize((size_t)ldw, -1);
+            if ((int64_t)wwho_[t].size() < ldw) wwho_[t].resize((size_t)ldw, 0);
+        }
+        pool_.run((int64_t)multi.size(), [&](int64_t mi, int tid) {
+            const int32_t L = multi[mi];
+            int64_t* st = wstamp_[tid].data();
+            int64_t* who = wwho_[tid].data();
+            const int64_t stamp = ((F_.panels + 1) << 32) | L, rt = lev_rt[L];
+            for (int64_t i = lfirst[L]; i < lfirst[L + 1]; i++) {
+                const BlockDesc& B = F_.blocks[tl[i].t.block];
+                const int32_t* br = F_.block_rows[tl[i].t.block].data();
+                const int64_t k0 = tl[i].t.k0;
+                for (int64_t r = B.w; r < B.nrows; r++) {
+                    const int32_t pos = hrowpos_[br[r]];
+                    const int64_t mine = rtoff[i] + (r - k0) / rt;
+                    if (st[pos] == stamp) {
+                        cmark[(size_t)mine] = 1;
+                        cmark[(size_t)who[pos]] = 1;
+                    } else {
+                        st[pos] = stamp;
+                        who[pos] = mine;
+                    }
+                }
+            }
+        });
+    }
     for (size_t i = 0; i < tl.size(); i++) {
         const UpdTask& T = tasks[i];
         const int32_t L = tl[i].level;
@@ -715,7 +765,7 @@ int Driver::panel(int64_t j0, int64_t j1)
                         u.xo = T.xoff + (int64_t)cc * m;
                         u.nrows = B.nrows; u.w = B.w; u.k0 = T.k0; u.row0 = r0;
                         u.cfirst = T.col_off + cc; u.nc = std::min<int32_t>(CTG, T.ncol - cc);
-                        u.spos0 = T.spos0; u.flags = lev_ntask[L] > 1 ? FPA_TILE_SHARED : 0;
+                        u.spos0 = T.spos0; u.flags = lev_ntask[L] > 1 && cmark[(size_t)(rtoff[i] + (r0 - T.k0) / rt)] ? FPA_TILE_SHARED : 0;
                         u.block = T.block; u.pad = 0;
                     }
             }
@@ -781,6 +831,12 @@ int Driver::panel(int64_t j0, int64_t j1)
         F_.lev_hist[bk].maxm += lev_maxm[L];
     }
     for (int32_t L = 1; L <= maxlevel; L++) F_.shared_levels += lev_ntask[L] > 1;
+    for (size_t i = 0; i < tl.size(); i++)
+        if (lev_ntask[tl[i].level] > 1) {
+            const int64_t ncc = (tl[i].t.ncol + CTG - 1) / CTG;
+            F_.shared_level_tiles += (rtoff[i + 1] - rtoff[i]) * ncc;
+            for (int64_t q = rtoff[i]; q < rtoff[i + 1]; q++) F_.atomic_tiles += cmark[(size_t)q] * ncc;
+        }
     if (rec_) {   // what a refactorization replays: the tasks, the tiles and the levels (in the order they run)
         ReplayPanel P{};
         P.j0 = j0; P.np = np; P.Ru = Ru; P.Rl = Rl; P.ldw = ldw; P.xtot = xtot;
@@ -1213,6 +1269,8 @@ int Driver::run()
     ws0_.init(nn);
     wsT_.resize(pool_.size());
     tmark_.assign(pool_.size(), {});
+    wstamp_.assign(pool_.size(), {});
+    wwho_.assign(pool_.size(), {});
     for (SWork& w : wsT_) w.init(nn);
     miss_.assign(nn, 0); ustamp_.assign(nn, 0); uidx_.assign(nn, 0);
     hrowpos_.assign(nn, -1); lposmap_.assign(nn, -1);

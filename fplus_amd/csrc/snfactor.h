@@ -106,6 +106,7 @@ struct DevFactor {
     struct LevHist { int64_t levels = 0, tiles = 0; double issued = 0, maxm = 0; } lev_hist[13];
     int64_t rows_upper = 0, rows_lower = 0, rows_lower_max = 0;   // workspace rows summed over the panels
     int64_t shared_levels = 0;   // dependency levels whose tasks can write the same rows (their updates are atomic)
+    int64_t shared_level_tiles = 0, atomic_tiles = 0;   // the update tiles of such levels, and those that write with atomics
     int64_t max_block_w = 0;     // the widest block
     bool coop_aborted = false;   // the cooperative panel LU gave up at a barrier (the factorization failed; retry without it)
     bool coop_retried = false;   // this factorization is the retry with the one-workgroup panel LU
@@ -149,6 +150,10 @@ void set_refactor_graph(bool on);
 // the update tile height per dependency level (tuning knob "update_tile_policy", 0 to 4; see snfactor.cpp)
 int update_tile_policy();
 bool set_update_tile_policy(int p);
+// on a dependency level of several update tasks, atomics only for the row tiles with rows another task of the level
+// writes too (tuning knob "update_atomic_rows", on by default), else for every tile of the level
+bool update_atomic_rows();
+void set_update_atomic_rows(bool on);
 
 // a same-pattern refactorization: the values d_Bx (the recorded factorization's analysed, equilibrated pattern) factored
 // in the recorded pivot order by replaying its device work; static_pivots 0: a pivot below pivot_rel * colmax fails it

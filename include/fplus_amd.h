@@ -181,7 +181,9 @@ int fplus_amd_dev_sync(void);
  * launches them one by one.  "update_tile_policy": the block updates' tile height per dependency level, for
  * factorizations made afterwards -- 0 (default) 64-row tiles for real levels with at least 4 tiles per compute unit and
  * 32-row tiles otherwise, 1 all 32-row, 2 all 64-row, 3 and 4 the real rule for real and complex with the bound at 4 and
- * 16 tiles per compute unit (fplus_amd.autotune times them).  "solve_dataflow": 1 runs the triangular solves with A as
+ * 16 tiles per compute unit (fplus_amd.autotune times them).  "update_atomic_rows": 1 (default) makes only the update
+ * tiles whose rows another task of their dependency level also writes use atomics, 0 every tile of such a level (for
+ * factorizations made afterwards).  "solve_dataflow": 1 runs the triangular solves with A as
  * two persistent launches whose items wait for their inputs through per-block counters, 0 (default) as one launch per
  * dependency level and step.  "solve_noalloc_limit_bytes": the bytes fplus_amd_solve_device_noalloc may use of its
  * array (0: all of it; for tests of the limit).  Unknown names are FPLUS_ERR_BADARG. */
