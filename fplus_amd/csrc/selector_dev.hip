@@ -19,7 +19,7 @@ template <class T> __device__ inline double mag(T v) { return Sc<T>::absval(v); 
 // the three largest moduli of a block -- its inverse form (nr x w), its L21 (rows w.. of its values) and L11^-1 (w x w)
 // -- over a grid of (block, column group): group g takes the columns g, g + FG, ...; the maxima meet in fmax[3 b ..]
 // (bmax: non-negative doubles as bits, max-accumulated; zero on entry)
-constexpr int FG = 16;
+constexpr int FG = 64;   // workgroups per block (column groups): the block's values and inverse form are read once
 template <class T>
 __global__ __launch_bounds__(TB) void k_feat_max(const int32_t* __restrict__ blks, const BlockDesc* __restrict__ blocks,
                                                   unsigned long long* __restrict__ bmax)
@@ -34,9 +34,23 @@ __global__ __launch_bounds__(TB) void k_feat_max(const int32_t* __restrict__ blk
     for (int32_t c = blockIdx.y; c < w; c += FG) {
         const T* ivc = IV + (int64_t)c * nr;
         const T* vc = V + (int64_t)c * nr;
-        for (int32_t r = threadIdx.x; r < nr; r += TB) {
-            m0 = fmax(m0, mag(ivc[r]));
-            if (r >= w) m1 = fmax(m1, mag(vc[r]));
+        // four rows per thread per pass, their loads issued before the maxima (the column is read once)
+        for (int32_t r0 = threadIdx.x; r0 < nr; r0 += 4 * TB) {
+            T a[4], v[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int32_t r = min(r0 + j * TB, nr - 1);
+                a[j] = ivc[r];
+                v[j] = vc[r];
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int32_t r = r0 + j * TB;
+                if (r < nr) {
+                    m0 = fmax(m0, mag(a[j]));
+                    if (r >= w) m1 = fmax(m1, mag(v[j]));
+                }
+            }
         }
         for (int32_t r = threadIdx.x; r < w; r += TB) m2 = fmax(m2, mag(LI[(int64_t)c * w + r]));
     }

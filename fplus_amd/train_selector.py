@@ -43,11 +43,12 @@ def log_matrix(seed, modes):
     return n, f, d, v
 
 
-def calibrate_threshold(sel, thresholds=(0.5, 0.9, 0.99, 0.999), seeds=(910000, 910001), modes=16, nrhs=16, tol=1e-13):
+def calibrate_threshold(sel, thresholds=(0.5, 0.9, 0.99, 0.999, 1.0), seeds=(910000, 910001), modes=16, nrhs=16, tol=1e-13):
     """the decision threshold by the time of a refactorization + refined solve (tol, nrhs right-hand sides) on held-out
     members of the family at the largest training size: for each seed, a pivoting factorization at a neighbouring
     shift is replayed at the member's own values under each threshold; the threshold with the least total time wins
-    (the lowest on ties).  Returns (best, {threshold: seconds})."""
+    (the lowest on ties).  A threshold of 1.0 marks no block (a probability never exceeds it): every update in FP64,
+    the refined solve then usually done at once.  Returns (best, {threshold: seconds})."""
     tot = {t: 0.0 for t in thresholds}
     for seed in seeds:
         n, Ap, Ai, Ax = gen.shift_invert(modes, 4, 1024, seed=seed, scale_exp=8.0, scale_hi=9.0)
