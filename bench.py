@@ -61,12 +61,12 @@ WORKLOADS = {
     "sweep-32x4x1024": dict(modes=32, nfields=4, ncoef=1024),
     # the batching configuration's independent systems: 256 members of the shift-invert family, each its own seed, with
     # mode counts drawn so that n is log-uniform in [4096, 65536], 4 right-hand sides each, factored and solved one
-    # after another (no cross-system batching in this build)
+    # after another, and side by side on several host threads with a stream each
     "batch-256": dict(systems=256, nfields=4, ncoef=1024),
 }
 # the legs each workload runs (the default: the headline's four)
 WORKLOAD_LEGS = {"convdiff3d-64": ("fp64_gpu", "fp32_factor_ir", "bf16_factor_ir"),
-                 "sweep-32x4x1024": ("sweep", "sweep_graph"), "batch-256": ("batch",)}
+                 "sweep-32x4x1024": ("sweep", "sweep_graph"), "batch-256": ("batch", "batch_streams")}
 WORKLOAD_NRHS = {"convdiff3d-64": 8, "shiftinvert-128x4x1024-s17": 64, "sweep-32x4x1024": 4, "batch-256": 4}
 WORKLOAD_TOL = {"convdiff3d-64": 1e-14}
 DEFAULT_WORKLOAD = "shiftinvert-64x4x1024"   # n = 262144, the plain FP64 GPU drop-in configuration
@@ -78,7 +78,7 @@ PEAK_HBM = 8.0e12             # MI355X HBM3E, B/s
 NRHS_DEFAULT = 16
 TOL_DEFAULT = 1e-13
 LEGS = ("fp64_gpu", "fp64_refactor", "fp32_refactor_ir", "learned_refactor_ir")
-ALL_LEGS = LEGS + ("fp32_factor_ir", "bf16_factor_ir", "sweep", "sweep_graph", "batch")
+ALL_LEGS = LEGS + ("fp32_factor_ir", "bf16_factor_ir", "sweep", "sweep_graph", "batch", "batch_streams")
 SELECTOR = os.path.join(ROOT, "fplus_amd", "data", "selector_shiftinvert.npz")
 # the dominant kernel's HBM bytes per launch, from the rocprofv3 FETCH_SIZE and WRITE_SIZE passes summarized there
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blk_upd-hbm_traffic-shiftinvert-64x4x1024.txt")
@@ -291,13 +291,15 @@ def sweep_main(args, kw, legs, nrhs):
 
 def batch_main(args, kw, nrhs):
     """many independent systems: each one's symbolic analysis made before the timed region, then every system's
-    factorization and solve from device-resident values and right-hand sides, one after another"""
+    factorization and solve from device-resident values and right-hand sides -- one after another (leg "batch"), and
+    side by side on --batch-threads host threads, each with its own stream (fplus_amd_thread_stream) and a share of the
+    host cores for its factorizations' pools (leg "batch_streams")"""
+    import threading
     import scipy.sparse as sp
     from fplus_amd import accuracy as acc
     rng = np.random.default_rng(2024)
     L = _lib.lib()
     opts = fplus_amd.default_options()
-    err = C.create_string_buffer(512)
     P64 = _lib.I64P
     systems = []
     t0 = time.perf_counter()
@@ -311,44 +313,102 @@ def batch_main(args, kw, nrhs):
             log(f"batch: {i + 1} systems analysed, {time.perf_counter() - t0:.1f} s")
     prep_s = time.perf_counter() - t0
     ns = [x[0] for x in systems]
-    times, worst, singular = [], {}, []
-    t_all = time.perf_counter()
-    for i, (n, Ap, Ai, Ax, sym, b, d_ax, d_b) in enumerate(systems):
-        t1 = time.perf_counter()
-        num = C.c_void_p()
-        rc = L.fplus_amd_factor_device(sym.ptr, Ap.ctypes.data_as(P64), Ai.ctypes.data_as(P64), d_ax, 1, C.byref(opts),
-                                       C.byref(num), err, 512)
-        if rc == _lib.FPLUS_ERR_SINGULAR:   # a member whose pivot fails the threshold test: reported, as a caller would see it
-            times.append(time.perf_counter() - t1)
-            singular.append(i)
-            continue
-        _lib.check(rc, err)
-        _lib.check(L.fplus_amd_solve_device(num, d_b, nrhs, n, 0, err, 512), err)
-        L.fplus_amd_dev_sync()
-        times.append(time.perf_counter() - t1)
-        if i % 16 == 0:   # every 16th system's solutions checked (outside the per-system time)
-            x = np.empty_like(b)
-            dev_download(d_b, x)
-            Dr = np.ctypeslib.as_array(L.fplus_numeric_Dr(num), (n,)).copy()
-            Dc = np.ctypeslib.as_array(L.fplus_numeric_Dc(num), (n,)).copy()
+    legs = [x for x in (args.legs or "batch,batch_streams").split(",") if x]
+
+    def run(nthreads):
+        """every system factored and solved, on nthreads host threads taking systems in turn; (wall seconds, per-system
+        seconds, singular systems, worst checked backward errors)"""
+        times, singular, checks = [0.0] * len(systems), [], []
+        nxt, mu = [0], threading.Lock()
+        errs = []
+
+        def worker():
+            err = C.create_string_buffer(512)
+            if nthreads > 1:
+                _lib.check(L.fplus_amd_thread_stream(1), None)
+            try:
+                while True:
+                    with mu:
+                        i = nxt[0]
+                        nxt[0] += 1
+                    if i >= len(systems):
+                        return
+                    n, Ap, Ai, Ax, sym, b, d_ax, d_b = systems[i]
+                    t1 = time.perf_counter()
+                    num = C.c_void_p()
+                    rc = L.fplus_amd_factor_device(sym.ptr, Ap.ctypes.data_as(P64), Ai.ctypes.data_as(P64), d_ax, 1, C.byref(opts),
+                                                   C.byref(num), err, 512)
+                    if rc == _lib.FPLUS_ERR_SINGULAR:   # a member whose pivot fails the threshold test: reported, as a caller would see it
+                        times[i] = time.perf_counter() - t1
+                        with mu:
+                            singular.append(i)
+                        continue
+                    _lib.check(rc, err)
+                    _lib.check(L.fplus_amd_solve_device(num, d_b, nrhs, n, 0, err, 512), err)
+                    L.fplus_amd_dev_sync()
+                    times[i] = time.perf_counter() - t1
+                    if i % 16 == 0:   # every 16th system's solutions kept for the check (outside the per-system time)
+                        x = np.empty_like(b)
+                        dev_download(d_b, x)
+                        Dr = np.ctypeslib.as_array(L.fplus_numeric_Dr(num), (n,)).copy()
+                        Dc = np.ctypeslib.as_array(L.fplus_numeric_Dc(num), (n,)).copy()
+                        with mu:
+                            checks.append((i, x, Dr, Dc))
+                    L.fplus_numeric_free(num)
+            except Exception as e:   # noqa: BLE001 -- reported by the caller
+                errs.append(e)
+            finally:
+                if nthreads > 1:
+                    L.fplus_amd_thread_stream(0)
+
+        for x in systems:   # the right-hand sides afresh (a run overwrites them with the solutions)
+            if L.fplus_amd_dev_copy(x[7], C.c_void_p(x[5].ctypes.data), x[5].nbytes, 0) != 0:
+                raise RuntimeError("upload failed")
+        tune_threads = max(1, 15 // nthreads)
+        fplus_amd.tune("host_threads", tune_threads if nthreads > 1 else 0)
+        t_all = time.perf_counter()
+        th = [threading.Thread(target=worker) for _ in range(nthreads)]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        total = time.perf_counter() - t_all
+        fplus_amd.tune("host_threads", 0)
+        if errs:
+            raise errs[0]
+        worst = {}
+        for i, x, Dr, Dc in checks:
+            n, Ap, Ai, Ax, _, b = systems[i][:6]
             m = acc.all_measures(sp.csc_matrix((Ax, Ai, Ap), shape=(n, n)), x, b, Dr, Dc)
             worst = max(worst, m, key=lambda d: d.get("equilibrated", 0.0)) if worst else m
-        L.fplus_numeric_free(num)
-        if (i + 1) % 32 == 0:
-            log(f"batch: {i + 1} systems factored and solved, {time.perf_counter() - t_all:.1f} s")
-    total = time.perf_counter() - t_all
+        return total, times, sorted(singular), worst
+
+    out = {}
+    for leg in legs:
+        nt = 1 if leg == "batch" else args.batch_threads
+        total, times, singular, worst = run(nt)
+        out[leg] = {"s_per_system": total / len(systems), "total_s": total, "host_threads": nt,
+                    "per_system_s": {"min": min(times), "median": float(np.median(times)), "max": max(times)},
+                    "singular_systems": singular, "backward_error_worst_checked": worst,
+                    "meets_target": bool(worst and worst["equilibrated"] <= 1e-13)}
+        log(f"{leg}: {nt} host threads, {total:.2f} s for {len(systems)} systems ({total / len(systems):.4f} s/system), "
+            f"worst checked {worst}")
     for x in systems:
         L.fplus_amd_dev_free(x[6])
         L.fplus_amd_dev_free(x[7])
+    head = "batch_streams" if "batch_streams" in out else legs[0]
+    total = out[head]["total_s"]
     res = {"metric": "factor+solve s/system at bwd err <=1e-13: mixed/learned vs FP64 GPU vs CPU",
            "value": total / len(systems), "unit": "s/system", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": total * 1e3,
            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "c128", "data": "synthetic",
            "config": {"workload": args.workload, "matrix": "complex shift-invert pencil family, own seed per system", **kw,
-                      "n_min": min(ns), "n_max": max(ns), "n_median": int(np.median(ns)), "nrhs": nrhs,
-                      "batching": "none: one system after another"},
-           "total_s": total, "analysis_s": prep_s, "singular_systems": singular, "per_system_s": {"min": min(times), "median": float(np.median(times)),
-                                                                     "max": max(times)},
-           "backward_error_worst_checked": worst, "roofline": None, "cpu_baseline": None}
+                      "n_min": min(ns), "n_max": max(ns), "n_median": int(np.median(ns)), "nrhs": nrhs, "value_leg": head,
+                      "batching": f"{args.batch_threads} host threads, each with its own stream" if head == "batch_streams"
+                      else "none: one system after another"},
+           "legs": out, "analysis_s": prep_s,
+           "speedup_vs_one_after_another": (out["batch"]["total_s"] / out["batch_streams"]["total_s"]
+                                            if "batch" in out and "batch_streams" in out else None),
+           "roofline": None, "cpu_baseline": None}
     print(json.dumps(res))
 
 
@@ -364,6 +424,7 @@ def main():
     ap.add_argument("--max-outer", type=int, default=4, help="refinement steps at most in the refined legs")
     ap.add_argument("--verbose", action="store_true", help="the factorization's progress lines (for long profiled runs)")
     ap.add_argument("--copy-probe", action="store_true", help="only the HBM counters' calibration copies")
+    ap.add_argument("--batch-threads", type=int, default=4, help="host threads of the batch workload's side-by-side leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported (the path would shard by "

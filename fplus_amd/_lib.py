@@ -127,6 +127,7 @@ SIGNATURES = {
     "fplus_amd_dev_free": (C.c_int, [VP]),
     "fplus_amd_dev_copy": (C.c_int, [VP, VP, C.c_size_t, C.c_int]),
     "fplus_amd_dev_sync": (C.c_int, []),
+    "fplus_amd_thread_stream": (C.c_int, [C.c_int]),
     "fplus_amd_profile": (C.c_int, [C.c_int]),
     "fplus_amd_profile_get": (C.c_int, [C.c_char_p, DP, I64P, DP]),
     "fplus_amd_tune": (C.c_int, [C.c_char_p, C.c_int64]),

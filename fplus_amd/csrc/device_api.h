@@ -27,6 +27,11 @@ struct DevCtx {
     int ncu = 0;
 };
 DevCtx& devctx();   // initialises on first use; throws GpuError when no device is usable
+// the calling thread's work on a stream of its own (on) or on the process's stream (off); see fplus_amd_thread_stream
+void set_thread_stream(bool on);
+// host threads of each factorization's pool (tuning knob "host_threads"; 0: the default, 15)
+int host_threads();
+bool set_host_threads(int n);
 
 // copies between device memory and pageable host memory, through page-locked bounce buffers (hostcopy.cpp): copy_h2d
 // returns once h may be reused (the device copy is ordered on s), copy_d2h once h holds the data

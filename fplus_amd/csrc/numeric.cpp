@@ -192,7 +192,7 @@ void prepare_solves(fplus_numeric* num, hipStream_t s)
     const int32_t nb = (int32_t)F.blocks.size();
     double tp = now_s();
     auto lap = [&](int i) { const double t = now_s(); num->t_prep_part[i] = t - tp; tp = t; };
-    HostPool pool;
+    HostPool pool(host_threads() + 1);
     std::vector<int32_t> colblk(n > 0 ? n : 1);
     // every block's U11^-1, one job per column, on the device while the host prepares the rest (kept until the
     // final synchronization below: the upload reads the host list)
@@ -1338,6 +1338,10 @@ extern "C" int fplus_amd_tune(const char* name, int64_t value)
         set_refactor_graph(value != 0);
         return FPLUS_OK;
     }
+    if (std::strcmp(name, "host_threads") == 0) {
+        if (!set_host_threads((int)std::max<int64_t>(std::min<int64_t>(value, 65), -1))) return FPLUS_ERR_BADARG;
+        return FPLUS_OK;
+    }
     if (std::strcmp(name, "update_atomic_rows") == 0) {
         if (value != 0 && value != 1) return FPLUS_ERR_BADARG;
         set_update_atomic_rows(value != 0);
@@ -1400,6 +1404,15 @@ int fplus_amd_dev_copy(void* dst, const void* src, size_t bytes, int kind)
         else if (kind == 1) copy_d2h(dst, src, bytes, s);
         else if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, s) != hipSuccess) return FPLUS_ERR_GPU;
         return hipStreamSynchronize(s) == hipSuccess ? FPLUS_OK : FPLUS_ERR_GPU;
+    } catch (const GpuError&) {
+        return FPLUS_ERR_GPU;
+    }
+}
+int fplus_amd_thread_stream(int on)
+{
+    try {
+        set_thread_stream(on != 0);
+        return FPLUS_OK;
     } catch (const GpuError&) {
         return FPLUS_ERR_GPU;
     }

@@ -253,7 +253,12 @@ KernelProf& kprof()
     return p;
 }
 
-DevCtx& devctx()
+namespace {
+thread_local hipStream_t t_stream = nullptr;   // the calling thread's own stream, once it asked for one
+thread_local bool t_own = false;
+}  // namespace
+
+static DevCtx& devctx_process()
 {
     // made once, by the first caller, under the language's guarantee for local statics (callers on other threads wait
     // for it); a failed set-up throws and leaves it to the next call
@@ -271,6 +276,25 @@ DevCtx& devctx()
         return c;
     }();
     return ctx;
+}
+
+DevCtx& devctx()
+{
+    DevCtx& ctx = devctx_process();
+    if (!t_own) return ctx;
+    // a thread with its own stream: the process's context with that stream
+    thread_local DevCtx tl;
+    tl = ctx;
+    tl.stream = t_stream;
+    return tl;
+}
+
+void set_thread_stream(bool on)
+{
+    devctx_process();
+    if (on && !t_stream)   // kept until the process ends (no HIP call at exit), at the default priority
+        hip_check(hipStreamCreateWithFlags(&t_stream, hipStreamNonBlocking), "hipStreamCreate (thread)");
+    t_own = on;
 }
 
 }  // namespace fpa

@@ -47,6 +47,18 @@ bool set_update_tile_policy(int p)
     g_tile_policy.store(p, std::memory_order_relaxed);
     return true;
 }
+static std::atomic<int> g_host_threads{0};
+int host_threads()
+{
+    const int n = g_host_threads.load(std::memory_order_relaxed);
+    return n > 0 ? n : 15;
+}
+bool set_host_threads(int n)
+{
+    if (n < 0 || n > 64) return false;
+    g_host_threads.store(n, std::memory_order_relaxed);
+    return true;
+}
 static std::atomic<bool> g_atomic_rows{true};
 bool update_atomic_rows() { return g_atomic_rows.load(std::memory_order_relaxed); }
 void set_update_atomic_rows(bool on) { g_atomic_rows.store(on, std::memory_order_relaxed); }
@@ -305,7 +317,7 @@ private:
     std::vector<std::vector<int64_t>> wstamp_, wwho_;
     SWork ws0_;                     // the sequential phases' marks
     std::vector<SWork> wsT_;        // one per thread of the parallel search
-    HostPool pool_{15};             // one core left for the helper thread
+    HostPool pool_{host_threads()};   // (15 by default: one core left for the helper thread)
     int64_t relax_ = 0, rz_w_[3] = {4, 16, 48}, rz_cap_ = 32;
     double rz_z_[3] = {0.8, 0.1, 0.05};
     bool relax_auto_ = false;

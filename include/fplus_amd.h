@@ -173,6 +173,12 @@ int fplus_amd_dev_free(void *ptr);
 int fplus_amd_dev_copy(void *dst, const void *src, size_t bytes, int kind);
 int fplus_amd_dev_sync(void);
 
+/* Independent systems side by side: on = 1 sends the library's device work from the calling thread to a stream of
+ * its own (made on the first call of the thread, kept until the process ends), so that factorizations and solves made
+ * on several threads overlap on the device; on = 0 returns the thread to the process's stream.  Handles made on one
+ * stream may be used from another thread afterwards (the calls that return results wait for their own stream). */
+int fplus_amd_thread_stream(int on);
+
 /* Tuning knobs (process-wide; tests and benchmarks).  "panel_lu_max_workgroups": the most workgroups of the
  * cooperative panel LU (at most 512, the default); 0 runs every sub-panel on the one-workgroup kernel.
  * "xcd_tile_order": 1 (default) orders each update level's tiles so that those reading one segment slice run on one
@@ -181,7 +187,8 @@ int fplus_amd_dev_sync(void);
  * launches them one by one.  "update_tile_policy": the block updates' tile height per dependency level, for
  * factorizations made afterwards -- 0 (default) 64-row tiles for real levels with at least 4 tiles per compute unit and
  * 32-row tiles otherwise, 1 all 32-row, 2 all 64-row, 3 and 4 the real rule for real and complex with the bound at 4 and
- * 16 tiles per compute unit (fplus_amd.autotune times them).  "update_atomic_rows": 1 (default) makes only the update
+ * 16 tiles per compute unit (fplus_amd.autotune times them).  "host_threads": the host threads of each factorization's
+ * pool (1 to 64; 0 the default, 15), for many factorizations at once on several threads.  "update_atomic_rows": 1 (default) makes only the update
  * tiles whose rows another task of their dependency level also writes use atomics, 0 every tile of such a level (for
  * factorizations made afterwards).  "solve_dataflow": 1 runs the triangular solves with A as
  * two persistent launches whose items wait for their inputs through per-block counters, 0 (default) as one launch per
