@@ -254,6 +254,92 @@ typedef struct {
 #undef ORC_ABS
 #undef ORC_SUF
 
+/* one row of the row kernel: sorted columns, values of the scalar type in v */
+typedef struct {
+    int64_t *c;
+    void *v;
+    int64_t nz, cap;
+} orc_rrow;
+
+/* position of column col in a sorted row, -1 when absent (src/factor_row.c:89-105) */
+static int64_t orc_rrow_find(const orc_rrow *r, int64_t col)
+{
+    int64_t lo = 0, hi = r->nz - 1;
+    while (lo <= hi) {
+        const int64_t mid = lo + (hi - lo) / 2;
+        if (r->c[mid] == col) return mid;
+        if (r->c[mid] < col) lo = mid + 1; else hi = mid - 1;
+    }
+    return -1;
+}
+
+#define ORC_T double
+#define ORC_ABS fabs
+#define ORC_SUF d
+#include "orc_row.inc"
+#undef ORC_T
+#undef ORC_ABS
+#undef ORC_SUF
+#define ORC_T double complex
+#define ORC_ABS cabs
+#define ORC_SUF z
+#include "orc_row.inc"
+#undef ORC_T
+#undef ORC_ABS
+#undef ORC_SUF
+
+void orc_rowlu_free(orc_rowlu *f)
+{
+    if (!f) return;
+    free(f->Lp); free(f->Li); free(f->Lx); free(f->Up); free(f->Ui); free(f->Ux); free(f->p); free(f->q); free(f);
+}
+
+/* the row kernel's entry point (src/factor_row.c:1191-1330: argument checks, rows loaded, the elimination) */
+int orc_factor_row(int64_t n, const int64_t *Rp, const int64_t *Ri, const double *Rx, int cplx, double drop_tol, int strategy,
+                   double pivot_rel, orc_rowlu **out, char *err, size_t errlen)
+{
+    *out = NULL;
+    if (strategy < 0 || strategy > 2) { orc_msg(err, errlen, "pivot_strategy must be 0 (full), 1 (partial), or 2 (rook); got %d", strategy); return ORC_BADARG; }
+    if (!(pivot_rel >= 0.0)) { orc_msg(err, errlen, "pivot_rel must be >= 0"); return ORC_BADARG; }
+    if (!(drop_tol >= 0.0)) { orc_msg(err, errlen, "drop_tol must be >= 0 (got %g)", drop_tol); return ORC_BADARG; }
+    int rc = orc_check_csc(n, Rp, Ri, 1, err, errlen);
+    if (rc == ORC_OK) rc = orc_check_finite(Rp[n], Rx, cplx, err, errlen);
+    if (rc != ORC_OK) return rc;
+    const size_t vs = cplx ? sizeof(double complex) : sizeof(double);
+    orc_rowlu *f = (orc_rowlu *)calloc(1, sizeof(orc_rowlu));
+    orc_rrow *M = (orc_rrow *)calloc((size_t)(n > 0 ? n : 1), sizeof(orc_rrow));
+    orc_rrow *Lm = (orc_rrow *)calloc((size_t)(n > 0 ? n : 1), sizeof(orc_rrow));
+    if (f) { f->p = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t)); f->q = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t)); }
+    rc = (!f || !M || !Lm || !f->p || !f->q) ? ORC_NOMEM : ORC_OK;
+    if (rc == ORC_OK) {
+        f->n = n; f->cplx = cplx; f->fail_step = -1;
+        for (int64_t i = 0; i < n; i++) { f->p[i] = i; f->q[i] = i; }
+        for (int64_t i = 0; i < n && rc == ORC_OK; i++) {
+            const int64_t nz = Rp[i + 1] - Rp[i];
+            M[i].c = (int64_t *)malloc((size_t)(nz > 0 ? nz : 1) * sizeof(int64_t));
+            M[i].v = malloc((size_t)(nz > 0 ? nz : 1) * vs);
+            if (!M[i].c || !M[i].v) { rc = ORC_NOMEM; break; }
+            M[i].cap = nz > 0 ? nz : 1;
+            M[i].nz = nz;
+            memcpy(M[i].c, Ri + Rp[i], (size_t)nz * sizeof(int64_t));
+            memcpy(M[i].v, (const char *)Rx + (size_t)Rp[i] * vs, (size_t)nz * vs);
+        }
+    }
+    if (rc == ORC_OK)
+        rc = cplx ? row_kernel_z(M, Lm, n, Rp, (const double complex *)Rx, drop_tol, strategy, pivot_rel, f, err, errlen)
+                  : row_kernel_d(M, Lm, n, Rp, Rx, drop_tol, strategy, pivot_rel, f, err, errlen);
+    if (rc == ORC_NOMEM) orc_msg(err, errlen, "out of memory");
+    for (int64_t i = 0; M && Lm && i < n; i++) { free(M[i].c); free(M[i].v); free(Lm[i].c); free(Lm[i].v); }
+    free(M); free(Lm);
+    if (rc != ORC_OK) {
+        if (f && rc == ORC_SINGULAR) { f->Lp = f->Li = f->Up = f->Ui = NULL; f->Lx = f->Ux = NULL; *out = f; return rc; }
+        orc_rowlu_free(f);
+        return rc;
+    }
+    *out = f;
+    return ORC_OK;
+}
+
 /* the rows a search reads from supernode s: the open one's rows past its pivot rows, a closed one's list (only its
  * pivotal part once pruned) (src/factor_sn.c:97-108) */
 static inline const int64_t *orc_srch_rows(const OWork *W, int64_t s, int64_t open, int64_t *end)

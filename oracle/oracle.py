@@ -38,6 +38,12 @@ class OrcNum(C.Structure):
     _fields_ = [("lu", C.POINTER(OrcLU)), ("p", _I64P), ("q", _I64P), ("Dr", _DP), ("Dc", _DP), ("ruiz_iters", C.c_int)]
 
 
+class OrcRowLU(C.Structure):
+    _fields_ = [("n", C.c_int64), ("cplx", C.c_int32), ("pad", C.c_int32), ("Lp", _I64P), ("Li", _I64P), ("Lx", _DP),
+                ("Up", _I64P), ("Ui", _I64P), ("Ux", _DP), ("p", _I64P), ("q", _I64P), ("n_replaced", C.c_int64),
+                ("fail_step", C.c_int64)]
+
+
 _lib = None
 
 
@@ -63,6 +69,9 @@ def lib():
         L.orc_backward_error.restype = C.c_double
         L.orc_plan_nsupernodes.argtypes = [C.c_int64, _I64P, _I64P, _I64P, _I64P]
         L.orc_plan_nsupernodes.restype = C.c_int64
+        L.orc_factor_row.argtypes = [C.c_int64, _I64P, _I64P, _DP, C.c_int, C.c_double, C.c_int, C.c_double,
+                                     C.POINTER(C.POINTER(OrcRowLU)), C.c_char_p, C.c_size_t]
+        L.orc_rowlu_free.argtypes = [C.POINTER(OrcRowLU)]
         _lib = L
     return _lib
 
@@ -215,6 +224,48 @@ def backward_error(n, Ap, Ai, Ax, x, b, trans=0):
     return lib().orc_backward_error(n, _p64(Ap), _p64(Ai), _pd(_f64view(Ax)), cplx, _pd(_f64view(x)), _pd(_f64view(b)), trans)
 
 
+def _row_factors(n, cplx, Lp, Li, Lx, Up, Ui, Ux, p, q, nrep):
+    """CSR factors out of C arrays (copied)"""
+    dt = np.complex128 if cplx else np.float64
+    vs = 2 if cplx else 1
+
+    def arr(ptr, cnt):
+        return np.ctypeslib.as_array(ptr, shape=(cnt,)).copy() if cnt > 0 else np.zeros(0, np.int64)
+
+    def vals(ptr, cnt):
+        if cnt == 0:
+            return np.zeros(0, dt)
+        return np.ctypeslib.as_array(C.cast(ptr, _DP), shape=(cnt * vs,)).copy().view(dt)
+
+    Lp_, Up_ = arr(Lp, n + 1), arr(Up, n + 1)
+    return dict(n=n, Lp=Lp_, Li=arr(Li, int(Lp_[n])), Lx=vals(Lx, int(Lp_[n])), Up=Up_, Ui=arr(Ui, int(Up_[n])),
+                Ux=vals(Ux, int(Up_[n])), p=arr(p, n), q=arr(q, n), n_replaced=int(nrep))
+
+
+def factor_row(n, Rp, Ri, Rx, drop_tol=0.0, pivot_strategy=1, pivot_rel=1.4901161193847656e-08):
+    """the oracle's row kernel on a CSR matrix: dict of CSR factors (L unit diagonal first, U from the diagonal), p, q,
+    n_replaced; OracleError (code, message; .fail_step on a singular matrix) otherwise"""
+    Rp = np.ascontiguousarray(Rp, np.int64)
+    Ri = np.ascontiguousarray(Ri, np.int64)
+    Rx = np.ascontiguousarray(Rx)
+    cplx = int(np.iscomplexobj(Rx))
+    f = C.POINTER(OrcRowLU)()
+    err = C.create_string_buffer(256)
+    rc = lib().orc_factor_row(n, _p64(Rp), _p64(Ri), _pd(_f64view(Rx)), cplx, drop_tol, pivot_strategy, pivot_rel,
+                              C.byref(f), err, 256)
+    if rc != 0:
+        e = OracleError(rc, err.value.decode())
+        if f:
+            e.fail_step = f.contents.fail_step
+            lib().orc_rowlu_free(f)
+        raise e
+    try:
+        F = f.contents
+        return _row_factors(n, cplx, F.Lp, F.Li, F.Lx, F.Up, F.Ui, F.Ux, F.p, F.q, F.n_replaced)
+    finally:
+        lib().orc_rowlu_free(f)
+
+
 # ---------------------------------------------------------------- the reference's own numeric sources (oracle/_ref)
 class RefFactors(C.Structure):
     """fplus_factors of the reference header (include/fplus.h:37-46)"""
@@ -245,6 +296,8 @@ def ref():
         L.fplus_solve_plan_nsupernodes.argtypes = [C.c_void_p]
         L.fplus_solve_plan_nsupernodes.restype = C.c_int64
         L.fplus_solve_plan_free.argtypes = [C.c_void_p]
+        L.fplus_factor_row.argtypes = [C.c_int64, _I64P, _I64P, C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int,
+                                       C.c_double, C.POINTER(RefFactors), C.c_char_p, C.c_size_t]
         _ref = L
     return _ref
 
@@ -341,3 +394,21 @@ def cpu_panel_factor_solve(n, Ap, Ai, Ax, row_perm, col_perm, b, relax_rows=32, 
     info = {"prep_s": out[0], "factor_s": out[1], "solve_s": out[2], "flops": out[3], "supernodes": int(out[4]),
             "threads": int(out[5])}
     return (X[:, 0] if np.asarray(b).ndim == 1 else X), info
+
+
+def ref_factor_row(n, Rp, Ri, Rx, drop_tol=0.0, pivot_strategy=1, pivot_rel=1.4901161193847656e-08):
+    """the reference's own fplus_factor_row (oracle/_ref, built without OpenMP or a CBLAS) on a CSR matrix"""
+    Rp = np.ascontiguousarray(Rp, np.int64)
+    Ri = np.ascontiguousarray(Ri, np.int64)
+    Rx = np.ascontiguousarray(Rx)
+    cplx = int(np.iscomplexobj(Rx))
+    f = RefFactors()
+    err = C.create_string_buffer(256)
+    rc = ref().fplus_factor_row(n, _p64(Rp), _p64(Ri), Rx.ctypes.data, drop_tol, cplx, 0, pivot_strategy, pivot_rel,
+                                C.byref(f), err, 256)
+    if rc != 0:
+        raise OracleError(rc, err.value.decode())
+    try:
+        return _row_factors(n, cplx, f.Lp, f.Li, f.Lx, f.Up, f.Ui, f.Ux, f.p, f.q, f.n_pivots_replaced)
+    finally:
+        ref().fplus_factors_free(C.byref(f))
