@@ -17,7 +17,7 @@ from ._lib import (FPLUS_ORDER_AMD_ATA, FPLUS_ORDER_AUTO, FPLUS_ORDER_COLAMD, FP
 
 __all__ = ["SparseLU", "sparse_lu_full_pivot", "lu_solve", "lu_residual", "clear_ordering_cache", "Symbolic", "Numeric",
            "SolvePlan", "default_options", "amd_options", "FplusError", "ruiz", "factor_sn", "factor_sn_perm", "factor_sn_sym",
-           "tri_solve", "tune", "Selector", "log_blocks"]
+           "factor_row", "factor_row_sym", "tri_solve", "tune", "Selector", "log_blocks"]
 
 _I64P = C.POINTER(C.c_int64)
 _DP = C.POINTER(C.c_double)
@@ -236,7 +236,8 @@ class Numeric:
         p = np.ctypeslib.as_array(f.p, (n,)).copy()
         L = sp.csc_matrix((Lx, Li, Lp), shape=(n, n))
         U = sp.csc_matrix((Ux, Ui, Up), shape=(n, n))
-        return dict(L=L, U=U, p=p, nsupernodes=f.nsupernodes, Lp=Lp, Li=Li, Lx=Lx, Up=Up, Ui=Ui, Ux=Ux)
+        return dict(L=L, U=U, p=p, nsupernodes=f.nsupernodes, Lp=Lp, Li=Li, Lx=Lx, Up=Up, Ui=Ui, Ux=Ux,
+                    n_pivots_replaced=f.n_pivots_replaced)
 
     def __del__(self):
         if getattr(self, "ptr", None):
@@ -311,9 +312,50 @@ def _factors_dict(f, n, cplx):
                     Up=np.ctypeslib.as_array(f.Up, (n + 1,)).copy(),
                     Ui=np.ctypeslib.as_array(f.Ui, (max(nU, 1),))[:nU].copy(),
                     Ux=np.ctypeslib.as_array(C.cast(f.Ux, _DP), (max(nU * vs, 1),))[:nU * vs].copy().view(dt),
-                    p=np.ctypeslib.as_array(f.p, (n,)).copy())
+                    p=np.ctypeslib.as_array(f.p, (n,)).copy(),
+                    q=np.ctypeslib.as_array(f.q, (n,)).copy() if f.q else np.arange(n, dtype=np.int64),
+                    csc=int(f.csc), n_pivots_replaced=int(f.n_pivots_replaced))
     finally:
         lib().fplus_factors_free(C.byref(f))
+
+
+_PIVOT_CODES = {"full": 0, "partial": 1, "rook": 2, 0: 0, 1: 1, 2: 2}
+
+
+def factor_row(n, Rp, Ri, Rx, drop_tol=0.0, pivot_strategy="full", pivot_rel=FPLUS_PIVOT_REL_DEFAULT, verbose=0):
+    """fplus_factor_row (the reference's _sparse_lu_c.lu_factor): the row kernel on the device on a canonical CSR matrix,
+    full / partial / rook pivoting, drop_tol > 0 the incomplete factorization; CSR factors (L unit diagonal first, U
+    from the diagonal on), p, q and n_pivots_replaced"""
+    Rx = np.ascontiguousarray(Rx)
+    cplx = int(np.iscomplexobj(Rx))
+    f = Factors()
+    err = C.create_string_buffer(512)
+    rc = lib().fplus_factor_row(n, _p64(np.ascontiguousarray(Rp, np.int64)), _p64(np.ascontiguousarray(Ri, np.int64)), _vp(Rx),
+                                float(drop_tol), cplx, int(verbose), _PIVOT_CODES[pivot_strategy], float(pivot_rel), C.byref(f),
+                                err, 512)
+    check(rc, err)
+    return _factors_dict(f, n, cplx)
+
+
+def factor_row_sym(sym, Ap, Ai, Ax, drop_tol=0.0, pivot_strategy="full", equilibrate=True, pivot_rel=FPLUS_PIVOT_REL_DEFAULT,
+                   ruiz_max_iter=20, ruiz_tol=1e-2):
+    """fplus_factor_row_sym: the orders of a Symbolic, Ruiz, the row kernel; CSC factors of the permuted, equilibrated
+    matrix, p, q (positions of that matrix), Dr, Dc (its order)"""
+    Ax = np.ascontiguousarray(Ax)
+    cplx = int(np.iscomplexobj(Ax))
+    n = sym.n
+    f = Factors()
+    Dr = np.empty(n)
+    Dc = np.empty(n)
+    err = C.create_string_buffer(512)
+    rc = lib().fplus_factor_row_sym(sym.ptr, _p64(np.ascontiguousarray(Ap, np.int64)), _p64(np.ascontiguousarray(Ai, np.int64)),
+                                    _vp(Ax), cplx, int(equilibrate), ruiz_max_iter, ruiz_tol, 0, float(drop_tol),
+                                    _PIVOT_CODES[pivot_strategy], float(pivot_rel), C.byref(f), Dr.ctypes.data_as(_DP),
+                                    Dc.ctypes.data_as(_DP), err, 512)
+    check(rc, err)
+    out = _factors_dict(f, n, cplx)
+    out.update(Dr=Dr, Dc=Dc)
+    return out
 
 
 def factor_sn(n, Ap, Ai, Ax, relax_rows=32, pivot_rel=FPLUS_PIVOT_REL_DEFAULT):

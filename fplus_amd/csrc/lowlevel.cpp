@@ -1,6 +1,6 @@
 // lowlevel.cpp -- the low-level entry points of the FPLUS interface on the device path: Ruiz on host arrays, the
 // supernodal factorization of a given CSC matrix, the permute + Ruiz + factor pipelines (reference src/equil.c:186,
-// src/factor_sn.c:609, src/pipeline.c:84-127), and the row-kernel entry points this build does not provide.
+// src/factor_sn.c:609, src/pipeline.c:84-127); the row kernel's are in rowlu.cpp.
 #include "host_common.h"
 #include "snfactor.h"
 #include "symbolic.h"
@@ -209,32 +209,6 @@ int fplus_factor_sn_sym(const fplus_symbolic* sym, const int64_t* Ap, const int6
     }
     return fplus_factor_sn_perm(n, Ap, Ai, Ax, is_complex, sym->row_perm.data(), sym->col_perm.data(), equilibrate, ruiz_max_iter,
                                 ruiz_tol, verbose, relax_rows, panel, use_gpu, gpu_min_flops, pivot_rel, out, Dr, Dc, err, errlen);
-}
-
-static int no_row_kernel(fplus_factors* out, char* err, size_t errlen)
-{
-    if (out) std::memset(out, 0, sizeof(*out));
-    errf(err, errlen, "the row kernel (rook / full pivoting, incomplete LU) is not part of the MI355X build: use the supernodal "
-         "partial-pivoting path (fplus_factor_sn*, or fplus_factor with pivot_strategy FPLUS_PIVOT_PARTIAL and drop_tol 0)");
-    return FPLUS_ERR_BADARG;
-}
-
-int fplus_factor_row(int64_t, const int64_t*, const int64_t*, const void*, double, int, int, int, double, fplus_factors* out,
-                     char* err, size_t errlen)
-{
-    return no_row_kernel(out, err, errlen);
-}
-
-int fplus_factor_row_perm(int64_t, const int64_t*, const int64_t*, const void*, int, const int64_t*, const int64_t*, int, int,
-                          double, int, double, int, double, fplus_factors* out, double*, double*, char* err, size_t errlen)
-{
-    return no_row_kernel(out, err, errlen);
-}
-
-int fplus_factor_row_sym(const fplus_symbolic*, const int64_t*, const int64_t*, const void*, int, int, int, double, int, double,
-                         int, double, fplus_factors* out, double*, double*, char* err, size_t errlen)
-{
-    return no_row_kernel(out, err, errlen);
 }
 
 }  // extern "C"

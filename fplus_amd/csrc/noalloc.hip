@@ -50,6 +50,7 @@ extern "C" int fplus_amd_solve_device_noalloc(const fplus_numeric* num, void* d_
                                               size_t errlen)
 {
     if (!num) { errf(err, errlen, "no factorization"); return FPLUS_ERR_BADARG; }
+    if (num->row) return row_handle_error(err, errlen);
     const int64_t n = num->n;
     if (!num->valid) { errf(err, errlen, "the last refactorization of this factorization failed: factor again"); return FPLUS_ERR_BADARG; }
     if (trans < 0 || trans > 2 || nrhs < 0 || ldb < n || (nrhs > 0 && n > 0 && !d_b)) {

@@ -864,11 +864,6 @@ int check_options(const fplus_options* o, char* err, size_t errlen)
         return FPLUS_ERR_BADARG;
     }
     if (!(o->drop_tol >= 0.0)) { errf(err, errlen, "drop_tol must be >= 0 (got %g)", o->drop_tol); return FPLUS_ERR_BADARG; }
-    if (o->pivot_strategy != FPLUS_PIVOT_PARTIAL || o->drop_tol != 0.0) {
-        errf(err, errlen, "rook / full pivoting and incomplete LU (drop_tol > 0) use the row kernel, which this MI355X build "
-             "does not provide: use pivot_strategy FPLUS_PIVOT_PARTIAL with drop_tol 0");
-        return FPLUS_ERR_BADARG;
-    }
     if (!(o->pivot_rel >= 0.0)) { errf(err, errlen, "pivot_rel must be >= 0"); return FPLUS_ERR_BADARG; }
     return FPLUS_OK;
 }
@@ -983,6 +978,7 @@ int fplus_analyze_opts(int64_t n, const int64_t* Ap, const int64_t* Ai, const fp
 void fplus_numeric_free(fplus_numeric* num)
 {
     if (!num) return;
+    fplus_solve_plan_free(num->rplan);
     fplus_factors_free(&num->f);
     delete num;
 }
@@ -996,6 +992,41 @@ static int check_amd_options(const fplus_amd_options* ao, char* err, size_t errl
     }
     if (ao->precision == 2 && !ao->selector) { errf(err, errlen, "precision 2 needs a selector (fplus_amd_options.selector)"); return FPLUS_ERR_BADARG; }
     if (ao->static_pivots < 0 || ao->static_pivots > 1) { errf(err, errlen, "static_pivots must be 0 or 1"); return FPLUS_ERR_BADARG; }
+    return FPLUS_OK;
+}
+
+// rook / full pivoting or an incomplete factorization: the row kernel on the permuted, equilibrated matrix, the
+// factors in CSC with their solve plan, the orders and scalings composed (reference src/numeric.c:61-80)
+static int factor_row_entry(const fplus_symbolic* sym, const int64_t* Ap, const int64_t* Ai, const void* Ax, int is_complex,
+                            const fplus_options* opt, fplus_numeric** out, char* err, size_t errlen)
+{
+    const int64_t n = sym->n;
+    std::unique_ptr<fplus_numeric> num(new fplus_numeric());
+    num->n = n;
+    num->is_complex = is_complex;
+    num->row = true;
+    num->opt = *opt;
+    num->rp = sym->row_perm;
+    num->cp = sym->col_perm;
+    std::vector<double> Drp(n > 0 ? n : 1), Dcp(n > 0 ? n : 1);
+    const double t0 = now_s();
+    int rc = fplus_factor_row_sym(sym, Ap, Ai, Ax, is_complex, opt->equilibrate, opt->ruiz_max_iter, opt->ruiz_tol, opt->verbose,
+                                  opt->drop_tol, opt->pivot_strategy, opt->pivot_rel, &num->f, Drp.data(), Dcp.data(), err, errlen);
+    if (rc != FPLUS_OK) return rc;
+    num->exported = true;
+    const int64_t* rp = sym->row_perm.data();
+    const int64_t* cp = sym->col_perm.data();
+    num->p.resize(n); num->q.resize(n); num->Dr.resize(n); num->Dc.resize(n);
+    for (int64_t k2 = 0; k2 < n; k2++) {
+        num->p[k2] = rp[num->f.p[k2]];   // p = row_perm[p_lu]
+        num->q[k2] = cp[num->f.q[k2]];   // q = col_perm[q_lu]
+        num->Dr[rp[k2]] = Drp[k2];
+        num->Dc[cp[k2]] = Dcp[k2];
+    }
+    rc = fplus_solve_plan_create(n, num->f.Lp, num->f.Li, num->f.Up, num->f.Ui, &num->rplan, err, errlen);
+    if (rc != FPLUS_OK) return rc;
+    num->t_factor = num->t_total = now_s() - t0;
+    *out = num.release();
     return FPLUS_OK;
 }
 
@@ -1015,6 +1046,21 @@ static int factor_entry(const fplus_symbolic* sym, const int64_t* Ap, const int6
     if (Ap[n] != sym->nnz) {
         errf(err, errlen, "the matrix has %ld nonzeros, the symbolic analysis %ld", (long)Ap[n], (long)sym->nnz);
         return FPLUS_ERR_BADARG;
+    }
+    if (opt->pivot_strategy != FPLUS_PIVOT_PARTIAL || opt->drop_tol > 0.0) {
+        if (device_values || ao->refactor || ao->precision != 0) {
+            errf(err, errlen, "rook / full pivoting and drop_tol > 0 run the row kernel on host values, without the device "
+                              "factorization's extensions (refactor, precision, device-resident values)");
+            return FPLUS_ERR_BADARG;
+        }
+        try {
+            return factor_row_entry(sym, Ap, Ai, Ax, is_complex, opt, out, err, errlen);
+        } catch (const GpuError& e) {
+            return gpu_error(e, err, errlen);
+        } catch (const std::bad_alloc&) {
+            errf(err, errlen, "out of memory");
+            return FPLUS_ERR_NOMEM;
+        }
     }
     rc = check_compressed(n, Ap, Ai, true, err, errlen);
     if (rc == FPLUS_OK && !device_values) rc = check_finite(Ap[n], Ax, is_complex, err, errlen);
@@ -1100,12 +1146,57 @@ int fplus_factor(const fplus_symbolic* sym, const int64_t* Ap, const int64_t* Ai
     return factor_entry(sym, Ap, Ai, Ax, false, is_complex, opt, nullptr, out, err, errlen);
 }
 
+// a row-kernel factorization's solve: the right-hand sides scaled and permuted in groups of 32, the CSC factors' sweeps
+// on the device through the plan, the result scattered back (reference src/numeric.c:85-131)
+static int solve_row(const fplus_numeric* num, void* b, int64_t nrhs, int64_t ldb, int trans, char* err, size_t errlen)
+{
+    const int64_t n = num->n;
+    const int vs = num->is_complex ? 2 : 1;
+    const fplus_factors* f = &num->f;
+    const int64_t* p = num->p.data();
+    const int64_t* q = num->q.data();
+    const double* Dr = num->Dr.data();
+    const double* Dc = num->Dc.data();
+    const int64_t g = std::min<int64_t>(nrhs, 32);
+    std::vector<double> w((size_t)n * g * vs);
+    double* x = static_cast<double*>(b);
+    int rc = FPLUS_OK;
+    std::lock_guard<std::mutex> lk(num->solve_mu);
+    for (int64_t r0 = 0; r0 < nrhs && rc == FPLUS_OK; r0 += g) {
+        const int64_t m = std::min(g, nrhs - r0);
+        for (int64_t r = 0; r < m; r++) {
+            const double* xb = x + (size_t)((r0 + r) * ldb) * vs;
+            double* ww = w.data() + (size_t)(r * n) * vs;
+            for (int64_t k2 = 0; k2 < n; k2++) {
+                const int64_t src = trans == 0 ? p[k2] : q[k2];
+                const double d = trans == 0 ? Dr[src] : Dc[src];
+                for (int v = 0; v < vs; v++) ww[k2 * vs + v] = d * xb[src * vs + v];
+            }
+        }
+        rc = fplus_tri_solve_plan(num->rplan, n, f->Lp, f->Li, f->Lx, f->Up, f->Ui, f->Ux, w.data(), m, n, trans, num->is_complex);
+        if (rc != FPLUS_OK) break;
+        for (int64_t r = 0; r < m; r++) {
+            double* xb = x + (size_t)((r0 + r) * ldb) * vs;
+            const double* ww = w.data() + (size_t)(r * n) * vs;
+            for (int64_t k2 = 0; k2 < n; k2++) {
+                const int64_t dst = trans == 0 ? q[k2] : p[k2];
+                const double d = trans == 0 ? Dc[dst] : Dr[dst];
+                for (int v = 0; v < vs; v++) xb[dst * vs + v] = d * ww[k2 * vs + v];
+            }
+        }
+    }
+    if (rc == FPLUS_ERR_SINGULAR) errf(err, errlen, "fplus_solve: zero or missing diagonal in U");
+    else if (rc != FPLUS_OK) errf(err, errlen, "fplus_solve: error %d", rc);
+    return rc;
+}
+
 int fplus_solve(const fplus_numeric* num, void* b, int64_t nrhs, int64_t ldb, int trans, char* err, size_t errlen)
 {
     const int64_t n = num->n;
     if (!num->valid) { errf(err, errlen, "the last refactorization of this factorization failed: factor again"); return FPLUS_ERR_BADARG; }
     if (trans < 0 || trans > 2 || nrhs < 0 || ldb < n) { errf(err, errlen, "fplus_solve: bad trans, nrhs or ldb"); return FPLUS_ERR_BADARG; }
     if (nrhs == 0 || n == 0) return FPLUS_OK;
+    if (num->row) return solve_row(num, b, nrhs, ldb, trans, err, errlen);
     const int vs = num->F.vs;
     const int tr = vs == 1 ? (trans ? 1 : 0) : trans;
     try {
@@ -1160,7 +1251,7 @@ void export_factors_entry(const DevFactor& F, int is_complex, fplus_factors* f) 
 
 extern "C" int fplus_amd_numeric_stats(const fplus_numeric* num, fplus_amd_stats* st)
 {
-    if (!num || !st) return FPLUS_ERR_BADARG;
+    if (!num || !st || num->row) return FPLUS_ERR_BADARG;
     const DevFactor& F = num->F;
     std::memset(st, 0, sizeof(*st));
     for (int64_t s = 0; s < F.nsn; s++) {
@@ -1230,6 +1321,7 @@ extern "C" int fplus_amd_refactor(fplus_numeric* num, const void* Ax, int ax_on_
                                   size_t errlen)
 {
     if (!num) { errf(err, errlen, "no factorization"); return FPLUS_ERR_BADARG; }
+    if (num->row) return row_handle_error(err, errlen);
     if (!num->F.replay) {
         errf(err, errlen, "this factorization was made without fplus_amd_options.refactor = 1: nothing to replay");
         return FPLUS_ERR_BADARG;
@@ -1369,6 +1461,7 @@ int fplus_amd_factor_device(const fplus_symbolic* sym, const int64_t* Ap, const 
 
 int fplus_amd_solve_device(const fplus_numeric* num, void* d_b, int64_t nrhs, int64_t ldb, int trans, char* err, size_t errlen)
 {
+    if (num->row) return row_handle_error(err, errlen);
     const int64_t n = num->n;
     if (!num->valid) { errf(err, errlen, "the last refactorization of this factorization failed: factor again"); return FPLUS_ERR_BADARG; }
     if (trans < 0 || trans > 2 || nrhs < 0 || ldb < n) { errf(err, errlen, "fplus_amd_solve_device: bad trans, nrhs or ldb"); return FPLUS_ERR_BADARG; }

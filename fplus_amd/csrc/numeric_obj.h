@@ -83,6 +83,11 @@ struct fplus_numeric {
     DevBuf<int64_t> d_plu;
     mutable DevBuf<double> ir_V, ir_w, ir_bb, ir_u, ir_r, ir_part, ir_d, ir_c, ir_sc;
     mutable DevBuf<unsigned long long> ir_norm;
+    // a factorization by the row kernel (rook / full pivoting, or drop_tol > 0): its factors in f (CSC, exported from
+    // the start) and the solve plan of their patterns; everything above that belongs to the device factorization stays
+    // empty, and the extensions that need it refuse such a handle
+    bool row = false;
+    fplus_solve_plan* rplan = nullptr;
     // timings of the last factorization
     double t_ruiz = 0, t_factor = 0, t_total = 0, t_prepare = 0;
     double t_prep_part[6] = {0, 0, 0, 0, 0, 0};   // the preparation by step: L levels, U pieces, U levels, slots, tiles, uploads
@@ -90,6 +95,13 @@ struct fplus_numeric {
 
 
 namespace fpa {
+// the extensions that need the device factorization refuse a row-kernel handle with this
+inline int row_handle_error(char* err, size_t errlen)
+{
+    errf(err, errlen, "this factorization comes from the row kernel (rook / full pivoting or drop_tol > 0): fplus_solve and the "
+                      "factor accessors take it, the device-factorization extensions do not");
+    return FPLUS_ERR_BADARG;
+}
 // the selector's device model and threshold with the scalings its features read (selector.cpp)
 SelectorArgs selector_args(const fplus_amd_selector* sel, const double* d_Dr, const double* d_Dc);
 // the sweeps on w (n x R row-major, pivot order, R <= solve_rhs_per_pass) in place: trans 0 L U x = w, the result in

@@ -242,6 +242,7 @@ extern "C" void fplus_amd_default_refine_options(fplus_amd_refine_options* o)
 extern "C" int fplus_amd_solve_refined(const fplus_numeric* num, void* b, int b_on_device, int64_t nrhs, int64_t ldb,
                                        const fplus_amd_refine_options* ro, fplus_amd_refine_info* info, char* err, size_t errlen)
 {
+    if (num && num->row) return row_handle_error(err, errlen);
     fplus_amd_refine_options d;
     if (!ro) { fplus_amd_default_refine_options(&d); ro = &d; }
     fplus_amd_refine_info inf;
@@ -309,6 +310,7 @@ extern "C" int fplus_amd_refactor_solve(fplus_numeric* num, const void* Ax, int 
 {
     if (fallback) *fallback = 0;
     if (!num) { errf(err, errlen, "no factorization"); return FPLUS_ERR_BADARG; }
+    if (num->row) return row_handle_error(err, errlen);
     const int64_t n = num->n;
     if (nrhs < 0 || ldb < n) { errf(err, errlen, "fplus_amd_refactor_solve: bad nrhs or ldb"); return FPLUS_ERR_BADARG; }
     fplus_amd_refine_options d;
@@ -386,6 +388,7 @@ extern "C" int fplus_amd_refactor_sweep(fplus_numeric* num, const void* Ax, int6
                                         fplus_amd_refine_info* info, int* fallback, char* err, size_t errlen)
 {
     if (!num) { errf(err, errlen, "no factorization"); return FPLUS_ERR_BADARG; }
+    if (num->row) return row_handle_error(err, errlen);
     if (nshift < 0 || nrhs < 0 || ldb < num->n || !Ax || (nrhs > 0 && !b)) {
         errf(err, errlen, "fplus_amd_refactor_sweep: bad nshift, nrhs, ldb, Ax or b");
         return FPLUS_ERR_BADARG;

@@ -1,0 +1,386 @@
+// rowlu.cpp -- the row kernel's entry points (reference include/fplus.h:60-62, :137-147): fplus_factor_row (CSR in,
+// CSR factors out) and the pipelines fplus_factor_row_perm / _sym (orders, Ruiz, the kernel, CSC factors; reference
+// src/pipeline.c:129-225).  The elimination runs on the device (rowlu_dev.hip); the host lays out the rows, grows the
+// device pools when a run reports them full, and assembles the factors the kernel leaves in its pools.
+#include "rowlu.h"
+
+#include "device_api.h"
+#include "host_common.h"
+#include "rowlu_dev.h"
+#include "symbolic.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstdlib>
+#include <memory>
+#include <new>
+
+using namespace fpa;
+
+namespace fpa {
+// defined in numeric.cpp
+int ruiz_device_entry(int64_t n, int64_t nnz, const int32_t* d_erow, const int32_t* d_ecol, double* d_x, int vs, int max_iter,
+                      double tol, double* d_Dr, double* d_Dc, int verbose, int* iters, char* err, size_t errlen);
+
+void compressed_transpose(int64_t n, const int64_t* P, const int64_t* I, std::vector<int64_t>& TP, std::vector<int64_t>& TI,
+                          std::vector<int64_t>& map)
+{
+    const int64_t nnz = P[n];
+    TP.assign(n + 1, 0);
+    TI.resize(nnz);
+    map.resize(nnz);
+    for (int64_t q = 0; q < nnz; q++) TP[I[q] + 1]++;
+    for (int64_t i = 0; i < n; i++) TP[i + 1] += TP[i];
+    std::vector<int64_t> next(TP.begin(), TP.end() - 1);
+    for (int64_t j = 0; j < n; j++)   // lines in order: the transposed indices come out ascending
+        for (int64_t q = P[j]; q < P[j + 1]; q++) {
+            const int64_t d = next[I[q]]++;
+            TI[d] = j;
+            map[d] = q;
+        }
+}
+
+namespace {
+
+inline double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+template <class T> T* cmalloc(size_t cnt)
+{
+    T* p = static_cast<T*>(std::malloc((cnt > 0 ? cnt : 1) * sizeof(T)));
+    if (!p) throw std::bad_alloc();
+    return p;
+}
+
+void transpose_one(int64_t n, int vs, int64_t*& P, int64_t*& I, void*& X)
+{
+    std::vector<int64_t> TP, TI, map;
+    compressed_transpose(n, P, I, TP, TI, map);
+    const int64_t nnz = P[n];
+    int64_t* nP = cmalloc<int64_t>(n + 1);
+    int64_t* nI = cmalloc<int64_t>(nnz);
+    double* nX = cmalloc<double>((size_t)nnz * vs);
+    const double* x = static_cast<const double*>(X);
+    std::copy(TP.begin(), TP.end(), nP);
+    std::copy(TI.begin(), TI.end(), nI);
+    for (int64_t t = 0; t < nnz; t++)
+        for (int v = 0; v < vs; v++) nX[t * vs + v] = x[map[t] * vs + v];
+    std::free(P); std::free(I); std::free(X);
+    P = nP; I = nI; X = nX;
+}
+
+inline double absval(const double* x, int vs) { return vs == 2 ? std::hypot(x[0], x[1]) : std::fabs(x[0]); }
+
+// the device state of one run of the kernel
+struct RowRun {
+    DevBuf<int64_t> off, base, cap_end, p, q;
+    DevBuf<int32_t> nnz, lead, slot, cols, log_row, log_step, cand, scratch_col;
+    DevBuf<double> vals, log_val, scratch_val;
+    DevBuf<unsigned long long> tops;   // pool top, log top
+    DevBuf<RowLuStatus> status;
+};
+
+}  // namespace
+
+void factors_to_csc(fplus_factors* f)
+{
+    if (f->csc) return;
+    const int vs = f->is_complex ? 2 : 1;
+    transpose_one(f->n, vs, f->Lp, f->Li, f->Lx);
+    transpose_one(f->n, vs, f->Up, f->Ui, f->Ux);
+    f->csc = 1;
+}
+
+int row_factor_device(int64_t n, const int64_t* Rp, const int64_t* Ri, const double* d_Rx, int vs, double drop_tol, int strategy,
+                      double pivot_rel, int verbose, fplus_factors* out, char* err, size_t errlen)
+{
+    std::memset(out, 0, sizeof(*out));
+    if (n >= (int64_t)RL_NONE) { errf(err, errlen, "the row kernel takes n < 2^31 - 1 (got %ld)", (long)n); return FPLUS_ERR_BADARG; }
+    const int64_t nnz = Rp[n];
+    const char* sname = strategy == FPLUS_PIVOT_FULL ? "full" : strategy == FPLUS_PIVOT_PARTIAL ? "partial" : "rook";
+    if (verbose >= 1)
+        printf("[FPLUS] LU factorisation start: n=%ld, nnz(A)=%ld, %s, drop_tol=%.1e, pivot=%s (row kernel, device)\n", (long)n,
+               (long)nnz, vs == 2 ? "complex128" : "float64", drop_tol, sname);
+    hipStream_t s = devctx().stream;
+    const int32_t nn = (int32_t)n;
+    // the rows as given: row i at [Rp[i], Rp[i + 1]) of the pool, its block exactly as long
+    std::vector<int64_t> h_off(Rp, Rp + n), h_end(Rp + 1, Rp + n + 1), h_id(n);
+    std::vector<int32_t> h_nnz(n), h_lead(n), h_slot(n), h_cols(nnz);
+    for (int64_t i = 0; i < n; i++) {
+        h_nnz[i] = (int32_t)(Rp[i + 1] - Rp[i]);
+        h_lead[i] = h_nnz[i] ? (int32_t)Ri[Rp[i]] : RL_NONE;
+        h_slot[i] = (int32_t)i;
+        h_id[i] = i;
+    }
+    for (int64_t q = 0; q < nnz; q++) h_cols[q] = (int32_t)Ri[q];
+    int64_t pool_cap = 16 * nnz + 64 * n + 1024, log_cap = 4 * nnz + 2 * n + 1024;
+    RowRun R;
+    RowLuStatus st{};
+    const double t0 = now_s();
+    for (int attempt = 0;; attempt++) {
+        try {
+            R.off.alloc(n); R.base.alloc(n); R.cap_end.alloc(n); R.p.alloc(n); R.q.alloc(n);
+            R.nnz.alloc(n); R.lead.alloc(n); R.slot.alloc(n); R.cand.alloc(n);
+            R.scratch_col.alloc((size_t)16 * n);
+            R.scratch_val.alloc((size_t)16 * n * vs);
+            R.cols.alloc(pool_cap);
+            R.vals.alloc((size_t)pool_cap * vs);
+            R.log_row.alloc(log_cap); R.log_step.alloc(log_cap);
+            R.log_val.alloc((size_t)log_cap * vs);
+            R.tops.alloc(2);
+            R.status.alloc(1);
+        } catch (const GpuError&) {
+            errf(err, errlen, "out of memory (device: %ld pool entries, %ld multipliers)", (long)pool_cap, (long)log_cap);
+            return FPLUS_ERR_NOMEM;
+        }
+        R.off.upload(h_off.data(), n, s); R.base.upload(h_off.data(), n, s); R.cap_end.upload(h_end.data(), n, s);
+        R.p.upload(h_id.data(), n, s); R.q.upload(h_id.data(), n, s);
+        R.nnz.upload(h_nnz.data(), n, s); R.lead.upload(h_lead.data(), n, s); R.slot.upload(h_slot.data(), n, s);
+        R.cols.upload(h_cols.data(), nnz, s);
+        if (nnz > 0) hip_check(hipMemcpyAsync(R.vals.p, d_Rx, (size_t)nnz * vs * sizeof(double), hipMemcpyDeviceToDevice, s), "row values");
+        const unsigned long long tops[2] = {(unsigned long long)nnz, 0ull};
+        R.tops.upload(tops, 2, s);
+        RowLuArgs a{};
+        a.n = nn; a.strategy = strategy; a.drop_tol = drop_tol; a.pivot_rel = pivot_rel;
+        a.off = R.off.p; a.base = R.base.p; a.cap_end = R.cap_end.p;
+        a.nnz = R.nnz.p; a.lead = R.lead.p; a.slot = R.slot.p; a.p = R.p.p; a.q = R.q.p;
+        a.cols = R.cols.p; a.vals = R.vals.p; a.pool_cap = pool_cap; a.pool_top = R.tops.p;
+        a.log_row = R.log_row.p; a.log_step = R.log_step.p; a.log_val = R.log_val.p; a.log_cap = log_cap; a.log_top = R.tops.p + 1;
+        a.cand = R.cand.p; a.scratch_col = R.scratch_col.p; a.scratch_val = R.scratch_val.p; a.status = R.status.p;
+        if (n > 0) row_lu(vs, a, s);
+        else st = RowLuStatus{};
+        if (n > 0) R.status.download(&st, 1, s);
+        dev_sync(s);
+        if (st.code == RL_POOL && attempt < 12) { pool_cap *= 2; continue; }
+        if (st.code == RL_LOG && attempt < 12) { log_cap *= 2; continue; }
+        break;
+    }
+    if (st.code == RL_SINGULAR) {
+        errf(err, errlen, "Near-zero or zero pivot %.6e at step %ld (threshold %.6e). Matrix is singular.", st.best, (long)st.step, st.thr);
+        return FPLUS_ERR_SINGULAR;
+    }
+    if (st.code == RL_POOL || st.code == RL_LOG) { errf(err, errlen, "out of memory (the row kernel's pools)"); return FPLUS_ERR_NOMEM; }
+    if (st.code != RL_OK) {
+        errf(err, errlen, "Pivot entry missing at step %ld after swap.", (long)st.step);
+        return FPLUS_ERR_INTERNAL;
+    }
+    const double t1 = now_s();
+    // the factors out of the pools: U row i is the row at position i; L row i its logged multipliers in step order
+    std::vector<int64_t> d_off(n), p(n), q(n);
+    std::vector<int32_t> d_nnz(n), slot(n);
+    std::vector<int32_t> pc(st.pool_used > 0 ? st.pool_used : 0), lr(st.log_used), ls(st.log_used);
+    std::vector<double> pv((size_t)std::max<int64_t>(st.pool_used, 0) * vs), lv((size_t)st.log_used * vs);
+    if (n > 0) {
+        R.off.download(d_off.data(), n, s); R.nnz.download(d_nnz.data(), n, s); R.slot.download(slot.data(), n, s);
+        R.p.download(p.data(), n, s); R.q.download(q.data(), n, s);
+        R.cols.download(pc.data(), pc.size(), s); R.vals.download(pv.data(), pv.size(), s);
+        R.log_row.download(lr.data(), lr.size(), s); R.log_step.download(ls.data(), ls.size(), s);
+        R.log_val.download(lv.data(), lv.size(), s);
+        dev_sync(s);
+    }
+    std::vector<int64_t> pos(n);
+    for (int64_t i = 0; i < n; i++) pos[slot[i]] = i;
+    auto dropped = [&](const double* v, int64_t j, int64_t i) { return drop_tol > 0.0 && j != i && absval(v, vs) <= drop_tol; };
+    try {
+        out->n = n;
+        out->is_complex = vs == 2;
+        out->csc = 0;
+        // U (src/factor_row.c:1081-1133 with upper = 1)
+        out->Up = cmalloc<int64_t>(n + 1);
+        out->Up[0] = 0;
+        for (int64_t i = 0; i < n; i++) {
+            const int32_t r = slot[i];
+            int64_t c = 0;
+            for (int64_t t = d_off[r]; t < d_off[r] + d_nnz[r]; t++)
+                if (!dropped(&pv[t * vs], pc[t], i) && pc[t] >= i) c++;
+            out->Up[i + 1] = out->Up[i] + c;
+        }
+        out->Ui = cmalloc<int64_t>(out->Up[n]);
+        double* Ux = cmalloc<double>((size_t)out->Up[n] * vs);
+        out->Ux = Ux;
+        for (int64_t i = 0, o = 0; i < n; i++) {
+            const int32_t r = slot[i];
+            for (int64_t t = d_off[r]; t < d_off[r] + d_nnz[r]; t++) {
+                if (dropped(&pv[t * vs], pc[t], i) || pc[t] < i) continue;
+                out->Ui[o] = pc[t];
+                for (int v = 0; v < vs; v++) Ux[o * vs + v] = pv[t * vs + v];
+                o++;
+            }
+        }
+        // L (upper = 0): the log grouped by row, stably, so every row's multipliers stay in step order
+        std::vector<int64_t> lcnt(n + 1, 0);
+        for (int64_t e = 0; e < st.log_used; e++)
+            if (!dropped(&lv[e * vs], ls[e], pos[lr[e]])) lcnt[pos[lr[e]] + 1]++;
+        out->Lp = cmalloc<int64_t>(n + 1);
+        out->Lp[0] = 0;
+        for (int64_t i = 0; i < n; i++) out->Lp[i + 1] = out->Lp[i] + 1 + lcnt[i + 1];
+        out->Li = cmalloc<int64_t>(out->Lp[n]);
+        double* Lx = cmalloc<double>((size_t)out->Lp[n] * vs);
+        out->Lx = Lx;
+        std::vector<int64_t> nxt(n);
+        for (int64_t i = 0; i < n; i++) {
+            const int64_t o = out->Lp[i];
+            out->Li[o] = i;
+            Lx[o * vs] = 1.0;
+            if (vs == 2) Lx[o * vs + 1] = 0.0;
+            nxt[i] = o + 1;
+        }
+        for (int64_t e = 0; e < st.log_used; e++) {
+            const int64_t i = pos[lr[e]];
+            if (dropped(&lv[e * vs], ls[e], i)) continue;
+            const int64_t o = nxt[i]++;
+            out->Li[o] = ls[e];
+            for (int v = 0; v < vs; v++) Lx[o * vs + v] = lv[e * vs + v];
+        }
+        out->p = cmalloc<int64_t>(n);
+        out->q = cmalloc<int64_t>(n);
+        std::copy(p.begin(), p.end(), out->p);
+        std::copy(q.begin(), q.end(), out->q);
+        out->nsupernodes = 0;
+        out->n_pivots_replaced = st.nrep;
+    } catch (const std::bad_alloc&) {
+        fplus_factors_free(out);
+        errf(err, errlen, "out of memory");
+        return FPLUS_ERR_NOMEM;
+    }
+    if (verbose >= 1) {
+        printf("[FPLUS] LU factorisation complete: nnz(L)=%ld, nnz(U)=%ld, total nnz=%ld, time=%.3fs (device elimination %.3fs, "
+               "pool %ld of %ld entries, %ld multipliers)\n", (long)out->Lp[n], (long)out->Up[n], (long)(out->Lp[n] + out->Up[n]),
+               now_s() - t0, t1 - t0, (long)st.pool_used, (long)pool_cap, (long)st.log_used);
+        if (st.nrep > 0)
+            printf("[FPLUS] ILU: %ld zero / negligible pivots replaced by drop_tol * max|a_ij| = %.3e\n", (long)st.nrep, drop_tol * st.max0);
+    }
+    return FPLUS_OK;
+}
+
+}  // namespace fpa
+
+namespace {
+
+int gpu_error(const std::exception& e, char* err, size_t errlen)
+{
+    errf(err, errlen, "GPU: %s", e.what());
+    return FPLUS_ERR_GPU;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fplus_factor_row(int64_t n, const int64_t* indptr, const int64_t* indices, const void* data, double drop_tol, int is_complex,
+                     int verbose, int pivot_strategy, double pivot_rel, fplus_factors* out, char* err, size_t errlen)
+{
+    std::memset(out, 0, sizeof(*out));
+    if (pivot_strategy != FPLUS_PIVOT_FULL && pivot_strategy != FPLUS_PIVOT_PARTIAL && pivot_strategy != FPLUS_PIVOT_ROOK) {
+        errf(err, errlen, "pivot_strategy must be 0 (full), 1 (partial), or 2 (rook); got %d", pivot_strategy);
+        return FPLUS_ERR_BADARG;
+    }
+    if (!(pivot_rel >= 0.0)) { errf(err, errlen, "pivot_rel must be >= 0"); return FPLUS_ERR_BADARG; }
+    if (!(drop_tol >= 0.0)) { errf(err, errlen, "drop_tol must be >= 0 (got %g)", drop_tol); return FPLUS_ERR_BADARG; }
+    int rc = check_compressed(n, indptr, indices, true, err, errlen);   // the row kernel needs sorted, unique rows
+    if (rc == FPLUS_OK) rc = check_finite(indptr[n], data, is_complex, err, errlen);
+    if (rc != FPLUS_OK) return rc;
+    const int vs = is_complex ? 2 : 1;
+    try {
+        hipStream_t s = devctx().stream;
+        const int64_t nnz = indptr[n];
+        DevBuf<double> d_x((size_t)(nnz > 0 ? nnz : 1) * vs);
+        d_x.upload(static_cast<const double*>(data), (size_t)nnz * vs, s);
+        return row_factor_device(n, indptr, indices, d_x.p, vs, drop_tol, pivot_strategy, pivot_rel, verbose, out, err, errlen);
+    } catch (const GpuError& e) {
+        return gpu_error(e, err, errlen);
+    } catch (const std::bad_alloc&) {
+        errf(err, errlen, "out of memory");
+        return FPLUS_ERR_NOMEM;
+    }
+}
+
+int fplus_factor_row_perm(int64_t n, const int64_t* Ap, const int64_t* Ai, const void* Ax, int is_complex, const int64_t* row_perm,
+                          const int64_t* col_perm, int equilibrate, int ruiz_max_iter, double ruiz_tol, int verbose,
+                          double drop_tol, int pivot_strategy, double pivot_rel, fplus_factors* out, double* Dr, double* Dc,
+                          char* err, size_t errlen)
+{
+    std::memset(out, 0, sizeof(*out));
+    int rc = check_compressed(n, Ap, Ai, true, err, errlen);
+    if (rc == FPLUS_OK) rc = check_perm(n, row_perm, "row_perm", err, errlen);
+    if (rc == FPLUS_OK) rc = check_perm(n, col_perm, "col_perm", err, errlen);
+    if (rc == FPLUS_OK) rc = check_finite(Ap[n], Ax, is_complex, err, errlen);
+    if (rc != FPLUS_OK) return rc;
+    if (pivot_strategy != FPLUS_PIVOT_FULL && pivot_strategy != FPLUS_PIVOT_PARTIAL && pivot_strategy != FPLUS_PIVOT_ROOK) {
+        errf(err, errlen, "pivot_strategy must be 0 (full), 1 (partial), or 2 (rook); got %d", pivot_strategy);
+        return FPLUS_ERR_BADARG;
+    }
+    if (!(pivot_rel >= 0.0)) { errf(err, errlen, "pivot_rel must be >= 0"); return FPLUS_ERR_BADARG; }
+    if (!(drop_tol >= 0.0)) { errf(err, errlen, "drop_tol must be >= 0 (got %g)", drop_tol); return FPLUS_ERR_BADARG; }
+    const int vs = is_complex ? 2 : 1;
+    try {
+        hipStream_t s = devctx().stream;
+        const int64_t nnz = Ap[n];
+        // B = A[row_perm, :][:, col_perm], canonical CSC (src/pipeline.c:14-82)
+        std::vector<int64_t> inv(n), Bp(n + 1, 0), Bi(nnz), src(nnz);
+        for (int64_t k = 0; k < n; k++) inv[row_perm[k]] = k;
+        for (int64_t k = 0; k < n; k++) Bp[k + 1] = Bp[k] + Ap[col_perm[k] + 1] - Ap[col_perm[k]];
+        std::vector<std::pair<int64_t, int64_t>> col;
+        for (int64_t k = 0; k < n; k++) {
+            const int64_t c = col_perm[k];
+            col.clear();
+            for (int64_t q = Ap[c]; q < Ap[c + 1]; q++) col.push_back({inv[Ai[q]], q});
+            std::sort(col.begin(), col.end());
+            for (size_t t = 0; t < col.size(); t++) { Bi[Bp[k] + t] = col[t].first; src[Bp[k] + t] = col[t].second; }
+        }
+        // the values gathered on the device, equilibrated there (CSC route), then laid out by rows
+        std::vector<int32_t> erow(nnz), ecol(nnz);
+        for (int64_t k = 0; k < n; k++)
+            for (int64_t q = Bp[k]; q < Bp[k + 1]; q++) { erow[q] = (int32_t)Bi[q]; ecol[q] = (int32_t)k; }
+        DevBuf<double> d_A((size_t)(nnz > 0 ? nnz : 1) * vs), d_B((size_t)(nnz > 0 ? nnz : 1) * vs), d_R((size_t)(nnz > 0 ? nnz : 1) * vs);
+        DevBuf<int64_t> d_map(nnz > 0 ? nnz : 1);
+        DevBuf<int32_t> d_er(nnz > 0 ? nnz : 1), d_ec(nnz > 0 ? nnz : 1);
+        d_A.upload(static_cast<const double*>(Ax), (size_t)nnz * vs, s);
+        d_map.upload(src.data(), nnz, s);
+        k::gather_vals(vs, nnz, d_map.p, d_A.p, d_B.p, s);
+        d_er.upload(erow.data(), nnz, s);
+        d_ec.upload(ecol.data(), nnz, s);
+        if (equilibrate) {
+            DevBuf<double> d_Dr(n > 0 ? n : 1), d_Dc(n > 0 ? n : 1);
+            rc = ruiz_device_entry(n, nnz, d_er.p, d_ec.p, d_B.p, vs, ruiz_max_iter, ruiz_tol, d_Dr.p, d_Dc.p, verbose, nullptr, err, errlen);
+            if (rc != FPLUS_OK) return rc;
+            d_Dr.download(Dr, n, s);
+            d_Dc.download(Dc, n, s);
+        } else {
+            for (int64_t i = 0; i < n; i++) { Dr[i] = 1.0; Dc[i] = 1.0; }
+        }
+        std::vector<int64_t> Rp, Ri, rmap;
+        compressed_transpose(n, Bp.data(), Bi.data(), Rp, Ri, rmap);
+        d_map.upload(rmap.data(), nnz, s);
+        k::gather_vals(vs, nnz, d_map.p, d_B.p, d_R.p, s);
+        rc = row_factor_device(n, Rp.data(), Ri.data(), d_R.p, vs, drop_tol, pivot_strategy, pivot_rel, verbose, out, err, errlen);
+        if (rc != FPLUS_OK) return rc;
+        factors_to_csc(out);
+        return FPLUS_OK;
+    } catch (const GpuError& e) {
+        fplus_factors_free(out);
+        return gpu_error(e, err, errlen);
+    } catch (const std::bad_alloc&) {
+        fplus_factors_free(out);
+        errf(err, errlen, "out of memory");
+        return FPLUS_ERR_NOMEM;
+    }
+}
+
+int fplus_factor_row_sym(const fplus_symbolic* sym, const int64_t* Ap, const int64_t* Ai, const void* Ax, int is_complex,
+                         int equilibrate, int ruiz_max_iter, double ruiz_tol, int verbose, double drop_tol, int pivot_strategy,
+                         double pivot_rel, fplus_factors* out, double* Dr, double* Dc, char* err, size_t errlen)
+{
+    const int64_t n = sym->n;
+    if (Ap[n] != sym->nnz) {
+        std::memset(out, 0, sizeof(*out));
+        errf(err, errlen, "the matrix has %ld nonzeros, the symbolic analysis %ld", (long)Ap[n], (long)sym->nnz);
+        return FPLUS_ERR_BADARG;
+    }
+    return fplus_factor_row_perm(n, Ap, Ai, Ax, is_complex, sym->row_perm.data(), sym->col_perm.data(), equilibrate, ruiz_max_iter,
+                                 ruiz_tol, verbose, drop_tol, pivot_strategy, pivot_rel, out, Dr, Dc, err, errlen);
+}
+
+}  // extern "C"

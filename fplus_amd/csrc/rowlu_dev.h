@@ -1,0 +1,54 @@
+// rowlu_dev.h -- the row kernel on the device: right-looking elimination on sparse rows with full, partial or rook
+// pivoting and the incomplete-LU drop rules (reference src/factor_row.c:1191-1560), one workgroup per system.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace fpa {
+
+constexpr int32_t RL_NONE = 0x7fffffff;   // the leading column of an empty row
+
+// what the kernel reports back
+struct RowLuStatus {
+    int32_t code;        // RL_OK, RL_SINGULAR, RL_POOL (entry pool full), RL_LOG (multiplier log full), RL_INTERNAL
+    int32_t pad;
+    int64_t step;        // the step that stopped the elimination
+    int64_t nrep;        // pivots replaced at incomplete-LU breakdowns
+    double best, thr;    // the failing step's best pivot magnitude and the threshold
+    double max0;         // max |a_ij| of the input (and the first pivot)
+    int64_t pool_used, log_used;
+};
+enum { RL_OK = 0, RL_SINGULAR = 1, RL_POOL = 2, RL_LOG = 3, RL_INTERNAL = 4 };
+
+// The device state of one system.  Rows are addressed by a physical id r (the input row r at the start); slot[i] is
+// the physical row at position i, p[i] / q[i] the input row / column at position i.  Row r's entries, columns
+// ascending, sit at cols/vals[off[r] .. off[r] + nnz[r]) inside its block [base[r], cap_end[r]) of the entry pool;
+// lead[r] is its first column (RL_NONE when empty).  Blocks that outgrow their room move to fresh space at the pool's
+// top (pool_top).  Each multiplier m(i, k) is logged as (physical row, step, value) at log_top.  scratch: n entries
+// per wavefront (the union of two rows before its drops).  vals / log_val / scratch_val hold T (double or complex).
+struct RowLuArgs {
+    int32_t n;
+    int32_t strategy;                     // 0 full, 1 partial, 2 rook
+    double drop_tol, pivot_rel;
+    int64_t *off, *base, *cap_end;
+    int32_t *nnz, *lead, *slot;
+    int64_t *p, *q;
+    int32_t* cols;
+    double* vals;
+    int64_t pool_cap;
+    unsigned long long* pool_top;
+    int32_t *log_row, *log_step;
+    double* log_val;
+    int64_t log_cap;
+    unsigned long long* log_top;
+    int32_t* cand;                        // rows with an entry in the pivot column (n)
+    int32_t* scratch_col;
+    double* scratch_val;
+    RowLuStatus* status;
+};
+
+// the whole elimination of one system in one launch of one workgroup (vs 1 real, 2 complex)
+void row_lu(int vs, const RowLuArgs& a, hipStream_t s);
+
+}  // namespace fpa

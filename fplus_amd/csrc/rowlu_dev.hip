@@ -1,0 +1,498 @@
+// rowlu_dev.hip -- the row kernel's elimination on the device: one workgroup of 1024 threads runs every step of one
+// system (reference src/factor_row.c:1330-1560, restated for the GPU).
+//
+// A step is: the pivot search (a block-wide arg-max whose ties go to the first entry in the reference's scan order:
+// rows by position, then columns), the row and column exchanges, the pivot row's drops (incomplete LU), and the
+// elimination of every row with an entry in the pivot column.  Active rows hold only columns >= k at step k, so the
+// pivot column's entry of a row, when present, is its first: the candidates are the rows whose leading column is k.
+// The pivot row (without its pivot) is staged in LDS once per step; each wavefront then merges one candidate row with
+// it: every entry's place in the union comes from a binary search into the other row and a ballot prefix of the
+// matches, the union goes to the wavefront's scratch with dropped entries marked, and a second pass compacts it into
+// the row's block (or a fresh one at the pool's top when it has outgrown its room).  Values are formed with the
+// reference's operations in its order (m = a_ik / a_kk; a_ij + (-m) a_kj; (-m) a_kj), compiled without contraction, so
+// real factors are bit-identical to the reference's; complex division uses Smith's method.
+#include "dev_common.h"
+#include "prof.h"
+#include "rowlu_dev.h"
+
+namespace fpa {
+namespace {
+
+constexpr int RL_THREADS = 1024;
+constexpr int RL_WAVES = RL_THREADS / 64;
+constexpr int RL_BMAX = 2048;   // pivot-row entries staged in LDS (longer rows are read from the pool)
+
+__device__ inline uint64_t lanes_below() { return (1ull << (threadIdx.x & 63)) - 1ull; }
+
+__device__ inline double shx(double v, int m) { return __shfl_xor(v, m); }
+__device__ inline int64_t shx(int64_t v, int m) { return (int64_t)__shfl_xor((long long)v, m); }
+__device__ inline Zd shx(Zd v, int m) { return zmake(__shfl_xor(v.x, m), __shfl_xor(v.y, m)); }
+
+// a larger magnitude wins; equal magnitudes go to the smaller key (the earlier entry in the reference's scan)
+__device__ inline bool better(double a, int64_t ka, double b, int64_t kb) { return a > b || (a == b && ka < kb); }
+
+template <class T> struct Best {
+    double a;
+    int64_t k;
+    T v;
+};
+
+template <class T> __device__ inline Best<T> none() { return Best<T>{-1.0, INT64_MAX, Sc<T>::zero()}; }
+
+template <class T> __device__ inline void wave_best(Best<T>& b)
+{
+    for (int m = 32; m > 0; m >>= 1) {
+        const double oa = shx(b.a, m);
+        const int64_t ok = shx(b.k, m);
+        const T ov = shx(b.v, m);
+        if (better(oa, ok, b.a, b.k)) { b.a = oa; b.k = ok; b.v = ov; }
+    }
+}
+
+// the block's best candidate, the same in every thread
+template <class T> __device__ Best<T> block_best(Best<T> b, double* s_a, int64_t* s_k, T* s_v)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    wave_best(b);
+    if (lane == 0) { s_a[wave] = b.a; s_k[wave] = b.k; s_v[wave] = b.v; }
+    __syncthreads();
+    if (wave == 0) {
+        Best<T> c = none<T>();
+        if (lane < RL_WAVES) { c.a = s_a[lane]; c.k = s_k[lane]; c.v = s_v[lane]; }
+        wave_best(c);
+        if (lane == 0) { s_a[RL_WAVES] = c.a; s_k[RL_WAVES] = c.k; s_v[RL_WAVES] = c.v; }
+    }
+    __syncthreads();
+    const Best<T> r{s_a[RL_WAVES], s_k[RL_WAVES], s_v[RL_WAVES]};
+    __syncthreads();
+    return r;
+}
+
+__device__ double block_max(double v, double* s_a)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int m = 32; m > 0; m >>= 1) v = fmax(v, __shfl_xor(v, m));
+    if (lane == 0) s_a[wave] = v;
+    __syncthreads();
+    double r = 0.0;
+    for (int w = 0; w < RL_WAVES; w++) r = fmax(r, s_a[w]);
+    __syncthreads();
+    return r;
+}
+
+// first index of c[0 .. cnt) holding a column >= x
+__device__ inline int32_t lbound(const int32_t* c, int32_t cnt, int32_t x)
+{
+    int32_t lo = 0, hi = cnt;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi) >> 1;
+        if (c[mid] < x) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+__device__ inline int32_t find_col(const int32_t* c, int32_t cnt, int32_t x)
+{
+    const int32_t t = lbound(c, cnt, x);
+    return t < cnt && c[t] == x ? t : -1;
+}
+
+// a value formed by the elimination stays when nonzero and, for the incomplete factorization, above drop_tol
+__device__ inline bool keep_formed(double av, double dt) { return av > 0.0 && (dt == 0.0 || av > dt); }
+
+// columns j1 < j2 exchange their labels in row r (src/factor_row.c:215-238)
+template <class T> __device__ void col_swap_row(const RowLuArgs& a, int32_t r, int32_t j1, int32_t j2)
+{
+    const int32_t nz = a.nnz[r];
+    if (nz == 0) return;
+    int32_t* c = a.cols + a.off[r];
+    T* v = reinterpret_cast<T*>(a.vals) + a.off[r];
+    const int32_t t1 = find_col(c, nz, j1), t2 = find_col(c, nz, j2);
+    if (t1 < 0 && t2 < 0) return;
+    if (t1 >= 0 && t2 >= 0) {
+        const T x = v[t1];
+        v[t1] = v[t2];
+        v[t2] = x;
+        return;
+    }
+    int32_t t = t1 >= 0 ? t1 : t2;
+    const int32_t col = t1 >= 0 ? j2 : j1;
+    const T x = v[t];
+    while (t + 1 < nz && c[t + 1] < col) { c[t] = c[t + 1]; v[t] = v[t + 1]; t++; }
+    while (t > 0 && c[t - 1] > col) { c[t] = c[t - 1]; v[t] = v[t - 1]; t--; }
+    c[t] = col;
+    v[t] = x;
+    a.lead[r] = c[0];
+}
+
+// an incomplete-LU breakdown: the entry (k, k) of the row at position k set to rep, inserted when absent (one thread)
+template <class T> __device__ bool ilu_replace(const RowLuArgs& a, int32_t k, double rep)
+{
+    T* vals = reinterpret_cast<T*>(a.vals);
+    int32_t* cols = a.cols;
+    const int32_t r = a.slot[k];
+    const T rv = Sc<T>::scale(Sc<T>::one(), rep);
+    int64_t o = a.off[r];
+    const int32_t nz = a.nnz[r];
+    if (nz > 0 && cols[o] == k) { vals[o] = rv; return true; }
+    if (o > a.base[r]) {
+        o--;
+    } else if (o + nz < a.cap_end[r]) {
+        for (int32_t t = nz; t > 0; t--) { cols[o + t] = cols[o + t - 1]; vals[o + t] = vals[o + t - 1]; }
+    } else {
+        const int64_t size = (int64_t)nz + nz / 2 + 8;
+        const int64_t s = (int64_t)atomicAdd(a.pool_top, (unsigned long long)size);
+        if (s + size > a.pool_cap) return false;
+        for (int32_t t = 0; t < nz; t++) { cols[s + 1 + t] = cols[o + t]; vals[s + 1 + t] = vals[o + t]; }
+        a.base[r] = s;
+        a.cap_end[r] = s + size;
+        o = s;
+    }
+    cols[o] = k;
+    vals[o] = rv;
+    a.off[r] = o;
+    a.nnz[r] = nz + 1;
+    a.lead[r] = k;
+    return true;
+}
+
+// one wavefront eliminates column k from row r with the pivot row (piv, then Bc / Bv: its nB later entries)
+// (src/factor_row.c:1507-1548 and the merge of :930-1002)
+template <class T>
+__device__ void eliminate_row(const RowLuArgs& a, int32_t r, int32_t k, T piv, const int32_t* Bc, const T* Bv, int32_t nB,
+                              int32_t* scols, T* svals, int32_t* s_status)
+{
+    const int lane = threadIdx.x & 63;
+    int32_t* cols = a.cols;
+    T* vals = reinterpret_cast<T*>(a.vals);
+    const double dt = a.drop_tol;
+    const int64_t o = a.off[r];
+    const int32_t nz = a.nnz[r];
+    const T m = Sc<T>::div(vals[o], piv);
+    if (dt > 0.0 && Sc<T>::absval(m) <= dt) {   // the multiplier is dropped with its entry
+        if (lane == 0) {
+            a.off[r] = o + 1;
+            a.nnz[r] = nz - 1;
+            a.lead[r] = nz > 1 ? cols[o + 1] : RL_NONE;
+        }
+        return;
+    }
+    if (lane == 0) {
+        const unsigned long long li = atomicAdd(a.log_top, 1ull);
+        if ((int64_t)li < a.log_cap) {
+            a.log_row[li] = r;
+            a.log_step[li] = k;
+            reinterpret_cast<T*>(a.log_val)[li] = m;
+        } else {
+            *s_status = RL_LOG;
+        }
+    }
+    const T alpha = Sc<T>::scale(m, -1.0);
+    const int32_t nA = nz - 1;
+    const int64_t oA = o + 1;
+    const uint64_t below = lanes_below();
+    const int32_t ucap = a.n;
+    // row r's entries past column k: their places in the union
+    int32_t carry = 0, kept = 0;
+    bool bad = false;
+    for (int32_t b0 = 0; b0 < nA; b0 += 64) {
+        const int32_t i = b0 + lane;
+        const bool valid = i < nA;
+        const int32_t c = valid ? cols[oA + i] : RL_NONE;
+        const int32_t lb = valid ? lbound(Bc, nB, c) : 0;
+        const bool match = valid && lb < nB && Bc[lb] == c;
+        const uint64_t mm = __ballot(match);
+        const int32_t mb = carry + __popcll(mm & below);
+        carry += __popcll(mm);
+        bool keep = false;
+        if (valid) {
+            T w = vals[oA + i];
+            keep = true;
+            if (match) {
+                w = Sc<T>::add(w, Sc<T>::mul(alpha, Bv[lb]));
+                keep = keep_formed(Sc<T>::absval(w), dt);
+            }
+            const int32_t u = i + lb - mb;
+            if (u >= 0 && u < ucap) { scols[u] = keep ? c : -1; svals[u] = w; } else bad = true;
+        }
+        kept += __popcll(__ballot(keep));
+    }
+    const int32_t matches = carry;
+    // the pivot row's entries that row r lacks: fill
+    carry = 0;
+    for (int32_t b0 = 0; b0 < nB; b0 += 64) {
+        const int32_t j = b0 + lane;
+        const bool valid = j < nB;
+        const int32_t c = valid ? Bc[j] : RL_NONE;
+        const int32_t la = valid ? lbound(cols + oA, nA, c) : 0;
+        const bool match = valid && la < nA && cols[oA + la] == c;
+        const uint64_t mm = __ballot(match);
+        const int32_t mb = carry + __popcll(mm & below);
+        carry += __popcll(mm);
+        bool keep = false;
+        if (valid && !match) {
+            const T w = Sc<T>::mul(alpha, Bv[j]);
+            keep = keep_formed(Sc<T>::absval(w), dt);
+            const int32_t u = j + la - mb;
+            if (u >= 0 && u < ucap) { scols[u] = keep ? c : -1; svals[u] = w; } else bad = true;
+        }
+        kept += __popcll(__ballot(keep));
+    }
+    if (__ballot(bad)) {
+        if (lane == 0) *s_status = RL_INTERNAL;
+        return;
+    }
+    const int32_t U = nA + nB - matches;
+    __threadfence_block();
+    // the row's new place: its own block when the kept entries fit, else a fresh block at the pool's top
+    int64_t d = -1;
+    if (lane == 0) {
+        if (a.base[r] + kept <= a.cap_end[r]) {
+            d = a.base[r];
+        } else {
+            const int64_t size = (int64_t)kept + kept / 2 + 8;
+            const int64_t s = (int64_t)atomicAdd(a.pool_top, (unsigned long long)size);
+            if (s + size <= a.pool_cap) {
+                a.base[r] = s;
+                a.cap_end[r] = s + size;
+                d = s;
+            } else {
+                *s_status = RL_POOL;
+            }
+        }
+    }
+    d = (int64_t)__shfl((long long)d, 0);
+    if (d < 0) return;
+    int32_t w = 0, first = RL_NONE;
+    for (int32_t b0 = 0; b0 < U; b0 += 64) {
+        const int32_t u = b0 + lane;
+        const bool valid = u < U;
+        const int32_t c = valid ? scols[u] : -1;
+        const T v = valid ? svals[u] : Sc<T>::zero();
+        const bool keep = c >= 0;
+        const uint64_t mm = __ballot(keep);
+        if (keep) {
+            const int64_t dd = d + w + __popcll(mm & below);
+            cols[dd] = c;
+            vals[dd] = v;
+        }
+        if (w == 0 && mm) first = __shfl(c, __ffsll((unsigned long long)mm) - 1);
+        w += __popcll(mm);
+    }
+    if (lane == 0) {
+        a.off[r] = d;
+        a.nnz[r] = w;
+        a.lead[r] = w ? first : RL_NONE;
+    }
+}
+
+template <class T> __global__ __launch_bounds__(RL_THREADS) void k_row_lu(RowLuArgs a)
+{
+    __shared__ int32_t sBc[RL_BMAX];
+    __shared__ T sBv[RL_BMAX];
+    __shared__ double s_a[RL_WAVES + 1];
+    __shared__ int64_t s_k[RL_WAVES + 1];
+    __shared__ T s_v[RL_WAVES + 1];
+    __shared__ int32_t s_ncand, s_status, s_rk;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int32_t n = a.n;
+    T* vals = reinterpret_cast<T*>(a.vals);
+    int32_t* cols = a.cols;
+    int32_t* scols = a.scratch_col + (int64_t)wave * n;
+    T* svals = reinterpret_cast<T*>(a.scratch_val) + (int64_t)wave * n;
+    const double dt = a.drop_tol;
+    const uint64_t below = lanes_below();
+
+    double mx = 0.0;   // max |a_ij| of the input
+    for (int32_t r = tid; r < n; r += RL_THREADS) {
+        const int64_t o = a.off[r];
+        const int32_t nz = a.nnz[r];
+        for (int32_t t = 0; t < nz; t++) mx = fmax(mx, Sc<T>::absval(vals[o + t]));
+    }
+    const double in_max = block_max(mx, s_a);
+    if (tid == 0) s_status = RL_OK;
+    double max0 = 0.0, thr = 0.0, best = 0.0;
+    int64_t nrep = 0, kstop = n;
+
+    for (int32_t k = 0; k < n; k++) {
+        if (tid == 0) s_ncand = 0;
+        __syncthreads();
+        // ---- 1. pivot search (src/factor_row.c:273-342, :423-479, :558-658)
+        int64_t pi = -1, pj = -1;
+        bool found = false;
+        T pv = Sc<T>::zero();
+        if (a.strategy == 1) {                                   // partial: the rows led by column k
+            Best<T> b = none<T>();
+            for (int32_t i = k + tid; i < n; i += RL_THREADS) {
+                const int32_t r = a.slot[i];
+                if (a.lead[r] != k) continue;
+                const T v = vals[a.off[r]];
+                const double av = Sc<T>::absval(v);
+                if (better(av, i, b.a, b.k)) b = Best<T>{av, i, v};
+                a.cand[atomicAdd(&s_ncand, 1)] = r;
+            }
+            b = block_best(b, s_a, s_k, s_v);
+            if (b.k != INT64_MAX) { pi = b.k; pv = b.v; }
+            pj = k;
+            found = pi >= 0 && b.a > 0.0;
+        } else if (a.strategy == 0) {                            // full: every active entry, rows then columns
+            Best<T> b = none<T>();
+            for (int32_t i = k + wave; i < n; i += RL_WAVES) {
+                const int32_t r = a.slot[i];
+                const int64_t o = a.off[r];
+                const int32_t nz = a.nnz[r];
+                for (int32_t t = lane; t < nz; t += 64) {
+                    const int32_t c = cols[o + t];
+                    if (c < k) continue;
+                    const T v = vals[o + t];
+                    const double av = Sc<T>::absval(v);
+                    const int64_t key = (int64_t)i * n + c;
+                    if (better(av, key, b.a, b.k)) b = Best<T>{av, key, v};
+                }
+            }
+            b = block_best(b, s_a, s_k, s_v);
+            if (b.k != INT64_MAX) { pi = b.k / n; pj = b.k % n; pv = b.v; }
+            found = pi >= 0 && b.a > 0.0;
+        } else {                                                  // rook: column and row scans in turn, 8 rounds
+            int64_t ci = k, ri = -1;
+            double babs = -1.0;
+            T bval = Sc<T>::zero();
+            bool done = false;
+            for (int it = 0; it < 8; it++) {
+                Best<T> b = none<T>();
+                for (int32_t i = k + tid; i < n; i += RL_THREADS) {
+                    const int32_t r = a.slot[i];
+                    const int64_t o = a.off[r];
+                    const int32_t t = ci == k ? (a.lead[r] == k ? 0 : -1) : find_col(cols + o, a.nnz[r], (int32_t)ci);
+                    if (t < 0) continue;
+                    const T v = vals[o + t];
+                    const double av = Sc<T>::absval(v);
+                    if (better(av, i, b.a, b.k)) b = Best<T>{av, i, v};
+                }
+                b = block_best(b, s_a, s_k, s_v);
+                if (b.k == INT64_MAX) { pi = -1; pj = -1; pv = Sc<T>::zero(); found = false; done = true; break; }
+                ri = b.k; bval = b.v; babs = b.a;
+                const int32_t r = a.slot[ri];
+                const int64_t o = a.off[r];
+                const int32_t nz = a.nnz[r];
+                Best<T> c = none<T>();
+                for (int32_t t = tid; t < nz; t += RL_THREADS) {
+                    const int32_t cc = cols[o + t];
+                    if (cc < k) continue;
+                    const T v = vals[o + t];
+                    const double av = Sc<T>::absval(v);
+                    if (better(av, cc, c.a, c.k)) c = Best<T>{av, cc, v};
+                }
+                c = block_best(c, s_a, s_k, s_v);
+                if (c.k == INT64_MAX) { pi = -1; pj = -1; pv = Sc<T>::zero(); found = false; done = true; break; }
+                if (c.k == ci) { pi = ri; pj = ci; pv = bval; found = babs > 0.0; done = true; break; }
+                ci = c.k; bval = c.v; babs = c.a;
+            }
+            if (!done) { pi = ri; pj = ci; pv = bval; found = babs > 0.0; }
+        }
+        best = Sc<T>::absval(pv);
+        if (k == 0) { max0 = fmax(best, in_max); thr = a.pivot_rel * max0; }
+        // ---- 1b. an incomplete-LU breakdown: the pivot (k, k) replaced by drop_tol * max|a_ij|
+        if ((!found || best < thr || best == 0.0) && dt > 0.0 && dt * max0 > 0.0) {
+            const double rep = dt * max0;
+            if (tid == 0 && !ilu_replace<T>(a, k, rep)) s_status = RL_POOL;
+            pi = k; pj = k; found = true; best = rep;
+            nrep++;
+            __syncthreads();
+            if (s_status != RL_OK) { kstop = k; break; }
+        }
+        if (!found || best < thr) {
+            if (tid == 0) s_status = RL_SINGULAR;
+            kstop = k;
+            break;
+        }
+        // ---- 2, 3. the exchanges
+        if (tid == 0 && pi != k) {
+            const int32_t t = a.slot[k]; a.slot[k] = a.slot[pi]; a.slot[pi] = t;
+            const int64_t x = a.p[k]; a.p[k] = a.p[pi]; a.p[pi] = x;
+        }
+        __syncthreads();
+        if (pj != k) {
+            for (int32_t r = tid; r < n; r += RL_THREADS) col_swap_row<T>(a, r, k, (int32_t)pj);
+            if (tid == 0) { const int64_t x = a.q[k]; a.q[k] = a.q[pj]; a.q[pj] = x; }
+            __syncthreads();
+        }
+        if (a.strategy != 1) {                                    // the rows led by column k after the exchanges
+            for (int32_t i = k + 1 + tid; i < n; i += RL_THREADS) {
+                const int32_t r = a.slot[i];
+                if (a.lead[r] == k) a.cand[atomicAdd(&s_ncand, 1)] = r;
+            }
+        }
+        if (tid == 0) {
+            const int32_t r = a.slot[k];
+            s_rk = r;
+            if (a.nnz[r] == 0 || a.lead[r] != k) s_status = RL_INTERNAL;
+        }
+        __syncthreads();
+        if (s_status != RL_OK) { kstop = k; break; }
+        const int32_t rk = s_rk;
+        // ---- 4b. the pivot row, final, loses its off-pivot entries of at most drop_tol
+        if (dt > 0.0 && wave == 0) {
+            const int64_t o = a.off[rk];
+            const int32_t nz = a.nnz[rk];
+            int32_t w = 0;
+            for (int32_t b0 = 0; b0 < nz; b0 += 64) {
+                const int32_t t = b0 + lane;
+                const bool valid = t < nz;
+                const int32_t c = valid ? cols[o + t] : 0;
+                const T v = valid ? vals[o + t] : Sc<T>::zero();
+                const bool keep = valid && (c == k || Sc<T>::absval(v) > dt);
+                const uint64_t mm = __ballot(keep);
+                if (keep) {
+                    const int32_t d = w + __popcll(mm & below);
+                    cols[o + d] = c;
+                    vals[o + d] = v;
+                }
+                w += __popcll(mm);
+            }
+            if (lane == 0) a.nnz[rk] = w;
+        }
+        __syncthreads();
+        const int64_t ok = a.off[rk];
+        const int32_t nB = a.nnz[rk] - 1;
+        const T piv = vals[ok];
+        const bool staged = nB <= RL_BMAX;
+        if (staged)
+            for (int32_t t = tid; t < nB; t += RL_THREADS) { sBc[t] = cols[ok + 1 + t]; sBv[t] = vals[ok + 1 + t]; }
+        __syncthreads();
+        // ---- 5. elimination: one wavefront per row led by column k
+        const int32_t* Bc = staged ? sBc : cols + ok + 1;
+        const T* Bv = staged ? sBv : vals + ok + 1;
+        const int32_t ncand = s_ncand;
+        for (int32_t c = wave; c < ncand; c += RL_WAVES) {
+            const int32_t r = a.cand[c];
+            if (r == rk) continue;
+            eliminate_row<T>(a, r, k, piv, Bc, Bv, nB, scols, svals, &s_status);
+        }
+        __syncthreads();
+        if (s_status != RL_OK) { kstop = k; break; }
+    }
+    if (tid == 0) {
+        RowLuStatus* st = a.status;
+        st->code = s_status;
+        st->step = kstop;
+        st->nrep = nrep;
+        st->best = best;
+        st->thr = thr;
+        st->max0 = max0;
+        st->pool_used = (int64_t)*a.pool_top;
+        st->log_used = (int64_t)*a.log_top;
+    }
+}
+
+}  // namespace
+
+void row_lu(int vs, const RowLuArgs& a, hipStream_t s)
+{
+    ProfScope ps("row_lu", 0.0, s);
+    if (vs == 2) hipLaunchKernelGGL(k_row_lu<Zd>, dim3(1), dim3(RL_THREADS), 0, s, a);
+    else hipLaunchKernelGGL(k_row_lu<double>, dim3(1), dim3(RL_THREADS), 0, s, a);
+}
+
+}  // namespace fpa

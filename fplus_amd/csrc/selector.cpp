@@ -180,7 +180,7 @@ int fplus_amd_selector_predict(const fplus_amd_selector* sel, const float* featu
 
 int fplus_amd_selector_log(const fplus_numeric* num, float* features, double* diff, double* pmax, int64_t cap, int64_t* nblocks)
 {
-    if (!num || !nblocks) return FPLUS_ERR_BADARG;
+    if (!num || !nblocks || num->row) return FPLUS_ERR_BADARG;
     const DevFactor& F = num->F;
     const int64_t nb = (int64_t)F.blocks.size();
     *nblocks = nb;

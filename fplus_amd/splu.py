@@ -7,8 +7,8 @@ The keyword set and its defaults are the reference's: ``colamd=False`` (the iden
 ``ordering='auto'``, ``relax_rows=0``, ``panel=1``, ``equilibrate=True`` (an int k runs at most k Ruiz iterations, 0
 none), ``pivot_strategy='partial'``, ``pivot_rel=None`` (sqrt(eps)), ``cache_ordering=True``.  ``gpu`` and
 ``gpu_min_flops`` are accepted: the device path is the only one.  ``pivot_strategy='rook'`` / ``'full'`` and
-``drop_tol > 0`` need the reference's row kernel, which this build does not provide: they raise ``FplusError``
-(FPLUS_ERR_BADARG) with a message.
+``drop_tol > 0`` run the row kernel on the device (one workgroup per system; the reference's fplus_factor dispatch,
+src/numeric.c:51-64).
 """
 import collections
 import hashlib
@@ -17,9 +17,9 @@ import threading
 import numpy as np
 import scipy.sparse as sp
 
-from ._lib import FPLUS_ORDER_AMD_ATA, FPLUS_ORDER_AUTO, FPLUS_ORDER_COLAMD, FPLUS_PIVOT_REL_DEFAULT, FplusError
+from ._lib import FPLUS_ORDER_AMD_ATA, FPLUS_ORDER_AUTO, FPLUS_ORDER_COLAMD, FPLUS_PIVOT_REL_DEFAULT
 
-_PIVOTS = ("partial", "rook", "full")
+_PIVOTS = {"partial": 1, "rook": 2, "full": 0}
 _ORDERINGS = {"colamd": FPLUS_ORDER_COLAMD, "amd_ata": FPLUS_ORDER_AMD_ATA, "auto": FPLUS_ORDER_AUTO}
 _RUIZ_MAX_ITER = 20
 
@@ -89,9 +89,8 @@ def _factor(A, drop_tol=0.0, equilibrate=True, colamd=False, pin_dense_rows=Fals
         raise ValueError(f"pivot_strategy must be 'partial', 'rook', or 'full'; got {pivot_strategy!r}.")
     if ordering not in _ORDERINGS:
         raise ValueError(f"ordering must be 'colamd', 'amd_ata' or 'auto'; got {ordering!r}.")
-    if pivot_strategy != "partial" or drop_tol != 0.0:
-        raise FplusError(-3, "rook / full pivoting and incomplete LU (drop_tol > 0) use the reference's row kernel, which "
-                             "this MI355X build does not provide: use pivot_strategy='partial' with drop_tol=0")
+    if not drop_tol >= 0.0:
+        raise ValueError(f"drop_tol must be >= 0; got {drop_tol!r}.")
     n, Ap, Ai, Ax = _canonical_csc(A)
     cplx = np.iscomplexobj(Ax)
     if dtype is not None:
@@ -112,8 +111,17 @@ def _factor(A, drop_tol=0.0, equilibrate=True, colamd=False, pin_dense_rows=Fals
     sym = symbolic_from_orders(n, Ap, Ai, rp, cp)
     o = default_options(equilibrate=int(iters > 0), ruiz_max_iter=max(iters, 1), relax_rows=int(relax_rows),
                         panel=int(panel), verbose=int(verbose), use_gpu=1, gpu_min_flops=float(gpu_min_flops),
-                        pivot_rel=FPLUS_PIVOT_REL_DEFAULT if pivot_rel is None else float(pivot_rel))
-    return Numeric(sym, Ap, Ai, Ax, o), sym, n, Ap, Ai, Ax
+                        pivot_rel=FPLUS_PIVOT_REL_DEFAULT if pivot_rel is None else float(pivot_rel),
+                        pivot_strategy=_PIVOTS[pivot_strategy], drop_tol=float(drop_tol))
+    num = Numeric(sym, Ap, Ai, Ax, o)
+    if pivot_strategy != "partial" or drop_tol != 0.0:
+        nrep = num.factors()["n_pivots_replaced"]
+        if nrep > 0:
+            import warnings
+            warnings.warn(f"ILU breakdown: {nrep} zero or negligible pivots replaced by drop_tol * max|a_ij| (the exact "
+                          f"factorization's trailing pivots are at or below drop_tol={drop_tol:g}); the factors are a "
+                          "preconditioner of reduced quality", RuntimeWarning, stacklevel=3)
+    return num, sym, n, Ap, Ai, Ax
 
 
 def sparse_lu_full_pivot(A, *, _return_csc=False, **kw):

@@ -50,3 +50,27 @@ def parity_matrices(small=True):
         out.append(("convdiff2d-96", *gen.convdiff2d(96), 32))
         out.append(("shiftinvert-8x4x64", *gen.shift_invert(8, 4, 64), 32))
     return out
+
+
+def csr_arrays(A):
+    """(n, indptr, indices, data) of a canonical CSR matrix"""
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    return A.shape[0], A.indptr.astype(np.int64), A.indices.astype(np.int64), A.data
+
+
+def row_kernel_cases():
+    """(name, n, Rp, Ri, Rx) CSR cases of the row kernel, equilibrated by the oracle's Ruiz (CSR route) as its pipeline
+    would (src/pipeline.c:186-200)"""
+    import oracle
+    out = []
+    for cplx in (False, True):
+        n, Ap, Ai, Ax = gen.reference_test_matrix(120, cplx, 2.0)
+        # the CSC arrays of A are the CSR arrays of A^T, as the reference's own test feeds them (test_fplus.c:236)
+        out.append((f"reftest120-{'z' if cplx else 'd'}", n, Ap, Ai, Ax))
+    n, Ap, Ai, Ax = gen.convdiff2d(12)
+    out.append(("convdiff2d-12", n, Ap, Ai, Ax))
+    rng = np.random.default_rng(5)
+    A = sp.random(90, 90, density=0.05, random_state=rng, format="csr") + sp.eye(90) * 0.1
+    out.append(("random90", *csr_arrays(A)))
+    return [(name, n, Rp, Ri, oracle.ruiz(n, Rp, Ri, Rx, is_csc=False)[0]) for name, n, Rp, Ri, Rx in out]

@@ -148,12 +148,32 @@ def test_validation_without_device():
     assert rc == _lib.FPLUS_ERR_BADARG and b"permutation" in e.value
 
 
-def test_row_kernel_is_a_clear_error():
+def test_row_kernel_validation_without_device():
+    """the row kernel's argument checks come before any device work, with the reference's messages
+    (src/factor_row.c:1200-1210, src/numeric.c:51-57)"""
+    L = _lib.lib()
+    P = _lib.I64P
+    p = np.array([0, 2, 3], np.int64)
+    i_ok = np.array([0, 1, 1], np.int64)
+    i_unsorted = np.array([1, 0, 1], np.int64)
+    x = np.array([1.0, 2.0, 3.0])
+    f = _lib.Factors()
+    e = _err()
+    args = (p.ctypes.data_as(P), i_ok.ctypes.data_as(P), x.ctypes.data)
+    rc = L.fplus_factor_row(2, *args, 0.0, 0, 0, 3, 1e-8, C.byref(f), e, 256)
+    assert rc == _lib.FPLUS_ERR_BADARG and b"pivot_strategy must be 0 (full), 1 (partial), or 2 (rook); got 3" in e.value
+    rc = L.fplus_factor_row(2, *args, -1.0, 0, 0, 2, 1e-8, C.byref(f), e, 256)
+    assert rc == _lib.FPLUS_ERR_BADARG and b"drop_tol must be >= 0" in e.value
+    rc = L.fplus_factor_row(2, *args, 0.0, 0, 0, 2, -1.0, C.byref(f), e, 256)
+    assert rc == _lib.FPLUS_ERR_BADARG and b"pivot_rel must be >= 0" in e.value
+    rc = L.fplus_factor_row(2, p.ctypes.data_as(P), i_unsorted.ctypes.data_as(P), x.ctypes.data, 0.0, 0, 0, 2, 1e-8, C.byref(f), e, 256)
+    assert rc == _lib.FPLUS_ERR_BADARG and b"strictly increasing" in e.value
     n, Ap, Ai, Ax = gen.convdiff2d(4)
     s = fplus_amd.Symbolic(n, Ap, Ai)
-    with pytest.raises(fplus_amd.FplusError) as e:
-        fplus_amd.Numeric(s, Ap, Ai, Ax, fplus_amd.default_options(pivot_strategy=0))
-    assert e.value.code == _lib.FPLUS_ERR_BADARG and "row kernel" in str(e.value)
+    for kw in (dict(pivot_strategy=7), dict(drop_tol=-1.0)):
+        with pytest.raises(fplus_amd.FplusError) as er:
+            fplus_amd.Numeric(s, Ap, Ai, Ax, fplus_amd.default_options(**kw))
+        assert er.value.code == _lib.FPLUS_ERR_BADARG
 
 
 def test_generators_are_deterministic():

@@ -8,15 +8,9 @@ import scipy.linalg as sla
 import scipy.sparse as sp
 
 import oracle
-from fplus_amd import gen
+from helpers import csr_arrays as csr, row_kernel_cases
 
 STRATS = {0: "full", 1: "partial", 2: "rook"}
-
-
-def csr(A):
-    A = sp.csr_matrix(A)
-    A.sort_indices()
-    return A.shape[0], A.indptr.astype(np.int64), A.indices.astype(np.int64), A.data
 
 
 def dense_factors(F, n):
@@ -25,27 +19,7 @@ def dense_factors(F, n):
     return L, U
 
 
-def equilibrated(name, n, Rp, Ri, Rx):
-    """Ruiz on the CSR matrix first, as the row-kernel pipeline does (src/pipeline.c:186-200)"""
-    x = oracle.ruiz(n, Rp, Ri, Rx, is_csc=False)[0]
-    return name, n, Rp, Ri, x
-
-
-def row_cases():
-    out = []
-    for cplx in (False, True):
-        n, Ap, Ai, Ax = gen.reference_test_matrix(120, cplx, 2.0)
-        # the CSC arrays of A are the CSR arrays of A^T, as the reference's own test feeds them (test_fplus.c:236)
-        out.append((f"reftest120-{'z' if cplx else 'd'}", n, Ap, Ai, Ax))
-    n, Ap, Ai, Ax = gen.convdiff2d(12)
-    out.append(("convdiff2d-12", n, Ap, Ai, Ax))
-    rng = np.random.default_rng(5)
-    A = sp.random(90, 90, density=0.05, random_state=rng, format="csr") + sp.eye(90) * 0.1
-    out.append(("random90", *csr(A)))
-    return [equilibrated(*c) for c in out]
-
-
-CASES = row_cases()
+CASES = row_kernel_cases()
 
 
 @pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref is not built (the reference tree is absent)")
