@@ -63,10 +63,16 @@ WORKLOADS = {
     # mode counts drawn so that n is log-uniform in [4096, 65536], 4 right-hand sides each, factored and solved one
     # after another, and side by side on several host threads with a stream each
     "batch-256": dict(systems=256, nfields=4, ncoef=1024),
+    # the row kernel (rook / full / partial pivoting, incomplete LU): the 96 x 96 convection-diffusion matrix (n = 9216)
+    # in its analysed order, equilibrated, factored by fplus_factor_row once per leg
+    "rowlu-convdiff2d-96": dict(g=96),
 }
+# the row kernel's legs: (pivot strategy 0 full / 1 partial / 2 rook, drop_tol)
+ROW_LEGS = {"row_partial_ilu": (1, 1e-3), "row_rook_ilu": (2, 1e-3), "row_full_ilu": (0, 1e-2), "row_partial": (1, 0.0)}
 # the legs each workload runs (the default: the headline's four)
 WORKLOAD_LEGS = {"convdiff3d-64": ("fp64_gpu", "fp32_factor_ir", "bf16_factor_ir"),
-                 "sweep-32x4x1024": ("sweep", "sweep_graph"), "batch-256": ("batch", "batch_streams")}
+                 "sweep-32x4x1024": ("sweep", "sweep_graph"), "batch-256": ("batch", "batch_streams"),
+                 "rowlu-convdiff2d-96": tuple(ROW_LEGS)}
 WORKLOAD_NRHS = {"convdiff3d-64": 8, "shiftinvert-128x4x1024-s17": 64, "sweep-32x4x1024": 4, "batch-256": 4}
 WORKLOAD_TOL = {"convdiff3d-64": 1e-14}
 DEFAULT_WORKLOAD = "shiftinvert-64x4x1024"   # n = 262144, the plain FP64 GPU drop-in configuration
@@ -78,7 +84,7 @@ PEAK_HBM = 8.0e12             # MI355X HBM3E, B/s
 NRHS_DEFAULT = 16
 TOL_DEFAULT = 1e-13
 LEGS = ("fp64_gpu", "fp64_refactor", "fp32_refactor_ir", "learned_refactor_ir")
-ALL_LEGS = LEGS + ("fp32_factor_ir", "bf16_factor_ir", "sweep", "sweep_graph", "batch", "batch_streams")
+ALL_LEGS = LEGS + ("fp32_factor_ir", "bf16_factor_ir", "sweep", "sweep_graph", "batch", "batch_streams") + tuple(ROW_LEGS)
 SELECTOR = os.path.join(ROOT, "fplus_amd", "data", "selector_shiftinvert.npz")
 # the dominant kernel's HBM bytes per launch, from the rocprofv3 FETCH_SIZE and WRITE_SIZE passes summarized there
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blk_upd-hbm_traffic-shiftinvert-64x4x1024.txt")
@@ -412,6 +418,86 @@ def batch_main(args, kw, nrhs):
     print(json.dumps(res))
 
 
+def rowlu_main(args, kw, legs):
+    """the row kernel: the matrix in its analysed order (the symbolic orders), equilibrated by the device Ruiz and laid
+    out by rows before the timed region; per leg, fplus_factor_row's seconds (the values uploaded, the elimination on
+    the device, the factors back on the host), median of --steps after --warmup, and the device elimination's row
+    merges and their algorithmic bytes; beside it the reference's own row kernel (src/factor_row.c built with OpenMP,
+    oracle/_ref/libfplus_ref_row_omp.so) on the host's cores with the same input, and whether both factorizations agree
+    (p, q, patterns; values)"""
+    import scipy.sparse as sp
+    n, Ap, Ai, Ax = gen.convdiff2d(**kw)
+    sym = fplus_amd.Symbolic(n, Ap, Ai)
+    B = sp.csc_matrix((Ax, Ai, Ap), shape=(n, n))[sym.row_perm][:, sym.col_perm].tocsc()
+    B.sort_indices()
+    Bp, Bi = B.indptr.astype(np.int64), B.indices.astype(np.int64)
+    Bx, _, _ = fplus_amd.ruiz(n, Bp, Bi, B.data)
+    R = sp.csc_matrix((Bx, Bi, Bp), shape=(n, n)).tocsr()
+    R.sort_indices()
+    Rp, Ri, Rx = R.indptr.astype(np.int64), R.indices.astype(np.int64), np.ascontiguousarray(R.data)
+    log(f"workload rowlu: n {n}, nnz {len(Ri)}")
+    threads = min(16, os.cpu_count() or 1)
+    ref = None
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libfplus_ref_row_omp.so")
+    if not args.no_cpu and os.path.exists(ref_path):
+        os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+        ref = C.CDLL(ref_path)
+        ref.fplus_factor_row.argtypes = [C.c_int64, _lib.I64P, _lib.I64P, C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int,
+                                         C.c_double, C.POINTER(_lib.Factors), C.c_char_p, C.c_size_t]
+    out = {}
+    for leg in legs:
+        strategy, dt = ROW_LEGS[leg]
+        for _ in range(args.warmup):
+            fplus_amd.factor_row(n, Rp, Ri, Rx, dt, strategy)
+        ts, st = [], None
+        for _ in range(max(1, args.steps)):
+            t0 = time.perf_counter()
+            d = fplus_amd.factor_row(n, Rp, Ri, Rx, dt, strategy)
+            ts.append(time.perf_counter() - t0)
+            st = fplus_amd.row_stats()
+        g = float(np.median(ts))
+        r = {"strategy": ["full", "partial", "rook"][strategy], "drop_tol": dt, "s": g, "device_s": st["t_device"],
+             "runs": st["runs"], "nnz_LU": int(d["Lp"][-1] + d["Up"][-1]), "merges": st["merges"],
+             "merge_GB": st["merge_bytes"] / 1e9, "merge_GB/s": st["merge_bytes"] / st["t_device"] / 1e9,
+             "frac_hbm": st["merge_bytes"] / st["t_device"] / PEAK_HBM, "n_pivots_replaced": d["n_pivots_replaced"]}
+        log(f"{leg}: {g:.3f} s (device {st['t_device']:.3f} s, {st['runs']} runs), nnz(L+U) {r['nnz_LU']}")
+        if ref is not None:
+            cts = []
+            for _ in range(max(1, args.cpu_runs)):
+                f = _lib.Factors()
+                e = C.create_string_buffer(256)
+                t0 = time.perf_counter()
+                rc = ref.fplus_factor_row(n, Rp.ctypes.data_as(_lib.I64P), Ri.ctypes.data_as(_lib.I64P), Rx.ctypes.data, dt, 0,
+                                          0, strategy, _lib.FPLUS_PIVOT_REL_DEFAULT, C.byref(f), e, 256)
+                cts.append(time.perf_counter() - t0)
+                if rc != 0:
+                    raise SystemExit(f"reference row kernel: {rc} {e.value.decode()}")
+                nL, nU = f.Lp[n], f.Up[n]
+                same = (nL + nU == r["nnz_LU"] and np.array_equal(np.ctypeslib.as_array(f.p, (n,)), d["p"])
+                        and np.array_equal(np.ctypeslib.as_array(f.q, (n,)), d["q"])
+                        and np.array_equal(np.ctypeslib.as_array(f.Ui, (nU,)), d["Ui"])
+                        and np.array_equal(np.ctypeslib.as_array(C.cast(f.Ux, _lib.DP), (nU,)), d["Ux"]))
+                _lib.lib().fplus_factors_free(C.byref(f))   # the factors' arrays are plain malloc'ed blocks
+            c = float(np.median(cts))
+            r.update({"cpu_s": c, "speedup_vs_cpu": c / g, "same_factors_as_reference": bool(same)})
+            log(f"{leg}: reference CPU {c:.3f} s ({threads} threads), identical factors: {same}")
+        out[leg] = r
+    head = out[legs[0]]
+    line = {"metric": "row-kernel factorization seconds", "value": head["s"], "unit": "s/system", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": head["s"] * 1e3, "higher_is_better": False,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": args.workload, "n": n, "nnz": int(len(Ri)), "legs": legs}, "legs": out,
+            "roofline": {"bound": "hbm", "achieved": head["merge_GB/s"], "peak": PEAK_HBM / 1e9, "unit": "GB/s",
+                         "frac": head["frac_hbm"], "traffic": None,
+                         "note": "one workgroup per system: the bound is its chain of dependent loads per step, not HBM"}}
+    if ref is not None:
+        line["cpu_baseline"] = {"value": head["cpu_s"], "unit": "s/system", "cores": threads, "kind": "reference",
+                                "sample": f"the same matrix and leg: the reference's src/factor_row.c built with OpenMP "
+                                          f"(-O3 -march=x86-64-v3), OMP_NUM_THREADS={os.environ.get('OMP_NUM_THREADS')}, "
+                                          f"median of {args.cpu_runs} runs"}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -446,6 +532,8 @@ def main():
         return sweep_main(args, kw, legs, NRHS)
     if args.workload.startswith("batch"):
         return batch_main(args, kw, NRHS)
+    if args.workload.startswith("rowlu"):
+        return rowlu_main(args, kw, legs)
     shift = args.workload.startswith("shiftinvert")
     if shift:
         n, Ap, Ai, Ax = gen.shift_invert(**kw, sigma=SIGMA)

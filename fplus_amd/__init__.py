@@ -12,12 +12,12 @@ import numpy as np
 import scipy.sparse as sp
 
 from ._lib import (FPLUS_ORDER_AMD_ATA, FPLUS_ORDER_AUTO, FPLUS_ORDER_COLAMD, FPLUS_PIVOT_PARTIAL,  # noqa: F401
-                   FPLUS_PIVOT_REL_DEFAULT, AmdOptions, Factors, FplusError, Options, RefineInfo, RefineOptions, Stats,
+                   FPLUS_PIVOT_REL_DEFAULT, AmdOptions, Factors, FplusError, Options, RefineInfo, RefineOptions, RowInfo, Stats,
                    check, lib)
 
 __all__ = ["SparseLU", "sparse_lu_full_pivot", "lu_solve", "lu_residual", "clear_ordering_cache", "Symbolic", "Numeric",
            "SolvePlan", "default_options", "amd_options", "FplusError", "ruiz", "factor_sn", "factor_sn_perm", "factor_sn_sym",
-           "factor_row", "factor_row_sym", "tri_solve", "tune", "Selector", "log_blocks"]
+           "factor_row", "factor_row_sym", "row_stats", "tri_solve", "tune", "Selector", "log_blocks"]
 
 _I64P = C.POINTER(C.c_int64)
 _DP = C.POINTER(C.c_double)
@@ -284,6 +284,14 @@ def stats(num):
     st = Stats()
     check(lib().fplus_amd_numeric_stats(num.ptr, C.byref(st)), None)
     return {k: getattr(st, k) for k, _ in Stats._fields_}
+
+
+def row_stats():
+    """fplus_amd_row_stats: the calling thread's last row-kernel factorization (device runs, seconds, row merges and
+    their algorithmic bytes, pool entries and multipliers) as a dict"""
+    st = RowInfo()
+    check(lib().fplus_amd_row_stats(C.byref(st)), None)
+    return {k: getattr(st, k) for k, _ in RowInfo._fields_}
 
 
 def ruiz(n, indptr, indices, data, is_csc=True, max_iter=20, tol=1e-2):

@@ -51,6 +51,12 @@ class Stats(C.Structure):
                 ("f32_blocks", C.c_int64), ("f32_flop_share", C.c_double)]
 
 
+class RowInfo(C.Structure):
+    """fplus_amd_row_info (include/fplus_amd.h)"""
+    _fields_ = [("runs", C.c_int64), ("restarts", C.c_int64), ("merges", C.c_int64), ("pool_entries", C.c_int64),
+                ("multipliers", C.c_int64), ("t_device", C.c_double), ("t_total", C.c_double), ("merge_bytes", C.c_double)]
+
+
 class AmdOptions(C.Structure):
     """fplus_amd_options (include/fplus_amd.h)"""
     _fields_ = [("precision", C.c_int), ("refactor", C.c_int), ("static_pivots", C.c_int), ("reserved", C.c_int),
@@ -128,6 +134,7 @@ SIGNATURES = {
     "fplus_amd_dev_copy": (C.c_int, [VP, VP, C.c_size_t, C.c_int]),
     "fplus_amd_dev_sync": (C.c_int, []),
     "fplus_amd_thread_stream": (C.c_int, [C.c_int]),
+    "fplus_amd_row_stats": (C.c_int, [VP]),
     "fplus_amd_profile": (C.c_int, [C.c_int]),
     "fplus_amd_profile_get": (C.c_int, [C.c_char_p, DP, I64P, DP]),
     "fplus_amd_tune": (C.c_int, [C.c_char_p, C.c_int64]),

@@ -5,6 +5,7 @@
 #include "rowlu.h"
 
 #include "device_api.h"
+#include "fplus_amd.h"
 #include "host_common.h"
 #include "rowlu_dev.h"
 #include "symbolic.h"
@@ -14,6 +15,7 @@
 #include <cstdlib>
 #include <memory>
 #include <new>
+#include <type_traits>
 
 using namespace fpa;
 
@@ -41,6 +43,9 @@ void compressed_transpose(int64_t n, const int64_t* P, const int64_t* I, std::ve
 }
 
 namespace {
+
+thread_local fplus_amd_row_info t_row_info;
+thread_local bool t_row_info_set = false;
 
 inline double now_s()
 {
@@ -119,29 +124,60 @@ int row_factor_device(int64_t n, const int64_t* Rp, const int64_t* Ri, const dou
     RowRun R;
     RowLuStatus st{};
     const double t0 = now_s();
-    for (int attempt = 0;; attempt++) {
+    int runs = 0, restarts = 0;
+    auto alloc_all = [&]() {
+        R.off.alloc(n); R.base.alloc(n); R.cap_end.alloc(n); R.p.alloc(n); R.q.alloc(n);
+        R.nnz.alloc(n); R.lead.alloc(n); R.slot.alloc(n); R.cand.alloc(n);
+        R.scratch_col.alloc((size_t)16 * n);
+        R.scratch_val.alloc((size_t)16 * n * vs);
+        R.cols.alloc(pool_cap);
+        R.vals.alloc((size_t)pool_cap * vs);
+        R.log_row.alloc(log_cap); R.log_step.alloc(log_cap);
+        R.log_val.alloc((size_t)log_cap * vs);
+        R.tops.alloc(2);
+        R.status.alloc(1);
+    };
+    // a pool grown in place: the entries (or multipliers) so far copied into a larger buffer
+    auto grow = [&](auto& buf, int64_t used, size_t per, size_t cap) {
+        std::remove_reference_t<decltype(buf)> nb(cap * per);
+        if (used > 0) hip_check(hipMemcpyAsync(nb.p, buf.p, (size_t)used * per * sizeof(*buf.p), hipMemcpyDeviceToDevice, s), "pool growth");
+        buf = std::move(nb);
+    };
+    bool fresh = true;
+    int32_t k0 = 0, resume = 0;
+    int64_t merges = 0;
+    double merge_bytes = 0.0;
+    for (;;) {
         try {
-            R.off.alloc(n); R.base.alloc(n); R.cap_end.alloc(n); R.p.alloc(n); R.q.alloc(n);
-            R.nnz.alloc(n); R.lead.alloc(n); R.slot.alloc(n); R.cand.alloc(n);
-            R.scratch_col.alloc((size_t)16 * n);
-            R.scratch_val.alloc((size_t)16 * n * vs);
-            R.cols.alloc(pool_cap);
-            R.vals.alloc((size_t)pool_cap * vs);
-            R.log_row.alloc(log_cap); R.log_step.alloc(log_cap);
-            R.log_val.alloc((size_t)log_cap * vs);
-            R.tops.alloc(2);
-            R.status.alloc(1);
+            if (fresh) {
+                alloc_all();
+                R.off.upload(h_off.data(), n, s); R.base.upload(h_off.data(), n, s); R.cap_end.upload(h_end.data(), n, s);
+                R.p.upload(h_id.data(), n, s); R.q.upload(h_id.data(), n, s);
+                R.nnz.upload(h_nnz.data(), n, s); R.lead.upload(h_lead.data(), n, s); R.slot.upload(h_slot.data(), n, s);
+                R.cols.upload(h_cols.data(), nnz, s);
+                if (nnz > 0)
+                    hip_check(hipMemcpyAsync(R.vals.p, d_Rx, (size_t)nnz * vs * sizeof(double), hipMemcpyDeviceToDevice, s), "row values");
+                const unsigned long long tops[2] = {(unsigned long long)nnz, 0ull};
+                R.tops.upload(tops, 2, s);
+                fresh = false;
+                k0 = 0;
+                resume = 0;
+            } else if (st.code == RL_POOL) {
+                const int64_t used = std::min(st.pool_used, pool_cap);
+                pool_cap *= 2;
+                grow(R.cols, used, 1, pool_cap);
+                grow(R.vals, used, vs, pool_cap);
+            } else if (st.code == RL_LOG) {
+                const int64_t used = std::min(st.log_used, log_cap);
+                log_cap *= 2;
+                grow(R.log_row, used, 1, log_cap);
+                grow(R.log_step, used, 1, log_cap);
+                grow(R.log_val, used, vs, log_cap);
+            }
         } catch (const GpuError&) {
             errf(err, errlen, "out of memory (device: %ld pool entries, %ld multipliers)", (long)pool_cap, (long)log_cap);
             return FPLUS_ERR_NOMEM;
         }
-        R.off.upload(h_off.data(), n, s); R.base.upload(h_off.data(), n, s); R.cap_end.upload(h_end.data(), n, s);
-        R.p.upload(h_id.data(), n, s); R.q.upload(h_id.data(), n, s);
-        R.nnz.upload(h_nnz.data(), n, s); R.lead.upload(h_lead.data(), n, s); R.slot.upload(h_slot.data(), n, s);
-        R.cols.upload(h_cols.data(), nnz, s);
-        if (nnz > 0) hip_check(hipMemcpyAsync(R.vals.p, d_Rx, (size_t)nnz * vs * sizeof(double), hipMemcpyDeviceToDevice, s), "row values");
-        const unsigned long long tops[2] = {(unsigned long long)nnz, 0ull};
-        R.tops.upload(tops, 2, s);
         RowLuArgs a{};
         a.n = nn; a.strategy = strategy; a.drop_tol = drop_tol; a.pivot_rel = pivot_rel;
         a.off = R.off.p; a.base = R.base.p; a.cap_end = R.cap_end.p;
@@ -149,19 +185,40 @@ int row_factor_device(int64_t n, const int64_t* Rp, const int64_t* Ri, const dou
         a.cols = R.cols.p; a.vals = R.vals.p; a.pool_cap = pool_cap; a.pool_top = R.tops.p;
         a.log_row = R.log_row.p; a.log_step = R.log_step.p; a.log_val = R.log_val.p; a.log_cap = log_cap; a.log_top = R.tops.p + 1;
         a.cand = R.cand.p; a.scratch_col = R.scratch_col.p; a.scratch_val = R.scratch_val.p; a.status = R.status.p;
-        if (n > 0) row_lu(vs, a, s);
-        else st = RowLuStatus{};
-        if (n > 0) R.status.download(&st, 1, s);
+        a.k0 = k0; a.resume = resume;
+        if (n > 0) {
+            row_lu(vs, a, s);
+            runs++;
+            R.status.download(&st, 1, s);
+            merges += st.merges;
+            merge_bytes += st.merge_bytes;
+        } else {
+            st = RowLuStatus{};
+        }
         dev_sync(s);
-        if (st.code == RL_POOL && attempt < 12) { pool_cap *= 2; continue; }
-        if (st.code == RL_LOG && attempt < 12) { log_cap *= 2; continue; }
+        if (st.code == RL_POOL || st.code == RL_LOG) {   // grown in place, the run resumes at the step it stopped
+            if (runs > 64) break;
+            k0 = (int32_t)st.step;
+            resume = st.phase;
+            continue;
+        }
+        if (st.code == RL_RESTART && restarts < 8) {       // a pool ran out within a step: from the start, larger
+            restarts++;
+            pool_cap *= 2;
+            log_cap *= 2;
+            fresh = true;
+            continue;
+        }
         break;
     }
     if (st.code == RL_SINGULAR) {
         errf(err, errlen, "Near-zero or zero pivot %.6e at step %ld (threshold %.6e). Matrix is singular.", st.best, (long)st.step, st.thr);
         return FPLUS_ERR_SINGULAR;
     }
-    if (st.code == RL_POOL || st.code == RL_LOG) { errf(err, errlen, "out of memory (the row kernel's pools)"); return FPLUS_ERR_NOMEM; }
+    if (st.code == RL_POOL || st.code == RL_LOG || st.code == RL_RESTART) {
+        errf(err, errlen, "out of memory (the row kernel's pools)");
+        return FPLUS_ERR_NOMEM;
+    }
     if (st.code != RL_OK) {
         errf(err, errlen, "Pivot entry missing at step %ld after swap.", (long)st.step);
         return FPLUS_ERR_INTERNAL;
@@ -245,10 +302,20 @@ int row_factor_device(int64_t n, const int64_t* Rp, const int64_t* Ri, const dou
         errf(err, errlen, "out of memory");
         return FPLUS_ERR_NOMEM;
     }
+    t_row_info.runs = runs;
+    t_row_info.restarts = restarts;
+    t_row_info.merges = merges;
+    t_row_info.merge_bytes = merge_bytes;
+    t_row_info.pool_entries = st.pool_used;
+    t_row_info.multipliers = st.log_used;
+    t_row_info.t_device = t1 - t0;
+    t_row_info.t_total = now_s() - t0;
+    t_row_info_set = true;
     if (verbose >= 1) {
-        printf("[FPLUS] LU factorisation complete: nnz(L)=%ld, nnz(U)=%ld, total nnz=%ld, time=%.3fs (device elimination %.3fs, "
-               "pool %ld of %ld entries, %ld multipliers)\n", (long)out->Lp[n], (long)out->Up[n], (long)(out->Lp[n] + out->Up[n]),
-               now_s() - t0, t1 - t0, (long)st.pool_used, (long)pool_cap, (long)st.log_used);
+        printf("[FPLUS] LU factorisation complete: nnz(L)=%ld, nnz(U)=%ld, total nnz=%ld, time=%.3fs (device elimination %.3fs in %d "
+               "runs, %d from the start; pool %ld of %ld entries, %ld multipliers)\n", (long)out->Lp[n], (long)out->Up[n],
+               (long)(out->Lp[n] + out->Up[n]), now_s() - t0, t1 - t0, runs, restarts, (long)st.pool_used, (long)pool_cap,
+               (long)st.log_used);
         if (st.nrep > 0)
             printf("[FPLUS] ILU: %ld zero / negligible pivots replaced by drop_tol * max|a_ij| = %.3e\n", (long)st.nrep, drop_tol * st.max0);
     }
@@ -381,6 +448,13 @@ int fplus_factor_row_sym(const fplus_symbolic* sym, const int64_t* Ap, const int
     }
     return fplus_factor_row_perm(n, Ap, Ai, Ax, is_complex, sym->row_perm.data(), sym->col_perm.data(), equilibrate, ruiz_max_iter,
                                  ruiz_tol, verbose, drop_tol, pivot_strategy, pivot_rel, out, Dr, Dc, err, errlen);
+}
+
+int fplus_amd_row_stats(fplus_amd_row_info* info)
+{
+    if (!info || !t_row_info_set) return FPLUS_ERR_BADARG;
+    *info = t_row_info;
+    return FPLUS_OK;
 }
 
 }  // extern "C"

@@ -39,6 +39,18 @@ typedef struct {
 } fplus_amd_stats;
 int fplus_amd_numeric_stats(const fplus_numeric *num, fplus_amd_stats *st);
 
+/* The row kernel's last factorization on the calling thread (fplus_factor_row, _perm, _sym, or fplus_factor with rook
+ * / full pivoting or drop_tol > 0): its device runs (the first, those that resumed after growing the entry pool or the
+ * multiplier log, and those that started over), the seconds of the elimination on the device and of the whole call,
+ * the row merges and their algorithmic bytes (each merged row's and the pivot row's entries read, the kept entries and
+ * the multiplier written, a column index and a value each), and the pool entries and multipliers it used.  For
+ * benchmarks and logs; FPLUS_ERR_BADARG when the thread has made none. */
+typedef struct {
+    int64_t runs, restarts, merges, pool_entries, multipliers;
+    double t_device, t_total, merge_bytes;
+} fplus_amd_row_info;
+int fplus_amd_row_stats(fplus_amd_row_info *info);
+
 /* Device-resident inputs, for callers that keep the values and right-hand sides in HBM (refactorization loops,
  * benchmarks): fplus_factor with Ax a device pointer (the analysed pattern Ap, Ai on the host), and fplus_solve on a
  * device array b (column-major, leading dimension ldb), in place. */

@@ -9,7 +9,7 @@ HERE    := $(dir $(abspath $(lastword $(MAKEFILE_LIST))))
 OUT     := $(HERE)_ref
 CFLAGS  := -std=c99 -O3 -fno-math-errno -fno-signed-zeros -DNDEBUG -fPIC -I$(REF)/include -I$(REF)/src
 
-all: $(OUT)/libfplus_ref.so
+all: $(OUT)/libfplus_ref.so $(OUT)/libfplus_ref_row_omp.so
 
 $(OUT)/libfplus_ref.so: $(REF)/src/equil.c $(REF)/src/factor_sn.c $(REF)/src/factor_row.c $(REF)/src/solve.c $(REF)/src/solve_plan.c
 	mkdir -p $(OUT)
@@ -20,5 +20,12 @@ $(OUT)/libfplus_ref.so: $(REF)/src/equil.c $(REF)/src/factor_sn.c $(REF)/src/fac
 	$(CC) $(CFLAGS) -ffp-contract=off -c $(REF)/src/solve_plan.c -o $(OUT)/solve_plan.o
 	$(CC) -shared -o $@ $(OUT)/equil.o $(OUT)/factor_sn.o $(OUT)/factor_row.o $(OUT)/solve.o $(OUT)/solve_plan.o -lm
 	rm -f $(OUT)/*.o
+
+# the row kernel as the reference ships it for timing: OpenMP on (its default, CMakeLists.txt:5), vectorised for any
+# x86-64 host with AVX2 (the reference's -march=native would tie the library to this container's CPU); the CPU baseline
+# of the row-kernel benchmark
+$(OUT)/libfplus_ref_row_omp.so: $(REF)/src/factor_row.c
+	mkdir -p $(OUT)
+	$(CC) $(CFLAGS) -march=x86-64-v3 -fopenmp -shared -o $@ $(REF)/src/factor_row.c -lm
 
 .PHONY: all
