@@ -82,7 +82,8 @@ inline double absval(const double* x, int vs) { return vs == 2 ? std::hypot(x[0]
 struct RowRun {
     DevBuf<int64_t> off, base, cap_end, p, q;
     DevBuf<int32_t> nnz, lead, slot, cols, log_row, log_step, cand, scratch_col;
-    DevBuf<double> vals, log_val, scratch_val;
+    DevBuf<double> vals, log_val, scratch_val, rmax;
+    DevBuf<int32_t> rmaxc;
     DevBuf<unsigned long long> tops;   // pool top, log top
     DevBuf<RowLuStatus> status;
 };
@@ -136,6 +137,8 @@ int row_factor_device(int64_t n, const int64_t* Rp, const int64_t* Ri, const dou
         R.log_val.alloc((size_t)log_cap * vs);
         R.tops.alloc(2);
         R.status.alloc(1);
+        R.rmax.alloc(n);
+        R.rmaxc.alloc(n);
     };
     // a pool grown in place: the entries (or multipliers) so far copied into a larger buffer
     auto grow = [&](auto& buf, int64_t used, size_t per, size_t cap) {
@@ -185,6 +188,7 @@ int row_factor_device(int64_t n, const int64_t* Rp, const int64_t* Ri, const dou
         a.cols = R.cols.p; a.vals = R.vals.p; a.pool_cap = pool_cap; a.pool_top = R.tops.p;
         a.log_row = R.log_row.p; a.log_step = R.log_step.p; a.log_val = R.log_val.p; a.log_cap = log_cap; a.log_top = R.tops.p + 1;
         a.cand = R.cand.p; a.scratch_col = R.scratch_col.p; a.scratch_val = R.scratch_val.p; a.status = R.status.p;
+        a.rmax = R.rmax.p; a.rmaxc = R.rmaxc.p;
         a.k0 = k0; a.resume = resume;
         if (n > 0) {
             row_lu(vs, a, s);
@@ -244,25 +248,36 @@ int row_factor_device(int64_t n, const int64_t* Rp, const int64_t* Ri, const dou
         out->n = n;
         out->is_complex = vs == 2;
         out->csc = 0;
-        // U (src/factor_row.c:1081-1133 with upper = 1)
+        // U (src/factor_row.c:1081-1133 with upper = 1).  With column exchanges the rows hold input column ids:
+        // mapped to the final positions and sorted
+        std::vector<int64_t> qinv;
+        if (strategy != FPLUS_PIVOT_PARTIAL) {
+            qinv.resize(n);
+            for (int64_t j = 0; j < n; j++) qinv[q[j]] = j;
+        }
+        auto ucol = [&](int64_t t) -> int64_t { return qinv.empty() ? pc[t] : qinv[pc[t]]; };
         out->Up = cmalloc<int64_t>(n + 1);
         out->Up[0] = 0;
         for (int64_t i = 0; i < n; i++) {
             const int32_t r = slot[i];
             int64_t c = 0;
             for (int64_t t = d_off[r]; t < d_off[r] + d_nnz[r]; t++)
-                if (!dropped(&pv[t * vs], pc[t], i) && pc[t] >= i) c++;
+                if (!dropped(&pv[t * vs], ucol(t), i) && ucol(t) >= i) c++;
             out->Up[i + 1] = out->Up[i] + c;
         }
         out->Ui = cmalloc<int64_t>(out->Up[n]);
         double* Ux = cmalloc<double>((size_t)out->Up[n] * vs);
         out->Ux = Ux;
+        std::vector<std::pair<int64_t, int64_t>> row;
         for (int64_t i = 0, o = 0; i < n; i++) {
             const int32_t r = slot[i];
-            for (int64_t t = d_off[r]; t < d_off[r] + d_nnz[r]; t++) {
-                if (dropped(&pv[t * vs], pc[t], i) || pc[t] < i) continue;
-                out->Ui[o] = pc[t];
-                for (int v = 0; v < vs; v++) Ux[o * vs + v] = pv[t * vs + v];
+            row.clear();
+            for (int64_t t = d_off[r]; t < d_off[r] + d_nnz[r]; t++)
+                if (!dropped(&pv[t * vs], ucol(t), i) && ucol(t) >= i) row.push_back({ucol(t), t});
+            if (!qinv.empty()) std::sort(row.begin(), row.end());
+            for (const auto& e : row) {
+                out->Ui[o] = e.first;
+                for (int v = 0; v < vs; v++) Ux[o * vs + v] = pv[e.second * vs + v];
                 o++;
             }
         }

@@ -30,7 +30,9 @@ enum { RL_OK = 0, RL_SINGULAR = 1, RL_POOL = 2, RL_LOG = 3, RL_INTERNAL = 4, RL_
 // The device state of one system.  Rows are addressed by a physical id r (the input row r at the start); slot[i] is
 // the physical row at position i, p[i] / q[i] the input row / column at position i.  Row r's entries, columns
 // ascending, sit at cols/vals[off[r] .. off[r] + nnz[r]) inside its block [base[r], cap_end[r]) of the entry pool;
-// lead[i] is the first column of the row at position i (RL_NONE when empty).  Blocks that outgrow their room move to fresh space at the pool's
+// lead[i] is the first column of the row at position i (RL_NONE when empty).  With column exchanges (full and rook
+// pivoting) the rows finished at earlier steps hold input column ids (q of their step), so that an exchange relabels
+// the active rows only; the host maps them to the final positions.  Blocks that outgrow their room move to fresh space at the pool's
 // top (pool_top).  Each multiplier m(i, k) is logged as (physical row, step, value) at log_top.  scratch: n entries
 // per wavefront (the union of two rows before its drops).  vals / log_val / scratch_val hold T (double or complex).
 struct RowLuArgs {
@@ -52,6 +54,8 @@ struct RowLuArgs {
     int32_t* scratch_col;
     double* scratch_val;
     RowLuStatus* status;
+    double* rmax;                         // full pivoting: the largest |entry| of the row at each position (-1 empty)
+    int32_t* rmaxc;                       // and the first column holding it
     int32_t k0;                           // the first step of this run (0, or the step a resumed run continues)
     int32_t resume;                       // 1: step k0 starts at its elimination (status holds max0, thr, nrep)
 };

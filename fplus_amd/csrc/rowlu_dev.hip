@@ -101,8 +101,32 @@ __device__ inline int32_t find_col(const int32_t* c, int32_t cnt, int32_t x)
 // a value formed by the elimination stays when nonzero and, for the incomplete factorization, above drop_tol
 __device__ inline bool keep_formed(double av, double dt) { return av > 0.0 && (dt == 0.0 || av > dt); }
 
-// columns j1 < j2 exchange their labels in the row at position i (src/factor_row.c:215-238)
-template <class T> __device__ void col_swap_row(const RowLuArgs& a, int32_t i, int32_t j1, int32_t j2)
+// the largest |entry| of a row and the first column holding it (-1 and RL_NONE for an empty row), one thread
+template <class T> __device__ void row_max(const int32_t* c, const T* v, int32_t nz, double& m, int32_t& mc)
+{
+    m = -1.0;
+    mc = RL_NONE;
+    for (int32_t t = 0; t < nz; t++) {
+        const double av = Sc<T>::absval(v[t]);
+        if (av > m) { m = av; mc = c[t]; }
+    }
+}
+
+// the same over a row by one wavefront (the same in every lane)
+template <class T> __device__ void wave_row_max(const int32_t* c, const T* v, int32_t nz, double& m, int32_t& mc)
+{
+    Best<T> b = none<T>();
+    for (int32_t t = threadIdx.x & 63; t < nz; t += 64) {
+        const double av = Sc<T>::absval(v[t]);
+        if (better(av, c[t], b.a, b.k)) b = Best<T>{av, c[t], Sc<T>::zero()};
+    }
+    wave_best(b);
+    m = b.a;
+    mc = b.k == INT64_MAX ? RL_NONE : (int32_t)b.k;
+}
+
+// columns j1 < j2 exchange their labels in the active row at position i (src/factor_row.c:215-238)
+template <class T, int S> __device__ void col_swap_row(const RowLuArgs& a, int32_t i, int32_t j1, int32_t j2)
 {
     const int32_t r = a.slot[i];
     const int32_t nz = a.nnz[r];
@@ -115,6 +139,7 @@ template <class T> __device__ void col_swap_row(const RowLuArgs& a, int32_t i, i
         const T x = v[t1];
         v[t1] = v[t2];
         v[t2] = x;
+        if (S == 0) row_max(c, v, nz, a.rmax[i], a.rmaxc[i]);
         return;
     }
     int32_t t = t1 >= 0 ? t1 : t2;
@@ -125,6 +150,7 @@ template <class T> __device__ void col_swap_row(const RowLuArgs& a, int32_t i, i
     c[t] = col;
     v[t] = x;
     a.lead[i] = c[0];
+    if (S == 0) row_max(c, v, nz, a.rmax[i], a.rmaxc[i]);
 }
 
 // an incomplete-LU breakdown: the entry (k, k) of the row at position k set to rep, inserted when absent (one thread)
@@ -164,7 +190,7 @@ template <class T> __device__ bool ilu_replace(const RowLuArgs& a, int32_t k, do
 // the log.  Rows of at most 64 entries past the pivot column merged with a pivot row of at most 64 are merged in
 // registers (the union's kept positions as a 128-bit mask, each entry's place its popcount); longer ones through the
 // wavefront's scratch in the pool.
-template <class T>
+template <class T, int S>
 __device__ void eliminate_row(const RowLuArgs& a, int32_t ip, int32_t k, T piv, const int32_t* Bc, const T* Bv, int32_t nB,
                               int32_t* scols, T* svals, int32_t* sA, unsigned long long* skeep, unsigned long long* pool_top,
                               unsigned long long* log_top, unsigned long long* s_work, int32_t* s_status)
@@ -188,6 +214,12 @@ __device__ void eliminate_row(const RowLuArgs& a, int32_t ip, int32_t k, T piv, 
             a.off[r] = o + 1;
             a.nnz[r] = nz - 1;
             a.lead[ip] = nz > 1 ? c0 : RL_NONE;
+        }
+        if (S == 0) {
+            double mx;
+            int32_t mc;
+            wave_row_max(cols + oA, vals + oA, nA, mx, mc);
+            if (lane == 0) { a.rmax[ip] = mx; a.rmaxc[ip] = mc; }
         }
         return;
     }
@@ -266,6 +298,16 @@ __device__ void eliminate_row(const RowLuArgs& a, int32_t ip, int32_t k, T piv, 
             const uint64_t fa = __ballot(kA && uA == u0), fb = __ballot(kB && uB == u0);
             first = fa ? __shfl(c0, __ffsll((unsigned long long)fa) - 1) : __shfl(cB, __ffsll((unsigned long long)fb) - 1);
         }
+        if (S == 0) {   // the row's largest kept entry, first column among equals
+            Best<T> b = none<T>();
+            if (kA) b = Best<T>{Sc<T>::absval(wA), c0, Sc<T>::zero()};
+            if (kB) {
+                const double av = Sc<T>::absval(wB);
+                if (better(av, cB, b.a, b.k)) b = Best<T>{av, cB, Sc<T>::zero()};
+            }
+            wave_best(b);
+            if (lane == 0) { a.rmax[ip] = b.a; a.rmaxc[ip] = b.k == INT64_MAX ? RL_NONE : (int32_t)b.k; }
+        }
         if (lane == 0) {
             a.off[r] = d;
             a.nnz[r] = kept;
@@ -334,6 +376,7 @@ __device__ void eliminate_row(const RowLuArgs& a, int32_t ip, int32_t k, T piv, 
     if (d < 0) return;
     if (lane == 0) atomicAdd(s_work, (unsigned long long)(nA + nB + kept + 1));
     int32_t w = 0, first = RL_NONE;
+    Best<T> rb = none<T>();
     for (int32_t b0 = 0; b0 < U; b0 += 64) {
         const int32_t u = b0 + lane;
         const bool valid = u < U;
@@ -345,9 +388,15 @@ __device__ void eliminate_row(const RowLuArgs& a, int32_t ip, int32_t k, T piv, 
             const int64_t dd = d + w + __popcll(mm & below);
             cols[dd] = c;
             vals[dd] = v;
+            const double av = Sc<T>::absval(v);
+            if (better(av, c, rb.a, rb.k)) rb = Best<T>{av, c, Sc<T>::zero()};
         }
         if (w == 0 && mm) first = __shfl(c, __ffsll((unsigned long long)mm) - 1);
         w += __popcll(mm);
+    }
+    if (S == 0) {
+        wave_best(rb);
+        if (lane == 0) { a.rmax[ip] = rb.a; a.rmaxc[ip] = rb.k == INT64_MAX ? RL_NONE : (int32_t)rb.k; }
     }
     if (lane == 0) {
         a.off[r] = d;
@@ -356,14 +405,15 @@ __device__ void eliminate_row(const RowLuArgs& a, int32_t ip, int32_t k, T piv, 
     }
 }
 
-template <class T> __global__ __launch_bounds__(RL_THREADS) void k_row_lu(RowLuArgs a)
+// S: the pivot strategy (0 full, 1 partial, 2 rook)
+template <class T, int S> __global__ __launch_bounds__(RL_THREADS) void k_row_lu(RowLuArgs a)
 {
     __shared__ int32_t sBc[RL_BMAX];
     __shared__ T sBv[RL_BMAX];
     __shared__ double s_a[RL_WAVES + 1];
     __shared__ int64_t s_k[RL_WAVES + 1];
     __shared__ T s_v[RL_WAVES + 1];
-    __shared__ int32_t s_ncand, s_status, s_rk;
+    __shared__ int32_t s_ncand, s_status, s_rk, s_npl, s_plcol;
     __shared__ int32_t sA[RL_WAVES][RL_ASTAGE];
     __shared__ unsigned long long s_keep[RL_WAVES][2];
     __shared__ unsigned long long s_pool_top, s_log_top, s_cand_nnz, s_work, s_log0;
@@ -388,6 +438,8 @@ template <class T> __global__ __launch_bounds__(RL_THREADS) void k_row_lu(RowLuA
             for (int32_t t = 0; t < nz; t++) mx = fmax(mx, Sc<T>::absval(vals[o + t]));
         }
         in_max = block_max(mx, s_a);
+        if (S == 0)
+            for (int32_t r = tid; r < n; r += RL_THREADS) row_max(cols + a.off[r], vals + a.off[r], a.nnz[r], a.rmax[r], a.rmaxc[r]);
     } else {
         in_max = a.status->in_max;
         max0 = a.status->max0;
@@ -404,14 +456,14 @@ template <class T> __global__ __launch_bounds__(RL_THREADS) void k_row_lu(RowLuA
 
     for (int32_t k = a.k0; k < n; k++) {
         const bool elim_only = k == a.k0 && a.resume;
-        if (tid == 0) { s_ncand = 0; s_cand_nnz = 0; }
+        if (tid == 0) { s_ncand = 0; s_cand_nnz = 0; s_npl = 0; s_plcol = -1; }
         __syncthreads();
         if (!elim_only) {
             // ---- 1. pivot search (src/factor_row.c:273-342, :423-479, :558-658)
             int64_t pi = -1, pj = -1;
             bool found = false;
             T pv = Sc<T>::zero();
-            if (a.strategy == 1) {                                   // partial: the rows led by column k
+            if (S == 1) {                                   // partial: the rows led by column k
                 Best<T> b = none<T>();
                 unsigned long long cnz = 0;
                 for (int32_t i0 = (k & ~3) + 4 * tid; i0 < n; i0 += 4 * RL_THREADS) {
@@ -440,23 +492,31 @@ template <class T> __global__ __launch_bounds__(RL_THREADS) void k_row_lu(RowLuA
                 if (b.k != INT64_MAX) { pi = b.k; pv = b.v; }
                 pj = k;
                 found = pi >= 0 && b.a > 0.0;
-            } else if (a.strategy == 0) {                            // full: every active entry, rows then columns
+            } else if (S == 0) {
+                // full: every active entry, rows then columns -- each active row's largest entry and its first column
+                // are kept up to date by the merges and exchanges, so the search reads one maximum per row
                 Best<T> b = none<T>();
                 for (int32_t i = k + tid; i < n; i += RL_THREADS) {
-                    const int32_t r = a.slot[i];
-                    const int64_t o = a.off[r];
-                    const int32_t nz = a.nnz[r];
-                    for (int32_t t = 0; t < nz; t++) {
-                        const int32_t c = cols[o + t];
-                        if (c < k) continue;
-                        const T v = vals[o + t];
-                        const double av = Sc<T>::absval(v);
-                        const int64_t key = (int64_t)i * n + c;
-                        if (better(av, key, b.a, b.k)) b = Best<T>{av, key, v};
-                    }
+                    const double av = a.rmax[i];
+                    if (av < 0.0) continue;
+                    const int64_t key = (int64_t)i * n + a.rmaxc[i];
+                    if (better(av, key, b.a, b.k)) b = Best<T>{av, key, Sc<T>::zero()};
                 }
                 b = block_best(b, s_a, s_k, s_v);
-                if (b.k != INT64_MAX) { pi = b.k / n; pj = b.k % n; pv = b.v; }
+                if (b.k != INT64_MAX) {
+                    pi = b.k / n;
+                    pj = b.k % n;
+                    const int32_t r = a.slot[pi];
+                    const int64_t o = a.off[r];
+                    if (tid == 0) {
+                        const int32_t t = find_col(cols + o, a.nnz[r], (int32_t)pj);
+                        if (t < 0) s_status = RL_INTERNAL;
+                        s_v[RL_WAVES] = t < 0 ? Sc<T>::zero() : vals[o + t];
+                    }
+                    __syncthreads();
+                    pv = s_v[RL_WAVES];
+                    __syncthreads();
+                }
                 found = pi >= 0 && b.a > 0.0;
             } else {                                                  // rook: column and row scans in turn, 8 rounds
                 int64_t ci = k, ri = -1;
@@ -464,6 +524,10 @@ template <class T> __global__ __launch_bounds__(RL_THREADS) void k_row_lu(RowLuA
                 T bval = Sc<T>::zero();
                 bool done = false;
                 for (int it = 0; it < 8; it++) {
+                    if (it > 0) {                                      // the rows of this scan's column, listed
+                        if (tid == 0) { s_npl = 0; s_plcol = (int32_t)ci; }
+                        __syncthreads();
+                    }
                     Best<T> b = none<T>();
                     for (int32_t i = k + tid; i < n; i += RL_THREADS) {
                         int64_t o;
@@ -476,6 +540,7 @@ template <class T> __global__ __launch_bounds__(RL_THREADS) void k_row_lu(RowLuA
                             o = a.off[r];
                             t = find_col(cols + o, a.nnz[r], (int32_t)ci);
                             if (t < 0) continue;
+                            a.cand[atomicAdd(&s_npl, 1)] = i;
                         }
                         const T v = vals[o + t];
                         const double av = Sc<T>::absval(v);
@@ -522,16 +587,35 @@ template <class T> __global__ __launch_bounds__(RL_THREADS) void k_row_lu(RowLuA
             if (tid == 0 && pi != k) {
                 const int32_t t = a.slot[k]; a.slot[k] = a.slot[pi]; a.slot[pi] = t;
                 const int32_t l = a.lead[k]; a.lead[k] = a.lead[pi]; a.lead[pi] = l;
+                if (S == 0) {
+                    const double m = a.rmax[k]; a.rmax[k] = a.rmax[pi]; a.rmax[pi] = m;
+                    const int32_t mc = a.rmaxc[k]; a.rmaxc[k] = a.rmaxc[pi]; a.rmaxc[pi] = mc;
+                }
                 const int64_t x = a.p[k]; a.p[k] = a.p[pi]; a.p[pi] = x;
             }
-            if (a.strategy != 1) __syncthreads();
+            if (S != 1) __syncthreads();
             if (pj != k) {
-                for (int32_t i = tid; i < n; i += RL_THREADS) col_swap_row<T>(a, i, k, (int32_t)pj);
+                // the active rows only (finished rows hold input column ids)
                 if (tid == 0) { const int64_t x = a.q[k]; a.q[k] = a.q[pj]; a.q[pj] = x; }
+                if (S == 2 && s_plcol == (int32_t)pj) {
+                    // rook, the last column scan was pj's: the rows holding column k (led by it), then those of the
+                    // scan's list that hold pj but not k
+                    for (int32_t i = k + tid; i < n; i += RL_THREADS)
+                        if (a.lead[i] == k) col_swap_row<T, S>(a, i, k, (int32_t)pj);
+                    __syncthreads();
+                    const int32_t npl = s_npl;
+                    for (int32_t c = tid; c < npl; c += RL_THREADS) {
+                        int32_t i = a.cand[c];
+                        i = i == (int32_t)pi ? k : (i == k ? (int32_t)pi : i);   // the positions after the row exchange
+                        if (a.lead[i] != k) col_swap_row<T, S>(a, i, k, (int32_t)pj);
+                    }
+                } else {
+                    for (int32_t i = k + tid; i < n; i += RL_THREADS) col_swap_row<T, S>(a, i, k, (int32_t)pj);
+                }
                 __syncthreads();
             }
         }
-        if (a.strategy != 1 || elim_only) {                       // the rows led by column k after the exchanges
+        if (S != 1 || elim_only) {                       // the rows led by column k after the exchanges
             unsigned long long cnz = 0;
             for (int32_t i = k + 1 + tid; i < n; i += RL_THREADS)
                 if (a.lead[i] == k) {
@@ -593,9 +677,11 @@ template <class T> __global__ __launch_bounds__(RL_THREADS) void k_row_lu(RowLuA
         for (int32_t c = wave; c < ncand; c += RL_WAVES) {
             const int32_t i = a.cand[c];
             if (i == k || a.lead[i] != k) continue;
-            eliminate_row<T>(a, i, k, piv, Bc, Bv, nB, scols, svals, sA[wave], s_keep[wave], &s_pool_top, &s_log_top, &s_work,
+            eliminate_row<T, S>(a, i, k, piv, Bc, Bv, nB, scols, svals, sA[wave], s_keep[wave], &s_pool_top, &s_log_top, &s_work,
                              &s_status);
         }
+        if (S != 1)   // the finished row keeps input column ids from here on; later exchanges skip it
+            for (int32_t t = tid; t <= nB; t += RL_THREADS) cols[ok + t] = (int32_t)a.q[cols[ok + t]];
         __syncthreads();
         if (s_status != RL_OK) { kstop = k; break; }
     }
@@ -623,8 +709,16 @@ template <class T> __global__ __launch_bounds__(RL_THREADS) void k_row_lu(RowLuA
 void row_lu(int vs, const RowLuArgs& a, hipStream_t s)
 {
     ProfScope ps("row_lu", 0.0, s);
-    if (vs == 2) hipLaunchKernelGGL(k_row_lu<Zd>, dim3(1), dim3(RL_THREADS), 0, s, a);
-    else hipLaunchKernelGGL(k_row_lu<double>, dim3(1), dim3(RL_THREADS), 0, s, a);
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(1), dim3(RL_THREADS), 0, s, a); };
+    if (vs == 2) {
+        if (a.strategy == 0) go(k_row_lu<Zd, 0>);
+        else if (a.strategy == 1) go(k_row_lu<Zd, 1>);
+        else go(k_row_lu<Zd, 2>);
+    } else {
+        if (a.strategy == 0) go(k_row_lu<double, 0>);
+        else if (a.strategy == 1) go(k_row_lu<double, 1>);
+        else go(k_row_lu<double, 2>);
+    }
 }
 
 }  // namespace fpa
