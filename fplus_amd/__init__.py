@@ -17,7 +17,7 @@ from ._lib import (FPLUS_ORDER_AMD_ATA, FPLUS_ORDER_AUTO, FPLUS_ORDER_COLAMD, FP
 
 __all__ = ["SparseLU", "sparse_lu_full_pivot", "lu_solve", "lu_residual", "clear_ordering_cache", "Symbolic", "Numeric",
            "SolvePlan", "default_options", "amd_options", "FplusError", "ruiz", "factor_sn", "factor_sn_perm", "factor_sn_sym",
-           "factor_row", "factor_row_sym", "row_stats", "tri_solve", "tune", "Selector", "log_blocks"]
+           "factor_row", "factor_row_sym", "factor_row_batch", "row_stats", "tri_solve", "tune", "Selector", "log_blocks"]
 
 _I64P = C.POINTER(C.c_int64)
 _DP = C.POINTER(C.c_double)
@@ -343,6 +343,33 @@ def factor_row(n, Rp, Ri, Rx, drop_tol=0.0, pivot_strategy="full", pivot_rel=FPL
                                 err, 512)
     check(rc, err)
     return _factors_dict(f, n, cplx)
+
+
+def factor_row_batch(systems, drop_tol=0.0, pivot_strategy="full", pivot_rel=FPLUS_PIVOT_REL_DEFAULT):
+    """fplus_amd_factor_row_batch: independent CSR systems [(n, Rp, Ri, Rx), ...] of one scalar type through the row
+    kernel in shared launches, one workgroup per system; a list with, per system, the dict factor_row returns or the
+    FplusError it would raise"""
+    k = len(systems)
+    cplx = int(any(np.iscomplexobj(s[3]) for s in systems))
+    dt = np.complex128 if cplx else np.float64
+    keep = [(int(n), np.ascontiguousarray(Rp, np.int64), np.ascontiguousarray(Ri, np.int64), np.ascontiguousarray(Rx, dt))
+            for n, Rp, Ri, Rx in systems]
+    ns = np.array([s[0] for s in keep] or [0], np.int64)
+    P = (C.c_void_p * max(k, 1))(*[s[1].ctypes.data for s in keep])
+    I = (C.c_void_p * max(k, 1))(*[s[2].ctypes.data for s in keep])
+    X = (C.c_void_p * max(k, 1))(*[s[3].ctypes.data for s in keep])
+    out = (Factors * max(k, 1))()
+    rc = (C.c_int * max(k, 1))()
+    err = C.create_string_buffer(512)
+    check(lib().fplus_amd_factor_row_batch(k, _p64(ns), P, I, X, cplx, float(drop_tol), _PIVOT_CODES[pivot_strategy],
+                                           float(pivot_rel), out, rc, err, 512), err)
+    res = []
+    for i in range(k):
+        if rc[i] != 0:
+            res.append(FplusError(rc[i], f"system {i}"))
+        else:
+            res.append(_factors_dict(out[i], keep[i][0], cplx))
+    return res
 
 
 def factor_row_sym(sym, Ap, Ai, Ax, drop_tol=0.0, pivot_strategy="full", equilibrate=True, pivot_rel=FPLUS_PIVOT_REL_DEFAULT,

@@ -135,6 +135,8 @@ SIGNATURES = {
     "fplus_amd_dev_sync": (C.c_int, []),
     "fplus_amd_thread_stream": (C.c_int, [C.c_int]),
     "fplus_amd_row_stats": (C.c_int, [VP]),
+    "fplus_amd_factor_row_batch": (C.c_int, [C.c_int64, I64P, VP, VP, VP, C.c_int, C.c_double, C.c_int, C.c_double, VP,
+                                             C.POINTER(C.c_int), C.c_char_p, C.c_size_t]),
     "fplus_amd_profile": (C.c_int, [C.c_int]),
     "fplus_amd_profile_get": (C.c_int, [C.c_char_p, DP, I64P, DP]),
     "fplus_amd_tune": (C.c_int, [C.c_char_p, C.c_int64]),

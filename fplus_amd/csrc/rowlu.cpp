@@ -85,7 +85,6 @@ struct RowRun {
     DevBuf<double> vals, log_val, scratch_val, rmax;
     DevBuf<int32_t> rmaxc;
     DevBuf<unsigned long long> tops;   // pool top, log top
-    DevBuf<RowLuStatus> status;
 };
 
 }  // namespace
@@ -99,241 +98,354 @@ void factors_to_csc(fplus_factors* f)
     f->csc = 1;
 }
 
-int row_factor_device(int64_t n, const int64_t* Rp, const int64_t* Ri, const double* d_Rx, int vs, double drop_tol, int strategy,
-                      double pivot_rel, int verbose, fplus_factors* out, char* err, size_t errlen)
-{
-    std::memset(out, 0, sizeof(*out));
-    if (n >= (int64_t)RL_NONE) { errf(err, errlen, "the row kernel takes n < 2^31 - 1 (got %ld)", (long)n); return FPLUS_ERR_BADARG; }
-    const int64_t nnz = Rp[n];
-    const char* sname = strategy == FPLUS_PIVOT_FULL ? "full" : strategy == FPLUS_PIVOT_PARTIAL ? "partial" : "rook";
-    if (verbose >= 1)
-        printf("[FPLUS] LU factorisation start: n=%ld, nnz(A)=%ld, %s, drop_tol=%.1e, pivot=%s (row kernel, device)\n", (long)n,
-               (long)nnz, vs == 2 ? "complex128" : "float64", drop_tol, sname);
-    hipStream_t s = devctx().stream;
-    const int32_t nn = (int32_t)n;
-    // the rows as given: row i at [Rp[i], Rp[i + 1]) of the pool, its block exactly as long
-    std::vector<int64_t> h_off(Rp, Rp + n), h_end(Rp + 1, Rp + n + 1), h_id(n);
-    std::vector<int32_t> h_nnz(n), h_lead(n), h_slot(n), h_cols(nnz);
-    for (int64_t i = 0; i < n; i++) {
-        h_nnz[i] = (int32_t)(Rp[i + 1] - Rp[i]);
-        h_lead[i] = h_nnz[i] ? (int32_t)Ri[Rp[i]] : RL_NONE;
-        h_slot[i] = (int32_t)i;
-        h_id[i] = i;
-    }
-    for (int64_t q = 0; q < nnz; q++) h_cols[q] = (int32_t)Ri[q];
-    int64_t pool_cap = 16 * nnz + 64 * n + 1024, log_cap = 4 * nnz + 2 * n + 1024;
+// one system's state across the device runs of its elimination: the host layout of its rows, its device buffers,
+// what the last run reported
+struct RowSystem {
+    int64_t n = 0, nnz = 0;
+    int vs = 1, strategy = 1;
+    double drop_tol = 0.0, pivot_rel = 0.0;
+    const int64_t* Rp = nullptr;
+    const int64_t* Ri = nullptr;
+    const double* d_Rx = nullptr;
+    RowLuStatus* d_status = nullptr;   // the run's report (its own buffer, or a slot of a batch's array)
+    std::vector<int64_t> h_off, h_end, h_id;
+    std::vector<int32_t> h_nnz, h_lead, h_slot, h_cols;
+    int64_t pool_cap = 0, log_cap = 0;
     RowRun R;
     RowLuStatus st{};
-    const double t0 = now_s();
     int runs = 0, restarts = 0;
-    auto alloc_all = [&]() {
-        R.off.alloc(n); R.base.alloc(n); R.cap_end.alloc(n); R.p.alloc(n); R.q.alloc(n);
-        R.nnz.alloc(n); R.lead.alloc(n); R.slot.alloc(n); R.cand.alloc(n);
-        R.scratch_col.alloc((size_t)16 * n);
-        R.scratch_val.alloc((size_t)16 * n * vs);
-        R.cols.alloc(pool_cap);
-        R.vals.alloc((size_t)pool_cap * vs);
-        R.log_row.alloc(log_cap); R.log_step.alloc(log_cap);
-        R.log_val.alloc((size_t)log_cap * vs);
-        R.tops.alloc(2);
-        R.status.alloc(1);
-        R.rmax.alloc(n);
-        R.rmaxc.alloc(n);
-    };
-    // a pool grown in place: the entries (or multipliers) so far copied into a larger buffer
-    auto grow = [&](auto& buf, int64_t used, size_t per, size_t cap) {
-        std::remove_reference_t<decltype(buf)> nb(cap * per);
-        if (used > 0) hip_check(hipMemcpyAsync(nb.p, buf.p, (size_t)used * per * sizeof(*buf.p), hipMemcpyDeviceToDevice, s), "pool growth");
-        buf = std::move(nb);
-    };
-    bool fresh = true;
     int32_t k0 = 0, resume = 0;
+    bool fresh = true;
     int64_t merges = 0;
     double merge_bytes = 0.0;
-    for (;;) {
-        try {
-            if (fresh) {
-                alloc_all();
-                R.off.upload(h_off.data(), n, s); R.base.upload(h_off.data(), n, s); R.cap_end.upload(h_end.data(), n, s);
-                R.p.upload(h_id.data(), n, s); R.q.upload(h_id.data(), n, s);
-                R.nnz.upload(h_nnz.data(), n, s); R.lead.upload(h_lead.data(), n, s); R.slot.upload(h_slot.data(), n, s);
-                R.cols.upload(h_cols.data(), nnz, s);
-                if (nnz > 0)
-                    hip_check(hipMemcpyAsync(R.vals.p, d_Rx, (size_t)nnz * vs * sizeof(double), hipMemcpyDeviceToDevice, s), "row values");
-                const unsigned long long tops[2] = {(unsigned long long)nnz, 0ull};
-                R.tops.upload(tops, 2, s);
-                fresh = false;
-                k0 = 0;
-                resume = 0;
-            } else if (st.code == RL_POOL) {
-                const int64_t used = std::min(st.pool_used, pool_cap);
-                pool_cap *= 2;
-                grow(R.cols, used, 1, pool_cap);
-                grow(R.vals, used, vs, pool_cap);
-            } else if (st.code == RL_LOG) {
-                const int64_t used = std::min(st.log_used, log_cap);
-                log_cap *= 2;
-                grow(R.log_row, used, 1, log_cap);
-                grow(R.log_step, used, 1, log_cap);
-                grow(R.log_val, used, vs, log_cap);
-            }
-        } catch (const GpuError&) {
-            errf(err, errlen, "out of memory (device: %ld pool entries, %ld multipliers)", (long)pool_cap, (long)log_cap);
-            return FPLUS_ERR_NOMEM;
+
+    void layout()
+    {
+        // the rows as given: row i at [Rp[i], Rp[i + 1]) of the pool, its block exactly as long
+        nnz = Rp[n];
+        h_off.assign(Rp, Rp + n);
+        h_end.assign(Rp + 1, Rp + n + 1);
+        h_id.resize(n);
+        h_nnz.resize(n); h_lead.resize(n); h_slot.resize(n); h_cols.resize(nnz);
+        for (int64_t i = 0; i < n; i++) {
+            h_nnz[i] = (int32_t)(Rp[i + 1] - Rp[i]);
+            h_lead[i] = h_nnz[i] ? (int32_t)Ri[Rp[i]] : RL_NONE;
+            h_slot[i] = (int32_t)i;
+            h_id[i] = i;
         }
+        for (int64_t q = 0; q < nnz; q++) h_cols[q] = (int32_t)Ri[q];
+        pool_cap = 16 * nnz + 64 * n + 1024;
+        log_cap = 4 * nnz + 2 * n + 1024;
+    }
+
+    // a pool grown in place: the entries (or multipliers) so far copied into a larger buffer
+    template <class B> static void grow(B& buf, int64_t used, size_t per, size_t cap, hipStream_t s)
+    {
+        B nb(cap * per);
+        if (used > 0) hip_check(hipMemcpyAsync(nb.p, buf.p, (size_t)used * per * sizeof(*buf.p), hipMemcpyDeviceToDevice, s), "pool growth");
+        buf = std::move(nb);
+    }
+
+    // the buffers for the next run: laid out afresh, or the pool / log grown (GpuError when memory runs out)
+    void prepare(hipStream_t s)
+    {
+        if (fresh) {
+            R.off.alloc(n); R.base.alloc(n); R.cap_end.alloc(n); R.p.alloc(n); R.q.alloc(n);
+            R.nnz.alloc(n); R.lead.alloc(n); R.slot.alloc(n); R.cand.alloc(n);
+            R.scratch_col.alloc((size_t)16 * n);
+            R.scratch_val.alloc((size_t)16 * n * vs);
+            R.cols.alloc(pool_cap);
+            R.vals.alloc((size_t)pool_cap * vs);
+            R.log_row.alloc(log_cap); R.log_step.alloc(log_cap);
+            R.log_val.alloc((size_t)log_cap * vs);
+            R.tops.alloc(2);
+            R.rmax.alloc(n);
+            R.rmaxc.alloc(n);
+            R.off.upload(h_off.data(), n, s); R.base.upload(h_off.data(), n, s); R.cap_end.upload(h_end.data(), n, s);
+            R.p.upload(h_id.data(), n, s); R.q.upload(h_id.data(), n, s);
+            R.nnz.upload(h_nnz.data(), n, s); R.lead.upload(h_lead.data(), n, s); R.slot.upload(h_slot.data(), n, s);
+            R.cols.upload(h_cols.data(), nnz, s);
+            if (nnz > 0)
+                hip_check(hipMemcpyAsync(R.vals.p, d_Rx, (size_t)nnz * vs * sizeof(double), hipMemcpyDeviceToDevice, s), "row values");
+            const unsigned long long tops[2] = {(unsigned long long)nnz, 0ull};
+            R.tops.upload(tops, 2, s);
+            fresh = false;
+            k0 = 0;
+            resume = 0;
+        } else if (st.code == RL_POOL) {
+            const int64_t used = std::min(st.pool_used, pool_cap);
+            pool_cap *= 2;
+            grow(R.cols, used, 1, pool_cap, s);
+            grow(R.vals, used, vs, pool_cap, s);
+        } else if (st.code == RL_LOG) {
+            const int64_t used = std::min(st.log_used, log_cap);
+            log_cap *= 2;
+            grow(R.log_row, used, 1, log_cap, s);
+            grow(R.log_step, used, 1, log_cap, s);
+            grow(R.log_val, used, vs, log_cap, s);
+        }
+    }
+
+    RowLuArgs args() const
+    {
         RowLuArgs a{};
-        a.n = nn; a.strategy = strategy; a.drop_tol = drop_tol; a.pivot_rel = pivot_rel;
+        a.n = (int32_t)n; a.strategy = strategy; a.drop_tol = drop_tol; a.pivot_rel = pivot_rel;
         a.off = R.off.p; a.base = R.base.p; a.cap_end = R.cap_end.p;
         a.nnz = R.nnz.p; a.lead = R.lead.p; a.slot = R.slot.p; a.p = R.p.p; a.q = R.q.p;
         a.cols = R.cols.p; a.vals = R.vals.p; a.pool_cap = pool_cap; a.pool_top = R.tops.p;
         a.log_row = R.log_row.p; a.log_step = R.log_step.p; a.log_val = R.log_val.p; a.log_cap = log_cap; a.log_top = R.tops.p + 1;
-        a.cand = R.cand.p; a.scratch_col = R.scratch_col.p; a.scratch_val = R.scratch_val.p; a.status = R.status.p;
+        a.cand = R.cand.p; a.scratch_col = R.scratch_col.p; a.scratch_val = R.scratch_val.p; a.status = d_status;
         a.rmax = R.rmax.p; a.rmaxc = R.rmaxc.p;
         a.k0 = k0; a.resume = resume;
-        if (n > 0) {
-            row_lu(vs, a, s);
-            runs++;
-            R.status.download(&st, 1, s);
-            merges += st.merges;
-            merge_bytes += st.merge_bytes;
-        } else {
-            st = RowLuStatus{};
-        }
-        dev_sync(s);
-        if (st.code == RL_POOL || st.code == RL_LOG) {   // grown in place, the run resumes at the step it stopped
-            if (runs > 64) break;
+        return a;
+    }
+
+    // after a run (st holds its report): whether another run is needed, and from where
+    bool next()
+    {
+        runs++;
+        merges += st.merges;
+        merge_bytes += st.merge_bytes;
+        if ((st.code == RL_POOL || st.code == RL_LOG) && runs <= 64) {   // grown in place, resumed at the step it stopped
             k0 = (int32_t)st.step;
             resume = st.phase;
-            continue;
+            return true;
         }
         if (st.code == RL_RESTART && restarts < 8) {       // a pool ran out within a step: from the start, larger
             restarts++;
             pool_cap *= 2;
             log_cap *= 2;
             fresh = true;
-            continue;
+            return true;
         }
-        break;
+        return false;
     }
-    if (st.code == RL_SINGULAR) {
-        errf(err, errlen, "Near-zero or zero pivot %.6e at step %ld (threshold %.6e). Matrix is singular.", st.best, (long)st.step, st.thr);
-        return FPLUS_ERR_SINGULAR;
+
+    // the outcome: an error with the reference's messages, or the CSR factors assembled from the pools
+    int result(fplus_factors* out, char* err, size_t errlen, hipStream_t s)
+    {
+        std::memset(out, 0, sizeof(*out));
+        if (st.code == RL_SINGULAR) {
+            errf(err, errlen, "Near-zero or zero pivot %.6e at step %ld (threshold %.6e). Matrix is singular.", st.best, (long)st.step, st.thr);
+            return FPLUS_ERR_SINGULAR;
+        }
+        if (st.code == RL_POOL || st.code == RL_LOG || st.code == RL_RESTART) {
+            errf(err, errlen, "out of memory (the row kernel's pools)");
+            return FPLUS_ERR_NOMEM;
+        }
+        if (st.code != RL_OK) {
+            errf(err, errlen, "Pivot entry missing at step %ld after swap.", (long)st.step);
+            return FPLUS_ERR_INTERNAL;
+        }
+        // U row i is the row at position i; L row i its logged multipliers in step order
+        std::vector<int64_t> d_off(n), p(n), q(n);
+        std::vector<int32_t> d_nnz(n), slot(n);
+        std::vector<int32_t> pc(st.pool_used > 0 ? st.pool_used : 0), lr(st.log_used), ls(st.log_used);
+        std::vector<double> pv((size_t)std::max<int64_t>(st.pool_used, 0) * vs), lv((size_t)st.log_used * vs);
+        if (n > 0) {
+            R.off.download(d_off.data(), n, s); R.nnz.download(d_nnz.data(), n, s); R.slot.download(slot.data(), n, s);
+            R.p.download(p.data(), n, s); R.q.download(q.data(), n, s);
+            R.cols.download(pc.data(), pc.size(), s); R.vals.download(pv.data(), pv.size(), s);
+            R.log_row.download(lr.data(), lr.size(), s); R.log_step.download(ls.data(), ls.size(), s);
+            R.log_val.download(lv.data(), lv.size(), s);
+            dev_sync(s);
+        }
+        std::vector<int64_t> pos(n);
+        for (int64_t i = 0; i < n; i++) pos[slot[i]] = i;
+        auto dropped = [&](const double* v, int64_t j, int64_t i) { return drop_tol > 0.0 && j != i && absval(v, vs) <= drop_tol; };
+        try {
+            out->n = n;
+            out->is_complex = vs == 2;
+            out->csc = 0;
+            // U (src/factor_row.c:1081-1133 with upper = 1).  With column exchanges the rows hold input column ids:
+            // mapped to the final positions and sorted
+            std::vector<int64_t> qinv;
+            if (strategy != FPLUS_PIVOT_PARTIAL) {
+                qinv.resize(n);
+                for (int64_t j = 0; j < n; j++) qinv[q[j]] = j;
+            }
+            auto ucol = [&](int64_t t) -> int64_t { return qinv.empty() ? pc[t] : qinv[pc[t]]; };
+            out->Up = cmalloc<int64_t>(n + 1);
+            out->Up[0] = 0;
+            for (int64_t i = 0; i < n; i++) {
+                const int32_t r = slot[i];
+                int64_t c = 0;
+                for (int64_t t = d_off[r]; t < d_off[r] + d_nnz[r]; t++)
+                    if (!dropped(&pv[t * vs], ucol(t), i) && ucol(t) >= i) c++;
+                out->Up[i + 1] = out->Up[i] + c;
+            }
+            out->Ui = cmalloc<int64_t>(out->Up[n]);
+            double* Ux = cmalloc<double>((size_t)out->Up[n] * vs);
+            out->Ux = Ux;
+            std::vector<std::pair<int64_t, int64_t>> row;
+            for (int64_t i = 0, o = 0; i < n; i++) {
+                const int32_t r = slot[i];
+                row.clear();
+                for (int64_t t = d_off[r]; t < d_off[r] + d_nnz[r]; t++)
+                    if (!dropped(&pv[t * vs], ucol(t), i) && ucol(t) >= i) row.push_back({ucol(t), t});
+                if (!qinv.empty()) std::sort(row.begin(), row.end());
+                for (const auto& e : row) {
+                    out->Ui[o] = e.first;
+                    for (int v = 0; v < vs; v++) Ux[o * vs + v] = pv[e.second * vs + v];
+                    o++;
+                }
+            }
+            // L (upper = 0): the log grouped by row, stably, so every row's multipliers stay in step order
+            std::vector<int64_t> lcnt(n + 1, 0);
+            for (int64_t e = 0; e < st.log_used; e++)
+                if (!dropped(&lv[e * vs], ls[e], pos[lr[e]])) lcnt[pos[lr[e]] + 1]++;
+            out->Lp = cmalloc<int64_t>(n + 1);
+            out->Lp[0] = 0;
+            for (int64_t i = 0; i < n; i++) out->Lp[i + 1] = out->Lp[i] + 1 + lcnt[i + 1];
+            out->Li = cmalloc<int64_t>(out->Lp[n]);
+            double* Lx = cmalloc<double>((size_t)out->Lp[n] * vs);
+            out->Lx = Lx;
+            std::vector<int64_t> nxt(n);
+            for (int64_t i = 0; i < n; i++) {
+                const int64_t o = out->Lp[i];
+                out->Li[o] = i;
+                Lx[o * vs] = 1.0;
+                if (vs == 2) Lx[o * vs + 1] = 0.0;
+                nxt[i] = o + 1;
+            }
+            for (int64_t e = 0; e < st.log_used; e++) {
+                const int64_t i = pos[lr[e]];
+                if (dropped(&lv[e * vs], ls[e], i)) continue;
+                const int64_t o = nxt[i]++;
+                out->Li[o] = ls[e];
+                for (int v = 0; v < vs; v++) Lx[o * vs + v] = lv[e * vs + v];
+            }
+            out->p = cmalloc<int64_t>(n);
+            out->q = cmalloc<int64_t>(n);
+            std::copy(p.begin(), p.end(), out->p);
+            std::copy(q.begin(), q.end(), out->q);
+            out->nsupernodes = 0;
+            out->n_pivots_replaced = st.nrep;
+        } catch (const std::bad_alloc&) {
+            fplus_factors_free(out);
+            errf(err, errlen, "out of memory");
+            return FPLUS_ERR_NOMEM;
+        }
+        return FPLUS_OK;
     }
-    if (st.code == RL_POOL || st.code == RL_LOG || st.code == RL_RESTART) {
-        errf(err, errlen, "out of memory (the row kernel's pools)");
-        return FPLUS_ERR_NOMEM;
+};
+
+void record_info(const RowSystem* const* sys, size_t cnt, double t0, double t1)
+{
+    fplus_amd_row_info inf{};
+    for (size_t i = 0; i < cnt; i++) {
+        inf.runs += sys[i]->runs;
+        inf.restarts += sys[i]->restarts;
+        inf.merges += sys[i]->merges;
+        inf.merge_bytes += sys[i]->merge_bytes;
+        inf.pool_entries += sys[i]->st.pool_used;
+        inf.multipliers += sys[i]->st.log_used;
     }
-    if (st.code != RL_OK) {
-        errf(err, errlen, "Pivot entry missing at step %ld after swap.", (long)st.step);
-        return FPLUS_ERR_INTERNAL;
+    inf.t_device = t1 - t0;
+    inf.t_total = now_s() - t0;
+    t_row_info = inf;
+    t_row_info_set = true;
+}
+
+int row_factor_device(int64_t n, const int64_t* Rp, const int64_t* Ri, const double* d_Rx, int vs, double drop_tol, int strategy,
+                      double pivot_rel, int verbose, fplus_factors* out, char* err, size_t errlen)
+{
+    std::memset(out, 0, sizeof(*out));
+    if (n >= (int64_t)RL_NONE) { errf(err, errlen, "the row kernel takes n < 2^31 - 1 (got %ld)", (long)n); return FPLUS_ERR_BADARG; }
+    const char* sname = strategy == FPLUS_PIVOT_FULL ? "full" : strategy == FPLUS_PIVOT_PARTIAL ? "partial" : "rook";
+    if (verbose >= 1)
+        printf("[FPLUS] LU factorisation start: n=%ld, nnz(A)=%ld, %s, drop_tol=%.1e, pivot=%s (row kernel, device)\n", (long)n,
+               (long)Rp[n], vs == 2 ? "complex128" : "float64", drop_tol, sname);
+    hipStream_t s = devctx().stream;
+    RowSystem S;
+    S.n = n; S.vs = vs; S.strategy = strategy; S.drop_tol = drop_tol; S.pivot_rel = pivot_rel;
+    S.Rp = Rp; S.Ri = Ri; S.d_Rx = d_Rx;
+    S.layout();
+    DevBuf<RowLuStatus> d_status(1);
+    S.d_status = d_status.p;
+    const double t0 = now_s();
+    for (;;) {
+        try {
+            S.prepare(s);
+        } catch (const GpuError&) {
+            errf(err, errlen, "out of memory (device: %ld pool entries, %ld multipliers)", (long)S.pool_cap, (long)S.log_cap);
+            return FPLUS_ERR_NOMEM;
+        }
+        if (n == 0) break;
+        row_lu(vs, S.args(), s);
+        d_status.download(&S.st, 1, s);
+        dev_sync(s);
+        if (!S.next()) break;
     }
     const double t1 = now_s();
-    // the factors out of the pools: U row i is the row at position i; L row i its logged multipliers in step order
-    std::vector<int64_t> d_off(n), p(n), q(n);
-    std::vector<int32_t> d_nnz(n), slot(n);
-    std::vector<int32_t> pc(st.pool_used > 0 ? st.pool_used : 0), lr(st.log_used), ls(st.log_used);
-    std::vector<double> pv((size_t)std::max<int64_t>(st.pool_used, 0) * vs), lv((size_t)st.log_used * vs);
-    if (n > 0) {
-        R.off.download(d_off.data(), n, s); R.nnz.download(d_nnz.data(), n, s); R.slot.download(slot.data(), n, s);
-        R.p.download(p.data(), n, s); R.q.download(q.data(), n, s);
-        R.cols.download(pc.data(), pc.size(), s); R.vals.download(pv.data(), pv.size(), s);
-        R.log_row.download(lr.data(), lr.size(), s); R.log_step.download(ls.data(), ls.size(), s);
-        R.log_val.download(lv.data(), lv.size(), s);
-        dev_sync(s);
-    }
-    std::vector<int64_t> pos(n);
-    for (int64_t i = 0; i < n; i++) pos[slot[i]] = i;
-    auto dropped = [&](const double* v, int64_t j, int64_t i) { return drop_tol > 0.0 && j != i && absval(v, vs) <= drop_tol; };
-    try {
-        out->n = n;
-        out->is_complex = vs == 2;
-        out->csc = 0;
-        // U (src/factor_row.c:1081-1133 with upper = 1).  With column exchanges the rows hold input column ids:
-        // mapped to the final positions and sorted
-        std::vector<int64_t> qinv;
-        if (strategy != FPLUS_PIVOT_PARTIAL) {
-            qinv.resize(n);
-            for (int64_t j = 0; j < n; j++) qinv[q[j]] = j;
-        }
-        auto ucol = [&](int64_t t) -> int64_t { return qinv.empty() ? pc[t] : qinv[pc[t]]; };
-        out->Up = cmalloc<int64_t>(n + 1);
-        out->Up[0] = 0;
-        for (int64_t i = 0; i < n; i++) {
-            const int32_t r = slot[i];
-            int64_t c = 0;
-            for (int64_t t = d_off[r]; t < d_off[r] + d_nnz[r]; t++)
-                if (!dropped(&pv[t * vs], ucol(t), i) && ucol(t) >= i) c++;
-            out->Up[i + 1] = out->Up[i] + c;
-        }
-        out->Ui = cmalloc<int64_t>(out->Up[n]);
-        double* Ux = cmalloc<double>((size_t)out->Up[n] * vs);
-        out->Ux = Ux;
-        std::vector<std::pair<int64_t, int64_t>> row;
-        for (int64_t i = 0, o = 0; i < n; i++) {
-            const int32_t r = slot[i];
-            row.clear();
-            for (int64_t t = d_off[r]; t < d_off[r] + d_nnz[r]; t++)
-                if (!dropped(&pv[t * vs], ucol(t), i) && ucol(t) >= i) row.push_back({ucol(t), t});
-            if (!qinv.empty()) std::sort(row.begin(), row.end());
-            for (const auto& e : row) {
-                out->Ui[o] = e.first;
-                for (int v = 0; v < vs; v++) Ux[o * vs + v] = pv[e.second * vs + v];
-                o++;
-            }
-        }
-        // L (upper = 0): the log grouped by row, stably, so every row's multipliers stay in step order
-        std::vector<int64_t> lcnt(n + 1, 0);
-        for (int64_t e = 0; e < st.log_used; e++)
-            if (!dropped(&lv[e * vs], ls[e], pos[lr[e]])) lcnt[pos[lr[e]] + 1]++;
-        out->Lp = cmalloc<int64_t>(n + 1);
-        out->Lp[0] = 0;
-        for (int64_t i = 0; i < n; i++) out->Lp[i + 1] = out->Lp[i] + 1 + lcnt[i + 1];
-        out->Li = cmalloc<int64_t>(out->Lp[n]);
-        double* Lx = cmalloc<double>((size_t)out->Lp[n] * vs);
-        out->Lx = Lx;
-        std::vector<int64_t> nxt(n);
-        for (int64_t i = 0; i < n; i++) {
-            const int64_t o = out->Lp[i];
-            out->Li[o] = i;
-            Lx[o * vs] = 1.0;
-            if (vs == 2) Lx[o * vs + 1] = 0.0;
-            nxt[i] = o + 1;
-        }
-        for (int64_t e = 0; e < st.log_used; e++) {
-            const int64_t i = pos[lr[e]];
-            if (dropped(&lv[e * vs], ls[e], i)) continue;
-            const int64_t o = nxt[i]++;
-            out->Li[o] = ls[e];
-            for (int v = 0; v < vs; v++) Lx[o * vs + v] = lv[e * vs + v];
-        }
-        out->p = cmalloc<int64_t>(n);
-        out->q = cmalloc<int64_t>(n);
-        std::copy(p.begin(), p.end(), out->p);
-        std::copy(q.begin(), q.end(), out->q);
-        out->nsupernodes = 0;
-        out->n_pivots_replaced = st.nrep;
-    } catch (const std::bad_alloc&) {
-        fplus_factors_free(out);
-        errf(err, errlen, "out of memory");
-        return FPLUS_ERR_NOMEM;
-    }
-    t_row_info.runs = runs;
-    t_row_info.restarts = restarts;
-    t_row_info.merges = merges;
-    t_row_info.merge_bytes = merge_bytes;
-    t_row_info.pool_entries = st.pool_used;
-    t_row_info.multipliers = st.log_used;
-    t_row_info.t_device = t1 - t0;
-    t_row_info.t_total = now_s() - t0;
-    t_row_info_set = true;
+    const int rc = S.result(out, err, errlen, s);
+    if (rc != FPLUS_OK) return rc;
+    const RowSystem* one = &S;
+    record_info(&one, 1, t0, t1);
     if (verbose >= 1) {
         printf("[FPLUS] LU factorisation complete: nnz(L)=%ld, nnz(U)=%ld, total nnz=%ld, time=%.3fs (device elimination %.3fs in %d "
                "runs, %d from the start; pool %ld of %ld entries, %ld multipliers)\n", (long)out->Lp[n], (long)out->Up[n],
-               (long)(out->Lp[n] + out->Up[n]), now_s() - t0, t1 - t0, runs, restarts, (long)st.pool_used, (long)pool_cap,
-               (long)st.log_used);
-        if (st.nrep > 0)
-            printf("[FPLUS] ILU: %ld zero / negligible pivots replaced by drop_tol * max|a_ij| = %.3e\n", (long)st.nrep, drop_tol * st.max0);
+               (long)(out->Lp[n] + out->Up[n]), now_s() - t0, t1 - t0, S.runs, S.restarts, (long)S.st.pool_used, (long)S.pool_cap,
+               (long)S.st.log_used);
+        if (S.st.nrep > 0)
+            printf("[FPLUS] ILU: %ld zero / negligible pivots replaced by drop_tol * max|a_ij| = %.3e\n", (long)S.st.nrep,
+                   drop_tol * S.st.max0);
     }
+    return FPLUS_OK;
+}
+
+// independent systems, one launch per round: every system that still needs a run gets one workgroup
+int row_factor_batch(int64_t nsys, const int64_t* n, const int64_t* const* Rp, const int64_t* const* Ri, const double* const* d_Rx,
+                     int vs, double drop_tol, int strategy, double pivot_rel, fplus_factors* out, int* rc, char* err, size_t errlen)
+{
+    hipStream_t s = devctx().stream;
+    std::vector<std::unique_ptr<RowSystem>> sys(nsys);
+    DevBuf<RowLuStatus> d_status(nsys > 0 ? nsys : 1);
+    std::vector<int64_t> active;
+    for (int64_t i = 0; i < nsys; i++) {
+        sys[i].reset(new RowSystem());
+        RowSystem& S = *sys[i];
+        S.n = n[i]; S.vs = vs; S.strategy = strategy; S.drop_tol = drop_tol; S.pivot_rel = pivot_rel;
+        S.Rp = Rp[i]; S.Ri = Ri[i]; S.d_Rx = d_Rx[i];
+        S.d_status = d_status.p + i;
+        S.layout();
+        if (S.n > 0) active.push_back(i);
+    }
+    const double t0 = now_s();
+    std::vector<RowLuArgs> args;
+    DevBuf<RowLuArgs> d_args(nsys > 0 ? nsys : 1);
+    std::vector<RowLuStatus> st(nsys > 0 ? nsys : 1);
+    while (!active.empty()) {
+        args.clear();
+        for (int64_t i : active) {
+            try {
+                sys[i]->prepare(s);
+            } catch (const GpuError&) {
+                errf(err, errlen, "out of memory (device: system %ld, %ld pool entries, %ld multipliers)", (long)i, (long)sys[i]->pool_cap,
+                     (long)sys[i]->log_cap);
+                return FPLUS_ERR_NOMEM;
+            }
+            args.push_back(sys[i]->args());
+        }
+        d_args.upload(args.data(), args.size(), s);
+        row_lu_batch(vs, strategy, d_args.p, (int)args.size(), s);
+        d_status.download(st.data(), nsys, s);
+        dev_sync(s);
+        std::vector<int64_t> again;
+        for (int64_t i : active) {
+            sys[i]->st = st[i];
+            if (sys[i]->next()) again.push_back(i);
+        }
+        active.swap(again);
+    }
+    const double t1 = now_s();
+    for (int64_t i = 0; i < nsys; i++) {
+        char e2[256];
+        rc[i] = sys[i]->result(&out[i], e2, sizeof e2, s);
+    }
+    std::vector<const RowSystem*> ptrs;
+    for (auto& S : sys) ptrs.push_back(S.get());
+    record_info(ptrs.data(), ptrs.size(), t0, t1);
     return FPLUS_OK;
 }
 
@@ -463,6 +575,60 @@ int fplus_factor_row_sym(const fplus_symbolic* sym, const int64_t* Ap, const int
     }
     return fplus_factor_row_perm(n, Ap, Ai, Ax, is_complex, sym->row_perm.data(), sym->col_perm.data(), equilibrate, ruiz_max_iter,
                                  ruiz_tol, verbose, drop_tol, pivot_strategy, pivot_rel, out, Dr, Dc, err, errlen);
+}
+
+int fplus_amd_factor_row_batch(int64_t nsys, const int64_t* n, const int64_t* const* indptr, const int64_t* const* indices,
+                               const void* const* data, int is_complex, double drop_tol, int pivot_strategy, double pivot_rel,
+                               fplus_factors* out, int* rc, char* err, size_t errlen)
+{
+    if (nsys < 0 || (nsys > 0 && (!n || !indptr || !indices || !data || !out || !rc))) {
+        errf(err, errlen, "fplus_amd_factor_row_batch: bad nsys or null arrays");
+        return FPLUS_ERR_BADARG;
+    }
+    for (int64_t i = 0; i < nsys; i++) { std::memset(&out[i], 0, sizeof(out[i])); rc[i] = FPLUS_OK; }
+    if (pivot_strategy != FPLUS_PIVOT_FULL && pivot_strategy != FPLUS_PIVOT_PARTIAL && pivot_strategy != FPLUS_PIVOT_ROOK) {
+        errf(err, errlen, "pivot_strategy must be 0 (full), 1 (partial), or 2 (rook); got %d", pivot_strategy);
+        return FPLUS_ERR_BADARG;
+    }
+    if (!(pivot_rel >= 0.0)) { errf(err, errlen, "pivot_rel must be >= 0"); return FPLUS_ERR_BADARG; }
+    if (!(drop_tol >= 0.0)) { errf(err, errlen, "drop_tol must be >= 0 (got %g)", drop_tol); return FPLUS_ERR_BADARG; }
+    const int vs = is_complex ? 2 : 1;
+    std::vector<int64_t> idx, bn;
+    std::vector<const int64_t*> bp, bi;
+    for (int64_t i = 0; i < nsys; i++) {   // each system's checks; a system that fails them keeps its code and is left out
+        char e2[256];
+        int r = n[i] >= (int64_t)RL_NONE ? FPLUS_ERR_BADARG : check_compressed(n[i], indptr[i], indices[i], true, e2, sizeof e2);
+        if (r == FPLUS_OK) r = check_finite(indptr[i][n[i]], data[i], is_complex, e2, sizeof e2);
+        rc[i] = r;
+        if (r != FPLUS_OK) continue;
+        idx.push_back(i); bn.push_back(n[i]); bp.push_back(indptr[i]); bi.push_back(indices[i]);
+    }
+    try {
+        hipStream_t s = devctx().stream;
+        std::vector<DevBuf<double>> vals(idx.size());
+        std::vector<const double*> dx(idx.size());
+        for (size_t t = 0; t < idx.size(); t++) {
+            const int64_t nnz = indptr[idx[t]][n[idx[t]]];
+            vals[t].alloc((size_t)(nnz > 0 ? nnz : 1) * vs);
+            vals[t].upload(static_cast<const double*>(data[idx[t]]), (size_t)nnz * vs, s);
+            dx[t] = vals[t].p;
+        }
+        std::vector<fplus_factors> o(idx.size());
+        std::vector<int> r(idx.size());
+        const int ret = row_factor_batch((int64_t)idx.size(), bn.data(), bp.data(), bi.data(), dx.data(), vs, drop_tol, pivot_strategy,
+                                         pivot_rel, o.data(), r.data(), err, errlen);
+        if (ret != FPLUS_OK) {
+            for (auto& f : o) fplus_factors_free(&f);
+            return ret;
+        }
+        for (size_t t = 0; t < idx.size(); t++) { out[idx[t]] = o[t]; rc[idx[t]] = r[t]; }
+        return FPLUS_OK;
+    } catch (const GpuError& e) {
+        return gpu_error(e, err, errlen);
+    } catch (const std::bad_alloc&) {
+        errf(err, errlen, "out of memory");
+        return FPLUS_ERR_NOMEM;
+    }
 }
 
 int fplus_amd_row_stats(fplus_amd_row_info* info)

@@ -63,5 +63,8 @@ struct RowLuArgs {
 // the elimination of one system from step k0 on, in one launch of one workgroup (vs 1 real, 2 complex); it stops at
 // the start of a step (or of its elimination) whose worst-case growth the pool or the log cannot take
 void row_lu(int vs, const RowLuArgs& a, hipStream_t s);
+// nsys independent systems of one scalar type and strategy in one launch, one workgroup each (d_args: their
+// arguments in device memory)
+void row_lu_batch(int vs, int strategy, const RowLuArgs* d_args, int nsys, hipStream_t s);
 
 }  // namespace fpa

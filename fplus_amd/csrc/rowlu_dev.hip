@@ -405,8 +405,8 @@ __device__ void eliminate_row(const RowLuArgs& a, int32_t ip, int32_t k, T piv, 
     }
 }
 
-// S: the pivot strategy (0 full, 1 partial, 2 rook)
-template <class T, int S> __global__ __launch_bounds__(RL_THREADS) void k_row_lu(RowLuArgs a)
+// the elimination of one system by the whole workgroup; S: the pivot strategy (0 full, 1 partial, 2 rook)
+template <class T, int S> __device__ __forceinline__ void row_lu_body(const RowLuArgs& a)
 {
     __shared__ int32_t sBc[RL_BMAX];
     __shared__ T sBv[RL_BMAX];
@@ -704,7 +704,32 @@ template <class T, int S> __global__ __launch_bounds__(RL_THREADS) void k_row_lu
     }
 }
 
+template <class T, int S> __global__ __launch_bounds__(RL_THREADS) void k_row_lu(RowLuArgs a) { row_lu_body<T, S>(a); }
+
+// independent systems, one workgroup each
+template <class T, int S> __global__ __launch_bounds__(RL_THREADS) void k_row_lu_batch(const RowLuArgs* args)
+{
+    const RowLuArgs a = args[blockIdx.x];
+    row_lu_body<T, S>(a);
+}
+
 }  // namespace
+
+void row_lu_batch(int vs, int strategy, const RowLuArgs* d_args, int nsys, hipStream_t s)
+{
+    if (nsys <= 0) return;
+    ProfScope ps("row_lu_batch", 0.0, s);
+    auto go = [&](auto kern) { hipLaunchKernelGGL(kern, dim3(nsys), dim3(RL_THREADS), 0, s, d_args); };
+    if (vs == 2) {
+        if (strategy == 0) go(k_row_lu_batch<Zd, 0>);
+        else if (strategy == 1) go(k_row_lu_batch<Zd, 1>);
+        else go(k_row_lu_batch<Zd, 2>);
+    } else {
+        if (strategy == 0) go(k_row_lu_batch<double, 0>);
+        else if (strategy == 1) go(k_row_lu_batch<double, 1>);
+        else go(k_row_lu_batch<double, 2>);
+    }
+}
 
 void row_lu(int vs, const RowLuArgs& a, hipStream_t s)
 {

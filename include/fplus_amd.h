@@ -51,6 +51,16 @@ typedef struct {
 } fplus_amd_row_info;
 int fplus_amd_row_stats(fplus_amd_row_info *info);
 
+/* Many independent systems through the row kernel at once (no counterpart in the reference, which factors one system
+ * per call): system i is the canonical CSR matrix (n[i], indptr[i], indices[i], data[i]), all of one scalar type and
+ * factored with the same pivot_strategy, drop_tol and pivot_rel as fplus_factor_row would; each system runs as one
+ * workgroup of a shared launch.  out[i] and rc[i] receive what fplus_factor_row would return for system i (CSR factors,
+ * or its error code; out[i] zeroed then).  The call returns FPLUS_OK when the batch ran, or an error for the whole
+ * batch (bad arguments, device failure, out of memory).  fplus_amd_row_stats then reports the whole batch. */
+int fplus_amd_factor_row_batch(int64_t nsys, const int64_t *n, const int64_t *const *indptr, const int64_t *const *indices,
+                               const void *const *data, int is_complex, double drop_tol, int pivot_strategy, double pivot_rel,
+                               fplus_factors *out, int *rc, char *err, size_t errlen);
+
 /* Device-resident inputs, for callers that keep the values and right-hand sides in HBM (refactorization loops,
  * benchmarks): fplus_factor with Ax a device pointer (the analysed pattern Ap, Ai on the host), and fplus_solve on a
  * device array b (column-major, leading dimension ldb), in place. */

@@ -252,3 +252,43 @@ def test_extensions_refuse_a_row_handle():
     assert rc == FPLUS_ERR_BADARG
     with pytest.raises(FplusError):
         fplus_amd.Numeric(sym, Ap, Ai, Ax, fplus_amd.default_options(pivot_strategy=2), fplus_amd.amd_options(refactor=1))
+
+
+def test_batched_systems_match_one_by_one():
+    """fplus_amd_factor_row_batch: independent systems of mixed sizes in shared launches (one workgroup each) give what
+    fplus_factor_row gives for each alone, bit for bit, with per-system errors (a singular system, a non-canonical one)
+    and per-system pool growth (an arrow matrix that fills in completely)"""
+    rng = np.random.default_rng(9)
+    systems = [c[1:] for c in CASES if not np.iscomplexobj(c[4])]
+    for g in (6, 10, 17, 23):
+        n, Ap, Ai, Ax = gen.convdiff2d(g, seed=g)
+        systems.append((n, Ap, Ai, Ax))   # CSC arrays read as the CSR of the transpose
+    n = 150
+    A = sp.lil_matrix((n, n))
+    A[0, :] = rng.uniform(0.5, 1.0, n)
+    A[:, 0] = rng.uniform(0.5, 1.0, (n, 1))
+    A.setdiag(np.concatenate([[10.0], rng.uniform(1.0, 2.0, n - 1)]))
+    A[0, 0] = 10.0
+    systems.append(csr_arrays(A.tocsr()))
+    systems.append(csr_arrays(np.ones((3, 3))))                       # singular
+    bad = list(csr_arrays(np.eye(3)))
+    bad[2] = np.array([2, 0, 1], np.int64)                             # not canonical: row 0 holds columns 2, 0
+    bad[1] = np.array([0, 2, 2, 3], np.int64)
+    systems.append(tuple(bad))
+    for strategy, dt in ((1, 1e-3), (2, 0.0), (0, 1e-2)):
+        print(f"batch of {len(systems)} systems, {NAMES[strategy]} drop_tol {dt}", flush=True)
+        got = fplus_amd.factor_row_batch(systems, dt, strategy)
+        assert len(got) == len(systems)
+        assert fplus_amd.row_stats()["runs"] >= len(systems) - 1   # every valid system ran at least once
+        for i, (sysm, g) in enumerate(zip(systems, got)):
+            try:
+                one = fplus_amd.factor_row(*sysm, dt, strategy)
+            except FplusError as e:
+                assert isinstance(g, FplusError) and g.code == e.code, (i, g)
+                continue
+            assert not isinstance(g, FplusError), (i, g)
+            for key in ("p", "q", "Lp", "Li", "Lx", "Up", "Ui", "Ux"):
+                assert np.array_equal(g[key], one[key]), (i, key)
+        if dt == 0.0:   # with dropping, the all-ones matrix's missing pivots are replaced instead (an ILU breakdown)
+            assert isinstance(got[-2], FplusError) and got[-2].code == FPLUS_ERR_SINGULAR
+        assert isinstance(got[-1], FplusError) and got[-1].code == FPLUS_ERR_BADARG
