@@ -66,13 +66,16 @@ WORKLOADS = {
     # the row kernel (rook / full / partial pivoting, incomplete LU): the 96 x 96 convection-diffusion matrix (n = 9216)
     # in its analysed order, equilibrated, factored by fplus_factor_row once per leg
     "rowlu-convdiff2d-96": dict(g=96),
+    # many independent systems through the row kernel: 256 convection-diffusion matrices on g x g grids, g log-uniform
+    # in [32, 96] (n 1024 .. 9216), each its own seed, partial-pivoting ILU (drop_tol 1e-3)
+    "rowlu-batch-256": dict(systems=256, gmin=32, gmax=96, strategy=1, drop_tol=1e-3),
 }
 # the row kernel's legs: (pivot strategy 0 full / 1 partial / 2 rook, drop_tol)
 ROW_LEGS = {"row_partial_ilu": (1, 1e-3), "row_rook_ilu": (2, 1e-3), "row_full_ilu": (0, 1e-2), "row_partial": (1, 0.0)}
 # the legs each workload runs (the default: the headline's four)
 WORKLOAD_LEGS = {"convdiff3d-64": ("fp64_gpu", "fp32_factor_ir", "bf16_factor_ir"),
                  "sweep-32x4x1024": ("sweep", "sweep_graph"), "batch-256": ("batch", "batch_streams"),
-                 "rowlu-convdiff2d-96": tuple(ROW_LEGS)}
+                 "rowlu-convdiff2d-96": tuple(ROW_LEGS), "rowlu-batch-256": ("row_batch", "row_one_by_one")}
 WORKLOAD_NRHS = {"convdiff3d-64": 8, "shiftinvert-128x4x1024-s17": 64, "sweep-32x4x1024": 4, "batch-256": 4}
 WORKLOAD_TOL = {"convdiff3d-64": 1e-14}
 DEFAULT_WORKLOAD = "shiftinvert-64x4x1024"   # n = 262144, the plain FP64 GPU drop-in configuration
@@ -84,7 +87,8 @@ PEAK_HBM = 8.0e12             # MI355X HBM3E, B/s
 NRHS_DEFAULT = 16
 TOL_DEFAULT = 1e-13
 LEGS = ("fp64_gpu", "fp64_refactor", "fp32_refactor_ir", "learned_refactor_ir")
-ALL_LEGS = LEGS + ("fp32_factor_ir", "bf16_factor_ir", "sweep", "sweep_graph", "batch", "batch_streams") + tuple(ROW_LEGS)
+ALL_LEGS = (LEGS + ("fp32_factor_ir", "bf16_factor_ir", "sweep", "sweep_graph", "batch", "batch_streams") + tuple(ROW_LEGS)
+            + ("row_batch", "row_one_by_one"))
 SELECTOR = os.path.join(ROOT, "fplus_amd", "data", "selector_shiftinvert.npz")
 # the dominant kernel's HBM bytes per launch, from the rocprofv3 FETCH_SIZE and WRITE_SIZE passes summarized there
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "blk_upd-hbm_traffic-shiftinvert-64x4x1024.txt")
@@ -425,16 +429,7 @@ def rowlu_main(args, kw, legs):
     merges and their algorithmic bytes; beside it the reference's own row kernel (src/factor_row.c built with OpenMP,
     oracle/_ref/libfplus_ref_row_omp.so) on the host's cores with the same input, and whether both factorizations agree
     (p, q, patterns; values)"""
-    import scipy.sparse as sp
-    n, Ap, Ai, Ax = gen.convdiff2d(**kw)
-    sym = fplus_amd.Symbolic(n, Ap, Ai)
-    B = sp.csc_matrix((Ax, Ai, Ap), shape=(n, n))[sym.row_perm][:, sym.col_perm].tocsc()
-    B.sort_indices()
-    Bp, Bi = B.indptr.astype(np.int64), B.indices.astype(np.int64)
-    Bx, _, _ = fplus_amd.ruiz(n, Bp, Bi, B.data)
-    R = sp.csc_matrix((Bx, Bi, Bp), shape=(n, n)).tocsr()
-    R.sort_indices()
-    Rp, Ri, Rx = R.indptr.astype(np.int64), R.indices.astype(np.int64), np.ascontiguousarray(R.data)
+    n, Rp, Ri, Rx = row_input(*gen.convdiff2d(**kw))
     log(f"workload rowlu: n {n}, nnz {len(Ri)}")
     threads = min(16, os.cpu_count() or 1)
     ref = None
@@ -498,6 +493,130 @@ def rowlu_main(args, kw, legs):
     print(json.dumps(line), flush=True)
 
 
+def row_input(n, Ap, Ai, Ax):
+    """the row kernel's input as its pipelines make it: the matrix in its analysed order, equilibrated (device Ruiz),
+    laid out by rows"""
+    import scipy.sparse as sp
+    sym = fplus_amd.Symbolic(n, Ap, Ai)
+    B = sp.csc_matrix((Ax, Ai, Ap), shape=(n, n))[sym.row_perm][:, sym.col_perm].tocsc()
+    B.sort_indices()
+    Bp, Bi = B.indptr.astype(np.int64), B.indices.astype(np.int64)
+    Bx, _, _ = fplus_amd.ruiz(n, Bp, Bi, B.data)
+    R = sp.csc_matrix((Bx, Bi, Bp), shape=(n, n)).tocsr()
+    R.sort_indices()
+    return n, R.indptr.astype(np.int64), R.indices.astype(np.int64), np.ascontiguousarray(R.data)
+
+
+def rowlu_batch_main(args, kw, legs):
+    """many small systems through the row kernel: each system's input prepared before the timed region; leg row_batch
+    factors all of them in one fplus_amd_factor_row_batch call (one workgroup per system in shared launches), leg
+    row_one_by_one with one fplus_factor_row call per system; beside them the reference's own row kernel (OpenMP build)
+    one system after another on the host's cores, and on as many host threads at once, one system per thread with
+    OpenMP off inside; both device legs checked against the reference system by system (p, q, U pattern and values)"""
+    import threading
+    rng = np.random.default_rng(77)
+    systems = []
+    for i in range(kw["systems"]):
+        g = int(np.rint(np.exp(rng.uniform(np.log(kw["gmin"]), np.log(kw["gmax"])))))
+        systems.append(row_input(*gen.convdiff2d(g, seed=100 + i)))
+    ns = [s[0] for s in systems]
+    log(f"workload rowlu-batch: {len(systems)} systems, n {min(ns)} .. {max(ns)}, total {sum(ns)}")
+    strategy, dt = kw["strategy"], kw["drop_tol"]
+    out = {}
+    res = {}
+    for leg in legs:
+        for _ in range(args.warmup):
+            if leg == "row_batch":
+                fplus_amd.factor_row_batch(systems, dt, strategy)
+            else:
+                for sm in systems:
+                    fplus_amd.factor_row(*sm, dt, strategy)
+        ts = []
+        for _ in range(max(1, args.steps)):
+            t0 = time.perf_counter()
+            if leg == "row_batch":
+                got = fplus_amd.factor_row_batch(systems, dt, strategy)
+                st = fplus_amd.row_stats()
+            else:
+                got = [fplus_amd.factor_row(*sm, dt, strategy) for sm in systems]
+            ts.append(time.perf_counter() - t0)
+        res[leg] = got
+        g = float(np.median(ts))
+        out[leg] = {"s": g, "systems_per_s": len(systems) / g}
+        if leg == "row_batch":
+            out[leg].update({"device_s": st["t_device"], "runs": st["runs"], "merges": st["merges"],
+                             "merge_GB/s": st["merge_bytes"] / st["t_device"] / 1e9})
+        log(f"{leg}: {g:.3f} s for {len(systems)} systems")
+    threads = min(16, os.cpu_count() or 1)
+    ref_path = os.path.join(ROOT, "oracle", "_ref", "libfplus_ref_row_omp.so")
+    cpu = None
+    if not args.no_cpu and os.path.exists(ref_path):
+        os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+        ref = C.CDLL(ref_path)
+        ref.fplus_factor_row.argtypes = [C.c_int64, _lib.I64P, _lib.I64P, C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_int,
+                                         C.c_double, C.POINTER(_lib.Factors), C.c_char_p, C.c_size_t]
+        gomp = C.CDLL("libgomp.so.1")
+
+        def one(sm, check):
+            n, Rp, Ri, Rx = sm
+            f = _lib.Factors()
+            e = C.create_string_buffer(256)
+            rc = ref.fplus_factor_row(n, Rp.ctypes.data_as(_lib.I64P), Ri.ctypes.data_as(_lib.I64P), Rx.ctypes.data, dt, 0, 0,
+                                      strategy, _lib.FPLUS_PIVOT_REL_DEFAULT, C.byref(f), e, 256)
+            same = None
+            if rc == 0 and check is not None:
+                nU = f.Up[n]
+                same = all(not isinstance(d, Exception) and np.array_equal(np.ctypeslib.as_array(f.p, (n,)), d["p"])
+                           and np.array_equal(np.ctypeslib.as_array(f.Ui, (nU,)), d["Ui"])
+                           and np.array_equal(np.ctypeslib.as_array(C.cast(f.Ux, _lib.DP), (nU,)), d["Ux"]) for d in check)
+            if rc == 0:
+                _lib.lib().fplus_factors_free(C.byref(f))
+            return rc, same
+
+        t0 = time.perf_counter()
+        checks = [one(sm, [res[l][i] for l in res]) for i, sm in enumerate(systems)]
+        seq = time.perf_counter() - t0
+        same = all(c[1] for c in checks)
+        nxt, mu = [0], threading.Lock()
+
+        def worker():
+            gomp.omp_set_num_threads(1)
+            while True:
+                with mu:
+                    i = nxt[0]
+                    nxt[0] += 1
+                if i >= len(systems):
+                    return
+                one(systems[i], None)
+
+        t0 = time.perf_counter()
+        ths = [threading.Thread(target=worker) for _ in range(threads)]
+        for t in ths:
+            t.start()
+        for t in ths:
+            t.join()
+        par = time.perf_counter() - t0
+        cpu = {"sequential_s": seq, "threads_s": par}
+        log(f"reference CPU: {seq:.3f} s one after another (OpenMP {threads}), {par:.3f} s on {threads} threads one system "
+            f"each; device factors identical: {same}")
+        for leg in out:
+            out[leg].update({"speedup_vs_cpu_sequential": seq / out[leg]["s"], "speedup_vs_cpu_threads": par / out[leg]["s"],
+                             "same_factors_as_reference": bool(same)})
+    head = out[legs[0]]
+    line = {"metric": "row-kernel batch seconds", "value": head["s"], "unit": "s/batch", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": head["s"] * 1e3, "higher_is_better": False, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": args.workload, "systems": len(systems), "n_min": min(ns), "n_max": max(ns),
+                       "strategy": ["full", "partial", "rook"][strategy], "drop_tol": dt, "legs": legs}, "legs": out}
+    if cpu is not None:
+        line["cpu_baseline"] = {"value": min(cpu["sequential_s"], cpu["threads_s"]), "unit": "s/batch", "cores": threads,
+                                "kind": "reference",
+                                "sample": f"the same {len(systems)} systems through the reference's src/factor_row.c built with "
+                                          f"OpenMP: {cpu['sequential_s']:.3f} s one after another with {threads} OpenMP threads, "
+                                          f"{cpu['threads_s']:.3f} s on {threads} host threads one system each (value: the faster)"}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -532,6 +651,8 @@ def main():
         return sweep_main(args, kw, legs, NRHS)
     if args.workload.startswith("batch"):
         return batch_main(args, kw, NRHS)
+    if args.workload.startswith("rowlu-batch"):
+        return rowlu_batch_main(args, kw, legs)
     if args.workload.startswith("rowlu"):
         return rowlu_main(args, kw, legs)
     shift = args.workload.startswith("shiftinvert")

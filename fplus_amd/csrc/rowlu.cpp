@@ -11,10 +11,12 @@
 #include "symbolic.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdlib>
 #include <memory>
 #include <new>
+#include <thread>
 #include <type_traits>
 
 using namespace fpa;
@@ -108,6 +110,15 @@ struct RowSystem {
     const int64_t* Ri = nullptr;
     const double* d_Rx = nullptr;
     RowLuStatus* d_status = nullptr;   // the run's report (its own buffer, or a slot of a batch's array)
+    // the device arrays the runs use: the system's own buffers (R), or a batch's shared arenas until a pool grows
+    struct Ptrs {
+        int64_t *off = nullptr, *base = nullptr, *cap_end = nullptr, *p = nullptr, *q = nullptr;
+        int32_t *nnz = nullptr, *lead = nullptr, *slot = nullptr, *cand = nullptr, *scratch_col = nullptr, *cols = nullptr;
+        int32_t *log_row = nullptr, *log_step = nullptr, *rmaxc = nullptr;
+        double *vals = nullptr, *log_val = nullptr, *scratch_val = nullptr, *rmax = nullptr;
+        unsigned long long* tops = nullptr;
+    } P;
+    bool in_arena = false;             // P points into a batch's arenas, laid out and uploaded by the batch
     std::vector<int64_t> h_off, h_end, h_id;
     std::vector<int32_t> h_nnz, h_lead, h_slot, h_cols;
     int64_t pool_cap = 0, log_cap = 0;
@@ -138,18 +149,24 @@ struct RowSystem {
         log_cap = 4 * nnz + 2 * n + 1024;
     }
 
-    // a pool grown in place: the entries (or multipliers) so far copied into a larger buffer
-    template <class B> static void grow(B& buf, int64_t used, size_t per, size_t cap, hipStream_t s)
+    // a pool grown in place: the entries (or multipliers) so far copied into a larger buffer of the system's own
+    template <class B, class T> static void grow(B& buf, T*& cur, int64_t used, size_t per, size_t cap, hipStream_t s)
     {
         B nb(cap * per);
-        if (used > 0) hip_check(hipMemcpyAsync(nb.p, buf.p, (size_t)used * per * sizeof(*buf.p), hipMemcpyDeviceToDevice, s), "pool growth");
+        if (used > 0) hip_check(hipMemcpyAsync(nb.p, cur, (size_t)used * per * sizeof(T), hipMemcpyDeviceToDevice, s), "pool growth");
         buf = std::move(nb);
+        cur = buf.p;
     }
 
     // the buffers for the next run: laid out afresh, or the pool / log grown (GpuError when memory runs out)
     void prepare(hipStream_t s)
     {
-        if (fresh) {
+        if (fresh && in_arena) {   // the batch laid out and uploaded the first run's arrays
+            fresh = false;
+            in_arena = false;
+            k0 = 0;
+            resume = 0;
+        } else if (fresh) {
             R.off.alloc(n); R.base.alloc(n); R.cap_end.alloc(n); R.p.alloc(n); R.q.alloc(n);
             R.nnz.alloc(n); R.lead.alloc(n); R.slot.alloc(n); R.cand.alloc(n);
             R.scratch_col.alloc((size_t)16 * n);
@@ -169,20 +186,24 @@ struct RowSystem {
                 hip_check(hipMemcpyAsync(R.vals.p, d_Rx, (size_t)nnz * vs * sizeof(double), hipMemcpyDeviceToDevice, s), "row values");
             const unsigned long long tops[2] = {(unsigned long long)nnz, 0ull};
             R.tops.upload(tops, 2, s);
+            P.off = R.off.p; P.base = R.base.p; P.cap_end = R.cap_end.p; P.p = R.p.p; P.q = R.q.p;
+            P.nnz = R.nnz.p; P.lead = R.lead.p; P.slot = R.slot.p; P.cand = R.cand.p; P.scratch_col = R.scratch_col.p;
+            P.cols = R.cols.p; P.log_row = R.log_row.p; P.log_step = R.log_step.p; P.rmaxc = R.rmaxc.p;
+            P.vals = R.vals.p; P.log_val = R.log_val.p; P.scratch_val = R.scratch_val.p; P.rmax = R.rmax.p; P.tops = R.tops.p;
             fresh = false;
             k0 = 0;
             resume = 0;
         } else if (st.code == RL_POOL) {
             const int64_t used = std::min(st.pool_used, pool_cap);
             pool_cap *= 2;
-            grow(R.cols, used, 1, pool_cap, s);
-            grow(R.vals, used, vs, pool_cap, s);
+            grow(R.cols, P.cols, used, 1, pool_cap, s);
+            grow(R.vals, P.vals, used, vs, pool_cap, s);
         } else if (st.code == RL_LOG) {
             const int64_t used = std::min(st.log_used, log_cap);
             log_cap *= 2;
-            grow(R.log_row, used, 1, log_cap, s);
-            grow(R.log_step, used, 1, log_cap, s);
-            grow(R.log_val, used, vs, log_cap, s);
+            grow(R.log_row, P.log_row, used, 1, log_cap, s);
+            grow(R.log_step, P.log_step, used, 1, log_cap, s);
+            grow(R.log_val, P.log_val, used, vs, log_cap, s);
         }
     }
 
@@ -190,12 +211,12 @@ struct RowSystem {
     {
         RowLuArgs a{};
         a.n = (int32_t)n; a.strategy = strategy; a.drop_tol = drop_tol; a.pivot_rel = pivot_rel;
-        a.off = R.off.p; a.base = R.base.p; a.cap_end = R.cap_end.p;
-        a.nnz = R.nnz.p; a.lead = R.lead.p; a.slot = R.slot.p; a.p = R.p.p; a.q = R.q.p;
-        a.cols = R.cols.p; a.vals = R.vals.p; a.pool_cap = pool_cap; a.pool_top = R.tops.p;
-        a.log_row = R.log_row.p; a.log_step = R.log_step.p; a.log_val = R.log_val.p; a.log_cap = log_cap; a.log_top = R.tops.p + 1;
-        a.cand = R.cand.p; a.scratch_col = R.scratch_col.p; a.scratch_val = R.scratch_val.p; a.status = d_status;
-        a.rmax = R.rmax.p; a.rmaxc = R.rmaxc.p;
+        a.off = P.off; a.base = P.base; a.cap_end = P.cap_end;
+        a.nnz = P.nnz; a.lead = P.lead; a.slot = P.slot; a.p = P.p; a.q = P.q;
+        a.cols = P.cols; a.vals = P.vals; a.pool_cap = pool_cap; a.pool_top = P.tops;
+        a.log_row = P.log_row; a.log_step = P.log_step; a.log_val = P.log_val; a.log_cap = log_cap; a.log_top = P.tops + 1;
+        a.cand = P.cand; a.scratch_col = P.scratch_col; a.scratch_val = P.scratch_val; a.status = d_status;
+        a.rmax = P.rmax; a.rmaxc = P.rmaxc;
         a.k0 = k0; a.resume = resume;
         return a;
     }
@@ -221,8 +242,35 @@ struct RowSystem {
         return false;
     }
 
-    // the outcome: an error with the reference's messages, or the CSR factors assembled from the pools
-    int result(fplus_factors* out, char* err, size_t errlen, hipStream_t s)
+    // what the factors are assembled from, copied back from the device
+    std::vector<int64_t> h_doff, h_p, h_q;
+    std::vector<int32_t> h_dnnz, h_dslot, h_pc, h_lr, h_ls;
+    std::vector<double> h_pv, h_lv;
+
+    // the pools and logs back to the host (the per-row arrays too, unless the caller copied them: skip_rows)
+    void fetch(hipStream_t s, bool skip_rows = false)
+    {
+        if (st.code != RL_OK || n == 0) return;
+        if (!skip_rows) {
+            h_doff.resize(n); h_p.resize(n); h_q.resize(n); h_dnnz.resize(n); h_dslot.resize(n);
+            copy_d2h(h_doff.data(), P.off, n * sizeof(int64_t), s);
+            copy_d2h(h_dnnz.data(), P.nnz, n * sizeof(int32_t), s);
+            copy_d2h(h_dslot.data(), P.slot, n * sizeof(int32_t), s);
+            copy_d2h(h_p.data(), P.p, n * sizeof(int64_t), s);
+            copy_d2h(h_q.data(), P.q, n * sizeof(int64_t), s);
+        }
+        h_pc.resize(std::max<int64_t>(st.pool_used, 0));
+        h_pv.resize((size_t)std::max<int64_t>(st.pool_used, 0) * vs);
+        h_lr.resize(st.log_used); h_ls.resize(st.log_used); h_lv.resize((size_t)st.log_used * vs);
+        copy_d2h(h_pc.data(), P.cols, h_pc.size() * sizeof(int32_t), s);
+        copy_d2h(h_pv.data(), P.vals, h_pv.size() * sizeof(double), s);
+        copy_d2h(h_lr.data(), P.log_row, h_lr.size() * sizeof(int32_t), s);
+        copy_d2h(h_ls.data(), P.log_step, h_ls.size() * sizeof(int32_t), s);
+        copy_d2h(h_lv.data(), P.log_val, h_lv.size() * sizeof(double), s);
+    }
+
+    // the outcome: an error with the reference's messages, or the CSR factors assembled from what fetch copied back
+    int result(fplus_factors* out, char* err, size_t errlen)
     {
         std::memset(out, 0, sizeof(*out));
         if (st.code == RL_SINGULAR) {
@@ -238,19 +286,17 @@ struct RowSystem {
             return FPLUS_ERR_INTERNAL;
         }
         // U row i is the row at position i; L row i its logged multipliers in step order
-        std::vector<int64_t> d_off(n), p(n), q(n);
-        std::vector<int32_t> d_nnz(n), slot(n);
-        std::vector<int32_t> pc(st.pool_used > 0 ? st.pool_used : 0), lr(st.log_used), ls(st.log_used);
-        std::vector<double> pv((size_t)std::max<int64_t>(st.pool_used, 0) * vs), lv((size_t)st.log_used * vs);
-        if (n > 0) {
-            R.off.download(d_off.data(), n, s); R.nnz.download(d_nnz.data(), n, s); R.slot.download(slot.data(), n, s);
-            R.p.download(p.data(), n, s); R.q.download(q.data(), n, s);
-            R.cols.download(pc.data(), pc.size(), s); R.vals.download(pv.data(), pv.size(), s);
-            R.log_row.download(lr.data(), lr.size(), s); R.log_step.download(ls.data(), ls.size(), s);
-            R.log_val.download(lv.data(), lv.size(), s);
-            dev_sync(s);
-        }
-        std::vector<int64_t> pos(n);
+        const std::vector<int64_t>& d_off = h_doff;
+        const std::vector<int64_t>& p = h_p;
+        const std::vector<int64_t>& q = h_q;
+        const std::vector<int32_t>& d_nnz = h_dnnz;
+        const std::vector<int32_t>& slot = h_dslot;
+        const std::vector<int32_t>& pc = h_pc;
+        const std::vector<int32_t>& lr = h_lr;
+        const std::vector<int32_t>& ls = h_ls;
+        const std::vector<double>& pv = h_pv;
+        const std::vector<double>& lv = h_lv;
+    std::vector<int64_t> pos(n);
         for (int64_t i = 0; i < n; i++) pos[slot[i]] = i;
         auto dropped = [&](const double* v, int64_t j, int64_t i) { return drop_tol > 0.0 && j != i && absval(v, vs) <= drop_tol; };
         try {
@@ -378,7 +424,8 @@ int row_factor_device(int64_t n, const int64_t* Rp, const int64_t* Ri, const dou
         if (!S.next()) break;
     }
     const double t1 = now_s();
-    const int rc = S.result(out, err, errlen, s);
+    S.fetch(s);
+    const int rc = S.result(out, err, errlen);
     if (rc != FPLUS_OK) return rc;
     const RowSystem* one = &S;
     record_info(&one, 1, t0, t1);
@@ -394,21 +441,93 @@ int row_factor_device(int64_t n, const int64_t* Rp, const int64_t* Ri, const dou
     return FPLUS_OK;
 }
 
-// independent systems, one launch per round: every system that still needs a run gets one workgroup
-int row_factor_batch(int64_t nsys, const int64_t* n, const int64_t* const* Rp, const int64_t* const* Ri, const double* const* d_Rx,
+// independent systems, one launch per round: every system that still needs a run gets one workgroup.  The first
+// run's arrays of all systems live in shared arenas (one allocation and one upload per array), their inputs packed;
+// a system whose pool or log grows moves that buffer to one of its own.  The factors are assembled on host threads.
+int row_factor_batch(int64_t nsys, const int64_t* n, const int64_t* const* Rp, const int64_t* const* Ri, const double* const* Rx,
                      int vs, double drop_tol, int strategy, double pivot_rel, fplus_factors* out, int* rc, char* err, size_t errlen)
 {
     hipStream_t s = devctx().stream;
     std::vector<std::unique_ptr<RowSystem>> sys(nsys);
     DevBuf<RowLuStatus> d_status(nsys > 0 ? nsys : 1);
-    std::vector<int64_t> active;
+    // the arenas' offsets: per-row arrays, scratch, pools, logs, counters
+    std::vector<int64_t> o_row(nsys + 1, 0), o_pool(nsys + 1, 0), o_log(nsys + 1, 0), o_nnz(nsys + 1, 0);
     for (int64_t i = 0; i < nsys; i++) {
         sys[i].reset(new RowSystem());
         RowSystem& S = *sys[i];
         S.n = n[i]; S.vs = vs; S.strategy = strategy; S.drop_tol = drop_tol; S.pivot_rel = pivot_rel;
-        S.Rp = Rp[i]; S.Ri = Ri[i]; S.d_Rx = d_Rx[i];
+        S.Rp = Rp[i]; S.Ri = Ri[i];
         S.d_status = d_status.p + i;
         S.layout();
+        o_row[i + 1] = o_row[i] + S.n;
+        o_pool[i + 1] = o_pool[i] + S.pool_cap;
+        o_log[i + 1] = o_log[i] + S.log_cap;
+        o_nnz[i + 1] = o_nnz[i] + S.nnz;
+    }
+    const int64_t NR = o_row[nsys], NP = o_pool[nsys], NL = o_log[nsys], NZ = o_nnz[nsys];
+    DevBuf<int64_t> a_off, a_base, a_end, a_p, a_q;
+    DevBuf<int32_t> a_nnz, a_lead, a_slot, a_cand, a_scol, a_cols, a_lrow, a_lstep, a_rmaxc, a_incols;
+    DevBuf<double> a_vals, a_lval, a_sval, a_rmax, a_in;
+    DevBuf<unsigned long long> a_tops;
+    try {
+        a_off.alloc(NR); a_base.alloc(NR); a_end.alloc(NR); a_p.alloc(NR); a_q.alloc(NR);
+        a_nnz.alloc(NR); a_lead.alloc(NR); a_slot.alloc(NR); a_cand.alloc(NR); a_rmaxc.alloc(NR); a_rmax.alloc(NR);
+        a_scol.alloc((size_t)16 * NR); a_sval.alloc((size_t)16 * NR * vs);
+        a_cols.alloc(NP); a_vals.alloc((size_t)NP * vs);
+        a_lrow.alloc(NL); a_lstep.alloc(NL); a_lval.alloc((size_t)NL * vs);
+        a_tops.alloc(2 * (nsys > 0 ? nsys : 1));
+        a_in.alloc((size_t)(NZ > 0 ? NZ : 1) * vs);
+        a_incols.alloc(NZ > 0 ? NZ : 1);
+    } catch (const GpuError&) {
+        errf(err, errlen, "out of memory (device: %ld rows, %ld pool entries, %ld multipliers in the batch)", (long)NR, (long)NP, (long)NL);
+        return FPLUS_ERR_NOMEM;
+    }
+    {   // the first run's per-row arrays and counters, packed; the inputs' columns and values through a staging area
+        std::vector<int64_t> off(NR), end(NR), id(NR);
+        std::vector<int32_t> nz(NR), lead(NR), slot(NR), cols(NZ > 0 ? NZ : 1);
+        std::vector<double> vals((size_t)(NZ > 0 ? NZ : 1) * vs);
+        std::vector<unsigned long long> tops(2 * (nsys > 0 ? nsys : 1), 0ull);
+        for (int64_t i = 0; i < nsys; i++) {
+            const RowSystem& S = *sys[i];
+            std::copy(S.h_off.begin(), S.h_off.end(), off.begin() + o_row[i]);
+            std::copy(S.h_end.begin(), S.h_end.end(), end.begin() + o_row[i]);
+            std::copy(S.h_id.begin(), S.h_id.end(), id.begin() + o_row[i]);
+            std::copy(S.h_nnz.begin(), S.h_nnz.end(), nz.begin() + o_row[i]);
+            std::copy(S.h_lead.begin(), S.h_lead.end(), lead.begin() + o_row[i]);
+            std::copy(S.h_slot.begin(), S.h_slot.end(), slot.begin() + o_row[i]);
+            std::copy(S.h_cols.begin(), S.h_cols.end(), cols.begin() + o_nnz[i]);
+            std::copy(Rx[i], Rx[i] + S.nnz * vs, vals.begin() + o_nnz[i] * vs);
+            tops[2 * i] = (unsigned long long)S.nnz;
+        }
+        a_off.upload(off.data(), NR, s); a_base.upload(off.data(), NR, s); a_end.upload(end.data(), NR, s);
+        a_p.upload(id.data(), NR, s); a_q.upload(id.data(), NR, s);
+        a_nnz.upload(nz.data(), NR, s); a_lead.upload(lead.data(), NR, s); a_slot.upload(slot.data(), NR, s);
+        a_tops.upload(tops.data(), 2 * nsys, s);
+        a_incols.upload(cols.data(), NZ, s);
+        a_in.upload(vals.data(), (size_t)NZ * vs, s);
+        for (int64_t i = 0; i < nsys; i++) {
+            const int64_t z = sys[i]->nnz;
+            if (z == 0) continue;
+            hip_check(hipMemcpyAsync(a_cols.p + o_pool[i], a_incols.p + o_nnz[i], z * sizeof(int32_t), hipMemcpyDeviceToDevice, s),
+                      "batch columns");
+            hip_check(hipMemcpyAsync(a_vals.p + o_pool[i] * vs, a_in.p + o_nnz[i] * vs, z * vs * sizeof(double), hipMemcpyDeviceToDevice, s),
+                      "batch values");
+        }
+    }
+    std::vector<int64_t> active;
+    for (int64_t i = 0; i < nsys; i++) {
+        RowSystem& S = *sys[i];
+        auto& P = S.P;
+        P.off = a_off.p + o_row[i]; P.base = a_base.p + o_row[i]; P.cap_end = a_end.p + o_row[i];
+        P.p = a_p.p + o_row[i]; P.q = a_q.p + o_row[i];
+        P.nnz = a_nnz.p + o_row[i]; P.lead = a_lead.p + o_row[i]; P.slot = a_slot.p + o_row[i]; P.cand = a_cand.p + o_row[i];
+        P.rmaxc = a_rmaxc.p + o_row[i]; P.rmax = a_rmax.p + o_row[i];
+        P.scratch_col = a_scol.p + 16 * o_row[i]; P.scratch_val = a_sval.p + 16 * o_row[i] * vs;
+        P.cols = a_cols.p + o_pool[i]; P.vals = a_vals.p + o_pool[i] * vs;
+        P.log_row = a_lrow.p + o_log[i]; P.log_step = a_lstep.p + o_log[i]; P.log_val = a_lval.p + o_log[i] * vs;
+        P.tops = a_tops.p + 2 * i;
+        S.d_Rx = a_in.p + o_nnz[i] * vs;   // a system that starts over lays out its rows again from here
+        S.in_arena = true;
         if (S.n > 0) active.push_back(i);
     }
     const double t0 = now_s();
@@ -439,10 +558,42 @@ int row_factor_batch(int64_t nsys, const int64_t* n, const int64_t* const* Rp, c
         active.swap(again);
     }
     const double t1 = now_s();
-    for (int64_t i = 0; i < nsys; i++) {
-        char e2[256];
-        rc[i] = sys[i]->result(&out[i], e2, sizeof e2, s);
+    // the per-row arrays of every system back in one copy each (a system that restarted keeps its own), then each
+    // system's pool and log, then the factors assembled on host threads
+    {
+        std::vector<int64_t> off(NR), p(NR), q(NR);
+        std::vector<int32_t> nz(NR), slot(NR);
+        a_off.download(off.data(), NR, s); a_p.download(p.data(), NR, s); a_q.download(q.data(), NR, s);
+        a_nnz.download(nz.data(), NR, s); a_slot.download(slot.data(), NR, s);
+        for (int64_t i = 0; i < nsys; i++) {
+            RowSystem& S = *sys[i];
+            const bool own = S.P.off != a_off.p + o_row[i];
+            if (!own && S.st.code == RL_OK && S.n > 0) {
+                S.h_doff.assign(off.begin() + o_row[i], off.begin() + o_row[i + 1]);
+                S.h_p.assign(p.begin() + o_row[i], p.begin() + o_row[i + 1]);
+                S.h_q.assign(q.begin() + o_row[i], q.begin() + o_row[i + 1]);
+                S.h_dnnz.assign(nz.begin() + o_row[i], nz.begin() + o_row[i + 1]);
+                S.h_dslot.assign(slot.begin() + o_row[i], slot.begin() + o_row[i + 1]);
+            }
+            S.fetch(s, !own);
+        }
     }
+    std::vector<int> r(nsys, FPLUS_OK);
+    {
+        std::atomic<int64_t> next{0};
+        auto work = [&]() {
+            for (int64_t i; (i = next.fetch_add(1)) < nsys;) {
+                char e2[256];
+                r[i] = sys[i]->result(&out[i], e2, sizeof e2);
+            }
+        };
+        const int nt = (int)std::min<int64_t>(std::max(1, host_threads()), std::max<int64_t>(nsys, 1));
+        std::vector<std::thread> th;
+        for (int t = 1; t < nt; t++) th.emplace_back(work);
+        work();
+        for (auto& t : th) t.join();
+    }
+    for (int64_t i = 0; i < nsys; i++) rc[i] = r[i];
     std::vector<const RowSystem*> ptrs;
     for (auto& S : sys) ptrs.push_back(S.get());
     record_info(ptrs.data(), ptrs.size(), t0, t1);
@@ -604,18 +755,11 @@ int fplus_amd_factor_row_batch(int64_t nsys, const int64_t* n, const int64_t* co
         idx.push_back(i); bn.push_back(n[i]); bp.push_back(indptr[i]); bi.push_back(indices[i]);
     }
     try {
-        hipStream_t s = devctx().stream;
-        std::vector<DevBuf<double>> vals(idx.size());
-        std::vector<const double*> dx(idx.size());
-        for (size_t t = 0; t < idx.size(); t++) {
-            const int64_t nnz = indptr[idx[t]][n[idx[t]]];
-            vals[t].alloc((size_t)(nnz > 0 ? nnz : 1) * vs);
-            vals[t].upload(static_cast<const double*>(data[idx[t]]), (size_t)nnz * vs, s);
-            dx[t] = vals[t].p;
-        }
+        std::vector<const double*> hx(idx.size());
+        for (size_t t = 0; t < idx.size(); t++) hx[t] = static_cast<const double*>(data[idx[t]]);
         std::vector<fplus_factors> o(idx.size());
         std::vector<int> r(idx.size());
-        const int ret = row_factor_batch((int64_t)idx.size(), bn.data(), bp.data(), bi.data(), dx.data(), vs, drop_tol, pivot_strategy,
+        const int ret = row_factor_batch((int64_t)idx.size(), bn.data(), bp.data(), bi.data(), hx.data(), vs, drop_tol, pivot_strategy,
                                          pivot_rel, o.data(), r.data(), err, errlen);
         if (ret != FPLUS_OK) {
             for (auto& f : o) fplus_factors_free(&f);
