@@ -86,6 +86,8 @@ struct RefactorStatus {
     int32_t err_col;
     int32_t nreplaced;
     unsigned long long min_ratio_bits;
+    uint32_t ticket;     // lu_nopiv: its workgroups' arrivals after reading the pivot block (0 between launches)
+    uint32_t pad;
 };
 
 // one block column copied out of the panel workspace: dst[r] = Wl[pos[pos_off + r] + ci * ldw], r < nrows; for a

@@ -1004,8 +1004,8 @@ __global__ __launch_bounds__(TB) void k_gemm_sub(double* __restrict__ Wd, int64_
 // The lower section's rows are already in pivot order, so sub-panel [a, a + nb) is an LU without row exchanges: every
 // workgroup factors the sub-panel's nb x nb diagonal block in LDS (the same operations in every workgroup, so the same
 // values), then its own rows below (L21 = A21 U11^-1, one row per thread, the sub-panel's nb values in registers);
-// workgroup 0 writes the diagonal block back and solves the block's rows in the columns right of the sub-panel
-// (U12 = L11^-1 A12, one column per thread).  The operations per entry are those of the pivoting kernels (divide by
+// the workgroup that reads the diagonal block last writes it back factored, and the workgroups share the solve of the
+// block's rows in the columns right of the sub-panel (U12 = L11^-1 A12, sixteen lanes per column).  The operations per entry are those of the pivoting kernels (divide by
 // the pivot, then subtract the multiple of the pivot row column by column).  A pivot whose modulus is below
 // pivot_rel * colmax of its column: mode 0 records the first such column in st (the refactorization fails and the
 // caller factors with pivoting); mode 1 replaces the pivot by pivot_rel * colmax with the pivot's phase (static
@@ -1020,9 +1020,17 @@ __global__ __launch_bounds__(TB) void k_lu_nopiv(double* __restrict__ Wd, int64_
     T* Wl = reinterpret_cast<T*>(Wd);
     const int tid = threadIdx.x, g = blockIdx.x;
     const int dr = tid % PNB, dc = tid / PNB;   // TB = PNB * PNB: one element of the diagonal block per thread
+    __shared__ int sh_last;
     D[dr][dc] = (dr < nb && dc < nb) ? Wl[(a + dr) + (int64_t)(a + dc) * ld] : Sc<T>::zero();
     if (tid < PNB) cmx[tid] = tid < nb ? colmax[a + tid] : 0.0;
     __syncthreads();
+    // the factored pivot block goes back to W from the workgroup that read it last: its arrival ticket, taken once the
+    // block is in LDS, is the launch's last one only when every workgroup has read the block (a workgroup may start
+    // only after another one has finished, when the device is busy with other streams' work)
+    if (tid == 0) {
+        __threadfence();
+        sh_last = atomicAdd(&st->ticket, 1u) == gridDim.x - 1;
+    }
     // workgroup 0's statistics, kept in thread 0 and posted once after the loop (a barrier waits for a thread's
     // memory operations in flight: an atomic per column would put its round trip on every step)
     int32_t n_small = 0, first_small = -1;
@@ -1082,9 +1090,10 @@ __global__ __launch_bounds__(TB) void k_lu_nopiv(double* __restrict__ Wd, int64_
         for (int c = 0; c < PNB; c++)
             if (c < nb) Wl[row + (int64_t)(a + c) * ld] = x[c];
     }
-    if (g == 0) {
+    if (sh_last) {
         const int r = tid % PNB, c = tid / PNB;
         if (r < nb && c < nb) Wl[(a + r) + (int64_t)(a + c) * ld] = D[r][c];
+        if (tid == 0) atomicExch(&st->ticket, 0u);   // every arrival of this launch is in: ready for the next one
     }
     // the sub-panel's pivot rows in the columns right of it, L11 y = b: sixteen lanes per column, lane i holding y_i and
     // taking y_k from lane k of its group (a column per thread would keep the whole triangle of L11 in registers); the

@@ -1430,6 +1430,11 @@ extern "C" int fplus_amd_tune(const char* name, int64_t value)
         set_refactor_graph(value != 0);
         return FPLUS_OK;
     }
+    if (std::strcmp(name, "refactor_pipeline") == 0) {
+        if (value != 0 && value != 1) return FPLUS_ERR_BADARG;
+        set_refactor_pipeline(value != 0);
+        return FPLUS_OK;
+    }
     if (std::strcmp(name, "host_threads") == 0) {
         if (!set_host_threads((int)std::max<int64_t>(std::min<int64_t>(value, 65), -1))) return FPLUS_ERR_BADARG;
         return FPLUS_OK;

@@ -64,6 +64,9 @@ bool update_atomic_rows() { return g_atomic_rows.load(std::memory_order_relaxed)
 void set_update_atomic_rows(bool on) { g_atomic_rows.store(on, std::memory_order_relaxed); }
 static std::atomic<bool> g_refactor_graph{false};
 bool refactor_graph() { return g_refactor_graph.load(std::memory_order_relaxed); }
+static std::atomic<bool> g_refactor_pipeline{true};
+bool refactor_pipeline() { return g_refactor_pipeline.load(std::memory_order_relaxed); }
+void set_refactor_pipeline(bool on) { g_refactor_pipeline.store(on, std::memory_order_relaxed); }
 void set_refactor_graph(bool on) { g_refactor_graph.store(on, std::memory_order_relaxed); }
 
 namespace {
@@ -863,6 +866,7 @@ int Driver::panel(int64_t j0, int64_t j1)
         for (int32_t L = 1; L <= maxlevel; L++)
             if (lev_gt[L + 1] > lev_gt[L]) rec_->levels.push_back(ReplayLevel{tb + lev_gt[L], lev_gt[L + 1] - lev_gt[L], lev_rt[L], lev_flops[L]});
         P.nlev = (int64_t)rec_->levels.size() - P.lev_off;
+
         rec_->max_w = std::max(rec_->max_w, ldw * np * vs_);
         rec_->max_x = std::max(rec_->max_x, xtot * vs_);
         rec_->max_np = std::max(rec_->max_np, np);
@@ -1232,6 +1236,8 @@ int Driver::panel_out()
     up_.end(s_);
     if (rec_) {   // (rec_panel_ is the main thread's again only after it joined this job)
         ReplayPanel& P = rec_panel_;
+        P.blk0 = new_blocks_.empty() ? (int64_t)F_.blocks.size() : (int64_t)*std::min_element(new_blocks_.begin(), new_blocks_.end());
+        P.blk1 = new_blocks_.empty() ? P.blk0 : (int64_t)*std::max_element(new_blocks_.begin(), new_blocks_.end()) + 1;
         P.ex_off = (int64_t)rec_->h_ex.size();
         P.nex = (int64_t)ex.size();
         rec_->h_ex.insert(rec_->h_ex.end(), ex.begin(), ex.end());
@@ -1418,10 +1424,29 @@ template <class T> void up_vec(DevBuf<T>& d, std::vector<T>& h, hipStream_t s)
 Replay::~Replay()
 {
     if (gexec) (void)hipGraphExecDestroy(gexec);
+    for (hipEvent_t e : ev_pre) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ev_done) (void)hipEventDestroy(e);
+    if (s_side) (void)hipStreamDestroy(s_side);
 }
 
 void Replay::upload(hipStream_t s)
 {
+    // the levels a pipelined replay may run while the previous panel finishes: those before the first level with a
+    // tile of a block the previous panel made (its values and inverse form are written at that panel's end)
+    for (size_t p = 0; p < panels.size(); p++) {
+        ReplayPanel& P = panels[p];
+        P.npre = p == 0 ? P.nlev : 0;
+        if (p == 0) continue;
+        const int64_t b0 = panels[p - 1].blk0, b1 = panels[p - 1].blk1;
+        int64_t L = P.lev_off;
+        for (; L < P.lev_off + P.nlev; L++) {
+            const ReplayLevel& V = levels[L];
+            bool dep = false;
+            for (int64_t t = V.tile0; t < V.tile0 + V.ntile && !dep; t++) dep = h_tiles[t].block >= b0 && h_tiles[t].block < b1;
+            if (dep) break;
+        }
+        P.npre = L - P.lev_off;
+    }
     up_vec(d_rows, h_rows, s);
     up_vec(d_cols, h_cols, s);
     up_vec(d_expos, h_expos, s);
@@ -1470,6 +1495,8 @@ int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, c
     st0.err_col = 0;
     st0.nreplaced = 0;
     st0.min_ratio_bits = 0x7FF0000000000000ULL;   // +inf
+    st0.ticket = 0;
+    st0.pad = 0;
     R.d_st.upload(&st0, 1, s);
     if (precision == 1) k::fill_u8(F.d_blk_f32.p, (int64_t)F.blocks.size(), 1, s);
     else k::fill_u8(F.d_blk_f32.p, (int64_t)F.blocks.size(), 0, s);
@@ -1487,39 +1514,94 @@ int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, c
         }
     }
     const BlockDesc* blocks = F.d_blocks.p;
-    auto replay = [&] {
-    for (const ReplayPanel& P : R.panels) {
+    // one workspace set: the panel workspace W, the solved segments X, the column maxima, the row map
+    struct Buf {
+        double *W, *X, *colmax;
+        int32_t* rowmap;
+    };
+    auto setup = [&](const ReplayPanel& P, const Buf& B, hipStream_t st) {
         const int32_t* rows = R.d_rows.p + P.rows_off;
         const int64_t nrow = P.Ru + P.Rl, ldw = P.ldw;
-        k::rowmap_set(nrow, rows, R.d_rowmap.p, 0, s);
-        k::fill_f64(R.d_W.p, ldw * P.np * vs, 0.0, s);
-        k::scatter_cols(vs, (int32_t)P.np, d_Bp, d_Bi, d_Bx, P.j0, R.d_rowmap.p, R.d_W.p, ldw, R.d_colmax.p, s);
+        k::rowmap_set(nrow, rows, B.rowmap, 0, st);
+        k::fill_f64(B.W, ldw * P.np * vs, 0.0, st);
+        k::scatter_cols(vs, (int32_t)P.np, d_Bp, d_Bi, d_Bx, P.j0, B.rowmap, B.W, ldw, B.colmax, st);
+    };
+    auto levels = [&](const ReplayPanel& P, int64_t L0, int64_t L1, const Buf& B, hipStream_t st) {
         const int32_t* cols = R.d_cols.p + P.cols_off;
-        for (int64_t L = P.lev_off; L < P.lev_off + P.nlev; L++) {
+        for (int64_t L = L0; L < L1; L++) {
             const ReplayLevel& V = R.levels[L];
-            k::blk_upd(vs, V.ntile, R.d_tiles.p + V.tile0, cols, R.d_rowmap.p, R.d_W.p, ldw, V.rt, V.flops, R.d_X.p, precision,
-                       F.d_blk_f32.p, F.d_blk_err.p, s);
+            k::blk_upd(vs, V.ntile, R.d_tiles.p + V.tile0, cols, B.rowmap, B.W, P.ldw, V.rt, V.flops, B.X, precision,
+                       F.d_blk_f32.p, F.d_blk_err.p, st);
         }
-        if (P.ntask) k::x_back(vs, (int32_t)P.ntask, R.d_tasks.p + P.tasks_off, cols, blocks, R.d_W.p, ldw, R.d_X.p, s);
-        double* Wl = R.d_W.p + P.Ru * vs;
+    };
+    auto tail = [&](const ReplayPanel& P, const Buf& B, hipStream_t st) {
+        const int32_t* rows = R.d_rows.p + P.rows_off;
+        const int64_t nrow = P.Ru + P.Rl, ldw = P.ldw;
+        const int32_t* cols = R.d_cols.p + P.cols_off;
+        if (P.ntask) k::x_back(vs, (int32_t)P.ntask, R.d_tasks.p + P.tasks_off, cols, blocks, B.W, ldw, B.X, st);
+        double* Wl = B.W + P.Ru * vs;
         for (int64_t a = 0; a < P.np; a += FPA_PANEL_NB) {
             const int32_t nb = (int32_t)std::min<int64_t>(FPA_PANEL_NB, P.np - a);
-            k::lu_nopiv(vs, Wl, ldw, (int32_t)P.Rl, (int32_t)P.np, (int32_t)a, nb, R.d_colmax.p, pivot_rel, static_pivots, P.j0,
-                        R.d_st.p, s);
-            k::gemm_sub(vs, Wl, ldw, (int32_t)P.Rl, (int32_t)P.np, (int32_t)a, nb, nullptr, s);
+            k::lu_nopiv(vs, Wl, ldw, (int32_t)P.Rl, (int32_t)P.np, (int32_t)a, nb, B.colmax, pivot_rel, static_pivots, P.j0,
+                        R.d_st.p, st);
+            k::gemm_sub(vs, Wl, ldw, (int32_t)P.Rl, (int32_t)P.np, (int32_t)a, nb, nullptr, st);
         }
-        if (P.nex) k::extract_cols(vs, (int32_t)P.nex, R.d_ex.p + P.ex_off, R.d_expos.p + P.expos_off, rows + P.Ru, Wl, ldw, s);
+        if (P.nex) k::extract_cols(vs, (int32_t)P.nex, R.d_ex.p + P.ex_off, R.d_expos.p + P.expos_off, rows + P.Ru, Wl, ldw, st);
         if (P.nij)
-            k::blk_inverse(vs, (int32_t)P.nij, R.d_ij.p + P.ij_off, (int32_t)P.nit, R.d_it.p + P.it_off, blocks, P.iflops, s);
+            k::blk_inverse(vs, (int32_t)P.nij, R.d_ij.p + P.ij_off, (int32_t)P.nit, R.d_it.p + P.it_off, blocks, P.iflops, st);
         if ((precision == 2 || precision == 3) && P.nnb) {
-            k::block_features(vs, (int32_t)P.nnb, R.d_newblk.p + P.nb_off, blocks, R.d_colmax.p, P.j0, sel.d_Dr, sel.d_Dc, n,
-                              F.d_featmax.p, F.d_feat.p, s);
+            k::block_features(vs, (int32_t)P.nnb, R.d_newblk.p + P.nb_off, blocks, B.colmax, P.j0, sel.d_Dr, sel.d_Dc, n,
+                              F.d_featmax.p, F.d_feat.p, st);
             if (precision == 2)
-                k::sel_infer(P.nnb, R.d_newblk.p + P.nb_off, F.d_feat.p, sel.d_model, sel.threshold, nullptr, F.d_blk_f32.p, s);
+                k::sel_infer(P.nnb, R.d_newblk.p + P.nb_off, F.d_feat.p, sel.d_model, sel.threshold, nullptr, F.d_blk_f32.p, st);
         }
-        if (P.nruns) k::u_runs(vs, (int32_t)P.nruns, R.d_runs.p + P.runs_off, R.d_W.p, F.d_Ux.p, F.d_Ui.p, s);
-        k::rowmap_set(nrow, rows, R.d_rowmap.p, 1, s);
-    }
+        if (P.nruns) k::u_runs(vs, (int32_t)P.nruns, R.d_runs.p + P.runs_off, B.W, F.d_Ux.p, F.d_Ui.p, st);
+        k::rowmap_set(nrow, rows, B.rowmap, 1, st);
+    };
+    const Buf B0{R.d_W.p, R.d_X.p, R.d_colmax.p, R.d_rowmap.p};
+    auto replay = [&] {
+        for (const ReplayPanel& P : R.panels) {
+            setup(P, B0, s);
+            levels(P, P.lev_off, P.lev_off + P.nlev, B0, s);
+            tail(P, B0, s);
+        }
+    };
+    // the pipelined replay: panel p + 1's set-up and its levels that read no block of panel p run on a second stream
+    // (with the second workspace set) while panel p finishes on the first; panel p + 1 waits for panel p - 1's end
+    // (its workspace set free, the blocks it reads made), and its remaining levels wait for its first ones
+    auto replay_pipelined = [&] {
+        const size_t np = R.panels.size();
+        if (!R.s_side) hip_check(hipStreamCreateWithFlags(&R.s_side, hipStreamNonBlocking), "hipStreamCreateWithFlags");
+        while (R.ev_pre.size() < np + 1) {
+            hipEvent_t a, b;
+            hip_check(hipEventCreateWithFlags(&a, hipEventDisableTiming), "hipEventCreateWithFlags");
+            hip_check(hipEventCreateWithFlags(&b, hipEventDisableTiming), "hipEventCreateWithFlags");
+            R.ev_pre.push_back(a);
+            R.ev_done.push_back(b);
+        }
+        const Buf B1{R.d_W2.p, R.d_X2.p, R.d_colmax2.p, R.d_rowmap2.p};
+        hipStream_t side = R.s_side;
+        hip_check(hipEventRecord(R.ev_pre[np], s), "hipEventRecord");   // the set-up above is done
+        hip_check(hipStreamWaitEvent(side, R.ev_pre[np], 0), "hipStreamWaitEvent");
+        const ReplayPanel& P0 = R.panels[0];
+        setup(P0, B0, s);
+        levels(P0, P0.lev_off, P0.lev_off + P0.npre, B0, s);
+        for (size_t p = 0; p < np; p++) {
+            const Buf& Bp = (p & 1) ? B1 : B0;
+            if (p + 1 < np) {
+                const ReplayPanel& Q = R.panels[p + 1];
+                const Buf& Bq = ((p + 1) & 1) ? B1 : B0;
+                if (p >= 1) hip_check(hipStreamWaitEvent(side, R.ev_done[p - 1], 0), "hipStreamWaitEvent");
+                setup(Q, Bq, side);
+                levels(Q, Q.lev_off, Q.lev_off + Q.npre, Bq, side);
+                hip_check(hipEventRecord(R.ev_pre[p + 1], side), "hipEventRecord");
+            }
+            const ReplayPanel& P = R.panels[p];
+            if (p >= 1) hip_check(hipStreamWaitEvent(s, R.ev_pre[p], 0), "hipStreamWaitEvent");
+            levels(P, P.lev_off + P.npre, P.lev_off + P.nlev, Bp, s);
+            tail(P, Bp, s);
+            hip_check(hipEventRecord(R.ev_done[p], s), "hipEventRecord");
+        }
     };
     // the panel loop as launched, or through the graph captured from it (no host work and no allocation inside it:
     // every buffer is reserved above, the status reset and read around it)
@@ -1556,6 +1638,15 @@ int sn_refactor_device(DevFactor& F, const int64_t* d_Bp, const int32_t* d_Bi, c
             R.gkey = key;
         }
         hip_check(hipGraphLaunch(R.gexec, s), "hipGraphLaunch");
+    } else if (refactor_pipeline() && R.panels.size() > 1) {
+        R.d_W2.reserve((size_t)R.max_w);
+        R.d_X2.reserve((size_t)R.max_x);
+        R.d_colmax2.reserve((size_t)R.max_np);
+        if (R.d_rowmap2.n < (size_t)std::max<int64_t>(n, 1)) {
+            R.d_rowmap2.alloc((size_t)std::max<int64_t>(n, 1));
+            k::fill_i32(R.d_rowmap2.p, n, -1, s);
+        }
+        replay_pipelined();
     } else {
         replay();
     }

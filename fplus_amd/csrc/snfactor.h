@@ -27,6 +27,8 @@ struct ReplayPanel {
     double iflops;
     int64_t runs_off, nruns;
     int64_t nb_off, nnb;           // the panel's new blocks (their features and precision marks)
+    int64_t blk0, blk1;            // the blocks the panel made at its end (ids blk0 .. blk1 - 1)
+    int64_t npre;                  // the update levels before the first one that reads a block of the previous panel
 };
 struct Replay {
     std::vector<ReplayPanel> panels;
@@ -49,9 +51,11 @@ struct Replay {
     DevBuf<URun> d_runs;
     std::vector<DevBuf<double>> iv_chunks;   // the blocks' inverse forms (rewritten by every refactorization)
     int64_t max_w = 1, max_x = 1, max_np = 1;  // workspace, solved-segment buffer (scalars), widest panel
-    // a refactorization's workspace, kept across refactorizations
-    DevBuf<double> d_W, d_X, d_colmax;
-    DevBuf<int32_t> d_rowmap;
+    // a refactorization's workspace, kept across refactorizations; the pipelined replay's second set and its stream
+    DevBuf<double> d_W, d_X, d_colmax, d_W2, d_X2, d_colmax2;
+    DevBuf<int32_t> d_rowmap, d_rowmap2;
+    hipStream_t s_side = nullptr;
+    std::vector<hipEvent_t> ev_pre, ev_done;
     DevBuf<RefactorStatus> d_st;
     // the captured replay (refactor_graph) and what it was captured with
     hipGraphExec_t gexec = nullptr;
@@ -146,6 +150,10 @@ void set_xcd_tile_order(bool on);
 // refactorizations replayed through a captured HIP graph (tuning knob "refactor_graph"): the panel loop's launches are
 // captured once per set of buffers and arguments and the graph is launched for later refactorizations
 bool refactor_graph();
+// refactorizations replayed as a pipeline of two panels (tuning knob "refactor_pipeline", default on): a panel's update
+// levels that read no block of the previous panel run on a second stream while the previous panel finishes
+bool refactor_pipeline();
+void set_refactor_pipeline(bool on);
 void set_refactor_graph(bool on);
 // the update tile height per dependency level (tuning knob "update_tile_policy", 0 to 4; see snfactor.cpp)
 int update_tile_policy();

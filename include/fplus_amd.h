@@ -206,7 +206,9 @@ int fplus_amd_thread_stream(int on);
  * "xcd_tile_order": 1 (default) orders each update level's tiles so that those reading one segment slice run on one
  * XCD, 0 keeps them in task order (for measurements; applies to factorizations made afterwards).  "refactor_graph": 1
  * replays refactorizations through a HIP graph captured once per factorization and set of arguments, 0 (default)
- * launches them one by one.  "update_tile_policy": the block updates' tile height per dependency level, for
+ * launches them one by one.  "refactor_pipeline": 1 (default) replays a refactorization as a pipeline of two panels on
+ * two streams -- a panel's update levels that read no block of the previous panel run while that panel finishes -- 0
+ * one panel after another (the graph replay is always one after another).  "update_tile_policy": the block updates' tile height per dependency level, for
  * factorizations made afterwards -- 0 (default) 64-row tiles for real levels with at least 4 tiles per compute unit and
  * 32-row tiles otherwise, 1 all 32-row, 2 all 64-row, 3 and 4 the real rule for real and complex with the bound at 4 and
  * 16 tiles per compute unit (fplus_amd.autotune times them).  "host_threads": the host threads of each factorization's

@@ -146,3 +146,50 @@ def test_static_pivots():
     m = acc.all_measures(A2, x, b, num.Dr, num.Dc)
     print(f"static pivots: replaced {st['refactor_replaced']}, refined {info}, {m}")
     assert info["converged"] == 1 and m["equilibrated"] < 1e-14
+
+
+def test_pipelined_replay_matches_one_panel_after_another():
+    """the pipelined replay (tuning knob refactor_pipeline: a panel's update levels that read no block of the previous
+    panel run on a second stream while that panel finishes) gives the factors of the replay one panel after another --
+    the same pivots and patterns, values to the atomic updates' round-off -- on a pencil of 32 panels"""
+    n, Ap, Ai, Ax = gen.shift_invert(8, 4, 256, sigma=0.3 + 0.7j)
+    _, _, _, Ax2 = gen.shift_invert(8, 4, 256, sigma=0.3 + 0.71j)
+    sym = fplus_amd.Symbolic(n, Ap, Ai)
+    num = fplus_amd.Numeric(sym, Ap, Ai, Ax, amd_options=fplus_amd.amd_options(refactor=1))
+    A2 = sp.csc_matrix((Ax2, Ai, Ap), shape=(n, n))
+    b = gen.rhs(n, 2, True)
+    try:
+        out = {}
+        for pipe in (0, 1, 1):
+            fplus_amd.tune("refactor_pipeline", pipe)
+            num.refactor(Ax2)
+            L, U, _ = _lu(num)
+            out.setdefault(pipe, []).append((L, U, num.p.copy()))
+            x, info = num.solve_refined(b)
+            assert acc.equilibrated(A2, x, b, num.Dr, num.Dc) < 1e-14
+        (L0, U0, p0), = out[0]
+        for L1, U1, p1 in out[1]:
+            assert np.array_equal(p1, p0) and same_pattern(L0, L1) and same_pattern(U0, U1)
+            dl, du = norm_diff(L1, L0), norm_diff(U1, U0)
+            print(f"pipelined vs one panel after another: L {dl:.2e}, U {du:.2e}")
+            assert dl < 1e-11 and du < 1e-11
+    finally:
+        fplus_amd.tune("refactor_pipeline", 1)
+
+
+def test_pipelined_replay_of_a_large_pencil():
+    """a pencil of 131072 unknowns (512 panels) refactored, pipelined, in the pivot order of its neighbouring shift: the
+    panel LU without row exchanges writes its factored pivot block back only after every workgroup has read it, also
+    when the second stream's work delays some of its workgroups (a refactorization that read half-factored pivot blocks
+    fails the pivot check or the accuracy bar)"""
+    n, Ap, Ai, Ax = gen.shift_invert(32, 4, 1024, sigma=0.3 + 0.7j)
+    _, _, _, Ax2 = gen.shift_invert(32, 4, 1024, sigma=0.3 + 0.69j)
+    sym = fplus_amd.Symbolic(n, Ap, Ai)
+    num = fplus_amd.Numeric(sym, Ap, Ai, Ax2, amd_options=fplus_amd.amd_options(refactor=1))
+    A = sp.csc_matrix((Ax, Ai, Ap), shape=(n, n))
+    b = gen.rhs(n, 1, True)
+    fplus_amd.tune("refactor_pipeline", 1)
+    for _ in range(2):
+        num.refactor(Ax)
+        x, info = num.solve_refined(b)
+        assert acc.equilibrated(A, x, b, num.Dr, num.Dc) < 1e-14
