@@ -287,7 +287,7 @@ def sweep_main(args, kw, legs, nrhs):
     L.fplus_numeric_free(num)
     for p in (d_ax0, d_axs, d_b0, d_b):
         L.fplus_amd_dev_free(p)
-    head = "sweep_graph" if "sweep_graph" in out else legs[0]
+    head = min(out, key=lambda k: out[k]["sweep_s"])   # the faster launch form
     res = {"metric": "factor+solve s/system at bwd err <=1e-13: mixed/learned vs FP64 GPU vs CPU",
            "value": out[head]["s_per_system_with_setup"], "unit": "s/system", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": out[head]["sweep_s"] * 1e3, "higher_is_better": False, "scaling": "weak",
