@@ -1439,6 +1439,11 @@ extern "C" int fplus_amd_tune(const char* name, int64_t value)
         if (!set_host_threads((int)std::max<int64_t>(std::min<int64_t>(value, 65), -1))) return FPLUS_ERR_BADARG;
         return FPLUS_OK;
     }
+    if (std::strcmp(name, "deterministic_updates") == 0) {
+        if (value != 0 && value != 1) return FPLUS_ERR_BADARG;
+        set_deterministic_updates(value != 0);
+        return FPLUS_OK;
+    }
     if (std::strcmp(name, "update_atomic_rows") == 0) {
         if (value != 0 && value != 1) return FPLUS_ERR_BADARG;
         set_update_atomic_rows(value != 0);

@@ -62,6 +62,9 @@ bool set_host_threads(int n)
 static std::atomic<bool> g_atomic_rows{true};
 bool update_atomic_rows() { return g_atomic_rows.load(std::memory_order_relaxed); }
 void set_update_atomic_rows(bool on) { g_atomic_rows.store(on, std::memory_order_relaxed); }
+static std::atomic<bool> g_det_updates{false};
+bool deterministic_updates() { return g_det_updates.load(std::memory_order_relaxed); }
+void set_deterministic_updates(bool on) { g_det_updates.store(on, std::memory_order_relaxed); }
 static std::atomic<bool> g_refactor_graph{false};
 bool refactor_graph() { return g_refactor_graph.load(std::memory_order_relaxed); }
 static std::atomic<bool> g_refactor_pipeline{true};
@@ -314,6 +317,7 @@ private:
     std::vector<uint64_t> cbits_;
     std::vector<int32_t> pivcol_;
     std::vector<std::vector<int32_t>> succ_;   // per update task: the later tasks whose segments its rows below reach
+    std::vector<int32_t> det_next_, det_stamp_;   // deterministic levels: per workspace row, the next free sub-level
     std::vector<std::vector<int64_t>> tmark_;   // one mark array per thread of the pool, indexed by task
     // per thread of the pool, indexed by workspace row: the (panel, level) that last wrote the row and the row tile that
     // wrote it (the rows two tasks of one level write)
@@ -645,6 +649,40 @@ int Driver::panel(int64_t j0, int64_t j1)
             maxlevel = std::max(maxlevel, lev);
             for (const int32_t t : succ_[i]) lin[t] = std::max(lin[t], lev);
         }
+    }
+    if (deterministic_updates() && maxlevel > 0) {
+        // each level split into sub-levels in task order: a task goes one past the last sub-level that wrote any of its
+        // rows below its block in this level, so no two tasks of a sub-level share a row and each row is updated in task
+        // order -- no sum depends on the order in which a launch's tiles run.  Dependencies stay satisfied: a level's
+        // sub-levels all follow the previous level's.
+        std::vector<std::vector<int32_t>> bylev((size_t)maxlevel + 1);
+        for (int64_t i = 0; i < nt; i++) bylev[(size_t)tl[i].level].push_back((int32_t)i);
+        if ((int64_t)det_next_.size() < ldw) { det_next_.resize((size_t)ldw, 0); det_stamp_.resize((size_t)ldw, -1); }
+        std::fill(det_stamp_.begin(), det_stamp_.begin() + ldw, -1);
+        std::vector<int32_t> newlev((size_t)nt, 0);
+        int32_t base = 0;
+        for (int32_t L = 1; L <= maxlevel; L++) {
+            int32_t nsub = 0;
+            for (const int32_t i : bylev[(size_t)L]) {
+                const BlockDesc& B = F_.blocks[tl[i].t.block];
+                const int32_t* br = F_.block_rows[tl[i].t.block].data();
+                int32_t sub = 0;
+                for (int64_t r = B.w; r < B.nrows; r++) {
+                    const int32_t pos = hrowpos_[br[r]];
+                    if (det_stamp_[(size_t)pos] == L) sub = std::max(sub, det_next_[(size_t)pos]);
+                }
+                for (int64_t r = B.w; r < B.nrows; r++) {
+                    const int32_t pos = hrowpos_[br[r]];
+                    det_stamp_[(size_t)pos] = L;
+                    det_next_[(size_t)pos] = sub + 1;
+                }
+                newlev[(size_t)i] = base + sub + 1;
+                nsub = std::max(nsub, sub + 1);
+            }
+            base += nsub;
+        }
+        for (int64_t i = 0; i < nt; i++) tl[i].level = newlev[(size_t)i];
+        maxlevel = base;
     }
     F_.t_lev += now_s() - th_ws;
     F_.t_lev_pass += now_s() - tlp0;

@@ -162,6 +162,11 @@ bool set_update_tile_policy(int p);
 // writes too (tuning knob "update_atomic_rows", on by default), else for every tile of the level
 bool update_atomic_rows();
 void set_update_atomic_rows(bool on);
+// deterministic update levels (tuning knob "deterministic_updates", off by default): each dependency level split so that
+// no two of its tasks write one workspace row, every row then updated by its tasks in task order (bit-reproducible
+// factors, more launches); for factorizations made afterwards
+bool deterministic_updates();
+void set_deterministic_updates(bool on);
 
 // a same-pattern refactorization: the values d_Bx (the recorded factorization's analysed, equilibrated pattern) factored
 // in the recorded pivot order by replaying its device work; static_pivots 0: a pivot below pivot_rel * colmax fails it

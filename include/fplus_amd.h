@@ -214,7 +214,10 @@ int fplus_amd_thread_stream(int on);
  * 16 tiles per compute unit (fplus_amd.autotune times them).  "host_threads": the host threads of each factorization's
  * pool (1 to 64; 0 the default, 15), for many factorizations at once on several threads.  "update_atomic_rows": 1 (default) makes only the update
  * tiles whose rows another task of their dependency level also writes use atomics, 0 every tile of such a level (for
- * factorizations made afterwards).  "solve_dataflow": 1 runs the triangular solves with A as
+ * factorizations made afterwards).  "deterministic_updates": 1 splits every dependency level of the block updates so that no
+ * two of its tasks write one row (each row updated in task order: bit-reproducible factors, at the cost of more
+ * launches), 0 (default) lets tasks of a level share rows through atomics, whose order of arrival sets the rounding
+ * (for factorizations made afterwards).  "solve_dataflow": 1 runs the triangular solves with A as
  * two persistent launches whose items wait for their inputs through per-block counters, 0 (default) as one launch per
  * dependency level and step.  "solve_noalloc_limit_bytes": the bytes fplus_amd_solve_device_noalloc may use of its
  * array (0: all of it; for tests of the limit).  Unknown names are FPLUS_ERR_BADARG. */

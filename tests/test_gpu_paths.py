@@ -179,6 +179,42 @@ def test_factor_twice_same_structure_and_values():
     assert dl < 1e-10 and du < 1e-10
 
 
+def test_deterministic_updates_are_bit_reproducible():
+    """with the tuning knob deterministic_updates every dependency level is split so that no two of its update tasks
+    write one workspace row: two factorizations of the pencil above, and two refactorizations replaying the recorded
+    schedule, give the same factors bit for bit (the knob's levels hold no row written twice, so no atomics run), and
+    the factors lie within the run-to-run bound of the default form's"""
+    n, Ap, Ai, Ax = gen.shift_invert(8, 4, 128, seed=4)
+    _, _, _, Ax2 = gen.shift_invert(8, 4, 128, seed=4, sigma=0.3 + 0.71j)
+    sym = fplus_amd.Symbolic(n, Ap, Ai)
+    base = fplus_amd.Numeric(sym, Ap, Ai, Ax)
+    fb = base.factors()
+    lev0 = fplus_amd.stats(base)["levels"]
+    try:
+        fplus_amd.tune("deterministic_updates", 1)
+        outs, refs = [], []
+        for _ in range(2):
+            num = fplus_amd.Numeric(sym, Ap, Ai, Ax, amd_options=fplus_amd.amd_options(refactor=1))
+            outs.append((num.p.copy(), num.factors(), fplus_amd.stats(num)))
+            num.refactor(Ax2)
+            refs.append(num.factors())
+    finally:
+        fplus_amd.tune("deterministic_updates", 0)
+    (p1, f1, st), (p2, f2, _) = outs
+    print(f"levels: {lev0} by default, {st['levels']} split")
+    assert st["levels"] > lev0
+    assert np.array_equal(p1, p2)
+    for key in ("Lp", "Li", "Up", "Ui"):
+        assert np.array_equal(f1[key], f2[key]), key
+        assert np.array_equal(refs[0][key], refs[1][key]), key
+    for key in ("Lx", "Ux"):
+        assert np.array_equal(f1[key].view(np.float64), f2[key].view(np.float64)), key
+        assert np.array_equal(refs[0][key].view(np.float64), refs[1][key].view(np.float64)), key
+    assert np.array_equal(p1, base.p)
+    assert np.linalg.norm(f1["Lx"] - fb["Lx"]) / np.linalg.norm(fb["Lx"]) < 1e-10
+    assert np.linalg.norm(f1["Ux"] - fb["Ux"]) / np.linalg.norm(fb["Ux"]) < 1e-10
+
+
 def test_sparselu_default_arguments_against_oracle():
     """SparseLU(A) with the reference's defaults: identity orders, relax 0, Ruiz to 1e-2 in at most 20 iterations"""
     n, Ap, Ai, Ax = gen.shift_invert(4, 4, 32, seed=2)
