@@ -48,7 +48,7 @@ class Stats(C.Structure):
                 ("t_total", C.c_double), ("ruiz_iters", C.c_int), ("gemm_bytes", C.c_double),
                 ("shared_levels", C.c_int64), ("max_block_w", C.c_int64), ("coop_retried", C.c_int), ("precision", C.c_int),
                 ("refactor_replaced", C.c_int64), ("refactor_min_ratio", C.c_double), ("replay_bytes", C.c_double),
-                ("f32_blocks", C.c_int64), ("f32_flop_share", C.c_double)]
+                ("f32_blocks", C.c_int64), ("f32_flop_share", C.c_double), ("pivot_ties", C.c_int64)]
 
 
 class RowInfo(C.Structure):

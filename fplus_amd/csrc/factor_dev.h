@@ -69,7 +69,7 @@ struct URun {
 
 struct PanelStatus {
     int32_t err;         // 0, or 1 + the panel-local column of a singular pivot
-    int32_t pad;
+    int32_t ties;        // with err 0: the sub-panel's columns whose largest candidate modulus two rows or more held
     double best, thr;    // the failing column's best modulus and threshold
 };
 

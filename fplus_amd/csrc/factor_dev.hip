@@ -651,6 +651,19 @@ __global__ __launch_bounds__(TB) void k_x_back(const UpdTask* __restrict__ tasks
 
 constexpr int LU_TB = 1024;
 
+// one pivot candidate merged into another: the larger modulus, the lower row on equal moduli (rows < 0: no candidate);
+// tie: another candidate has the kept one's modulus exactly (the panel LU counts such columns: there the reference,
+// which takes the first candidate in its column's own search order, can take another row)
+__device__ inline void cand_merge(double& bv, int32_t& br, int& bt, double ov, int32_t orr, int ot)
+{
+    if (orr < 0) return;
+    if (br < 0 || ov > bv) { bv = ov; br = orr; bt = ot; return; }
+    if (ov == bv) {
+        bt = 1;
+        if (orr < br) br = orr;
+    }
+}
+
 template <class T>
 __global__ __launch_bounds__(LU_TB) void k_getf2(double* __restrict__ Wd, int64_t ld, int32_t nr, int32_t P, int32_t a,
                                                   int32_t nb, const double* __restrict__ colmax, double pivot_rel,
@@ -658,32 +671,37 @@ __global__ __launch_bounds__(LU_TB) void k_getf2(double* __restrict__ Wd, int64_
                                                   PanelOut* __restrict__ hout)
 {
     __shared__ double bestv[LU_TB / 64];
-    __shared__ int32_t bestr[LU_TB / 64];
+    __shared__ int32_t bestr[LU_TB / 64], bestt[LU_TB / 64];
     __shared__ int32_t pr;
     T* Wl = reinterpret_cast<T*>(Wd);
     if (st->err) return;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int32_t jend = min(a + nb, P);
+    int32_t nties = 0;   // (thread 0) the sub-panel's columns with an exact tie for the pivot
     for (int32_t j = a; j < jend; j++) {
         // argmax of the modulus over rows j .. nr-1, the lowest row on ties
         double bv = -1.0;
         int32_t br = -1;
+        int bt = 0;
         for (int32_t r = j + threadIdx.x; r < nr; r += LU_TB) {
             const double v = Sc<T>::absval(Wl[r + j * ld]);
-            if (v > bv) { bv = v; br = r; }
+            if (v > bv) { bv = v; br = r; bt = 0; }
+            else if (v == bv) bt = 1;
         }
         for (int off = 32; off > 0; off >>= 1) {
             const double ov = __shfl_xor(bv, off);
             const int32_t orr = __shfl_xor(br, off);
-            if (ov > bv || (ov == bv && orr >= 0 && (br < 0 || orr < br))) { bv = ov; br = orr; }
+            const int ot = __shfl_xor(bt, off);
+            cand_merge(bv, br, bt, ov, orr, ot);
         }
-        if (lane == 0) { bestv[wid] = bv; bestr[wid] = br; }
+        if (lane == 0) { bestv[wid] = bv; bestr[wid] = br; bestt[wid] = bt; }
         __syncthreads();
         if (threadIdx.x == 0) {
             double v = -1.0;
             int32_t r = -1;
-            for (int w = 0; w < LU_TB / 64; w++)
-                if (bestv[w] > v || (bestv[w] == v && bestr[w] >= 0 && (r < 0 || bestr[w] < r))) { v = bestv[w]; r = bestr[w]; }
+            int t = 0;
+            for (int w = 0; w < LU_TB / 64; w++) cand_merge(v, r, t, bestv[w], bestr[w], bestt[w]);
+            if (t && v > 0.0) nties++;
             const double thr = pivot_rel * colmax[j];
             if (r < 0 || v <= 0.0 || v < thr) {
                 st->err = j + 1;
@@ -715,7 +733,10 @@ __global__ __launch_bounds__(LU_TB) void k_getf2(double* __restrict__ Wd, int64_
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) hout->st.err = 0;   // the sub-panel is factored: its status for the host
+    if (threadIdx.x == 0) {   // the sub-panel is factored: its status for the host
+        hout->st.ties = nties;
+        hout->st.err = 0;
+    }
 }
 
 // The same sub-panel LU on many workgroups at once (a grid the device holds at once: every workgroup is resident).  Workgroup g
@@ -738,7 +759,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
                                                     int32_t* __restrict__ counter, int32_t base, PanelOut* __restrict__ hout)
 {
     __shared__ double rv[TB / 64];
-    __shared__ int32_t rr[TB / 64];
+    __shared__ int32_t rr[TB / 64], rtie[TB / 64];
+    __shared__ int32_t sh_tie;
+    int32_t nties = 0;   // (workgroup 0, thread 0) the sub-panel's columns with an exact tie for the pivot
     __shared__ T pvals[PNB];
     __shared__ T Lsub[PNB][PNB + 1];
     __shared__ int32_t sh_row, sh_gi, sh_abort;
@@ -756,6 +779,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         const int par = jj & 1;
         double bv = -1.0;
         int32_t br = -1;
+        int bt = 0;
         if (have && r >= j) {
 #pragma unroll
             for (int c = 0; c < PNB; c++)
@@ -765,17 +789,19 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         for (int off = 32; off > 0; off >>= 1) {
             const double ov = __shfl_xor(bv, off);
             const int32_t orr = __shfl_xor(br, off);
-            if (ov > bv || (ov == bv && orr >= 0 && (br < 0 || orr < br))) { bv = ov; br = orr; }
+            const int ot = __shfl_xor(bt, off);
+            cand_merge(bv, br, bt, ov, orr, ot);
         }
-        if (lane == 0) { rv[wid] = bv; rr[wid] = br; }
+        if (lane == 0) { rv[wid] = bv; rr[wid] = br; rtie[wid] = bt; }
         __syncthreads();
         if (tid == 0) {
             double v = -1.0;
             int32_t row = -1;
-            for (int w = 0; w < TB / 64; w++)
-                if (rv[w] > v || (rv[w] == v && rr[w] >= 0 && (row < 0 || rr[w] < row))) { v = rv[w]; row = rr[w]; }
+            int t = 0;
+            for (int w = 0; w < TB / 64; w++) cand_merge(v, row, t, rv[w], rr[w], rtie[w]);
             cand[par * G + g].val = v;
             cand[par * G + g].row = row;
+            cand[par * G + g].pad = t;
             sh_row = row;
         }
         __syncthreads();
@@ -817,17 +843,21 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         if (wid == 0) {
             double v = -1.0;
             int32_t row = -1, gi = -1;
+            int t = 0;
             for (int q = lane; q < G; q += 64) {
-                const double cv = cand[par * G + q].val;
-                const int32_t cr = cand[par * G + q].row;
-                if (cv > v || (cv == v && cr >= 0 && (row < 0 || cr < row))) { v = cv; row = cr; gi = q; }
+                const int32_t r0 = row;
+                cand_merge(v, row, t, cand[par * G + q].val, cand[par * G + q].row, cand[par * G + q].pad);
+                if (row != r0) gi = q;
             }
             for (int off = 32; off > 0; off >>= 1) {
                 const double ov = __shfl_xor(v, off);
                 const int32_t orr = __shfl_xor(row, off), og = __shfl_xor(gi, off);
-                if (ov > v || (ov == v && orr >= 0 && (row < 0 || orr < row))) { v = ov; row = orr; gi = og; }
+                const int ot = __shfl_xor(t, off);
+                const int32_t r0 = row;
+                cand_merge(v, row, t, ov, orr, ot);
+                if (row != r0) gi = og;
             }
-            if (lane == 0) { sh_val = v; sh_row = row; sh_gi = gi; }
+            if (lane == 0) { sh_val = v; sh_row = row; sh_gi = gi; sh_tie = t; }
         }
         __syncthreads();
         const double thr = pivot_rel * colmax[j];
@@ -848,6 +878,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         if (g == 0 && tid == 0) {
             piv[j] = pr;
             hout->piv[j] = pr;
+            nties += sh_tie;
         }
         if (pr != j) {
             if (have && r == j) {
@@ -876,7 +907,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(1, 2))) void
         }
     }
     // the sub-panel is factored: its status for the host (unless a workgroup gave up at a barrier meanwhile)
-    if (g == 0 && tid == 0 && __hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) hout->st.err = 0;
+    if (g == 0 && tid == 0 && __hip_atomic_load(&st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) {
+        hout->st.ties = nties;
+        hout->st.err = 0;
+    }
     if (have) {
 #pragma unroll
         for (int c = 0; c < PNB; c++)

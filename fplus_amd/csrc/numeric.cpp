@@ -1273,6 +1273,7 @@ extern "C" int fplus_amd_numeric_stats(const fplus_numeric* num, fplus_amd_stats
     st->t_total = num->t_total;
     st->ruiz_iters = num->ruiz_iters;
     st->shared_levels = F.shared_levels;
+    st->pivot_ties = F.pivot_ties;
     st->max_block_w = F.max_block_w;
     st->coop_retried = F.coop_retried ? 1 : 0;
     st->precision = F.precision;

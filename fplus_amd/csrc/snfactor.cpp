@@ -1018,6 +1018,7 @@ int Driver::panel(int64_t j0, int64_t j1)
                 errf(err_, errlen_, "Near-zero or zero pivot %.6e at step %ld (threshold %.6e). Matrix is singular.", st.best, (long)j, st.thr);
                 return FPLUS_ERR_SINGULAR;
             }
+            F_.pivot_ties += st.ties;
         }
         const int32_t pv = h_pout_.p[ci / FPA_PANEL_NB].piv[ci];
         if (pv < ci || pv >= Rl) {

@@ -110,6 +110,7 @@ struct DevFactor {
     struct LevHist { int64_t levels = 0, tiles = 0; double issued = 0, maxm = 0; } lev_hist[13];
     int64_t rows_upper = 0, rows_lower = 0, rows_lower_max = 0;   // workspace rows summed over the panels
     int64_t shared_levels = 0;   // dependency levels whose tasks can write the same rows (their updates are atomic)
+    int64_t pivot_ties = 0;      // columns whose pivot search met an exact tie for the largest modulus
     int64_t shared_level_tiles = 0, atomic_tiles = 0;   // the update tiles of such levels, and those that write with atomics
     int64_t max_block_w = 0;     // the widest block
     bool coop_aborted = false;   // the cooperative panel LU gave up at a barrier (the factorization failed; retry without it)

@@ -36,6 +36,9 @@ typedef struct {
     double replay_bytes;          /* device bytes kept for refactorizations (schedule and inverse forms) */
     int64_t f32_blocks;           /* blocks whose updates ran in FP32 in the last (re)factorization */
     double f32_flop_share;        /* their share of the update flops */
+    int64_t pivot_ties;           /* columns of the pivoting factorization whose largest candidate modulus was held by two
+                                     rows or more: the device takes the lowest workspace position, the reference the first
+                                     candidate of its column's search order, so there the pivots can differ */
 } fplus_amd_stats;
 int fplus_amd_numeric_stats(const fplus_numeric *num, fplus_amd_stats *st);
 

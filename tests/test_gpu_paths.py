@@ -215,6 +215,31 @@ def test_deterministic_updates_are_bit_reproducible():
     assert np.linalg.norm(f1["Ux"] - fb["Ux"]) / np.linalg.norm(fb["Ux"]) < 1e-10
 
 
+def test_pivot_ties_are_counted():
+    """the panel LU counts the columns whose largest candidate modulus two rows hold exactly (fplus_amd_stats.pivot_ties:
+    there the device's rule, the lowest workspace position, and the reference's, the first candidate of the column's
+    search order, can choose differently): k blocks [[1, 1], [1, -1]] (Ruiz leaves them as they are) have one such
+    column each, in the cooperative kernel and in the one-workgroup kernel; a matrix of random values has none"""
+    k = 300
+    A = sp.kron(sp.identity(k), sp.csc_matrix(np.array([[1.0, 1.0], [1.0, -1.0]]))).tocsc()
+    A.sort_indices()
+    n = A.shape[0]
+    b = np.arange(1.0, n + 1)
+    try:
+        for cap in (512, 0):
+            fplus_amd.tune("panel_lu_max_workgroups", cap)
+            sym = fplus_amd.Symbolic(n, A.indptr, A.indices)
+            num = fplus_amd.Numeric(sym, A.indptr, A.indices, A.data)
+            assert fplus_amd.stats(num)["pivot_ties"] == k, cap
+            x = num.solve(b)
+            assert np.abs(A @ x - b).max() < 1e-12 * np.abs(b).max()
+    finally:
+        fplus_amd.tune("panel_lu_max_workgroups", 512)
+    n, Ap, Ai, Ax = gen.shift_invert(4, 4, 64, seed=3)
+    num = fplus_amd.Numeric(fplus_amd.Symbolic(n, Ap, Ai), Ap, Ai, Ax)
+    assert fplus_amd.stats(num)["pivot_ties"] == 0
+
+
 def test_sparselu_default_arguments_against_oracle():
     """SparseLU(A) with the reference's defaults: identity orders, relax 0, Ruiz to 1e-2 in at most 20 iterations"""
     n, Ap, Ai, Ax = gen.shift_invert(4, 4, 32, seed=2)
