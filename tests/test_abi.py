@@ -182,3 +182,28 @@ def test_generators_are_deterministic():
     assert n == 9216 and len(Ai) == 45696
     n, Ap, Ai, Ax = gen.convdiff3d(8)
     assert len(Ai) == 7 * 512 - 6 * 64
+
+
+def test_tuning_knobs_accept_their_range_and_reject_the_rest():
+    """fplus_amd_tune (include/fplus_amd.h): every on/off knob takes 0 and 1 and returns FPLUS_ERR_BADARG for 2, the
+    numeric ones their documented ranges, unknown names FPLUS_ERR_BADARG; the defaults are restored afterwards"""
+    L = _lib.lib()
+    defaults = {"refactor_graph": 0, "refactor_pipeline": 1, "deterministic_updates": 0, "update_atomic_rows": 1,
+                "solve_dataflow": 0, "xcd_tile_order": 1}
+    try:
+        for name in defaults:
+            for v in (0, 1):
+                assert L.fplus_amd_tune(name.encode(), v) == 0, (name, v)
+            assert L.fplus_amd_tune(name.encode(), 2) == _lib.FPLUS_ERR_BADARG, name
+        assert L.fplus_amd_tune(b"panel_lu_max_workgroups", 0) == 0
+        assert L.fplus_amd_tune(b"panel_lu_max_workgroups", 513) == _lib.FPLUS_ERR_BADARG
+        for p in range(5):
+            assert L.fplus_amd_tune(b"update_tile_policy", p) == 0
+        assert L.fplus_amd_tune(b"update_tile_policy", 5) == _lib.FPLUS_ERR_BADARG
+        assert L.fplus_amd_tune(b"no_such_knob", 0) == _lib.FPLUS_ERR_BADARG
+        assert L.fplus_amd_tune(None, 0) == _lib.FPLUS_ERR_BADARG
+    finally:
+        for name, v in defaults.items():
+            L.fplus_amd_tune(name.encode(), v)
+        L.fplus_amd_tune(b"panel_lu_max_workgroups", 512)
+        L.fplus_amd_tune(b"update_tile_policy", 0)
